@@ -1,0 +1,17 @@
+"""dwost_amd -- Walk-on-Stars (diff_wost) on AMD MI355X.
+
+Scene and solver API of zihay/dwost's ``diff_wost`` package; every query and
+walk runs in hand-written gfx950 HIP kernels in ``_build/libdwost.so`` (C ABI:
+``include/dwost.h``).  There is no CPU fallback: importing the solvers without
+the built library raises.
+"""
+
+from .shapes import (BC_EXPCOS, BC_LINEAR, BC_PER_VERTEX, BC_QUAD, BoundaryCondition, BoundaryType, Scene2D,
+                     Scene3D)
+from ._lib import SUB_DIRICHLET, SUB_FULL, SUB_NEUMANN, DwostError
+from .solvers import Solver, WoSt, WoSt3D
+
+__all__ = [
+    "BC_EXPCOS", "BC_LINEAR", "BC_PER_VERTEX", "BC_QUAD", "BoundaryCondition", "BoundaryType", "Scene2D", "Scene3D",
+    "SUB_DIRICHLET", "SUB_FULL", "SUB_NEUMANN", "DwostError", "Solver", "WoSt", "WoSt3D",
+]

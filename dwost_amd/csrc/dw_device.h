@@ -1,0 +1,781 @@
+// dw_device.h -- device-side geometry, sampling and traversal routines for
+// the Walk-on-Stars kernels (gfx950).
+//
+// Arithmetic contract (shared with the CPU restatement used as the test
+// oracle): fp32, no implicit contraction (-ffp-contract=off); Dr.Jit's fused
+// forms are explicit fmaf (dot = fma chain, lerp = fma(b,t,fma(-a,t,a)),
+// 3D cross = fma(u.y,v.z,-(u.z*v.y)) ...); sqrt and division are correctly
+// rounded; log/exp/sin/cos/acos/asin/atan2/cbrt are evaluated in fp64 and
+// rounded once (gfx950 runs fp64 at half the fp32 rate).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dw_layout.h"
+
+namespace dwd {
+
+#define DWD __device__ __forceinline__
+
+constexpr float kInf = __builtin_inff();
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kPi2 = 1.57079632679489661923f;
+constexpr float kTwoPi = 6.28318530717958647692f;
+constexpr float kFourPi = 12.5663706143591729538f;
+constexpr float kDrEps = 5.9604644775390625e-08f;  // 2^-24
+constexpr float kNegInv2Pi = -0.159154943091895335768f;
+
+struct F2 { float x, y; };
+struct F3 { float x, y, z; };
+
+DWD F2 f2(float x, float y) { return F2{x, y}; }
+DWD F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+DWD F2 operator+(F2 a, F2 b) { return f2(a.x + b.x, a.y + b.y); }
+DWD F3 operator+(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+DWD F2 operator-(F2 a, F2 b) { return f2(a.x - b.x, a.y - b.y); }
+DWD F3 operator-(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+DWD F2 operator*(F2 a, float s) { return f2(a.x * s, a.y * s); }
+DWD F3 operator*(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+DWD F2 operator/(F2 a, float s) { return f2(a.x / s, a.y / s); }
+DWD F3 operator/(F3 a, float s) { return f3(a.x / s, a.y / s, a.z / s); }
+DWD F2 operator-(F2 a) { return f2(-a.x, -a.y); }
+DWD F3 operator-(F3 a) { return f3(-a.x, -a.y, -a.z); }
+DWD float dot(F2 a, F2 b) { return __builtin_fmaf(a.y, b.y, a.x * b.x); }
+DWD float dot(F3 a, F3 b) { return __builtin_fmaf(a.z, b.z, __builtin_fmaf(a.y, b.y, a.x * b.x)); }
+DWD float norm(F2 a) { return __builtin_sqrtf(dot(a, a)); }
+DWD float norm(F3 a) { return __builtin_sqrtf(dot(a, a)); }
+DWD F2 normalize(F2 a) { return a * (1.0f / __builtin_sqrtf(dot(a, a))); }
+DWD F3 normalize(F3 a) { return a * (1.0f / __builtin_sqrtf(dot(a, a))); }
+DWD F3 cross(F3 u, F3 v) {
+    return f3(__builtin_fmaf(u.y, v.z, -(u.z * v.y)), __builtin_fmaf(u.z, v.x, -(u.x * v.z)),
+              __builtin_fmaf(u.x, v.y, -(u.y * v.x)));
+}
+DWD float cross2(F2 u, F2 v) { return u.x * v.y - u.y * v.x; }
+DWD float lerpf(float a, float b, float t) { return __builtin_fmaf(b, t, __builtin_fmaf(-a, t, a)); }
+DWD F2 lerp(F2 a, F2 b, float t) { return f2(lerpf(a.x, b.x, t), lerpf(a.y, b.y, t)); }
+DWD F3 lerp(F3 a, F3 b, float t) { return f3(lerpf(a.x, b.x, t), lerpf(a.y, b.y, t), lerpf(a.z, b.z, t)); }
+DWD float fmn(float a, float b) { return __builtin_fminf(a, b); }
+DWD float fmx(float a, float b) { return __builtin_fmaxf(a, b); }
+DWD float clampf(float x, float lo, float hi) { return fmx(fmn(x, hi), lo); }
+DWD F2 vmin(F2 a, F2 b) { return f2(fmn(a.x, b.x), fmn(a.y, b.y)); }
+DWD F2 vmax(F2 a, F2 b) { return f2(fmx(a.x, b.x), fmx(a.y, b.y)); }
+DWD F3 vmin(F3 a, F3 b) { return f3(fmn(a.x, b.x), fmn(a.y, b.y), fmn(a.z, b.z)); }
+DWD F3 vmax(F3 a, F3 b) { return f3(fmx(a.x, b.x), fmx(a.y, b.y), fmx(a.z, b.z)); }
+DWD F2 vabs(F2 a) { return f2(__builtin_fabsf(a.x), __builtin_fabsf(a.y)); }
+DWD F3 vabs(F3 a) { return f3(__builtin_fabsf(a.x), __builtin_fabsf(a.y), __builtin_fabsf(a.z)); }
+DWD bool bad(float v) { return __builtin_isnan(v) || __builtin_isinf(v); }
+
+// fp64-evaluated, once-rounded transcendentals
+DWD float cr_log(float x) { return (float)log((double)x); }
+DWD float cr_exp(float x) { return (float)exp((double)x); }
+DWD float cr_sin(float x) { return (float)sin((double)x); }
+DWD float cr_cos(float x) { return (float)cos((double)x); }
+DWD float cr_acos(float x) { return (float)acos((double)x); }
+DWD float cr_asin(float x) { return (float)asin((double)x); }
+DWD float cr_atan2(float y, float x) { return (float)atan2((double)y, (double)x); }
+DWD float cr_cbrt(float x) { return (float)cbrt((double)x); }
+
+// ---------------------------------------------------------------------------
+// TEA seeding (diff_wost/core/math.py:137-157) and PCG32 (Dr.Jit PCG32)
+// ---------------------------------------------------------------------------
+DWD void tea4(uint32_t v0, uint32_t v1, uint32_t& o0, uint32_t& o1) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        s += 0x9E3779B9u;
+        v0 += ((v1 << 4) + 0xA341316Cu) ^ (v1 + s) ^ ((v1 >> 5) + 0xC8013EA4u);
+        v1 += ((v0 << 4) + 0xAD90777Du) ^ (v0 + s) ^ ((v0 >> 5) + 0x7E95761Eu);
+    }
+    o0 = v0;
+    o1 = v1;
+}
+
+struct Rng {
+    uint64_t state, inc;
+    DWD uint32_t next_u32() {
+        uint64_t old = state;
+        state = old * 0x5851f42d4c957f2dULL + inc;
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    }
+    DWD float next() { return __uint_as_float((next_u32() >> 9) | 0x3f800000u) - 1.0f; }
+    // PCG32(size, initstate, initseq) lane `lane`: seed(initstate+lane, initseq+lane)
+    DWD void seed_lane(uint32_t v0, uint32_t v1, uint64_t lane) {
+        uint64_t initstate = (uint64_t)v0 + lane, initseq = (uint64_t)v1 + lane;
+        state = 0;
+        inc = (initseq << 1) | 1u;
+        next_u32();
+        state += initstate;
+        next_u32();
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Records
+// ---------------------------------------------------------------------------
+struct CP {  // closest point: 2D uses t, 3D uses (u, v)
+    F3 p, n;
+    float t, u, v, d;
+    int prim;
+};
+struct Hit {
+    bool valid, on_boundary;
+    F3 p, n;
+    float t, u, v, d;
+    int prim;
+};
+
+// BoundingBox.distance (bvh.py:31-36)
+DWD void box_dist(F2 mn, F2 mx, F2 p, float& dmin, float& dmax) {
+    F2 u = mn - p, v = p - mx;
+    dmin = norm(vmax(vmax(u, v), f2(0.0f, 0.0f)));
+    dmax = norm(vmin(u, v));
+}
+DWD void box_dist(F3 mn, F3 mx, F3 p, float& dmin, float& dmax) {
+    F3 u = mn - p, v = p - mx;
+    dmin = norm(vmax(vmax(u, v), f3(0.0f, 0.0f, 0.0f)));
+    dmax = norm(vmin(u, v));
+}
+DWD float box_dmin(F2 mn, F2 mx, F2 p) {
+    F2 u = mn - p, v = p - mx;
+    return norm(vmax(vmax(u, v), f2(0.0f, 0.0f)));
+}
+DWD float box_dmin(F3 mn, F3 mx, F3 p) {
+    F3 u = mn - p, v = p - mx;
+    return norm(vmax(vmax(u, v), f3(0.0f, 0.0f, 0.0f)));
+}
+// BoundingBox.intersect (bvh.py:42-59)
+DWD bool box_ray(F2 mn, F2 mx, F2 x, F2 v, float rmax, float& tmin) {
+    F2 t0 = f2((mn.x - x.x) / v.x, (mn.y - x.y) / v.y);
+    F2 t1 = f2((mx.x - x.x) / v.x, (mx.y - x.y) / v.y);
+    F2 tn = vmin(t0, t1), tf = vmax(t0, t1);
+    float a = fmx(fmx(tn.x, tn.y), 0.0f);
+    float b = fmn(fmn(tf.x, tf.y), rmax);
+    tmin = a > b ? kInf : a;
+    return !(a > b);
+}
+DWD bool box_ray(F3 mn, F3 mx, F3 x, F3 v, float rmax, float& tmin) {
+    F3 t0 = f3((mn.x - x.x) / v.x, (mn.y - x.y) / v.y, (mn.z - x.z) / v.z);
+    F3 t1 = f3((mx.x - x.x) / v.x, (mx.y - x.y) / v.y, (mx.z - x.z) / v.z);
+    F3 tn = vmin(t0, t1), tf = vmax(t0, t1);
+    float a = fmx(fmx(fmx(tn.x, tn.y), tn.z), 0.0f);
+    float b = fmn(fmn(fmn(tf.x, tf.y), tf.z), rmax);
+    tmin = a > b ? kInf : a;
+    return !(a > b);
+}
+
+DWD F2 nmin(const DNode2& n) { return f2(n.mnx, n.mny); }
+DWD F2 nmax(const DNode2& n) { return f2(n.mxx, n.mxy); }
+DWD F3 nmin(const DNode3& n) { return f3(n.mnx, n.mny, n.mnz); }
+DWD F3 nmax(const DNode3& n) { return f3(n.mxx, n.mxy, n.mxz); }
+
+// ---------------------------------------------------------------------------
+// 2D: BVH closest point (bvh.py:111-188 + line_segment.py:74-85)
+// ---------------------------------------------------------------------------
+DWD CP closest2(const SubDev& S, F2 p) {
+    const DNode2* nodes = (const DNode2*)S.nodes;
+    const DSeg* prims = (const DSeg*)S.prims;
+    CP its;
+    its.p = f3(p.x, p.y, 0.0f);
+    its.n = f3(0.0f, 0.0f, 0.0f);
+    its.t = kInf;
+    its.u = its.v = 0.0f;
+    its.d = kInf;
+    its.prim = -1;
+    int best = -1;
+    float r_max = kInf;
+    int st_i[DW_MAX_STACK];
+    float st_d[DW_MAX_STACK];
+    int sp = 0;
+    st_i[0] = 0;
+    st_d[0] = kInf;
+    while (sp >= 0) {
+        int ni = st_i[sp];
+        float cd = st_d[sp];
+        sp--;
+        if (cd > r_max) continue;
+        DNode2 nd = nodes[ni];
+        if (nd.nref > 0) {
+            for (int j = 0; j < nd.nref; ++j) {
+                DSeg s = prims[nd.off + j];
+                F2 a = f2(s.ax, s.ay), b = f2(s.bx, s.by);
+                F2 pa = p - a, ba = b - a;
+                float h = clampf(dot(pa, ba) / dot(ba, ba), 0.0f, 1.0f);
+                F2 i = a + ba * h;
+                float d = norm(i - p);
+                if (d < its.d) {
+                    its.d = d;
+                    its.t = h;
+                    its.p = f3(i.x, i.y, 0.0f);
+                    best = nd.off + j;
+                }
+            }
+            r_max = fmn(r_max, its.d);
+        } else {
+            int L = ni + 1, R = ni + nd.off;
+            DNode2 nl = nodes[L], nr = nodes[R];
+            float dminL, dmaxL, dminR, dmaxR;
+            box_dist(nmin(nl), nmax(nl), p, dminL, dmaxL);
+            bool hitL = dminL <= r_max;
+            r_max = fmn(r_max, dmaxL);
+            box_dist(nmin(nr), nmax(nr), p, dminR, dmaxR);
+            bool hitR = dminR <= r_max;
+            r_max = fmn(r_max, dmaxR);
+            if (hitL && hitR) {
+                int closer = L, other = R;
+                if (dminR < dminL) {
+                    closer = R, other = L;
+                    float t = dminL;
+                    dminL = dminR;
+                    dminR = t;
+                }
+                st_i[++sp] = other;
+                st_d[sp] = dminR;
+                st_i[++sp] = closer;
+                st_d[sp] = dminL;
+            } else if (hitL) {
+                st_i[++sp] = L;
+                st_d[sp] = dminL;
+            } else if (hitR) {
+                st_i[++sp] = R;
+                st_d[sp] = dminR;
+            }
+        }
+    }
+    if (best >= 0) {  // normal of the winning segment (LineSegment.closest_point)
+        DSeg s = prims[best];
+        F2 t = normalize(f2(s.bx - s.ax, s.by - s.ay));
+        its.n = f3(t.y, -t.x, 0.0f);
+        its.prim = S.prim_index[best];
+    }
+    return its;
+}
+
+// BVH.intersect (bvh.py:190-257): the first valid hit in traversal order wins
+// (`r_max = its.d` reads the zero-initialised record, :216), so the traversal
+// stops at the first hit.
+DWD Hit intersect2(const SubDev& S, F2 x, F2 v, float r_max) {
+    const DNode2* nodes = (const DNode2*)S.nodes;
+    const DSeg* prims = (const DSeg*)S.prims;
+    Hit its;
+    its.valid = false;
+    its.on_boundary = false;
+    its.p = its.n = f3(0.0f, 0.0f, 0.0f);
+    its.t = its.u = its.v = its.d = 0.0f;
+    its.prim = 0;
+    float tmin;
+    DNode2 root = nodes[0];
+    if (!box_ray(nmin(root), nmax(root), x, v, r_max, tmin)) return its;
+    int st_i[DW_MAX_STACK];
+    float st_d[DW_MAX_STACK];
+    int sp = 0;
+    st_i[0] = 0;
+    st_d[0] = tmin;
+    while (sp >= 0) {
+        int ni = st_i[sp];
+        float cd = st_d[sp];
+        sp--;
+        if (!(cd <= r_max)) continue;
+        DNode2 nd = nodes[ni];
+        if (nd.nref > 0) {
+            for (int j = 0; j < nd.nref; ++j) {  // LineSegment.ray_intersect (line_segment.py:47-72)
+                DSeg s = prims[nd.off + j];
+                F2 a = f2(s.ax, s.ay), b = f2(s.bx, s.by);
+                F2 u = a - x, w = b - a;
+                float dv = v.x * w.y - v.y * w.x;
+                if (!(__builtin_fabsf(dv) > kDrEps)) continue;
+                float ud = u.x * v.y - u.y * v.x;
+                float s_ = ud / dv;
+                if (!((s_ >= 0.0f) && (s_ <= 1.0f))) continue;
+                float uv = u.x * w.y - u.y * w.x;
+                float t = uv / dv;
+                if (!((t >= 0.0f) && (t <= r_max))) continue;
+                float d = __builtin_fabsf(t);
+                if (!(d < r_max)) continue;
+                F2 n = normalize(f2(w.y, -w.x));
+                F2 p = x + v * t;
+                its.valid = true;
+                its.on_boundary = true;
+                its.p = f3(p.x, p.y, 0.0f);
+                its.n = f3(n.x, n.y, 0.0f);
+                its.t = s_;
+                its.d = d;
+                its.prim = S.prim_index[nd.off + j];
+                return its;
+            }
+        } else {
+            int L = ni + 1, R = ni + nd.off;
+            DNode2 nl = nodes[L], nr = nodes[R];
+            float t0, t1;
+            bool h0 = box_ray(nmin(nl), nmax(nl), x, v, r_max, t0);
+            bool h1 = box_ray(nmin(nr), nmax(nr), x, v, r_max, t1);
+            if (h0 && h1) {
+                int closer = L, other = R;
+                if (t1 < t0) {
+                    closer = R, other = L;
+                    float t = t0;
+                    t0 = t1;
+                    t1 = t;
+                }
+                st_i[++sp] = other;
+                st_d[sp] = t1;
+                st_i[++sp] = closer;
+                st_d[sp] = t0;
+            } else if (h0) {
+                st_i[++sp] = L;
+                st_d[sp] = t0;
+            } else if (h1) {
+                st_i[++sp] = R;
+                st_d[sp] = t1;
+            }
+        }
+    }
+    return its;
+}
+
+// project_to_plane (bvh.py:16-20, snch3d.py:30-45)
+DWD float project_to_plane(F2 n, F2 e) { return __builtin_fabsf(dot(e, vabs(f2(-n.y, n.x)))); }
+DWD float project_to_plane(F3 n, F3 e) {
+    float sign = __builtin_copysignf(1.0f, n.z);
+    float a = -1.0f / (sign + n.z);
+    float b = n.x * n.y * a;
+    F3 b1 = f3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    F3 b2 = f3(b, sign + n.y * n.y * a, -n.y);
+    float r1 = dot(e, vabs(b1));
+    float r2 = dot(e, vabs(b2));
+    return __builtin_sqrtf(r1 * r1 + r2 * r2);
+}
+DWD bool in_range(float x, float a, float b) { return (x >= a) && (x <= b); }
+
+// SNCHNode.overlaps (snch.py:46-97, snch3d.py:65-115)
+template <typename V>
+DWD bool snch_overlaps(V mn, V mx, V axis, float ha, float radius, V x, float r_max, float& d_min) {
+    float d_max;
+    box_dist(mn, mx, x, d_min, d_max);
+    bool ov = d_min <= r_max;
+    if ((ha >= 0.0f) && ov) {
+        ov = (ha > kPi2) && (d_min < kDrEps);
+        if (!ov) {
+            V cen = (mn + mx) / 2.0f;
+            V vca = cen - x;
+            float len = norm(vca);
+            vca = vca / len;
+            float dax = cr_acos(clampf(dot(axis, vca), -1.0f, 1.0f));
+            ov = in_range(kPi2, dax - ha, dax + ha);
+            if (!ov) {
+                if (len > radius) {
+                    float hs = ha + cr_asin(radius / len);
+                    ov = (hs > kPi2) || in_range(kPi2, dax - hs, dax + hs);
+                } else {
+                    V e = mx - cen;
+                    float d = dot(e, vabs(vca));
+                    float s = len - d;
+                    ov = s < 0.0f;
+                    if (!ov) {
+                        d = project_to_plane(vca, e);
+                        float hs = ha + cr_atan2(d, s);
+                        ov = (hs > kPi2) || in_range(kPi2, dax - hs, dax + hs);
+                    }
+                }
+            }
+        }
+    }
+    return ov;
+}
+
+// SNCH.closest_silhouette 2D (snch.py:312-378 + silhouette_vertex.py:80-93)
+DWD bool closest_sil2(const SubDev& S, F2 x, float r_max, float& d_out, int& ref_out, F2& p_out) {
+    const DNode2* nodes = (const DNode2*)S.nodes;
+    const DCone2* cones = (const DCone2*)S.cones;
+    const DSilV* sils = (const DSilV*)S.sils;
+    bool found = false;
+    d_out = 0.0f;
+    ref_out = -1;
+    DNode2 root = nodes[0];
+    float dmin = box_dmin(nmin(root), nmax(root), x);
+    if (!(dmin <= r_max)) return false;
+    int st_i[DW_MAX_STACK];
+    float st_d[DW_MAX_STACK];
+    int sp = 0;
+    st_i[0] = 0;
+    st_d[0] = dmin;
+    while (sp >= 0) {
+        int ni = st_i[sp];
+        float cd = st_d[sp];
+        sp--;
+        if (!(cd <= r_max)) continue;
+        DNode2 nd = nodes[ni];
+        if (nd.nref > 0) {
+            for (int j = 0; j < nd.nsil; ++j) {
+                int ref = nd.sil_off + j;
+                DSilV s = sils[ref];
+                F2 a = f2(s.ax, s.ay), b = f2(s.bx, s.by), c = f2(s.cx, s.cy);
+                float d = norm(x - b);
+                if (s.both && (d < r_max)) {
+                    if (cross2(b - a, x - a) * cross2(c - b, x - b) < 0.0f) {
+                        r_max = fmn(r_max, d);
+                        found = true;
+                        d_out = d;
+                        ref_out = ref;
+                        p_out = b;
+                    }
+                }
+            }
+        } else {
+            int L = ni + 1, R = ni + nd.off;
+            DNode2 nl = nodes[L], nr = nodes[R];
+            DCone2 cl = cones[L], cr = cones[R];
+            float d0, d1;
+            bool h0 = snch_overlaps(nmin(nl), nmax(nl), f2(cl.ax, cl.ay), cl.half_angle, cl.radius, x, r_max, d0);
+            bool h1 = snch_overlaps(nmin(nr), nmax(nr), f2(cr.ax, cr.ay), cr.half_angle, cr.radius, x, r_max, d1);
+            if (h0 && h1) {
+                int closer = L, other = R;
+                if (d1 < d0) {
+                    closer = R, other = L;
+                    float t = d0;
+                    d0 = d1;
+                    d1 = t;
+                }
+                st_i[++sp] = other;
+                st_d[sp] = d1;
+                st_i[++sp] = closer;
+                st_d[sp] = d0;
+            } else if (h0) {
+                st_i[++sp] = L;
+                st_d[sp] = d0;
+            } else if (h1) {
+                st_i[++sp] = R;
+                st_d[sp] = d1;
+            }
+        }
+    }
+    return found;
+}
+
+// SNCH3D.closest_silhouette (snch3d.py:594-663 + silhouette_edge.py:83-97)
+DWD bool closest_sil3(const SubDev& S, F3 x, float r_max, float& d_out, int& ref_out, F3& p_out) {
+    const DNode3* nodes = (const DNode3*)S.nodes;
+    const DCone3* cones = (const DCone3*)S.cones;
+    const DSilE* sils = (const DSilE*)S.sils;
+    bool found = false;
+    d_out = 0.0f;
+    ref_out = -1;
+    DNode3 root = nodes[0];
+    float dmin = box_dmin(nmin(root), nmax(root), x);
+    if (!(dmin <= r_max)) return false;
+    int st_i[DW_MAX_STACK];
+    float st_d[DW_MAX_STACK];
+    int sp = 0;
+    st_i[0] = 0;
+    st_d[0] = dmin;
+    while (sp >= 0) {
+        int ni = st_i[sp];
+        float cd = st_d[sp];
+        sp--;
+        if (!(cd <= r_max)) continue;
+        DNode3 nd = nodes[ni];
+        if (nd.nref > 0) {
+            DCone3 cn = cones[ni];
+            for (int j = 0; j < cn.nsil; ++j) {
+                int ref = cn.sil_off + j;
+                DSilE s = sils[ref];
+                F3 a = f3(s.ax, s.ay, s.az), b = f3(s.bx, s.by, s.bz);
+                F3 u = b - a, w = x - a;
+                float t = clampf(dot(w, u) / dot(u, u), 0.0f, 1.0f);
+                F3 pt = lerp(a, b, t);
+                float d = norm(pt - x);
+                if (d < r_max) {
+                    bool is = s.boundary != 0;
+                    if (!is) {
+                        F3 c = f3(s.cx, s.cy, s.cz), dd = f3(s.dx, s.dy, s.dz);
+                        F3 n0 = normalize(cross(b - a, c - a));
+                        F3 n1 = normalize(cross(dd - a, b - a));
+                        F3 view = pt - x;
+                        is = dot(view, n0) * dot(view, n1) < 0.0f;
+                    }
+                    if (is) {
+                        r_max = fmn(r_max, d);
+                        found = true;
+                        d_out = d;
+                        ref_out = ref;
+                        p_out = pt;
+                    }
+                }
+            }
+        } else {
+            int L = ni + 1, R = ni + nd.off;
+            DNode3 nl = nodes[L], nr = nodes[R];
+            DCone3 cl = cones[L], cr = cones[R];
+            float d0, d1;
+            bool h0 =
+                snch_overlaps(nmin(nl), nmax(nl), f3(cl.ax, cl.ay, cl.az), cl.half_angle, cl.radius, x, r_max, d0);
+            bool h1 =
+                snch_overlaps(nmin(nr), nmax(nr), f3(cr.ax, cr.ay, cr.az), cr.half_angle, cr.radius, x, r_max, d1);
+            if (h0 && h1) {
+                int closer = L, other = R;
+                if (d1 < d0) {
+                    closer = R, other = L;
+                    float t = d0;
+                    d0 = d1;
+                    d1 = t;
+                }
+                st_i[++sp] = other;
+                st_d[sp] = d1;
+                st_i[++sp] = closer;
+                st_d[sp] = d0;
+            } else if (h0) {
+                st_i[++sp] = L;
+                st_d[sp] = d0;
+            } else if (h1) {
+                st_i[++sp] = R;
+                st_d[sp] = d1;
+            }
+        }
+    }
+    return found;
+}
+
+// Polyline.star_radius (polyline.py:304-314) / Mesh.star_radius (mesh.py:337-346)
+DWD float star_radius2(const SubDev& S, F2 x, float r_max) {
+    F2 m = f2(fmx(__builtin_fabsf(x.x - S.bb_min[0]), __builtin_fabsf(x.x - S.bb_max[0])),
+              fmx(__builtin_fabsf(x.y - S.bb_min[1]), __builtin_fabsf(x.y - S.bb_max[1])));
+    float R = fmn(norm(m), r_max);
+    if (!S.py_sils) return R;
+    float d;
+    int ref;
+    F2 p;
+    return closest_sil2(S, x, R, d, ref, p) ? d : R;
+}
+DWD float star_radius3(const SubDev& S, F3 x, float r_max) {
+    F3 m = f3(fmx(__builtin_fabsf(x.x - S.bb_min[0]), __builtin_fabsf(x.x - S.bb_max[0])),
+              fmx(__builtin_fabsf(x.y - S.bb_min[1]), __builtin_fabsf(x.y - S.bb_max[1])),
+              fmx(__builtin_fabsf(x.z - S.bb_min[2]), __builtin_fabsf(x.z - S.bb_max[2])));
+    float R = fmn(norm(m), r_max);
+    if (!S.py_sils) return R;
+    float d;
+    int ref;
+    F3 p;
+    return closest_sil3(S, x, R, d, ref, p) ? d : R;
+}
+
+// ---------------------------------------------------------------------------
+// 3D: closest point on triangles (math.py:257-322) and SNCH3D.closest_point
+// ---------------------------------------------------------------------------
+DWD void tri_closest(const DTri& tr, F3 p, F3& pt, float& u_out, float& v_out, float& d) {
+    F3 a = f3(tr.ax, tr.ay, tr.az), b = f3(tr.bx, tr.by, tr.bz), c = f3(tr.cx, tr.cy, tr.cz);
+    F3 ab = b - a, ac = c - a;
+    F3 ax = p - a;
+    float d1 = dot(ab, ax), d2 = dot(ac, ax);
+    F3 bx = p - b;
+    float d3 = dot(ab, bx), d4 = dot(ac, bx);
+    F3 cx = p - c;
+    float d5 = dot(ab, cx), d6 = dot(ac, cx);
+    float u0, u1;
+    if ((d1 <= 0.0f) && (d2 <= 0.0f)) {
+        pt = a, u0 = 1.0f, u1 = 0.0f;
+    } else if ((d3 >= 0.0f) && (d4 <= d3)) {
+        pt = b, u0 = 0.0f, u1 = 1.0f;
+    } else if ((d6 >= 0.0f) && (d5 <= d6)) {
+        pt = c, u0 = 0.0f, u1 = 0.0f;
+    } else {
+        float vc = d1 * d4 - d3 * d2;
+        float vb = d5 * d2 - d1 * d6;
+        float va = d3 * d6 - d5 * d4;
+        if ((vc <= 0.0f) && (d1 >= 0.0f) && (d3 <= 0.0f)) {
+            float v = d1 / (d1 - d3);
+            pt = a + ab * v;
+            u0 = 1.0f - v, u1 = v;
+        } else if ((vb <= 0.0f) && (d2 >= 0.0f) && (d6 <= 0.0f)) {
+            float w = d2 / (d2 - d6);
+            pt = a + ac * w;
+            u0 = 1.0f - w, u1 = 0.0f;
+        } else if ((va <= 0.0f) && ((d4 - d3) >= 0.0f) && ((d5 - d6) >= 0.0f)) {
+            float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+            pt = b + (c - b) * w;
+            u0 = 0.0f, u1 = 1.0f - w;
+        } else {
+            float denom = 1.0f / (va + vb + vc);
+            float v = vb * denom, w = vc * denom;
+            pt = (a + ab * v) + ac * w;
+            u0 = 1.0f - v - w, u1 = v;
+        }
+    }
+    d = norm(p - pt);
+    u_out = u1;
+    v_out = 1.0f - u0 - u1;
+}
+
+DWD CP closest3(const SubDev& S, F3 p) {
+    const DNode3* nodes = (const DNode3*)S.nodes;
+    const DTri* prims = (const DTri*)S.prims;
+    CP its;
+    its.p = its.n = f3(0.0f, 0.0f, 0.0f);
+    its.t = its.u = its.v = 0.0f;
+    its.d = kInf;
+    its.prim = -1;
+    int best = -1;
+    float r_max = kInf;
+    int st_i[DW_MAX_STACK];
+    float st_d[DW_MAX_STACK];
+    int sp = 0;
+    st_i[0] = 0;
+    st_d[0] = kInf;
+    while (sp >= 0) {
+        int ni = st_i[sp];
+        float cd = st_d[sp];
+        sp--;
+        if (!(cd <= r_max)) continue;
+        DNode3 nd = nodes[ni];
+        if (nd.nref > 0) {
+            for (int j = 0; j < nd.nref; ++j) {
+                DTri tr = prims[nd.off + j];
+                F3 pt;
+                float u, v, d;
+                tri_closest(tr, p, pt, u, v, d);
+                if (d < its.d) {
+                    its.d = d;
+                    its.p = pt;
+                    its.u = u;
+                    its.v = v;
+                    best = nd.off + j;
+                    r_max = fmn(r_max, d);
+                }
+            }
+        } else {
+            int L = ni + 1, R = ni + nd.off;
+            DNode3 nl = nodes[L], nr = nodes[R];
+            float dminL, dmaxL, dminR, dmaxR;
+            box_dist(nmin(nl), nmax(nl), p, dminL, dmaxL);
+            bool hitL = dminL <= r_max;
+            r_max = fmn(r_max, dmaxL);
+            box_dist(nmin(nr), nmax(nr), p, dminR, dmaxR);
+            bool hitR = dminR <= r_max;
+            r_max = fmn(r_max, dmaxR);
+            if (hitL && hitR) {
+                int closer = L, other = R;
+                if ((dminL == 0.0f) && (dminR == 0.0f) && (dmaxR < dmaxL)) closer = R, other = L;
+                if (dminR < dminL) {
+                    int t = closer;
+                    closer = other;
+                    other = t;
+                    float tt = dminL;
+                    dminL = dminR;
+                    dminR = tt;
+                }
+                st_i[++sp] = other;
+                st_d[sp] = dminR;
+                st_i[++sp] = closer;
+                st_d[sp] = dminL;
+            } else if (hitL) {
+                st_i[++sp] = L;
+                st_d[sp] = dminL;
+            } else if (hitR) {
+                st_i[++sp] = R;
+                st_d[sp] = dminR;
+            }
+        }
+    }
+    if (best >= 0) {
+        DTri tr = prims[best];
+        F3 a = f3(tr.ax, tr.ay, tr.az), b = f3(tr.bx, tr.by, tr.bz), c = f3(tr.cx, tr.cy, tr.cz);
+        its.n = normalize(cross(b - a, c - a));
+        its.prim = tr.index;
+    }
+    return its;
+}
+
+// SNCH3D.intersect (snch3d.py:411-476) + Triangle.ray_intersect (triangle.py:59-86);
+// prim = BVH-sorted index (triangle.py:82)
+DWD Hit intersect3(const SubDev& S, F3 x, F3 v, float r_max) {
+    const DNode3* nodes = (const DNode3*)S.nodes;
+    const DTri* prims = (const DTri*)S.prims;
+    Hit its;
+    its.valid = false;
+    its.on_boundary = false;
+    its.p = its.n = f3(0.0f, 0.0f, 0.0f);
+    its.t = its.u = its.v = its.d = 0.0f;
+    its.prim = 0;
+    int best = -1;
+    float bv = 0.0f, bw = 0.0f;
+    float tmin;
+    DNode3 root = nodes[0];
+    if (!box_ray(nmin(root), nmax(root), x, v, r_max, tmin)) return its;
+    int st_i[DW_MAX_STACK];
+    float st_d[DW_MAX_STACK];
+    int sp = 0;
+    st_i[0] = 0;
+    st_d[0] = tmin;
+    while (sp >= 0) {
+        int ni = st_i[sp];
+        float cd = st_d[sp];
+        sp--;
+        if (!(cd <= r_max)) continue;
+        DNode3 nd = nodes[ni];
+        if (nd.nref > 0) {
+            for (int j = 0; j < nd.nref; ++j) {
+                DTri tr = prims[nd.off + j];
+                F3 a = f3(tr.ax, tr.ay, tr.az), b = f3(tr.bx, tr.by, tr.bz), c = f3(tr.cx, tr.cy, tr.cz);
+                F3 v1 = b - a, v2 = c - a;
+                F3 pp = cross(v, v2);
+                float det = dot(v1, pp);
+                if (!(__builtin_fabsf(det) > kDrEps)) continue;
+                float inv = 1.0f / det;
+                F3 s = x - a;
+                float bu = dot(s, pp) * inv;
+                if (!((bu >= 0.0f) && (bu <= 1.0f))) continue;
+                F3 q = cross(s, v1);
+                float bw_ = dot(v, q) * inv;
+                if (!((bw_ >= 0.0f) && (bu + bw_ <= 1.0f))) continue;
+                float t = dot(v2, q) * inv;
+                if (!((t >= 0.0f) && (t <= r_max))) continue;
+                if (!(t < r_max)) continue;
+                r_max = t;
+                best = nd.off + j;
+                bv = bu;
+                bw = bw_;
+            }
+        } else {
+            int L = ni + 1, R = ni + nd.off;
+            DNode3 nl = nodes[L], nr = nodes[R];
+            float t0, t1;
+            bool h0 = box_ray(nmin(nl), nmax(nl), x, v, r_max, t0);
+            bool h1 = box_ray(nmin(nr), nmax(nr), x, v, r_max, t1);
+            if (h0 && h1) {
+                int closer = L, other = R;
+                if (t1 < t0) {
+                    closer = R, other = L;
+                    float t = t0;
+                    t0 = t1;
+                    t1 = t;
+                }
+                st_i[++sp] = other;
+                st_d[sp] = t1;
+                st_i[++sp] = closer;
+                st_d[sp] = t0;
+            } else if (h0) {
+                st_i[++sp] = L;
+                st_d[sp] = t0;
+            } else if (h1) {
+                st_i[++sp] = R;
+                st_d[sp] = t1;
+            }
+        }
+    }
+    if (best >= 0) {
+        DTri tr = prims[best];
+        F3 a = f3(tr.ax, tr.ay, tr.az), b = f3(tr.bx, tr.by, tr.bz), c = f3(tr.cx, tr.cy, tr.cz);
+        F3 v1 = b - a, v2 = c - a;
+        its.valid = true;
+        its.on_boundary = true;
+        its.p = (a + v1 * bv) + v2 * bw;
+        its.n = normalize(cross(v1, v2));
+        its.u = 1.0f - bv - bw;
+        its.v = bv;
+        its.d = r_max;
+        its.prim = best;
+    }
+    return its;
+}
+
+}  // namespace dwd

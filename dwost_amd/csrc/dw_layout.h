@@ -1,0 +1,99 @@
+// dw_layout.h -- device-resident scene layout shared by the host builder and
+// the HIP kernels.
+//
+// One scene holds up to three sub-scenes, mirroring diff_wost's Scene2D /
+// Scene3D (polyline.py:623-692, mesh.py:592-635): the full boundary, the
+// Dirichlet part and the Neumann part.  Each sub-scene owns a binary BVH in
+// preorder (bvh.h:402-477) whose nodes also carry the SNCH cone and silhouette
+// range (snch.h:21-36).  Records are 16-byte aligned SoA streams so every
+// node / primitive fetch is one or two dwordx4 loads.
+#pragma once
+#include <stdint.h>
+
+#define DW_MAX_STACK 64
+
+struct alignas(16) DNode2 {  // 32 B
+    float mnx, mny, mxx, mxy;
+    int off;   // first primitive (leaf) or right-child offset (internal)
+    int nref;  // > 0 for leaves
+    int sil_off, nsil;
+};
+struct alignas(16) DCone2 {  // 16 B
+    float ax, ay, half_angle, radius;
+};
+struct alignas(16) DNode3 {  // 32 B
+    float mnx, mny, mnz, mxx;
+    float mxy, mxz;
+    int off, nref;
+};
+struct alignas(16) DCone3 {  // 32 B
+    float ax, ay, az, half_angle;
+    float radius;
+    int sil_off, nsil, pad;
+};
+struct alignas(16) DSeg {  // 2D primitive in BVH order, 16 B (+ index stream)
+    float ax, ay, bx, by;
+};
+struct alignas(16) DTri {  // 3D primitive in BVH order, 48 B
+    float ax, ay, az, bx;
+    float by, bz, cx, cy;
+    float cz;
+    int index;  // original (sub-scene local) primitive index
+    int pad0, pad1;
+};
+struct alignas(16) DSilV {  // 2D silhouette vertex in reference order, 32 B
+    float ax, ay, bx, by;
+    float cx, cy;
+    int both;  // indices[0] != -1 && indices[2] != -1
+    int pad;
+};
+struct alignas(16) DSilE {  // 3D silhouette edge in reference order, 64 B
+    float ax, ay, az;
+    int boundary;  // indices[0] == -1 || indices[3] == -1
+    float bx, by, bz, p0;
+    float cx, cy, cz, p1;
+    float dx, dy, dz, p2;
+};
+
+// Everything one sub-scene exposes to the kernels (device pointers).
+struct SubDev {
+    int present;
+    int n_nodes, n_prims, n_refs;
+    int py_sils;        // star_radius skips the SNCH when false
+    float bb_min[3], bb_max[3];
+    const void* nodes;  // DNode2 / DNode3
+    const void* cones;  // DCone2 / DCone3
+    const void* prims;  // DSeg / DTri (BVH order)
+    const int* prim_index;  // BVH order -> local primitive index (2D)
+    const void* sils;   // DSilV / DSilE (reference ref order)
+    // sampling (local primitive order)
+    const float* cdf;
+    const float* pmf;
+    float pmf_sum, pmf_norm;
+    const float* local_prims;  // dim*dim floats per primitive (a, b[, c])
+    const int* local_vglobal;  // dim global vertex ids per primitive
+    const float* vnormal;      // 2D vertex normals (local vertex order), full scene only
+    const int* local_vidx;     // dim local vertex ids per primitive
+    int n_local_verts;
+};
+
+struct SceneDev {
+    int dim;
+    SubDev sub[3];  // 0 full, 1 Dirichlet, 2 Neumann
+};
+
+// Boundary data: g = u(p) on Dirichlet, h = n . grad u on Neumann.
+#include "../../include/dwost.h"  // DW_BC_* / DW_SRC_* codes
+struct BCDev {
+    int code;
+    float c[4];
+    const float* gv;
+    const float* hv;
+    int src_code;
+    float src_c;
+};
+struct ParamsDev {
+    int nwalks, nsteps, nsubwalks, cv, av;
+    uint32_t seed;
+    float eps, min_R;
+};

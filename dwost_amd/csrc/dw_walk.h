@@ -1,0 +1,305 @@
+// dw_walk.h -- boundary data, sampling and the Walk-on-Stars estimators.
+#pragma once
+#include "dw_device.h"
+
+namespace dwd {
+
+// --------------------------------------------------------------------------
+// Boundary data (diff_wost/shapes/polyline.py:52-58, mesh.py:59-63): g = u(p),
+// h = n . grad u(p) for an analytic harmonic u, or per-vertex g/h buffers
+// interpolated linearly (2D: lerp by t; 3D: barycentric (u, v) as in
+// math.py:356-357).
+// --------------------------------------------------------------------------
+DWD float bc_u(const BCDev& bc, F3 p) {
+    if (bc.code == DW_BC_EXPCOS) return cr_exp(p.x) * cr_cos(p.y);
+    if (bc.code == DW_BC_QUAD) return p.x * p.x - p.y * p.y;
+    return ((bc.c[0] + bc.c[1] * p.x) + bc.c[2] * p.y) + bc.c[3] * p.z;
+}
+DWD F3 bc_grad(const BCDev& bc, F3 p) {
+    if (bc.code == DW_BC_EXPCOS) {
+        float ex = cr_exp(p.x);
+        return f3(ex * cr_cos(p.y), -ex * cr_sin(p.y), 0.0f);
+    }
+    if (bc.code == DW_BC_QUAD) return f3(2.0f * p.x, -2.0f * p.y, 0.0f);
+    return f3(bc.c[1], bc.c[2], bc.c[3]);
+}
+DWD float src_f(const BCDev& bc) { return bc.src_code == DW_SRC_CONST ? bc.src_c : 0.0f; }
+
+DWD float g2(const BCDev& bc, const SubDev& S, const CP& c) {
+    if (bc.code != DW_BC_PER_VERTEX) return bc_u(bc, c.p);
+    return lerpf(bc.gv[S.local_vglobal[2 * c.prim]], bc.gv[S.local_vglobal[2 * c.prim + 1]], c.t);
+}
+DWD float g3(const BCDev& bc, const SubDev& S, const CP& c) {
+    if (bc.code != DW_BC_PER_VERTEX) return bc_u(bc, c.p);
+    float ga = bc.gv[S.local_vglobal[3 * c.prim]], gb = bc.gv[S.local_vglobal[3 * c.prim + 1]],
+          gc = bc.gv[S.local_vglobal[3 * c.prim + 2]];
+    return (gb * c.u + gc * c.v) + ga * ((1.0f - c.u) - c.v);
+}
+DWD float h2(const BCDev& bc, const SubDev& S, int prim, float t, F2 p, F2 n) {
+    if (bc.code != DW_BC_PER_VERTEX) {
+        F3 g = bc_grad(bc, f3(p.x, p.y, 0.0f));
+        return dot(n, f2(g.x, g.y));
+    }
+    return lerpf(bc.hv[S.local_vglobal[2 * prim]], bc.hv[S.local_vglobal[2 * prim + 1]], t);
+}
+DWD float h3(const BCDev& bc, const SubDev& S, int prim, float u, float v, F3 p, F3 n) {
+    if (bc.code != DW_BC_PER_VERTEX) return dot(n, bc_grad(bc, p));
+    float ha = bc.hv[S.local_vglobal[3 * prim]], hb = bc.hv[S.local_vglobal[3 * prim + 1]],
+          hc = bc.hv[S.local_vglobal[3 * prim + 2]];
+    return (hb * u + hc * v) + ha * ((1.0f - u) - v);
+}
+
+// DiscreteDistribution.sample_reuse_pmf (distr.py:52-107) with Dr.Jit's
+// binary_search over [0, n-1] (log2(n-1)+1 halvings).
+DWD int cdf_sample(const SubDev& S, float value, float& reuse, float& pmf) {
+    float sample = value * S.pmf_sum;
+    uint32_t start = 0, end = (uint32_t)(S.n_prims - 1);
+    int iters = start < end ? (32 - __builtin_clz(end - start)) : 0;
+    for (int it = 0; it < iters; ++it) {
+        uint32_t mid = (start + end) >> 1;
+        float cv = S.cdf[mid];
+        bool cond = ((cv < sample) || (cv == 0.0f)) && (cv != S.pmf_sum);
+        if (cond)
+            start = (mid + 1 < end) ? mid + 1 : end;
+        else
+            end = mid;
+    }
+    int idx = (int)start;
+    pmf = S.pmf[idx] * S.pmf_norm;
+    float cdf = idx > 0 ? S.cdf[idx - 1] * S.pmf_norm : 0.0f;
+    reuse = (value - cdf) / pmf;
+    return idx;
+}
+
+// GreensBall (greens_fn.py:14-287) and GreensBall3D (greens_fn_3d.py:5-56)
+DWD float greens2_G(F2 c, float R, F2 x, F2 y) {
+    x = (x - c) / R;
+    y = (y - c) / R;
+    float v = norm(x - y);
+    return kNegInv2Pi * cr_log((1.0f - dot(x, y)) / v);
+}
+DWD void greens2_sample(F2 c, float R, F2 dir, Rng& rng, F2& p, float& r, float& pdf) {
+    float bound = 1.5f / R;
+    int iteration = 0;
+    bool active = true;
+    r = 0.0f;
+    pdf = 0.0f;
+    while ((iteration < 1000) && active) {  // rejection sampling of r ~ r ln(R/r)
+        float u = rng.next();
+        r = rng.next() * R;
+        pdf = (cr_log(R / r) / kTwoPi) / (R * R / 4.0f);
+        float pdf_radius = pdf / (1.0f / (kTwoPi * r));
+        iteration += 1;
+        if (u < pdf_radius / bound) active = false;
+    }
+    r = fmx(r, 1e-6f);
+    if (r > R) r = R / 2.0f;
+    p = c + dir * r;
+}
+DWD F2 greens2_gradient(F2 c, float R, float r, F2 y) {
+    float gn = (1.0f / (r * r) - 1.0f / (R * R)) / kTwoPi;
+    return (y - c) * gn;
+}
+DWD float greens3_G(F3 c, float R, F3 x, F3 y) {
+    float r = norm(y - x);
+    return -(1.0f / r - R / (R * R - dot(x - c, y - c))) / kFourPi;
+}
+DWD void greens3_sample(F3 c, float R, F3 dir, Rng& rng, F3& p, float& r, float& pdf) {
+    float u1 = rng.next();
+    float u2 = rng.next();
+    float phi = kTwoPi * u2;  // Ulrich's polar method
+    r = (1.0f + __builtin_sqrtf(1.0f - cr_cbrt(u1 * u1)) * cr_cos(phi)) * R / 2.0f;
+    r = fmx(r, 1e-6f);
+    if (r > R) r = R / 2.0f;
+    pdf = ((1.0f / r - 1.0f / R) / kFourPi) / (R * R / 6.0f);
+    p = c + dir * r;
+}
+DWD F3 greens3_gradient(F3 c, float R, float r, F3 y) {
+    float r3 = r * r * r;
+    float gn = (1.0f / r3 - 1.0f / (R * R * R)) / kFourPi;
+    return (y - c) * gn;
+}
+DWD F2 uniform_dir2(Rng& rng) {  // uniform_angle (math.py:325-327)
+    float theta = (rng.next() * 2.0f) * kPi;
+    return f2(cr_cos(theta), cr_sin(theta));
+}
+DWD F3 uniform_sphere(Rng& rng) {  // math.py:348-353
+    float u0 = rng.next();
+    float u1 = rng.next();
+    float z = 1.0f - 2.0f * u0;
+    float r = __builtin_sqrtf(fmx(0.0f, 1.0f - z * z));
+    float theta = kTwoPi * u1;
+    return f3(r * cr_cos(theta), r * cr_sin(theta), z);
+}
+
+// sdf / inside on the full scene (polyline.py:188-234, mesh.py:290-297)
+DWD bool inside2(const SceneDev& sc, F2 x) {
+    const SubDev& F = sc.sub[0];
+    CP c = closest2(F, x);
+    const float2* vn = (const float2*)F.vnormal;
+    float2 a = vn[F.local_vidx[2 * c.prim]], b = vn[F.local_vidx[2 * c.prim + 1]];
+    F2 n = lerp(f2(a.x, a.y), f2(b.x, b.y), c.t);
+    float s = dot(n, x - f2(c.p.x, c.p.y)) >= 0.0f ? 1.0f : -1.0f;
+    return c.d * s < 0.0f;
+}
+DWD bool inside3(const SceneDev& sc, F3 x) {
+    CP c = closest3(sc.sub[0], x);
+    float s = dot(c.n, x - c.p) >= 0.0f ? 1.0f : -1.0f;
+    return c.d * s < 0.0f;
+}
+
+// --------------------------------------------------------------------------
+// WoSt.u (wost.py:145-221)
+// --------------------------------------------------------------------------
+DWD float walk_u2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, F2 x, Rng& rng, int& steps, int& term) {
+    const SubDev& F = sc.sub[0];
+    const SubDev& Dn = sc.sub[1];
+    const SubDev& Nm = sc.sub[2];
+    F2 ip = x, in = f2(0.0f, 0.0f);
+    bool ob = false;
+    float result = 0.0f;
+    int i = 0;
+    bool active = true;
+    term = -1;
+    while ((i < P.nsteps) && active) {
+        float R = kInf, d_dir = kInf;
+        if (Dn.present) R = d_dir = __builtin_fabsf(closest2(Dn, ip).d);
+        if (Nm.present) {
+            F2 p = ob ? ip - in * 1e-5f : ip;
+            float sr = star_radius2(Nm, p, d_dir) * 0.98f;
+            sr = fmx(sr, P.min_R);
+            R = fmn(R, sr);
+        }
+        if (d_dir < P.eps) {
+            CP cp = closest2(Dn, ip);
+            result += g2(bc, Dn, cp);
+            term = cp.prim;
+            active = false;
+        } else {
+            F2 ref = f2(0.0f, 0.0f);
+            if (Nm.present) {
+                float rnd, pmf;
+                int idx = cdf_sample(Nm, rng.next(), rnd, pmf);
+                float4 e = ((const float4*)Nm.local_prims)[idx];
+                F2 a = f2(e.x, e.y), b = f2(e.z, e.w);
+                F2 tg = b - a;
+                F2 bn = normalize(f2(tg.y, -tg.x));
+                F2 bp = lerp(a, b, rnd);
+                float bpdf = pmf / norm(tg);
+                CP cr = closest2(F, ip);  // boundary_control_variates (wost.py:123-126)
+                float hc = h2(bc, F, cr.prim, cr.t, f2(cr.p.x, cr.p.y), f2(cr.n.x, cr.n.y));
+                ref = f2(hc * cr.n.x, hc * cr.n.y);
+                float alpha = ob ? 2.0f : 1.0f;  // handle_boundary (wost.py:104-121)
+                float hb = 0.0f;
+                float G = clampf(greens2_G(ip, R, ip, bp), -10.0f, 10.0f);
+                if (norm(ip - bp) <= R) {
+                    float h = h2(bc, Nm, idx, rnd, bp, bn);
+                    if (P.cv) h = h - dot(ref, bn);
+                    hb = 0.0f - G * h / bpdf;
+                }
+                result += alpha * hb;
+            }
+            F2 dir = uniform_dir2(rng);
+            if (ob && (dot(dir, in) > 0.0f)) dir = -dir;
+            F2 _p = ip;
+            F2 o = ob ? ip - in * 1e-5f : ip;  // Polyline.intersect (polyline.py:435-466)
+            Hit its = intersect2(F, o, dir, R);
+            if (its.valid) {
+                ip = f2(its.p.x, its.p.y);
+                in = f2(its.n.x, its.n.y);
+                ob = true;
+            } else {
+                ip = o + dir * R;
+                in = dir;
+                its.d = R;
+                ob = false;
+            }
+            if (P.cv) result += dot(_p - ip, ref);
+            F2 sp;  // handle_source (wost.py:128-143)
+            float sr_, spdf;
+            greens2_sample(_p, R, dir, rng, sp, sr_, spdf);
+            float src = 0.0f;
+            if (norm(sp - _p) < its.d) src += (R * R / 4.0f) * src_f(bc);
+            result += src;
+        }
+        i += 1;
+    }
+    if (bad(result) || __builtin_fabsf(result) > 100.0f) result = 0.0f;
+    steps = i;
+    return result;
+}
+
+// WoSt3D.u (wost_3d.py:43-106)
+DWD float walk_u3(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, F3 x, Rng& rng, int& steps, int& term) {
+    const SubDev& F = sc.sub[0];
+    const SubDev& Dn = sc.sub[1];
+    const SubDev& Nm = sc.sub[2];
+    F3 ip = x, in = f3(0.0f, 0.0f, 0.0f);
+    bool ob = false;
+    float result = 0.0f;
+    int i = 0;
+    bool active = true;
+    term = -1;
+    while ((i < P.nsteps) && active) {
+        float R = kInf, d_dir = kInf;
+        if (Dn.present) R = d_dir = __builtin_fabsf(closest3(Dn, ip).d);
+        F3 p = ob ? ip - in * 1e-3f : ip;
+        float sr = star_radius3(F, p, d_dir) * 0.98f;
+        sr = fmx(sr, P.min_R);
+        R = fmn(R, sr);
+        if (d_dir < P.eps) {
+            CP cp = closest3(Dn, ip);
+            result += g3(bc, Dn, cp);
+            term = cp.prim;
+            active = false;
+        } else {
+            if (Nm.present) {  // Mesh.sample_boundary (mesh.py:474-490)
+                float rnd, pmf;
+                int idx = cdf_sample(Nm, rng.next(), rnd, pmf);
+                const float* t = Nm.local_prims + 9 * idx;
+                F3 a = f3(t[0], t[1], t[2]), b = f3(t[3], t[4], t[5]), c = f3(t[6], t[7], t[8]);
+                F3 cr = cross(b - a, c - a);
+                float area = norm(cr) / 2.0f;
+                float r1 = rng.next();
+                float r2 = rng.next();
+                float su = __builtin_sqrtf(r1) * (1.0f - r2);
+                float sv = __builtin_sqrtf(r1) * r2;
+                F3 bp = (b * su + c * sv) + a * ((1.0f - su) - sv);
+                F3 bn = normalize(cr);
+                float bpdf = pmf / area;
+                float G = clampf(greens3_G(ip, R, ip, bp), -100.0f, 100.0f);
+                float alpha = ob ? 2.0f : 1.0f;
+                if (norm(ip - bp) <= R) {
+                    float h = h3(bc, Nm, idx, su, sv, bp, bn);
+                    result -= G * h * alpha / bpdf;
+                }
+            }
+            F3 dir = uniform_sphere(rng);
+            if (ob && (dot(dir, in) > 0.0f)) dir = -dir;
+            F3 _p = ip;
+            F3 o = ob ? ip - in * 1e-5f : ip;  // Mesh.intersect (mesh.py:383-415)
+            Hit its = intersect3(F, o, dir, R);
+            if (its.valid) {
+                ip = its.p;
+                in = its.n;
+                ob = true;
+            } else {
+                ip = o + dir * R;
+                in = dir;
+                its.d = R;
+                ob = false;
+            }
+            F3 sp;
+            float sr_, spdf;
+            greens3_sample(_p, R, dir, rng, sp, sr_, spdf);
+            float src = 0.0f;
+            if (norm(sp - _p) < its.d) src += (R * R / 6.0f) * src_f(bc);
+            result += src;
+        }
+        i += 1;
+    }
+    steps = i;
+    return result;
+}
+
+}  // namespace dwd

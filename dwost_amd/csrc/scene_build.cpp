@@ -1,0 +1,532 @@
+// scene_build.cpp -- see scene_build.h.  Pure host C++ (no HIP), compiled with
+// -ffp-contract=off so that every fp32 operation rounds exactly as written.
+#include "scene_build.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+#include <utility>
+
+namespace dwb {
+namespace {
+
+const float kEps = (float)1e-9;  // include/fwd.h:14 (Float = float)
+const float kPiF = (float)3.14159265358979323846;
+
+inline float cmin(float a, float b) { return (b < a) ? b : a; }  // Eigen cwiseMin
+inline float cmax(float a, float b) { return (a < b) ? b : a; }  // Eigen cwiseMax
+
+template <int D>
+struct Bx {
+    float lo[D], hi[D];
+    void reset() {
+        for (int d = 0; d < D; ++d) lo[d] = FLT_MAX, hi[d] = -FLT_MAX;
+    }
+    void grow_pt(const float* p) {  // bounding_volume.h:119-123 (epsilon-inflated point)
+        for (int d = 0; d < D; ++d) {
+            lo[d] = cmin(lo[d], p[d] - kEps);
+            hi[d] = cmax(hi[d], p[d] + kEps);
+        }
+    }
+    void grow(const Bx& o) {
+        for (int d = 0; d < D; ++d) {
+            lo[d] = cmin(lo[d], o.lo[d]);
+            hi[d] = cmax(hi[d], o.hi[d]);
+        }
+    }
+    float area() const {  // bounding_volume.h:191-194, Eigen reduction order
+        float e0 = cmax(hi[0] - lo[0], 1e-5f), e1 = cmax(hi[1] - lo[1], 1e-5f);
+        if (D == 2) {
+            float p = e0 * e1;
+            return 2.0f * (p / e0 + p / e1);
+        }
+        float e2 = cmax(hi[D - 1] - lo[D - 1], 1e-5f);
+        float p = e0 * (e1 * e2);
+        return 2.0f * (p / e0 + (p / e1 + p / e2));
+    }
+};
+
+template <int D>
+struct Bvh {
+    int n = 0;
+    std::vector<Bx<D>> pbox;
+    std::vector<float> pcen;  // D per primitive
+    std::vector<int> order;   // current position -> local primitive id
+    // output (preorder)
+    std::vector<Bx<D>> nbox;
+    std::vector<int> noff, nref;
+    int max_depth = 0;
+
+    void run(const std::vector<float>& V, const std::vector<int>& I) {
+        n = (int)(I.size() / D);
+        pbox.resize(n);
+        pcen.resize((size_t)n * D);
+        order.resize(n);
+        for (int p = 0; p < n; ++p) {
+            const float* a = &V[(size_t)D * I[(size_t)D * p]];
+            const float* b = &V[(size_t)D * I[(size_t)D * p + 1]];
+            pbox[p].reset();
+            pbox[p].grow_pt(a);
+            pbox[p].grow_pt(b);
+            if (D == 3) {
+                const float* c = &V[(size_t)D * I[(size_t)D * p + 2]];
+                pbox[p].grow_pt(c);
+                for (int d = 0; d < 3; ++d) pcen[(size_t)3 * p + d] = ((a[d] + b[d]) + c[d]) / 3.0f;  // primitive.h:99-101
+            } else {
+                for (int d = 0; d < 2; ++d) pcen[(size_t)2 * p + d] = (a[d] + b[d]) * 0.5f;  // primitive.h:29-31
+            }
+            order[p] = p;
+        }
+        if (n == 0) return;
+        nbox.reserve(2 * n);
+        noff.reserve(2 * n);
+        nref.reserve(2 * n);
+        node(-1, 0, n, 0);
+    }
+
+    void swap_prim(int i, int j) {
+        std::swap(order[i], order[j]);
+        std::swap(pbox[i], pbox[j]);
+        for (int d = 0; d < D; ++d) std::swap(pcen[(size_t)D * i + d], pcen[(size_t)D * j + d]);
+    }
+
+    // buildRecursive (bvh.h:402-477)
+    void node(int parent, int start, int end, int depth) {
+        if (depth > max_depth) max_depth = depth;
+        int me = (int)noff.size();
+        int cnt = end - start;
+        Bx<D> bb, bc;
+        bb.reset();
+        bc.reset();
+        for (int p = start; p < end; ++p) {
+            bb.grow(pbox[p]);
+            bc.grow_pt(&pcen[(size_t)D * p]);
+        }
+        bool leaf = cnt <= 4 || depth == 62;
+        nbox.push_back(bb);
+        noff.push_back(leaf ? start : -1);  // -1 == UNTOUCHED
+        nref.push_back(leaf ? cnt : 0);
+        if (parent >= 0) {
+            // second touch of the parent: right-child offset (bvh.h:446-454)
+            noff[parent] -= 1;
+            if (noff[parent] == -3) noff[parent] = me - parent;
+        }
+        if (leaf) return;
+        int dim;
+        float coord;
+        split(bb, bc, start, end, dim, coord);
+        int mid = start;  // performObjectSplit (bvh.h:622-652)
+        for (int i = start; i < end; ++i)
+            if (pcen[(size_t)D * i + dim] < coord) swap_prim(i, mid++);
+        if (mid == start || mid == end) mid = start + (end - start) / 2;
+        node(me, start, mid, depth + 1);
+        node(me, mid, end, depth + 1);
+    }
+
+    // computeObjectSplit (bvh.h:537-620): 8 buckets per axis, strict-min SAH
+    void split(const Bx<D>& bb, const Bx<D>& bc, int start, int end, int& dim, float& coord) {
+        float best = FLT_MAX;
+        dim = -1;
+        coord = 0.0f;
+        for (int d = 0; d < D; ++d) {
+            float ext = bb.hi[d] - bb.lo[d];
+            if (ext < 1e-6f) continue;
+            float w = ext / 8.0f;
+            Bx<D> bk[8];
+            int bn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int b = 0; b < 8; ++b) bk[b].reset();
+            for (int p = start; p < end; ++p) {
+                int bi = (int)((pcen[(size_t)D * p + d] - bb.lo[d]) / w);
+                bi = bi < 0 ? 0 : (bi > 7 ? 7 : bi);
+                bk[bi].grow(pbox[p]);
+                bn[bi] += 1;
+            }
+            Bx<D> rb[8];
+            int rn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            Bx<D> acc;
+            acc.reset();
+            for (int b = 7; b > 0; --b) {
+                acc.grow(bk[b]);
+                rb[b] = acc;
+                rn[b] = bn[b] + (b != 7 ? rn[b + 1] : 0);
+            }
+            Bx<D> lb;
+            lb.reset();
+            int ln = 0;
+            for (int b = 1; b < 8; ++b) {
+                lb.grow(bk[b - 1]);
+                ln += bn[b - 1];
+                if (ln > 0 && rn[b] > 0) {
+                    float cost = (float)ln * lb.area() + (float)rn[b] * rb[b].area();
+                    if (cost < best) {
+                        best = cost;
+                        dim = d;
+                        coord = bb.lo[d] + (float)b * w;
+                    }
+                }
+            }
+        }
+        if (dim == -1) {  // longest centroid axis, first index of the max
+            dim = 0;
+            float e = bc.hi[0] - bc.lo[0];
+            for (int d = 1; d < D; ++d)
+                if (bc.hi[d] - bc.lo[d] > e) e = bc.hi[d] - bc.lo[d], dim = d;
+            coord = (bc.lo[dim] + bc.hi[dim]) * 0.5f;
+        }
+    }
+};
+
+// Eigen-order helpers for the SNCH set-up
+template <int D>
+inline float sqn(const float* v) {
+    return D == 2 ? v[0] * v[0] + v[1] * v[1] : v[0] * v[0] + (v[1] * v[1] + v[2] * v[2]);
+}
+template <int D>
+inline void nrmz(float* v) {  // Eigen normalized()
+    float z = sqn<D>(v);
+    if (z > 0.0f) {
+        float s = std::sqrt(z);
+        for (int d = 0; d < D; ++d) v[d] /= s;
+    }
+}
+template <int D>
+inline float edot(const float* a, const float* b) {
+    return D == 2 ? a[0] * b[0] + a[1] * b[1] : a[0] * b[0] + (a[1] * b[1] + a[2] * b[2]);
+}
+
+// Face normals of a topology silhouette (silhouette_vertex.h:53-76,
+// silhouette_edge.h:63-88).  pts: 2D a,b,c / 3D a,b,c,d; idx as in the header.
+template <int D>
+inline bool has_face(const int* idx, int f) {
+    if (D == 2) return f == 0 ? idx[2] != -1 : idx[0] != -1;
+    return f == 0 ? idx[0] != -1 : idx[3] != -1;
+}
+template <int D>
+inline void face_normal(const float* pts, int f, float* n) {
+    if (D == 2) {
+        const float* pa = f == 0 ? pts + 2 : pts;
+        const float* pb = f == 0 ? pts + 4 : pts + 2;
+        n[0] = pb[1] - pa[1];
+        n[1] = -(pb[0] - pa[0]);
+    } else {
+        const float *a = pts, *b = pts + 3, *c = pts + 6, *d = pts + 9;
+        float u[3], v[3];
+        for (int k = 0; k < 3; ++k) {
+            u[k] = f == 0 ? b[k] - a[k] : d[k] - a[k];
+            v[k] = f == 0 ? c[k] - a[k] : b[k] - a[k];
+        }
+        n[0] = u[1] * v[2] - u[2] * v[1];
+        n[1] = u[2] * v[0] - u[0] * v[2];
+        n[2] = u[0] * v[1] - u[1] * v[0];
+    }
+}
+
+template <int D>
+void build_snch(SubHost& S, const std::vector<int>& conn) {
+    const int P = D == 2 ? 6 : 12, X = D == 2 ? 3 : 4;
+    size_t nn = S.n_nodes();
+    S.node_sil_off.assign(nn, 0);
+    S.node_nsil.assign(nn, 0);
+    std::vector<float> refN, refF;
+    std::vector<int> seen_stamp(S.sil_idx.size() / X, -1);
+    for (size_t i = 0; i < nn; ++i) {  // assignGeometricDataToNodes (snch.h:174-306)
+        int start = (int)S.refs.size();
+        for (int j = 0; j < S.node_nref[i]; ++j) {
+            int prim = S.prim_order[S.node_off[i] + j];
+            for (int k = 0; k < D; ++k) {
+                int si = conn[(size_t)D * prim + k];
+                if (seen_stamp[si] == (int)i) continue;
+                seen_stamp[si] = (int)i;
+                const float* pts = &S.sil_pts[(size_t)P * si];
+                const int* idx = &S.sil_idx[(size_t)X * si];
+                float n0[3] = {0, 0, 0}, n1[3] = {0, 0, 0}, n[3] = {0, 0, 0}, t[3];
+                if (has_face<D>(idx, 0)) face_normal<D>(pts, 0, n0), nrmz<D>(n0);
+                if (has_face<D>(idx, 1)) face_normal<D>(pts, 1, n1), nrmz<D>(n1);
+                for (int f = 0; f < 2; ++f)
+                    if (has_face<D>(idx, f)) {
+                        face_normal<D>(pts, f, t);
+                        for (int q = 0; q < D; ++q) n[q] += t[q];
+                    }
+                nrmz<D>(n);
+                S.refs.push_back(si);
+                for (int q = 0; q < D; ++q) refN.push_back(n[q]);
+                for (int q = 0; q < D; ++q) refF.push_back(n0[q]);
+                for (int q = 0; q < D; ++q) refF.push_back(n1[q]);
+            }
+        }
+        S.node_sil_off[i] = start;
+        S.node_nsil[i] = (int)S.refs.size() - start;
+    }
+    S.node_axis.assign(nn * D, 0.0f);
+    S.node_half_angle.assign(nn, kPiF);
+    S.node_radius.assign(nn, 0.0f);
+    // computeBoundingConesRecursive (snch.h:308-385); the subtree of node s is
+    // the contiguous preorder range [s, e).  Explicit stack, same visit order.
+    std::vector<std::pair<int, int>> todo;
+    if (nn) todo.emplace_back(0, (int)nn);
+    while (!todo.empty()) {
+        auto [s, e] = todo.back();
+        todo.pop_back();
+        float cen[3];
+        for (int d = 0; d < D; ++d) cen[d] = (S.node_box[(size_t)2 * D * s + d] + S.node_box[(size_t)2 * D * s + D + d]) * 0.5f;
+        float axis[3] = {0, 0, 0}, radius = 0.0f;
+        bool any = false, two = true;
+        int r0 = S.node_sil_off[s], r1 = r0;
+        for (int i = s; i < e; ++i)
+            if (S.node_nsil[i]) r1 = S.node_sil_off[i] + S.node_nsil[i];
+        // refs of the subtree are contiguous: [first nonempty node's offset, last end)
+        for (int i = s; i < e; ++i) {
+            if (!S.node_nsil[i]) continue;
+            r0 = std::min(r0, S.node_sil_off[i]);
+        }
+        for (int i = s; i < e; ++i)
+            for (int j = 0; j < S.node_nsil[i]; ++j) {
+                int r = S.node_sil_off[i] + j;
+                int si = S.refs[r];
+                const float* pts = &S.sil_pts[(size_t)P * si];
+                const int* idx = &S.sil_idx[(size_t)X * si];
+                for (int q = 0; q < D; ++q) axis[q] += refN[(size_t)D * r + q];
+                float sc[3], dv[3];
+                if (D == 2) {
+                    sc[0] = pts[2], sc[1] = pts[3];
+                } else {
+                    for (int q = 0; q < 3; ++q) sc[q] = (pts[q] + pts[3 + q]) * 0.5f;
+                }
+                for (int q = 0; q < D; ++q) dv[q] = sc[q] - cen[q];
+                radius = std::max(radius, std::sqrt(sqn<D>(dv)));
+                two = two && has_face<D>(idx, 0) && has_face<D>(idx, 1);
+                any = true;
+            }
+        (void)r1;
+        if (!any) {
+            S.node_half_angle[s] = -kPiF;
+        } else if (!two) {
+            S.node_half_angle[s] = kPiF;
+        } else {
+            float an = std::sqrt(sqn<D>(axis));
+            if (an > kEps) {
+                for (int q = 0; q < D; ++q) axis[q] /= an;
+                float ha = 0.0f;
+                for (int i = s; i < e; ++i)
+                    for (int j = 0; j < S.node_nsil[i]; ++j) {
+                        int r = S.node_sil_off[i] + j;
+                        for (int k = 0; k < 2; ++k) {
+                            const float* fn = &refF[((size_t)2 * r + k) * D];
+                            float c = std::max(-1.0f, std::min(1.0f, edot<D>(axis, fn)));
+                            ha = std::max(ha, (float)std::acos((double)c));
+                        }
+                    }
+                for (int q = 0; q < D; ++q) S.node_axis[(size_t)D * s + q] = axis[q];
+                S.node_half_angle[s] = ha;
+                S.node_radius[s] = radius;
+            }
+        }
+        if (S.node_nref[s] == 0) {
+            int r = s + S.node_off[s];
+            todo.emplace_back(r, e);      // processed second
+            todo.emplace_back(s + 1, r);  // processed first
+        }
+    }
+}
+
+template <int D>
+void finish_sub(SubHost& S) {
+    S.present = true;
+    S.dim = D;
+    size_t nv = S.V.size() / D, np = S.I.size() / D;
+    for (int d = 0; d < D; ++d) S.bb_min[d] = INFINITY, S.bb_max[d] = -INFINITY;
+    for (size_t v = 0; v < nv; ++v)
+        for (int d = 0; d < D; ++d) {
+            S.bb_min[d] = std::min(S.bb_min[d], S.V[D * v + d]);
+            S.bb_max[d] = std::max(S.bb_max[d], S.V[D * v + d]);
+        }
+    // sampling pmf + sequential fp32 cdf (distr.py:44-50)
+    S.pmf.resize(np);
+    S.cdf.resize(np);
+    float acc = 0.0f;
+    for (size_t p = 0; p < np; ++p) {
+        float w;
+        const float* a = &S.V[(size_t)D * S.I[D * p]];
+        const float* b = &S.V[(size_t)D * S.I[D * p + 1]];
+        if (D == 2) {
+            float dx = b[0] - a[0], dy = b[1] - a[1];
+            w = std::sqrt(std::fmaf(dy, dy, dx * dx));
+        } else {
+            const float* c = &S.V[(size_t)3 * S.I[3 * p + 2]];
+            float u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, v[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+            float x = std::fmaf(u[1], v[2], -(u[2] * v[1]));
+            float y = std::fmaf(u[2], v[0], -(u[0] * v[2]));
+            float z = std::fmaf(u[0], v[1], -(u[1] * v[0]));
+            w = std::sqrt(std::fmaf(z, z, std::fmaf(y, y, x * x)));
+        }
+        S.pmf[p] = w;
+        acc += w;
+        S.cdf[p] = acc;
+    }
+    S.pmf_sum = np ? S.cdf[np - 1] : 0.0f;
+    S.pmf_norm = 1.0f / S.pmf_sum;
+    if (D == 2) {
+        // vertex normals (polyline.py:173-186): scatter edge normals to .x then .y
+        S.vnormal.assign(nv * 2, 0.0f);
+        std::vector<float> en(np * 2);
+        for (size_t p = 0; p < np; ++p) {
+            const float* a = &S.V[2 * S.I[2 * p]];
+            const float* b = &S.V[2 * S.I[2 * p + 1]];
+            float nx = b[1] - a[1], ny = -(b[0] - a[0]);
+            float inv = 1.0f / std::sqrt(std::fmaf(ny, ny, nx * nx));
+            en[2 * p] = nx * inv;
+            en[2 * p + 1] = ny * inv;
+        }
+        for (int pass = 0; pass < 2; ++pass)
+            for (size_t p = 0; p < np; ++p) {
+                int v = S.I[2 * p + pass];
+                S.vnormal[2 * v] += en[2 * p];
+                S.vnormal[2 * v + 1] += en[2 * p + 1];
+            }
+        for (size_t v = 0; v < nv; ++v) {
+            float x = S.vnormal[2 * v], y = S.vnormal[2 * v + 1];
+            float inv = 1.0f / std::sqrt(std::fmaf(y, y, x * x));
+            S.vnormal[2 * v] = x * inv;
+            S.vnormal[2 * v + 1] = y * inv;
+        }
+        // python-side silhouettes exist iff some vertex has both prev and next (polyline.py:118-130)
+        std::vector<char> hp(nv, 0), hn(nv, 0);
+        for (size_t p = 0; p < np; ++p) hp[S.I[2 * p + 1]] = 1, hn[S.I[2 * p]] = 1;
+        S.py_sils = false;
+        for (size_t v = 0; v < nv; ++v) S.py_sils = S.py_sils || (hp[v] && hn[v]);
+    } else {
+        // python-side silhouettes exist iff an undirected edge has exactly two faces (mesh.py:177-199)
+        std::unordered_map<uint64_t, int> cnt;
+        cnt.reserve(np * 3);
+        for (size_t p = 0; p < np; ++p)
+            for (int k = 0; k < 3; ++k) {
+                uint64_t u = (uint32_t)S.I[3 * p + k], v = (uint32_t)S.I[3 * p + (k + 1) % 3];
+                cnt[std::min(u, v) << 32 | std::max(u, v)]++;
+            }
+        S.py_sils = false;
+        for (auto& kv : cnt) S.py_sils = S.py_sils || kv.second == 2;
+    }
+    // BVH
+    Bvh<D> B;
+    B.run(S.V, S.I);
+    S.max_depth = B.max_depth;
+    size_t nn = B.noff.size();
+    S.node_box.resize(nn * 2 * D);
+    for (size_t i = 0; i < nn; ++i)
+        for (int d = 0; d < D; ++d) {
+            S.node_box[i * 2 * D + d] = B.nbox[i].lo[d];
+            S.node_box[i * 2 * D + D + d] = B.nbox[i].hi[d];
+        }
+    S.node_off = B.noff;
+    S.node_nref = B.nref;
+    S.prim_order = B.order;
+    // SNCH topology (snch.h:407-489)
+    std::vector<int> conn;
+    if (D == 2) {
+        std::vector<std::vector<int>> adj(nv);
+        for (size_t p = 0; p < np; ++p) {
+            adj[S.I[2 * p]].push_back(S.I[2 * p + 1]);
+            adj[S.I[2 * p + 1]].push_back(S.I[2 * p]);
+        }
+        S.sil_pts.assign(nv * 6, 0.0f);
+        S.sil_idx.assign(nv * 3, -1);
+        for (size_t v = 0; v < nv; ++v) {
+            int pv = adj[v].size() >= 1 ? adj[v][0] : -1, nx = adj[v].size() >= 2 ? adj[v][1] : -1;
+            float* pts = &S.sil_pts[6 * v];
+            if (pv != -1) pts[0] = S.V[2 * pv], pts[1] = S.V[2 * pv + 1];
+            pts[2] = S.V[2 * v], pts[3] = S.V[2 * v + 1];
+            if (nx != -1) pts[4] = S.V[2 * nx], pts[5] = S.V[2 * nx + 1];
+            S.sil_idx[3 * v] = pv;
+            S.sil_idx[3 * v + 1] = (int)v;
+            S.sil_idx[3 * v + 2] = nx;
+        }
+        conn = S.I;
+    } else {
+        std::unordered_map<uint64_t, int> emap;
+        emap.reserve(np * 2);
+        conn.assign(np * 3, -1);
+        for (size_t f = 0; f < np; ++f)
+            for (int k = 0; k < 3; ++k) {
+                int u = S.I[3 * f + k], v = S.I[3 * f + (k + 1) % 3], w = S.I[3 * f + (k + 2) % 3];
+                uint64_t key = (uint64_t)(uint32_t)std::min(u, v) << 32 | (uint32_t)std::max(u, v);
+                auto it = emap.find(key);
+                if (it == emap.end()) {
+                    int e = (int)(S.sil_idx.size() / 4);
+                    emap.emplace(key, e);
+                    for (int q = 0; q < 3; ++q) S.sil_pts.push_back(S.V[3 * u + q]);
+                    for (int q = 0; q < 3; ++q) S.sil_pts.push_back(S.V[3 * v + q]);
+                    for (int q = 0; q < 3; ++q) S.sil_pts.push_back(S.V[3 * w + q]);
+                    for (int q = 0; q < 3; ++q) S.sil_pts.push_back(0.0f);
+                    S.sil_idx.push_back(w);
+                    S.sil_idx.push_back(u);
+                    S.sil_idx.push_back(v);
+                    S.sil_idx.push_back(-1);
+                    conn[3 * f + k] = e;
+                } else {
+                    int e = it->second;
+                    for (int q = 0; q < 3; ++q) S.sil_pts[(size_t)12 * e + 9 + q] = S.V[3 * w + q];
+                    S.sil_idx[(size_t)4 * e + 3] = w;
+                    conn[3 * f + k] = e;
+                }
+            }
+    }
+    build_snch<D>(S, conn);
+}
+
+// _build_subscene_from_mask (polyline.py:672-692, mesh.py:617-635)
+template <int D>
+void make_sub(const SceneHost& sc, int want, SubHost& S) {
+    size_t np = sc.types.size();
+    for (size_t p = 0; p < np; ++p)
+        if (want < 0 || sc.types[p] == want) S.pglobal.push_back((int)p);
+    if (S.pglobal.empty()) {
+        S.present = false;
+        S.dim = D;
+        return;
+    }
+    size_t nvg = sc.V.size() / D;
+    std::vector<int> used(nvg, 0);
+    for (int p : S.pglobal)
+        for (int k = 0; k < D; ++k) used[sc.I[(size_t)D * p + k]] = 1;
+    std::vector<int> remap(nvg, -1);
+    for (size_t v = 0; v < nvg; ++v)
+        if (used[v]) {
+            remap[v] = (int)S.vglobal.size();
+            S.vglobal.push_back((int)v);
+        }
+    S.V.resize(S.vglobal.size() * D);
+    for (size_t i = 0; i < S.vglobal.size(); ++i)
+        for (int d = 0; d < D; ++d) S.V[i * D + d] = sc.V[(size_t)D * S.vglobal[i] + d];
+    S.I.resize(S.pglobal.size() * D);
+    for (size_t j = 0; j < S.pglobal.size(); ++j)
+        for (int k = 0; k < D; ++k) S.I[j * D + k] = remap[sc.I[(size_t)D * S.pglobal[j] + k]];
+    finish_sub<D>(S);
+}
+
+}  // namespace
+
+std::string build_scene(int dim, const float* V, int64_t nv, const int32_t* I, const int32_t* types, int64_t np,
+                        SceneHost& out) {
+    if (dim != 2 && dim != 3) return "dim must be 2 or 3";
+    if (nv <= 0 || np <= 0) return "scene needs at least one vertex and one primitive";
+    for (int64_t i = 0; i < np * dim; ++i)
+        if (I[i] < 0 || I[i] >= nv) return "primitive index out of range";
+    for (int64_t i = 0; i < np; ++i)
+        if (types[i] != 0 && types[i] != 1) return "types must be 0 (Neumann) or 1 (Dirichlet)";
+    out.dim = dim;
+    out.V.assign(V, V + nv * dim);
+    out.I.assign(I, I + np * dim);
+    out.types.assign(types, types + np);
+    for (int s = 0; s < 3; ++s) {
+        int want = s == 0 ? -1 : (s == 1 ? 1 : 0);
+        if (dim == 2)
+            make_sub<2>(out, want, out.sub[s]);
+        else
+            make_sub<3>(out, want, out.sub[s]);
+    }
+    return "";
+}
+
+}  // namespace dwb

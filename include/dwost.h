@@ -1,0 +1,159 @@
+/* dwost.h -- C ABI of libdwost, the MI355X Walk-on-Stars engine.
+ *
+ * Plain C: pointers, sizes and status codes; no torch or C++ types cross the
+ * boundary.  Every array argument of a query / solve entry point is a DEVICE
+ * pointer (HIP global memory on the current device); `stream` is a hipStream_t
+ * (NULL = default stream).  Calls are asynchronous on `stream`; outputs are
+ * valid once the stream is synchronised.  Functions return DW_OK (0) or a
+ * negative DW_ERR_* code; dw_last_error() gives a thread-local message.
+ *
+ * The entry points replace the reference's nanobind module and the Dr.Jit
+ * traversals that the Python layer runs per query (zihay/dwost):
+ *   scene construction  -> src/python/bind_bvh.cpp:101-109, bind_snch.cpp:110-134
+ *                          + diff_wost/shapes/polyline.py:623-692, mesh.py:592-635
+ *   queries             -> diff_wost/shapes/bvh.py:111-257, snch.py:312-378,
+ *                          snch3d.py:334-476,594-663, polyline.py:304-314
+ *   solvers             -> diff_wost/solvers/solver.py:36-71, wost.py:21-221,
+ *                          wost_3d.py:43-175
+ */
+#ifndef DWOST_H
+#define DWOST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DW_ABI_VERSION 1
+
+#define DW_OK 0
+#define DW_ERR_ARG (-1)
+#define DW_ERR_HIP (-2)
+#define DW_ERR_INTERNAL (-3)
+
+/* sub-scene selector (Scene2D/Scene3D and their dirichlet_scene / neumann_scene) */
+#define DW_SUB_FULL 0
+#define DW_SUB_DIRICHLET 1
+#define DW_SUB_NEUMANN 2
+
+/* boundary types (diff_wost/shapes/primitive.py:18-19) */
+#define DW_NEUMANN 0
+#define DW_DIRICHLET 1
+
+/* boundary-value codes: g = u(p) on Dirichlet, h = n . grad u(p) on Neumann */
+#define DW_BC_EXPCOS 0     /* u = e^x cos y                                  */
+#define DW_BC_QUAD 1       /* u = x^2 - y^2                                  */
+#define DW_BC_LINEAR 2     /* u = c0 + c1 x + c2 y + c3 z                    */
+#define DW_BC_PER_VERTEX 3 /* g, h interpolated from per-vertex buffers      */
+/* source codes (the reference's scene.f, wost.py:128-143) */
+#define DW_SRC_ZERO 0
+#define DW_SRC_CONST 1
+
+typedef struct dw_scene dw_scene;
+
+typedef struct dw_scene_desc {
+    int32_t dim;            /* 2: polyline segments, 3: triangle mesh               */
+    int64_t n_vertices;
+    int64_t n_prims;
+    const float* vertices;  /* HOST, n_vertices x dim, row-major                    */
+    const int32_t* indices; /* HOST, n_prims x dim                                  */
+    const int32_t* types;   /* HOST, n_prims, DW_NEUMANN / DW_DIRICHLET             */
+    int32_t flags;          /* DW_SCENE_HOST_ONLY: build on the host, no upload    */
+} dw_scene_desc;
+
+/* Build the trees on the host only (no HIP call; query / solve calls on such
+ * a scene fail with DW_ERR_ARG).  Used to inspect and check the builder. */
+#define DW_SCENE_HOST_ONLY 1
+
+typedef struct dw_boundary {
+    int32_t bc_code;        /* DW_BC_*                                              */
+    float coeffs[4];        /* DW_BC_LINEAR coefficients                            */
+    const float* g_vertex;  /* DEVICE, n_vertices (DW_BC_PER_VERTEX)                */
+    const float* h_vertex;  /* DEVICE, n_vertices (DW_BC_PER_VERTEX)                */
+    int32_t src_code;       /* DW_SRC_*                                             */
+    float src_value;
+} dw_boundary;
+
+typedef struct dw_solver_params { /* Solver / WoSt fields (solver.py:30-34, wost.py:17-19) */
+    int32_t nwalks;
+    int32_t nsteps;
+    int32_t nsubwalks;
+    int32_t control_variates;
+    int32_t antithetic_variates;
+    uint32_t seed;
+    float eps;
+    float min_R;
+} dw_solver_params;
+
+const char* dw_last_error(void);
+int dw_version(void);
+
+/* Scene2D(vertices, indices, types) / Scene3D(...): builds the full,
+ * Dirichlet and Neumann sub-scenes with their BVH + SNCH on the host and
+ * uploads them once (polyline.py:623-692, mesh.py:592-635; builder
+ * include/bvh.h:359-652, include/snch.h:387-489). */
+int dw_scene_create(const dw_scene_desc* desc, dw_scene** out);
+int dw_scene_destroy(dw_scene* scene);
+
+/* out8 = {present, n_vertices, n_prims, n_nodes, n_silhouette_refs, max_depth,
+ *         has_silhouettes, n_topology_silhouettes} of one sub-scene (HOST). */
+int dw_scene_info(const dw_scene* scene, int32_t sub, int64_t* out8);
+
+/* HOST copies of one sub-scene's tree, sized from dw_scene_info:
+ * box[n_nodes][2*dim] (min, max), ints[n_nodes][4] (offset, n_refs,
+ * silhouette offset, n_silhouettes), cone[n_nodes][dim+2] (axis, half angle,
+ * radius), prim_index[n_prims], refs[n_silhouette_refs].  Any may be NULL.
+ * Mirrors convert_snch_nodes / silhouette_refs_soa (bind_snch.cpp:48-134). */
+int dw_scene_export(const dw_scene* scene, int32_t sub, float* box, int32_t* ints, float* cone, int32_t* prim_index,
+                    int32_t* refs);
+
+/* BVH.closest_point (bvh.py:111-188) / SNCH3D.closest_point (snch3d.py:334-409).
+ * pts[n][dim]; outputs (each may be NULL): prim[n] (sub-scene primitive id),
+ * dist[n], point[n][dim], normal[n][dim], tuv[n][2] (2D: t, 0; 3D: u, v). */
+int dw_closest_point(const dw_scene* scene, int32_t sub, const float* pts, int64_t n, int32_t* prim, float* dist,
+                     float* point, float* normal, float* tuv, void* stream);
+
+/* BVH.intersect (bvh.py:190-257, first hit in traversal order) /
+ * SNCH3D.intersect (snch3d.py:411-476, nearest hit; prim = BVH-sorted index).
+ * origin[n][dim], dir[n][dim], rmax[n]; prim[n] = -1 on a miss. */
+int dw_intersect(const dw_scene* scene, int32_t sub, const float* origin, const float* dir, const float* rmax,
+                 int64_t n, int32_t* prim, float* dist, float* point, float* normal, float* tuv, void* stream);
+
+/* SNCH.closest_silhouette (snch.py:312-378, snch3d.py:594-663).  sil[n] is the
+ * silhouette reference index (-1: none within rmax). */
+int dw_closest_silhouette(const dw_scene* scene, int32_t sub, const float* pts, const float* rmax, int64_t n,
+                          int32_t* sil, float* dist, float* point, void* stream);
+
+/* Polyline.star_radius (polyline.py:304-314) / Mesh.star_radius (mesh.py:337-346). */
+int dw_star_radius(const dw_scene* scene, int32_t sub, const float* pts, const float* rmax, int64_t n,
+                   float* radius, void* stream);
+
+/* Solver.solve with WoSt.u / WoSt3D.u (solver.py:36-71, wost.py:145-221,
+ * wost_3d.py:43-106).  u[n] = per-point mean; optional per-walk outputs
+ * walk_u / walk_steps / walk_term [n*nwalks]; total_steps (DEVICE uint64) =
+ * sum of loop iterations.  Without walk_u the library uses an internal
+ * device workspace. */
+int dw_solve_u(const dw_scene* scene, const float* pts, int64_t n, const dw_solver_params* params,
+               const dw_boundary* boundary, float* u, float* walk_u, int32_t* walk_steps, int32_t* walk_term,
+               uint64_t* total_steps, void* stream);
+
+/* WoSt.solve_dude / WoSt3D.solve_dude (wost.py:21-102, wost_3d.py:108-175):
+ * directional derivative du/de at pts, one PCG32 stream per point. */
+int dw_solve_dude(const dw_scene* scene, const float* pts, const float* e, int64_t n, const dw_solver_params* params,
+                  const dw_boundary* boundary, float* out, uint64_t* total_steps, void* stream);
+
+/* Fused u + grad u (the throughput form; no single reference counterpart):
+ * per antithetic pair, the WoSt.dude first-ball step (wost.py:51-99) yields
+ * both u(p) (mean value over the ball) and grad u.  u[n], grad[n][dim];
+ * optional per-pair outputs pair_vals[n*npairs][dim+1], pair_steps[n*npairs]
+ * with npairs = (nwalks+1)/2. */
+int dw_solve_fused(const dw_scene* scene, const float* pts, int64_t n, const dw_solver_params* params,
+                   const dw_boundary* boundary, float* u, float* grad, float* pair_vals, int32_t* pair_steps,
+                   uint64_t* total_steps, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DWOST_H */

@@ -1,0 +1,217 @@
+"""ctypes wrapper of the CPU restatement (oracle/wost_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product (dwost_amd/) never
+imports this module.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+KIND_CLOSEST, KIND_RAY, KIND_SIL, KIND_STAR = 0, 1, 2, 3
+KIND_BRUTE_CLOSEST, KIND_BRUTE_RAY, KIND_BRUTE_SIL = 10, 11, 12
+SUB_FULL, SUB_DIRICHLET, SUB_NEUMANN = 0, 1, 2
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", _HERE])
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        L = _lib
+        L.orc_scene_create.restype = C.c_void_p
+        L.orc_scene_create.argtypes = [C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64]
+        L.orc_scene_destroy.argtypes = [C.c_void_p]
+        L.orc_sub_info.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_sub_export.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 5
+        L.orc_query.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64] + [
+            C.c_void_p] * 5
+        L.orc_ref_topology.restype = C.c_int32
+        L.orc_ref_topology.argtypes = [C.c_void_p, C.c_int, C.c_int32]
+        common = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_float]
+        L.orc_solve_u.argtypes = [C.c_void_p, C.c_void_p, C.c_int64] + common + [C.c_void_p] * 5
+        L.orc_solve_dude.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64] + common + [C.c_void_p] * 2
+        L.orc_solve_fused.argtypes = [C.c_void_p, C.c_void_p, C.c_int64] + common + [C.c_void_p] * 6
+        L.orc_tea.argtypes = [C.c_uint32, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p]
+        L.orc_pcg_seq.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_void_p]
+        L.orc_pcg_lane_floats.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_void_p]
+        L.orc_math.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+        L.orc_num_threads.restype = C.c_int
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def tea(v0: int, v1: int, rounds: int = 4, key=None):
+    out = np.zeros(2, np.uint32)
+    k = None if key is None else np.ascontiguousarray(key, dtype=np.uint32)
+    lib().orc_tea(v0, v1, rounds, _p(k), _p(out))
+    return int(out[0]), int(out[1])
+
+
+def pcg_sequence(initstate: int, initseq: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.uint32)
+    lib().orc_pcg_seq(initstate, initseq, n, _p(out))
+    return out
+
+
+def pcg_lane_floats(seed: int, point: int, lane: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.float32)
+    lib().orc_pcg_lane_floats(seed, point, lane, n, _p(out))
+    return out
+
+
+MATH_FUNCS = {"log": 0, "exp": 1, "sin": 2, "cos": 3, "acos": 4, "asin": 5, "atan2": 6, "cbrt": 7}
+
+
+def math_fn(name: str, x, y=None) -> np.ndarray:
+    x = _f32(x)
+    y = _f32(np.zeros_like(x) if y is None else y)
+    out = np.zeros_like(x)
+    lib().orc_math(MATH_FUNCS[name], _p(x), _p(y), x.size, _p(out))
+    return out
+
+
+class OracleScene:
+    """CPU restatement of Scene2D / Scene3D with its sub-scenes."""
+
+    def __init__(self, vertices, indices, types):
+        self.V = _f32(vertices)
+        self.I = np.ascontiguousarray(indices, dtype=np.int32)
+        self.T = np.ascontiguousarray(types, dtype=np.int32)
+        self.dim = self.V.shape[1]
+        self.h = lib().orc_scene_create(self.dim, _p(self.V), len(self.V), _p(self.I), _p(self.T), len(self.I))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def info(self, sub: int) -> dict:
+        out = np.zeros(8, np.int64)
+        lib().orc_sub_info(self.h, sub, _p(out))
+        keys = ["present", "nverts", "nprims", "nnodes", "nrefs", "max_depth", "py_sils", "nsil_topology"]
+        return {k: int(v) for k, v in zip(keys, out)}
+
+    def export(self, sub: int) -> dict:
+        inf = self.info(sub)
+        D = self.dim
+        nn, npr, nr = inf["nnodes"], inf["nprims"], inf["nrefs"]
+        box = np.zeros((nn, 2 * D), np.float32)
+        ints = np.zeros((nn, 4), np.int32)
+        cone = np.zeros((nn, D + 2), np.float32)
+        pidx = np.zeros(npr, np.int32)
+        refs = np.zeros(max(nr, 1), np.int32)
+        lib().orc_sub_export(self.h, sub, _p(box), _p(ints), _p(cone), _p(pidx), _p(refs))
+        return dict(box=box, ints=ints, cone=cone, prim_index=pidx, refs=refs[:nr], **inf)
+
+    def query(self, sub: int, kind: int, pts, dirs=None, rmax=None):
+        pts = _f32(pts)
+        n = len(pts)
+        D = self.dim
+        dirs = _f32(np.zeros_like(pts) if dirs is None else dirs)
+        rmax = _f32(np.full(n, np.inf) if rmax is None else rmax)
+        prim = np.zeros(n, np.int32)
+        d = np.zeros(n, np.float32)
+        p = np.zeros((n, D), np.float32)
+        nrm = np.zeros((n, D), np.float32)
+        tuv = np.zeros((n, 2), np.float32)
+        lib().orc_query(self.h, sub, kind, _p(pts), _p(dirs), _p(rmax), n, _p(prim), _p(d), _p(p), _p(nrm), _p(tuv))
+        return dict(prim=prim, d=d, p=p, n=nrm, tuv=tuv)
+
+    def ref_topology(self, sub: int, ref: int) -> int:
+        return lib().orc_ref_topology(self.h, sub, ref)
+
+    @staticmethod
+    def _params(nwalks, nsteps, nsubwalks, cv, av, seed, eps, min_R):
+        ip = np.array([nwalks, nsteps, nsubwalks, int(cv), int(av), np.uint32(seed).astype(np.int32)], np.int32)
+        fp = np.array([eps, min_R], np.float32)
+        return ip, fp
+
+    @staticmethod
+    def _bc(bc_code, coeffs, gv, hv):
+        c = _f32(np.asarray(coeffs if coeffs is not None else (0, 0, 0, 0)))
+        gv = None if gv is None else _f32(gv)
+        hv = None if hv is None else _f32(hv)
+        return c, gv, hv
+
+    def solve_u(self, pts, bc_code, nwalks, nsteps=128, eps=1e-3, min_R=1e-3, cv=True, av=True, seed=0,
+                nsubwalks=1, coeffs=None, gv=None, hv=None, src_code=0, src_c=0.0, per_walk=False,
+                count_bytes=False):
+        pts = _f32(pts)
+        n = len(pts)
+        ip, fp = self._params(nwalks, nsteps, nsubwalks, cv, av, seed, eps, min_R)
+        c, gv, hv = self._bc(bc_code, coeffs, gv, hv)
+        mean = np.zeros(n, np.float32)
+        wv = np.zeros(n * nwalks, np.float32) if per_walk else None
+        ws = np.zeros(n * nwalks, np.int32) if per_walk else None
+        wt = np.zeros(n * nwalks, np.int32) if per_walk else None
+        bt = np.zeros(1, np.uint64) if count_bytes else None
+        lib().orc_solve_u(self.h, _p(pts), n, _p(ip), _p(fp), bc_code, _p(c), _p(gv), _p(hv), src_code, src_c,
+                          _p(mean), _p(wv), _p(ws), _p(wt), _p(bt))
+        out = dict(u=mean)
+        if per_walk:
+            out.update(walk_u=wv, walk_steps=ws, walk_term=wt)
+        if count_bytes:
+            out["bytes"] = int(bt[0])
+        return out
+
+    def solve_dude(self, pts, e, bc_code, nwalks, nsteps=128, eps=1e-3, min_R=1e-3, cv=True, av=True, seed=0,
+                   coeffs=None, gv=None, hv=None, src_code=0, src_c=0.0):
+        pts = _f32(pts)
+        e = _f32(e)
+        n = len(pts)
+        ip, fp = self._params(nwalks, nsteps, 1, cv, av, seed, eps, min_R)
+        c, gv, hv = self._bc(bc_code, coeffs, gv, hv)
+        out = np.zeros(n, np.float32)
+        st = np.zeros(1, np.uint64)
+        lib().orc_solve_dude(self.h, _p(pts), _p(e), n, _p(ip), _p(fp), bc_code, _p(c), _p(gv), _p(hv), src_code,
+                             src_c, _p(out), _p(st))
+        return dict(dude=out, steps=int(st[0]))
+
+    def solve_fused(self, pts, bc_code, nwalks, nsteps=128, eps=1e-3, min_R=1e-3, seed=0, coeffs=None, gv=None,
+                    hv=None, src_code=0, src_c=0.0, per_pair=False, cv=True):
+        pts = _f32(pts)
+        n = len(pts)
+        D = self.dim
+        ip, fp = self._params(nwalks, nsteps, 1, cv, True, seed, eps, min_R)
+        c, gv, hv = self._bc(bc_code, coeffs, gv, hv)
+        npairs = (nwalks + 1) // 2
+        u = np.zeros(n, np.float32)
+        g = np.zeros((n, D), np.float32)
+        pu = np.zeros(n * npairs, np.float32) if per_pair else None
+        pg = np.zeros((n * npairs, D), np.float32) if per_pair else None
+        ps = np.zeros(n * npairs, np.int32) if per_pair else None
+        st = np.zeros(1, np.uint64)
+        lib().orc_solve_fused(self.h, _p(pts), n, _p(ip), _p(fp), bc_code, _p(c), _p(gv), _p(hv), src_code, src_c,
+                              _p(u), _p(g), _p(pu), _p(pg), _p(ps), _p(st))
+        out = dict(u=u, grad=g, steps=int(st[0]))
+        if per_pair:
+            out.update(pair_u=pu, pair_g=pg, pair_steps=ps)
+        return out
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()

@@ -1,0 +1,2426 @@
+// wost_oracle.cpp -- CPU restatement of the diff_wost Walk-on-Stars hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP product in
+// dwost_amd/: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg may load it.  The product never links, loads or calls anything here.
+//
+// Every function restates a reference function literally (citations are to
+// files of zihay/dwost).  Floating-point conventions, shared with the product
+// so that the two agree bit for bit:
+//   * fp32 everywhere, compiled with -ffp-contract=off (no implicit FMA).
+//   * Dr.Jit's fused forms are written out as explicit fmaf():
+//       dot(a,b)   = fma(a_n,b_n, ... fma(a_1,b_1, a_0*b_0))
+//       lerp(a,b,t)= fma(b,t, fma(-a,t,a))
+//       cross(u,v) = (fma(u.y,v.z,-(u.z*v.y)), ...)   [3D only]
+//     Python-level expressions (u[0]*v[1]-u[1]*v[0], a+b*t, ...) stay plain.
+//   * sqrt and division are IEEE correctly rounded.
+//   * log/exp/sin/cos/acos/asin/atan2/cbrt are evaluated in fp64 and rounded
+//     once to fp32 (correctly rounded except in ~2^-28 of inputs).
+//   * Dr.Jit's own bit-level choices (its transcendental polynomials, its
+//     fmadd usage inside dot, PCG32 lane offsets) are not observable in this
+//     container; the choices above are documented in DESIGN.md.
+//
+// The C++ builder restates include/bvh.h + include/snch.h (Eigen reductions
+// written out in Eigen's unrolled order: x0 op (x1 op x2) for 3-vectors).
+
+#include <algorithm>
+#include <array>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+static const float kInf = std::numeric_limits<float>::infinity();
+static const float kPi = (float)3.14159265358979323846;          // f32(pi)
+static const float kPi2 = (float)(3.14159265358979323846 / 2.0);  // f32(pi/2)
+static const float kTwoPi = (float)(2.0 * 3.14159265358979323846);
+static const float kFourPi = (float)(4.0 * 3.14159265358979323846);
+static const float kDrEps = 5.9604644775390625e-08f;  // dr.epsilon(Float32) = 2^-24
+
+// ----------------------------------------------------------------------------
+// fp32 helpers
+// ----------------------------------------------------------------------------
+struct V2 { float x, y; };
+struct V3 { float x, y, z; };
+
+static inline V2 v2(float x, float y) { return V2{x, y}; }
+static inline V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+static inline V2 add(V2 a, V2 b) { return v2(a.x + b.x, a.y + b.y); }
+static inline V3 add(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline V2 sub(V2 a, V2 b) { return v2(a.x - b.x, a.y - b.y); }
+static inline V3 sub(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline V2 mul(V2 a, float s) { return v2(a.x * s, a.y * s); }
+static inline V3 mul(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+static inline V2 divs(V2 a, float s) { return v2(a.x / s, a.y / s); }
+static inline V3 divs(V3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
+static inline V2 neg(V2 a) { return v2(-a.x, -a.y); }
+static inline V3 neg(V3 a) { return v3(-a.x, -a.y, -a.z); }
+static inline float dot(V2 a, V2 b) { return std::fma(a.y, b.y, a.x * b.x); }
+static inline float dot(V3 a, V3 b) { return std::fma(a.z, b.z, std::fma(a.y, b.y, a.x * b.x)); }
+static inline float norm(V2 a) { return std::sqrt(dot(a, a)); }
+static inline float norm(V3 a) { return std::sqrt(dot(a, a)); }
+static inline V2 normalize(V2 a) { return mul(a, 1.0f / std::sqrt(dot(a, a))); }
+static inline V3 normalize(V3 a) { return mul(a, 1.0f / std::sqrt(dot(a, a))); }
+static inline V3 cross(V3 u, V3 v) {
+    return v3(std::fma(u.y, v.z, -(u.z * v.y)), std::fma(u.z, v.x, -(u.x * v.z)),
+              std::fma(u.x, v.y, -(u.y * v.x)));
+}
+static inline float lerpf(float a, float b, float t) { return std::fma(b, t, std::fma(-a, t, a)); }
+static inline V2 lerp(V2 a, V2 b, float t) { return v2(lerpf(a.x, b.x, t), lerpf(a.y, b.y, t)); }
+static inline V3 lerp(V3 a, V3 b, float t) {
+    return v3(lerpf(a.x, b.x, t), lerpf(a.y, b.y, t), lerpf(a.z, b.z, t));
+}
+static inline float fminf_(float a, float b) { return std::fmin(a, b); }
+static inline float fmaxf_(float a, float b) { return std::fmax(a, b); }
+static inline float clampf(float x, float lo, float hi) { return fmaxf_(fminf_(x, hi), lo); }
+static inline V2 vmin(V2 a, V2 b) { return v2(fminf_(a.x, b.x), fminf_(a.y, b.y)); }
+static inline V2 vmax(V2 a, V2 b) { return v2(fmaxf_(a.x, b.x), fmaxf_(a.y, b.y)); }
+static inline V3 vmin(V3 a, V3 b) { return v3(fminf_(a.x, b.x), fminf_(a.y, b.y), fminf_(a.z, b.z)); }
+static inline V3 vmax(V3 a, V3 b) { return v3(fmaxf_(a.x, b.x), fmaxf_(a.y, b.y), fmaxf_(a.z, b.z)); }
+static inline V2 vabs(V2 a) { return v2(std::fabs(a.x), std::fabs(a.y)); }
+static inline V3 vabs(V3 a) { return v3(std::fabs(a.x), std::fabs(a.y), std::fabs(a.z)); }
+// 2D cross as written in diff_wost/core/math.py:85-95 (plain ops)
+static inline float cross2(V2 u, V2 v) { return u.x * v.y - u.y * v.x; }
+
+// correctly rounded (via fp64) transcendentals
+static inline float cr_log(float x) { return (float)std::log((double)x); }
+static inline float cr_exp(float x) { return (float)std::exp((double)x); }
+static inline float cr_sin(float x) { return (float)std::sin((double)x); }
+static inline float cr_cos(float x) { return (float)std::cos((double)x); }
+static inline float cr_acos(float x) { return (float)std::acos((double)x); }
+static inline float cr_asin(float x) { return (float)std::asin((double)x); }
+static inline float cr_atan2(float y, float x) { return (float)std::atan2((double)y, (double)x); }
+static inline float cr_cbrt(float x) { return (float)std::cbrt((double)x); }
+
+static inline bool bad(float v) { return std::isnan(v) || std::isinf(v); }
+
+// ----------------------------------------------------------------------------
+// TEA (diff_wost/core/math.py:137-157) and PCG32 (Dr.Jit PCG32; O'Neill's
+// pcg32_srandom_r / pcg32_random_r; next_float32 = bits(u>>9 | 1.0f) - 1)
+// ----------------------------------------------------------------------------
+static inline void tea(uint32_t v0, uint32_t v1, int rounds, const uint32_t key[4], uint32_t* o0,
+                       uint32_t* o1) {
+    uint32_t s = 0;
+    for (int r = 0; r < rounds; ++r) {
+        s += 0x9E3779B9u;
+        v0 += ((v1 << 4) + key[0]) ^ (v1 + s) ^ ((v1 >> 5) + key[1]);
+        v1 += ((v0 << 4) + key[2]) ^ (v0 + s) ^ ((v0 >> 5) + key[3]);
+    }
+    *o0 = v0;
+    *o1 = v1;
+}
+static const uint32_t kTeaKey[4] = {0xA341316Cu, 0xC8013EA4u, 0xAD90777Du, 0x7E95761Eu};
+
+struct Pcg {
+    uint64_t state, inc;
+    uint32_t next_u32() {
+        uint64_t old = state;
+        state = old * 0x5851f42d4c957f2dULL + inc;
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((~rot + 1u) & 31u));
+    }
+    float next_float32() {
+        uint32_t bits = (next_u32() >> 9) | 0x3f800000u;
+        float f;
+        std::memcpy(&f, &bits, 4);
+        return f - 1.0f;
+    }
+    void seed(uint64_t initstate, uint64_t initseq) {
+        state = 0;
+        inc = (initseq << 1) | 1u;
+        next_u32();
+        state += initstate;
+        next_u32();
+    }
+};
+// PCG32(size, initstate=v0, initseq=v1) adds arange(size) to both inputs
+// (Dr.Jit constructor semantics; solver.py:61-63).
+static inline Pcg pcg_for_lane(uint32_t v0, uint32_t v1, uint64_t lane) {
+    Pcg r;
+    r.seed((uint64_t)v0 + lane, (uint64_t)v1 + lane);
+    return r;
+}
+static inline Pcg pcg_for(uint32_t seed, uint32_t point, uint64_t lane) {
+    uint32_t a, b;
+    tea(seed, point, 4, kTeaKey, &a, &b);
+    return pcg_for_lane(a, b, lane);
+}
+
+// ----------------------------------------------------------------------------
+// Builder: include/bounding_volume.h, include/primitive.h, include/bvh.h,
+// include/snch.h, include/silhouette_{vertex,edge}.h
+// ----------------------------------------------------------------------------
+static const float kBoxEps = (float)1e-9;  // fwd.h:14 `Epsilon` (Float=float)
+
+static inline float emin(float a, float b) { return (b < a) ? b : a; }  // Eigen cwiseMin
+static inline float emax(float a, float b) { return (a < b) ? b : a; }  // Eigen cwiseMax
+
+template <int D>
+struct Box {
+    float mn[D], mx[D];
+    Box() {
+        for (int d = 0; d < D; ++d) {
+            mn[d] = FLT_MAX;
+            mx[d] = -FLT_MAX;
+        }
+    }
+    static Box point(const float* p) {  // bounding_volume.h:112-116
+        Box b;
+        for (int d = 0; d < D; ++d) {
+            b.mn[d] = p[d] - kBoxEps;
+            b.mx[d] = p[d] + kBoxEps;
+        }
+        return b;
+    }
+    void expand(const float* p) {  // :119-123
+        for (int d = 0; d < D; ++d) {
+            mn[d] = emin(mn[d], p[d] - kBoxEps);
+            mx[d] = emax(mx[d], p[d] + kBoxEps);
+        }
+    }
+    void expand(const Box& b) {  // :126-129
+        for (int d = 0; d < D; ++d) {
+            mn[d] = emin(mn[d], b.mn[d]);
+            mx[d] = emax(mx[d], b.mx[d]);
+        }
+    }
+    float extent(int d) const { return mx[d] - mn[d]; }
+    float surface_area() const {  // :191-194
+        float e[D];
+        for (int d = 0; d < D; ++d) e[d] = emax(extent(d), 1e-5f);
+        if (D == 2) {
+            float prod = e[0] * e[1];
+            return 2.0f * (prod / e[0] + prod / e[1]);
+        } else {
+            float prod = e[0] * (e[1] * e[2]);
+            return 2.0f * (prod / e[0] + (prod / e[1] + prod / e[2]));
+        }
+    }
+    int max_dimension() const {  // :178-183 (first index of the max)
+        int idx = 0;
+        float best = mx[0] - mn[0];
+        for (int d = 1; d < D; ++d) {
+            float v = mx[d] - mn[d];
+            if (v > best) {
+                best = v;
+                idx = d;
+            }
+        }
+        return idx;
+    }
+    void centroid(float* c) const {
+        for (int d = 0; d < D; ++d) c[d] = (mn[d] + mx[d]) * 0.5f;
+    }
+};
+
+// Eigen-order reductions (size 2: x0+x1; size 3: x0+(x1+x2))
+template <int D>
+static inline float e_sqnorm(const float* v) {
+    if (D == 2) return v[0] * v[0] + v[1] * v[1];
+    return v[0] * v[0] + (v[1] * v[1] + v[2] * v[2]);
+}
+template <int D>
+static inline float e_dot(const float* a, const float* b) {
+    if (D == 2) return a[0] * b[0] + a[1] * b[1];
+    return a[0] * b[0] + (a[1] * b[1] + a[2] * b[2]);
+}
+template <int D>
+static inline void e_normalized(float* v) {  // Eigen normalized(): v/sqrt(|v|^2) if |v|^2>0
+    float z = e_sqnorm<D>(v);
+    if (z > 0.0f) {
+        float s = std::sqrt(z);
+        for (int d = 0; d < D; ++d) v[d] = v[d] / s;
+    }
+}
+static inline void e_cross(const float* u, const float* v, float* o) {  // Eigen cross (plain ops)
+    o[0] = u[1] * v[2] - u[2] * v[1];
+    o[1] = u[2] * v[0] - u[0] * v[2];
+    o[2] = u[0] * v[1] - u[1] * v[0];
+}
+
+template <int D>
+struct Prim {  // LineSegment (D=2) or Triangle (D=3), primitive.h:12-114
+    float v[D][D];  // vertices a, b[, c]
+    int pIndex;
+    Box<D> bbox() const {
+        Box<D> b = Box<D>::point(v[0]);
+        for (int k = 1; k < D; ++k) b.expand(v[k]);
+        return b;
+    }
+    void centroid(float* c) const {
+        if (D == 2) {
+            for (int d = 0; d < 2; ++d) c[d] = (v[0][d] + v[1][d]) * 0.5f;
+        } else {
+            for (int d = 0; d < 3; ++d) c[d] = ((v[0][d] + v[1][d]) + v[2][d]) / 3.0f;
+        }
+    }
+};
+
+template <int D>
+struct Node {
+    Box<D> box;
+    int off;   // referenceOffset (leaf) / secondChildOffset (internal)
+    int nref;  // > 0 for leaves
+    // SNCH extension
+    float axis[D];
+    float half_angle;
+    float radius;
+    int sil_off, nsil;
+};
+
+template <int D>
+struct Tree {
+    std::vector<Node<D>> nodes;
+    std::vector<Prim<D>> prims;  // sorted
+    int max_depth = 0;
+};
+
+template <int D>
+struct Builder {  // bvh.h:359-652 (SurfaceArea, no leaf packing, leaf 4, 8 buckets)
+    std::vector<Prim<D>>& prims;
+    std::vector<Box<D>> boxes;
+    std::vector<std::array<float, D>> cents;
+    std::vector<Node<D>> nodes;
+    int max_depth = 0;
+    static const int kLeaf = 4, kBuckets = 8;
+    static const int UNTOUCHED = (int)0xffffffff, TOUCHED_TWICE = (int)0xfffffffd;
+    explicit Builder(std::vector<Prim<D>>& p) : prims(p) {}
+
+    void build() {
+        if (prims.empty()) return;
+        size_t n = prims.size();
+        boxes.resize(n);
+        cents.resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            boxes[i] = prims[i].bbox();
+            prims[i].centroid(cents[i].data());
+        }
+        nodes.reserve(2 * n);
+        recurse(-1, 0, (int)n, 0);
+    }
+
+    void recurse(int parent, int start, int end, int depth) {
+        max_depth = std::max(max_depth, depth);
+        Node<D> node;
+        for (int d = 0; d < D; ++d) node.axis[d] = 0.0f;
+        node.half_angle = 0.0f;
+        node.radius = 0.0f;
+        node.sil_off = node.nsil = 0;
+        int current = (int)nodes.size();
+        int nrefs = end - start;
+        Box<D> bb, bc;
+        for (int p = start; p < end; ++p) {
+            bb.expand(boxes[p]);
+            bc.expand(cents[p].data());
+        }
+        node.box = bb;
+        if (nrefs <= kLeaf || depth == 64 - 2) {
+            node.off = start;
+            node.nref = nrefs;
+        } else {
+            node.off = UNTOUCHED;
+            node.nref = 0;
+        }
+        nodes.push_back(node);
+        if (parent >= 0) {
+            nodes[parent].off--;
+            if (nodes[parent].off == TOUCHED_TWICE) nodes[parent].off = (int)nodes.size() - 1 - parent;
+        }
+        if (node.nref > 0) return;
+        int split_dim;
+        float split_coord;
+        object_split(bb, bc, start, end, split_dim, split_coord);
+        int mid = start;
+        for (int i = start; i < end; ++i) {  // performObjectSplit bvh.h:622-652
+            if (cents[i][split_dim] < split_coord) {
+                std::swap(prims[i], prims[mid]);
+                std::swap(boxes[i], boxes[mid]);
+                std::swap(cents[i], cents[mid]);
+                mid++;
+            }
+        }
+        if (mid == start || mid == end) mid = start + (end - start) / 2;
+        recurse(current, start, mid, depth + 1);
+        recurse(current, mid, end, depth + 1);
+    }
+
+    void object_split(const Box<D>& nb, const Box<D>& cb, int start, int end, int& split_dim,
+                      float& split_coord) {  // computeObjectSplit bvh.h:537-620
+        float split_cost = FLT_MAX;
+        split_dim = -1;
+        split_coord = 0.0f;
+        for (int dim = 0; dim < D; ++dim) {
+            float extent = nb.mx[dim] - nb.mn[dim];
+            if (extent < 1e-6f) continue;
+            float width = extent / (float)kBuckets;
+            Box<D> bk[kBuckets];
+            int cnt[kBuckets] = {0};
+            for (int p = start; p < end; ++p) {
+                int bi = (int)((cents[p][dim] - nb.mn[dim]) / width);
+                bi = std::clamp(bi, 0, kBuckets - 1);
+                bk[bi].expand(boxes[p]);
+                cnt[bi] += 1;
+            }
+            Box<D> rbox[kBuckets];
+            int rcnt[kBuckets] = {0};
+            Box<D> acc;
+            for (int b = kBuckets - 1; b > 0; --b) {
+                acc.expand(bk[b]);
+                rbox[b] = acc;
+                rcnt[b] = cnt[b];
+                if (b != kBuckets - 1) rcnt[b] += rcnt[b + 1];
+            }
+            Box<D> lbox;
+            int nleft = 0;
+            for (int b = 1; b < kBuckets; ++b) {
+                lbox.expand(bk[b - 1]);
+                nleft += cnt[b - 1];
+                if (nleft > 0 && rcnt[b] > 0) {
+                    float cost = (float)nleft * lbox.surface_area() + (float)rcnt[b] * rbox[b].surface_area();
+                    if (cost < split_cost) {
+                        split_cost = cost;
+                        split_dim = dim;
+                        split_coord = nb.mn[dim] + (float)b * width;
+                    }
+                }
+            }
+        }
+        if (split_dim == -1) {
+            split_dim = cb.max_dimension();
+            split_coord = (cb.mn[split_dim] + cb.mx[split_dim]) * 0.5f;
+        }
+    }
+};
+
+// SNCH silhouettes (silhouette_vertex.h / silhouette_edge.h)
+struct Sil2 {
+    float a[2], b[2], c[2];
+    int idx[3];  // prev, center, next
+    int pIndex;
+    bool has_face(int f) const { return f == 0 ? idx[2] != -1 : idx[0] != -1; }
+    void face_normal(int f, bool nrm, float* n) const {  // silhouette_vertex.h:58-66
+        const float* pa = f == 0 ? b : a;
+        const float* pb = f == 0 ? c : b;
+        float s0 = pb[0] - pa[0], s1 = pb[1] - pa[1];
+        n[0] = s1;
+        n[1] = -s0;
+        if (nrm) e_normalized<2>(n);
+    }
+    void normal(float* n) const {
+        n[0] = n[1] = 0.0f;
+        float t[2];
+        if (has_face(0)) {
+            face_normal(0, false, t);
+            n[0] += t[0];
+            n[1] += t[1];
+        }
+        if (has_face(1)) {
+            face_normal(1, false, t);
+            n[0] += t[0];
+            n[1] += t[1];
+        }
+        e_normalized<2>(n);
+    }
+    void centroid(float* o) const { o[0] = b[0], o[1] = b[1]; }
+};
+struct Sil3 {
+    float a[3], b[3], c[3], d[3];
+    int idx[4];  // c, a, b, d
+    int pIndex;
+    bool has_face(int f) const { return f == 0 ? idx[0] != -1 : idx[3] != -1; }
+    void face_normal(int f, bool nrm, float* n) const {  // silhouette_edge.h:70-78
+        float u[3], v[3];
+        if (f == 0) {
+            for (int k = 0; k < 3; ++k) u[k] = b[k] - a[k], v[k] = c[k] - a[k];
+        } else {
+            for (int k = 0; k < 3; ++k) u[k] = d[k] - a[k], v[k] = b[k] - a[k];
+        }
+        e_cross(u, v, n);
+        if (nrm) e_normalized<3>(n);
+    }
+    void normal(float* n) const {
+        n[0] = n[1] = n[2] = 0.0f;
+        float t[3];
+        if (has_face(0)) {
+            face_normal(0, false, t);
+            for (int k = 0; k < 3; ++k) n[k] += t[k];
+        }
+        if (has_face(1)) {
+            face_normal(1, false, t);
+            for (int k = 0; k < 3; ++k) n[k] += t[k];
+        }
+        e_normalized<3>(n);
+    }
+    void centroid(float* o) const {
+        for (int k = 0; k < 3; ++k) o[k] = (a[k] + b[k]) * 0.5f;
+    }
+};
+
+template <int D, typename S>
+struct SnchData {
+    std::vector<S> sils;       // topology silhouettes (one per vertex / edge)
+    std::vector<int> refs;     // silhouette refs in node order (indices into sils)
+    std::vector<S> ref_sils;   // copies in ref order (what traversal reads)
+};
+
+// buildTopology 2D (snch.h:407-443)
+static void topology2(const std::vector<float>& V, const std::vector<int>& I, std::vector<Sil2>& sils) {
+    size_t nv = V.size() / 2, ne = I.size() / 2;
+    std::vector<std::vector<int>> adj(nv);
+    for (size_t i = 0; i < ne; ++i) {
+        int u = I[2 * i], w = I[2 * i + 1];
+        adj[u].push_back(w);
+        adj[w].push_back(u);
+    }
+    sils.resize(nv);
+    for (size_t i = 0; i < nv; ++i) {
+        int prev = adj[i].size() >= 1 ? adj[i][0] : -1;
+        int next = adj[i].size() >= 2 ? adj[i][1] : -1;
+        Sil2& s = sils[i];
+        s.a[0] = prev != -1 ? V[2 * prev] : 0.0f;
+        s.a[1] = prev != -1 ? V[2 * prev + 1] : 0.0f;
+        s.b[0] = V[2 * i];
+        s.b[1] = V[2 * i + 1];
+        s.c[0] = next != -1 ? V[2 * next] : 0.0f;
+        s.c[1] = next != -1 ? V[2 * next + 1] : 0.0f;
+        s.idx[0] = prev;
+        s.idx[1] = (int)i;
+        s.idx[2] = next;
+        s.pIndex = (int)i;
+    }
+}
+
+// buildTopology 3D (snch.h:446-489); conn[f][k] = edge id
+static void topology3(const std::vector<float>& V, const std::vector<int>& I, std::vector<Sil3>& sils,
+                      std::vector<int>& conn) {
+    size_t nf = I.size() / 3;
+    std::map<std::pair<int, int>, int> emap;
+    conn.assign(nf * 3, -1);
+    for (size_t i = 0; i < nf; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            int u = I[3 * i + k], v = I[3 * i + (k + 1) % 3], w = I[3 * i + (k + 2) % 3];
+            std::pair<int, int> key = u < v ? std::make_pair(u, v) : std::make_pair(v, u);
+            auto it = emap.find(key);
+            if (it == emap.end()) {
+                int e = (int)sils.size();
+                emap[key] = e;
+                Sil3 s;
+                std::memset(&s, 0, sizeof(s));
+                for (int q = 0; q < 3; ++q) {
+                    s.a[q] = V[3 * u + q];
+                    s.b[q] = V[3 * v + q];
+                    s.c[q] = V[3 * w + q];
+                    s.d[q] = 0.0f;
+                }
+                s.idx[0] = w;
+                s.idx[1] = u;
+                s.idx[2] = v;
+                s.idx[3] = -1;
+                s.pIndex = e;
+                sils.push_back(s);
+                conn[3 * i + k] = e;
+            } else {
+                int e = it->second;
+                Sil3& s = sils[e];
+                for (int q = 0; q < 3; ++q) s.d[q] = V[3 * w + q];
+                s.idx[3] = w;
+                conn[3 * i + k] = e;
+            }
+        }
+    }
+}
+
+// assignGeometricDataToNodes (snch.h:174-306) + computeBoundingConesRecursive (:308-385)
+template <int D, typename S>
+static void snch_assign(Tree<D>& T, const std::vector<S>& sils, const std::vector<int>& conn,
+                        SnchData<D, S>& out) {
+    std::vector<float> refN, refFN;  // D per ref, 2*D per ref
+    for (size_t i = 0; i < T.nodes.size(); ++i) {
+        Node<D>& node = T.nodes[i];
+        std::unordered_map<int, bool> seen;
+        int start = (int)out.refs.size();
+        for (int j = 0; j < node.nref; ++j) {
+            const Prim<D>& prim = T.prims[node.off + j];
+            int pi = prim.pIndex;
+            for (int k = 0; k < D; ++k) {
+                int si = conn[D * pi + k];
+                if (seen.find(si) != seen.end()) continue;
+                seen[si] = true;
+                const S& s = sils[si];
+                float n0[D], n1[D], n[D];
+                for (int q = 0; q < D; ++q) n0[q] = n1[q] = 0.0f;
+                if (s.has_face(0)) s.face_normal(0, true, n0);
+                if (s.has_face(1)) s.face_normal(1, true, n1);
+                s.normal(n);
+                out.refs.push_back(si);
+                for (int q = 0; q < D; ++q) refN.push_back(n[q]);
+                for (int q = 0; q < D; ++q) refFN.push_back(n0[q]);
+                for (int q = 0; q < D; ++q) refFN.push_back(n1[q]);
+            }
+        }
+        node.sil_off = start;
+        node.nsil = (int)out.refs.size() - start;
+    }
+    for (auto& nd : T.nodes) {  // default cone: axis 0, halfAngle pi, radius 0
+        for (int q = 0; q < D; ++q) nd.axis[q] = 0.0f;
+        nd.half_angle = kPi;
+        nd.radius = 0.0f;
+    }
+    struct Rec {
+        static void go(Tree<D>& T, const std::vector<S>& sils, const std::vector<int>& refs,
+                       const std::vector<float>& refN, const std::vector<float>& refFN, int start, int end) {
+            Node<D>& node = T.nodes[start];
+            float cen[D];
+            node.box.centroid(cen);
+            float axis[D];
+            for (int q = 0; q < D; ++q) axis[q] = 0.0f;
+            float radius = 0.0f;
+            bool any = false, two = true;
+            for (int i = start; i < end; ++i) {
+                const Node<D>& ch = T.nodes[i];
+                for (int j = 0; j < ch.nsil; ++j) {
+                    int r = ch.sil_off + j;
+                    const S& s = sils[refs[r]];
+                    for (int q = 0; q < D; ++q) axis[q] += refN[D * r + q];
+                    float sc[D], dv[D];
+                    s.centroid(sc);
+                    for (int q = 0; q < D; ++q) dv[q] = sc[q] - cen[q];
+                    float len = std::sqrt(e_sqnorm<D>(dv));
+                    radius = std::max(radius, len);
+                    two = two && s.has_face(0) && s.has_face(1);
+                    any = true;
+                }
+            }
+            if (!any) {
+                node.half_angle = -kPi;
+            } else if (!two) {
+                node.half_angle = kPi;
+            } else {
+                float an = std::sqrt(e_sqnorm<D>(axis));
+                if (an > kBoxEps) {
+                    for (int q = 0; q < D; ++q) axis[q] /= an;
+                    float ha = 0.0f;
+                    for (int i = start; i < end; ++i) {
+                        const Node<D>& ch = T.nodes[i];
+                        for (int j = 0; j < ch.nsil; ++j) {
+                            int r = ch.sil_off + j;
+                            for (int k = 0; k < 2; ++k) {
+                                const float* n = &refFN[(2 * r + k) * D];
+                                float ang = cr_acos(std::max(-1.0f, std::min(1.0f, e_dot<D>(axis, n))));
+                                ha = std::max(ha, ang);
+                            }
+                        }
+                    }
+                    for (int q = 0; q < D; ++q) node.axis[q] = axis[q];
+                    node.half_angle = ha;
+                    node.radius = radius;
+                }
+            }
+            if (node.nref == 0) {
+                go(T, sils, refs, refN, refFN, start + 1, start + node.off);
+                go(T, sils, refs, refN, refFN, start + node.off, end);
+            }
+        }
+    };
+    if (!T.nodes.empty()) Rec::go(T, sils, out.refs, refN, refFN, 0, (int)T.nodes.size());
+    out.ref_sils.resize(out.refs.size());
+    for (size_t r = 0; r < out.refs.size(); ++r) out.ref_sils[r] = sils[out.refs[r]];
+}
+
+// ----------------------------------------------------------------------------
+// Scenes (diff_wost/shapes/polyline.py, mesh.py)
+// ----------------------------------------------------------------------------
+enum { BC_EXPCOS = 0, BC_QUAD = 1, BC_LINEAR = 2, BC_PER_VERTEX = 3 };
+enum { SRC_ZERO = 0, SRC_CONST = 1 };
+
+template <int D>
+struct Sub {  // one Polyline / Mesh
+    bool present = false;
+    std::vector<float> V;        // local vertices
+    std::vector<int> I;          // local indices
+    std::vector<int> vglobal;    // local vertex -> global vertex
+    std::vector<int> pglobal;    // local prim -> global prim
+    float bb_min[D], bb_max[D];  // python bbox (no epsilon)
+    bool py_sils = false;        // configure_silhouette left silhouettes != None
+    std::vector<float> pmf, cdf; // edge lengths / doubled face areas
+    float pmf_sum = 0.0f, pmf_norm = 0.0f;
+    std::vector<float> vnormal;  // 2D vertex normals (for sdf)
+    Tree<D> tree;
+    std::vector<Sil2> sil2, ref2;
+    std::vector<Sil3> sil3, ref3;
+    std::vector<int> refs;
+};
+
+struct Scene {
+    int dim = 2;
+    Sub<2> s2[3];  // 0 full, 1 dirichlet, 2 neumann
+    Sub<3> s3[3];
+    std::vector<float> V;  // global vertices
+    std::vector<int> I;
+    std::vector<int> types;
+};
+
+template <int D>
+static void build_sub(Sub<D>& S) {
+    S.present = true;
+    size_t nv = S.V.size() / D, np = S.I.size() / D;
+    for (int d = 0; d < D; ++d) {
+        S.bb_min[d] = kInf;
+        S.bb_max[d] = -kInf;
+    }
+    for (size_t v = 0; v < nv; ++v)
+        for (int d = 0; d < D; ++d) {
+            S.bb_min[d] = std::min(S.bb_min[d], S.V[D * v + d]);
+            S.bb_max[d] = std::max(S.bb_max[d], S.V[D * v + d]);
+        }
+    // pmf: polyline.py:166-171 (edge length), mesh.py:279-283 (|cross|)
+    S.pmf.resize(np);
+    S.cdf.resize(np);
+    float acc = 0.0f;
+    for (size_t p = 0; p < np; ++p) {
+        float w;
+        if (D == 2) {
+            V2 a = v2(S.V[2 * S.I[2 * p]], S.V[2 * S.I[2 * p] + 1]);
+            V2 b = v2(S.V[2 * S.I[2 * p + 1]], S.V[2 * S.I[2 * p + 1] + 1]);
+            w = norm(sub(b, a));
+        } else {
+            const float* A = &S.V[3 * S.I[3 * p]];
+            const float* B = &S.V[3 * S.I[3 * p + 1]];
+            const float* C = &S.V[3 * S.I[3 * p + 2]];
+            V3 a = v3(A[0], A[1], A[2]), b = v3(B[0], B[1], B[2]), c = v3(C[0], C[1], C[2]);
+            w = norm(cross(sub(b, a), sub(c, a)));
+        }
+        S.pmf[p] = w;
+        acc += w;  // dr.cumsum, sequential fp32
+        S.cdf[p] = acc;
+    }
+    S.pmf_sum = np ? S.cdf[np - 1] : 0.0f;
+    S.pmf_norm = 1.0f / S.pmf_sum;
+    if (D == 2) {
+        // configure_normals (polyline.py:173-186)
+        S.vnormal.assign(nv * 2, 0.0f);
+        std::vector<V2> en(np);
+        for (size_t p = 0; p < np; ++p) {
+            V2 a = v2(S.V[2 * S.I[2 * p]], S.V[2 * S.I[2 * p] + 1]);
+            V2 b = v2(S.V[2 * S.I[2 * p + 1]], S.V[2 * S.I[2 * p + 1] + 1]);
+            V2 d = sub(b, a);
+            en[p] = normalize(v2(d.y, -d.x));
+        }
+        for (int pass = 0; pass < 2; ++pass)
+            for (size_t p = 0; p < np; ++p) {
+                int v = S.I[2 * p + pass];
+                S.vnormal[2 * v] += en[p].x;
+                S.vnormal[2 * v + 1] += en[p].y;
+            }
+        for (size_t v = 0; v < nv; ++v) {
+            V2 n = normalize(v2(S.vnormal[2 * v], S.vnormal[2 * v + 1]));
+            S.vnormal[2 * v] = n.x;
+            S.vnormal[2 * v + 1] = n.y;
+        }
+        // configure_silhouette (polyline.py:102-164): None unless a vertex has prev and next
+        std::vector<int> prv(nv, -1), nxt(nv, -1);
+        for (size_t p = 0; p < np; ++p) {
+            prv[S.I[2 * p + 1]] = S.I[2 * p];
+            nxt[S.I[2 * p]] = S.I[2 * p + 1];
+        }
+        S.py_sils = false;
+        for (size_t v = 0; v < nv; ++v)
+            if (prv[v] != -1 && nxt[v] != -1) S.py_sils = true;
+    } else {
+        // mesh.py:129-199: None unless an undirected edge is shared by exactly two faces
+        std::map<std::pair<int, int>, int> cnt;
+        for (size_t p = 0; p < np; ++p)
+            for (int k = 0; k < 3; ++k) {
+                int u = S.I[3 * p + k], v = S.I[3 * p + (k + 1) % 3];
+                cnt[std::make_pair(std::min(u, v), std::max(u, v))]++;
+            }
+        S.py_sils = false;
+        for (auto& kv : cnt)
+            if (kv.second == 2) S.py_sils = true;
+    }
+    // BVH/SNCH (bind_bvh.cpp / bind_snch.cpp -> bvh.h / snch.h)
+    S.tree.prims.resize(np);
+    for (size_t p = 0; p < np; ++p) {
+        Prim<D>& pr = S.tree.prims[p];
+        for (int k = 0; k < D; ++k)
+            for (int d = 0; d < D; ++d) pr.v[k][d] = S.V[D * S.I[D * p + k] + d];
+        pr.pIndex = (int)p;
+    }
+    Builder<D> B(S.tree.prims);
+    B.build();
+    S.tree.nodes = B.nodes;
+    S.tree.max_depth = B.max_depth;
+}
+
+template <int D>
+static void build_snch(Sub<D>& S);
+template <>
+void build_snch<2>(Sub<2>& S) {
+    topology2(S.V, S.I, S.sil2);
+    SnchData<2, Sil2> sd;
+    snch_assign<2, Sil2>(S.tree, S.sil2, S.I, sd);  // connectivity = indices
+    S.refs = sd.refs;
+    S.ref2 = sd.ref_sils;
+}
+template <>
+void build_snch<3>(Sub<3>& S) {
+    std::vector<int> conn;
+    topology3(S.V, S.I, S.sil3, conn);
+    SnchData<3, Sil3> sd;
+    snch_assign<3, Sil3>(S.tree, S.sil3, conn, sd);
+    S.refs = sd.refs;
+    S.ref3 = sd.ref_sils;
+}
+
+// _build_subscene_from_mask (polyline.py:672-692, mesh.py:617-635)
+template <int D>
+static void make_sub(const Scene& sc, int want_type, Sub<D>& S) {
+    size_t np = sc.types.size();
+    std::vector<int> sel;
+    for (size_t p = 0; p < np; ++p)
+        if (want_type < 0 || sc.types[p] == want_type) sel.push_back((int)p);
+    if (sel.empty()) {
+        S.present = false;
+        return;
+    }
+    std::vector<int> uniq;
+    for (int p : sel)
+        for (int k = 0; k < D; ++k) uniq.push_back(sc.I[D * p + k]);
+    std::sort(uniq.begin(), uniq.end());
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    std::unordered_map<int, int> remap;
+    for (size_t i = 0; i < uniq.size(); ++i) remap[uniq[i]] = (int)i;
+    S.V.resize(uniq.size() * D);
+    for (size_t i = 0; i < uniq.size(); ++i)
+        for (int d = 0; d < D; ++d) S.V[D * i + d] = sc.V[D * uniq[i] + d];
+    S.I.resize(sel.size() * D);
+    for (size_t j = 0; j < sel.size(); ++j)
+        for (int k = 0; k < D; ++k) S.I[D * j + k] = remap[sc.I[D * sel[j] + k]];
+    S.vglobal = uniq;
+    S.pglobal = sel;
+    build_sub<D>(S);
+    build_snch<D>(S);
+}
+
+// ----------------------------------------------------------------------------
+// Counters for algorithmic bytes (reference SoA record sizes)
+// ----------------------------------------------------------------------------
+struct Cnt {
+    uint64_t bytes = 0, nodes = 0, prims = 0, sils = 0;
+};
+enum { NODE2 = 24, SNODE2 = 48, SNODE3 = 60, SEG = 20, TRI = 40, SILV = 40, SILE = 68 };
+
+// ----------------------------------------------------------------------------
+// 2D queries (diff_wost/shapes/bvh.py, snch.py, line_segment.py, silhouette_vertex.py)
+// ----------------------------------------------------------------------------
+struct CP {  // ClosestPointRecord (2D uses t, 3D uses uv)
+    V3 p, n;
+    float t, u, v, d;
+    int prim;
+};
+struct Its {  // Intersection
+    bool valid, on_boundary;
+    V3 p, n;
+    float t, u, v, d;
+    int prim;
+};
+struct SilRec {
+    bool valid;
+    V3 p;
+    float d;
+    int prim;
+};
+
+static inline V2 node_min2(const Node<2>& n) { return v2(n.box.mn[0], n.box.mn[1]); }
+static inline V2 node_max2(const Node<2>& n) { return v2(n.box.mx[0], n.box.mx[1]); }
+static inline V3 node_min3(const Node<3>& n) { return v3(n.box.mn[0], n.box.mn[1], n.box.mn[2]); }
+static inline V3 node_max3(const Node<3>& n) { return v3(n.box.mx[0], n.box.mx[1], n.box.mx[2]); }
+
+// BoundingBox.distance (bvh.py:31-36)
+static inline void box_dist(V2 mn, V2 mx, V2 p, float& dmin, float& dmax) {
+    V2 u = sub(mn, p), v = sub(p, mx);
+    dmin = norm(vmax(vmax(u, v), v2(0.0f, 0.0f)));
+    dmax = norm(vmin(u, v));
+}
+static inline void box_dist(V3 mn, V3 mx, V3 p, float& dmin, float& dmax) {
+    V3 u = sub(mn, p), v = sub(p, mx);
+    dmin = norm(vmax(vmax(u, v), v3(0.0f, 0.0f, 0.0f)));
+    dmax = norm(vmin(u, v));
+}
+// BoundingBox.intersect (bvh.py:42-59)
+static inline bool box_ray(V2 mn, V2 mx, V2 x, V2 v, float rmax, float& tmin) {
+    V2 t0 = v2((mn.x - x.x) / v.x, (mn.y - x.y) / v.y);
+    V2 t1 = v2((mx.x - x.x) / v.x, (mx.y - x.y) / v.y);
+    V2 tn = vmin(t0, t1), tf = vmax(t0, t1);
+    float tnm = fmaxf_(fmaxf_(tn.x, tn.y), 0.0f);
+    float tfm = fminf_(fminf_(tf.x, tf.y), rmax);
+    if (tnm > tfm) {
+        tmin = kInf;
+        return false;
+    }
+    tmin = tnm;
+    return true;
+}
+static inline bool box_ray(V3 mn, V3 mx, V3 x, V3 v, float rmax, float& tmin) {
+    V3 t0 = v3((mn.x - x.x) / v.x, (mn.y - x.y) / v.y, (mn.z - x.z) / v.z);
+    V3 t1 = v3((mx.x - x.x) / v.x, (mx.y - x.y) / v.y, (mx.z - x.z) / v.z);
+    V3 tn = vmin(t0, t1), tf = vmax(t0, t1);
+    float tnm = fmaxf_(fmaxf_(fmaxf_(tn.x, tn.y), tn.z), 0.0f);
+    float tfm = fminf_(fminf_(fminf_(tf.x, tf.y), tf.z), rmax);
+    if (tnm > tfm) {
+        tmin = kInf;
+        return false;
+    }
+    tmin = tnm;
+    return true;
+}
+
+// LineSegment.closest_point (line_segment.py:74-85)
+static inline void seg_closest(const Prim<2>& s, V2 p, CP& r) {
+    V2 a = v2(s.v[0][0], s.v[0][1]), b = v2(s.v[1][0], s.v[1][1]);
+    V2 pa = sub(p, a), ba = sub(b, a);
+    V2 t = normalize(ba);
+    V2 n = v2(t.y, -t.x);
+    float h = clampf(dot(pa, ba) / dot(ba, ba), 0.0f, 1.0f);
+    V2 i = add(a, mul(ba, h));
+    float d = norm(sub(i, p));
+    r.p = v3(i.x, i.y, 0.0f);
+    r.n = v3(n.x, n.y, 0.0f);
+    r.t = h;
+    r.u = r.v = 0.0f;
+    r.d = d;
+    r.prim = s.pIndex;
+}
+
+// BVH.closest_point (bvh.py:111-188)
+static CP bvh2_closest(const Tree<2>& T, V2 p, Cnt* c) {
+    CP its;
+    its.p = v3(p.x, p.y, 0.0f);
+    its.n = v3(0.0f, 0.0f, 0.0f);
+    its.t = kInf;
+    its.u = its.v = 0.0f;
+    its.d = kInf;
+    its.prim = -1;
+    float r_max = kInf;
+    int st_i[64];
+    float st_d[64];
+    int sp = 0;
+    st_i[0] = 0;
+    st_d[0] = r_max;
+    while (sp >= 0) {
+        int ni = st_i[sp];
+        float cd = st_d[sp];
+        sp--;
+        if (cd > r_max) continue;
+        const Node<2>& node = T.nodes[ni];
+        if (c) c->bytes += NODE2, c->nodes++;
+        if (node.nref > 0) {
+            for (int j = 0; j < node.nref; ++j) {
+                CP r;
+                seg_closest(T.prims[node.off + j], p, r);
+                if (c) c->bytes += SEG, c->prims++;
+                if (r.d < its.d) its = r;
+            }
+            r_max = fminf_(r_max, its.d);
+        } else {
+            int L = ni + 1, R = ni + node.off;
+            float dminL, dmaxL, dminR, dmaxR;
+            box_dist(node_min2(T.nodes[L]), node_max2(T.nodes[L]), p, dminL, dmaxL);
+            bool hitL = dminL <= r_max;
+            r_max = fminf_(r_max, dmaxL);
+            box_dist(node_min2(T.nodes[R]), node_max2(T.nodes[R]), p, dminR, dmaxR);
+            bool hitR = dminR <= r_max;
+            r_max = fminf_(r_max, dmaxR);
+            if (c) c->bytes += 2 * NODE2, c->nodes += 2;
+            if (hitL && hitR) {
+                int closer = L, other = R;
+                // `d_max_right < d_max_right` at bvh.py:163 never swaps
+                if (dminR < dminL) {
+                    std::swap(closer, other);
+                    std::swap(dminL, dminR);
+                }
+                st_i[++sp] = other;
+                st_d[sp] = dminR;
+                st_i[++sp] = closer;
+                st_d[sp] = dminL;
+            } else if (hitL) {
+                st_i[++sp] = L;
+                st_d[sp] = dminL;
+            } else if (hitR) {
+                st_i[++sp] = R;
+                st_d[sp] = dminR;
+            }
+        }
+    }
+    return its;
+}
+
+// LineSegment.ray_intersect (line_segment.py:47-72)
+static inline bool seg_ray(const Prim<2>& s, V2 x, V2 d, float rmax, Its& r) {
+    V2 a = v2(s.v[0][0], s.v[0][1]), b = v2(s.v[1][0], s.v[1][1]);
+    V2 u = sub(a, x), v = sub(b, a);
+    float dv = d.x * v.y - d.y * v.x;
+    if (!(std::fabs(dv) > kDrEps)) return false;
+    float ud = u.x * d.y - u.y * d.x;
+    float s_ = ud / dv;
+    if (!((s_ >= 0.0f) && (s_ <= 1.0f))) return false;
+    float uv = u.x * v.y - u.y * v.x;
+    float t = uv / dv;
+    if (!((t >= 0.0f) && (t <= rmax))) return false;
+    V2 n = normalize(v2(v.y, -v.x));
+    V2 p = add(x, mul(d, t));
+    r.valid = true;
+    r.on_boundary = true;
+    r.p = v3(p.x, p.y, 0.0f);
+    r.n = v3(n.x, n.y, 0.0f);
+    r.t = s_;
+    r.u = r.v = 0.0f;
+    r.d = std::fabs(t);
+    r.prim = s.pIndex;
+    return true;
+}
+
+// BVH.intersect (bvh.py:190-257).  `r_max = its.d` (:216) reads the previous
+// record (d = 0 initially), so the first valid hit in traversal order wins.
+static Its bvh2_intersect(const Tree<2>& T, V2 x, V2 v, float r_max, Cnt* c) {
+    Its its;
+    std::memset(&its, 0, sizeof(its));
+    its.prim = 0;
+    float tmin;
+    if (c) c->bytes += NODE2, c->nodes++;
+    if (!box_ray(node_min2(T.nodes[0]), node_max2(T.nodes[0]), x, v, r_max, tmin)) return its;
+    int st_i[64];
+    float st_d[64];
+    int sp = 0;
+    st_i[0] = 0;
+    st_d[0] = tmin;
+    while (sp >= 0) {
+        int ni = st_i[sp];
+        float cd = st_d[sp];
+        sp--;
+        if (!(cd <= r_max)) continue;
+        const Node<2>& node = T.nodes[ni];
+        if (c) c->bytes += NODE2, c->nodes++;
+        if (node.nref > 0) {
+            for (int j = 0; j < node.nref; ++j) {
+                Its r;
+                if (c) c->bytes += SEG, c->prims++;
+                if (seg_ray(T.prims[node.off + j], x, v, r_max, r) && r.d < r_max) {
+                    r_max = its.d;
+                    its = r;
+                }
+            }
+        } else {
+            int L = ni + 1, R = ni + node.off;
+            float t0, t1;
+            bool h0 = box_ray(node_min2(T.nodes[L]), node_max2(T.nodes[L]), x, v, r_max, t0);
+            bool h1 = box_ray(node_min2(T.nodes[R]), node_max2(T.nodes[R]), x, v, r_max, t1);
+            if (c) c->bytes += 2 * 16, c->nodes += 2;
+            if (h0 && h1) {
+                int closer = L, other = R;
+                if (t1 < t0) {
+                    std::swap(closer, other);
+                    std::swap(t0, t1);
+                }
+                st_i[++sp] = other;
+                st_d[sp] = t1;
+                st_i[++sp] = closer;
+                st_d[sp] = t0;
+            } else if (h0) {
+                st_i[++sp] = L;
+                st_d[sp] = t0;
+            } else if (h1) {
+                st_i[++sp] = R;
+                st_d[sp] = t1;
+            }
+        }
+    }
+    return its;
+}
+
+// project_to_plane (bvh.py:16-20) and the 3D basis version (snch3d.py:30-45)
+static inline float project_to_plane(V2 n, V2 e) {
+    V2 b = v2(-n.y, n.x);
+    return std::fabs(dot(e, vabs(b)));
+}
+static inline float project_to_plane(V3 n, V3 e) {
+    float sign = std::copysign(1.0f, n.z);
+    float a = -1.0f / (sign + n.z);
+    float b = n.x * n.y * a;
+    V3 b1 = v3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    V3 b2 = v3(b, sign + n.y * n.y * a, -n.y);
+    float r1 = dot(e, vabs(b1));
+    float r2 = dot(e, vabs(b2));
+    return std::sqrt(r1 * r1 + r2 * r2);
+}
+static inline bool in_range(float x, float a, float b) { return (x >= a) && (x <= b); }
+
+// SNCHNode.overlaps (snch.py:46-97, snch3d.py:65-115)
+template <typename VT>
+static inline bool snch_overlaps(VT mn, VT mx, const float* axis_, float ha, float radius, VT x, float r_max,
+                                 float& d_min) {
+    float d_max;
+    box_dist(mn, mx, x, d_min, d_max);
+    bool ov = d_min <= r_max;
+    if ((ha >= 0.0f) && ov) {
+        ov = (ha > kPi2) && (d_min < kDrEps);
+        if (!ov) {
+            VT cen = divs(add(mn, mx), 2.0f);
+            VT vca = sub(cen, x);
+            float len = norm(vca);
+            vca = divs(vca, len);
+            VT axis;
+            std::memcpy(&axis, axis_, sizeof(VT));
+            float dax = cr_acos(clampf(dot(axis, vca), -1.0f, 1.0f));
+            ov = in_range(kPi2, dax - ha, dax + ha);
+            if (!ov) {
+                if (len > radius) {
+                    float vch = cr_asin(radius / len);
+                    float hs = ha + vch;
+                    ov = (hs > kPi2) || in_range(kPi2, dax - hs, dax + hs);
+                } else {
+                    VT e = sub(mx, cen);
+                    float d = dot(e, vabs(vca));
+                    float s = len - d;
+                    ov = s < 0.0f;
+                    if (!ov) {
+                        d = project_to_plane(vca, e);
+                        float vch = cr_atan2(d, s);
+                        float hs = ha + vch;
+                        ov = (hs > kPi2) || in_range(kPi2, dax - hs, dax + hs);
+                    }
+                }
+            }
+        }
+    }
+    return ov;
+}
+
+// is_silhouette (math.py:194-209) + SilhouetteVertex.closest_silhouette (silhouette_vertex.py:80-93)
+static inline bool silv_closest(const Sil2& s, int ref, V2 x, float r_max, SilRec& r) {
+    V2 a = v2(s.a[0], s.a[1]), b = v2(s.b[0], s.b[1]), c = v2(s.c[0], s.c[1]);
+    V2 view = sub(x, b);
+    float d = norm(view);
+    if ((s.idx[0] != -1) && (s.idx[2] != -1)) {
+        if (d < r_max) {
+            if (cross2(sub(b, a), sub(x, a)) * cross2(sub(c, b), sub(x, b)) < 0.0f) {
+                r.valid = true;
+                r.p = v3(b.x, b.y, 0.0f);
+                r.d = d;
+                r.prim = ref;
+                return true;
+            }
+        }
+    }
+    return false;
+}
+
+// closest_point_line_segment (math.py:174-191) in 3D + silhouette_edge.py:8-13,83-97
+static inline bool sile_closest(const Sil3& s, int ref, V3 x, float r_max, SilRec& r) {
+    V3 a = v3(s.a[0], s.a[1], s.a[2]), b = v3(s.b[0], s.b[1], s.b[2]);
+    V3 u = sub(b, a), v = sub(x, a);
+    float t = clampf(dot(v, u) / dot(u, u), 0.0f, 1.0f);
+    V3 pt = lerp(a, b, t);
+    float d = norm(sub(pt, x));
+    if (d < r_max) {
+        bool is = (s.idx[0] == -1) || (s.idx[3] == -1);
+        if (!is) {
+            V3 c = v3(s.c[0], s.c[1], s.c[2]), dd = v3(s.d[0], s.d[1], s.d[2]);
+            V3 n0 = normalize(cross(sub(b, a), sub(c, a)));
+            V3 n1 = normalize(cross(sub(dd, a), sub(b, a)));
+            V3 view = sub(pt, x);
+            is = dot(view, n0) * dot(view, n1) < 0.0f;
+        }
+        if (is) {
+            r.valid = true;
+            r.p = pt;
+            r.d = d;
+            r.prim = ref;
+            return true;
+        }
+    }
+    return false;
+}
+
+// SNCH.closest_silhouette (snch.py:312-378, snch3d.py:594-663)
+template <int D>
+static SilRec snch_closest_sil(const Sub<D>& S, const float* xq, float r_max, Cnt* c) {
+    typedef typename std::conditional<D == 2, V2, V3>::type VT;
+    SilRec rec;
+    std::memset(&rec, 0, sizeof(rec));
+    VT x;
+    std::memcpy(&x, xq, sizeof(VT));
+    const Tree<D>& T = S.tree;
+    const int NB = D == 2 ? SNODE2 : SNODE3;
+    float dmin, dmax;
+    VT mn, mx;
+    std::memcpy(&mn, T.nodes[0].box.mn, sizeof(VT));
+    std::memcpy(&mx, T.nodes[0].box.mx, sizeof(VT));
+    box_dist(mn, mx, x, dmin, dmax);
+    if (c) c->bytes += NB, c->nodes++;
+    if (!(dmin <= r_max)) return rec;
+    int st_i[64];
+    float st_d[64];
+    int sp = 0;
+    st_i[0] = 0;
+    st_d[0] = dmin;
+    while (sp >= 0) {
+        int ni = st_i[sp];
+        float cd = st_d[sp];
+        sp--;
+        if (!(cd <= r_max)) continue;
+        const Node<D>& node = T.nodes[ni];
+        if (c) c->bytes += NB, c->nodes++;
+        if (node.nref > 0) {
+            for (int j = 0; j < node.nsil; ++j) {
+                int ref = node.sil_off + j;
+                SilRec r;
+                bool ok;
+                if constexpr (D == 2) {
+                    ok = silv_closest(S.ref2[ref], ref, v2(xq[0], xq[1]), r_max, r);
+                    if (c) c->bytes += SILV, c->sils++;
+                } else {
+                    ok = sile_closest(S.ref3[ref], ref, v3(xq[0], xq[1], xq[2]), r_max, r);
+                    if (c) c->bytes += SILE, c->sils++;
+                }
+                if (ok && r.d < r_max) {
+                    r_max = fminf_(r_max, r.d);
+                    rec = r;
+                }
+            }
+        } else {
+            int L = ni + 1, R = ni + node.off;
+            const Node<D>& nl = T.nodes[L];
+            const Node<D>& nr = T.nodes[R];
+            VT mnl, mxl, mnr, mxr;
+            std::memcpy(&mnl, nl.box.mn, sizeof(VT));
+            std::memcpy(&mxl, nl.box.mx, sizeof(VT));
+            std::memcpy(&mnr, nr.box.mn, sizeof(VT));
+            std::memcpy(&mxr, nr.box.mx, sizeof(VT));
+            float d0, d1;
+            bool h0 = snch_overlaps(mnl, mxl, nl.axis, nl.half_angle, nl.radius, x, r_max, d0);
+            bool h1 = snch_overlaps(mnr, mxr, nr.axis, nr.half_angle, nr.radius, x, r_max, d1);
+            if (c) c->bytes += 2 * NB, c->nodes += 2;
+            if (h0 && h1) {
+                int closer = L, other = R;
+                if (d1 < d0) {
+                    std::swap(closer, other);
+                    std::swap(d0, d1);
+                }
+                st_i[++sp] = other;
+                st_d[sp] = d1;
+                st_i[++sp] = closer;
+                st_d[sp] = d0;
+            } else if (h0) {
+                st_i[++sp] = L;
+                st_d[sp] = d0;
+            } else if (h1) {
+                st_i[++sp] = R;
+                st_d[sp] = d1;
+            }
+        }
+    }
+    return rec;
+}
+
+// Polyline.star_radius (polyline.py:304-314) / Mesh.star_radius (mesh.py:337-346)
+template <int D>
+static float star_radius(const Sub<D>& S, const float* x, float r_max, Cnt* c) {
+    float m[D];
+    for (int d = 0; d < D; ++d) m[d] = fmaxf_(std::fabs(x[d] - S.bb_min[d]), std::fabs(x[d] - S.bb_max[d]));
+    float R;
+    if (D == 2)
+        R = norm(v2(m[0], m[1]));
+    else
+        R = norm(v3(m[0], m[1], m[2]));
+    R = fminf_(R, r_max);
+    if (!S.py_sils) return R;
+    SilRec s = snch_closest_sil<D>(S, x, R, c);
+    return s.valid ? s.d : R;
+}
+
+// ----------------------------------------------------------------------------
+// 3D queries (snch3d.py:334-476, triangle.py, math.py:257-322)
+// ----------------------------------------------------------------------------
+static inline void tri_closest(const Prim<3>& tr, V3 p, CP& r) {  // closest_point_triangle
+    V3 a = v3(tr.v[0][0], tr.v[0][1], tr.v[0][2]);
+    V3 b = v3(tr.v[1][0], tr.v[1][1], tr.v[1][2]);
+    V3 c = v3(tr.v[2][0], tr.v[2][1], tr.v[2][2]);
+    V3 ab = sub(b, a), ac = sub(c, a);
+    V3 ax = sub(p, a);
+    float d1 = dot(ab, ax), d2 = dot(ac, ax);
+    V3 pt;
+    float u0, u1;
+    V3 bx = sub(p, b);
+    float d3 = dot(ab, bx), d4 = dot(ac, bx);
+    V3 cx = sub(p, c);
+    float d5 = dot(ab, cx), d6 = dot(ac, cx);
+    float vc = d1 * d4 - d3 * d2;
+    float vb = d5 * d2 - d1 * d6;
+    float va = d3 * d6 - d5 * d4;
+    if ((d1 <= 0.0f) && (d2 <= 0.0f)) {
+        pt = a, u0 = 1.0f, u1 = 0.0f;
+    } else if ((d3 >= 0.0f) && (d4 <= d3)) {
+        pt = b, u0 = 0.0f, u1 = 1.0f;
+    } else if ((d6 >= 0.0f) && (d5 <= d6)) {
+        pt = c, u0 = 0.0f, u1 = 0.0f;
+    } else if ((vc <= 0.0f) && (d1 >= 0.0f) && (d3 <= 0.0f)) {
+        float v = d1 / (d1 - d3);
+        pt = add(a, mul(ab, v));
+        u0 = 1.0f - v, u1 = v;
+    } else if ((vb <= 0.0f) && (d2 >= 0.0f) && (d6 <= 0.0f)) {
+        float w = d2 / (d2 - d6);
+        pt = add(a, mul(ac, w));
+        u0 = 1.0f - w, u1 = 0.0f;
+    } else if ((va <= 0.0f) && ((d4 - d3) >= 0.0f) && ((d5 - d6) >= 0.0f)) {
+        float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        pt = add(b, mul(sub(c, b), w));
+        u0 = 0.0f, u1 = 1.0f - w;
+    } else {
+        float denom = 1.0f / (va + vb + vc);
+        float v = vb * denom, w = vc * denom;
+        pt = add(add(a, mul(ab, v)), mul(ac, w));
+        u0 = 1.0f - v - w, u1 = v;
+    }
+    r.p = pt;
+    r.d = norm(sub(p, pt));
+    r.u = u1;
+    r.v = 1.0f - u0 - u1;
+    r.t = 0.0f;
+    r.n = normalize(cross(ab, ac));
+    r.prim = tr.pIndex;
+}
+
+// SNCH3D.closest_point (snch3d.py:334-409)
+static CP snch3_closest(const Tree<3>& T, V3 p, Cnt* c) {
+    CP its;
+    std::memset(&its, 0, sizeof(its));
+    its.d = kInf;
+    its.prim = -1;
+    float r_max = kInf;
+    int st_i[64];
+    float st_d[64];
+    int sp = 0;
+    st_i[0] = 0;
+    st_d[0] = r_max;
+    while (sp >= 0) {
+        int ni = st_i[sp];
+        float cd = st_d[sp];
+        sp--;
+        if (!(cd <= r_max)) continue;
+        const Node<3>& node = T.nodes[ni];
+        if (c) c->bytes += SNODE3, c->nodes++;
+        if (node.nref > 0) {
+            for (int j = 0; j < node.nref; ++j) {
+                CP r;
+                tri_closest(T.prims[node.off + j], p, r);
+                if (c) c->bytes += TRI, c->prims++;
+                if (r.d < its.d) {
+                    its = r;
+                    r_max = fminf_(r_max, its.d);
+                }
+            }
+        } else {
+            int L = ni + 1, R = ni + node.off;
+            float dminL, dmaxL, dminR, dmaxR;
+            box_dist(node_min3(T.nodes[L]), node_max3(T.nodes[L]), p, dminL, dmaxL);
+            bool hitL = dminL <= r_max;
+            r_max = fminf_(r_max, dmaxL);
+            box_dist(node_min3(T.nodes[R]), node_max3(T.nodes[R]), p, dminR, dmaxR);
+            bool hitR = dminR <= r_max;
+            r_max = fminf_(r_max, dmaxR);
+            if (c) c->bytes += 2 * SNODE3, c->nodes += 2;
+            if (hitL && hitR) {
+                int closer = L, other = R;
+                if ((dminL == 0.0f) && (dminR == 0.0f)) {
+                    if (dmaxR < dmaxL) std::swap(closer, other);
+                }
+                if (dminR < dminL) {
+                    std::swap(closer, other);
+                    std::swap(dminL, dminR);
+                }
+                st_i[++sp] = other;
+                st_d[sp] = dminR;
+                st_i[++sp] = closer;
+                st_d[sp] = dminL;
+            } else if (hitL) {
+                st_i[++sp] = L;
+                st_d[sp] = dminL;
+            } else if (hitR) {
+                st_i[++sp] = R;
+                st_d[sp] = dminR;
+            }
+        }
+    }
+    return its;
+}
+
+// Triangle.ray_intersect (triangle.py:59-86); prim_id = sorted index
+static inline bool tri_ray(const Prim<3>& tr, int sorted, V3 x, V3 d, float rmax, Its& r) {
+    V3 a = v3(tr.v[0][0], tr.v[0][1], tr.v[0][2]);
+    V3 b = v3(tr.v[1][0], tr.v[1][1], tr.v[1][2]);
+    V3 c = v3(tr.v[2][0], tr.v[2][1], tr.v[2][2]);
+    V3 v1 = sub(b, a), v2_ = sub(c, a);
+    V3 p = cross(d, v2_);
+    float det = dot(v1, p);
+    if (!(std::fabs(det) > kDrEps)) return false;
+    float inv = 1.0f / det;
+    V3 s = sub(x, a);
+    float v = dot(s, p) * inv;
+    if (!((v >= 0.0f) && (v <= 1.0f))) return false;
+    V3 q = cross(s, v1);
+    float w = dot(d, q) * inv;
+    if (!((w >= 0.0f) && (v + w <= 1.0f))) return false;
+    float t = dot(v2_, q) * inv;
+    if (!((t >= 0.0f) && (t <= rmax))) return false;
+    r.valid = true;
+    r.on_boundary = true;
+    r.p = add(add(a, mul(v1, v)), mul(v2_, w));
+    r.n = normalize(cross(v1, v2_));
+    r.u = 1.0f - v - w;
+    r.v = v;
+    r.t = 0.0f;
+    r.d = t;
+    r.prim = sorted;
+    return true;
+}
+
+// SNCH3D.intersect (snch3d.py:411-476)
+static Its snch3_intersect(const Tree<3>& T, V3 x, V3 v, float r_max, Cnt* c) {
+    Its its;
+    std::memset(&its, 0, sizeof(its));
+    float tmin;
+    if (c) c->bytes += SNODE3, c->nodes++;
+    if (!box_ray(node_min3(T.nodes[0]), node_max3(T.nodes[0]), x, v, r_max, tmin)) return its;
+    int st_i[64];
+    float st_d[64];
+    int sp = 0;
+    st_i[0] = 0;
+    st_d[0] = tmin;
+    while (sp >= 0) {
+        int ni = st_i[sp];
+        float cd = st_d[sp];
+        sp--;
+        if (!(cd <= r_max)) continue;
+        const Node<3>& node = T.nodes[ni];
+        if (c) c->bytes += SNODE3, c->nodes++;
+        if (node.nref > 0) {
+            for (int j = 0; j < node.nref; ++j) {
+                Its r;
+                if (c) c->bytes += TRI, c->prims++;
+                if (tri_ray(T.prims[node.off + j], node.off + j, x, v, r_max, r) && r.d < r_max) {
+                    r_max = r.d;
+                    its = r;
+                }
+            }
+        } else {
+            int L = ni + 1, R = ni + node.off;
+            float t0, t1;
+            bool h0 = box_ray(node_min3(T.nodes[L]), node_max3(T.nodes[L]), x, v, r_max, t0);
+            bool h1 = box_ray(node_min3(T.nodes[R]), node_max3(T.nodes[R]), x, v, r_max, t1);
+            if (c) c->bytes += 2 * 24, c->nodes += 2;
+            if (h0 && h1) {
+                int closer = L, other = R;
+                if (t1 < t0) {
+                    std::swap(closer, other);
+                    std::swap(t0, t1);
+                }
+                st_i[++sp] = other;
+                st_d[sp] = t1;
+                st_i[++sp] = closer;
+                st_d[sp] = t0;
+            } else if (h0) {
+                st_i[++sp] = L;
+                st_d[sp] = t0;
+            } else if (h1) {
+                st_i[++sp] = R;
+                st_d[sp] = t1;
+            }
+        }
+    }
+    return its;
+}
+
+// ----------------------------------------------------------------------------
+// Brute-force queries: the reference's *_baseline methods
+// (polyline.py:246-302,468-496; mesh.py:312-335,440-472,558-570)
+// ----------------------------------------------------------------------------
+static CP brute_closest2(const Sub<2>& S, V2 p) {
+    float dmin = kInf;
+    int idx = -1;
+    size_t np = S.I.size() / 2;
+    for (size_t i = 0; i < np; ++i) {
+        V2 a = v2(S.V[2 * S.I[2 * i]], S.V[2 * S.I[2 * i] + 1]);
+        V2 b = v2(S.V[2 * S.I[2 * i + 1]], S.V[2 * S.I[2 * i + 1] + 1]);
+        V2 pa = sub(p, a), ba = sub(b, a);
+        float h = clampf(dot(pa, ba) / dot(ba, ba), 0.0f, 1.0f);
+        float d = norm(sub(pa, mul(ba, h)));
+        if (d < dmin) dmin = d, idx = (int)i;
+    }
+    CP r;
+    std::memset(&r, 0, sizeof(r));
+    r.d = dmin;
+    r.prim = idx;
+    return r;
+}
+static Its brute_ray2(const Sub<2>& S, V2 x, V2 v, float rmax) {
+    float dmin = rmax;
+    int idx = -1;
+    size_t np = S.I.size() / 2;
+    for (size_t i = 0; i < np; ++i) {
+        V2 a = v2(S.V[2 * S.I[2 * i]], S.V[2 * S.I[2 * i] + 1]);
+        V2 b = v2(S.V[2 * S.I[2 * i + 1]], S.V[2 * S.I[2 * i + 1] + 1]);
+        V2 u = sub(b, a), w = sub(x, a);
+        float d = cross2(v, u);
+        float s = cross2(v, w) / d;
+        float t = cross2(u, w) / d;
+        bool valid = (t > 0.0f) && (s >= 0.0f) && (s <= 1.0f);
+        float dd = valid ? t : kInf;
+        if (dd < dmin) dmin = dd, idx = (int)i;
+    }
+    Its r;
+    std::memset(&r, 0, sizeof(r));
+    r.valid = idx >= 0;
+    r.d = dmin;
+    r.prim = idx;
+    return r;
+}
+static CP brute_closest3(const Sub<3>& S, V3 p) {
+    float dmin = kInf;
+    int idx = -1;
+    for (size_t i = 0; i < S.tree.prims.size(); ++i) {
+        CP r;
+        tri_closest(S.tree.prims[i], p, r);
+        if (r.d < dmin) dmin = r.d, idx = r.prim;
+    }
+    CP r;
+    std::memset(&r, 0, sizeof(r));
+    r.d = dmin;
+    r.prim = idx;
+    return r;
+}
+static Its brute_ray3(const Sub<3>& S, V3 x, V3 v, float rmax) {
+    float dmin = rmax;
+    int idx = -1;
+    for (size_t i = 0; i < S.tree.prims.size(); ++i) {
+        Its r;
+        if (tri_ray(S.tree.prims[i], (int)i, x, v, kInf, r) && r.d < dmin) dmin = r.d, idx = S.tree.prims[i].pIndex;
+    }
+    Its r;
+    std::memset(&r, 0, sizeof(r));
+    r.valid = idx >= 0;
+    r.d = dmin;
+    r.prim = idx;
+    return r;
+}
+template <int D>
+static SilRec brute_sil(const Sub<D>& S, const float* x, float r_max) {
+    SilRec rec;
+    std::memset(&rec, 0, sizeof(rec));
+    float dmin = r_max;
+    size_t n = D == 2 ? S.sil2.size() : S.sil3.size();
+    for (size_t i = 0; i < n; ++i) {
+        SilRec r;
+        bool ok;
+        if constexpr (D == 2)
+            ok = silv_closest(S.sil2[i], (int)i, v2(x[0], x[1]), r_max, r);
+        else
+            ok = sile_closest(S.sil3[i], (int)i, v3(x[0], x[1], x[2]), r_max, r);
+        if (ok && r.d < dmin) dmin = r.d, rec = r;
+    }
+    return rec;
+}
+
+// ----------------------------------------------------------------------------
+// Boundary data (g = u(p) on Dirichlet, h = n . grad u on Neumann)
+// ----------------------------------------------------------------------------
+struct BC {
+    int code;
+    float c[4];
+    const float* gv;  // per-vertex g (global vertex ids) when code == BC_PER_VERTEX
+    const float* hv;
+    int src_code;
+    float src_c;
+};
+static inline float bc_u(const BC& bc, V3 p) {
+    switch (bc.code) {
+        case BC_EXPCOS: return cr_exp(p.x) * cr_cos(p.y);
+        case BC_QUAD: return p.x * p.x - p.y * p.y;
+        default: return ((bc.c[0] + bc.c[1] * p.x) + bc.c[2] * p.y) + bc.c[3] * p.z;
+    }
+}
+static inline V3 bc_grad(const BC& bc, V3 p) {
+    switch (bc.code) {
+        case BC_EXPCOS: {
+            float ex = cr_exp(p.x);
+            return v3(ex * cr_cos(p.y), -ex * cr_sin(p.y), 0.0f);
+        }
+        case BC_QUAD: return v3(2.0f * p.x, -2.0f * p.y, 0.0f);
+        default: return v3(bc.c[1], bc.c[2], bc.c[3]);
+    }
+}
+static inline float src_f(const BC& bc) { return bc.src_code == SRC_CONST ? bc.src_c : 0.0f; }
+
+// g on a Dirichlet closest-point record of sub-scene S
+template <int D>
+static inline float g_of(const BC& bc, const Sub<D>& S, const CP& c) {
+    if (bc.code != BC_PER_VERTEX) return bc_u(bc, c.p);
+    int gp = S.pglobal[c.prim];
+    (void)gp;
+    if (D == 2) {
+        float g0 = bc.gv[S.vglobal[S.I[2 * c.prim]]], g1 = bc.gv[S.vglobal[S.I[2 * c.prim + 1]]];
+        return lerpf(g0, g1, c.t);
+    } else {
+        float ga = bc.gv[S.vglobal[S.I[3 * c.prim]]], gb = bc.gv[S.vglobal[S.I[3 * c.prim + 1]]],
+              gc = bc.gv[S.vglobal[S.I[3 * c.prim + 2]]];
+        return (gb * c.u + gc * c.v) + ga * ((1.0f - c.u) - c.v);
+    }
+}
+// h at a point on prim `prim` of sub-scene S with parameters (t) or (u,v), normal n
+template <int D>
+static inline float h_of(const BC& bc, const Sub<D>& S, int prim, float t, float u, float v, V3 p, V3 n) {
+    if (bc.code != BC_PER_VERTEX) {
+        V3 g = bc_grad(bc, p);
+        return D == 2 ? dot(v2(n.x, n.y), v2(g.x, g.y)) : dot(n, g);
+    }
+    if (D == 2) {
+        float h0 = bc.hv[S.vglobal[S.I[2 * prim]]], h1 = bc.hv[S.vglobal[S.I[2 * prim + 1]]];
+        return lerpf(h0, h1, t);
+    } else {
+        float ha = bc.hv[S.vglobal[S.I[3 * prim]]], hb = bc.hv[S.vglobal[S.I[3 * prim + 1]]],
+              hc = bc.hv[S.vglobal[S.I[3 * prim + 2]]];
+        return (hb * u + hc * v) + ha * ((1.0f - u) - v);
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Sampling (distr.py, greens_fn.py, greens_fn_3d.py, math.py:325-353)
+// ----------------------------------------------------------------------------
+// DiscreteDistribution.sample_reuse_pmf (distr.py:52-107) with Dr.Jit binary_search
+template <int D>
+static inline int cdf_sample(const Sub<D>& S, float value, float& reuse, float& pmf, Cnt* c) {
+    float sample = value * S.pmf_sum;
+    int n = (int)S.cdf.size();
+    uint32_t start = 0, end = (uint32_t)(n - 1);
+    int iters = 0;
+    if (start < end) {
+        uint32_t x = end - start;
+        int l2 = 31 - __builtin_clz(x);
+        iters = l2 + 1;
+    }
+    for (int it = 0; it < iters; ++it) {
+        uint32_t mid = (start + end) >> 1;
+        float cv = S.cdf[mid];
+        bool cond = ((cv < sample) || (cv == 0.0f)) && (cv != S.pmf_sum);
+        if (cond)
+            start = std::min(mid + 1, end);
+        else
+            end = mid;
+    }
+    if (c) c->bytes += 4 * iters + 8;
+    int idx = (int)start;
+    pmf = S.pmf[idx] * S.pmf_norm;
+    float cdf = idx > 0 ? S.cdf[idx - 1] * S.pmf_norm : 0.0f;
+    reuse = (value - cdf) / pmf;
+    return idx;
+}
+
+static const float kInv2Pi_neg = (float)(-1.0 / (2.0 * 3.14159265358979323846));
+
+// GreensBall (greens_fn.py:14-287); ball centred at c
+static inline float greens2_G(V2 c, float R, V2 x, V2 y) {
+    x = sub(x, c);
+    y = sub(y, c);
+    x = divs(x, R);
+    y = divs(y, R);
+    float v = norm(sub(x, y));
+    return kInv2Pi_neg * cr_log((1.0f - dot(x, y)) / v);
+}
+// GreensBall._sample rejection loop (greens_fn.py:173-206)
+static inline void greens2_sample(V2 c, float R, V2 dir, Pcg& rng, V2& p, float& r, float& pdf) {
+    float bound = 1.5f / R;
+    int iteration = 0;
+    bool active = true;
+    r = 0.0f;
+    pdf = 0.0f;
+    while ((iteration < 1000) && active) {
+        float u = rng.next_float32();
+        r = rng.next_float32() * R;
+        pdf = (cr_log(R / r) / kTwoPi) / (R * R / 4.0f);
+        float pdf_radius = pdf / (1.0f / (kTwoPi * r));
+        iteration += 1;
+        if (u < pdf_radius / bound) active = false;
+    }
+    r = fmaxf_(r, 1e-6f);
+    if (r > R) r = R / 2.0f;
+    p = add(c, mul(dir, r));
+}
+static inline V2 greens2_gradient(V2 c, float R, float r, V2 y) {
+    float gn = (1.0f / (r * r) - 1.0f / (R * R)) / kTwoPi;
+    return mul(sub(y, c), gn);
+}
+static inline float greens3_G(V3 c, float R, V3 x, V3 y) {  // greens_fn_3d.py:10-12
+    float r = norm(sub(y, x));
+    return -(1.0f / r - R / (R * R - dot(sub(x, c), sub(y, c)))) / kFourPi;
+}
+static inline void greens3_sample(V3 c, float R, V3 dir, Pcg& rng, V3& p, float& r, float& pdf) {
+    float u1 = rng.next_float32();
+    float u2 = rng.next_float32();
+    float phi = kTwoPi * u2;
+    r = (1.0f + std::sqrt(1.0f - cr_cbrt(u1 * u1)) * cr_cos(phi)) * R / 2.0f;
+    r = fmaxf_(r, 1e-6f);
+    if (r > R) r = R / 2.0f;
+    pdf = ((1.0f / r - 1.0f / R) / kFourPi) / (R * R / 6.0f);
+    p = add(c, mul(dir, r));
+}
+static inline V3 greens3_gradient(V3 c, float R, float r, V3 y) {
+    float r3 = r * r * r;
+    float gn = (1.0f / r3 - 1.0f / (R * R * R)) / kFourPi;
+    return mul(sub(y, c), gn);
+}
+static inline V2 uniform_dir2(Pcg& rng) {  // uniform_angle (math.py:325-327)
+    float theta = (rng.next_float32() * 2.0f) * kPi;
+    return v2(cr_cos(theta), cr_sin(theta));
+}
+static inline V3 uniform_sphere(Pcg& rng) {  // math.py:348-353
+    float u0 = rng.next_float32();
+    float u1 = rng.next_float32();
+    float z = 1.0f - 2.0f * u0;
+    float r = std::sqrt(fmaxf_(0.0f, 1.0f - z * z));
+    float theta = kTwoPi * u1;
+    return v3(r * cr_cos(theta), r * cr_sin(theta), z);
+}
+
+// ----------------------------------------------------------------------------
+// Walks
+// ----------------------------------------------------------------------------
+struct Params {
+    int nwalks, nsteps, nsubwalks;
+    float eps, min_R;
+    int cv, av;
+    uint32_t seed;
+};
+struct WalkOut {
+    float value;
+    int steps;
+    int term_prim;  // Dirichlet prim the walk terminated on (-1: none)
+    Cnt cnt;
+};
+
+// sdf / inside (polyline.py:188-234, mesh.py:290-297) on the full scene
+static bool inside2(const Scene& sc, V2 x) {
+    const Sub<2>& F = sc.s2[0];
+    CP its = bvh2_closest(F.tree, x, nullptr);
+    int i0 = F.I[2 * its.prim], i1 = F.I[2 * its.prim + 1];
+    V2 n1 = v2(F.vnormal[2 * i0], F.vnormal[2 * i0 + 1]), n2 = v2(F.vnormal[2 * i1], F.vnormal[2 * i1 + 1]);
+    V2 n = lerp(n1, n2, its.t);
+    float s = dot(n, sub(x, v2(its.p.x, its.p.y))) >= 0.0f ? 1.0f : -1.0f;
+    return its.d * s < 0.0f;
+}
+static bool inside3(const Scene& sc, V3 x) {
+    CP c = snch3_closest(sc.s3[0].tree, x, nullptr);
+    float s = dot(c.n, sub(x, c.p)) >= 0.0f ? 1.0f : -1.0f;
+    return c.d * s < 0.0f;
+}
+
+// WoSt.u (wost.py:145-221)
+static WalkOut wost_u2(const Scene& sc, const BC& bc, const Params& P, V2 x, Pcg& rng, bool ob0, V2 n0,
+                       bool count) {
+    WalkOut out;
+    out.term_prim = -1;
+    Cnt* c = count ? &out.cnt : nullptr;
+    const Sub<2>& F = sc.s2[0];
+    const Sub<2>& Dn = sc.s2[1];
+    const Sub<2>& Nm = sc.s2[2];
+    V2 ip = x, in = n0;
+    bool ob = ob0;
+    float result = 0.0f;
+    int i = 0;
+    bool active = true;
+    while ((i < P.nsteps) && active) {
+        float R = kInf, d_dir = kInf;
+        if (Dn.present) {
+            CP cp = bvh2_closest(Dn.tree, ip, c);
+            R = d_dir = std::fabs(cp.d);
+        }
+        if (Nm.present) {
+            V2 p = ob ? sub(ip, mul(in, 1e-5f)) : ip;
+            float sr = star_radius<2>(Nm, &p.x, d_dir, c) * 0.98f;
+            sr = fmaxf_(sr, P.min_R);
+            R = fminf_(R, sr);
+        }
+        if (d_dir < P.eps) {
+            CP cp = bvh2_closest(Dn.tree, ip, c);
+            result += g_of<2>(bc, Dn, cp);
+            out.term_prim = cp.prim;
+            active = false;
+        } else {
+            V2 ref = v2(0.0f, 0.0f);
+            if (Nm.present) {
+                float rnd, pmf;
+                int idx = cdf_sample<2>(Nm, rng.next_float32(), rnd, pmf, c);
+                V2 a = v2(Nm.V[2 * Nm.I[2 * idx]], Nm.V[2 * Nm.I[2 * idx] + 1]);
+                V2 b = v2(Nm.V[2 * Nm.I[2 * idx + 1]], Nm.V[2 * Nm.I[2 * idx + 1] + 1]);
+                V2 tg = sub(b, a);
+                V2 bn = normalize(v2(tg.y, -tg.x));
+                V2 bp = lerp(a, b, rnd);
+                float bpdf = pmf / norm(tg);
+                // boundary_control_variates (wost.py:123-126): full-scene closest point
+                CP cr = bvh2_closest(F.tree, ip, c);
+                float hc = h_of<2>(bc, F, cr.prim, cr.t, 0.0f, 0.0f, cr.p, cr.n);
+                ref = v2(hc * cr.n.x, hc * cr.n.y);
+                // handle_boundary (wost.py:104-121)
+                float alpha = ob ? 2.0f : 1.0f;
+                float hb = 0.0f;
+                float G = greens2_G(ip, R, ip, bp);
+                G = clampf(G, -10.0f, 10.0f);
+                if (norm(sub(ip, bp)) <= R) {
+                    float h = h_of<2>(bc, Nm, idx, rnd, 0.0f, 0.0f, v3(bp.x, bp.y, 0.0f), v3(bn.x, bn.y, 0.0f));
+                    if (P.cv) h = h - dot(ref, bn);
+                    hb = 0.0f - G * h / bpdf;
+                }
+                result += alpha * hb;
+            }
+            V2 dir = uniform_dir2(rng);
+            if (ob && (dot(dir, in) > 0.0f)) dir = neg(dir);
+            V2 _p = ip;
+            // Polyline.intersect (polyline.py:435-466)
+            V2 o = ob ? sub(ip, mul(in, 1e-5f)) : ip;
+            Its its = bvh2_intersect(F.tree, o, dir, R, c);
+            if (!its.valid) {
+                V2 q = add(o, mul(dir, R));
+                its.p = v3(q.x, q.y, 0.0f);
+                its.n = v3(dir.x, dir.y, 0.0f);
+                its.d = R;
+                its.on_boundary = false;
+            }
+            ip = v2(its.p.x, its.p.y);
+            in = v2(its.n.x, its.n.y);
+            ob = its.on_boundary;
+            if (P.cv) result += dot(sub(_p, ip), ref);
+            // handle_source (wost.py:128-143)
+            V2 sp;
+            float sr_, spdf;
+            greens2_sample(_p, R, dir, rng, sp, sr_, spdf);
+            float src = 0.0f;
+            if (norm(sub(sp, _p)) < its.d) src += (R * R / 4.0f) * src_f(bc);
+            result += src;
+        }
+        i += 1;
+    }
+    if (bad(result) || std::fabs(result) > 100.0f) result = 0.0f;
+    out.value = result;
+    out.steps = i;
+    return out;
+}
+
+// WoSt3D.u (wost_3d.py:43-106)
+static WalkOut wost_u3(const Scene& sc, const BC& bc, const Params& P, V3 x, Pcg& rng, bool ob0, V3 n0,
+                       bool count) {
+    WalkOut out;
+    out.term_prim = -1;
+    Cnt* c = count ? &out.cnt : nullptr;
+    const Sub<3>& F = sc.s3[0];
+    const Sub<3>& Dn = sc.s3[1];
+    const Sub<3>& Nm = sc.s3[2];
+    V3 ip = x, in = n0;
+    bool ob = ob0;
+    float result = 0.0f;
+    int i = 0;
+    bool active = true;
+    while ((i < P.nsteps) && active) {
+        float R = kInf, d_dir = kInf;
+        if (Dn.present) {
+            CP cp = snch3_closest(Dn.tree, ip, c);
+            R = d_dir = std::fabs(cp.d);
+        }
+        V3 p = ob ? sub(ip, mul(in, 1e-3f)) : ip;
+        float sr = star_radius<3>(F, &p.x, d_dir, c) * 0.98f;
+        sr = fmaxf_(sr, P.min_R);
+        R = fminf_(R, sr);
+        if (d_dir < P.eps) {
+            CP cp = snch3_closest(Dn.tree, ip, c);
+            result += g_of<3>(bc, Dn, cp);
+            out.term_prim = cp.prim;
+            active = false;
+        } else {
+            if (Nm.present) {
+                // Mesh.sample_boundary (mesh.py:474-490)
+                float rnd, pmf;
+                int idx = cdf_sample<3>(Nm, rng.next_float32(), rnd, pmf, c);
+                const float* A = &Nm.V[3 * Nm.I[3 * idx]];
+                const float* B = &Nm.V[3 * Nm.I[3 * idx + 1]];
+                const float* C = &Nm.V[3 * Nm.I[3 * idx + 2]];
+                V3 a = v3(A[0], A[1], A[2]), b = v3(B[0], B[1], B[2]), cc = v3(C[0], C[1], C[2]);
+                V3 cr = cross(sub(b, a), sub(cc, a));
+                float area = norm(cr) / 2.0f;
+                float r1 = rng.next_float32();
+                float r2 = rng.next_float32();
+                float su = std::sqrt(r1) * (1.0f - r2);
+                float sv = std::sqrt(r1) * r2;
+                V3 bp = add(add(mul(b, su), mul(cc, sv)), mul(a, (1.0f - su) - sv));
+                V3 bn = normalize(cr);
+                float bpdf = pmf / area;
+                float G = greens3_G(ip, R, ip, bp);
+                G = clampf(G, -100.0f, 100.0f);
+                float alpha = ob ? 2.0f : 1.0f;
+                if (norm(sub(ip, bp)) <= R) {
+                    float h = h_of<3>(bc, Nm, idx, 0.0f, su, sv, bp, bn);
+                    result -= G * h * alpha / bpdf;
+                }
+            }
+            V3 dir = uniform_sphere(rng);
+            if (ob && (dot(dir, in) > 0.0f)) dir = neg(dir);
+            V3 _p = ip;
+            V3 o = ob ? sub(ip, mul(in, 1e-5f)) : ip;
+            Its its = snch3_intersect(F.tree, o, dir, R, c);
+            if (!its.valid) {
+                its.p = add(o, mul(dir, R));
+                its.n = dir;
+                its.d = R;
+                its.on_boundary = false;
+            }
+            ip = its.p;
+            in = its.n;
+            ob = its.on_boundary;
+            V3 sp;
+            float sr_, spdf;
+            greens3_sample(_p, R, dir, rng, sp, sr_, spdf);
+            float src = 0.0f;
+            if (norm(sub(sp, _p)) < its.d) src += (R * R / 6.0f) * src_f(bc);
+            result += src;
+        }
+        i += 1;
+    }
+    out.value = result;
+    out.steps = i;
+    return out;
+}
+
+// Solver.walk (solver.py:73-92): nsubwalks sequential walks, averaged
+static WalkOut walk_u(const Scene& sc, const BC& bc, const Params& P, const float* x, Pcg& rng, bool count) {
+    WalkOut acc;
+    acc.value = 0.0f;
+    acc.steps = 0;
+    acc.term_prim = -1;
+    float res = 0.0f;
+    for (int s = 0; s < P.nsubwalks; ++s) {
+        WalkOut w = sc.dim == 2 ? wost_u2(sc, bc, P, v2(x[0], x[1]), rng, false, v2(0, 0), count)
+                                : wost_u3(sc, bc, P, v3(x[0], x[1], x[2]), rng, false, v3(0, 0, 0), count);
+        res += w.value;
+        acc.steps += w.steps;
+        acc.term_prim = w.term_prim;
+        acc.cnt.bytes += w.cnt.bytes;
+        acc.cnt.nodes += w.cnt.nodes;
+        acc.cnt.prims += w.cnt.prims;
+        acc.cnt.sils += w.cnt.sils;
+    }
+    acc.value = res / (float)P.nsubwalks;
+    return acc;
+}
+
+// WoSt.dude (wost.py:30-102); returns result / nwalks
+static float dude2(const Scene& sc, const BC& bc, const Params& P, V2 p, V2 e, Pcg& rng, uint64_t* steps) {
+    CP cp = bvh2_closest(sc.s2[0].tree, p, nullptr);
+    float R = std::fabs(cp.d);
+    float u_sum = 0.0f, u_count = 0.0f, src_sum = 0.0f, src_count = 0.0f, result = 0.0f;
+    int i = 0;
+    bool active = inside2(sc, p);
+    float gnorm = R * R / 4.0f;
+    while ((i < P.nwalks) && active) {
+        V2 dir = uniform_dir2(rng);
+        V2 dir_src = uniform_dir2(rng);
+        V2 p_src;
+        float r_src, pdf_src;
+        greens2_sample(p, R, dir_src, rng, p_src, r_src, pdf_src);
+        int niters = P.av ? 2 : 1;
+        uint64_t state = rng.state;
+        int it = 0;
+        while ((it < niters) && (i < P.nwalks)) {
+            if (it == 1) {
+                dir_src = neg(dir_src);
+                p_src = sub(mul(p, 2.0f), p_src);
+            }
+            float src = gnorm * src_f(bc);
+            float src_grad = dot(greens2_gradient(p, R, r_src, p_src), e) / (pdf_src * gnorm);
+            src_sum += src;
+            src_count += 1.0f;
+            float _src = src;
+            if (P.cv) _src = src - (src_sum / src_count);
+            result += _src * src_grad;
+            if (it == 1) dir = neg(dir);
+            V2 _p = add(p, mul(dir, R));
+            rng.state = state;
+            WalkOut w = wost_u2(sc, bc, P, _p, rng, false, v2(0, 0), false);
+            if (steps) *steps += (uint64_t)w.steps;
+            float u = w.value;
+            u_sum += u;
+            u_count += 1.0f;
+            float _u = u;
+            if (P.cv) _u = u - u_sum / u_count;
+            float r2 = 2.0f / R * _u * dot(e, dir);
+            if ((std::fabs(r2) > 1e6f) || bad(r2)) r2 = 0.0f;
+            result += r2;
+            i += 1;
+            it += 1;
+        }
+    }
+    return result / (float)P.nwalks;
+}
+
+// WoSt3D.dude (wost_3d.py:108-175)
+static float dude3(const Scene& sc, const BC& bc, const Params& P, V3 p, V3 e, Pcg& rng, uint64_t* steps) {
+    CP cp = snch3_closest(sc.s3[0].tree, p, nullptr);
+    float R = std::fabs(cp.d);
+    float u_mean = 0.0f, u_count = 0.0f, src_mean = 0.0f, src_count = 0.0f, result = 0.0f;
+    int i = 0;
+    bool active = inside3(sc, p);
+    float gnorm = R * R / 6.0f;
+    while ((i < P.nwalks) && active) {
+        V3 dir = uniform_sphere(rng);
+        V3 p_src;
+        float r_src, pdf_src;
+        greens3_sample(p, R, dir, rng, p_src, r_src, pdf_src);
+        int niters = P.av ? 2 : 1;
+        uint64_t state = rng.state;
+        int it = 0;
+        while ((it < niters) && (i < P.nwalks)) {
+            if (it == 1) {
+                dir = neg(dir);
+                p_src = sub(mul(p, 2.0f), p_src);
+            }
+            float src = gnorm * src_f(bc);
+            float src_grad = dot(greens3_gradient(p, R, r_src, p_src), e) / (pdf_src * gnorm);
+            src_mean = src_mean * (src_count / (src_count + 1.0f)) + src / (src_count + 1.0f);
+            src_count += 1.0f;
+            float _src = src;
+            if (P.cv) _src = src - src_mean;
+            result += _src * src_grad;
+            if (it == 1) dir = neg(dir);  // second flip (wost_3d.py:152-153)
+            V3 _p = add(p, mul(dir, R));
+            rng.state = state;
+            WalkOut w = wost_u3(sc, bc, P, _p, rng, false, v3(0, 0, 0), false);
+            if (steps) *steps += (uint64_t)w.steps;
+            float u = w.value;
+            if (bad(u) || std::fabs(u) > 10000.0f) u = 0.0f;
+            u_mean = u_mean * (u_count / (u_count + 1.0f)) + u / (u_count + 1.0f);
+            u_count += 1.0f;
+            float _u = u;
+            if (P.cv) _u = u - u_mean;
+            result += 3.0f / R * _u * dot(e, dir);
+            i += 1;
+            it += 1;
+        }
+    }
+    if (bad(result) || std::fabs(result) > 10000.0f) result = 0.0f;
+    return result / (float)P.nwalks;
+}
+
+// Fused u + grad u estimator (this build's throughput form; not a reference
+// function): per antithetic pair k of point i, a PCG32 stream seeded like
+// Solver.solve with lane = i*npairs + k; direction (and the source radius) are
+// drawn as in WoSt.dude / WoSt3D.dude, the state is saved, and the two walks
+// from p +- R*dir replay the same stream.
+struct PairOut {
+    float usum;      // u(+) + u(-) + src(+) + src(-)
+    float g[3];      // sum over the two walks of the gradient terms
+    int steps;
+};
+static PairOut fused_pair(const Scene& sc, const BC& bc, const Params& P, const float* x, float R, bool inside,
+                          Pcg& rng) {
+    PairOut o;
+    std::memset(&o, 0, sizeof(o));
+    if (!inside) return o;
+    int D = sc.dim;
+    if (D == 2) {
+        V2 p = v2(x[0], x[1]);
+        V2 dir = uniform_dir2(rng);
+        V2 dir_src = uniform_dir2(rng);
+        V2 p_src;
+        float r_src, pdf_src;
+        greens2_sample(p, R, dir_src, rng, p_src, r_src, pdf_src);
+        float gnorm = R * R / 4.0f;
+        uint64_t state = rng.state;
+        WalkOut wp = wost_u2(sc, bc, P, add(p, mul(dir, R)), rng, false, v2(0, 0), false);
+        rng.state = state;
+        WalkOut wm = wost_u2(sc, bc, P, sub(p, mul(dir, R)), rng, false, v2(0, 0), false);
+        float fsrc = src_f(bc);
+        float s_p = gnorm * fsrc, s_m = gnorm * fsrc;
+        V2 gs = divs(greens2_gradient(p, R, r_src, p_src), pdf_src * gnorm);
+        float du = wp.value - wm.value;
+        float k = 2.0f / R * du;
+        o.usum = (wp.value + wm.value) + (s_p + s_m);
+        o.g[0] = k * dir.x + (s_p - s_m) * gs.x;
+        o.g[1] = k * dir.y + (s_p - s_m) * gs.y;
+        o.steps = wp.steps + wm.steps;
+    } else {
+        V3 p = v3(x[0], x[1], x[2]);
+        V3 dir = uniform_sphere(rng);
+        V3 p_src;
+        float r_src, pdf_src;
+        greens3_sample(p, R, dir, rng, p_src, r_src, pdf_src);
+        float gnorm = R * R / 6.0f;
+        uint64_t state = rng.state;
+        WalkOut wp = wost_u3(sc, bc, P, add(p, mul(dir, R)), rng, false, v3(0, 0, 0), false);
+        rng.state = state;
+        WalkOut wm = wost_u3(sc, bc, P, sub(p, mul(dir, R)), rng, false, v3(0, 0, 0), false);
+        float up = wp.value, um = wm.value;
+        if (bad(up) || std::fabs(up) > 10000.0f) up = 0.0f;
+        if (bad(um) || std::fabs(um) > 10000.0f) um = 0.0f;
+        float fsrc = src_f(bc);
+        float s_p = gnorm * fsrc, s_m = gnorm * fsrc;
+        V3 gs = divs(greens3_gradient(p, R, r_src, p_src), pdf_src * gnorm);
+        float k = 3.0f / R * (up - um);
+        o.usum = (up + um) + (s_p + s_m);
+        o.g[0] = k * dir.x + (s_p - s_m) * gs.x;
+        o.g[1] = k * dir.y + (s_p - s_m) * gs.y;
+        o.g[2] = k * dir.z + (s_p - s_m) * gs.z;
+        o.steps = wp.steps + wm.steps;
+    }
+    for (int q = 0; q < 3; ++q)
+        if (bad(o.g[q])) o.g[q] = 0.0f;
+    return o;
+}
+
+}  // namespace orc
+
+// ============================================================================
+// C ABI for ctypes (tests / bench cpu_baseline only)
+// ============================================================================
+using namespace orc;
+
+extern "C" {
+
+void* orc_scene_create(int dim, const float* V, int64_t nv, const int32_t* I, const int32_t* types, int64_t np) {
+    Scene* sc = new Scene();
+    sc->dim = dim;
+    sc->V.assign(V, V + nv * dim);
+    sc->I.assign(I, I + np * dim);
+    sc->types.assign(types, types + np);
+    if (dim == 2) {
+        make_sub<2>(*sc, -1, sc->s2[0]);
+        make_sub<2>(*sc, 1, sc->s2[1]);
+        make_sub<2>(*sc, 0, sc->s2[2]);
+    } else {
+        make_sub<3>(*sc, -1, sc->s3[0]);
+        make_sub<3>(*sc, 1, sc->s3[1]);
+        make_sub<3>(*sc, 0, sc->s3[2]);
+    }
+    return sc;
+}
+void orc_scene_destroy(void* s) { delete (Scene*)s; }
+
+// sizes: [present, nverts, nprims, nnodes, nrefs(silhouette refs), max_depth, py_sils, nsil_topology]
+void orc_sub_info(void* s, int sub, int64_t* out) {
+    Scene* sc = (Scene*)s;
+    if (sc->dim == 2) {
+        Sub<2>& S = sc->s2[sub];
+        out[0] = S.present;
+        out[1] = S.V.size() / 2;
+        out[2] = S.I.size() / 2;
+        out[3] = S.tree.nodes.size();
+        out[4] = S.refs.size();
+        out[5] = S.tree.max_depth;
+        out[6] = S.py_sils;
+        out[7] = S.sil2.size();
+    } else {
+        Sub<3>& S = sc->s3[sub];
+        out[0] = S.present;
+        out[1] = S.V.size() / 3;
+        out[2] = S.I.size() / 3;
+        out[3] = S.tree.nodes.size();
+        out[4] = S.refs.size();
+        out[5] = S.tree.max_depth;
+        out[6] = S.py_sils;
+        out[7] = S.sil3.size();
+    }
+}
+
+// node export: box[nn*2*dim] (min then max per node), ints[nn*4] (off, nref, sil_off, nsil),
+// cone[nn*(dim+2)] (axis, half_angle, radius); prims_index[np] (sorted pIndex);
+// refs[nrefs]
+void orc_sub_export(void* s, int sub, float* box, int32_t* ints, float* cone, int32_t* prim_index, int32_t* refs) {
+    Scene* sc = (Scene*)s;
+    auto go = [&](auto& S, int D) {
+        for (size_t i = 0; i < S.tree.nodes.size(); ++i) {
+            auto& n = S.tree.nodes[i];
+            for (int d = 0; d < D; ++d) {
+                box[i * 2 * D + d] = n.box.mn[d];
+                box[i * 2 * D + D + d] = n.box.mx[d];
+                cone[i * (D + 2) + d] = n.axis[d];
+            }
+            cone[i * (D + 2) + D] = n.half_angle;
+            cone[i * (D + 2) + D + 1] = n.radius;
+            ints[i * 4 + 0] = n.off;
+            ints[i * 4 + 1] = n.nref;
+            ints[i * 4 + 2] = n.sil_off;
+            ints[i * 4 + 3] = n.nsil;
+        }
+        for (size_t p = 0; p < S.tree.prims.size(); ++p) prim_index[p] = S.tree.prims[p].pIndex;
+        for (size_t r = 0; r < S.refs.size(); ++r) refs[r] = S.refs[r];
+    };
+    if (sc->dim == 2)
+        go(sc->s2[sub], 2);
+    else
+        go(sc->s3[sub], 3);
+}
+
+// Queries.  kind: 0 BVH closest point, 1 ray intersect (BVH), 2 closest silhouette,
+// 3 star radius, 10/11/12 brute force closest point / ray / silhouette.
+// in: pts[n*dim], dirs[n*dim] (ray), rmax[n].  out: prim[n], d[n], p[n*dim], nrm[n*dim], tuv[n*2]
+void orc_query(void* s, int sub, int kind, const float* pts, const float* dirs, const float* rmax, int64_t n,
+               int32_t* prim, float* d, float* p, float* nrm, float* tuv) {
+    Scene* sc = (Scene*)s;
+    int D = sc->dim;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t q = 0; q < n; ++q) {
+        const float* x = pts + q * D;
+        int pr = -1;
+        float dd = 0, pp[3] = {0, 0, 0}, nn[3] = {0, 0, 0}, tt[2] = {0, 0};
+        if (D == 2) {
+            const Sub<2>& S = sc->s2[sub];
+            V2 xv = v2(x[0], x[1]);
+            if (kind == 0) {
+                CP c = bvh2_closest(S.tree, xv, nullptr);
+                pr = c.prim, dd = c.d, pp[0] = c.p.x, pp[1] = c.p.y, nn[0] = c.n.x, nn[1] = c.n.y, tt[0] = c.t;
+            } else if (kind == 1) {
+                V2 v = v2(dirs[q * 2], dirs[q * 2 + 1]);
+                Its it = bvh2_intersect(S.tree, xv, v, rmax[q], nullptr);
+                pr = it.valid ? it.prim : -1;
+                dd = it.d, pp[0] = it.p.x, pp[1] = it.p.y, nn[0] = it.n.x, nn[1] = it.n.y, tt[0] = it.t;
+            } else if (kind == 2) {
+                SilRec r = snch_closest_sil<2>(S, x, rmax[q], nullptr);
+                pr = r.valid ? r.prim : -1, dd = r.d, pp[0] = r.p.x, pp[1] = r.p.y;
+            } else if (kind == 3) {
+                dd = star_radius<2>(S, x, rmax[q], nullptr);
+            } else if (kind == 10) {
+                CP c = brute_closest2(S, xv);
+                pr = c.prim, dd = c.d;
+            } else if (kind == 11) {
+                V2 v = v2(dirs[q * 2], dirs[q * 2 + 1]);
+                Its it = brute_ray2(S, xv, v, rmax[q]);
+                pr = it.valid ? it.prim : -1, dd = it.d;
+            } else if (kind == 12) {
+                SilRec r = brute_sil<2>(S, x, rmax[q]);
+                pr = r.valid ? S.refs.empty() ? r.prim : r.prim : -1, dd = r.d;
+            }
+        } else {
+            const Sub<3>& S = sc->s3[sub];
+            V3 xv = v3(x[0], x[1], x[2]);
+            if (kind == 0) {
+                CP c = snch3_closest(S.tree, xv, nullptr);
+                pr = c.prim, dd = c.d;
+                pp[0] = c.p.x, pp[1] = c.p.y, pp[2] = c.p.z, nn[0] = c.n.x, nn[1] = c.n.y, nn[2] = c.n.z;
+                tt[0] = c.u, tt[1] = c.v;
+            } else if (kind == 1) {
+                V3 v = v3(dirs[q * 3], dirs[q * 3 + 1], dirs[q * 3 + 2]);
+                Its it = snch3_intersect(S.tree, xv, v, rmax[q], nullptr);
+                pr = it.valid ? it.prim : -1, dd = it.d;
+                pp[0] = it.p.x, pp[1] = it.p.y, pp[2] = it.p.z, nn[0] = it.n.x, nn[1] = it.n.y, nn[2] = it.n.z;
+                tt[0] = it.u, tt[1] = it.v;
+            } else if (kind == 2) {
+                SilRec r = snch_closest_sil<3>(S, x, rmax[q], nullptr);
+                pr = r.valid ? r.prim : -1, dd = r.d, pp[0] = r.p.x, pp[1] = r.p.y, pp[2] = r.p.z;
+            } else if (kind == 3) {
+                dd = star_radius<3>(S, x, rmax[q], nullptr);
+            } else if (kind == 10) {
+                CP c = brute_closest3(S, xv);
+                pr = c.prim, dd = c.d;
+            } else if (kind == 11) {
+                V3 v = v3(dirs[q * 3], dirs[q * 3 + 1], dirs[q * 3 + 2]);
+                Its it = brute_ray3(S, xv, v, rmax[q]);
+                pr = it.valid ? it.prim : -1, dd = it.d;
+            } else if (kind == 12) {
+                SilRec r = brute_sil<3>(S, x, rmax[q]);
+                pr = r.valid ? r.prim : -1, dd = r.d;
+            }
+        }
+        if (prim) prim[q] = pr;
+        if (d) d[q] = dd;
+        for (int k = 0; k < D; ++k) {
+            if (p) p[q * D + k] = pp[k];
+            if (nrm) nrm[q * D + k] = nn[k];
+        }
+        if (tuv) tuv[q * 2] = tt[0], tuv[q * 2 + 1] = tt[1];
+    }
+}
+
+// topology silhouette -> index used by brute force; returns the topology id of a ref
+int32_t orc_ref_topology(void* s, int sub, int32_t ref) {
+    Scene* sc = (Scene*)s;
+    return sc->dim == 2 ? sc->s2[sub].refs[ref] : sc->s3[sub].refs[ref];
+}
+
+static BC make_bc(int code, const float* c, const float* gv, const float* hv, int src_code, float src_c) {
+    BC bc;
+    bc.code = code;
+    for (int k = 0; k < 4; ++k) bc.c[k] = c ? c[k] : 0.0f;
+    bc.gv = gv;
+    bc.hv = hv;
+    bc.src_code = src_code;
+    bc.src_c = src_c;
+    return bc;
+}
+static Params make_params(const int32_t* ip, const float* fp) {
+    Params P;
+    P.nwalks = ip[0];
+    P.nsteps = ip[1];
+    P.nsubwalks = ip[2];
+    P.cv = ip[3];
+    P.av = ip[4];
+    P.seed = (uint32_t)ip[5];
+    P.eps = fp[0];
+    P.min_R = fp[1];
+    return P;
+}
+
+// Solver.solve: per-walk values/steps (n*W) and per-point means (Kahan, double)
+// ip = [nwalks, nsteps, nsubwalks, cv, av, seed], fp = [eps, min_R]
+void orc_solve_u(void* s, const float* pts, int64_t n, const int32_t* ip, const float* fp, int bc_code,
+                 const float* bc_c, const float* gv, const float* hv, int src_code, float src_c, float* mean,
+                 float* walk_val, int32_t* walk_steps, int32_t* walk_term, uint64_t* bytes_total) {
+    Scene* sc = (Scene*)s;
+    Params P = make_params(ip, fp);
+    BC bc = make_bc(bc_code, bc_c, gv, hv, src_code, src_c);
+    int D = sc->dim;
+    int64_t W = P.nwalks;
+    uint64_t bytes = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : bytes)
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t a, b;
+        tea(P.seed, (uint32_t)i, 4, kTeaKey, &a, &b);
+        double acc = 0.0;
+        for (int64_t k = 0; k < W; ++k) {
+            uint64_t lane = (uint64_t)(i * W + k);
+            Pcg rng = pcg_for_lane(a, b, lane);
+            WalkOut w = walk_u(*sc, bc, P, pts + i * D, rng, bytes_total != nullptr);
+            acc += (double)w.value;
+            bytes += w.cnt.bytes;
+            if (walk_val) walk_val[lane] = w.value;
+            if (walk_steps) walk_steps[lane] = w.steps;
+            if (walk_term) walk_term[lane] = w.term_prim;
+        }
+        if (mean) mean[i] = (float)(acc / (double)W);
+    }
+    if (bytes_total) *bytes_total = bytes;
+}
+
+// WoSt.solve_dude (wost.py:21-28): one stream per point
+void orc_solve_dude(void* s, const float* pts, const float* e, int64_t n, const int32_t* ip, const float* fp,
+                    int bc_code, const float* bc_c, const float* gv, const float* hv, int src_code, float src_c,
+                    float* out, uint64_t* steps_total) {
+    Scene* sc = (Scene*)s;
+    Params P = make_params(ip, fp);
+    BC bc = make_bc(bc_code, bc_c, gv, hv, src_code, src_c);
+    int D = sc->dim;
+    uint64_t steps = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : steps)
+    for (int64_t i = 0; i < n; ++i) {
+        Pcg rng = pcg_for(P.seed, (uint32_t)i, (uint64_t)i);
+        uint64_t st = 0;
+        if (D == 2)
+            out[i] = dude2(*sc, bc, P, v2(pts[2 * i], pts[2 * i + 1]), v2(e[2 * i], e[2 * i + 1]), rng, &st);
+        else
+            out[i] = dude3(*sc, bc, P, v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]),
+                           v3(e[3 * i], e[3 * i + 1], e[3 * i + 2]), rng, &st);
+        steps += st;
+    }
+    if (steps_total) *steps_total = steps;
+}
+
+// Fused u + grad u.  pair_u[n*np], pair_g[n*np*dim], pair_steps[n*np] optional.
+void orc_solve_fused(void* s, const float* pts, int64_t n, const int32_t* ip, const float* fp, int bc_code,
+                     const float* bc_c, const float* gv, const float* hv, int src_code, float src_c, float* u,
+                     float* grad, float* pair_u, float* pair_g, int32_t* pair_steps, uint64_t* steps_total) {
+    Scene* sc = (Scene*)s;
+    Params P = make_params(ip, fp);
+    BC bc = make_bc(bc_code, bc_c, gv, hv, src_code, src_c);
+    int D = sc->dim;
+    int64_t NP = (P.nwalks + 1) / 2;
+    uint64_t steps = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : steps)
+    for (int64_t i = 0; i < n; ++i) {
+        const float* x = pts + i * D;
+        float R;
+        bool ins;
+        if (D == 2) {
+            R = std::fabs(bvh2_closest(sc->s2[0].tree, v2(x[0], x[1]), nullptr).d);
+            ins = inside2(*sc, v2(x[0], x[1]));
+        } else {
+            R = std::fabs(snch3_closest(sc->s3[0].tree, v3(x[0], x[1], x[2]), nullptr).d);
+            ins = inside3(*sc, v3(x[0], x[1], x[2]));
+        }
+        uint32_t a, b;
+        tea(P.seed, (uint32_t)i, 4, kTeaKey, &a, &b);
+        double su = 0.0, sg[3] = {0, 0, 0};
+        for (int64_t k = 0; k < NP; ++k) {
+            uint64_t lane = (uint64_t)(i * NP + k);
+            Pcg rng = pcg_for_lane(a, b, lane);
+            PairOut o = fused_pair(*sc, bc, P, x, R, ins, rng);
+            su += o.usum;
+            for (int q = 0; q < D; ++q) sg[q] += o.g[q];
+            steps += (uint64_t)o.steps;
+            if (pair_u) pair_u[lane] = o.usum;
+            if (pair_g)
+                for (int q = 0; q < D; ++q) pair_g[lane * D + q] = o.g[q];
+            if (pair_steps) pair_steps[lane] = o.steps;
+        }
+        double nw = (double)(2 * NP);
+        if (u) u[i] = (float)(su / nw);
+        if (grad)
+            for (int q = 0; q < D; ++q) grad[i * D + q] = (float)(sg[q] / nw);
+    }
+    if (steps_total) *steps_total = steps;
+}
+
+// RNG / hashing exports for known-answer tests
+void orc_tea(uint32_t v0, uint32_t v1, int rounds, const uint32_t* key, uint32_t* out) {
+    tea(v0, v1, rounds, key ? key : kTeaKey, &out[0], &out[1]);
+}
+void orc_pcg_seq(uint64_t initstate, uint64_t initseq, int n, uint32_t* out) {
+    Pcg r;
+    r.seed(initstate, initseq);
+    for (int i = 0; i < n; ++i) out[i] = r.next_u32();
+}
+void orc_pcg_lane_floats(uint32_t seed, uint32_t point, uint64_t lane, int n, float* out) {
+    Pcg r = pcg_for(seed, point, lane);
+    for (int i = 0; i < n; ++i) out[i] = r.next_float32();
+}
+// math exports for pinning the correctly rounded functions
+void orc_math(int fn, const float* x, const float* y, int64_t n, float* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        float a = x[i];
+        switch (fn) {
+            case 0: out[i] = cr_log(a); break;
+            case 1: out[i] = cr_exp(a); break;
+            case 2: out[i] = cr_sin(a); break;
+            case 3: out[i] = cr_cos(a); break;
+            case 4: out[i] = cr_acos(a); break;
+            case 5: out[i] = cr_asin(a); break;
+            case 6: out[i] = cr_atan2(a, y[i]); break;
+            case 7: out[i] = cr_cbrt(a); break;
+        }
+    }
+}
+int orc_num_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"

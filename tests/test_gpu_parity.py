@@ -1,0 +1,186 @@
+"""GPU parity: the HIP kernels (called through the C ABI via ctypes) against
+the CPU restatement on the same seeded inputs.  Query kernels must agree bit
+for bit (primitive ids, distances, points, normals); walks must agree per walk
+(value bits, step count, terminal primitive) apart from a stated fraction of
+branch flips from once-rounded transcendentals (expected 0 at these sizes)."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import dwost_amd
+
+    dwost_amd._lib.lib()
+    return dwost_amd
+
+
+def _t(a):
+    return torch.as_tensor(np.ascontiguousarray(a)).cuda()
+
+
+def _np(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def _same_bits(a, b):
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    if a.dtype == np.float32:
+        return np.array_equal(a.view(np.uint32), b.astype(np.float32).view(np.uint32))
+    return np.array_equal(a, b)
+
+
+def _scenes(gpu, oracle, spec):
+    cls = gpu.Scene2D if spec.dim == 2 else gpu.Scene3D
+    return cls(spec.vertices, spec.indices, spec.types), oracle.OracleScene(spec.vertices, spec.indices, spec.types)
+
+
+def _dirs(n, dim, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal((n, dim))
+    return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+
+
+@pytest.mark.parametrize("which", ["star", "holes", "ico"])
+def test_queries_bit_exact(gpu, oracle, which, star_spec, holes_spec, ico_spec):
+    spec = dict(star=star_spec, holes=holes_spec, ico=ico_spec)[which]
+    gs, os_ = _scenes(gpu, oracle, spec)
+    rng = np.random.default_rng(11)
+    lo, hi = spec.vertices.min(0), spec.vertices.max(0)
+    q = np.concatenate([spec.queries[:1536], rng.uniform(lo - 0.3, hi + 0.3, (512, spec.dim)).astype(np.float32)])
+    D = spec.dim
+    for sub in range(3):
+        if not os_.info(sub)["present"]:
+            continue
+        g = gs.closest_point(_t(q), sub=sub)
+        o = os_.query(sub, oracle.KIND_CLOSEST, q)
+        assert _same_bits(_np(g["prim"]), o["prim"]), (sub, "prim")
+        assert _same_bits(_np(g["d"]), o["d"]), (sub, "d")
+        assert _same_bits(_np(g["p"]), o["p"]), (sub, "p")
+        assert _same_bits(_np(g["n"]), o["n"]), (sub, "n")
+        assert _same_bits(_np(g["tuv"]), o["tuv"] if D == 3 else np.stack([o["tuv"][:, 0], 0 * o["tuv"][:, 0]], 1))
+        for rm in (np.full(len(q), np.inf, np.float32), np.full(len(q), 0.05, np.float32)):
+            g = gs.closest_silhouette(_t(q), _t(rm), sub=sub)
+            o = os_.query(sub, oracle.KIND_SIL, q, rmax=rm)
+            assert _same_bits(_np(g["sil"]), o["prim"]), (sub, "sil")
+            ok = o["prim"] >= 0
+            assert _same_bits(_np(g["d"])[ok], o["d"][ok]), (sub, "sil d")
+            assert _same_bits(_np(gs.star_radius(_t(q), _t(rm), sub=sub)),
+                              os_.query(sub, oracle.KIND_STAR, q, rmax=rm)["d"]), (sub, "star")
+    v = _dirs(len(q), D, 12)
+    for rmax in (0.05, 0.3, 10.0):
+        rm = np.full(len(q), rmax, np.float32)
+        g = gs.intersect(_t(q), _t(v), _t(rm), sub=0)
+        o = os_.query(0, oracle.KIND_RAY, q, v, rm)
+        assert _same_bits(_np(g["prim"]), o["prim"]), ("ray prim", rmax)
+        hit = o["prim"] >= 0
+        for k in ("d", "p", "n", "tuv"):
+            ok_ = o[k][hit]
+            if k == "tuv" and D == 2:
+                ok_ = np.stack([o["tuv"][hit, 0], 0 * o["tuv"][hit, 0]], 1)
+            assert _same_bits(_np(g[k])[hit], ok_), ("ray", k, rmax)
+
+
+def _walk_parity(gpu, oracle, spec, n, W, nsteps, eps, solver_cls, **kw):
+    gs, os_ = _scenes(gpu, oracle, spec)
+    gs.boundary = gpu.BoundaryCondition(spec.bc_code)
+    q = spec.queries[:n]
+    sol = solver_cls(eps=eps, nwalks=W, nsteps=nsteps, **kw)
+    g = sol.solve(_t(q), gs, seed=7, per_walk=True, count_steps=True)
+    o = os_.solve_u(q, spec.bc_code, W, nsteps=nsteps, eps=eps, min_R=sol.min_R, cv=sol.control_variates,
+                    av=sol.antithetic_variates, seed=7, per_walk=True)
+    gu, gst, gtm = _np(g["walk_u"]), _np(g["walk_steps"]), _np(g["walk_term"])
+    same = (gu.view(np.uint32) == o["walk_u"].view(np.uint32)) & (gst == o["walk_steps"]) & (gtm == o["walk_term"])
+    print(f"walk parity {same.mean():.6f} ({(~same).sum()} of {same.size} differ); mean steps {gst.mean():.1f}")
+    assert same.mean() >= 0.999
+    assert int(_np(g["steps"])[0]) == int(gst.astype(np.int64).sum())
+    # per-point means: Kahan fp32 on the GPU vs fp64 accumulation on the CPU
+    assert np.allclose(_np(g["u"]), o["u"], rtol=2e-6, atol=2e-6)
+    return same.mean()
+
+
+def test_walk_u_parity_2d(gpu, oracle, star_spec):
+    _walk_parity(gpu, oracle, star_spec, 64, 64, 1024, 1e-3, gpu.WoSt)
+
+
+def test_walk_u_parity_2d_holes(gpu, oracle, holes_spec):
+    _walk_parity(gpu, oracle, holes_spec, 32, 64, 1024, 1e-4, gpu.WoSt)
+
+
+def test_walk_u_parity_2d_no_cv(gpu, oracle, star_spec):
+    _walk_parity(gpu, oracle, star_spec, 32, 32, 1024, 1e-3, gpu.WoSt, control_variates=False)
+
+
+def test_walk_u_parity_3d(gpu, oracle, ico_spec):
+    _walk_parity(gpu, oracle, ico_spec, 32, 32, 1024, 1e-3, gpu.WoSt3D)
+
+
+@pytest.mark.parametrize("which", ["star", "ico"])
+def test_dude_parity(gpu, oracle, which, star_spec, ico_spec):
+    spec = dict(star=star_spec, ico=ico_spec)[which]
+    gs, os_ = _scenes(gpu, oracle, spec)
+    gs.boundary = gpu.BoundaryCondition(spec.bc_code)
+    n, W = 32, 32
+    q = spec.queries[:n]
+    e = _dirs(n, spec.dim, 13)
+    sol = (gpu.WoSt if spec.dim == 2 else gpu.WoSt3D)(eps=1e-3, nwalks=W, nsteps=1024)
+    g = _np(sol.solve_dude(_t(q), _t(e), gs, seed=3))
+    o = os_.solve_dude(q, e, spec.bc_code, W, nsteps=1024, eps=1e-3, cv=sol.control_variates,
+                       av=sol.antithetic_variates, seed=3)["dude"]
+    same = g.view(np.uint32) == o.view(np.uint32)
+    print(f"dude parity {same.mean():.4f}")
+    assert same.mean() >= 0.97
+
+
+@pytest.mark.parametrize("which", ["star", "holes", "ico"])
+def test_fused_parity(gpu, oracle, which, star_spec, holes_spec, ico_spec):
+    spec = dict(star=star_spec, holes=holes_spec, ico=ico_spec)[which]
+    gs, os_ = _scenes(gpu, oracle, spec)
+    gs.boundary = gpu.BoundaryCondition(spec.bc_code)
+    n, W = 48, 64
+    eps = spec.meta["eps"]
+    q = spec.queries[:n]
+    sol = (gpu.WoSt if spec.dim == 2 else gpu.WoSt3D)(eps=eps, nwalks=W, nsteps=1024)
+    g = sol.solve_grad(_t(q), gs, seed=5, per_pair=True, count_steps=True)
+    o = os_.solve_fused(q, spec.bc_code, W, nsteps=1024, eps=eps, seed=5, per_pair=True)
+    pv = _np(g["pair_vals"])
+    same = (pv[:, 0].view(np.uint32) == o["pair_u"].view(np.uint32)) & np.all(
+        pv[:, 1:].view(np.uint32) == o["pair_g"].view(np.uint32), axis=1) & (_np(g["pair_steps"]) == o["pair_steps"])
+    print(f"fused pair parity {same.mean():.6f}")
+    assert same.mean() >= 0.999
+    assert int(_np(g["steps"])[0]) == o["steps"]
+    assert np.allclose(_np(g["u"]), o["u"], rtol=1e-5, atol=1e-5)
+    assert np.allclose(_np(g["grad"]), o["grad"], rtol=1e-4, atol=1e-4)
+
+
+def test_solve_matches_analytic_full_size(gpu, star_spec):
+    """Full-size star scene (1024 points x 256 walks): per-point means within
+    3 sigma of the FP64 harmonic solution e^x cos y."""
+    from dwost_amd import scenes
+
+    gs = gpu.Scene2D(star_spec.vertices, star_spec.indices, star_spec.types, gpu.BoundaryCondition.exp_cos())
+    q = star_spec.queries
+    sol = gpu.WoSt(eps=1e-3, nwalks=256, nsteps=1024)
+    r = sol.solve(_t(q), gs, seed=0, per_walk=True)
+    wu = _np(r["walk_u"]).reshape(len(q), 256).astype(np.float64)
+    ua = scenes.analytic_u(scenes.BC_EXPCOS, q)
+    z = (wu.mean(1) - ua) / (wu.std(1) / 16.0 + 1e-12)
+    rmse = float(np.sqrt(np.mean((_np(r["u"]) - ua) ** 2)))
+    print(f"u rmse {rmse:.4f}; |z|<3 for {np.mean(np.abs(z) < 3):.4f}")
+    assert np.mean(np.abs(z) < 3) > 0.97
+    assert rmse < 0.2
+    g = sol.solve_grad(_t(q), gs, seed=1)
+    ga = scenes.analytic_grad(scenes.BC_EXPCOS, q)
+    R = _np(gs.distance(_t(q)))
+    far = R > 0.05
+    err = np.sqrt(np.mean((_np(g["grad"])[far] - ga[far]) ** 2))
+    print(f"grad rmse (R>0.05) {err:.4f}")
+    assert err < 1.0

@@ -83,6 +83,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="holes", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--points", type=int, default=0, help="use only the first N query points (0: all)")
+    ap.add_argument("--progress", action="store_true", help="print a line after each timed step")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every box of this build has a single MI355X "
@@ -101,8 +103,11 @@ def main():
     scene = Scene(spec.vertices, spec.indices, spec.types, dwost_amd.BoundaryCondition(spec.bc_code))
     Solver = dwost_amd.WoSt if spec.dim == 2 else dwost_amd.WoSt3D
     solver = Solver(eps=cfg["eps"], nwalks=cfg["nwalks"], nsteps=1024)
+    if args.points:
+        spec.queries = spec.queries[: args.points]
     q = torch.as_tensor(spec.queries).cuda()
     torch.cuda.synchronize()
+    print(f"[bench] scene ready: {args.config}, {len(q)} points", file=sys.stderr, flush=True)
 
     for w in range(args.warmup):
         solver.solve_grad(q, scene, seed=1000 + w)
@@ -118,6 +123,9 @@ def main():
         starts[k].record()
         outs.append(solver.solve_grad(q, scene, seed=k, count_steps=True))
         ends[k].record()
+        if args.progress:
+            ends[k].synchronize()
+            print(f"[bench] step {k} done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kernel_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]

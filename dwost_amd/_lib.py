@@ -39,7 +39,9 @@ class Boundary(C.Structure):
 class SolverParams(C.Structure):
     _fields_ = [("nwalks", C.c_int32), ("nsteps", C.c_int32), ("nsubwalks", C.c_int32),
                 ("control_variates", C.c_int32), ("antithetic_variates", C.c_int32), ("seed", C.c_uint32),
-                ("eps", C.c_float), ("min_R", C.c_float)]
+                ("eps", C.c_float), ("min_R", C.c_float), ("flags", C.c_int32)]
+
+SOLVE_EXACT_CONES = 1
 
 
 def lib():
@@ -47,9 +49,10 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB):
-        raise DwostError(f"{LIB} is missing: build it with `python -m dwost_amd.build` (no CPU fallback exists)")
-    L = C.CDLL(LIB)
+    path = os.environ.get("DWOST_LIB", LIB)
+    if not os.path.exists(path):
+        raise DwostError(f"{path} is missing: build it with `python -m dwost_amd.build` (no CPU fallback exists)")
+    L = C.CDLL(path)
     vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
     L.dw_last_error.restype = C.c_char_p
     L.dw_version.restype = C.c_int

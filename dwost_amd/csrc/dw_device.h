@@ -66,6 +66,16 @@ DWD F3 vabs(F3 a) { return f3(__builtin_fabsf(a.x), __builtin_fabsf(a.y), __buil
 DWD bool bad(float v) { return __builtin_isnan(v) || __builtin_isinf(v); }
 
 // fp64-evaluated, once-rounded transcendentals
+#ifdef DW_FP32_TRANSCENDENTALS  // timing experiments only: breaks bit parity with the oracle
+DWD float cr_log(float x) { return logf(x); }
+DWD float cr_exp(float x) { return expf(x); }
+DWD float cr_sin(float x) { return sinf(x); }
+DWD float cr_cos(float x) { return cosf(x); }
+DWD float cr_acos(float x) { return acosf(x); }
+DWD float cr_asin(float x) { return asinf(x); }
+DWD float cr_atan2(float y, float x) { return atan2f(y, x); }
+DWD float cr_cbrt(float x) { return cbrtf(x); }
+#else
 DWD float cr_log(float x) { return (float)log((double)x); }
 DWD float cr_exp(float x) { return (float)exp((double)x); }
 DWD float cr_sin(float x) { return (float)sin((double)x); }
@@ -74,6 +84,7 @@ DWD float cr_acos(float x) { return (float)acos((double)x); }
 DWD float cr_asin(float x) { return (float)asin((double)x); }
 DWD float cr_atan2(float y, float x) { return (float)atan2((double)y, (double)x); }
 DWD float cr_cbrt(float x) { return (float)cbrt((double)x); }
+#endif
 
 // ---------------------------------------------------------------------------
 // TEA seeding (diff_wost/core/math.py:137-157) and PCG32 (Dr.Jit PCG32)
@@ -384,7 +395,49 @@ DWD bool snch_overlaps(V mn, V mx, V axis, float ha, float radius, V x, float r_
     return ov;
 }
 
+// The same decision without transcendentals.  With c = |axis . vca| = |cos dax|
+// and theta < pi/2:  |dax - pi/2| <= theta  <=>  c <= sin(theta);
+// ha + vch > pi/2  <=>  cos(ha + vch) < 0.  sin/cos of the view-cone half angle
+// come from radius/len (asin branch) or (d, s)/hypot(d, s) (atan2 branch).
+// kConeMargin makes every comparison conservative: a node the literal test
+// visits is always visited (its fp32 rounding error is below 1e-6 in cosine
+// space), so results differ only where the literal test's own rounding
+// prunes a node that does hold the nearest silhouette.
+constexpr float kConeMargin = 4e-6f;
+template <typename V>
+DWD bool snch_overlaps_fast(V mn, V mx, V axis, float ha, float sin_ha, float cos_ha, float radius, V x, float r_max,
+                            float& d_min) {
+    float d_max;
+    box_dist(mn, mx, x, d_min, d_max);
+    bool ov = d_min <= r_max;
+    if (!((ha >= 0.0f) && ov)) return ov;
+    if (ha > kPi2 - kConeMargin) return true;
+    V cen = (mn + mx) / 2.0f;
+    V vca = cen - x;
+    float len = norm(vca);
+    vca = vca / len;
+    float c = __builtin_fabsf(clampf(dot(axis, vca), -1.0f, 1.0f));
+    if (c <= sin_ha + kConeMargin) return true;
+    float sv, cv;
+    if (len > radius) {
+        sv = radius / len;
+        cv = __builtin_sqrtf(fmx(0.0f, 1.0f - sv * sv));
+    } else {
+        V e = mx - cen;
+        float d = dot(e, vabs(vca));
+        float s = len - d;
+        if (s < 0.0f) return true;
+        d = project_to_plane(vca, e);
+        float h = __builtin_sqrtf(d * d + s * s);
+        sv = d / h;
+        cv = s / h;
+    }
+    if (cos_ha * cv - sin_ha * sv < kConeMargin) return true;
+    return c <= (sin_ha * cv + cos_ha * sv) + kConeMargin;
+}
+
 // SNCH.closest_silhouette 2D (snch.py:312-378 + silhouette_vertex.py:80-93)
+template <bool X = true>
 DWD bool closest_sil2(const SubDev& S, F2 x, float r_max, float& d_out, int& ref_out, F2& p_out) {
     const DNode2* nodes = (const DNode2*)S.nodes;
     const DCone2* cones = (const DCone2*)S.cones;
@@ -427,8 +480,16 @@ DWD bool closest_sil2(const SubDev& S, F2 x, float r_max, float& d_out, int& ref
             DNode2 nl = nodes[L], nr = nodes[R];
             DCone2 cl = cones[L], cr = cones[R];
             float d0, d1;
-            bool h0 = snch_overlaps(nmin(nl), nmax(nl), f2(cl.ax, cl.ay), cl.half_angle, cl.radius, x, r_max, d0);
-            bool h1 = snch_overlaps(nmin(nr), nmax(nr), f2(cr.ax, cr.ay), cr.half_angle, cr.radius, x, r_max, d1);
+            bool h0, h1;
+            if constexpr (X) {
+                h0 = snch_overlaps(nmin(nl), nmax(nl), f2(cl.ax, cl.ay), cl.half_angle, cl.radius, x, r_max, d0);
+                h1 = snch_overlaps(nmin(nr), nmax(nr), f2(cr.ax, cr.ay), cr.half_angle, cr.radius, x, r_max, d1);
+            } else {
+                h0 = snch_overlaps_fast(nmin(nl), nmax(nl), f2(cl.ax, cl.ay), cl.half_angle, cl.sin_ha, cl.cos_ha,
+                                        cl.radius, x, r_max, d0);
+                h1 = snch_overlaps_fast(nmin(nr), nmax(nr), f2(cr.ax, cr.ay), cr.half_angle, cr.sin_ha, cr.cos_ha,
+                                        cr.radius, x, r_max, d1);
+            }
             if (h0 && h1) {
                 int closer = L, other = R;
                 if (d1 < d0) {
@@ -454,6 +515,7 @@ DWD bool closest_sil2(const SubDev& S, F2 x, float r_max, float& d_out, int& ref
 }
 
 // SNCH3D.closest_silhouette (snch3d.py:594-663 + silhouette_edge.py:83-97)
+template <bool X = true>
 DWD bool closest_sil3(const SubDev& S, F3 x, float r_max, float& d_out, int& ref_out, F3& p_out) {
     const DNode3* nodes = (const DNode3*)S.nodes;
     const DCone3* cones = (const DCone3*)S.cones;
@@ -508,10 +570,18 @@ DWD bool closest_sil3(const SubDev& S, F3 x, float r_max, float& d_out, int& ref
             DNode3 nl = nodes[L], nr = nodes[R];
             DCone3 cl = cones[L], cr = cones[R];
             float d0, d1;
-            bool h0 =
-                snch_overlaps(nmin(nl), nmax(nl), f3(cl.ax, cl.ay, cl.az), cl.half_angle, cl.radius, x, r_max, d0);
-            bool h1 =
-                snch_overlaps(nmin(nr), nmax(nr), f3(cr.ax, cr.ay, cr.az), cr.half_angle, cr.radius, x, r_max, d1);
+            bool h0, h1;
+            if constexpr (X) {
+                h0 = snch_overlaps(nmin(nl), nmax(nl), f3(cl.ax, cl.ay, cl.az), cl.half_angle, cl.radius, x, r_max,
+                                   d0);
+                h1 = snch_overlaps(nmin(nr), nmax(nr), f3(cr.ax, cr.ay, cr.az), cr.half_angle, cr.radius, x, r_max,
+                                   d1);
+            } else {
+                h0 = snch_overlaps_fast(nmin(nl), nmax(nl), f3(cl.ax, cl.ay, cl.az), cl.half_angle, cl.sin_ha,
+                                        cl.cos_ha, cl.radius, x, r_max, d0);
+                h1 = snch_overlaps_fast(nmin(nr), nmax(nr), f3(cr.ax, cr.ay, cr.az), cr.half_angle, cr.sin_ha,
+                                        cr.cos_ha, cr.radius, x, r_max, d1);
+            }
             if (h0 && h1) {
                 int closer = L, other = R;
                 if (d1 < d0) {
@@ -537,6 +607,7 @@ DWD bool closest_sil3(const SubDev& S, F3 x, float r_max, float& d_out, int& ref
 }
 
 // Polyline.star_radius (polyline.py:304-314) / Mesh.star_radius (mesh.py:337-346)
+template <bool X = true>
 DWD float star_radius2(const SubDev& S, F2 x, float r_max) {
     F2 m = f2(fmx(__builtin_fabsf(x.x - S.bb_min[0]), __builtin_fabsf(x.x - S.bb_max[0])),
               fmx(__builtin_fabsf(x.y - S.bb_min[1]), __builtin_fabsf(x.y - S.bb_max[1])));
@@ -545,8 +616,9 @@ DWD float star_radius2(const SubDev& S, F2 x, float r_max) {
     float d;
     int ref;
     F2 p;
-    return closest_sil2(S, x, R, d, ref, p) ? d : R;
+    return closest_sil2<X>(S, x, R, d, ref, p) ? d : R;
 }
+template <bool X = true>
 DWD float star_radius3(const SubDev& S, F3 x, float r_max) {
     F3 m = f3(fmx(__builtin_fabsf(x.x - S.bb_min[0]), __builtin_fabsf(x.x - S.bb_max[0])),
               fmx(__builtin_fabsf(x.y - S.bb_min[1]), __builtin_fabsf(x.y - S.bb_max[1])),
@@ -556,7 +628,7 @@ DWD float star_radius3(const SubDev& S, F3 x, float r_max) {
     float d;
     int ref;
     F3 p;
-    return closest_sil3(S, x, R, d, ref, p) ? d : R;
+    return closest_sil3<X>(S, x, R, d, ref, p) ? d : R;
 }
 
 // ---------------------------------------------------------------------------

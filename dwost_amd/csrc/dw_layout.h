@@ -18,18 +18,22 @@ struct alignas(16) DNode2 {  // 32 B
     int nref;  // > 0 for leaves
     int sil_off, nsil;
 };
-struct alignas(16) DCone2 {  // 16 B
+struct alignas(16) DCone2 {  // 32 B
     float ax, ay, half_angle, radius;
+    float sin_ha, cos_ha;  // of half_angle (transcendental-free cone test)
+    float pad0, pad1;
 };
 struct alignas(16) DNode3 {  // 32 B
     float mnx, mny, mnz, mxx;
     float mxy, mxz;
     int off, nref;
 };
-struct alignas(16) DCone3 {  // 32 B
+struct alignas(16) DCone3 {  // 48 B
     float ax, ay, az, half_angle;
     float radius;
-    int sil_off, nsil, pad;
+    int sil_off, nsil;
+    float sin_ha;
+    float cos_ha, pad0, pad1, pad2;
 };
 struct alignas(16) DSeg {  // 2D primitive in BVH order, 16 B (+ index stream)
     float ax, ay, bx, by;
@@ -96,4 +100,5 @@ struct ParamsDev {
     int nwalks, nsteps, nsubwalks, cv, av;
     uint32_t seed;
     float eps, min_R;
+    int flags;  // DW_SOLVE_*
 };

@@ -149,157 +149,220 @@ DWD bool inside3(const SceneDev& sc, F3 x) {
 }
 
 // --------------------------------------------------------------------------
-// WoSt.u (wost.py:145-221)
+// WoSt.u (wost.py:145-221) / WoSt3D.u (wost_3d.py:43-106) as a step machine:
+// walk_init, then walk_step until it returns false, then walk_value.
+// Two exact shortcuts against the literal loop: the terminating step reuses the
+// Dirichlet closest-point record (the reference queries the same point twice,
+// wost.py:171,183) and skips the star radius, which has no side effects there.
 // --------------------------------------------------------------------------
-DWD float walk_u2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, F2 x, Rng& rng, int& steps, int& term) {
-    const SubDev& F = sc.sub[0];
-    const SubDev& Dn = sc.sub[1];
-    const SubDev& Nm = sc.sub[2];
-    F2 ip = x, in = f2(0.0f, 0.0f);
-    bool ob = false;
-    float result = 0.0f;
-    int i = 0;
-    bool active = true;
-    term = -1;
-    while ((i < P.nsteps) && active) {
-        float R = kInf, d_dir = kInf;
-        if (Dn.present) R = d_dir = __builtin_fabsf(closest2(Dn, ip).d);
-        if (Nm.present) {
-            F2 p = ob ? ip - in * 1e-5f : ip;
-            float sr = star_radius2(Nm, p, d_dir) * 0.98f;
-            sr = fmx(sr, P.min_R);
-            R = fmn(R, sr);
-        }
-        if (d_dir < P.eps) {
-            CP cp = closest2(Dn, ip);
-            result += g2(bc, Dn, cp);
-            term = cp.prim;
-            active = false;
-        } else {
-            F2 ref = f2(0.0f, 0.0f);
-            if (Nm.present) {
-                float rnd, pmf;
-                int idx = cdf_sample(Nm, rng.next(), rnd, pmf);
-                float4 e = ((const float4*)Nm.local_prims)[idx];
-                F2 a = f2(e.x, e.y), b = f2(e.z, e.w);
-                F2 tg = b - a;
-                F2 bn = normalize(f2(tg.y, -tg.x));
-                F2 bp = lerp(a, b, rnd);
-                float bpdf = pmf / norm(tg);
-                CP cr = closest2(F, ip);  // boundary_control_variates (wost.py:123-126)
-                float hc = h2(bc, F, cr.prim, cr.t, f2(cr.p.x, cr.p.y), f2(cr.n.x, cr.n.y));
-                ref = f2(hc * cr.n.x, hc * cr.n.y);
-                float alpha = ob ? 2.0f : 1.0f;  // handle_boundary (wost.py:104-121)
-                float hb = 0.0f;
-                float G = clampf(greens2_G(ip, R, ip, bp), -10.0f, 10.0f);
-                if (norm(ip - bp) <= R) {
-                    float h = h2(bc, Nm, idx, rnd, bp, bn);
-                    if (P.cv) h = h - dot(ref, bn);
-                    hb = 0.0f - G * h / bpdf;
-                }
-                result += alpha * hb;
-            }
-            F2 dir = uniform_dir2(rng);
-            if (ob && (dot(dir, in) > 0.0f)) dir = -dir;
-            F2 _p = ip;
-            F2 o = ob ? ip - in * 1e-5f : ip;  // Polyline.intersect (polyline.py:435-466)
-            Hit its = intersect2(F, o, dir, R);
-            if (its.valid) {
-                ip = f2(its.p.x, its.p.y);
-                in = f2(its.n.x, its.n.y);
-                ob = true;
-            } else {
-                ip = o + dir * R;
-                in = dir;
-                its.d = R;
-                ob = false;
-            }
-            if (P.cv) result += dot(_p - ip, ref);
-            F2 sp;  // handle_source (wost.py:128-143)
-            float sr_, spdf;
-            greens2_sample(_p, R, dir, rng, sp, sr_, spdf);
-            float src = 0.0f;
-            if (norm(sp - _p) < its.d) src += (R * R / 4.0f) * src_f(bc);
-            result += src;
-        }
-        i += 1;
-    }
-    if (bad(result) || __builtin_fabsf(result) > 100.0f) result = 0.0f;
-    steps = i;
-    return result;
+template <int D>
+struct VecT;
+template <>
+struct VecT<2> {
+    using T = F2;
+};
+template <>
+struct VecT<3> {
+    using T = F3;
+};
+
+template <int D>
+struct Walk {
+    typename VecT<D>::T ip, in;
+    float result;
+    int i;
+    int term;
+    bool ob, live;
+};
+
+DWD void set_zero(F2& v) { v = f2(0.0f, 0.0f); }
+DWD void set_zero(F3& v) { v = f3(0.0f, 0.0f, 0.0f); }
+
+template <int D>
+DWD void walk_init(Walk<D>& w, typename VecT<D>::T x) {
+    w.ip = x;
+    set_zero(w.in);
+    w.ob = false;
+    w.result = 0.0f;
+    w.i = 0;
+    w.term = -1;
+    w.live = true;
 }
 
-// WoSt3D.u (wost_3d.py:43-106)
-DWD float walk_u3(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, F3 x, Rng& rng, int& steps, int& term) {
+template <bool X>
+DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk<2>& w, Rng& rng) {
     const SubDev& F = sc.sub[0];
     const SubDev& Dn = sc.sub[1];
     const SubDev& Nm = sc.sub[2];
-    F3 ip = x, in = f3(0.0f, 0.0f, 0.0f);
-    bool ob = false;
-    float result = 0.0f;
-    int i = 0;
-    bool active = true;
-    term = -1;
-    while ((i < P.nsteps) && active) {
-        float R = kInf, d_dir = kInf;
-        if (Dn.present) R = d_dir = __builtin_fabsf(closest3(Dn, ip).d);
-        F3 p = ob ? ip - in * 1e-3f : ip;
-        float sr = star_radius3(F, p, d_dir) * 0.98f;
+    if (!(w.i < P.nsteps)) {
+        w.live = false;
+        return false;
+    }
+    float R = kInf, d_dir = kInf;
+    CP cpd;
+    if (Dn.present) {
+        cpd = closest2(Dn, w.ip);
+        R = d_dir = __builtin_fabsf(cpd.d);
+    }
+    if (d_dir < P.eps) {
+        w.result += g2(bc, Dn, cpd);
+        w.term = cpd.prim;
+        w.i += 1;
+        w.live = false;
+        return false;
+    }
+    if (Nm.present) {
+        F2 p = w.ob ? w.ip - w.in * 1e-5f : w.ip;
+        float sr = star_radius2<X>(Nm, p, d_dir) * 0.98f;
         sr = fmx(sr, P.min_R);
         R = fmn(R, sr);
-        if (d_dir < P.eps) {
-            CP cp = closest3(Dn, ip);
-            result += g3(bc, Dn, cp);
-            term = cp.prim;
-            active = false;
-        } else {
-            if (Nm.present) {  // Mesh.sample_boundary (mesh.py:474-490)
-                float rnd, pmf;
-                int idx = cdf_sample(Nm, rng.next(), rnd, pmf);
-                const float* t = Nm.local_prims + 9 * idx;
-                F3 a = f3(t[0], t[1], t[2]), b = f3(t[3], t[4], t[5]), c = f3(t[6], t[7], t[8]);
-                F3 cr = cross(b - a, c - a);
-                float area = norm(cr) / 2.0f;
-                float r1 = rng.next();
-                float r2 = rng.next();
-                float su = __builtin_sqrtf(r1) * (1.0f - r2);
-                float sv = __builtin_sqrtf(r1) * r2;
-                F3 bp = (b * su + c * sv) + a * ((1.0f - su) - sv);
-                F3 bn = normalize(cr);
-                float bpdf = pmf / area;
-                float G = clampf(greens3_G(ip, R, ip, bp), -100.0f, 100.0f);
-                float alpha = ob ? 2.0f : 1.0f;
-                if (norm(ip - bp) <= R) {
-                    float h = h3(bc, Nm, idx, su, sv, bp, bn);
-                    result -= G * h * alpha / bpdf;
-                }
-            }
-            F3 dir = uniform_sphere(rng);
-            if (ob && (dot(dir, in) > 0.0f)) dir = -dir;
-            F3 _p = ip;
-            F3 o = ob ? ip - in * 1e-5f : ip;  // Mesh.intersect (mesh.py:383-415)
-            Hit its = intersect3(F, o, dir, R);
-            if (its.valid) {
-                ip = its.p;
-                in = its.n;
-                ob = true;
-            } else {
-                ip = o + dir * R;
-                in = dir;
-                its.d = R;
-                ob = false;
-            }
-            F3 sp;
-            float sr_, spdf;
-            greens3_sample(_p, R, dir, rng, sp, sr_, spdf);
-            float src = 0.0f;
-            if (norm(sp - _p) < its.d) src += (R * R / 6.0f) * src_f(bc);
-            result += src;
-        }
-        i += 1;
     }
-    steps = i;
-    return result;
+    F2 ref = f2(0.0f, 0.0f);
+    if (Nm.present) {
+        float rnd, pmf;
+        int idx = cdf_sample(Nm, rng.next(), rnd, pmf);
+        float4 e = ((const float4*)Nm.local_prims)[idx];
+        F2 a = f2(e.x, e.y), b = f2(e.z, e.w);
+        F2 tg = b - a;
+        F2 bn = normalize(f2(tg.y, -tg.x));
+        F2 bp = lerp(a, b, rnd);
+        float bpdf = pmf / norm(tg);
+        CP cr = closest2(F, w.ip);  // boundary_control_variates (wost.py:123-126)
+        float hc = h2(bc, F, cr.prim, cr.t, f2(cr.p.x, cr.p.y), f2(cr.n.x, cr.n.y));
+        ref = f2(hc * cr.n.x, hc * cr.n.y);
+        float alpha = w.ob ? 2.0f : 1.0f;  // handle_boundary (wost.py:104-121)
+        float hb = 0.0f;
+        float G = clampf(greens2_G(w.ip, R, w.ip, bp), -10.0f, 10.0f);
+        if (norm(w.ip - bp) <= R) {
+            float h = h2(bc, Nm, idx, rnd, bp, bn);
+            if (P.cv) h = h - dot(ref, bn);
+            hb = 0.0f - G * h / bpdf;
+        }
+        w.result += alpha * hb;
+    }
+    F2 dir = uniform_dir2(rng);
+    if (w.ob && (dot(dir, w.in) > 0.0f)) dir = -dir;
+    F2 _p = w.ip;
+    F2 o = w.ob ? w.ip - w.in * 1e-5f : w.ip;  // Polyline.intersect (polyline.py:435-466)
+    Hit its = intersect2(F, o, dir, R);
+    if (its.valid) {
+        w.ip = f2(its.p.x, its.p.y);
+        w.in = f2(its.n.x, its.n.y);
+        w.ob = true;
+    } else {
+        w.ip = o + dir * R;
+        w.in = dir;
+        its.d = R;
+        w.ob = false;
+    }
+    if (P.cv) w.result += dot(_p - w.ip, ref);
+    F2 sp;  // handle_source (wost.py:128-143)
+    float sr_, spdf;
+    greens2_sample(_p, R, dir, rng, sp, sr_, spdf);
+    float src = 0.0f;
+    if (norm(sp - _p) < its.d) src += (R * R / 4.0f) * src_f(bc);
+    w.result += src;
+    w.i += 1;
+    if (!(w.i < P.nsteps)) w.live = false;
+    return w.live;
+}
+
+template <bool X>
+DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk<3>& w, Rng& rng) {
+    const SubDev& F = sc.sub[0];
+    const SubDev& Dn = sc.sub[1];
+    const SubDev& Nm = sc.sub[2];
+    if (!(w.i < P.nsteps)) {
+        w.live = false;
+        return false;
+    }
+    float R = kInf, d_dir = kInf;
+    CP cpd;
+    if (Dn.present) {
+        cpd = closest3(Dn, w.ip);
+        R = d_dir = __builtin_fabsf(cpd.d);
+    }
+    if (d_dir < P.eps) {
+        w.result += g3(bc, Dn, cpd);
+        w.term = cpd.prim;
+        w.i += 1;
+        w.live = false;
+        return false;
+    }
+    F3 p = w.ob ? w.ip - w.in * 1e-3f : w.ip;
+    float sr = star_radius3<X>(F, p, d_dir) * 0.98f;
+    sr = fmx(sr, P.min_R);
+    R = fmn(R, sr);
+    if (Nm.present) {  // Mesh.sample_boundary (mesh.py:474-490)
+        float rnd, pmf;
+        int idx = cdf_sample(Nm, rng.next(), rnd, pmf);
+        const float* t = Nm.local_prims + 9 * idx;
+        F3 a = f3(t[0], t[1], t[2]), b = f3(t[3], t[4], t[5]), c = f3(t[6], t[7], t[8]);
+        F3 cr = cross(b - a, c - a);
+        float area = norm(cr) / 2.0f;
+        float r1 = rng.next();
+        float r2 = rng.next();
+        float su = __builtin_sqrtf(r1) * (1.0f - r2);
+        float sv = __builtin_sqrtf(r1) * r2;
+        F3 bp = (b * su + c * sv) + a * ((1.0f - su) - sv);
+        F3 bn = normalize(cr);
+        float bpdf = pmf / area;
+        float G = clampf(greens3_G(w.ip, R, w.ip, bp), -100.0f, 100.0f);
+        float alpha = w.ob ? 2.0f : 1.0f;
+        if (norm(w.ip - bp) <= R) {
+            float h = h3(bc, Nm, idx, su, sv, bp, bn);
+            w.result -= G * h * alpha / bpdf;
+        }
+    }
+    F3 dir = uniform_sphere(rng);
+    if (w.ob && (dot(dir, w.in) > 0.0f)) dir = -dir;
+    F3 _p = w.ip;
+    F3 o = w.ob ? w.ip - w.in * 1e-5f : w.ip;  // Mesh.intersect (mesh.py:383-415)
+    Hit its = intersect3(F, o, dir, R);
+    if (its.valid) {
+        w.ip = its.p;
+        w.in = its.n;
+        w.ob = true;
+    } else {
+        w.ip = o + dir * R;
+        w.in = dir;
+        its.d = R;
+        w.ob = false;
+    }
+    F3 sp;
+    float sr_, spdf;
+    greens3_sample(_p, R, dir, rng, sp, sr_, spdf);
+    float src = 0.0f;
+    if (norm(sp - _p) < its.d) src += (R * R / 6.0f) * src_f(bc);
+    w.result += src;
+    w.i += 1;
+    if (!(w.i < P.nsteps)) w.live = false;
+    return w.live;
+}
+
+// final value of a finished walk: 2D guard (wost.py:218-219); 3D has none
+DWD float walk_value(const Walk<2>& w) {
+    float r = w.result;
+    return (bad(r) || __builtin_fabsf(r) > 100.0f) ? 0.0f : r;
+}
+DWD float walk_value(const Walk<3>& w) { return w.result; }
+
+template <int D, bool X = true>
+DWD float walk_u(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, typename VecT<D>::T x, Rng& rng,
+                 int& steps, int& term) {
+    Walk<D> w;
+    walk_init<D>(w, x);
+    while (walk_step<X>(sc, bc, P, w, rng)) {
+    }
+    steps = w.i;
+    term = w.term;
+    return walk_value(w);
+}
+DWD float walk_u2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, F2 x, Rng& rng, int& steps, int& term) {
+    return walk_u<2>(sc, bc, P, x, rng, steps, term);
+}
+DWD float walk_u3(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, F3 x, Rng& rng, int& steps, int& term) {
+    return walk_u<3>(sc, bc, P, x, rng, steps, term);
 }
 
 }  // namespace dwd

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -20,6 +21,15 @@ using namespace dwd;
 namespace {
 
 constexpr int kBlock = 256;
+// Minimum resident waves per SIMD for the persistent walk kernels: the
+// literal-cone kernels keep their fp64 acos/asin/atan2 without spilling at 2;
+// the transcendental-free ones run best at 4 (a few VGPRs spill).
+#ifndef DW_PERS_WAVES_FAST
+#define DW_PERS_WAVES_FAST 4
+#endif
+#ifndef DW_PERS_WAVES_EXACT
+#define DW_PERS_WAVES_EXACT 2
+#endif
 
 __global__ void __launch_bounds__(kBlock) k_closest(SceneDev sc, int sub, const float* __restrict__ pts, int64_t n,
                                                     int* prim, float* dist, float* point, float* normal, float* tuv) {
@@ -183,6 +193,179 @@ __global__ void __launch_bounds__(kBlock) k_reduce_mean(const float* __restrict_
         for (int k = lane; k < W; k += 64) acc += (unsigned long long)steps[base + k];
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
         if (lane == 0) atomicAdd(step_total, acc);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent walk kernels with lane refill.  Each lane owns one job (a walk,
+// or an antithetic pair of walks) as a step machine; whenever lanes fall idle
+// the wave takes that many new job indices with one atomic and hands them out
+// by lane rank, so a wave is never held by its longest walk.  A job's result
+// depends only on its index (its PCG32 stream), not on the lane that runs it.
+// ---------------------------------------------------------------------------
+DWD uint32_t lane_rank(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Hands new job ids to the idle lanes of the wave; returns the id or -1 (no work left).
+DWD int64_t refill(bool idle, unsigned long long* counter, int64_t total) {
+    uint64_t m = __ballot(idle);
+    if (m == 0) return -1;
+    int leader = __ffsll((long long)m) - 1;
+    unsigned long long base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (!idle) return -1;
+    unsigned long long mine = base + lane_rank(m);
+    return mine < (unsigned long long)total ? (int64_t)mine : -2;
+}
+
+template <int D>
+DWD typename VecT<D>::T load_pt(const float* pts, int64_t i);
+template <>
+DWD F2 load_pt<2>(const float* pts, int64_t i) {
+    return f2(pts[2 * i], pts[2 * i + 1]);
+}
+template <>
+DWD F3 load_pt<3>(const float* pts, int64_t i) {
+    return f3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+}
+
+// Solver.solve jobs: job = i*W + k, nsubwalks walks on the job's stream.
+template <int D, bool X>
+__global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVES_FAST) k_walk_u_pers(const SceneDev* __restrict__ scp, BCDev bc, ParamsDev P,
+                                                        const float* __restrict__ pts, int64_t n, float* walk_u,
+                                                        int* walk_steps, int* walk_term,
+                                                        unsigned long long* counter) {
+    const SceneDev& sc = *scp;
+    using V = typename VecT<D>::T;
+    const int64_t total = n * (int64_t)P.nwalks;
+    int64_t job = -1;
+    bool done = false;
+    Walk<D> w;
+    Rng rng;
+    V x;
+    float res = 0.0f;
+    int steps = 0, sub = 0;
+    while (true) {
+        int64_t got = refill(!done && job < 0, counter, total);
+        if (got == -2) done = true;
+        if (got >= 0) {
+            job = got;
+            int64_t i = job / P.nwalks;
+            uint32_t a, b;
+            tea4(P.seed, (uint32_t)i, a, b);
+            rng.seed_lane(a, b, (uint64_t)job);
+            x = load_pt<D>(pts, i);
+            walk_init<D>(w, x);
+            res = 0.0f;
+            steps = 0;
+            sub = 0;
+        }
+        if (__ballot(!done) == 0) break;
+        if (job >= 0 && !walk_step<X>(sc, bc, P, w, rng)) {
+            res += walk_value(w);
+            steps += w.i;
+            if (++sub < P.nsubwalks) {
+                walk_init<D>(w, x);
+            } else {
+                walk_u[job] = res / (float)P.nsubwalks;
+                if (walk_steps) walk_steps[job] = steps;
+                if (walk_term) walk_term[job] = w.term;
+                job = -1;
+            }
+        }
+    }
+}
+
+// Fused jobs: job = i*npairs + k, the antithetic pair of WoSt.dude's first ball.
+template <int D, bool X>
+__global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVES_FAST) k_fused_pers(const SceneDev* __restrict__ scp, BCDev bc, ParamsDev P,
+                                                       const float* __restrict__ pts, const float* __restrict__ Rp,
+                                                       const int* __restrict__ insidep, int64_t n, int npairs,
+                                                       float* pair_vals, int* pair_steps,
+                                                       unsigned long long* counter) {
+    const SceneDev& sc = *scp;
+    using V = typename VecT<D>::T;
+    const int64_t total = n * (int64_t)npairs;
+    const float fsrc = src_f(bc);
+    int64_t job = -1;
+    bool done = false;
+    Walk<D> w;
+    Rng rng;
+    V p, dir, gs;
+    float R = 0.0f, up = 0.0f;
+    uint64_t saved = 0;
+    int phase = 0, steps = 0;
+    while (true) {
+        int64_t got = refill(!done && job < 0, counter, total);
+        if (got == -2) done = true;
+        if (got >= 0) {
+            int64_t i = got / npairs;
+            if (!insidep[i]) {
+                float* o = pair_vals + got * (D + 1);
+                for (int q = 0; q <= D; ++q) o[q] = 0.0f;
+                pair_steps[got] = 0;
+            } else {
+                job = got;
+                R = Rp[i];
+                p = load_pt<D>(pts, i);
+                uint32_t a, b;
+                tea4(P.seed, (uint32_t)i, a, b);
+                rng.seed_lane(a, b, (uint64_t)job);
+                V p_src;
+                float r_src, pdf_src;
+                if constexpr (D == 2) {
+                    dir = uniform_dir2(rng);
+                    F2 dir_src = uniform_dir2(rng);
+                    greens2_sample(p, R, dir_src, rng, p_src, r_src, pdf_src);
+                    gs = greens2_gradient(p, R, r_src, p_src) / (pdf_src * (R * R / 4.0f));
+                } else {
+                    dir = uniform_sphere(rng);
+                    greens3_sample(p, R, dir, rng, p_src, r_src, pdf_src);
+                    gs = greens3_gradient(p, R, r_src, p_src) / (pdf_src * (R * R / 6.0f));
+                }
+                saved = rng.state;
+                walk_init<D>(w, p + dir * R);
+                phase = 0;
+                steps = 0;
+            }
+        }
+        if (__ballot(!done) == 0) break;
+        if (job >= 0 && !walk_step<X>(sc, bc, P, w, rng)) {
+            float v = walk_value(w);
+            steps += w.i;
+            if (phase == 0) {
+                up = v;
+                rng.state = saved;
+                walk_init<D>(w, p - dir * R);
+                phase = 1;
+            } else {
+                float um = v;
+                float* o = pair_vals + job * (D + 1);
+                if constexpr (D == 2) {
+                    float s = (R * R / 4.0f) * fsrc;
+                    float k = 2.0f / R * (up - um);
+                    float g0 = k * dir.x + (s - s) * gs.x, g1 = k * dir.y + (s - s) * gs.y;
+                    o[0] = (up + um) + (s + s);
+                    o[1] = bad(g0) ? 0.0f : g0;
+                    o[2] = bad(g1) ? 0.0f : g1;
+                } else {
+                    float u1 = (bad(up) || __builtin_fabsf(up) > 10000.0f) ? 0.0f : up;
+                    float u2 = (bad(um) || __builtin_fabsf(um) > 10000.0f) ? 0.0f : um;
+                    float s = (R * R / 6.0f) * fsrc;
+                    float k = 3.0f / R * (u1 - u2);
+                    float g0 = k * dir.x + (s - s) * gs.x, g1 = k * dir.y + (s - s) * gs.y,
+                          g2_ = k * dir.z + (s - s) * gs.z;
+                    o[0] = (u1 + u2) + (s + s);
+                    o[1] = bad(g0) ? 0.0f : g0;
+                    o[2] = bad(g1) ? 0.0f : g1;
+                    o[3] = bad(g2_) ? 0.0f : g2_;
+                }
+                pair_steps[job] = steps;
+                job = -1;
+            }
+        }
     }
 }
 
@@ -379,6 +562,7 @@ struct dw_scene {
     dwb::SceneHost host;
     SceneDev dev;
     std::vector<void*> allocs;
+    const SceneDev* dev_ptr = nullptr;  // device copy of `dev`
     int device = 0;
     bool host_only = false;
 };
@@ -432,7 +616,9 @@ int upload_sub(dw_scene* s, const dwb::SubHost& H, SubDev& S) {
         for (size_t i = 0; i < nn; ++i) {
             const float* b = &H.node_box[i * 4];
             nodes[i] = DNode2{b[0], b[1], b[2], b[3], H.node_off[i], H.node_nref[i], H.node_sil_off[i], H.node_nsil[i]};
-            cones[i] = DCone2{H.node_axis[2 * i], H.node_axis[2 * i + 1], H.node_half_angle[i], H.node_radius[i]};
+            float ha = H.node_half_angle[i];
+            cones[i] = DCone2{H.node_axis[2 * i], H.node_axis[2 * i + 1], ha, H.node_radius[i],
+                              (float)std::sin((double)ha), (float)std::cos((double)ha), 0.0f, 0.0f};
         }
         std::vector<DSeg> prims(np);
         for (size_t j = 0; j < np; ++j) {
@@ -460,8 +646,10 @@ int upload_sub(dw_scene* s, const dwb::SubHost& H, SubDev& S) {
         for (size_t i = 0; i < nn; ++i) {
             const float* b = &H.node_box[i * 6];
             nodes[i] = DNode3{b[0], b[1], b[2], b[3], b[4], b[5], H.node_off[i], H.node_nref[i]};
-            cones[i] = DCone3{H.node_axis[3 * i], H.node_axis[3 * i + 1], H.node_axis[3 * i + 2], H.node_half_angle[i],
-                              H.node_radius[i], H.node_sil_off[i], H.node_nsil[i], 0};
+            float ha = H.node_half_angle[i];
+            cones[i] = DCone3{H.node_axis[3 * i], H.node_axis[3 * i + 1], H.node_axis[3 * i + 2], ha,
+                              H.node_radius[i], H.node_sil_off[i], H.node_nsil[i], (float)std::sin((double)ha),
+                              (float)std::cos((double)ha), 0.0f, 0.0f, 0.0f};
         }
         std::vector<DTri> prims(np);
         for (size_t j = 0; j < np; ++j) {
@@ -554,6 +742,7 @@ ParamsDev to_params(const dw_solver_params* p) {
     P.seed = p->seed;
     P.eps = p->eps;
     P.min_R = p->min_R;
+    P.flags = p->flags;
     return P;
 }
 int check_params(const dw_solver_params* p) {
@@ -573,6 +762,20 @@ struct Workspace {
 };
 Workspace g_ws;
 
+// One device counter per device for the persistent kernels' job queue.
+unsigned long long* g_counter[64] = {nullptr};
+std::mutex g_counter_mu;
+
+int counter_slot(unsigned long long** out) {
+    int dev = 0;
+    DW_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(DW_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_counter_mu);
+    if (!g_counter[dev]) DW_HIP(hipMalloc((void**)&g_counter[dev], 256));
+    *out = g_counter[dev];
+    return DW_OK;
+}
+
 int workspace(size_t bytes, void** out) {
     int dev = 0;
     DW_HIP(hipGetDevice(&dev));
@@ -589,6 +792,25 @@ int workspace(size_t bytes, void** out) {
     }
     *out = g_ws.ptr;
     return DW_OK;
+}
+
+// Grid of a persistent kernel: as many resident blocks as the occupancy allows
+// (DWOST_BLOCKS_PER_CU overrides the per-CU count for tuning).
+template <typename K>
+int persistent_grid(K kernel, unsigned& blocks) {
+    int dev = 0, cus = 0, per_cu = 0;
+    DW_HIP(hipGetDevice(&dev));
+    DW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    DW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0));
+    if (const char* e = getenv("DWOST_BLOCKS_PER_CU")) per_cu = atoi(e);
+    if (per_cu < 1) per_cu = 1;
+    blocks = (unsigned)(per_cu * cus);
+    return DW_OK;
+}
+
+bool use_simple_kernels() {
+    const char* e = getenv("DWOST_KERNEL");
+    return e && strcmp(e, "simple") == 0;
 }
 
 }  // namespace
@@ -628,6 +850,17 @@ int dw_scene_create(const dw_scene_desc* desc, dw_scene** out) {
             dw_scene_destroy(s);
             return fail(rc, m);
         }
+    }
+    {
+        std::vector<SceneDev> one(1, s->dev);
+        const void* p = nullptr;
+        int rc = upload(s, one, &p);
+        if (rc) {
+            std::string m = g_err;
+            dw_scene_destroy(s);
+            return fail(rc, m);
+        }
+        s->dev_ptr = (const SceneDev*)p;
     }
     *out = s;
     return DW_OK;
@@ -753,8 +986,33 @@ int dw_solve_u(const dw_scene* s, const float* pts, int64_t n, const dw_solver_p
         if (walk_u) wv = walk_u;
     }
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_walk_u, dim3(blocks_for(lanes)), dim3(kBlock), 0, st, s->dev, to_bc(boundary),
-                       to_params(params), pts, n, wv, ws, walk_term);
+    if (use_simple_kernels()) {
+        hipLaunchKernelGGL(k_walk_u, dim3(blocks_for(lanes)), dim3(kBlock), 0, st, s->dev, to_bc(boundary),
+                           to_params(params), pts, n, wv, ws, walk_term);
+    } else {
+        unsigned long long* counter = nullptr;
+        const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
+        if ((rc = counter_slot(&counter))) return rc;
+        DW_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned long long), st));
+        unsigned blocks;
+        if (s->dev.dim == 2) {
+            if (exact) {
+                if ((rc = persistent_grid(k_walk_u_pers<2, true>, blocks))) return rc;
+            } else {
+                if ((rc = persistent_grid(k_walk_u_pers<2, false>, blocks))) return rc;
+            }
+            hipLaunchKernelGGL((exact ? k_walk_u_pers<2, true> : k_walk_u_pers<2, false>), dim3(blocks), dim3(kBlock), 0, st, s->dev_ptr, to_bc(boundary),
+                               to_params(params), pts, n, wv, ws, walk_term, counter);
+        } else {
+            if (exact) {
+                if ((rc = persistent_grid(k_walk_u_pers<3, true>, blocks))) return rc;
+            } else {
+                if ((rc = persistent_grid(k_walk_u_pers<3, false>, blocks))) return rc;
+            }
+            hipLaunchKernelGGL((exact ? k_walk_u_pers<3, true> : k_walk_u_pers<3, false>), dim3(blocks), dim3(kBlock), 0, st, s->dev_ptr, to_bc(boundary),
+                               to_params(params), pts, n, wv, ws, walk_term, counter);
+        }
+    }
     DW_HIP(hipGetLastError());
     if (total_steps) DW_HIP(hipMemsetAsync(total_steps, 0, sizeof(uint64_t), st));
     hipLaunchKernelGGL(k_reduce_mean, dim3(blocks_for(n * 64)), dim3(kBlock), 0, st, wv, n, params->nwalks, 1,
@@ -811,8 +1069,33 @@ int dw_solve_fused(const dw_scene* s, const float* pts, int64_t n, const dw_solv
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(n)), dim3(kBlock), 0, st, s->dev, pts, n, R, ins);
     DW_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_fused, dim3(blocks_for(lanes)), dim3(kBlock), 0, st, s->dev, to_bc(boundary),
-                       to_params(params), pts, R, ins, n, npairs, pv, ps);
+    if (use_simple_kernels()) {
+        hipLaunchKernelGGL(k_fused, dim3(blocks_for(lanes)), dim3(kBlock), 0, st, s->dev, to_bc(boundary),
+                           to_params(params), pts, R, ins, n, npairs, pv, ps);
+    } else {
+        unsigned long long* counter = nullptr;
+        const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
+        if ((rc = counter_slot(&counter))) return rc;
+        DW_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned long long), st));
+        unsigned blocks;
+        if (D == 2) {
+            if (exact) {
+                if ((rc = persistent_grid(k_fused_pers<2, true>, blocks))) return rc;
+            } else {
+                if ((rc = persistent_grid(k_fused_pers<2, false>, blocks))) return rc;
+            }
+            hipLaunchKernelGGL((exact ? k_fused_pers<2, true> : k_fused_pers<2, false>), dim3(blocks), dim3(kBlock), 0, st, s->dev_ptr, to_bc(boundary),
+                               to_params(params), pts, R, ins, n, npairs, pv, ps, counter);
+        } else {
+            if (exact) {
+                if ((rc = persistent_grid(k_fused_pers<3, true>, blocks))) return rc;
+            } else {
+                if ((rc = persistent_grid(k_fused_pers<3, false>, blocks))) return rc;
+            }
+            hipLaunchKernelGGL((exact ? k_fused_pers<3, true> : k_fused_pers<3, false>), dim3(blocks), dim3(kBlock), 0, st, s->dev_ptr, to_bc(boundary),
+                               to_params(params), pts, R, ins, n, npairs, pv, ps, counter);
+        }
+    }
     DW_HIP(hipGetLastError());
     if (total_steps) DW_HIP(hipMemsetAsync(total_steps, 0, sizeof(uint64_t), st));
     // u = sum(usum) / (2 npairs); grad = sum(g) / (2 npairs)

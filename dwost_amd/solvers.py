@@ -36,11 +36,15 @@ class Solver:
     antithetic_variates: bool = True
     min_R: float = 1e-3
     dim: int = 2
+    # True: evaluate the SNCH cone test literally (acos/asin/atan2, bit-for-bit
+    # the reference arithmetic).  False: the transcendental-free conservative
+    # form, identical except where the literal test's rounding drops a silhouette.
+    exact_cones: bool = False
 
     def _params(self, seed: int) -> _lib.SolverParams:
         return _lib.SolverParams(int(self.nwalks), int(self.nsteps), int(self.nsubwalks), int(self.control_variates),
                                  int(self.antithetic_variates), int(seed) & 0xFFFFFFFF, float(self.eps),
-                                 float(self.min_R))
+                                 float(self.min_R), _lib.SOLVE_EXACT_CONES if self.exact_cones else 0)
 
     def _check(self, scene: _Scene):
         if scene.dim != self.dim:

@@ -84,7 +84,14 @@ typedef struct dw_solver_params { /* Solver / WoSt fields (solver.py:30-34, wost
     uint32_t seed;
     float eps;
     float min_R;
+    int32_t flags;          /* DW_SOLVE_* bits                                      */
 } dw_solver_params;
+
+/* Evaluate the SNCH cone test literally (acos/asin/atan2 per child, snch.py:46-97)
+ * instead of the transcendental-free form |cos| <= sin(half angle + view angle)
+ * with a 4e-6 conservative margin; the two differ only where the reference's
+ * own rounding would prune a node holding the nearest silhouette. */
+#define DW_SOLVE_EXACT_CONES 1
 
 const char* dw_last_error(void);
 int dw_version(void);

@@ -89,17 +89,18 @@ def test_queries_bit_exact(gpu, oracle, which, star_spec, holes_spec, ico_spec):
             assert _same_bits(_np(g[k])[hit], ok_), ("ray", k, rmax)
 
 
-def _walk_parity(gpu, oracle, spec, n, W, nsteps, eps, solver_cls, **kw):
+def _walk_parity(gpu, oracle, spec, n, W, nsteps, eps, solver_cls, exact_cones=True, **kw):
     gs, os_ = _scenes(gpu, oracle, spec)
     gs.boundary = gpu.BoundaryCondition(spec.bc_code)
     q = spec.queries[:n]
-    sol = solver_cls(eps=eps, nwalks=W, nsteps=nsteps, **kw)
+    sol = solver_cls(eps=eps, nwalks=W, nsteps=nsteps, exact_cones=exact_cones, **kw)
     g = sol.solve(_t(q), gs, seed=7, per_walk=True, count_steps=True)
     o = os_.solve_u(q, spec.bc_code, W, nsteps=nsteps, eps=eps, min_R=sol.min_R, cv=sol.control_variates,
                     av=sol.antithetic_variates, seed=7, per_walk=True)
     gu, gst, gtm = _np(g["walk_u"]), _np(g["walk_steps"]), _np(g["walk_term"])
     same = (gu.view(np.uint32) == o["walk_u"].view(np.uint32)) & (gst == o["walk_steps"]) & (gtm == o["walk_term"])
-    print(f"walk parity {same.mean():.6f} ({(~same).sum()} of {same.size} differ); mean steps {gst.mean():.1f}")
+    print(f"walk parity (exact_cones={exact_cones}) {same.mean():.6f} ({(~same).sum()} of {same.size} differ); "
+          f"mean steps {gst.mean():.1f}")
     assert same.mean() >= 0.999
     assert int(_np(g["steps"])[0]) == int(gst.astype(np.int64).sum())
     # per-point means: Kahan fp32 on the GPU vs fp64 accumulation on the CPU
@@ -107,20 +108,23 @@ def _walk_parity(gpu, oracle, spec, n, W, nsteps, eps, solver_cls, **kw):
     return same.mean()
 
 
-def test_walk_u_parity_2d(gpu, oracle, star_spec):
-    _walk_parity(gpu, oracle, star_spec, 64, 64, 1024, 1e-3, gpu.WoSt)
+@pytest.mark.parametrize("exact", [True, False])
+def test_walk_u_parity_2d(gpu, oracle, star_spec, exact):
+    _walk_parity(gpu, oracle, star_spec, 64, 64, 1024, 1e-3, gpu.WoSt, exact)
 
 
-def test_walk_u_parity_2d_holes(gpu, oracle, holes_spec):
-    _walk_parity(gpu, oracle, holes_spec, 32, 64, 1024, 1e-4, gpu.WoSt)
+@pytest.mark.parametrize("exact", [True, False])
+def test_walk_u_parity_2d_holes(gpu, oracle, holes_spec, exact):
+    _walk_parity(gpu, oracle, holes_spec, 32, 64, 1024, 1e-4, gpu.WoSt, exact)
 
 
 def test_walk_u_parity_2d_no_cv(gpu, oracle, star_spec):
-    _walk_parity(gpu, oracle, star_spec, 32, 32, 1024, 1e-3, gpu.WoSt, control_variates=False)
+    _walk_parity(gpu, oracle, star_spec, 32, 32, 1024, 1e-3, gpu.WoSt, False, control_variates=False)
 
 
-def test_walk_u_parity_3d(gpu, oracle, ico_spec):
-    _walk_parity(gpu, oracle, ico_spec, 32, 32, 1024, 1e-3, gpu.WoSt3D)
+@pytest.mark.parametrize("exact", [True, False])
+def test_walk_u_parity_3d(gpu, oracle, ico_spec, exact):
+    _walk_parity(gpu, oracle, ico_spec, 32, 32, 1024, 1e-3, gpu.WoSt3D, exact)
 
 
 @pytest.mark.parametrize("which", ["star", "ico"])
@@ -140,21 +144,22 @@ def test_dude_parity(gpu, oracle, which, star_spec, ico_spec):
     assert same.mean() >= 0.97
 
 
+@pytest.mark.parametrize("exact", [True, False])
 @pytest.mark.parametrize("which", ["star", "holes", "ico"])
-def test_fused_parity(gpu, oracle, which, star_spec, holes_spec, ico_spec):
+def test_fused_parity(gpu, oracle, which, exact, star_spec, holes_spec, ico_spec):
     spec = dict(star=star_spec, holes=holes_spec, ico=ico_spec)[which]
     gs, os_ = _scenes(gpu, oracle, spec)
     gs.boundary = gpu.BoundaryCondition(spec.bc_code)
     n, W = 48, 64
     eps = spec.meta["eps"]
     q = spec.queries[:n]
-    sol = (gpu.WoSt if spec.dim == 2 else gpu.WoSt3D)(eps=eps, nwalks=W, nsteps=1024)
+    sol = (gpu.WoSt if spec.dim == 2 else gpu.WoSt3D)(eps=eps, nwalks=W, nsteps=1024, exact_cones=exact)
     g = sol.solve_grad(_t(q), gs, seed=5, per_pair=True, count_steps=True)
     o = os_.solve_fused(q, spec.bc_code, W, nsteps=1024, eps=eps, seed=5, per_pair=True)
     pv = _np(g["pair_vals"])
     same = (pv[:, 0].view(np.uint32) == o["pair_u"].view(np.uint32)) & np.all(
         pv[:, 1:].view(np.uint32) == o["pair_g"].view(np.uint32), axis=1) & (_np(g["pair_steps"]) == o["pair_steps"])
-    print(f"fused pair parity {same.mean():.6f}")
+    print(f"fused pair parity (exact_cones={exact}) {same.mean():.6f}")
     assert same.mean() >= 0.999
     assert int(_np(g["steps"])[0]) == o["steps"]
     assert np.allclose(_np(g["u"]), o["u"], rtol=1e-5, atol=1e-5)

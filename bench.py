@@ -76,6 +76,50 @@ def cpu_baseline(spec, cfg, target_s: float = 12.0):
                 points_per_s=npts / total_t)
 
 
+def adjoint_ratio(points: int, reps: int = 2):
+    """fwd+bwd / fwd time on the 3D adjoint workload (noisy icosphere, 40,962
+    vertices, trainable per-vertex g and h, loss = MSE(u) + MSE(grad u) vs
+    e^x cos y, 256 walks per point, backward replays the walks from seeds)."""
+    import torch
+
+    import dwost_amd
+    from dwost_amd import scenes
+
+    prob = scenes.adjoint_problem_3d(level=6, n_query=points)
+    spec = prob.spec
+    g = torch.as_tensor(prob.g0).cuda().requires_grad_(True)
+    h = torch.as_tensor(prob.h0).cuda().requires_grad_(True)
+    scene = dwost_amd.Scene3D(spec.vertices, spec.indices, spec.types, dwost_amd.BoundaryCondition.per_vertex(g, h))
+    solver = dwost_amd.WoSt3D(eps=1e-3, nwalks=256, nsteps=1024)
+    cs = dwost_amd.CustomSolver(solver)
+    q = torch.as_tensor(spec.queries).cuda()
+    ut, gt = torch.as_tensor(prob.u_target).cuda(), torch.as_tensor(prob.grad_target).cuda()
+
+    def fwd_bwd(seed):
+        g.grad = None
+        h.grad = None
+        u, gr = cs.solve_grad(q, scene, seed=seed)
+        loss = torch.mean((u - ut) ** 2) + torch.mean(torch.sum((gr - gt) ** 2, 1))
+        loss.backward()
+
+    fwd_bwd(100)  # warm-up
+    solver.solve_grad(q, scene, seed=101)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(reps):
+        solver.solve_grad(q, scene, seed=k)
+    torch.cuda.synchronize()
+    t_f = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for k in range(reps):
+        fwd_bwd(k)
+    torch.cuda.synchronize()
+    t_fb = (time.perf_counter() - t0) / reps
+    return dict(workload=f"noisy icosphere 81920 triangles, per-vertex g/h, {points} points x 256 walks, "
+                         "loss MSE(u)+MSE(grad u)", fwd_ms=round(1000 * t_f, 3), fwd_bwd_ms=round(1000 * t_fb, 3),
+                ratio=round(t_fb / t_f, 3))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +129,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--points", type=int, default=0, help="use only the first N query points (0: all)")
     ap.add_argument("--progress", action="store_true", help="print a line after each timed step")
+    ap.add_argument("--adjoint-points", type=int, default=2048,
+                    help="points of the fwd+bwd/fwd measurement on the 3D adjoint workload (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every box of this build has a single MI355X "
@@ -153,6 +199,11 @@ def main():
                 kernel="k_fused (+ point prep + reduction in the same timed call)",
                 b_alg_bytes_per_step=B_ALG[args.config], avg_launch_ms=round(1000 * avg_kernel_s, 3))
 
+    adj = None
+    if args.adjoint_points:
+        adj = adjoint_ratio(args.adjoint_points)
+        print(f"[bench] adjoint ratio {adj['ratio']}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(spec, cfg)
@@ -175,6 +226,7 @@ def main():
                        **acc),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "adjoint": adj,
     }
     print(json.dumps(line), flush=True)
 

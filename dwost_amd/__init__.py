@@ -9,9 +9,9 @@ the built library raises.
 from .shapes import (BC_EXPCOS, BC_LINEAR, BC_PER_VERTEX, BC_QUAD, BoundaryCondition, BoundaryType, Scene2D,
                      Scene3D)
 from ._lib import SUB_DIRICHLET, SUB_FULL, SUB_NEUMANN, DwostError
-from .solvers import Solver, WoSt, WoSt3D
+from .solvers import CustomSolver, Solver, WoSt, WoSt3D
 
 __all__ = [
     "BC_EXPCOS", "BC_LINEAR", "BC_PER_VERTEX", "BC_QUAD", "BoundaryCondition", "BoundaryType", "Scene2D", "Scene3D",
-    "SUB_DIRICHLET", "SUB_FULL", "SUB_NEUMANN", "DwostError", "Solver", "WoSt", "WoSt3D",
+    "SUB_DIRICHLET", "SUB_FULL", "SUB_NEUMANN", "DwostError", "CustomSolver", "Solver", "WoSt", "WoSt3D",
 ]

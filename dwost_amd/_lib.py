@@ -64,12 +64,16 @@ def lib():
     L.dw_intersect.argtypes = [vp, i32, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp]
     L.dw_closest_silhouette.argtypes = [vp, i32, vp, vp, i64, vp, vp, vp, vp]
     L.dw_star_radius.argtypes = [vp, i32, vp, vp, i64, vp, vp]
-    L.dw_solve_u.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp, vp, vp, vp]
+    L.dw_solve_u.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp, vp, vp, vp, vp]
     L.dw_solve_dude.argtypes = [vp, vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp]
-    L.dw_solve_fused.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp, vp, vp, vp]
+    L.dw_solve_fused.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp, vp, vp, vp,
+                                 vp]
+    L.dw_adjoint_u.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp, vp, vp]
+    L.dw_adjoint_fused.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp, vp, vp,
+                                   vp]
     for name in ("dw_scene_create", "dw_scene_destroy", "dw_scene_info", "dw_scene_export", "dw_closest_point",
                  "dw_intersect", "dw_closest_silhouette", "dw_star_radius", "dw_solve_u", "dw_solve_dude",
-                 "dw_solve_fused"):
+                 "dw_solve_fused", "dw_adjoint_u", "dw_adjoint_fused"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

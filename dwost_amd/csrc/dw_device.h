@@ -182,11 +182,74 @@ DWD F3 nmin(const DNode3& n) { return f3(n.mnx, n.mny, n.mnz); }
 DWD F3 nmax(const DNode3& n) { return f3(n.mxx, n.mxy, n.mxz); }
 
 // ---------------------------------------------------------------------------
-// 2D: BVH closest point (bvh.py:111-188 + line_segment.py:74-85)
+// Traversal stack: the bottom kLdsStack entries of each lane live in LDS
+// (entry k of lane t at [k * blockDim.x + t], conflict-free), deeper entries in
+// a private overflow array.  Traversals descend into the nearer child and push
+// only the farther one, so the stack never holds more than the tree depth.
+// Kernels that traverse must launch with kStackLdsBytes of dynamic LDS.
+// ---------------------------------------------------------------------------
+constexpr int kLdsStack = 16;
+constexpr int kStackLdsBytesPerThread = kLdsStack * 8;
+extern __shared__ int dw_lds[];
+
+struct Stk {
+    int* si;
+    float* sd;
+    int stride, sp;
+    int oi[DW_MAX_STACK - kLdsStack];
+    float od[DW_MAX_STACK - kLdsStack];
+    DWD void push(int r, float d) {
+        if (sp < kLdsStack) {
+            si[sp * stride] = r;
+            sd[sp * stride] = d;
+        } else {
+            oi[sp - kLdsStack] = r;
+            od[sp - kLdsStack] = d;
+        }
+        ++sp;
+    }
+    DWD void pop(int& r, float& d) {
+        --sp;
+        if (sp < kLdsStack) {
+            r = si[sp * stride];
+            d = sd[sp * stride];
+        } else {
+            r = oi[sp - kLdsStack];
+            d = od[sp - kLdsStack];
+        }
+    }
+};
+DWD void stack_init(Stk& st) {
+    st.stride = (int)blockDim.x;
+    st.si = dw_lds + threadIdx.x;
+    st.sd = (float*)(dw_lds + kLdsStack * blockDim.x) + threadIdx.x;
+    st.sp = 0;
+}
+
+// leaf reference -> [off, off + cnt)
+DWD void leaf_range(const SubDev& S, int ref, int& off, int& cnt) {
+    uint32_t u = (uint32_t)ref;
+    int c = (int)((u >> 27) & 15u);
+    if (c != DW_LEAF_TABLE) {
+        cnt = c;
+        off = (int)(u & 0x7FFFFFFu);
+    } else {
+        int k = (int)(u & 0x7FFFFFFu);
+        off = S.leaf_table[2 * k];
+        cnt = S.leaf_table[2 * k + 1];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2D: BVH closest point (bvh.py:111-188 + line_segment.py:74-85).  Visit order
+// and pruning are the reference's: both children tested in order with r_max
+// tightened by each d_max, nearer child first, `cd > r_max` skips.
 // ---------------------------------------------------------------------------
 DWD CP closest2(const SubDev& S, F2 p) {
-    const DNode2* nodes = (const DNode2*)S.nodes;
+    const RBvh2* recs = (const RBvh2*)S.rec_bvh;
     const DSeg* prims = (const DSeg*)S.prims;
+    Stk st;
+    stack_init(st);
     CP its;
     its.p = f3(p.x, p.y, 0.0f);
     its.n = f3(0.0f, 0.0f, 0.0f);
@@ -196,67 +259,60 @@ DWD CP closest2(const SubDev& S, F2 p) {
     its.prim = -1;
     int best = -1;
     float r_max = kInf;
-    int st_i[DW_MAX_STACK];
-    float st_d[DW_MAX_STACK];
-    int sp = 0;
-    st_i[0] = 0;
-    st_d[0] = kInf;
-    while (sp >= 0) {
-        int ni = st_i[sp];
-        float cd = st_d[sp];
-        sp--;
-        if (cd > r_max) continue;
-        DNode2 nd = nodes[ni];
-        if (nd.nref > 0) {
-            for (int j = 0; j < nd.nref; ++j) {
-                DSeg s = prims[nd.off + j];
-                F2 a = f2(s.ax, s.ay), b = f2(s.bx, s.by);
-                F2 pa = p - a, ba = b - a;
-                float h = clampf(dot(pa, ba) / dot(ba, ba), 0.0f, 1.0f);
-                F2 i = a + ba * h;
-                float d = norm(i - p);
-                if (d < its.d) {
-                    its.d = d;
-                    its.t = h;
-                    its.p = f3(i.x, i.y, 0.0f);
-                    best = nd.off + j;
+    int ref = S.root_ref_bvh;
+    float cd = kInf;
+    while (true) {
+        if (!(cd > r_max)) {
+            if (ref < 0) {
+                int off, cnt;
+                leaf_range(S, ref, off, cnt);
+                for (int j = 0; j < cnt; ++j) {
+                    DSeg sg = prims[off + j];
+                    F2 a = f2(sg.ax, sg.ay), b = f2(sg.bx, sg.by);
+                    F2 pa = p - a, ba = b - a;
+                    float h = clampf(dot(pa, ba) / dot(ba, ba), 0.0f, 1.0f);
+                    F2 i = a + ba * h;
+                    float d = norm(i - p);
+                    if (d < its.d) {
+                        its.d = d;
+                        its.t = h;
+                        its.p = f3(i.x, i.y, 0.0f);
+                        best = off + j;
+                    }
                 }
-            }
-            r_max = fmn(r_max, its.d);
-        } else {
-            int L = ni + 1, R = ni + nd.off;
-            DNode2 nl = nodes[L], nr = nodes[R];
-            float dminL, dmaxL, dminR, dmaxR;
-            box_dist(nmin(nl), nmax(nl), p, dminL, dmaxL);
-            bool hitL = dminL <= r_max;
-            r_max = fmn(r_max, dmaxL);
-            box_dist(nmin(nr), nmax(nr), p, dminR, dmaxR);
-            bool hitR = dminR <= r_max;
-            r_max = fmn(r_max, dmaxR);
-            if (hitL && hitR) {
-                int closer = L, other = R;
-                if (dminR < dminL) {
-                    closer = R, other = L;
-                    float t = dminL;
-                    dminL = dminR;
-                    dminR = t;
+                r_max = fmn(r_max, its.d);
+            } else {
+                RBvh2 rc = recs[ref];
+                float dminL, dmaxL, dminR, dmaxR;
+                box_dist(f2(rc.lmnx, rc.lmny), f2(rc.lmxx, rc.lmxy), p, dminL, dmaxL);
+                bool hitL = dminL <= r_max;
+                r_max = fmn(r_max, dmaxL);
+                box_dist(f2(rc.rmnx, rc.rmny), f2(rc.rmxx, rc.rmxy), p, dminR, dmaxR);
+                bool hitR = dminR <= r_max;
+                r_max = fmn(r_max, dmaxR);
+                if (hitL && hitR) {
+                    bool sw = dminR < dminL;  // `d_max_right < d_max_right` (bvh.py:163) never swaps
+                    st.push(sw ? rc.ref_l : rc.ref_r, sw ? dminL : dminR);
+                    ref = sw ? rc.ref_r : rc.ref_l;
+                    cd = sw ? dminR : dminL;
+                    continue;
+                } else if (hitL) {
+                    ref = rc.ref_l;
+                    cd = dminL;
+                    continue;
+                } else if (hitR) {
+                    ref = rc.ref_r;
+                    cd = dminR;
+                    continue;
                 }
-                st_i[++sp] = other;
-                st_d[sp] = dminR;
-                st_i[++sp] = closer;
-                st_d[sp] = dminL;
-            } else if (hitL) {
-                st_i[++sp] = L;
-                st_d[sp] = dminL;
-            } else if (hitR) {
-                st_i[++sp] = R;
-                st_d[sp] = dminR;
             }
         }
+        if (st.sp == 0) break;
+        st.pop(ref, cd);
     }
     if (best >= 0) {  // normal of the winning segment (LineSegment.closest_point)
-        DSeg s = prims[best];
-        F2 t = normalize(f2(s.bx - s.ax, s.by - s.ay));
+        DSeg sg = prims[best];
+        F2 t = normalize(f2(sg.bx - sg.ax, sg.by - sg.ay));
         its.n = f3(t.y, -t.x, 0.0f);
         its.prim = S.prim_index[best];
     }
@@ -267,7 +323,7 @@ DWD CP closest2(const SubDev& S, F2 p) {
 // (`r_max = its.d` reads the zero-initialised record, :216), so the traversal
 // stops at the first hit.
 DWD Hit intersect2(const SubDev& S, F2 x, F2 v, float r_max) {
-    const DNode2* nodes = (const DNode2*)S.nodes;
+    const RBvh2* recs = (const RBvh2*)S.rec_bvh;
     const DSeg* prims = (const DSeg*)S.prims;
     Hit its;
     its.valid = false;
@@ -276,71 +332,65 @@ DWD Hit intersect2(const SubDev& S, F2 x, F2 v, float r_max) {
     its.t = its.u = its.v = its.d = 0.0f;
     its.prim = 0;
     float tmin;
-    DNode2 root = nodes[0];
-    if (!box_ray(nmin(root), nmax(root), x, v, r_max, tmin)) return its;
-    int st_i[DW_MAX_STACK];
-    float st_d[DW_MAX_STACK];
-    int sp = 0;
-    st_i[0] = 0;
-    st_d[0] = tmin;
-    while (sp >= 0) {
-        int ni = st_i[sp];
-        float cd = st_d[sp];
-        sp--;
-        if (!(cd <= r_max)) continue;
-        DNode2 nd = nodes[ni];
-        if (nd.nref > 0) {
-            for (int j = 0; j < nd.nref; ++j) {  // LineSegment.ray_intersect (line_segment.py:47-72)
-                DSeg s = prims[nd.off + j];
-                F2 a = f2(s.ax, s.ay), b = f2(s.bx, s.by);
-                F2 u = a - x, w = b - a;
-                float dv = v.x * w.y - v.y * w.x;
-                if (!(__builtin_fabsf(dv) > kDrEps)) continue;
-                float ud = u.x * v.y - u.y * v.x;
-                float s_ = ud / dv;
-                if (!((s_ >= 0.0f) && (s_ <= 1.0f))) continue;
-                float uv = u.x * w.y - u.y * w.x;
-                float t = uv / dv;
-                if (!((t >= 0.0f) && (t <= r_max))) continue;
-                float d = __builtin_fabsf(t);
-                if (!(d < r_max)) continue;
-                F2 n = normalize(f2(w.y, -w.x));
-                F2 p = x + v * t;
-                its.valid = true;
-                its.on_boundary = true;
-                its.p = f3(p.x, p.y, 0.0f);
-                its.n = f3(n.x, n.y, 0.0f);
-                its.t = s_;
-                its.d = d;
-                its.prim = S.prim_index[nd.off + j];
-                return its;
-            }
-        } else {
-            int L = ni + 1, R = ni + nd.off;
-            DNode2 nl = nodes[L], nr = nodes[R];
-            float t0, t1;
-            bool h0 = box_ray(nmin(nl), nmax(nl), x, v, r_max, t0);
-            bool h1 = box_ray(nmin(nr), nmax(nr), x, v, r_max, t1);
-            if (h0 && h1) {
-                int closer = L, other = R;
-                if (t1 < t0) {
-                    closer = R, other = L;
-                    float t = t0;
-                    t0 = t1;
-                    t1 = t;
+    if (!box_ray(f2(S.root_mn[0], S.root_mn[1]), f2(S.root_mx[0], S.root_mx[1]), x, v, r_max, tmin)) return its;
+    Stk st;
+    stack_init(st);
+    int ref = S.root_ref_bvh;
+    float cd = tmin;
+    while (true) {
+        if (cd <= r_max) {
+            if (ref < 0) {
+                int off, cnt;
+                leaf_range(S, ref, off, cnt);
+                for (int j = 0; j < cnt; ++j) {  // LineSegment.ray_intersect (line_segment.py:47-72)
+                    DSeg sg = prims[off + j];
+                    F2 a = f2(sg.ax, sg.ay), b = f2(sg.bx, sg.by);
+                    F2 u = a - x, w = b - a;
+                    float dv = v.x * w.y - v.y * w.x;
+                    if (!(__builtin_fabsf(dv) > kDrEps)) continue;
+                    float ud = u.x * v.y - u.y * v.x;
+                    float s_ = ud / dv;
+                    if (!((s_ >= 0.0f) && (s_ <= 1.0f))) continue;
+                    float uv = u.x * w.y - u.y * w.x;
+                    float t = uv / dv;
+                    if (!((t >= 0.0f) && (t <= r_max))) continue;
+                    float d = __builtin_fabsf(t);
+                    if (!(d < r_max)) continue;
+                    F2 n = normalize(f2(w.y, -w.x));
+                    F2 hp = x + v * t;
+                    its.valid = true;
+                    its.on_boundary = true;
+                    its.p = f3(hp.x, hp.y, 0.0f);
+                    its.n = f3(n.x, n.y, 0.0f);
+                    its.t = s_;
+                    its.d = d;
+                    its.prim = S.prim_index[off + j];
+                    return its;
                 }
-                st_i[++sp] = other;
-                st_d[sp] = t1;
-                st_i[++sp] = closer;
-                st_d[sp] = t0;
-            } else if (h0) {
-                st_i[++sp] = L;
-                st_d[sp] = t0;
-            } else if (h1) {
-                st_i[++sp] = R;
-                st_d[sp] = t1;
+            } else {
+                RBvh2 rc = recs[ref];
+                float t0, t1;
+                bool h0 = box_ray(f2(rc.lmnx, rc.lmny), f2(rc.lmxx, rc.lmxy), x, v, r_max, t0);
+                bool h1 = box_ray(f2(rc.rmnx, rc.rmny), f2(rc.rmxx, rc.rmxy), x, v, r_max, t1);
+                if (h0 && h1) {
+                    bool sw = t1 < t0;
+                    st.push(sw ? rc.ref_l : rc.ref_r, sw ? t0 : t1);
+                    ref = sw ? rc.ref_r : rc.ref_l;
+                    cd = sw ? t1 : t0;
+                    continue;
+                } else if (h0) {
+                    ref = rc.ref_l;
+                    cd = t0;
+                    continue;
+                } else if (h1) {
+                    ref = rc.ref_r;
+                    cd = t1;
+                    continue;
+                }
             }
         }
+        if (st.sp == 0) break;
+        st.pop(ref, cd);
     }
     return its;
 }
@@ -405,12 +455,7 @@ DWD bool snch_overlaps(V mn, V mx, V axis, float ha, float radius, V x, float r_
 // prunes a node that does hold the nearest silhouette.
 constexpr float kConeMargin = 4e-6f;
 template <typename V>
-DWD bool snch_overlaps_fast(V mn, V mx, V axis, float ha, float sin_ha, float cos_ha, float radius, V x, float r_max,
-                            float& d_min) {
-    float d_max;
-    box_dist(mn, mx, x, d_min, d_max);
-    bool ov = d_min <= r_max;
-    if (!((ha >= 0.0f) && ov)) return ov;
+DWD bool cone_fast(V mn, V mx, V axis, float ha, float sin_ha, float cos_ha, float radius, V x) {
     if (ha > kPi2 - kConeMargin) return true;
     V cen = (mn + mx) / 2.0f;
     V vca = cen - x;
@@ -435,81 +480,109 @@ DWD bool snch_overlaps_fast(V mn, V mx, V axis, float ha, float sin_ha, float co
     if (cos_ha * cv - sin_ha * sv < kConeMargin) return true;
     return c <= (sin_ha * cv + cos_ha * sv) + kConeMargin;
 }
+template <typename V>
+DWD bool snch_overlaps_fast(V mn, V mx, V axis, float ha, float sin_ha, float cos_ha, float radius, V x, float r_max,
+                            float& d_min) {
+    float d_max;
+    box_dist(mn, mx, x, d_min, d_max);
+    bool ov = d_min <= r_max;
+    if (!((ha >= 0.0f) && ov)) return ov;
+    return cone_fast(mn, mx, axis, ha, sin_ha, cos_ha, radius, x);
+}
+
+// Face-plane bound of a 3D SNCH node (built on the host with the fp32 error
+// of the silhouette test folded in): true when every face adjacent to the
+// node's edges sees x on the same side, so the node holds no silhouette.
+DWD bool face_pruned(F3 mn, F3 mx, F3 axis, float rho, float elo, float ehi, F3 x) {
+    F3 v = x - (mn + mx) / 2.0f;
+    float cv = dot(axis, v);
+    float sl = rho * norm(v);
+    return ((cv - sl) - ehi > 0.0f) || ((cv + sl) - elo < 0.0f);
+}
+// One child of a 3D SNCH record: box overlap, face-plane bound, cone test.
+template <bool X>
+DWD bool snch3_child(F3 mn, F3 mx, F3 axis, float ha, float sin_ha, float cos_ha, float radius, float rho, float elo,
+                     float ehi, F3 x, float r_max, float& d_min) {
+    if constexpr (X) {
+        return snch_overlaps(mn, mx, axis, ha, radius, x, r_max, d_min) &&
+               !face_pruned(mn, mx, axis, rho, elo, ehi, x);
+    } else {
+        float d_max;
+        box_dist(mn, mx, x, d_min, d_max);
+        if (!(d_min <= r_max)) return false;
+        if (face_pruned(mn, mx, axis, rho, elo, ehi, x)) return false;
+        if (!(ha >= 0.0f)) return true;
+        return cone_fast(mn, mx, axis, ha, sin_ha, cos_ha, radius, x);
+    }
+}
 
 // SNCH.closest_silhouette 2D (snch.py:312-378 + silhouette_vertex.py:80-93)
 template <bool X = true>
 DWD bool closest_sil2(const SubDev& S, F2 x, float r_max, float& d_out, int& ref_out, F2& p_out) {
-    const DNode2* nodes = (const DNode2*)S.nodes;
-    const DCone2* cones = (const DCone2*)S.cones;
+    const RSnch2* recs = (const RSnch2*)S.rec_snch;
     const DSilV* sils = (const DSilV*)S.sils;
     bool found = false;
     d_out = 0.0f;
     ref_out = -1;
-    DNode2 root = nodes[0];
-    float dmin = box_dmin(nmin(root), nmax(root), x);
+    float dmin = box_dmin(f2(S.root_mn[0], S.root_mn[1]), f2(S.root_mx[0], S.root_mx[1]), x);
     if (!(dmin <= r_max)) return false;
-    int st_i[DW_MAX_STACK];
-    float st_d[DW_MAX_STACK];
-    int sp = 0;
-    st_i[0] = 0;
-    st_d[0] = dmin;
-    while (sp >= 0) {
-        int ni = st_i[sp];
-        float cd = st_d[sp];
-        sp--;
-        if (!(cd <= r_max)) continue;
-        DNode2 nd = nodes[ni];
-        if (nd.nref > 0) {
-            for (int j = 0; j < nd.nsil; ++j) {
-                int ref = nd.sil_off + j;
-                DSilV s = sils[ref];
-                F2 a = f2(s.ax, s.ay), b = f2(s.bx, s.by), c = f2(s.cx, s.cy);
-                float d = norm(x - b);
-                if (s.both && (d < r_max)) {
-                    if (cross2(b - a, x - a) * cross2(c - b, x - b) < 0.0f) {
-                        r_max = fmn(r_max, d);
-                        found = true;
-                        d_out = d;
-                        ref_out = ref;
-                        p_out = b;
+    Stk st;
+    stack_init(st);
+    int ref = S.root_ref_snch;
+    float cd = dmin;
+    while (true) {
+        if (cd <= r_max) {
+            if (ref < 0) {
+                int off, cnt;
+                leaf_range(S, ref, off, cnt);
+                for (int j = 0; j < cnt; ++j) {
+                    DSilV sv = sils[off + j];
+                    F2 a = f2(sv.ax, sv.ay), b = f2(sv.bx, sv.by), c = f2(sv.cx, sv.cy);
+                    float d = norm(x - b);
+                    if (sv.both && (d < r_max)) {
+                        if (cross2(b - a, x - a) * cross2(c - b, x - b) < 0.0f) {
+                            r_max = fmn(r_max, d);
+                            found = true;
+                            d_out = d;
+                            ref_out = off + j;
+                            p_out = b;
+                        }
                     }
                 }
-            }
-        } else {
-            int L = ni + 1, R = ni + nd.off;
-            DNode2 nl = nodes[L], nr = nodes[R];
-            DCone2 cl = cones[L], cr = cones[R];
-            float d0, d1;
-            bool h0, h1;
-            if constexpr (X) {
-                h0 = snch_overlaps(nmin(nl), nmax(nl), f2(cl.ax, cl.ay), cl.half_angle, cl.radius, x, r_max, d0);
-                h1 = snch_overlaps(nmin(nr), nmax(nr), f2(cr.ax, cr.ay), cr.half_angle, cr.radius, x, r_max, d1);
             } else {
-                h0 = snch_overlaps_fast(nmin(nl), nmax(nl), f2(cl.ax, cl.ay), cl.half_angle, cl.sin_ha, cl.cos_ha,
-                                        cl.radius, x, r_max, d0);
-                h1 = snch_overlaps_fast(nmin(nr), nmax(nr), f2(cr.ax, cr.ay), cr.half_angle, cr.sin_ha, cr.cos_ha,
-                                        cr.radius, x, r_max, d1);
-            }
-            if (h0 && h1) {
-                int closer = L, other = R;
-                if (d1 < d0) {
-                    closer = R, other = L;
-                    float t = d0;
-                    d0 = d1;
-                    d1 = t;
+                RSnch2 rc = recs[ref];
+                float d0, d1;
+                bool h0, h1;
+                if constexpr (X) {
+                    h0 = snch_overlaps(f2(rc.lmnx, rc.lmny), f2(rc.lmxx, rc.lmxy), f2(rc.lax, rc.lay), rc.lha, rc.lrad,
+                                       x, r_max, d0);
+                    h1 = snch_overlaps(f2(rc.rmnx, rc.rmny), f2(rc.rmxx, rc.rmxy), f2(rc.rax, rc.ray), rc.rha, rc.rrad,
+                                       x, r_max, d1);
+                } else {
+                    h0 = snch_overlaps_fast(f2(rc.lmnx, rc.lmny), f2(rc.lmxx, rc.lmxy), f2(rc.lax, rc.lay), rc.lha,
+                                            rc.lsin, rc.lcos, rc.lrad, x, r_max, d0);
+                    h1 = snch_overlaps_fast(f2(rc.rmnx, rc.rmny), f2(rc.rmxx, rc.rmxy), f2(rc.rax, rc.ray), rc.rha,
+                                            rc.rsin, rc.rcos, rc.rrad, x, r_max, d1);
                 }
-                st_i[++sp] = other;
-                st_d[sp] = d1;
-                st_i[++sp] = closer;
-                st_d[sp] = d0;
-            } else if (h0) {
-                st_i[++sp] = L;
-                st_d[sp] = d0;
-            } else if (h1) {
-                st_i[++sp] = R;
-                st_d[sp] = d1;
+                if (h0 && h1) {
+                    bool sw = d1 < d0;
+                    st.push(sw ? rc.ref_l : rc.ref_r, sw ? d0 : d1);
+                    ref = sw ? rc.ref_r : rc.ref_l;
+                    cd = sw ? d1 : d0;
+                    continue;
+                } else if (h0) {
+                    ref = rc.ref_l;
+                    cd = d0;
+                    continue;
+                } else if (h1) {
+                    ref = rc.ref_r;
+                    cd = d1;
+                    continue;
+                }
             }
         }
+        if (st.sp == 0) break;
+        st.pop(ref, cd);
     }
     return found;
 }
@@ -517,91 +590,75 @@ DWD bool closest_sil2(const SubDev& S, F2 x, float r_max, float& d_out, int& ref
 // SNCH3D.closest_silhouette (snch3d.py:594-663 + silhouette_edge.py:83-97)
 template <bool X = true>
 DWD bool closest_sil3(const SubDev& S, F3 x, float r_max, float& d_out, int& ref_out, F3& p_out) {
-    const DNode3* nodes = (const DNode3*)S.nodes;
-    const DCone3* cones = (const DCone3*)S.cones;
+    const RSnch3* recs = (const RSnch3*)S.rec_snch;
     const DSilE* sils = (const DSilE*)S.sils;
     bool found = false;
     d_out = 0.0f;
     ref_out = -1;
-    DNode3 root = nodes[0];
-    float dmin = box_dmin(nmin(root), nmax(root), x);
+    float dmin = box_dmin(f3(S.root_mn[0], S.root_mn[1], S.root_mn[2]), f3(S.root_mx[0], S.root_mx[1], S.root_mx[2]), x);
     if (!(dmin <= r_max)) return false;
-    int st_i[DW_MAX_STACK];
-    float st_d[DW_MAX_STACK];
-    int sp = 0;
-    st_i[0] = 0;
-    st_d[0] = dmin;
-    while (sp >= 0) {
-        int ni = st_i[sp];
-        float cd = st_d[sp];
-        sp--;
-        if (!(cd <= r_max)) continue;
-        DNode3 nd = nodes[ni];
-        if (nd.nref > 0) {
-            DCone3 cn = cones[ni];
-            for (int j = 0; j < cn.nsil; ++j) {
-                int ref = cn.sil_off + j;
-                DSilE s = sils[ref];
-                F3 a = f3(s.ax, s.ay, s.az), b = f3(s.bx, s.by, s.bz);
-                F3 u = b - a, w = x - a;
-                float t = clampf(dot(w, u) / dot(u, u), 0.0f, 1.0f);
-                F3 pt = lerp(a, b, t);
-                float d = norm(pt - x);
-                if (d < r_max) {
-                    bool is = s.boundary != 0;
-                    if (!is) {
-                        F3 c = f3(s.cx, s.cy, s.cz), dd = f3(s.dx, s.dy, s.dz);
-                        F3 n0 = normalize(cross(b - a, c - a));
-                        F3 n1 = normalize(cross(dd - a, b - a));
-                        F3 view = pt - x;
-                        is = dot(view, n0) * dot(view, n1) < 0.0f;
-                    }
-                    if (is) {
-                        r_max = fmn(r_max, d);
-                        found = true;
-                        d_out = d;
-                        ref_out = ref;
-                        p_out = pt;
+    Stk st;
+    stack_init(st);
+    int ref = S.root_ref_snch;
+    float cd = dmin;
+    while (true) {
+        if (cd <= r_max) {
+            if (ref < 0) {
+                int off, cnt;
+                leaf_range(S, ref, off, cnt);
+                for (int j = 0; j < cnt; ++j) {
+                    DSilE se = sils[off + j];
+                    F3 a = f3(se.ax, se.ay, se.az), b = f3(se.bx, se.by, se.bz);
+                    F3 u = b - a, w = x - a;
+                    float t = clampf(dot(w, u) / dot(u, u), 0.0f, 1.0f);
+                    F3 pt = lerp(a, b, t);
+                    float d = norm(pt - x);
+                    if (d < r_max) {
+                        bool is = se.boundary != 0;
+                        if (!is) {
+                            F3 c = f3(se.cx, se.cy, se.cz), dd = f3(se.dx, se.dy, se.dz);
+                            F3 n0 = normalize(cross(b - a, c - a));
+                            F3 n1 = normalize(cross(dd - a, b - a));
+                            F3 view = pt - x;
+                            is = dot(view, n0) * dot(view, n1) < 0.0f;
+                        }
+                        if (is) {
+                            r_max = fmn(r_max, d);
+                            found = true;
+                            d_out = d;
+                            ref_out = off + j;
+                            p_out = pt;
+                        }
                     }
                 }
-            }
-        } else {
-            int L = ni + 1, R = ni + nd.off;
-            DNode3 nl = nodes[L], nr = nodes[R];
-            DCone3 cl = cones[L], cr = cones[R];
-            float d0, d1;
-            bool h0, h1;
-            if constexpr (X) {
-                h0 = snch_overlaps(nmin(nl), nmax(nl), f3(cl.ax, cl.ay, cl.az), cl.half_angle, cl.radius, x, r_max,
-                                   d0);
-                h1 = snch_overlaps(nmin(nr), nmax(nr), f3(cr.ax, cr.ay, cr.az), cr.half_angle, cr.radius, x, r_max,
-                                   d1);
             } else {
-                h0 = snch_overlaps_fast(nmin(nl), nmax(nl), f3(cl.ax, cl.ay, cl.az), cl.half_angle, cl.sin_ha,
-                                        cl.cos_ha, cl.radius, x, r_max, d0);
-                h1 = snch_overlaps_fast(nmin(nr), nmax(nr), f3(cr.ax, cr.ay, cr.az), cr.half_angle, cr.sin_ha,
-                                        cr.cos_ha, cr.radius, x, r_max, d1);
-            }
-            if (h0 && h1) {
-                int closer = L, other = R;
-                if (d1 < d0) {
-                    closer = R, other = L;
-                    float t = d0;
-                    d0 = d1;
-                    d1 = t;
+                RSnch3 rc = recs[ref];
+                F3 lmn = f3(rc.lmnx, rc.lmny, rc.lmnz), lmx = f3(rc.lmxx, rc.lmxy, rc.lmxz);
+                F3 rmn = f3(rc.rmnx, rc.rmny, rc.rmnz), rmx = f3(rc.rmxx, rc.rmxy, rc.rmxz);
+                float d0, d1;
+                bool h0 = snch3_child<X>(lmn, lmx, f3(rc.lax, rc.lay, rc.laz), rc.lha, rc.lsin, rc.lcos, rc.lrad,
+                                         rc.lrho, rc.lelo, rc.lehi, x, r_max, d0);
+                bool h1 = snch3_child<X>(rmn, rmx, f3(rc.rax, rc.ray, rc.raz), rc.rha, rc.rsin, rc.rcos, rc.rrad,
+                                         rc.rrho, rc.relo, rc.rehi, x, r_max, d1);
+                if (h0 && h1) {
+                    bool sw = d1 < d0;
+                    st.push(sw ? rc.ref_l : rc.ref_r, sw ? d0 : d1);
+                    ref = sw ? rc.ref_r : rc.ref_l;
+                    cd = sw ? d1 : d0;
+                    continue;
+                } else if (h0) {
+                    ref = rc.ref_l;
+                    cd = d0;
+                    continue;
+                } else if (h1) {
+                    ref = rc.ref_r;
+                    cd = d1;
+                    continue;
                 }
-                st_i[++sp] = other;
-                st_d[sp] = d1;
-                st_i[++sp] = closer;
-                st_d[sp] = d0;
-            } else if (h0) {
-                st_i[++sp] = L;
-                st_d[sp] = d0;
-            } else if (h1) {
-                st_i[++sp] = R;
-                st_d[sp] = d1;
             }
         }
+        if (st.sp == 0) break;
+        st.pop(ref, cd);
     }
     return found;
 }
@@ -679,8 +736,10 @@ DWD void tri_closest(const DTri& tr, F3 p, F3& pt, float& u_out, float& v_out, f
 }
 
 DWD CP closest3(const SubDev& S, F3 p) {
-    const DNode3* nodes = (const DNode3*)S.nodes;
+    const RBvh3* recs = (const RBvh3*)S.rec_bvh;
     const DTri* prims = (const DTri*)S.prims;
+    Stk st;
+    stack_init(st);
     CP its;
     its.p = its.n = f3(0.0f, 0.0f, 0.0f);
     its.t = its.u = its.v = 0.0f;
@@ -688,65 +747,65 @@ DWD CP closest3(const SubDev& S, F3 p) {
     its.prim = -1;
     int best = -1;
     float r_max = kInf;
-    int st_i[DW_MAX_STACK];
-    float st_d[DW_MAX_STACK];
-    int sp = 0;
-    st_i[0] = 0;
-    st_d[0] = kInf;
-    while (sp >= 0) {
-        int ni = st_i[sp];
-        float cd = st_d[sp];
-        sp--;
-        if (!(cd <= r_max)) continue;
-        DNode3 nd = nodes[ni];
-        if (nd.nref > 0) {
-            for (int j = 0; j < nd.nref; ++j) {
-                DTri tr = prims[nd.off + j];
-                F3 pt;
-                float u, v, d;
-                tri_closest(tr, p, pt, u, v, d);
-                if (d < its.d) {
-                    its.d = d;
-                    its.p = pt;
-                    its.u = u;
-                    its.v = v;
-                    best = nd.off + j;
-                    r_max = fmn(r_max, d);
+    int ref = S.root_ref_bvh;
+    float cd = kInf;
+    while (true) {
+        if (cd <= r_max) {
+            if (ref < 0) {
+                int off, cnt;
+                leaf_range(S, ref, off, cnt);
+                for (int j = 0; j < cnt; ++j) {
+                    DTri tr = prims[off + j];
+                    F3 pt;
+                    float u, v, d;
+                    tri_closest(tr, p, pt, u, v, d);
+                    if (d < its.d) {
+                        its.d = d;
+                        its.p = pt;
+                        its.u = u;
+                        its.v = v;
+                        best = off + j;
+                        r_max = fmn(r_max, d);
+                    }
                 }
-            }
-        } else {
-            int L = ni + 1, R = ni + nd.off;
-            DNode3 nl = nodes[L], nr = nodes[R];
-            float dminL, dmaxL, dminR, dmaxR;
-            box_dist(nmin(nl), nmax(nl), p, dminL, dmaxL);
-            bool hitL = dminL <= r_max;
-            r_max = fmn(r_max, dmaxL);
-            box_dist(nmin(nr), nmax(nr), p, dminR, dmaxR);
-            bool hitR = dminR <= r_max;
-            r_max = fmn(r_max, dmaxR);
-            if (hitL && hitR) {
-                int closer = L, other = R;
-                if ((dminL == 0.0f) && (dminR == 0.0f) && (dmaxR < dmaxL)) closer = R, other = L;
-                if (dminR < dminL) {
-                    int t = closer;
-                    closer = other;
-                    other = t;
-                    float tt = dminL;
-                    dminL = dminR;
-                    dminR = tt;
+            } else {
+                RBvh3 rc = recs[ref];
+                float dminL, dmaxL, dminR, dmaxR;
+                box_dist(f3(rc.lmnx, rc.lmny, rc.lmnz), f3(rc.lmxx, rc.lmxy, rc.lmxz), p, dminL, dmaxL);
+                bool hitL = dminL <= r_max;
+                r_max = fmn(r_max, dmaxL);
+                box_dist(f3(rc.rmnx, rc.rmny, rc.rmnz), f3(rc.rmxx, rc.rmxy, rc.rmxz), p, dminR, dmaxR);
+                bool hitR = dminR <= r_max;
+                r_max = fmn(r_max, dmaxR);
+                if (hitL && hitR) {
+                    // snch3d.py:383-388: equal zero d_min -> nearer d_max first; then d_min order
+                    int closer = rc.ref_l, other = rc.ref_r;
+                    if ((dminL == 0.0f) && (dminR == 0.0f) && (dmaxR < dmaxL)) closer = rc.ref_r, other = rc.ref_l;
+                    float dc = dminL, dO = dminR;
+                    if (dminR < dminL) {
+                        int t = closer;
+                        closer = other;
+                        other = t;
+                        dc = dminR;
+                        dO = dminL;
+                    }
+                    st.push(other, dO);
+                    ref = closer;
+                    cd = dc;
+                    continue;
+                } else if (hitL) {
+                    ref = rc.ref_l;
+                    cd = dminL;
+                    continue;
+                } else if (hitR) {
+                    ref = rc.ref_r;
+                    cd = dminR;
+                    continue;
                 }
-                st_i[++sp] = other;
-                st_d[sp] = dminR;
-                st_i[++sp] = closer;
-                st_d[sp] = dminL;
-            } else if (hitL) {
-                st_i[++sp] = L;
-                st_d[sp] = dminL;
-            } else if (hitR) {
-                st_i[++sp] = R;
-                st_d[sp] = dminR;
             }
         }
+        if (st.sp == 0) break;
+        st.pop(ref, cd);
     }
     if (best >= 0) {
         DTri tr = prims[best];
@@ -760,7 +819,7 @@ DWD CP closest3(const SubDev& S, F3 p) {
 // SNCH3D.intersect (snch3d.py:411-476) + Triangle.ray_intersect (triangle.py:59-86);
 // prim = BVH-sorted index (triangle.py:82)
 DWD Hit intersect3(const SubDev& S, F3 x, F3 v, float r_max) {
-    const DNode3* nodes = (const DNode3*)S.nodes;
+    const RBvh3* recs = (const RBvh3*)S.rec_bvh;
     const DTri* prims = (const DTri*)S.prims;
     Hit its;
     its.valid = false;
@@ -771,68 +830,64 @@ DWD Hit intersect3(const SubDev& S, F3 x, F3 v, float r_max) {
     int best = -1;
     float bv = 0.0f, bw = 0.0f;
     float tmin;
-    DNode3 root = nodes[0];
-    if (!box_ray(nmin(root), nmax(root), x, v, r_max, tmin)) return its;
-    int st_i[DW_MAX_STACK];
-    float st_d[DW_MAX_STACK];
-    int sp = 0;
-    st_i[0] = 0;
-    st_d[0] = tmin;
-    while (sp >= 0) {
-        int ni = st_i[sp];
-        float cd = st_d[sp];
-        sp--;
-        if (!(cd <= r_max)) continue;
-        DNode3 nd = nodes[ni];
-        if (nd.nref > 0) {
-            for (int j = 0; j < nd.nref; ++j) {
-                DTri tr = prims[nd.off + j];
-                F3 a = f3(tr.ax, tr.ay, tr.az), b = f3(tr.bx, tr.by, tr.bz), c = f3(tr.cx, tr.cy, tr.cz);
-                F3 v1 = b - a, v2 = c - a;
-                F3 pp = cross(v, v2);
-                float det = dot(v1, pp);
-                if (!(__builtin_fabsf(det) > kDrEps)) continue;
-                float inv = 1.0f / det;
-                F3 s = x - a;
-                float bu = dot(s, pp) * inv;
-                if (!((bu >= 0.0f) && (bu <= 1.0f))) continue;
-                F3 q = cross(s, v1);
-                float bw_ = dot(v, q) * inv;
-                if (!((bw_ >= 0.0f) && (bu + bw_ <= 1.0f))) continue;
-                float t = dot(v2, q) * inv;
-                if (!((t >= 0.0f) && (t <= r_max))) continue;
-                if (!(t < r_max)) continue;
-                r_max = t;
-                best = nd.off + j;
-                bv = bu;
-                bw = bw_;
-            }
-        } else {
-            int L = ni + 1, R = ni + nd.off;
-            DNode3 nl = nodes[L], nr = nodes[R];
-            float t0, t1;
-            bool h0 = box_ray(nmin(nl), nmax(nl), x, v, r_max, t0);
-            bool h1 = box_ray(nmin(nr), nmax(nr), x, v, r_max, t1);
-            if (h0 && h1) {
-                int closer = L, other = R;
-                if (t1 < t0) {
-                    closer = R, other = L;
-                    float t = t0;
-                    t0 = t1;
-                    t1 = t;
+    if (!box_ray(f3(S.root_mn[0], S.root_mn[1], S.root_mn[2]), f3(S.root_mx[0], S.root_mx[1], S.root_mx[2]), x, v,
+                 r_max, tmin))
+        return its;
+    Stk st;
+    stack_init(st);
+    int ref = S.root_ref_bvh;
+    float cd = tmin;
+    while (true) {
+        if (cd <= r_max) {
+            if (ref < 0) {
+                int off, cnt;
+                leaf_range(S, ref, off, cnt);
+                for (int j = 0; j < cnt; ++j) {
+                    DTri tr = prims[off + j];
+                    F3 a = f3(tr.ax, tr.ay, tr.az), b = f3(tr.bx, tr.by, tr.bz), c = f3(tr.cx, tr.cy, tr.cz);
+                    F3 v1 = b - a, v2 = c - a;
+                    F3 pp = cross(v, v2);
+                    float det = dot(v1, pp);
+                    if (!(__builtin_fabsf(det) > kDrEps)) continue;
+                    float inv = 1.0f / det;
+                    F3 s = x - a;
+                    float bu = dot(s, pp) * inv;
+                    if (!((bu >= 0.0f) && (bu <= 1.0f))) continue;
+                    F3 q = cross(s, v1);
+                    float bw_ = dot(v, q) * inv;
+                    if (!((bw_ >= 0.0f) && (bu + bw_ <= 1.0f))) continue;
+                    float t = dot(v2, q) * inv;
+                    if (!((t >= 0.0f) && (t <= r_max))) continue;
+                    if (!(t < r_max)) continue;
+                    r_max = t;
+                    best = off + j;
+                    bv = bu;
+                    bw = bw_;
                 }
-                st_i[++sp] = other;
-                st_d[sp] = t1;
-                st_i[++sp] = closer;
-                st_d[sp] = t0;
-            } else if (h0) {
-                st_i[++sp] = L;
-                st_d[sp] = t0;
-            } else if (h1) {
-                st_i[++sp] = R;
-                st_d[sp] = t1;
+            } else {
+                RBvh3 rc = recs[ref];
+                float t0, t1;
+                bool h0 = box_ray(f3(rc.lmnx, rc.lmny, rc.lmnz), f3(rc.lmxx, rc.lmxy, rc.lmxz), x, v, r_max, t0);
+                bool h1 = box_ray(f3(rc.rmnx, rc.rmny, rc.rmnz), f3(rc.rmxx, rc.rmxy, rc.rmxz), x, v, r_max, t1);
+                if (h0 && h1) {
+                    bool sw = t1 < t0;
+                    st.push(sw ? rc.ref_l : rc.ref_r, sw ? t0 : t1);
+                    ref = sw ? rc.ref_r : rc.ref_l;
+                    cd = sw ? t1 : t0;
+                    continue;
+                } else if (h0) {
+                    ref = rc.ref_l;
+                    cd = t0;
+                    continue;
+                } else if (h1) {
+                    ref = rc.ref_r;
+                    cd = t1;
+                    continue;
+                }
             }
         }
+        if (st.sp == 0) break;
+        st.pop(ref, cd);
     }
     if (best >= 0) {
         DTri tr = prims[best];

@@ -59,6 +59,48 @@ struct alignas(16) DSilE {  // 3D silhouette edge in reference order, 64 B
     float dx, dy, dz, p2;
 };
 
+// Child-in-parent records, one per internal node (indexed by the node's
+// preorder index): both children's boxes (and, for the SNCH, cones) plus an
+// encoded reference per child, so descending one level costs one dependent
+// fetch.  Child reference encoding (int32):
+//   >= 0                  internal child: its preorder node index
+//   bit31 | c<<27 | off   leaf with c (0..14) items starting at off (< 2^27)
+//   bit31 | 15<<27 | k    leaf described by leaf table entry k (int2 off, count)
+// BVH records reference primitive ranges; SNCH records silhouette ranges.
+#define DW_LEAF_BIT 0x80000000u
+#define DW_LEAF_TABLE 15
+struct alignas(16) RBvh2 {  // 48 B
+    float lmnx, lmny, lmxx, lmxy;
+    float rmnx, rmny, rmxx, rmxy;
+    int ref_l, ref_r, pad0, pad1;
+};
+struct alignas(16) RSnch2 {  // 96 B
+    float lmnx, lmny, lmxx, lmxy;
+    float rmnx, rmny, rmxx, rmxy;
+    float lax, lay, lha, lrad;
+    float rax, ray, rha, rrad;
+    float lsin, lcos, rsin, rcos;
+    int ref_l, ref_r, pad0, pad1;
+};
+struct alignas(16) RBvh3 {  // 64 B
+    float lmnx, lmny, lmnz, lmxx;
+    float lmxy, lmxz, rmnx, rmny;
+    float rmnz, rmxx, rmxy, rmxz;
+    int ref_l, ref_r, pad0, pad1;
+};
+struct alignas(16) RSnch3 {  // 144 B
+    float lmnx, lmny, lmnz, lmxx;
+    float lmxy, lmxz, rmnx, rmny;
+    float rmnz, rmxx, rmxy, rmxz;
+    float lax, lay, laz, lha;
+    float rax, ray, raz, rha;
+    float lrad, rrad, lsin, lcos;
+    float rsin, rcos;
+    int ref_l, ref_r;
+    float lrho, lelo, lehi, rrho;  // face-plane bounds (rho, e_lo, e_hi) per child
+    float relo, rehi, pad0, pad1;
+};
+
 // Everything one sub-scene exposes to the kernels (device pointers).
 struct SubDev {
     int present;
@@ -79,6 +121,13 @@ struct SubDev {
     const float* vnormal;      // 2D vertex normals (local vertex order), full scene only
     const int* local_vidx;     // dim local vertex ids per primitive
     int n_local_verts;
+    // child-in-parent records and the root
+    const void* rec_bvh;        // RBvh2 / RBvh3
+    const void* rec_snch;       // RSnch2 / RSnch3
+    const int* leaf_table;      // int2 (offset, count) per large leaf
+    int root_ref_bvh, root_ref_snch;
+    float root_mn[3], root_mx[3];
+    int max_depth;
 };
 
 struct SceneDev {

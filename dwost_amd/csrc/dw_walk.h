@@ -49,6 +49,23 @@ DWD float h3(const BCDev& bc, const SubDev& S, int prim, float u, float v, F3 p,
     return (hb * u + hc * v) + ha * ((1.0f - u) - v);
 }
 
+// Adjoint of the per-vertex boundary data: scatter w * d(value)/d(vertex value).
+// g2/h2: lerp(a, b, t) = a (1 - t) + b t;  g3/h3: b u + c v + a (1 - u - v).
+struct Adj {
+    float w;    // upstream dL/d(walk value)
+    float* dg;  // dL/dg per global vertex (accumulated)
+    float* dh;  // dL/dh per global vertex
+};
+DWD void adj_lin2(float* buf, const SubDev& S, int prim, float t, float w) {
+    atomicAdd(buf + S.local_vglobal[2 * prim], w * (1.0f - t));
+    atomicAdd(buf + S.local_vglobal[2 * prim + 1], w * t);
+}
+DWD void adj_lin3(float* buf, const SubDev& S, int prim, float u, float v, float w) {
+    atomicAdd(buf + S.local_vglobal[3 * prim], w * ((1.0f - u) - v));
+    atomicAdd(buf + S.local_vglobal[3 * prim + 1], w * u);
+    atomicAdd(buf + S.local_vglobal[3 * prim + 2], w * v);
+}
+
 // DiscreteDistribution.sample_reuse_pmf (distr.py:52-107) with Dr.Jit's
 // binary_search over [0, n-1] (log2(n-1)+1 halvings).
 DWD int cdf_sample(const SubDev& S, float value, float& reuse, float& pmf) {
@@ -189,8 +206,9 @@ DWD void walk_init(Walk<D>& w, typename VecT<D>::T x) {
     w.live = true;
 }
 
-template <bool X>
-DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk<2>& w, Rng& rng) {
+template <bool X, bool A = false>
+DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk<2>& w, Rng& rng,
+                   const Adj& adj = Adj{0.0f, nullptr, nullptr}) {
     const SubDev& F = sc.sub[0];
     const SubDev& Dn = sc.sub[1];
     const SubDev& Nm = sc.sub[2];
@@ -206,6 +224,7 @@ DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk
     }
     if (d_dir < P.eps) {
         w.result += g2(bc, Dn, cpd);
+        if constexpr (A) adj_lin2(adj.dg, Dn, cpd.prim, cpd.t, adj.w);
         w.term = cpd.prim;
         w.i += 1;
         w.live = false;
@@ -218,6 +237,9 @@ DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk
         R = fmn(R, sr);
     }
     F2 ref = f2(0.0f, 0.0f);
+    int cv_prim = -1;
+    float cv_t = 0.0f;
+    F2 cv_n = f2(0.0f, 0.0f);
     if (Nm.present) {
         float rnd, pmf;
         int idx = cdf_sample(Nm, rng.next(), rnd, pmf);
@@ -237,8 +259,14 @@ DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk
             float h = h2(bc, Nm, idx, rnd, bp, bn);
             if (P.cv) h = h - dot(ref, bn);
             hb = 0.0f - G * h / bpdf;
+            if constexpr (A) {
+                float k = adj.w * alpha * (G / bpdf);  // d/dh of -alpha G h / pdf is -k
+                adj_lin2(adj.dh, Nm, idx, rnd, -k);
+                if (P.cv) adj_lin2(adj.dh, F, cr.prim, cr.t, k * dot(f2(cr.n.x, cr.n.y), bn));
+            }
         }
         w.result += alpha * hb;
+        if constexpr (A) cv_prim = cr.prim, cv_t = cr.t, cv_n = f2(cr.n.x, cr.n.y);
     }
     F2 dir = uniform_dir2(rng);
     if (w.ob && (dot(dir, w.in) > 0.0f)) dir = -dir;
@@ -256,6 +284,9 @@ DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk
         w.ob = false;
     }
     if (P.cv) w.result += dot(_p - w.ip, ref);
+    if constexpr (A) {  // d/dhc of dot(_p - ip, hc n_c)
+        if (P.cv && cv_prim >= 0) adj_lin2(adj.dh, F, cv_prim, cv_t, adj.w * dot(_p - w.ip, cv_n));
+    }
     F2 sp;  // handle_source (wost.py:128-143)
     float sr_, spdf;
     greens2_sample(_p, R, dir, rng, sp, sr_, spdf);
@@ -267,8 +298,9 @@ DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk
     return w.live;
 }
 
-template <bool X>
-DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk<3>& w, Rng& rng) {
+template <bool X, bool A = false>
+DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk<3>& w, Rng& rng,
+                   const Adj& adj = Adj{0.0f, nullptr, nullptr}) {
     const SubDev& F = sc.sub[0];
     const SubDev& Dn = sc.sub[1];
     const SubDev& Nm = sc.sub[2];
@@ -284,6 +316,7 @@ DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk
     }
     if (d_dir < P.eps) {
         w.result += g3(bc, Dn, cpd);
+        if constexpr (A) adj_lin3(adj.dg, Dn, cpd.prim, cpd.u, cpd.v, adj.w);
         w.term = cpd.prim;
         w.i += 1;
         w.live = false;
@@ -312,6 +345,7 @@ DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk
         if (norm(w.ip - bp) <= R) {
             float h = h3(bc, Nm, idx, su, sv, bp, bn);
             w.result -= G * h * alpha / bpdf;
+            if constexpr (A) adj_lin3(adj.dh, Nm, idx, su, sv, -adj.w * ((G * alpha) / bpdf));
         }
     }
     F3 dir = uniform_sphere(rng);

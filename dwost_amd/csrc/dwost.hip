@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -21,6 +22,7 @@ using namespace dwd;
 namespace {
 
 constexpr int kBlock = 256;
+constexpr size_t kStackLds = (size_t)kBlock * kStackLdsBytesPerThread;  // traversal stacks
 // Minimum resident waves per SIMD for the persistent walk kernels: the
 // literal-cone kernels keep their fp64 acos/asin/atan2 without spilling at 2;
 // the transcendental-free ones run best at 4 (a few VGPRs spill).
@@ -207,17 +209,123 @@ DWD uint32_t lane_rank(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Hands new job ids to the idle lanes of the wave; returns the id or -1 (no work left).
-DWD int64_t refill(bool idle, unsigned long long* counter, int64_t total) {
+// Job queue of the persistent kernels.  A job is one walk (or walk pair) of
+// one point; its random stream is keyed by (point, walk) only, so the order in
+// which jobs run never changes a result.  Points are split into 8 contiguous
+// ranges of a spatial (Morton-cell) order, one per XCD: a wave draws jobs from
+// the range of the XCD it runs on, so each XCD's L2 holds the part of the scene
+// near its points, and moves on to the next range when its own is exhausted.
+struct JobQ {
+    unsigned long long* ctr;  // 8 counters, 128 B apart (ctr[16 * range])
+    const int* order;         // spatial position -> point index (nullptr: identity)
+    int64_t n;                // points
+    int64_t per;              // jobs per point
+};
+struct QState {
+    int part;  // current range (wave-uniform)
+    int left;  // ranges not yet found exhausted
+};
+DWD uint32_t xcc_id() {
+    uint32_t v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID, 0, 4)" : "=s"(v));
+    return v;
+}
+DWD void q_start(QState& q) {
+    q.part = (int)(xcc_id() & 7u);
+    q.left = 8;
+}
+// Hands job ids (point * per + walk) to the idle lanes of the wave: one atomic
+// per wave.  Returns the id, -1 (none this time) or -2 (all ranges exhausted).
+DWD int64_t q_next(bool idle, const JobQ& Q, QState& q) {
     uint64_t m = __ballot(idle);
     if (m == 0) return -1;
     int leader = __ffsll((long long)m) - 1;
-    unsigned long long base = 0;
-    if ((int)__lane_id() == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
-    base = __shfl(base, leader, 64);
-    if (!idle) return -1;
-    unsigned long long mine = base + lane_rank(m);
-    return mine < (unsigned long long)total ? (int64_t)mine : -2;
+    uint32_t rank = lane_rank(m);
+    while (q.left > 0) {
+        int64_t p0 = (Q.n * q.part) >> 3, p1 = (Q.n * (q.part + 1)) >> 3;
+        unsigned long long size = (unsigned long long)((p1 - p0) * Q.per);
+        unsigned long long base = 0;
+        if ((int)__lane_id() == leader) base = atomicAdd(Q.ctr + 16 * q.part, (unsigned long long)__popcll(m));
+        base = __shfl(base, leader, 64);
+        if (base < size) {
+            if (!idle) return -1;
+            unsigned long long j = base + rank;
+            if (j >= size) return -1;
+            int64_t pos = p0 + (int64_t)(j / (unsigned long long)Q.per);
+            int64_t k = (int64_t)(j % (unsigned long long)Q.per);
+            int64_t i = Q.order ? (int64_t)Q.order[pos] : pos;
+            return i * Q.per + k;
+        }
+        q.part = (q.part + 1) & 7;
+        q.left -= 1;
+    }
+    return idle ? -2 : -1;
+}
+
+// Spatial order of the query points: counting sort by a 12-bit Morton cell of
+// the scene box (16^3 cells in 3D, 64^2 in 2D).
+constexpr int kCells = 4096;
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {  // 4 bits -> every third bit
+    v &= 15u;
+    v = (v | (v << 4)) & 0x0C3u;
+    v = (v | (v << 2)) & 0x249u;
+    return v;
+}
+__device__ __forceinline__ uint32_t spread2(uint32_t v) {  // 6 bits -> every second bit
+    v &= 63u;
+    v = (v | (v << 4)) & 0x30Fu;
+    v = (v | (v << 2)) & 0x333u;
+    v = (v | (v << 1)) & 0x555u;
+    return v;
+}
+__global__ void k_cell_hist(const float* __restrict__ pts, int64_t n, int dim, float lx, float ly, float lz, float sx,
+                            float sy, float sz, int* __restrict__ cell, int* __restrict__ hist) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int nb = dim == 3 ? 16 : 64;
+    auto q = [&](float p, float lo, float sc) {
+        float t = (p - lo) * sc;
+        int c = t == t ? (int)fminf(fmaxf(t, 0.0f), (float)(nb - 1)) : 0;
+        return (uint32_t)c;
+    };
+    uint32_t c;
+    if (dim == 3)
+        c = spread3(q(pts[3 * i], lx, sx)) | (spread3(q(pts[3 * i + 1], ly, sy)) << 1) |
+            (spread3(q(pts[3 * i + 2], lz, sz)) << 2);
+    else
+        c = spread2(q(pts[2 * i], lx, sx)) | (spread2(q(pts[2 * i + 1], ly, sy)) << 1);
+    cell[i] = (int)c;
+    atomicAdd(hist + c, 1);
+}
+// Exclusive scan of kCells counts by one block of 1024 threads (4 cells each).
+__global__ void __launch_bounds__(1024) k_cell_scan(int* __restrict__ hist) {
+    __shared__ int part[1024];
+    int t = threadIdx.x;
+    int v[4], s = 0;
+    for (int k = 0; k < 4; ++k) v[k] = hist[4 * t + k], s += v[k];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int add = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    int run = part[t] - s;
+    for (int k = 0; k < 4; ++k) hist[4 * t + k] = run, run += v[k];
+}
+__global__ void k_cell_place(const int* __restrict__ cell, int64_t n, int* __restrict__ offs, int* __restrict__ order) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    order[atomicAdd(offs + cell[i], 1)] = (int)i;
+}
+__global__ void k_fill_u64(unsigned long long* p, int count, unsigned long long v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = v;
+}
+__global__ void k_fill_i32(int* p, int count, int v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = v;
 }
 
 template <int D>
@@ -235,23 +343,26 @@ DWD F3 load_pt<3>(const float* pts, int64_t i) {
 template <int D, bool X>
 __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVES_FAST) k_walk_u_pers(const SceneDev* __restrict__ scp, BCDev bc, ParamsDev P,
                                                         const float* __restrict__ pts, int64_t n, float* walk_u,
-                                                        int* walk_steps, int* walk_term,
-                                                        unsigned long long* counter) {
+                                                        int* walk_steps, int* walk_term, uint8_t* walk_flags,
+                                                        JobQ Q) {
     const SceneDev& sc = *scp;
     using V = typename VecT<D>::T;
-    const int64_t total = n * (int64_t)P.nwalks;
     int64_t job = -1;
     bool done = false;
+    QState qs;
+    q_start(qs);
     Walk<D> w;
     Rng rng;
     V x;
     float res = 0.0f;
     int steps = 0, sub = 0;
+    uint32_t kept = 0;
     while (true) {
-        int64_t got = refill(!done && job < 0, counter, total);
+        int64_t got = q_next(!done && job < 0, Q, qs);
         if (got == -2) done = true;
         if (got >= 0) {
             job = got;
+            kept = 0;
             int64_t i = job / P.nwalks;
             uint32_t a, b;
             tea4(P.seed, (uint32_t)i, a, b);
@@ -264,7 +375,9 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
         }
         if (__ballot(!done) == 0) break;
         if (job >= 0 && !walk_step<X>(sc, bc, P, w, rng)) {
-            res += walk_value(w);
+            float v = walk_value(w);
+            if (v == w.result) kept |= 1u << (sub & 7);
+            res += v;
             steps += w.i;
             if (++sub < P.nsubwalks) {
                 walk_init<D>(w, x);
@@ -272,6 +385,7 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
                 walk_u[job] = res / (float)P.nsubwalks;
                 if (walk_steps) walk_steps[job] = steps;
                 if (walk_term) walk_term[job] = w.term;
+                if (walk_flags) walk_flags[job] = (uint8_t)kept;
                 job = -1;
             }
         }
@@ -283,22 +397,24 @@ template <int D, bool X>
 __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVES_FAST) k_fused_pers(const SceneDev* __restrict__ scp, BCDev bc, ParamsDev P,
                                                        const float* __restrict__ pts, const float* __restrict__ Rp,
                                                        const int* __restrict__ insidep, int64_t n, int npairs,
-                                                       float* pair_vals, int* pair_steps,
-                                                       unsigned long long* counter) {
+                                                       float* pair_vals, int* pair_steps, uint8_t* pair_flags,
+                                                       JobQ Q) {
     const SceneDev& sc = *scp;
     using V = typename VecT<D>::T;
-    const int64_t total = n * (int64_t)npairs;
     const float fsrc = src_f(bc);
     int64_t job = -1;
     bool done = false;
+    QState qs;
+    q_start(qs);
     Walk<D> w;
     Rng rng;
     V p, dir, gs;
     float R = 0.0f, up = 0.0f;
     uint64_t saved = 0;
     int phase = 0, steps = 0;
+    bool kept_plus = false;
     while (true) {
-        int64_t got = refill(!done && job < 0, counter, total);
+        int64_t got = q_next(!done && job < 0, Q, qs);
         if (got == -2) done = true;
         if (got >= 0) {
             int64_t i = got / npairs;
@@ -306,6 +422,7 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
                 float* o = pair_vals + got * (D + 1);
                 for (int q = 0; q <= D; ++q) o[q] = 0.0f;
                 pair_steps[got] = 0;
+                if (pair_flags) pair_flags[got] = 0;
             } else {
                 job = got;
                 R = Rp[i];
@@ -335,7 +452,10 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
         if (job >= 0 && !walk_step<X>(sc, bc, P, w, rng)) {
             float v = walk_value(w);
             steps += w.i;
+            // walks the guards zero contribute nothing to the adjoint
+            bool kept = D == 2 ? v == w.result : !(bad(v) || __builtin_fabsf(v) > 10000.0f);
             if (phase == 0) {
+                kept_plus = kept;
                 up = v;
                 rng.state = saved;
                 walk_init<D>(w, p - dir * R);
@@ -363,6 +483,131 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
                     o[3] = bad(g2_) ? 0.0f : g2_;
                 }
                 pair_steps[job] = steps;
+                if (pair_flags) pair_flags[job] = (uint8_t)((kept_plus ? 1 : 0) | (kept ? 2 : 0));
+                job = -1;
+            }
+        }
+    }
+}
+
+// Adjoint of the fused estimator: replays every pair from its seed with the
+// upstream weight of each walk, dL/du+- = (dL/du[i] +- (dim/R) dL/dgrad[i].dir)
+// / (2 npairs), and scatter-adds dL/dg, dL/dh per vertex.  The walks are linear
+// in the per-vertex boundary data for fixed random numbers; walks zeroed by the
+// outlier guards in the forward pass (pair_flags) are skipped.
+template <int D, bool X>
+__global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVES_FAST)
+    k_adjoint_pers(const SceneDev* __restrict__ scp, BCDev bc, ParamsDev P, const float* __restrict__ pts,
+                   const float* __restrict__ Rp, const int* __restrict__ insidep, int64_t n, int npairs,
+                   const float* __restrict__ dl_du, const float* __restrict__ dl_dgrad,
+                   const uint8_t* __restrict__ pair_flags, float* dl_dg, float* dl_dh, JobQ Q) {
+    const SceneDev& sc = *scp;
+    using V = typename VecT<D>::T;
+    const float inv = 1.0f / (float)(2 * npairs);
+    int64_t job = -1;
+    bool done = false;
+    QState qs;
+    q_start(qs);
+    Walk<D> w;
+    Rng rng;
+    V p, dir;
+    float R = 0.0f, wm = 0.0f;
+    Adj adj{0.0f, dl_dg, dl_dh};
+    uint64_t saved = 0;
+    int phase = 0;
+    while (true) {
+        int64_t got = q_next(!done && job < 0, Q, qs);
+        if (got == -2) done = true;
+        if (got >= 0) {
+            int64_t i = got / npairs;
+            uint8_t fl = pair_flags[got];
+            if (insidep[i] && fl) {
+                job = got;
+                R = Rp[i];
+                p = load_pt<D>(pts, i);
+                uint32_t a, b;
+                tea4(P.seed, (uint32_t)i, a, b);
+                rng.seed_lane(a, b, (uint64_t)job);
+                V p_src;
+                float r_src, pdf_src;
+                float proj;
+                if constexpr (D == 2) {
+                    dir = uniform_dir2(rng);
+                    F2 dir_src = uniform_dir2(rng);
+                    greens2_sample(p, R, dir_src, rng, p_src, r_src, pdf_src);
+                    proj = dl_dgrad[2 * i] * dir.x + dl_dgrad[2 * i + 1] * dir.y;
+                } else {
+                    dir = uniform_sphere(rng);
+                    greens3_sample(p, R, dir, rng, p_src, r_src, pdf_src);
+                    proj = (dl_dgrad[3 * i] * dir.x + dl_dgrad[3 * i + 1] * dir.y) + dl_dgrad[3 * i + 2] * dir.z;
+                }
+                float k = (float)D / R * proj;
+                adj.w = (fl & 1) ? (dl_du[i] + k) * inv : 0.0f;
+                wm = (fl & 2) ? (dl_du[i] - k) * inv : 0.0f;
+                saved = rng.state;
+                walk_init<D>(w, p + dir * R);
+                phase = 0;
+            }
+        }
+        if (__ballot(!done) == 0) break;
+        if (job >= 0 && !walk_step<X, true>(sc, bc, P, w, rng, adj)) {
+            if (phase == 0) {
+                rng.state = saved;
+                walk_init<D>(w, p - dir * R);
+                adj.w = wm;
+                phase = 1;
+            } else {
+                job = -1;
+            }
+        }
+    }
+}
+
+// Adjoint of Solver.solve: job i*W + k replays its nsubwalks walks with weight
+// dL/du[i] / (W * nsubwalks) each (walks zeroed by the guard are skipped).
+template <int D, bool X>
+__global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVES_FAST)
+    k_adjoint_u_pers(const SceneDev* __restrict__ scp, BCDev bc, ParamsDev P, const float* __restrict__ pts,
+                     int64_t n, const float* __restrict__ dl_du, const uint8_t* __restrict__ walk_flags, float* dl_dg,
+                     float* dl_dh, JobQ Q) {
+    const SceneDev& sc = *scp;
+    using V = typename VecT<D>::T;
+    const float inv = 1.0f / ((float)P.nwalks * (float)P.nsubwalks);
+    int64_t job = -1;
+    bool done = false;
+    QState qs;
+    q_start(qs);
+    Walk<D> w;
+    Rng rng;
+    V x;
+    Adj adj{0.0f, dl_dg, dl_dh};
+    int sub = 0;
+    uint32_t fl = 0;
+    float wi = 0.0f;
+    while (true) {
+        int64_t got = q_next(!done && job < 0, Q, qs);
+        if (got == -2) done = true;
+        if (got >= 0) {
+            int64_t i = got / P.nwalks;
+            fl = walk_flags[got];
+            if (fl) {
+                job = got;
+                uint32_t a, b;
+                tea4(P.seed, (uint32_t)i, a, b);
+                rng.seed_lane(a, b, (uint64_t)job);
+                x = load_pt<D>(pts, i);
+                walk_init<D>(w, x);
+                sub = 0;
+                wi = dl_du[i] * inv;
+                adj.w = (fl & 1u) ? wi : 0.0f;
+            }
+        }
+        if (__ballot(!done) == 0) break;
+        if (job >= 0 && !walk_step<X, true>(sc, bc, P, w, rng, adj)) {
+            if (++sub < P.nsubwalks) {
+                walk_init<D>(w, x);
+                adj.w = (fl & (1u << (sub & 7))) ? wi : 0.0f;
+            } else {
                 job = -1;
             }
         }
@@ -594,6 +839,99 @@ int upload(dw_scene* s, const std::vector<T>& v, const void** out) {
     return DW_OK;
 }
 
+// Builds the child-in-parent records of one sub-scene (see dw_layout.h).
+struct RefEnc {
+    std::vector<int> table;  // flattened (offset, count) pairs
+    int leaf(int off, int cnt) {
+        if (cnt >= 0 && cnt < DW_LEAF_TABLE && off >= 0 && off < (1 << 27))
+            return (int)(DW_LEAF_BIT | ((unsigned)cnt << 27) | (unsigned)off);
+        int k = (int)(table.size() / 2);
+        table.push_back(off);
+        table.push_back(cnt);
+        return (int)(DW_LEAF_BIT | ((unsigned)DW_LEAF_TABLE << 27) | (unsigned)k);
+    }
+};
+
+int upload_records(dw_scene* s, const dwb::SubHost& H, SubDev& S) {
+    const int D = H.dim;
+    size_t nn = H.n_nodes();
+    RefEnc enc;
+    auto bvh_ref = [&](int i) { return H.node_nref[i] > 0 ? enc.leaf(H.node_off[i], H.node_nref[i]) : i; };
+    auto snch_ref = [&](int i) { return H.node_nref[i] > 0 ? enc.leaf(H.node_sil_off[i], H.node_nsil[i]) : i; };
+    auto box = [&](int i, int k) { return H.node_box[(size_t)i * 2 * D + k]; };
+    auto cs = [&](int i, bool sine) {
+        double ha = H.node_half_angle[i];
+        return (float)(sine ? std::sin(ha) : std::cos(ha));
+    };
+    int rc;
+    const void* p;
+    S.max_depth = H.max_depth;
+    S.root_ref_bvh = bvh_ref(0);
+    S.root_ref_snch = snch_ref(0);
+    for (int d = 0; d < D; ++d) S.root_mn[d] = box(0, d), S.root_mx[d] = box(0, D + d);
+    if (D == 2) {
+        std::vector<RBvh2> rb(nn);
+        std::vector<RSnch2> rs(nn);
+        for (size_t i = 0; i < nn; ++i) {
+            std::memset(&rb[i], 0, sizeof(RBvh2));
+            std::memset(&rs[i], 0, sizeof(RSnch2));
+            if (H.node_nref[i] > 0) continue;
+            int L = (int)i + 1, R = (int)i + H.node_off[i];
+            RBvh2& b = rb[i];
+            b.lmnx = box(L, 0), b.lmny = box(L, 1), b.lmxx = box(L, 2), b.lmxy = box(L, 3);
+            b.rmnx = box(R, 0), b.rmny = box(R, 1), b.rmxx = box(R, 2), b.rmxy = box(R, 3);
+            b.ref_l = bvh_ref(L), b.ref_r = bvh_ref(R);
+            RSnch2& c = rs[i];
+            c.lmnx = b.lmnx, c.lmny = b.lmny, c.lmxx = b.lmxx, c.lmxy = b.lmxy;
+            c.rmnx = b.rmnx, c.rmny = b.rmny, c.rmxx = b.rmxx, c.rmxy = b.rmxy;
+            c.lax = H.node_axis[2 * L], c.lay = H.node_axis[2 * L + 1];
+            c.lha = H.node_half_angle[L], c.lrad = H.node_radius[L];
+            c.rax = H.node_axis[2 * R], c.ray = H.node_axis[2 * R + 1];
+            c.rha = H.node_half_angle[R], c.rrad = H.node_radius[R];
+            c.lsin = cs(L, true), c.lcos = cs(L, false), c.rsin = cs(R, true), c.rcos = cs(R, false);
+            c.ref_l = snch_ref(L), c.ref_r = snch_ref(R);
+        }
+        if ((rc = upload(s, rb, &p))) return rc;
+        S.rec_bvh = p;
+        if ((rc = upload(s, rs, &p))) return rc;
+        S.rec_snch = p;
+    } else {
+        std::vector<RBvh3> rb(nn);
+        std::vector<RSnch3> rs(nn);
+        for (size_t i = 0; i < nn; ++i) {
+            std::memset(&rb[i], 0, sizeof(RBvh3));
+            std::memset(&rs[i], 0, sizeof(RSnch3));
+            if (H.node_nref[i] > 0) continue;
+            int L = (int)i + 1, R = (int)i + H.node_off[i];
+            float v[12] = {box(L, 0), box(L, 1), box(L, 2), box(L, 3), box(L, 4), box(L, 5),
+                           box(R, 0), box(R, 1), box(R, 2), box(R, 3), box(R, 4), box(R, 5)};
+            RBvh3& b = rb[i];
+            std::memcpy(&b, v, sizeof(v));
+            b.ref_l = bvh_ref(L), b.ref_r = bvh_ref(R);
+            RSnch3& c = rs[i];
+            std::memcpy(&c, v, sizeof(v));
+            c.lax = H.node_axis[3 * L], c.lay = H.node_axis[3 * L + 1], c.laz = H.node_axis[3 * L + 2];
+            c.lha = H.node_half_angle[L];
+            c.rax = H.node_axis[3 * R], c.ray = H.node_axis[3 * R + 1], c.raz = H.node_axis[3 * R + 2];
+            c.rha = H.node_half_angle[R];
+            c.lrad = H.node_radius[L], c.rrad = H.node_radius[R];
+            c.lsin = cs(L, true), c.lcos = cs(L, false), c.rsin = cs(R, true), c.rcos = cs(R, false);
+            c.ref_l = snch_ref(L), c.ref_r = snch_ref(R);
+            c.lrho = H.node_pb[3 * L], c.lelo = H.node_pb[3 * L + 1], c.lehi = H.node_pb[3 * L + 2];
+            c.rrho = H.node_pb[3 * R], c.relo = H.node_pb[3 * R + 1], c.rehi = H.node_pb[3 * R + 2];
+        }
+        if ((rc = upload(s, rb, &p))) return rc;
+        S.rec_bvh = p;
+        if ((rc = upload(s, rs, &p))) return rc;
+        S.rec_snch = p;
+    }
+    enc.table.push_back(0);  // never empty
+    enc.table.push_back(0);
+    if ((rc = upload(s, enc.table, &p))) return rc;
+    S.leaf_table = (const int*)p;
+    return DW_OK;
+}
+
 int upload_sub(dw_scene* s, const dwb::SubHost& H, SubDev& S) {
     std::memset(&S, 0, sizeof(S));
     S.present = H.present ? 1 : 0;
@@ -678,6 +1016,7 @@ int upload_sub(dw_scene* s, const dwb::SubHost& H, SubDev& S) {
         if ((rc = upload(s, prims, &S.prims))) return rc;
         if ((rc = upload(s, sils, &S.sils))) return rc;
     }
+    if ((rc = upload_records(s, H, S))) return rc;
     if ((rc = upload(s, H.prim_order, &p))) return rc;
     S.prim_index = (const int*)p;
     if ((rc = upload(s, H.cdf, &p))) return rc;
@@ -753,44 +1092,82 @@ int check_params(const dw_solver_params* p) {
     return DW_OK;
 }
 
-// Grow-only device workspace for per-walk / per-pair partial results.
-struct Workspace {
-    std::mutex mu;
-    void* ptr = nullptr;
-    size_t bytes = 0;
-    int device = -1;
+// Per (device, stream) state: the job-queue counters and a grow-only workspace
+// for partial results and the spatial order.  Launches on different streams
+// never share either; launches on one stream are ordered by the stream.
+struct StreamCtx {
+    unsigned long long* ctr = nullptr;  // 8 range counters, 128 B apart
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
 };
-Workspace g_ws;
+std::mutex g_ctx_mu;
+std::map<std::pair<int, hipStream_t>, StreamCtx> g_ctx;
+constexpr int kQueueCtrs = 8 * 16;  // u64 slots
 
-// One device counter per device for the persistent kernels' job queue.
-unsigned long long* g_counter[64] = {nullptr};
-std::mutex g_counter_mu;
-
-int counter_slot(unsigned long long** out) {
+int stream_ctx(hipStream_t st, StreamCtx** out) {
     int dev = 0;
     DW_HIP(hipGetDevice(&dev));
-    if (dev < 0 || dev >= 64) return fail(DW_ERR_ARG, "device index out of range");
-    std::lock_guard<std::mutex> lk(g_counter_mu);
-    if (!g_counter[dev]) DW_HIP(hipMalloc((void**)&g_counter[dev], 256));
-    *out = g_counter[dev];
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    StreamCtx& c = g_ctx[std::make_pair(dev, st)];
+    if (!c.ctr) DW_HIP(hipMalloc((void**)&c.ctr, kQueueCtrs * sizeof(unsigned long long)));
+    *out = &c;
     return DW_OK;
 }
 
-int workspace(size_t bytes, void** out) {
-    int dev = 0;
-    DW_HIP(hipGetDevice(&dev));
-    if (g_ws.ptr && (g_ws.bytes < bytes || g_ws.device != dev)) {
-        DW_HIP(hipDeviceSynchronize());
-        DW_HIP(hipFree(g_ws.ptr));
-        g_ws.ptr = nullptr;
-        g_ws.bytes = 0;
+int workspace(StreamCtx& c, size_t bytes, hipStream_t st, void** out) {
+    if (c.ws && c.ws_bytes < bytes) {
+        DW_HIP(hipStreamSynchronize(st));  // earlier launches on this stream may still read it
+        DW_HIP(hipFree(c.ws));
+        c.ws = nullptr;
+        c.ws_bytes = 0;
     }
-    if (!g_ws.ptr) {
-        DW_HIP(hipMalloc(&g_ws.ptr, bytes));
-        g_ws.bytes = bytes;
-        g_ws.device = dev;
+    if (!c.ws) {
+        bytes = bytes < 4096 ? 4096 : bytes;
+        DW_HIP(hipMalloc(&c.ws, bytes));
+        c.ws_bytes = bytes;
     }
-    *out = g_ws.ptr;
+    *out = c.ws;
+    return DW_OK;
+}
+
+// Bytes of workspace the spatial order of n points needs (0: identity order).
+size_t order_bytes(int64_t n) {
+    const char* e = getenv("DWOST_SPATIAL_ORDER");
+    if ((e && atoi(e) == 0) || n < 2 * kCells || n >= (int64_t)1 << 31) return 0;
+    return (size_t)n * 8 + kCells * 4;
+}
+
+// Builds the job queue of a persistent launch: zeroes the range counters (a
+// fill kernel, so the sequence can be captured in a graph) and, when
+// order_bytes(n) > 0, sorts the points by Morton cell into `ows`.
+int make_queue(const dw_scene* s, StreamCtx& c, const float* pts, int64_t n, int64_t per, void* ows, hipStream_t st,
+               JobQ& Q) {
+    Q.ctr = c.ctr;
+    Q.order = nullptr;
+    Q.n = n;
+    Q.per = per;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kQueueCtrs), 0, st, c.ctr, kQueueCtrs, 0ull);
+    DW_HIP(hipGetLastError());
+    if (!ows) return DW_OK;
+    const dwb::SubHost& F = s->host.sub[0];
+    const int D = s->dev.dim;
+    const float nb = D == 3 ? 16.0f : 64.0f;
+    float lo[3] = {0, 0, 0}, sc[3] = {0, 0, 0};
+    for (int d = 0; d < D; ++d) {
+        float ext = F.bb_max[d] - F.bb_min[d];
+        lo[d] = F.bb_min[d];
+        sc[d] = ext > 0.0f ? nb / ext : 0.0f;
+    }
+    int* order = (int*)ows;
+    int* cell = order + n;
+    int* hist = cell + n;
+    hipLaunchKernelGGL(k_fill_i32, dim3(kCells / 256), dim3(256), 0, st, hist, kCells, 0);
+    hipLaunchKernelGGL(k_cell_hist, dim3(blocks_for(n)), dim3(kBlock), 0, st, pts, n, D, lo[0], lo[1], lo[2], sc[0],
+                       sc[1], sc[2], cell, hist);
+    hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, st, hist);
+    hipLaunchKernelGGL(k_cell_place, dim3(blocks_for(n)), dim3(kBlock), 0, st, cell, n, hist, order);
+    DW_HIP(hipGetLastError());
+    Q.order = order;
     return DW_OK;
 }
 
@@ -801,10 +1178,32 @@ int persistent_grid(K kernel, unsigned& blocks) {
     int dev = 0, cus = 0, per_cu = 0;
     DW_HIP(hipGetDevice(&dev));
     DW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    DW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0));
+    DW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, kStackLds));
     if (const char* e = getenv("DWOST_BLOCKS_PER_CU")) per_cu = atoi(e);
     if (per_cu < 1) per_cu = 1;
     blocks = (unsigned)(per_cu * cus);
+    return DW_OK;
+}
+
+// Launches the exact- or fast-cone instance of a persistent kernel on its grid.
+template <typename KE, typename KF, typename... A>
+int launch_pers(bool exact, KE ke, KF kf, hipStream_t st, A... args) {
+    unsigned blocks = 0;
+    int rc;
+    if (exact) {
+        if ((rc = persistent_grid(ke, blocks))) return rc;
+        hipLaunchKernelGGL(ke, dim3(blocks), dim3(kBlock), kStackLds, st, args...);
+    } else {
+        if ((rc = persistent_grid(kf, blocks))) return rc;
+        hipLaunchKernelGGL(kf, dim3(blocks), dim3(kBlock), kStackLds, st, args...);
+    }
+    DW_HIP(hipGetLastError());
+    return DW_OK;
+}
+
+int zero_u64(unsigned long long* p, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, p, 1, 0ull);
+    DW_HIP(hipGetLastError());
     return DW_OK;
 }
 
@@ -923,7 +1322,7 @@ int dw_closest_point(const dw_scene* s, int32_t sub, const float* pts, int64_t n
     if (rc) return rc;
     if (n < 0 || (n > 0 && !pts)) return fail(DW_ERR_ARG, "bad points");
     if (n == 0) return DW_OK;
-    hipLaunchKernelGGL(k_closest, dim3(blocks_for(n)), dim3(kBlock), 0, (hipStream_t)stream, s->dev, sub, pts, n,
+    hipLaunchKernelGGL(k_closest, dim3(blocks_for(n)), dim3(kBlock), kStackLds, (hipStream_t)stream, s->dev, sub, pts, n,
                        prim, dist, point, normal, tuv);
     DW_HIP(hipGetLastError());
     return DW_OK;
@@ -935,7 +1334,7 @@ int dw_intersect(const dw_scene* s, int32_t sub, const float* origin, const floa
     if (rc) return rc;
     if (n < 0 || (n > 0 && (!origin || !dir || !rmax))) return fail(DW_ERR_ARG, "bad ray arrays");
     if (n == 0) return DW_OK;
-    hipLaunchKernelGGL(k_intersect, dim3(blocks_for(n)), dim3(kBlock), 0, (hipStream_t)stream, s->dev, sub, origin,
+    hipLaunchKernelGGL(k_intersect, dim3(blocks_for(n)), dim3(kBlock), kStackLds, (hipStream_t)stream, s->dev, sub, origin,
                        dir, rmax, n, prim, dist, point, normal, tuv);
     DW_HIP(hipGetLastError());
     return DW_OK;
@@ -947,7 +1346,7 @@ int dw_closest_silhouette(const dw_scene* s, int32_t sub, const float* pts, cons
     if (rc) return rc;
     if (n < 0 || (n > 0 && (!pts || !rmax))) return fail(DW_ERR_ARG, "bad arrays");
     if (n == 0) return DW_OK;
-    hipLaunchKernelGGL(k_closest_sil, dim3(blocks_for(n)), dim3(kBlock), 0, (hipStream_t)stream, s->dev, sub, pts,
+    hipLaunchKernelGGL(k_closest_sil, dim3(blocks_for(n)), dim3(kBlock), kStackLds, (hipStream_t)stream, s->dev, sub, pts,
                        rmax, n, sil, dist, point);
     DW_HIP(hipGetLastError());
     return DW_OK;
@@ -959,7 +1358,7 @@ int dw_star_radius(const dw_scene* s, int32_t sub, const float* pts, const float
     if (rc) return rc;
     if (n < 0 || (n > 0 && (!pts || !rmax || !radius))) return fail(DW_ERR_ARG, "bad arrays");
     if (n == 0) return DW_OK;
-    hipLaunchKernelGGL(k_star_radius, dim3(blocks_for(n)), dim3(kBlock), 0, (hipStream_t)stream, s->dev, sub, pts,
+    hipLaunchKernelGGL(k_star_radius, dim3(blocks_for(n)), dim3(kBlock), kStackLds, (hipStream_t)stream, s->dev, sub, pts,
                        rmax, n, radius);
     DW_HIP(hipGetLastError());
     return DW_OK;
@@ -967,7 +1366,7 @@ int dw_star_radius(const dw_scene* s, int32_t sub, const float* pts, const float
 
 int dw_solve_u(const dw_scene* s, const float* pts, int64_t n, const dw_solver_params* params,
                const dw_boundary* boundary, float* u, float* walk_u, int32_t* walk_steps, int32_t* walk_term,
-               uint64_t* total_steps, void* stream) {
+               uint8_t* walk_flags, uint64_t* total_steps, void* stream) {
     int rc;
     if ((rc = check_scene(s, 0))) return rc;
     if ((rc = check_params(params)) || (rc = check_bc(boundary))) return rc;
@@ -975,46 +1374,38 @@ int dw_solve_u(const dw_scene* s, const float* pts, int64_t n, const dw_solver_p
     if (n == 0) return DW_OK;
     if (!s->dev.sub[1].present && params->nsteps > 4096)
         return fail(DW_ERR_ARG, "scene without Dirichlet boundary: nsteps must be <= 4096");
-    int64_t lanes = n * (int64_t)params->nwalks;
-    float* wv = walk_u;
-    int32_t* ws = walk_steps;
-    if (!wv || (total_steps && !ws)) {
-        void* p;
-        if ((rc = workspace((size_t)lanes * 8, &p))) return rc;
-        wv = (float*)p;
-        ws = walk_steps ? walk_steps : (int32_t*)((char*)p + (size_t)lanes * 4);
-        if (walk_u) wv = walk_u;
-    }
+    if (walk_flags && params->nsubwalks > 8) return fail(DW_ERR_ARG, "walk_flags support at most 8 subwalks");
     hipStream_t st = (hipStream_t)stream;
-    if (use_simple_kernels()) {
-        hipLaunchKernelGGL(k_walk_u, dim3(blocks_for(lanes)), dim3(kBlock), 0, st, s->dev, to_bc(boundary),
+    StreamCtx* c;
+    if ((rc = stream_ctx(st, &c))) return rc;
+    const bool simple = use_simple_kernels() && !walk_flags;
+    int64_t lanes = n * (int64_t)params->nwalks;
+    size_t obytes = simple ? 0 : order_bytes(n);
+    size_t wbytes = (walk_u ? 0 : (size_t)lanes * 4) + ((total_steps && !walk_steps) ? (size_t)lanes * 4 : 0);
+    void* p;
+    if ((rc = workspace(*c, wbytes + obytes, st, &p))) return rc;
+    char* cur = (char*)p;
+    float* wv = walk_u;
+    if (!wv) wv = (float*)cur, cur += (size_t)lanes * 4;
+    int32_t* ws = walk_steps;
+    if (!ws && total_steps) ws = (int32_t*)cur, cur += (size_t)lanes * 4;
+    if (simple) {
+        hipLaunchKernelGGL(k_walk_u, dim3(blocks_for(lanes)), dim3(kBlock), kStackLds, st, s->dev, to_bc(boundary),
                            to_params(params), pts, n, wv, ws, walk_term);
+        DW_HIP(hipGetLastError());
     } else {
-        unsigned long long* counter = nullptr;
+        JobQ Q;
+        if ((rc = make_queue(s, *c, pts, n, params->nwalks, obytes ? cur : nullptr, st, Q))) return rc;
         const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
-        if ((rc = counter_slot(&counter))) return rc;
-        DW_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned long long), st));
-        unsigned blocks;
-        if (s->dev.dim == 2) {
-            if (exact) {
-                if ((rc = persistent_grid(k_walk_u_pers<2, true>, blocks))) return rc;
-            } else {
-                if ((rc = persistent_grid(k_walk_u_pers<2, false>, blocks))) return rc;
-            }
-            hipLaunchKernelGGL((exact ? k_walk_u_pers<2, true> : k_walk_u_pers<2, false>), dim3(blocks), dim3(kBlock), 0, st, s->dev_ptr, to_bc(boundary),
-                               to_params(params), pts, n, wv, ws, walk_term, counter);
-        } else {
-            if (exact) {
-                if ((rc = persistent_grid(k_walk_u_pers<3, true>, blocks))) return rc;
-            } else {
-                if ((rc = persistent_grid(k_walk_u_pers<3, false>, blocks))) return rc;
-            }
-            hipLaunchKernelGGL((exact ? k_walk_u_pers<3, true> : k_walk_u_pers<3, false>), dim3(blocks), dim3(kBlock), 0, st, s->dev_ptr, to_bc(boundary),
-                               to_params(params), pts, n, wv, ws, walk_term, counter);
-        }
+        if (s->dev.dim == 2)
+            rc = launch_pers(exact, k_walk_u_pers<2, true>, k_walk_u_pers<2, false>, st, s->dev_ptr, to_bc(boundary),
+                             to_params(params), pts, n, wv, ws, walk_term, walk_flags, Q);
+        else
+            rc = launch_pers(exact, k_walk_u_pers<3, true>, k_walk_u_pers<3, false>, st, s->dev_ptr, to_bc(boundary),
+                             to_params(params), pts, n, wv, ws, walk_term, walk_flags, Q);
+        if (rc) return rc;
     }
-    DW_HIP(hipGetLastError());
-    if (total_steps) DW_HIP(hipMemsetAsync(total_steps, 0, sizeof(uint64_t), st));
+    if (total_steps && (rc = zero_u64((unsigned long long*)total_steps, st))) return rc;
     hipLaunchKernelGGL(k_reduce_mean, dim3(blocks_for(n * 64)), dim3(kBlock), 0, st, wv, n, params->nwalks, 1,
                        1.0f / (float)params->nwalks, u, (float*)nullptr, total_steps ? ws : nullptr,
                        (unsigned long long*)total_steps);
@@ -1029,15 +1420,17 @@ int dw_solve_dude(const dw_scene* s, const float* pts, const float* e, int64_t n
     if ((rc = check_params(params)) || (rc = check_bc(boundary))) return rc;
     if (n < 0 || (n > 0 && (!pts || !e || !out))) return fail(DW_ERR_ARG, "bad arrays");
     if (n == 0) return DW_OK;
+    hipStream_t st = (hipStream_t)stream;
+    StreamCtx* c;
+    if ((rc = stream_ctx(st, &c))) return rc;
     void* p;
-    if ((rc = workspace((size_t)n * 8, &p))) return rc;
+    if ((rc = workspace(*c, (size_t)n * 8, st, &p))) return rc;
     float* R = (float*)p;
     int* ins = (int*)((char*)p + (size_t)n * 4);
-    hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(n)), dim3(kBlock), 0, st, s->dev, pts, n, R, ins);
+    hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(n)), dim3(kBlock), kStackLds, st, s->dev, pts, n, R, ins);
     DW_HIP(hipGetLastError());
-    if (total_steps) DW_HIP(hipMemsetAsync(total_steps, 0, sizeof(uint64_t), st));
-    hipLaunchKernelGGL(k_dude, dim3(blocks_for(n)), dim3(kBlock), 0, st, s->dev, to_bc(boundary), to_params(params),
+    if (total_steps && (rc = zero_u64((unsigned long long*)total_steps, st))) return rc;
+    hipLaunchKernelGGL(k_dude, dim3(blocks_for(n)), dim3(kBlock), kStackLds, st, s->dev, to_bc(boundary), to_params(params),
                        pts, e, R, ins, n, out, (unsigned long long*)total_steps);
     DW_HIP(hipGetLastError());
     return DW_OK;
@@ -1045,18 +1438,24 @@ int dw_solve_dude(const dw_scene* s, const float* pts, const float* e, int64_t n
 
 int dw_solve_fused(const dw_scene* s, const float* pts, int64_t n, const dw_solver_params* params,
                    const dw_boundary* boundary, float* u, float* grad, float* pair_vals, int32_t* pair_steps,
-                   uint64_t* total_steps, void* stream) {
+                   uint8_t* pair_flags, uint64_t* total_steps, void* stream) {
     int rc;
     if ((rc = check_scene(s, 0))) return rc;
     if ((rc = check_params(params)) || (rc = check_bc(boundary))) return rc;
     if (n < 0 || (n > 0 && (!pts || !u || !grad))) return fail(DW_ERR_ARG, "bad arrays");
     if (n == 0) return DW_OK;
-    int D = s->dev.dim;
-    int npairs = (params->nwalks + 1) / 2;
-    int64_t lanes = n * (int64_t)npairs;
-    size_t need = (size_t)n * 8 + (pair_vals ? 0 : (size_t)lanes * (D + 1) * 4) + (pair_steps ? 0 : (size_t)lanes * 4);
+    const int D = s->dev.dim;
+    const int npairs = (params->nwalks + 1) / 2;
+    const int64_t lanes = n * (int64_t)npairs;
+    hipStream_t st = (hipStream_t)stream;
+    StreamCtx* c;
+    if ((rc = stream_ctx(st, &c))) return rc;
+    const bool simple = use_simple_kernels();
+    size_t obytes = simple ? 0 : order_bytes(n);
+    size_t need = (size_t)n * 8 + (pair_vals ? 0 : (size_t)lanes * (D + 1) * 4) +
+                  (pair_steps ? 0 : (size_t)lanes * 4) + obytes;
     void* p;
-    if ((rc = workspace(need, &p))) return rc;
+    if ((rc = workspace(*c, need, st, &p))) return rc;
     char* cur = (char*)p;
     float* R = (float*)cur;
     cur += (size_t)n * 4;
@@ -1065,45 +1464,92 @@ int dw_solve_fused(const dw_scene* s, const float* pts, int64_t n, const dw_solv
     float* pv = pair_vals;
     if (!pv) pv = (float*)cur, cur += (size_t)lanes * (D + 1) * 4;
     int32_t* ps = pair_steps;
-    if (!ps) ps = (int32_t*)cur;
-    hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(n)), dim3(kBlock), 0, st, s->dev, pts, n, R, ins);
+    if (!ps) ps = (int32_t*)cur, cur += (size_t)lanes * 4;
+    hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(n)), dim3(kBlock), kStackLds, st, s->dev, pts, n, R, ins);
     DW_HIP(hipGetLastError());
-    if (use_simple_kernels()) {
-        hipLaunchKernelGGL(k_fused, dim3(blocks_for(lanes)), dim3(kBlock), 0, st, s->dev, to_bc(boundary),
+    if (simple) {
+        hipLaunchKernelGGL(k_fused, dim3(blocks_for(lanes)), dim3(kBlock), kStackLds, st, s->dev, to_bc(boundary),
                            to_params(params), pts, R, ins, n, npairs, pv, ps);
+        DW_HIP(hipGetLastError());
     } else {
-        unsigned long long* counter = nullptr;
+        JobQ Q;
+        if ((rc = make_queue(s, *c, pts, n, npairs, obytes ? cur : nullptr, st, Q))) return rc;
         const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
-        if ((rc = counter_slot(&counter))) return rc;
-        DW_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned long long), st));
-        unsigned blocks;
-        if (D == 2) {
-            if (exact) {
-                if ((rc = persistent_grid(k_fused_pers<2, true>, blocks))) return rc;
-            } else {
-                if ((rc = persistent_grid(k_fused_pers<2, false>, blocks))) return rc;
-            }
-            hipLaunchKernelGGL((exact ? k_fused_pers<2, true> : k_fused_pers<2, false>), dim3(blocks), dim3(kBlock), 0, st, s->dev_ptr, to_bc(boundary),
-                               to_params(params), pts, R, ins, n, npairs, pv, ps, counter);
-        } else {
-            if (exact) {
-                if ((rc = persistent_grid(k_fused_pers<3, true>, blocks))) return rc;
-            } else {
-                if ((rc = persistent_grid(k_fused_pers<3, false>, blocks))) return rc;
-            }
-            hipLaunchKernelGGL((exact ? k_fused_pers<3, true> : k_fused_pers<3, false>), dim3(blocks), dim3(kBlock), 0, st, s->dev_ptr, to_bc(boundary),
-                               to_params(params), pts, R, ins, n, npairs, pv, ps, counter);
-        }
+        if (D == 2)
+            rc = launch_pers(exact, k_fused_pers<2, true>, k_fused_pers<2, false>, st, s->dev_ptr, to_bc(boundary),
+                             to_params(params), pts, R, ins, n, npairs, pv, ps, pair_flags, Q);
+        else
+            rc = launch_pers(exact, k_fused_pers<3, true>, k_fused_pers<3, false>, st, s->dev_ptr, to_bc(boundary),
+                             to_params(params), pts, R, ins, n, npairs, pv, ps, pair_flags, Q);
+        if (rc) return rc;
     }
-    DW_HIP(hipGetLastError());
-    if (total_steps) DW_HIP(hipMemsetAsync(total_steps, 0, sizeof(uint64_t), st));
+    if (total_steps && (rc = zero_u64((unsigned long long*)total_steps, st))) return rc;
     // u = sum(usum) / (2 npairs); grad = sum(g) / (2 npairs)
     float scale = 1.0f / (float)(2 * npairs);
     hipLaunchKernelGGL(k_reduce_mean, dim3(blocks_for(n * 64)), dim3(kBlock), 0, st, pv, n, npairs, D + 1, scale, u,
                        grad, total_steps ? ps : nullptr, (unsigned long long*)total_steps);
     DW_HIP(hipGetLastError());
     return DW_OK;
+}
+
+int dw_adjoint_fused(const dw_scene* s, const float* pts, int64_t n, const dw_solver_params* params,
+                     const dw_boundary* boundary, const float* dl_du, const float* dl_dgrad, const uint8_t* pair_flags,
+                     float* dl_dg, float* dl_dh, void* stream) {
+    int rc;
+    if ((rc = check_scene(s, 0))) return rc;
+    if ((rc = check_params(params)) || (rc = check_bc(boundary))) return rc;
+    if (boundary->bc_code != DW_BC_PER_VERTEX)
+        return fail(DW_ERR_ARG, "the adjoint needs per-vertex boundary data (DW_BC_PER_VERTEX)");
+    if (n < 0 || (n > 0 && (!pts || !dl_du || !dl_dgrad || !pair_flags || !dl_dg || !dl_dh)))
+        return fail(DW_ERR_ARG, "bad arrays");
+    if (n == 0) return DW_OK;
+    const int D = s->dev.dim;
+    const int npairs = (params->nwalks + 1) / 2;
+    hipStream_t st = (hipStream_t)stream;
+    StreamCtx* c;
+    if ((rc = stream_ctx(st, &c))) return rc;
+    size_t obytes = order_bytes(n);
+    void* p;
+    if ((rc = workspace(*c, (size_t)n * 8 + obytes, st, &p))) return rc;
+    float* R = (float*)p;
+    int* ins = (int*)((char*)p + (size_t)n * 4);
+    hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(n)), dim3(kBlock), kStackLds, st, s->dev, pts, n, R, ins);
+    DW_HIP(hipGetLastError());
+    JobQ Q;
+    if ((rc = make_queue(s, *c, pts, n, npairs, obytes ? (char*)p + (size_t)n * 8 : nullptr, st, Q))) return rc;
+    const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
+    if (D == 2)
+        return launch_pers(exact, k_adjoint_pers<2, true>, k_adjoint_pers<2, false>, st, s->dev_ptr, to_bc(boundary),
+                           to_params(params), pts, R, ins, n, npairs, dl_du, dl_dgrad, pair_flags, dl_dg, dl_dh, Q);
+    return launch_pers(exact, k_adjoint_pers<3, true>, k_adjoint_pers<3, false>, st, s->dev_ptr, to_bc(boundary),
+                       to_params(params), pts, R, ins, n, npairs, dl_du, dl_dgrad, pair_flags, dl_dg, dl_dh, Q);
+}
+
+int dw_adjoint_u(const dw_scene* s, const float* pts, int64_t n, const dw_solver_params* params,
+                 const dw_boundary* boundary, const float* dl_du, const uint8_t* walk_flags, float* dl_dg,
+                 float* dl_dh, void* stream) {
+    int rc;
+    if ((rc = check_scene(s, 0))) return rc;
+    if ((rc = check_params(params)) || (rc = check_bc(boundary))) return rc;
+    if (boundary->bc_code != DW_BC_PER_VERTEX)
+        return fail(DW_ERR_ARG, "the adjoint needs per-vertex boundary data (DW_BC_PER_VERTEX)");
+    if (params->nsubwalks > 8) return fail(DW_ERR_ARG, "the adjoint supports at most 8 subwalks");
+    if (n < 0 || (n > 0 && (!pts || !dl_du || !walk_flags || !dl_dg || !dl_dh))) return fail(DW_ERR_ARG, "bad arrays");
+    if (n == 0) return DW_OK;
+    hipStream_t st = (hipStream_t)stream;
+    StreamCtx* c;
+    if ((rc = stream_ctx(st, &c))) return rc;
+    size_t obytes = order_bytes(n);
+    void* p = nullptr;
+    if (obytes && (rc = workspace(*c, obytes, st, &p))) return rc;
+    JobQ Q;
+    if ((rc = make_queue(s, *c, pts, n, params->nwalks, p, st, Q))) return rc;
+    const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
+    if (s->dev.dim == 2)
+        return launch_pers(exact, k_adjoint_u_pers<2, true>, k_adjoint_u_pers<2, false>, st, s->dev_ptr,
+                           to_bc(boundary), to_params(params), pts, n, dl_du, walk_flags, dl_dg, dl_dh, Q);
+    return launch_pers(exact, k_adjoint_u_pers<3, true>, k_adjoint_u_pers<3, false>, st, s->dev_ptr, to_bc(boundary),
+                       to_params(params), pts, n, dl_du, walk_flags, dl_dg, dl_dh, Q);
 }
 
 }  // extern "C"

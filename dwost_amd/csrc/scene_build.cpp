@@ -331,6 +331,105 @@ void build_snch(SubHost& S, const std::vector<int>& conn) {
     }
 }
 
+// Face-plane bound of every 3D SNCH node.  An edge (a, b) with faces (a, b, c)
+// and (a, d, b) is a silhouette for x iff the two face planes see x on
+// opposite sides (silhouette_edge.py:8-13: the view vector from the closest
+// point on the edge is dotted with both face normals).  With v = x - o,
+//   s_f(x) = n_f . (x - a_f) in [c.v - rho |v| - e_hi, c.v + rho |v| - e_lo],
+// so when that interval excludes 0 every face sees x on the same side and the
+// subtree holds no silhouette for x.  The test only removes nodes without
+// silhouettes, so query results (distances and ids) are unchanged.
+//
+// Error terms (eps = 2^-24), folded into rho and e: the fp32 test evaluates
+// dot(pt - x, normalize(cross(u, v))); the computed unit normal is within
+// (16 / sin(theta) + 8) eps of the exact one (theta the angle between u and v),
+// pt is within 4 eps |a| of the edge, and |pt - x| <= |v| + ext with ext the
+// largest distance from o to a vertex of the node's edges.  The fp32
+// evaluation of the bound itself adds a few eps (|v| + |e| + |o|).
+void face_bounds3(SubHost& S) {
+    const double eps = std::ldexp(1.0, -24);
+    size_t nn = S.n_nodes();
+    S.node_pb.assign(nn * 3, 0.0f);
+    std::vector<int> end(nn);
+    for (size_t k = nn; k-- > 0;)
+        end[k] = S.node_nref[k] > 0 ? (int)k + 1 : end[k + S.node_off[k]];
+    // per-edge face data in double: unit normals, sin(theta), vertices
+    size_t ne = S.sil_idx.size() / 4;
+    std::vector<double> fn(ne * 6), fsin(ne * 2);
+    std::vector<char> open(ne);
+    for (size_t e = 0; e < ne; ++e) {
+        const float* p = &S.sil_pts[12 * e];
+        const int* idx = &S.sil_idx[4 * e];
+        open[e] = idx[0] < 0 || idx[3] < 0;
+        if (open[e]) continue;
+        for (int f = 0; f < 2; ++f) {
+            double u[3], v[3], n[3];
+            for (int k = 0; k < 3; ++k) {
+                u[k] = f == 0 ? (double)p[3 + k] - p[k] : (double)p[9 + k] - p[k];
+                v[k] = f == 0 ? (double)p[6 + k] - p[k] : (double)p[3 + k] - p[k];
+            }
+            n[0] = u[1] * v[2] - u[2] * v[1];
+            n[1] = u[2] * v[0] - u[0] * v[2];
+            n[2] = u[0] * v[1] - u[1] * v[0];
+            double ln = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            double lu = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            double lv = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            fsin[2 * e + f] = (lu > 0 && lv > 0) ? ln / (lu * lv) : 0.0;
+            for (int k = 0; k < 3; ++k) fn[6 * e + 3 * f + k] = ln > 0 ? n[k] / ln : 0.0;
+        }
+    }
+    for (size_t s = 0; s < nn; ++s) {
+        float* out = &S.node_pb[3 * s];
+        out[0] = INFINITY, out[1] = -INFINITY, out[2] = INFINITY;
+        if (!(S.node_half_angle[s] >= 0.0f)) {  // no silhouettes below: always pruned
+            out[0] = 0.0f, out[1] = INFINITY, out[2] = -INFINITY;
+            continue;
+        }
+        float o[3];
+        for (int k = 0; k < 3; ++k) o[k] = (S.node_box[6 * s + k] + S.node_box[6 * s + 3 + k]) / 2.0f;
+        double c[3] = {S.node_axis[3 * s], S.node_axis[3 * s + 1], S.node_axis[3 * s + 2]};
+        double rho = 0.0, elo = INFINITY, ehi = -INFINITY, ext = 0.0, amax = 0.0, dn = 0.0;
+        bool ok = true;
+        for (int i = (int)s; i < end[s] && ok; ++i)
+            for (int j = 0; j < S.node_nsil[i] && ok; ++j) {
+                int e = S.refs[S.node_sil_off[i] + j];
+                if (open[e] || fsin[2 * e] <= 0.0 || fsin[2 * e + 1] <= 0.0) {
+                    ok = false;
+                    break;
+                }
+                const float* p = &S.sil_pts[12 * (size_t)e];
+                for (int q = 0; q < 2; ++q) {  // the edge's endpoints a, b
+                    double r2 = 0.0, a2 = 0.0;
+                    for (int k = 0; k < 3; ++k) {
+                        double d = (double)p[3 * q + k] - o[k];
+                        r2 += d * d;
+                        a2 += (double)p[3 * q + k] * p[3 * q + k];
+                    }
+                    ext = std::max(ext, std::sqrt(r2));
+                    amax = std::max(amax, std::sqrt(a2));
+                }
+                for (int f = 0; f < 2; ++f) {
+                    const double* n = &fn[6 * e + 3 * f];
+                    double dc[3] = {n[0] - c[0], n[1] - c[1], n[2] - c[2]};
+                    rho = std::max(rho, std::sqrt(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]));
+                    double ef = n[0] * ((double)p[0] - o[0]) + n[1] * ((double)p[1] - o[1]) + n[2] * ((double)p[2] - o[2]);
+                    elo = std::min(elo, ef);
+                    ehi = std::max(ehi, ef);
+                    dn = std::max(dn, (16.0 / fsin[2 * e + f] + 8.0) * eps);
+                }
+            }
+        if (!ok || !(ehi >= elo)) continue;
+        double onorm = std::sqrt((double)o[0] * o[0] + (double)o[1] * o[1] + (double)o[2] * o[2]);
+        double A = dn + 16.0 * eps * (1.0 + rho);
+        double B = ext * (dn + 16.0 * eps) + 16.0 * eps * (amax + onorm + std::max(std::fabs(elo), std::fabs(ehi)) + 1.0);
+        // widen by 1e-6 relative so the float rounding below stays outward
+        double rw = (rho + A) * (1.0 + 1e-6), lw = elo - B, hw = ehi + B;
+        out[0] = std::nextafter((float)rw, INFINITY);
+        out[1] = std::nextafter((float)lw, -INFINITY);
+        out[2] = std::nextafter((float)hw, INFINITY);
+    }
+}
+
 template <int D>
 void finish_sub(SubHost& S) {
     S.present = true;
@@ -473,6 +572,7 @@ void finish_sub(SubHost& S) {
             }
     }
     build_snch<D>(S, conn);
+    if (D == 3) face_bounds3(S);
 }
 
 // _build_subscene_from_mask (polyline.py:672-692, mesh.py:617-635)

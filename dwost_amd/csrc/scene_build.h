@@ -33,6 +33,11 @@ struct SubHost {
     std::vector<int> node_off, node_nref, node_sil_off, node_nsil;
     std::vector<float> node_axis;  // dim per node
     std::vector<float> node_half_angle, node_radius;
+    // 3D face-plane bound per node (rho, e_lo, e_hi): every face f adjacent to
+    // an edge of the subtree has |n_f - axis| <= rho and n_f . (a_f - o) in
+    // [e_lo, e_hi] with o the node's box centre, widened by the fp32 error of
+    // the silhouette test.  rho = +inf disables the bound (open edges).
+    std::vector<float> node_pb;
     std::vector<int> prim_order;   // BVH order -> local primitive index
     std::vector<int> refs;         // SNCH silhouette refs (topology ids)
     // topology silhouettes (2D: one per local vertex; 3D: one per edge)

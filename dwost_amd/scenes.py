@@ -315,3 +315,46 @@ def batch_scenes_2d(n_scenes: int = 64, seed: int = 5, eps: float = 1e-3, max_qu
         spec.meta["nwalks"] = W
         out.append(spec)
     return out
+
+
+def smooth_vertex_field(values: np.ndarray, faces: np.ndarray, n_vertices: int, rounds: int = 8) -> np.ndarray:
+    """``rounds`` passes of v <- (v + mean of edge neighbours) / 2 over the mesh graph."""
+    import scipy.sparse as sp
+
+    f = np.asarray(faces, np.int64)
+    k = f.shape[1]
+    rows = np.concatenate([f[:, i] for i in range(k) for j in range(k) if i != j])
+    cols = np.concatenate([f[:, j] for i in range(k) for j in range(k) if i != j])
+    A = sp.coo_matrix((np.ones(len(rows)), (rows, cols)), shape=(n_vertices, n_vertices)).tocsr()
+    A.data[:] = 1.0  # duplicate edges from the two faces sharing them count once
+    deg = np.maximum(np.asarray(A.sum(1)).ravel(), 1.0)
+    v = np.asarray(values, np.float64)
+    for _ in range(rounds):
+        v = 0.5 * v + 0.5 * (A @ v) / deg
+    return v
+
+
+@dataclass
+class AdjointProblem:
+    spec: SceneSpec
+    g0: np.ndarray  # (V,) float32 initial per-vertex Dirichlet values
+    h0: np.ndarray  # (V,) float32 initial per-vertex Neumann values
+    u_target: np.ndarray  # (N,) float32 e^x cos y at the queries
+    grad_target: np.ndarray  # (N, dim) float32 its gradient
+
+
+def adjoint_problem_3d(level: int = 6, n_query: int = 16384, eps: float = 1e-3, g_seed: int = 6, h_seed: int = 7,
+                       query_seed: int = 8, smooth_rounds: int = 10) -> AdjointProblem:
+    """Noisy icosphere (level 6: 40,962 vertices) with trainable per-vertex g and h
+    (N(0,1) from seeds 6 / 7, 10 rounds of uniform-Laplacian smoothing with
+    lambda = 0.5) and the target field e^x cos y at interior points (seed 8);
+    the loss is mean(|u - u*|^2 + |grad u - grad u*|^2)."""
+    spec = icosphere_scene(level=level, n_query=n_query, eps=eps, query_seed=query_seed)
+    V = len(spec.vertices)
+    g0 = smooth_vertex_field(np.random.default_rng(g_seed).standard_normal(V), spec.indices, V, smooth_rounds)
+    h0 = smooth_vertex_field(np.random.default_rng(h_seed).standard_normal(V), spec.indices, V, smooth_rounds)
+    g0, h0 = g0.astype(np.float32), h0.astype(np.float32)
+    spec.bc_code = BC_PER_VERTEX
+    q = spec.queries
+    return AdjointProblem(spec, g0, h0, analytic_u(BC_EXPCOS, q).astype(np.float32),
+                          analytic_grad(BC_EXPCOS, q).astype(np.float32))

@@ -22,7 +22,7 @@ import torch
 
 from . import _lib
 from ._lib import check, lib, ptr, stream_handle
-from .shapes import _as_points, _Scene
+from .shapes import BC_PER_VERTEX, _as_points, _Scene
 
 
 @dataclass
@@ -50,8 +50,12 @@ class Solver:
         if scene.dim != self.dim:
             raise ValueError(f"{type(self).__name__} needs a {self.dim}D scene")
 
-    def solve(self, p, scene: _Scene, seed: int = 0, per_walk: bool = False, count_steps: bool = False):
-        """u at each point: mean over ``nwalks`` WoSt walks (Solver.solve)."""
+    def solve(self, p, scene: _Scene, seed: int = 0, per_walk: bool = False, count_steps: bool = False,
+              walk_flags: torch.Tensor | None = None):
+        """u at each point: mean over ``nwalks`` WoSt walks (Solver.solve).
+
+        ``walk_flags`` (uint8, n * nwalks, optional) receives per walk a bit per
+        subwalk that survived the outlier guard; the adjoint replays need it."""
         self._check(scene)
         p = _as_points(p, scene.dim)
         n, dev = p.shape[0], p.device
@@ -64,8 +68,10 @@ class Solver:
         steps = torch.zeros(1, dtype=torch.int64, device=dev) if count_steps else None
         prm = self._params(seed)
         bc = scene.boundary.to_c(scene.n_vertices)
+        if walk_flags is not None:
+            _check_flags(walk_flags, n * self.nwalks, dev)
         check(lib().dw_solve_u(scene.handle, ptr(p), n, C.byref(prm), C.byref(bc), ptr(u), ptr(walk_u),
-                               ptr(walk_steps), ptr(walk_term), ptr(steps), stream_handle(dev)))
+                               ptr(walk_steps), ptr(walk_term), ptr(walk_flags), ptr(steps), stream_handle(dev)))
         if not (per_walk or count_steps):
             return u
         out = dict(u=u)
@@ -89,8 +95,12 @@ class Solver:
                                   stream_handle(dev)))
         return (out, steps) if count_steps else out
 
-    def solve_grad(self, p, scene: _Scene, seed: int = 0, per_pair: bool = False, count_steps: bool = False):
-        """Fused (u, grad u): both estimates come from the same antithetic walk pairs."""
+    def solve_grad(self, p, scene: _Scene, seed: int = 0, per_pair: bool = False, count_steps: bool = False,
+                   pair_flags: torch.Tensor | None = None):
+        """Fused (u, grad u): both estimates come from the same antithetic walk pairs.
+
+        ``pair_flags`` (uint8, n * ceil(nwalks / 2), optional) receives per pair
+        bit 0 / bit 1 set when walk + / walk - survived the outlier guard."""
         self._check(scene)
         p = _as_points(p, scene.dim)
         n, dev = p.shape[0], p.device
@@ -103,14 +113,146 @@ class Solver:
         steps = torch.zeros(1, dtype=torch.int64, device=dev) if count_steps else None
         prm = self._params(seed)
         bc = scene.boundary.to_c(scene.n_vertices)
+        if pair_flags is not None:
+            _check_flags(pair_flags, n * npairs, dev)
         check(lib().dw_solve_fused(scene.handle, ptr(p), n, C.byref(prm), C.byref(bc), ptr(u), ptr(g), ptr(pv),
-                                   ptr(ps), ptr(steps), stream_handle(dev)))
+                                   ptr(ps), ptr(pair_flags), ptr(steps), stream_handle(dev)))
         out = dict(u=u, grad=g)
         if per_pair:
             out.update(pair_vals=pv, pair_steps=ps)
         if count_steps:
             out["steps"] = steps
         return out
+
+
+def _check_flags(t: torch.Tensor, numel: int, dev) -> None:
+    if t.dtype != torch.uint8 or not t.is_contiguous() or t.numel() != numel or t.device != dev:
+        raise ValueError(f"flags must be a contiguous uint8 tensor of {numel} elements on {dev}")
+
+
+def _same_walks(a: Solver, b: Solver) -> bool:
+    """True when two solvers draw identical walks for the same seed."""
+    return (a.dim == b.dim and a.nwalks == b.nwalks and a.nsubwalks == b.nsubwalks and a.nsteps == b.nsteps
+            and a.control_variates == b.control_variates and a.antithetic_variates == b.antithetic_variates
+            and a.eps == b.eps and a.min_R == b.min_R and a.exact_cones == b.exact_cones)
+
+
+def _per_vertex(scene: _Scene):
+    bc = scene.boundary
+    if bc is None or bc.code != BC_PER_VERTEX:
+        raise ValueError("CustomSolver differentiates per-vertex boundary data: use BoundaryCondition.per_vertex(g, h)")
+    return bc.g, bc.h
+
+
+class _SolveU(torch.autograd.Function):
+    """u = fwd_solver.solve(p); backward replays bwd_solver's walks from the
+    same seed and scatter-adds dL/dg, dL/dh (solver.py:202-252)."""
+
+    @staticmethod
+    def forward(ctx, g, h, cs, p, scene, seed):
+        fwd, bwd = cs.fwd_solver, cs.bwd_solver
+        n = p.shape[0]
+        flags = None
+        if _same_walks(fwd, bwd):
+            flags = torch.empty(n * fwd.nwalks, dtype=torch.uint8, device=p.device)
+        u = fwd.solve(p, scene, seed, walk_flags=flags)
+        ctx.cs, ctx.p, ctx.scene, ctx.seed, ctx.flags = cs, p, scene, seed, flags
+        return u
+
+    @staticmethod
+    def backward(ctx, dl_du):
+        cs, p, scene, seed = ctx.cs, ctx.p, ctx.scene, ctx.seed
+        bwd = cs.bwd_solver
+        n, dev = p.shape[0], p.device
+        flags = ctx.flags
+        if flags is None:  # the replay solver draws other walks: find its guard flags first
+            flags = torch.empty(n * bwd.nwalks, dtype=torch.uint8, device=dev)
+            bwd.solve(p, scene, seed, walk_flags=flags)
+        V = scene.n_vertices
+        dl_dg = torch.zeros(V, device=dev)
+        dl_dh = torch.zeros(V, device=dev)
+        prm = bwd._params(seed)
+        bc = scene.boundary.to_c(V)
+        w = dl_du.detach().to(torch.float32).contiguous()
+        check(lib().dw_adjoint_u(scene.handle, ptr(p), n, C.byref(prm), C.byref(bc), ptr(w), ptr(flags), ptr(dl_dg),
+                                 ptr(dl_dh), stream_handle(dev)))
+        return (dl_dg if ctx.needs_input_grad[0] else None, dl_dh if ctx.needs_input_grad[1] else None,
+                None, None, None, None)
+
+
+class _SolveGrad(torch.autograd.Function):
+    """(u, grad u) from fused antithetic pairs; backward replays the pairs."""
+
+    @staticmethod
+    def forward(ctx, g, h, cs, p, scene, seed):
+        fwd, bwd = cs.fwd_solver, cs.bwd_solver
+        n = p.shape[0]
+        flags = None
+        if _same_walks(fwd, bwd):
+            flags = torch.empty(n * ((fwd.nwalks + 1) // 2), dtype=torch.uint8, device=p.device)
+        out = fwd.solve_grad(p, scene, seed, pair_flags=flags)
+        ctx.cs, ctx.p, ctx.scene, ctx.seed, ctx.flags = cs, p, scene, seed, flags
+        return out["u"], out["grad"]
+
+    @staticmethod
+    def backward(ctx, dl_du, dl_dgrad):
+        cs, p, scene, seed = ctx.cs, ctx.p, ctx.scene, ctx.seed
+        bwd = cs.bwd_solver
+        n, dev, D = p.shape[0], p.device, scene.dim
+        flags = ctx.flags
+        if flags is None:
+            flags = torch.empty(n * ((bwd.nwalks + 1) // 2), dtype=torch.uint8, device=dev)
+            bwd.solve_grad(p, scene, seed, pair_flags=flags)
+        if dl_du is None:
+            dl_du = torch.zeros(n, device=dev)
+        if dl_dgrad is None:
+            dl_dgrad = torch.zeros(n, D, device=dev)
+        V = scene.n_vertices
+        dl_dg = torch.zeros(V, device=dev)
+        dl_dh = torch.zeros(V, device=dev)
+        prm = bwd._params(seed)
+        bc = scene.boundary.to_c(V)
+        wu = dl_du.detach().to(torch.float32).contiguous()
+        wg = dl_dgrad.detach().to(torch.float32).contiguous()
+        check(lib().dw_adjoint_fused(scene.handle, ptr(p), n, C.byref(prm), C.byref(bc), ptr(wu), ptr(wg), ptr(flags),
+                                     ptr(dl_dg), ptr(dl_dh), stream_handle(dev)))
+        return (dl_dg if ctx.needs_input_grad[0] else None, dl_dh if ctx.needs_input_grad[1] else None,
+                None, None, None, None)
+
+
+@dataclass
+class CustomSolver:
+    """Differentiable solve (diff_wost/solvers/solver.py:255-293).
+
+    The forward value comes from ``fwd_solver``; the backward pass replays
+    ``bwd_solver``'s walks from the same seed and scatter-adds dL/dg and dL/dh
+    into the scene's per-vertex boundary tensors (``BoundaryCondition.per_vertex``
+    with ``requires_grad``).  Nothing is stored per walk but one flag byte."""
+
+    fwd_solver: Solver = None
+    bwd_solver: Solver = None
+
+    def __post_init__(self):
+        if self.bwd_solver is None:
+            self.bwd_solver = self.fwd_solver
+
+    def _prep(self, p, scene):
+        if self.fwd_solver is None:
+            raise ValueError("CustomSolver needs a fwd_solver")
+        self.fwd_solver._check(scene)
+        self.bwd_solver._check(scene)
+        g, h = _per_vertex(scene)
+        return g, h, _as_points(p, scene.dim)
+
+    def solve(self, p, scene: _Scene, seed: int = 0) -> torch.Tensor:
+        """u at each point, differentiable w.r.t. the per-vertex g and h."""
+        g, h, p = self._prep(p, scene)
+        return _SolveU.apply(g, h, self, p, scene, seed)
+
+    def solve_grad(self, p, scene: _Scene, seed: int = 0):
+        """(u, grad u) from the fused estimator, both differentiable w.r.t. g and h."""
+        g, h, p = self._prep(p, scene)
+        return _SolveGrad.apply(g, h, self, p, scene, seed)
 
 
 @dataclass

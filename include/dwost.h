@@ -138,12 +138,21 @@ int dw_star_radius(const dw_scene* scene, int32_t sub, const float* pts, const f
 
 /* Solver.solve with WoSt.u / WoSt3D.u (solver.py:36-71, wost.py:145-221,
  * wost_3d.py:43-106).  u[n] = per-point mean; optional per-walk outputs
- * walk_u / walk_steps / walk_term [n*nwalks]; total_steps (DEVICE uint64) =
+ * walk_u / walk_steps / walk_term / walk_flags [n*nwalks]; total_steps (DEVICE uint64) =
  * sum of loop iterations.  Without walk_u the library uses an internal
  * device workspace. */
 int dw_solve_u(const dw_scene* scene, const float* pts, int64_t n, const dw_solver_params* params,
                const dw_boundary* boundary, float* u, float* walk_u, int32_t* walk_steps, int32_t* walk_term,
-               uint64_t* total_steps, void* stream);
+               uint8_t* walk_flags, uint64_t* total_steps, void* stream);
+
+/* Adjoint of dw_solve_u for per-vertex boundary data (CustomSolver backward,
+ * solver.py:247-252): replays each walk from its seed with weight
+ * dl_du[i] / (nwalks * nsubwalks) and scatter-adds dL/dg, dL/dh (accumulated).
+ * walk_flags[n*nwalks] come from the forward dw_solve_u (bit s: subwalk s not
+ * zeroed by the outlier guard); nsubwalks <= 8. */
+int dw_adjoint_u(const dw_scene* scene, const float* pts, int64_t n, const dw_solver_params* params,
+                 const dw_boundary* boundary, const float* dl_du, const uint8_t* walk_flags, float* dl_dg,
+                 float* dl_dh, void* stream);
 
 /* WoSt.solve_dude / WoSt3D.solve_dude (wost.py:21-102, wost_3d.py:108-175):
  * directional derivative du/de at pts, one PCG32 stream per point. */
@@ -154,10 +163,22 @@ int dw_solve_dude(const dw_scene* scene, const float* pts, const float* e, int64
  * per antithetic pair, the WoSt.dude first-ball step (wost.py:51-99) yields
  * both u(p) (mean value over the ball) and grad u.  u[n], grad[n][dim];
  * optional per-pair outputs pair_vals[n*npairs][dim+1], pair_steps[n*npairs]
- * with npairs = (nwalks+1)/2. */
+ * with npairs = (nwalks+1)/2; pair_flags[n*npairs] (optional) records which
+ * walks survived the outlier guards, for dw_adjoint_fused. */
 int dw_solve_fused(const dw_scene* scene, const float* pts, int64_t n, const dw_solver_params* params,
                    const dw_boundary* boundary, float* u, float* grad, float* pair_vals, int32_t* pair_steps,
-                   uint64_t* total_steps, void* stream);
+                   uint8_t* pair_flags, uint64_t* total_steps, void* stream);
+
+/* Adjoint of dw_solve_fused for per-vertex boundary data (the reference's
+ * CustomSolver backward pass, solver.py:202-252, which replays the solver with
+ * the same seed under Dr.Jit AD).  Replays every walk pair from its seed and
+ * scatter-adds dL/dg and dL/dh into dl_dg / dl_dh (n_vertices each, accumulated:
+ * zero them first).  dl_du[n], dl_dgrad[n][dim] are the upstream gradients;
+ * pair_flags[n*npairs] come from the forward dw_solve_fused with the same
+ * params and seed (bit 0/1: walk +/- not zeroed by an outlier guard). */
+int dw_adjoint_fused(const dw_scene* scene, const float* pts, int64_t n, const dw_solver_params* params,
+                     const dw_boundary* boundary, const float* dl_du, const float* dl_dgrad, const uint8_t* pair_flags,
+                     float* dl_dg, float* dl_dh, void* stream);
 
 #ifdef __cplusplus
 }

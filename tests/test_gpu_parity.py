@@ -49,13 +49,33 @@ def _dirs(n, dim, seed):
     return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
 
 
+def _near_boundary(spec, m, rng):
+    """Points on and beside random boundary primitives (offsets 0, +-1e-4, +-1e-3,
+    +-1e-2 along the primitive normal): where silhouette pruning is tightest."""
+    V, I = spec.vertices.astype(np.float64), spec.indices
+    f = rng.integers(0, len(I), m)
+    if spec.dim == 2:
+        t = rng.uniform(0, 1, m)[:, None]
+        a, b = V[I[f, 0]], V[I[f, 1]]
+        p = a + (b - a) * t
+        n = np.stack([(b - a)[:, 1], -(b - a)[:, 0]], 1)
+    else:
+        w = rng.dirichlet([1, 1, 1], m)
+        p = (V[I[f]] * w[:, :, None]).sum(1)
+        n = np.cross(V[I[f, 1]] - V[I[f, 0]], V[I[f, 2]] - V[I[f, 0]])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    off = rng.choice([0.0, 1e-4, -1e-4, 1e-3, -1e-3, 1e-2, -1e-2], m)
+    return (p + off[:, None] * n).astype(np.float32)
+
+
 @pytest.mark.parametrize("which", ["star", "holes", "ico"])
 def test_queries_bit_exact(gpu, oracle, which, star_spec, holes_spec, ico_spec):
     spec = dict(star=star_spec, holes=holes_spec, ico=ico_spec)[which]
     gs, os_ = _scenes(gpu, oracle, spec)
     rng = np.random.default_rng(11)
     lo, hi = spec.vertices.min(0), spec.vertices.max(0)
-    q = np.concatenate([spec.queries[:1536], rng.uniform(lo - 0.3, hi + 0.3, (512, spec.dim)).astype(np.float32)])
+    q = np.concatenate([spec.queries[:1536], rng.uniform(lo - 0.3, hi + 0.3, (512, spec.dim)).astype(np.float32),
+                        _near_boundary(spec, 1024, rng)])
     D = spec.dim
     for sub in range(3):
         if not os_.info(sub)["present"]:
@@ -67,7 +87,8 @@ def test_queries_bit_exact(gpu, oracle, which, star_spec, holes_spec, ico_spec):
         assert _same_bits(_np(g["p"]), o["p"]), (sub, "p")
         assert _same_bits(_np(g["n"]), o["n"]), (sub, "n")
         assert _same_bits(_np(g["tuv"]), o["tuv"] if D == 3 else np.stack([o["tuv"][:, 0], 0 * o["tuv"][:, 0]], 1))
-        for rm in (np.full(len(q), np.inf, np.float32), np.full(len(q), 0.05, np.float32)):
+        for rm in (np.full(len(q), np.inf, np.float32), np.full(len(q), 0.05, np.float32),
+                   np.full(len(q), 0.4, np.float32)):
             g = gs.closest_silhouette(_t(q), _t(rm), sub=sub)
             o = os_.query(sub, oracle.KIND_SIL, q, rmax=rm)
             assert _same_bits(_np(g["sil"]), o["prim"]), (sub, "sil")

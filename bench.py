@@ -120,6 +120,48 @@ def adjoint_ratio(points: int, reps: int = 2):
                 ratio=round(t_fb / t_f, 3))
 
 
+def batch_speedup(max_queries: int, reps: int = 3):
+    """One persistent launch over the 64-scene uneven batch (segments
+    log-U[64, 16384], 0-4 holes, walks {64, 256, 1024}, queries log-U[100, 1e5]
+    capped at max_queries) against the same solves as per-scene launches."""
+    import torch
+
+    import dwost_amd
+    from dwost_amd import scenes
+
+    specs = scenes.batch_scenes_2d(n_scenes=64, seed=5, max_queries=max_queries)
+    items = []
+    for k, spec in enumerate(specs):
+        sc = dwost_amd.Scene2D(spec.vertices, spec.indices, spec.types, dwost_amd.BoundaryCondition(spec.bc_code))
+        sol = dwost_amd.WoSt(eps=spec.meta["eps"], nwalks=spec.meta["nwalks"], nsteps=1024)
+        items.append(dwost_amd.BatchItem(sol, sc, torch.as_tensor(spec.queries).cuda(), seed=k))
+
+    def per_scene():
+        for it in items:
+            it.solver.solve_grad(it.points, it.scene, seed=it.seed)
+
+    b = dwost_amd.BatchSolver(items, graph=True)
+    per_scene()
+    b.solve()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        per_scene()
+    torch.cuda.synchronize()
+    t_s = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.solve()
+    torch.cuda.synchronize()
+    t_b = (time.perf_counter() - t0) / reps
+    steps = int(b.steps[0])
+    b.close()
+    return dict(workload=f"64 uneven 2D scenes, {sum(len(s.queries) for s in specs)} points "
+                         f"(queries capped at {max_queries}), walks per point from {{64, 256, 1024}}",
+                per_scene_ms=round(1000 * t_s, 3), batched_graph_ms=round(1000 * t_b, 3),
+                speedup=round(t_s / t_b, 3), walk_steps_per_s=round(steps / t_b, 1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +171,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--points", type=int, default=0, help="use only the first N query points (0: all)")
     ap.add_argument("--progress", action="store_true", help="print a line after each timed step")
+    ap.add_argument("--batch-queries", type=int, default=4000,
+                    help="query cap per scene of the 64-scene batched-vs-per-scene measurement (0: skip)")
     ap.add_argument("--adjoint-points", type=int, default=2048,
                     help="points of the fwd+bwd/fwd measurement on the 3D adjoint workload (0: skip)")
     args = ap.parse_args()
@@ -204,6 +248,11 @@ def main():
         adj = adjoint_ratio(args.adjoint_points)
         print(f"[bench] adjoint ratio {adj['ratio']}", file=sys.stderr, flush=True)
 
+    bat = None
+    if args.batch_queries:
+        bat = batch_speedup(args.batch_queries)
+        print(f"[bench] batch speedup {bat['speedup']}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(spec, cfg)
@@ -227,6 +276,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "adjoint": adj,
+        "batch": bat,
     }
     print(json.dumps(line), flush=True)
 

@@ -39,9 +39,17 @@ class Boundary(C.Structure):
 class SolverParams(C.Structure):
     _fields_ = [("nwalks", C.c_int32), ("nsteps", C.c_int32), ("nsubwalks", C.c_int32),
                 ("control_variates", C.c_int32), ("antithetic_variates", C.c_int32), ("seed", C.c_uint32),
-                ("eps", C.c_float), ("min_R", C.c_float), ("flags", C.c_int32)]
+                ("eps", C.c_float), ("min_R", C.c_float), ("flags", C.c_int32), ("point_base", C.c_int64),
+                ("block_size", C.c_int32), ("blocks_per_cu", C.c_int32), ("spatial_order", C.c_int32)]
 
 SOLVE_EXACT_CONES = 1
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("pts", C.c_void_p), ("n", C.c_int64), ("params", SolverParams),
+                ("boundary", Boundary), ("u", C.c_void_p), ("grad", C.c_void_p)]
+
+BATCH_GRAPH = 1
 
 
 def lib():
@@ -68,12 +76,16 @@ def lib():
     L.dw_solve_dude.argtypes = [vp, vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp]
     L.dw_solve_fused.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp, vp, vp, vp,
                                  vp]
+    L.dw_batch_create.argtypes = [C.POINTER(BatchItem), i32, vp, i32, C.POINTER(vp)]
+    L.dw_batch_solve.argtypes = [vp, vp]
+    L.dw_batch_destroy.argtypes = [vp]
+    L.dw_batch_destroy.restype = None
     L.dw_adjoint_u.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp, vp, vp]
     L.dw_adjoint_fused.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp, vp, vp,
                                    vp]
     for name in ("dw_scene_create", "dw_scene_destroy", "dw_scene_info", "dw_scene_export", "dw_closest_point",
                  "dw_intersect", "dw_closest_silhouette", "dw_star_radius", "dw_solve_u", "dw_solve_dude",
-                 "dw_solve_fused", "dw_adjoint_u", "dw_adjoint_fused"):
+                 "dw_solve_fused", "dw_adjoint_u", "dw_adjoint_fused", "dw_batch_create", "dw_batch_solve"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

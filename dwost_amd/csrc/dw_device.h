@@ -111,6 +111,22 @@ struct Rng {
         return (xs >> rot) | (xs << ((0u - rot) & 31u));
     }
     DWD float next() { return __uint_as_float((next_u32() >> 9) | 0x3f800000u) - 1.0f; }
+    // Jump ahead by delta draws: the LCG's affine map composed by squaring
+    // (O'Neill's pcg_advance_lcg_64, after Brown, "Random number generation
+    // with arbitrary strides", 1994).
+    DWD void advance(uint64_t delta) {
+        uint64_t cur_mult = 0x5851f42d4c957f2dULL, cur_plus = inc, acc_mult = 1u, acc_plus = 0u;
+        while (delta > 0) {
+            if (delta & 1u) {
+                acc_mult *= cur_mult;
+                acc_plus = acc_plus * cur_mult + cur_plus;
+            }
+            cur_plus = (cur_mult + 1u) * cur_plus;
+            cur_mult *= cur_mult;
+            delta >>= 1;
+        }
+        state = acc_mult * state + acc_plus;
+    }
     // PCG32(size, initstate, initseq) lane `lane`: seed(initstate+lane, initseq+lane)
     DWD void seed_lane(uint32_t v0, uint32_t v1, uint64_t lane) {
         uint64_t initstate = (uint64_t)v0 + lane, initseq = (uint64_t)v1 + lane;

@@ -150,4 +150,6 @@ struct ParamsDev {
     uint32_t seed;
     float eps, min_R;
     int flags;  // DW_SOLVE_*
+    int64_t point_base;  // global index of point 0 (stream keys)
+    int block_size, blocks_per_cu;  // launch shape (0: defaults)
 };

@@ -49,6 +49,16 @@ DWD float h3(const BCDev& bc, const SubDev& S, int prim, float u, float v, F3 p,
     return (hb * u + hc * v) + ha * ((1.0f - u) - v);
 }
 
+// Random stream of walk (or walk pair) k of point i: TEA(seed, global point
+// index) and Dr.Jit PCG32 lane = global point index * per + k (solver.py:55-63),
+// so a shard with point_base b draws exactly what one launch over all points does.
+DWD void seed_stream(Rng& rng, const ParamsDev& P, int64_t i, int64_t per, int64_t k) {
+    int64_t gi = P.point_base + i;
+    uint32_t a, b;
+    tea4(P.seed, (uint32_t)gi, a, b);
+    rng.seed_lane(a, b, (uint64_t)(gi * per + k));
+}
+
 // Adjoint of the per-vertex boundary data: scatter w * d(value)/d(vertex value).
 // g2/h2: lerp(a, b, t) = a (1 - t) + b t;  g3/h3: b u + c v + a (1 - u - v).
 struct Adj {
@@ -305,6 +315,23 @@ DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk
     const SubDev& Dn = sc.sub[1];
     const SubDev& Nm = sc.sub[2];
     if (!(w.i < P.nsteps)) {
+        w.live = false;
+        return false;
+    }
+    if (__builtin_isnan(w.ip.x) || __builtin_isnan(w.ip.y) || __builtin_isnan(w.ip.z)) {
+        // A walk that overflowed to infinity keeps a NaN coordinate from here on
+        // (ip' = ip + dir R), and every remaining step is then fixed: no
+        // closest-point or hit test can succeed (all distances are NaN), so it
+        // never terminates early, the Neumann and source terms are skipped
+        // (their |y - x| tests are false) and only `result += 0` remains.  The
+        // traversals of such steps visit whole trees (NaN box distances clamp
+        // to 0), so they are replaced by the outcome: the step count runs out
+        // and the stream advances by the draws of the skipped steps (cdf, two
+        // triangle coordinates, two direction and two source draws).
+        uint64_t left = (uint64_t)(P.nsteps - w.i);
+        rng.advance(left * (Nm.present ? 7u : 4u));
+        w.result += 0.0f;
+        w.i = P.nsteps;
         w.live = false;
         return false;
     }

@@ -1,6 +1,7 @@
 // dwost.hip -- kernels and the C ABI of libdwost (declared in include/dwost.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -131,10 +132,8 @@ __global__ void __launch_bounds__(kBlock) k_walk_u(SceneDev sc, BCDev bc, Params
     int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= n * P.nwalks) return;
     int64_t i = lane / P.nwalks;
-    uint32_t a, b;
-    tea4(P.seed, (uint32_t)i, a, b);
     Rng rng;
-    rng.seed_lane(a, b, (uint64_t)lane);
+    seed_stream(rng, P, i, P.nwalks, lane - i * P.nwalks);
     float res = 0.0f;
     int steps = 0, term = -1;
     for (int s = 0; s < P.nsubwalks; ++s) {
@@ -156,14 +155,9 @@ __global__ void __launch_bounds__(kBlock) k_walk_u(SceneDev sc, BCDev bc, Params
 // Per-point mean over W per-walk values: one wavefront per point, fp32 Kahan
 // per lane, then a fixed-order Kahan combine across lanes (deterministic).
 // Optional `comps` extra components per walk (gradient) use the same order.
-__global__ void __launch_bounds__(kBlock) k_reduce_mean(const float* __restrict__ vals, int64_t n, int W, int comps,
-                                                        float scale, float* out0, float* out_rest,
-                                                        const int* __restrict__ steps,
-                                                        unsigned long long* step_total) {
-    int wave = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    int lane = threadIdx.x & 63;
-    if (wave >= n) return;
-    int64_t base = (int64_t)wave * W;
+DWD void reduce_point(const float* __restrict__ vals, int64_t point, int W, int comps, float scale, float* out0,
+                      float* out_rest, const int* __restrict__ steps, unsigned long long* step_total, int lane) {
+    int64_t base = point * W;
     for (int c = 0; c < comps; ++c) {
         float s = 0.0f, e = 0.0f;
         for (int k = lane; k < W; k += 64) {
@@ -185,9 +179,9 @@ __global__ void __launch_bounds__(kBlock) k_reduce_mean(const float* __restrict_
         if (lane == 0) {
             float m = (s - e) * scale;
             if (c == 0)
-                out0[wave] = m;
+                out0[point] = m;
             else
-                out_rest[(int64_t)wave * (comps - 1) + (c - 1)] = m;
+                out_rest[point * (comps - 1) + (c - 1)] = m;
         }
     }
     if (steps && step_total) {
@@ -196,6 +190,14 @@ __global__ void __launch_bounds__(kBlock) k_reduce_mean(const float* __restrict_
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
         if (lane == 0) atomicAdd(step_total, acc);
     }
+}
+__global__ void __launch_bounds__(kBlock) k_reduce_mean(const float* __restrict__ vals, int64_t n, int W, int comps,
+                                                        float scale, float* out0, float* out_rest,
+                                                        const int* __restrict__ steps,
+                                                        unsigned long long* step_total) {
+    int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (wave >= n) return;
+    reduce_point(vals, wave, W, comps, scale, out0, out_rest, steps, step_total, threadIdx.x & 63);
 }
 
 // ---------------------------------------------------------------------------
@@ -364,9 +366,7 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
             job = got;
             kept = 0;
             int64_t i = job / P.nwalks;
-            uint32_t a, b;
-            tea4(P.seed, (uint32_t)i, a, b);
-            rng.seed_lane(a, b, (uint64_t)job);
+            seed_stream(rng, P, i, P.nwalks, job - i * P.nwalks);
             x = load_pt<D>(pts, i);
             walk_init<D>(w, x);
             res = 0.0f;
@@ -392,16 +392,70 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
     }
 }
 
+// Distance to the full boundary and the inside flag per point (used by the
+// gradient estimators, wost.py:43-50).
+DWD void prep_point(const SceneDev& sc, const float* __restrict__ pts, int64_t i, float* R, int* inside) {
+    if (sc.dim == 2) {
+        F2 x = f2(pts[2 * i], pts[2 * i + 1]);
+        R[i] = __builtin_fabsf(closest2(sc.sub[0], x).d);
+        inside[i] = inside2(sc, x) ? 1 : 0;
+    } else {
+        F3 x = f3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+        R[i] = __builtin_fabsf(closest3(sc.sub[0], x).d);
+        inside[i] = inside3(sc, x) ? 1 : 0;
+    }
+}
+__global__ void __launch_bounds__(kBlock) k_point_prep(SceneDev sc, const float* __restrict__ pts, int64_t n,
+                                                       float* R, int* inside) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    prep_point(sc, pts, i, R, inside);
+}
+
+// One scene's share of a fused launch: geometry, boundary data, walk
+// parameters, its points, their first-ball radii / inside flags and where the
+// per-pair partial results go.  A plain solve is a batch of one.
+struct FusedItem {
+    const SceneDev* sc;
+    BCDev bc;
+    ParamsDev P;
+    const float* pts;
+    const float* R;
+    const int* inside;
+    float* pair_vals;  // [n * npairs][dim + 1]
+    int* pair_steps;   // [n * npairs]
+    uint8_t* pair_flags;
+    float* u;
+    float* grad;
+    int64_t n;
+    int64_t job0;  // first job of this item in the batch's job space
+    int64_t pt0;   // first point of this item in the batch's point space
+    int npairs;
+};
+// Index of the item holding global index g, by the item starts `first` (sorted).
+template <typename F>
+DWD int find_item(int n_items, int64_t g, F first) {
+    int lo = 0, hi = n_items - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (first(mid) <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
 // Fused jobs: job = i*npairs + k, the antithetic pair of WoSt.dude's first ball.
-template <int D, bool X>
-__global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVES_FAST) k_fused_pers(const SceneDev* __restrict__ scp, BCDev bc, ParamsDev P,
-                                                       const float* __restrict__ pts, const float* __restrict__ Rp,
-                                                       const int* __restrict__ insidep, int64_t n, int npairs,
-                                                       float* pair_vals, int* pair_steps, uint8_t* pair_flags,
-                                                       JobQ Q) {
-    const SceneDev& sc = *scp;
+// B = false: one item (kernel argument), jobs from the spatial queue.
+// B = true: many items (device array); the queue runs over the concatenated
+// job space and each lane looks up the item of its job.
+template <int D, bool X, bool B>
+__global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVES_FAST)
+    k_fused_pers(FusedItem one, const FusedItem* __restrict__ items, int n_items, JobQ Q) {
     using V = typename VecT<D>::T;
-    const float fsrc = src_f(bc);
+#define ITEM(f) (B ? items[it].f : one.f)
+    int it = 0;
     int64_t job = -1;
     bool done = false;
     QState qs;
@@ -417,19 +471,22 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
         int64_t got = q_next(!done && job < 0, Q, qs);
         if (got == -2) done = true;
         if (got >= 0) {
+            if constexpr (B) {
+                it = find_item(n_items, got, [&](int k) { return items[k].job0; });
+                got -= items[it].job0;
+            }
+            const int npairs = ITEM(npairs);
             int64_t i = got / npairs;
-            if (!insidep[i]) {
-                float* o = pair_vals + got * (D + 1);
+            if (!ITEM(inside)[i]) {
+                float* o = ITEM(pair_vals) + got * (D + 1);
                 for (int q = 0; q <= D; ++q) o[q] = 0.0f;
-                pair_steps[got] = 0;
-                if (pair_flags) pair_flags[got] = 0;
+                ITEM(pair_steps)[got] = 0;
+                if (ITEM(pair_flags)) ITEM(pair_flags)[got] = 0;
             } else {
                 job = got;
-                R = Rp[i];
-                p = load_pt<D>(pts, i);
-                uint32_t a, b;
-                tea4(P.seed, (uint32_t)i, a, b);
-                rng.seed_lane(a, b, (uint64_t)job);
+                R = ITEM(R)[i];
+                p = load_pt<D>(ITEM(pts), i);
+                seed_stream(rng, ITEM(P), i, npairs, job - i * npairs);
                 V p_src;
                 float r_src, pdf_src;
                 if constexpr (D == 2) {
@@ -449,7 +506,7 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
             }
         }
         if (__ballot(!done) == 0) break;
-        if (job >= 0 && !walk_step<X>(sc, bc, P, w, rng)) {
+        if (job >= 0 && !walk_step<X>(*ITEM(sc), ITEM(bc), ITEM(P), w, rng)) {
             float v = walk_value(w);
             steps += w.i;
             // walks the guards zero contribute nothing to the adjoint
@@ -462,7 +519,8 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
                 phase = 1;
             } else {
                 float um = v;
-                float* o = pair_vals + job * (D + 1);
+                float* o = ITEM(pair_vals) + job * (D + 1);
+                const float fsrc = src_f(ITEM(bc));
                 if constexpr (D == 2) {
                     float s = (R * R / 4.0f) * fsrc;
                     float k = 2.0f / R * (up - um);
@@ -482,12 +540,33 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
                     o[2] = bad(g1) ? 0.0f : g1;
                     o[3] = bad(g2_) ? 0.0f : g2_;
                 }
-                pair_steps[job] = steps;
-                if (pair_flags) pair_flags[job] = (uint8_t)((kept_plus ? 1 : 0) | (kept ? 2 : 0));
+                ITEM(pair_steps)[job] = steps;
+                if (ITEM(pair_flags)) ITEM(pair_flags)[job] = (uint8_t)((kept_plus ? 1 : 0) | (kept ? 2 : 0));
                 job = -1;
             }
         }
     }
+#undef ITEM
+}
+
+// Batched point preparation and reduction: thread (wave) per point of the
+// concatenated point space.
+__global__ void __launch_bounds__(kBlock) k_point_prep_batch(const FusedItem* __restrict__ items, int n_items,
+                                                             int64_t total) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    int it = find_item(n_items, g, [&](int k) { return items[k].pt0; });
+    const FusedItem& I = items[it];
+    prep_point(*I.sc, I.pts, g - I.pt0, (float*)I.R, (int*)I.inside);
+}
+__global__ void __launch_bounds__(kBlock) k_reduce_batch(const FusedItem* __restrict__ items, int n_items,
+                                                         int64_t total, int dim, unsigned long long* step_total) {
+    int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (g >= total) return;
+    int it = find_item(n_items, g, [&](int k) { return items[k].pt0; });
+    const FusedItem& I = items[it];
+    reduce_point(I.pair_vals, g - I.pt0, I.npairs, dim + 1, 1.0f / (float)(2 * I.npairs), I.u, I.grad,
+                 step_total ? I.pair_steps : nullptr, step_total, threadIdx.x & 63);
 }
 
 // Adjoint of the fused estimator: replays every pair from its seed with the
@@ -525,9 +604,7 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
                 job = got;
                 R = Rp[i];
                 p = load_pt<D>(pts, i);
-                uint32_t a, b;
-                tea4(P.seed, (uint32_t)i, a, b);
-                rng.seed_lane(a, b, (uint64_t)job);
+                seed_stream(rng, P, i, npairs, job - i * npairs);
                 V p_src;
                 float r_src, pdf_src;
                 float proj;
@@ -592,9 +669,7 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
             fl = walk_flags[got];
             if (fl) {
                 job = got;
-                uint32_t a, b;
-                tea4(P.seed, (uint32_t)i, a, b);
-                rng.seed_lane(a, b, (uint64_t)job);
+                seed_stream(rng, P, i, P.nwalks, job - i * P.nwalks);
                 x = load_pt<D>(pts, i);
                 walk_init<D>(w, x);
                 sub = 0;
@@ -614,23 +689,6 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
     }
 }
 
-// Distance to the full boundary and the inside flag per point (used by the
-// gradient estimators, wost.py:43-50).
-__global__ void __launch_bounds__(kBlock) k_point_prep(SceneDev sc, const float* __restrict__ pts, int64_t n,
-                                                       float* R, int* inside) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (sc.dim == 2) {
-        F2 x = f2(pts[2 * i], pts[2 * i + 1]);
-        R[i] = __builtin_fabsf(closest2(sc.sub[0], x).d);
-        inside[i] = inside2(sc, x) ? 1 : 0;
-    } else {
-        F3 x = f3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
-        R[i] = __builtin_fabsf(closest3(sc.sub[0], x).d);
-        inside[i] = inside3(sc, x) ? 1 : 0;
-    }
-}
-
 // Fused u + grad u: one antithetic pair per lane (pair k of point i uses the
 // stream of lane i*npairs + k).  Writes (usum, g[dim]) per pair and steps.
 __global__ void __launch_bounds__(kBlock) k_fused(SceneDev sc, BCDev bc, ParamsDev P, const float* __restrict__ pts,
@@ -647,10 +705,8 @@ __global__ void __launch_bounds__(kBlock) k_fused(SceneDev sc, BCDev bc, ParamsD
         return;
     }
     float R = Rp[i];
-    uint32_t a, b;
-    tea4(P.seed, (uint32_t)i, a, b);
     Rng rng;
-    rng.seed_lane(a, b, (uint64_t)lane);
+    seed_stream(rng, P, i, npairs, lane - i * npairs);
     float fsrc = src_f(bc);
     int st1, st2, tm;
     if (D == 2) {
@@ -706,10 +762,8 @@ __global__ void __launch_bounds__(kBlock) k_dude(SceneDev sc, BCDev bc, ParamsDe
                                                  unsigned long long* step_total) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t a, b;
-    tea4(P.seed, (uint32_t)i, a, b);
     Rng rng;
-    rng.seed_lane(a, b, (uint64_t)i);
+    seed_stream(rng, P, i, 1, 0);
     float R = Rp[i];
     bool active = insidep[i] != 0;
     float result = 0.0f;
@@ -1082,6 +1136,9 @@ ParamsDev to_params(const dw_solver_params* p) {
     P.eps = p->eps;
     P.min_R = p->min_R;
     P.flags = p->flags;
+    P.point_base = p->point_base;
+    P.block_size = p->block_size;
+    P.blocks_per_cu = p->blocks_per_cu;
     return P;
 }
 int check_params(const dw_solver_params* p) {
@@ -1089,6 +1146,11 @@ int check_params(const dw_solver_params* p) {
     if (p->nwalks < 1) return fail(DW_ERR_ARG, "nwalks must be >= 1");
     if (p->nsteps < 0) return fail(DW_ERR_ARG, "nsteps must be >= 0");
     if (p->nsubwalks < 1) return fail(DW_ERR_ARG, "nsubwalks must be >= 1");
+    if (p->point_base < 0) return fail(DW_ERR_ARG, "point_base must be >= 0");
+    if (p->block_size != 0 && p->block_size != 64 && p->block_size != 128 && p->block_size != 256)
+        return fail(DW_ERR_ARG, "block_size must be 0, 64, 128 or 256");
+    if (p->blocks_per_cu < 0) return fail(DW_ERR_ARG, "blocks_per_cu must be >= 0");
+    if (p->spatial_order < 0 || p->spatial_order > 2) return fail(DW_ERR_ARG, "spatial_order must be 0, 1 or 2");
     return DW_OK;
 }
 
@@ -1131,9 +1193,9 @@ int workspace(StreamCtx& c, size_t bytes, hipStream_t st, void** out) {
 }
 
 // Bytes of workspace the spatial order of n points needs (0: identity order).
-size_t order_bytes(int64_t n) {
-    const char* e = getenv("DWOST_SPATIAL_ORDER");
-    if ((e && atoi(e) == 0) || n < 2 * kCells || n >= (int64_t)1 << 31) return 0;
+size_t order_bytes(int64_t n, const dw_solver_params* p) {
+    if (p->spatial_order == DW_ORDER_OFF || n >= (int64_t)1 << 31) return 0;
+    if (p->spatial_order != DW_ORDER_ON && n < 2 * kCells) return 0;
     return (size_t)n * 8 + kCells * 4;
 }
 
@@ -1171,15 +1233,27 @@ int make_queue(const dw_scene* s, StreamCtx& c, const float* pts, int64_t n, int
     return DW_OK;
 }
 
-// Grid of a persistent kernel: as many resident blocks as the occupancy allows
-// (DWOST_BLOCKS_PER_CU overrides the per-CU count for tuning).
+// Launch shape of the persistent kernels (dw_solver_params.block_size /
+// blocks_per_cu; 0 picks 256 threads and as many resident blocks per CU as
+// the occupancy allows).  Each lane's traversal stack takes 128 B of LDS.
+struct LaunchCfg {
+    int block = kBlock;
+    int per_cu = 0;
+};
+LaunchCfg launch_cfg(const ParamsDev& P) {
+    LaunchCfg c;
+    if (P.block_size) c.block = P.block_size;
+    c.per_cu = P.blocks_per_cu;
+    return c;
+}
 template <typename K>
-int persistent_grid(K kernel, unsigned& blocks) {
+int persistent_grid(K kernel, const LaunchCfg& cfg, unsigned& blocks) {
     int dev = 0, cus = 0, per_cu = 0;
     DW_HIP(hipGetDevice(&dev));
     DW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    DW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, kStackLds));
-    if (const char* e = getenv("DWOST_BLOCKS_PER_CU")) per_cu = atoi(e);
+    DW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, cfg.block,
+                                                       (size_t)cfg.block * kStackLdsBytesPerThread));
+    if (cfg.per_cu > 0 && cfg.per_cu < per_cu) per_cu = cfg.per_cu;
     if (per_cu < 1) per_cu = 1;
     blocks = (unsigned)(per_cu * cus);
     return DW_OK;
@@ -1187,15 +1261,16 @@ int persistent_grid(K kernel, unsigned& blocks) {
 
 // Launches the exact- or fast-cone instance of a persistent kernel on its grid.
 template <typename KE, typename KF, typename... A>
-int launch_pers(bool exact, KE ke, KF kf, hipStream_t st, A... args) {
+int launch_pers(bool exact, const LaunchCfg& cfg, KE ke, KF kf, hipStream_t st, A... args) {
     unsigned blocks = 0;
     int rc;
+    const size_t lds = (size_t)cfg.block * kStackLdsBytesPerThread;
     if (exact) {
-        if ((rc = persistent_grid(ke, blocks))) return rc;
-        hipLaunchKernelGGL(ke, dim3(blocks), dim3(kBlock), kStackLds, st, args...);
+        if ((rc = persistent_grid(ke, cfg, blocks))) return rc;
+        hipLaunchKernelGGL(ke, dim3(blocks), dim3(cfg.block), lds, st, args...);
     } else {
-        if ((rc = persistent_grid(kf, blocks))) return rc;
-        hipLaunchKernelGGL(kf, dim3(blocks), dim3(kBlock), kStackLds, st, args...);
+        if ((rc = persistent_grid(kf, cfg, blocks))) return rc;
+        hipLaunchKernelGGL(kf, dim3(blocks), dim3(cfg.block), lds, st, args...);
     }
     DW_HIP(hipGetLastError());
     return DW_OK;
@@ -1210,6 +1285,74 @@ int zero_u64(unsigned long long* p, hipStream_t st) {
 bool use_simple_kernels() {
     const char* e = getenv("DWOST_KERNEL");
     return e && strcmp(e, "simple") == 0;
+}
+
+}  // namespace
+
+// A prepared multi-scene fused solve: device item descriptors, the partial
+// result buffers of every item, the queue counters and (optionally) the
+// captured graph of its launch sequence.
+struct dw_batch {
+    int device = 0;
+    int dim = 2;
+    bool exact = false;
+    bool use_graph = false;
+    LaunchCfg cfg;
+    int n_items = 0;
+    int64_t total_jobs = 0, total_pts = 0;
+    FusedItem* dev_items = nullptr;
+    void* ws = nullptr;
+    unsigned long long* ctr = nullptr;
+    unsigned long long* total_steps = nullptr;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void batch_free(dw_batch* b) {
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+    if (b->dev_items) (void)hipFree(b->dev_items);
+    if (b->ws) (void)hipFree(b->ws);
+    if (b->ctr) (void)hipFree(b->ctr);
+    delete b;
+}
+
+// The launch sequence of a batch: point preparation, queue reset, one
+// persistent launch over the concatenated job space, per-point reduction.
+// It touches only device memory owned by the batch or its items, so it can be
+// captured into a graph and replayed.
+int batch_launch(dw_batch* b, hipStream_t st) {
+    int rc;
+    hipLaunchKernelGGL(k_point_prep_batch, dim3(blocks_for(b->total_pts)), dim3(kBlock), kStackLds, st,
+                       (const FusedItem*)b->dev_items, b->n_items, b->total_pts);
+    DW_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kQueueCtrs), 0, st, b->ctr, kQueueCtrs, 0ull);
+    DW_HIP(hipGetLastError());
+    JobQ Q;
+    Q.ctr = b->ctr;
+    Q.order = nullptr;
+    Q.n = b->total_jobs;
+    Q.per = 1;
+    FusedItem none;
+    std::memset(&none, 0, sizeof(none));
+    const FusedItem* items = b->dev_items;
+    if (b->dim == 2)
+        rc = launch_pers(b->exact, b->cfg, k_fused_pers<2, true, true>, k_fused_pers<2, false, true>, st, none, items,
+                         b->n_items, Q);
+    else
+        rc = launch_pers(b->exact, b->cfg, k_fused_pers<3, true, true>, k_fused_pers<3, false, true>, st, none, items,
+                         b->n_items, Q);
+    if (rc) return rc;
+    if (b->total_steps && (rc = zero_u64(b->total_steps, st))) return rc;
+    hipLaunchKernelGGL(k_reduce_batch, dim3(blocks_for(b->total_pts * 64)), dim3(kBlock), 0, st, items, b->n_items,
+                       b->total_pts, b->dim, b->total_steps);
+    DW_HIP(hipGetLastError());
+    return DW_OK;
 }
 
 }  // namespace
@@ -1380,7 +1523,7 @@ int dw_solve_u(const dw_scene* s, const float* pts, int64_t n, const dw_solver_p
     if ((rc = stream_ctx(st, &c))) return rc;
     const bool simple = use_simple_kernels() && !walk_flags;
     int64_t lanes = n * (int64_t)params->nwalks;
-    size_t obytes = simple ? 0 : order_bytes(n);
+    size_t obytes = simple ? 0 : order_bytes(n, params);
     size_t wbytes = (walk_u ? 0 : (size_t)lanes * 4) + ((total_steps && !walk_steps) ? (size_t)lanes * 4 : 0);
     void* p;
     if ((rc = workspace(*c, wbytes + obytes, st, &p))) return rc;
@@ -1398,10 +1541,10 @@ int dw_solve_u(const dw_scene* s, const float* pts, int64_t n, const dw_solver_p
         if ((rc = make_queue(s, *c, pts, n, params->nwalks, obytes ? cur : nullptr, st, Q))) return rc;
         const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
         if (s->dev.dim == 2)
-            rc = launch_pers(exact, k_walk_u_pers<2, true>, k_walk_u_pers<2, false>, st, s->dev_ptr, to_bc(boundary),
+            rc = launch_pers(exact, launch_cfg(to_params(params)), k_walk_u_pers<2, true>, k_walk_u_pers<2, false>, st, s->dev_ptr, to_bc(boundary),
                              to_params(params), pts, n, wv, ws, walk_term, walk_flags, Q);
         else
-            rc = launch_pers(exact, k_walk_u_pers<3, true>, k_walk_u_pers<3, false>, st, s->dev_ptr, to_bc(boundary),
+            rc = launch_pers(exact, launch_cfg(to_params(params)), k_walk_u_pers<3, true>, k_walk_u_pers<3, false>, st, s->dev_ptr, to_bc(boundary),
                              to_params(params), pts, n, wv, ws, walk_term, walk_flags, Q);
         if (rc) return rc;
     }
@@ -1451,7 +1594,7 @@ int dw_solve_fused(const dw_scene* s, const float* pts, int64_t n, const dw_solv
     StreamCtx* c;
     if ((rc = stream_ctx(st, &c))) return rc;
     const bool simple = use_simple_kernels();
-    size_t obytes = simple ? 0 : order_bytes(n);
+    size_t obytes = simple ? 0 : order_bytes(n, params);
     size_t need = (size_t)n * 8 + (pair_vals ? 0 : (size_t)lanes * (D + 1) * 4) +
                   (pair_steps ? 0 : (size_t)lanes * 4) + obytes;
     void* p;
@@ -1475,12 +1618,24 @@ int dw_solve_fused(const dw_scene* s, const float* pts, int64_t n, const dw_solv
         JobQ Q;
         if ((rc = make_queue(s, *c, pts, n, npairs, obytes ? cur : nullptr, st, Q))) return rc;
         const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
+        FusedItem one;
+        std::memset(&one, 0, sizeof(one));
+        one.sc = s->dev_ptr;
+        one.bc = to_bc(boundary);
+        one.P = to_params(params);
+        one.pts = pts;
+        one.R = R;
+        one.inside = ins;
+        one.pair_vals = pv;
+        one.pair_steps = ps;
+        one.pair_flags = pair_flags;
+        one.n = n;
+        one.npairs = npairs;
+        const FusedItem* none = nullptr;
         if (D == 2)
-            rc = launch_pers(exact, k_fused_pers<2, true>, k_fused_pers<2, false>, st, s->dev_ptr, to_bc(boundary),
-                             to_params(params), pts, R, ins, n, npairs, pv, ps, pair_flags, Q);
+            rc = launch_pers(exact, launch_cfg(to_params(params)), k_fused_pers<2, true, false>, k_fused_pers<2, false, false>, st, one, none, 1, Q);
         else
-            rc = launch_pers(exact, k_fused_pers<3, true>, k_fused_pers<3, false>, st, s->dev_ptr, to_bc(boundary),
-                             to_params(params), pts, R, ins, n, npairs, pv, ps, pair_flags, Q);
+            rc = launch_pers(exact, launch_cfg(to_params(params)), k_fused_pers<3, true, false>, k_fused_pers<3, false, false>, st, one, none, 1, Q);
         if (rc) return rc;
     }
     if (total_steps && (rc = zero_u64((unsigned long long*)total_steps, st))) return rc;
@@ -1508,7 +1663,7 @@ int dw_adjoint_fused(const dw_scene* s, const float* pts, int64_t n, const dw_so
     hipStream_t st = (hipStream_t)stream;
     StreamCtx* c;
     if ((rc = stream_ctx(st, &c))) return rc;
-    size_t obytes = order_bytes(n);
+    size_t obytes = order_bytes(n, params);
     void* p;
     if ((rc = workspace(*c, (size_t)n * 8 + obytes, st, &p))) return rc;
     float* R = (float*)p;
@@ -1519,9 +1674,9 @@ int dw_adjoint_fused(const dw_scene* s, const float* pts, int64_t n, const dw_so
     if ((rc = make_queue(s, *c, pts, n, npairs, obytes ? (char*)p + (size_t)n * 8 : nullptr, st, Q))) return rc;
     const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
     if (D == 2)
-        return launch_pers(exact, k_adjoint_pers<2, true>, k_adjoint_pers<2, false>, st, s->dev_ptr, to_bc(boundary),
+        return launch_pers(exact, launch_cfg(to_params(params)), k_adjoint_pers<2, true>, k_adjoint_pers<2, false>, st, s->dev_ptr, to_bc(boundary),
                            to_params(params), pts, R, ins, n, npairs, dl_du, dl_dgrad, pair_flags, dl_dg, dl_dh, Q);
-    return launch_pers(exact, k_adjoint_pers<3, true>, k_adjoint_pers<3, false>, st, s->dev_ptr, to_bc(boundary),
+    return launch_pers(exact, launch_cfg(to_params(params)), k_adjoint_pers<3, true>, k_adjoint_pers<3, false>, st, s->dev_ptr, to_bc(boundary),
                        to_params(params), pts, R, ins, n, npairs, dl_du, dl_dgrad, pair_flags, dl_dg, dl_dh, Q);
 }
 
@@ -1539,17 +1694,133 @@ int dw_adjoint_u(const dw_scene* s, const float* pts, int64_t n, const dw_solver
     hipStream_t st = (hipStream_t)stream;
     StreamCtx* c;
     if ((rc = stream_ctx(st, &c))) return rc;
-    size_t obytes = order_bytes(n);
+    size_t obytes = order_bytes(n, params);
     void* p = nullptr;
     if (obytes && (rc = workspace(*c, obytes, st, &p))) return rc;
     JobQ Q;
     if ((rc = make_queue(s, *c, pts, n, params->nwalks, p, st, Q))) return rc;
     const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
     if (s->dev.dim == 2)
-        return launch_pers(exact, k_adjoint_u_pers<2, true>, k_adjoint_u_pers<2, false>, st, s->dev_ptr,
+        return launch_pers(exact, launch_cfg(to_params(params)), k_adjoint_u_pers<2, true>, k_adjoint_u_pers<2, false>, st, s->dev_ptr,
                            to_bc(boundary), to_params(params), pts, n, dl_du, walk_flags, dl_dg, dl_dh, Q);
-    return launch_pers(exact, k_adjoint_u_pers<3, true>, k_adjoint_u_pers<3, false>, st, s->dev_ptr, to_bc(boundary),
+    return launch_pers(exact, launch_cfg(to_params(params)), k_adjoint_u_pers<3, true>, k_adjoint_u_pers<3, false>, st, s->dev_ptr, to_bc(boundary),
                        to_params(params), pts, n, dl_du, walk_flags, dl_dg, dl_dh, Q);
 }
+
+int dw_batch_create(const dw_batch_item* items, int32_t n_items, uint64_t* total_steps, int32_t flags,
+                    dw_batch** out) {
+    if (!out) return fail(DW_ERR_ARG, "null output");
+    *out = nullptr;
+    if (n_items < 1 || !items) return fail(DW_ERR_ARG, "a batch needs at least one item");
+    int rc;
+    int dev = 0;
+    DW_HIP(hipGetDevice(&dev));
+    const int dim = items[0].scene ? items[0].scene->dev.dim : 0;
+    const int exact = items[0].params.flags & DW_SOLVE_EXACT_CONES;
+    for (int k = 0; k < n_items; ++k) {
+        const dw_batch_item& it = items[k];
+        if ((rc = check_scene(it.scene, 0))) return rc;
+        if (it.scene->device != dev) return fail(DW_ERR_ARG, "batch item " + std::to_string(k) + ": scene lives on another device");
+        if (it.scene->dev.dim != dim) return fail(DW_ERR_ARG, "batch item " + std::to_string(k) + ": all scenes need the same dimension");
+        if ((it.params.flags & DW_SOLVE_EXACT_CONES) != exact)
+            return fail(DW_ERR_ARG, "batch item " + std::to_string(k) + ": all items need the same cone mode");
+        if ((rc = check_params(&it.params)) || (rc = check_bc(&it.boundary))) return rc;
+        if (it.n < 0 || (it.n > 0 && (!it.pts || !it.u || !it.grad)))
+            return fail(DW_ERR_ARG, "batch item " + std::to_string(k) + ": bad arrays");
+    }
+    // Larger items first in the job space: their walks start early, so the
+    // launch does not wait on a big scene's tail.
+    std::vector<int> order;
+    for (int k = 0; k < n_items; ++k)
+        if (items[k].n > 0) order.push_back(k);
+    auto jobs_of = [&](int k) { return items[k].n * (int64_t)((items[k].params.nwalks + 1) / 2); };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return jobs_of(a) > jobs_of(c); });
+    dw_batch* b = new dw_batch();
+    b->device = dev;
+    b->dim = dim;
+    b->exact = exact != 0;
+    b->use_graph = (flags & DW_BATCH_GRAPH) != 0;
+    b->cfg = launch_cfg(to_params(&items[0].params));
+    b->n_items = (int)order.size();
+    b->total_steps = (unsigned long long*)total_steps;
+    size_t bytes = 0;
+    for (int k : order) {
+        int64_t j = jobs_of(k);
+        bytes += (size_t)items[k].n * 8 + (size_t)j * (dim + 1) * 4 + (size_t)j * 4;
+        bytes = (bytes + 255) & ~(size_t)255;
+    }
+    std::vector<FusedItem> host(order.size());
+    if (bytes && hipMalloc(&b->ws, bytes) != hipSuccess) {
+        batch_free(b);
+        return fail(DW_ERR_HIP, "batch workspace allocation failed (" + std::to_string(bytes) + " bytes)");
+    }
+    char* cur = (char*)b->ws;
+    for (size_t q = 0; q < order.size(); ++q) {
+        const dw_batch_item& it = items[order[q]];
+        FusedItem& f = host[q];
+        std::memset(&f, 0, sizeof(f));
+        f.sc = it.scene->dev_ptr;
+        f.bc = to_bc(&it.boundary);
+        f.P = to_params(&it.params);
+        f.pts = it.pts;
+        f.u = it.u;
+        f.grad = it.grad;
+        f.n = it.n;
+        f.npairs = (it.params.nwalks + 1) / 2;
+        f.job0 = b->total_jobs;
+        f.pt0 = b->total_pts;
+        int64_t j = it.n * (int64_t)f.npairs;
+        f.R = (const float*)cur, cur += (size_t)it.n * 4;
+        f.inside = (const int*)cur, cur += (size_t)it.n * 4;
+        f.pair_vals = (float*)cur, cur += (size_t)j * (dim + 1) * 4;
+        f.pair_steps = (int*)cur, cur += (size_t)j * 4;
+        cur = (char*)b->ws + (((size_t)(cur - (char*)b->ws) + 255) & ~(size_t)255);
+        b->total_jobs += j;
+        b->total_pts += it.n;
+    }
+    if (hipMalloc((void**)&b->dev_items, sizeof(FusedItem) * (host.empty() ? 1 : host.size())) != hipSuccess ||
+        hipMalloc((void**)&b->ctr, kQueueCtrs * sizeof(unsigned long long)) != hipSuccess) {
+        batch_free(b);
+        return fail(DW_ERR_HIP, "batch descriptor allocation failed");
+    }
+    if (!host.empty() &&
+        hipMemcpy(b->dev_items, host.data(), sizeof(FusedItem) * host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        batch_free(b);
+        return fail(DW_ERR_HIP, "batch descriptor upload failed");
+    }
+    *out = b;
+    return DW_OK;
+}
+
+int dw_batch_solve(dw_batch* b, void* stream) {
+    if (!b) return fail(DW_ERR_ARG, "null batch");
+    int dev = 0;
+    DW_HIP(hipGetDevice(&dev));
+    if (dev != b->device) return fail(DW_ERR_ARG, "batch was created on device " + std::to_string(b->device) + ", current is " + std::to_string(dev));
+    hipStream_t st = (hipStream_t)stream;
+    if (b->total_pts == 0) {
+        if (b->total_steps) return zero_u64(b->total_steps, st);
+        return DW_OK;
+    }
+    if (!b->use_graph) return batch_launch(b, st);
+    if (!b->exec) {
+        DW_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+        DW_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeRelaxed));
+        int rc = batch_launch(b, b->cap);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->cap, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(DW_ERR_HIP, std::string("graph capture failed: ") + hipGetErrorString(e));
+        b->graph = g;
+        DW_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    DW_HIP(hipGraphLaunch(b->exec, st));
+    return DW_OK;
+}
+
+void dw_batch_destroy(dw_batch* b) { batch_free(b); }
 
 }  // extern "C"

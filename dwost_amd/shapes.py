@@ -137,6 +137,10 @@ class _Scene:
     def n_vertices(self) -> int:
         return len(self.vertices)
 
+    @property
+    def n_prims(self) -> int:
+        return len(self.indices)
+
     # -- structure -------------------------------------------------------
     def info(self, sub: int = SUB_FULL) -> dict:
         out = np.zeros(8, np.int64)

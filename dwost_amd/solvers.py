@@ -40,22 +40,32 @@ class Solver:
     # the reference arithmetic).  False: the transcendental-free conservative
     # form, identical except where the literal test's rounding drops a silhouette.
     exact_cones: bool = False
+    # Launch shape of the walk kernels (never changes results; 0 = library
+    # default).  dwost_amd.autotune picks them per workload.
+    block_size: int = 0
+    blocks_per_cu: int = 0
+    spatial_order: int = 0  # 0 auto, 1 on, 2 off
 
-    def _params(self, seed: int) -> _lib.SolverParams:
+    def _params(self, seed: int, point_base: int = 0) -> _lib.SolverParams:
         return _lib.SolverParams(int(self.nwalks), int(self.nsteps), int(self.nsubwalks), int(self.control_variates),
                                  int(self.antithetic_variates), int(seed) & 0xFFFFFFFF, float(self.eps),
-                                 float(self.min_R), _lib.SOLVE_EXACT_CONES if self.exact_cones else 0)
+                                 float(self.min_R), _lib.SOLVE_EXACT_CONES if self.exact_cones else 0,
+                                 int(point_base), int(self.block_size), int(self.blocks_per_cu),
+                                 int(self.spatial_order))
 
     def _check(self, scene: _Scene):
         if scene.dim != self.dim:
             raise ValueError(f"{type(self).__name__} needs a {self.dim}D scene")
 
     def solve(self, p, scene: _Scene, seed: int = 0, per_walk: bool = False, count_steps: bool = False,
-              walk_flags: torch.Tensor | None = None):
+              walk_flags: torch.Tensor | None = None, point_base: int = 0):
         """u at each point: mean over ``nwalks`` WoSt walks (Solver.solve).
 
         ``walk_flags`` (uint8, n * nwalks, optional) receives per walk a bit per
-        subwalk that survived the outlier guard; the adjoint replays need it."""
+        subwalk that survived the outlier guard; the adjoint replays need it.
+        ``point_base`` is the global index of ``p[0]``: streams are keyed by the
+        global (point, walk) index, so slices of a point set reproduce the
+        whole-set result."""
         self._check(scene)
         p = _as_points(p, scene.dim)
         n, dev = p.shape[0], p.device
@@ -66,7 +76,7 @@ class Solver:
             walk_steps = torch.empty(n * self.nwalks, dtype=torch.int32, device=dev)
             walk_term = torch.empty(n * self.nwalks, dtype=torch.int32, device=dev)
         steps = torch.zeros(1, dtype=torch.int64, device=dev) if count_steps else None
-        prm = self._params(seed)
+        prm = self._params(seed, point_base)
         bc = scene.boundary.to_c(scene.n_vertices)
         if walk_flags is not None:
             _check_flags(walk_flags, n * self.nwalks, dev)
@@ -81,7 +91,7 @@ class Solver:
             out["steps"] = steps
         return out
 
-    def solve_dude(self, p, e, scene: _Scene, seed: int = 0, count_steps: bool = False):
+    def solve_dude(self, p, e, scene: _Scene, seed: int = 0, count_steps: bool = False, point_base: int = 0):
         """Directional derivative du/de (WoSt.solve_dude / WoSt3D.solve_dude)."""
         self._check(scene)
         p = _as_points(p, scene.dim)
@@ -89,14 +99,14 @@ class Solver:
         n, dev = p.shape[0], p.device
         out = torch.empty(n, device=dev)
         steps = torch.zeros(1, dtype=torch.int64, device=dev) if count_steps else None
-        prm = self._params(seed)
+        prm = self._params(seed, point_base)
         bc = scene.boundary.to_c(scene.n_vertices)
         check(lib().dw_solve_dude(scene.handle, ptr(p), ptr(e), n, C.byref(prm), C.byref(bc), ptr(out), ptr(steps),
                                   stream_handle(dev)))
         return (out, steps) if count_steps else out
 
     def solve_grad(self, p, scene: _Scene, seed: int = 0, per_pair: bool = False, count_steps: bool = False,
-                   pair_flags: torch.Tensor | None = None):
+                   pair_flags: torch.Tensor | None = None, point_base: int = 0):
         """Fused (u, grad u): both estimates come from the same antithetic walk pairs.
 
         ``pair_flags`` (uint8, n * ceil(nwalks / 2), optional) receives per pair
@@ -111,7 +121,7 @@ class Solver:
         pv = torch.empty(n * npairs, D + 1, device=dev) if per_pair else None
         ps = torch.empty(n * npairs, dtype=torch.int32, device=dev) if per_pair else None
         steps = torch.zeros(1, dtype=torch.int64, device=dev) if count_steps else None
-        prm = self._params(seed)
+        prm = self._params(seed, point_base)
         bc = scene.boundary.to_c(scene.n_vertices)
         if pair_flags is not None:
             _check_flags(pair_flags, n * npairs, dev)

@@ -85,7 +85,19 @@ typedef struct dw_solver_params { /* Solver / WoSt fields (solver.py:30-34, wost
     float eps;
     float min_R;
     int32_t flags;          /* DW_SOLVE_* bits                                      */
+    int64_t point_base;     /* global index of pts[0]: walk streams are keyed by the
+                               global (point, walk) index, so a launch over a slice
+                               of the points draws what one launch over all does  */
+    /* launch shape of the walk kernels; never changes results (0: defaults) */
+    int32_t block_size;     /* threads per block: 64, 128 or 256 (0: 256)           */
+    int32_t blocks_per_cu;  /* cap on resident blocks per CU (0: occupancy limit)  */
+    int32_t spatial_order;  /* DW_ORDER_AUTO / _ON / _OFF: sort points by Morton
+                               cell and give each XCD a contiguous range         */
 } dw_solver_params;
+
+#define DW_ORDER_AUTO 0 /* on from 8192 points */
+#define DW_ORDER_ON 1
+#define DW_ORDER_OFF 2
 
 /* Evaluate the SNCH cone test literally (acos/asin/atan2 per child, snch.py:46-97)
  * instead of the transcendental-free form |cos| <= sin(half angle + view angle)
@@ -179,6 +191,36 @@ int dw_solve_fused(const dw_scene* scene, const float* pts, int64_t n, const dw_
 int dw_adjoint_fused(const dw_scene* scene, const float* pts, int64_t n, const dw_solver_params* params,
                      const dw_boundary* boundary, const float* dl_du, const float* dl_dgrad, const uint8_t* pair_flags,
                      float* dl_dg, float* dl_dh, void* stream);
+
+/* Many scenes and query sets in one persistent launch (fused u + grad u).
+ * No reference counterpart (the reference solves one scene per call,
+ * solver.py:36-71); each item's results are bit-identical to dw_solve_fused
+ * with the same arguments, since walk streams are keyed by (point_base + i,
+ * pair) per item.  All items share the dimension and the DW_SOLVE_EXACT_CONES
+ * bit; pts / u / grad are device arrays that must outlive the batch (their
+ * contents may change between solves).  total_steps (device uint64, optional)
+ * receives the walk steps of each solve. */
+typedef struct dw_batch_item {
+    const dw_scene* scene;
+    const float* pts; /* [n][dim] */
+    int64_t n;
+    dw_solver_params params;
+    dw_boundary boundary;
+    float* u;    /* [n] */
+    float* grad; /* [n][dim] */
+} dw_batch_item;
+typedef struct dw_batch dw_batch;
+
+/* Capture the launch sequence into a hipGraph on the first solve and replay it
+ * afterwards (the sequence holds no memset and no host-memory reads). */
+#define DW_BATCH_GRAPH 1
+
+int dw_batch_create(const dw_batch_item* items, int32_t n_items, uint64_t* total_steps, int32_t flags,
+                    dw_batch** out);
+/* Enqueue one solve of every item on `stream`.  Solves of one batch must not
+ * overlap (they share its buffers and queue counters). */
+int dw_batch_solve(dw_batch* batch, void* stream);
+void dw_batch_destroy(dw_batch* batch);
 
 #ifdef __cplusplus
 }

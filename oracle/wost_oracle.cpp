@@ -1712,6 +1712,7 @@ struct Params {
     float eps, min_R;
     int cv, av;
     uint32_t seed;
+    int64_t base;  // global index of point 0: streams are keyed by (base + i, walk)
 };
 struct WalkOut {
     float value;
@@ -2282,6 +2283,7 @@ static Params make_params(const int32_t* ip, const float* fp) {
     P.cv = ip[3];
     P.av = ip[4];
     P.seed = (uint32_t)ip[5];
+    P.base = ip[6];
     P.eps = fp[0];
     P.min_R = fp[1];
     return P;
@@ -2301,17 +2303,18 @@ void orc_solve_u(void* s, const float* pts, int64_t n, const int32_t* ip, const 
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : bytes)
     for (int64_t i = 0; i < n; ++i) {
         uint32_t a, b;
-        tea(P.seed, (uint32_t)i, 4, kTeaKey, &a, &b);
+        tea(P.seed, (uint32_t)(P.base + i), 4, kTeaKey, &a, &b);
         double acc = 0.0;
         for (int64_t k = 0; k < W; ++k) {
-            uint64_t lane = (uint64_t)(i * W + k);
+            uint64_t lane = (uint64_t)((P.base + i) * W + k);
             Pcg rng = pcg_for_lane(a, b, lane);
             WalkOut w = walk_u(*sc, bc, P, pts + i * D, rng, bytes_total != nullptr);
             acc += (double)w.value;
             bytes += w.cnt.bytes;
-            if (walk_val) walk_val[lane] = w.value;
-            if (walk_steps) walk_steps[lane] = w.steps;
-            if (walk_term) walk_term[lane] = w.term_prim;
+            int64_t j = i * W + k;  // output slot (the stream lane is global)
+            if (walk_val) walk_val[j] = w.value;
+            if (walk_steps) walk_steps[j] = w.steps;
+            if (walk_term) walk_term[j] = w.term_prim;
         }
         if (mean) mean[i] = (float)(acc / (double)W);
     }
@@ -2329,7 +2332,7 @@ void orc_solve_dude(void* s, const float* pts, const float* e, int64_t n, const 
     uint64_t steps = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : steps)
     for (int64_t i = 0; i < n; ++i) {
-        Pcg rng = pcg_for(P.seed, (uint32_t)i, (uint64_t)i);
+        Pcg rng = pcg_for(P.seed, (uint32_t)(P.base + i), (uint64_t)(P.base + i));
         uint64_t st = 0;
         if (D == 2)
             out[i] = dude2(*sc, bc, P, v2(pts[2 * i], pts[2 * i + 1]), v2(e[2 * i], e[2 * i + 1]), rng, &st);
@@ -2364,19 +2367,20 @@ void orc_solve_fused(void* s, const float* pts, int64_t n, const int32_t* ip, co
             ins = inside3(*sc, v3(x[0], x[1], x[2]));
         }
         uint32_t a, b;
-        tea(P.seed, (uint32_t)i, 4, kTeaKey, &a, &b);
+        tea(P.seed, (uint32_t)(P.base + i), 4, kTeaKey, &a, &b);
         double su = 0.0, sg[3] = {0, 0, 0};
         for (int64_t k = 0; k < NP; ++k) {
-            uint64_t lane = (uint64_t)(i * NP + k);
+            uint64_t lane = (uint64_t)((P.base + i) * NP + k);
             Pcg rng = pcg_for_lane(a, b, lane);
             PairOut o = fused_pair(*sc, bc, P, x, R, ins, rng);
             su += o.usum;
             for (int q = 0; q < D; ++q) sg[q] += o.g[q];
             steps += (uint64_t)o.steps;
-            if (pair_u) pair_u[lane] = o.usum;
+            int64_t j = i * NP + k;  // output slot (the stream lane is global)
+            if (pair_u) pair_u[j] = o.usum;
             if (pair_g)
-                for (int q = 0; q < D; ++q) pair_g[lane * D + q] = o.g[q];
-            if (pair_steps) pair_steps[lane] = o.steps;
+                for (int q = 0; q < D; ++q) pair_g[j * D + q] = o.g[q];
+            if (pair_steps) pair_steps[j] = o.steps;
         }
         double nw = (double)(2 * NP);
         if (u) u[i] = (float)(su / nw);

@@ -210,3 +210,48 @@ def test_solve_matches_analytic_full_size(gpu, star_spec):
     err = np.sqrt(np.mean((_np(g["grad"])[far] - ga[far]) ** 2))
     print(f"grad rmse (R>0.05) {err:.4f}")
     assert err < 1.0
+
+
+def test_point_base_slices_match_whole(gpu, holes_spec):
+    """Streams are keyed by the global (point, walk) index: solving a slice of the
+    points with point_base = its offset reproduces the whole-set rows bit for bit
+    (the property a sharded launch relies on)."""
+    gs = gpu.Scene2D(holes_spec.vertices, holes_spec.indices, holes_spec.types,
+                     gpu.BoundaryCondition(holes_spec.bc_code))
+    q = holes_spec.queries[:96]
+    sol = gpu.WoSt(eps=1e-4, nwalks=32, nsteps=1024)
+    whole = sol.solve_grad(_t(q), gs, seed=11, per_pair=True)
+    part = sol.solve_grad(_t(q[40:72]), gs, seed=11, per_pair=True, point_base=40)
+    assert _same_bits(_np(part["pair_vals"]), _np(whole["pair_vals"])[40 * 16:72 * 16])
+    assert _same_bits(_np(part["u"]), _np(whole["u"])[40:72])
+    wu = sol.solve(_t(q), gs, seed=12, per_walk=True)["walk_u"]
+    pu = sol.solve(_t(q[7:19]), gs, seed=12, per_walk=True, point_base=7)["walk_u"]
+    assert _same_bits(_np(pu), _np(wu)[7 * 32:19 * 32])
+
+
+def test_escaped_walk_matches_oracle(gpu, oracle):
+    """A fused-estimator walk that leaves the noisy icosphere (81,920 triangles),
+    drifts to infinity, turns NaN and runs out its 1024 steps (global point 1712
+    of the 4096-point query set, pair 20, seed 1).  The kernels replace the NaN
+    steps by their fixed outcome; values and step counts still match the oracle,
+    which executes every step."""
+    import time
+
+    from dwost_amd import scenes
+
+    spec = scenes.icosphere_scene(level=6, n_query=4096)
+    gs = gpu.Scene3D(spec.vertices, spec.indices, spec.types, gpu.BoundaryCondition(spec.bc_code))
+    os_ = oracle.OracleScene(spec.vertices, spec.indices, spec.types)
+    q = spec.queries[1712:1713]
+    sol = gpu.WoSt3D(eps=1e-3, nwalks=512, nsteps=1024)
+    t = time.perf_counter()
+    g = sol.solve_grad(_t(q), gs, seed=1, per_pair=True, point_base=1712)
+    ps = _np(g["pair_steps"])
+    dt = time.perf_counter() - t
+    o = os_.solve_fused(q, spec.bc_code, 512, nsteps=1024, eps=1e-3, seed=1, per_pair=True, base=1712)
+    assert o["pair_steps"][20] == 1037  # the escaped pair
+    pv = _np(g["pair_vals"])
+    assert _same_bits(ps, o["pair_steps"])
+    assert _same_bits(pv[:, 0], o["pair_u"]) and _same_bits(pv[:, 1:], o["pair_g"])
+    print(f"escaped pair reproduced in {dt:.2f} s")
+    assert dt < 20.0

@@ -9,10 +9,10 @@ the built library raises.
 from .shapes import (BC_EXPCOS, BC_LINEAR, BC_PER_VERTEX, BC_QUAD, BoundaryCondition, BoundaryType, Scene2D,
                      Scene3D)
 from ._lib import SUB_DIRICHLET, SUB_FULL, SUB_NEUMANN, DwostError
-from .solvers import CustomSolver, Solver, WoSt, WoSt3D
+from .solvers import CustomSolver, Solver, WoS, WoS3D, WoSt, WoSt3D
 from .batch import BatchItem, BatchSolver
 
 __all__ = [
     "BC_EXPCOS", "BC_LINEAR", "BC_PER_VERTEX", "BC_QUAD", "BoundaryCondition", "BoundaryType", "Scene2D", "Scene3D",
-    "SUB_DIRICHLET", "SUB_FULL", "SUB_NEUMANN", "DwostError", "CustomSolver", "Solver", "WoSt", "WoSt3D", "BatchItem", "BatchSolver",
+    "SUB_DIRICHLET", "SUB_FULL", "SUB_NEUMANN", "DwostError", "CustomSolver", "Solver", "WoS", "WoS3D", "WoSt", "WoSt3D", "BatchItem", "BatchSolver",
 ]

@@ -43,6 +43,7 @@ class SolverParams(C.Structure):
                 ("block_size", C.c_int32), ("blocks_per_cu", C.c_int32), ("spatial_order", C.c_int32)]
 
 SOLVE_EXACT_CONES = 1
+SOLVE_WOS = 2
 
 
 class BatchItem(C.Structure):

@@ -24,6 +24,7 @@ HIPCC_FLAGS = [
     # every fp32 op rounds as written: no implicit FMA contraction
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-pthread",  # host scene builds use std::thread
 ]
 
 

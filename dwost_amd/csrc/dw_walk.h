@@ -216,9 +216,58 @@ DWD void walk_init(Walk<D>& w, typename VecT<D>::T x) {
     w.live = true;
 }
 
+// WoS.u (wos.py:16-36) / WoS3D.u (wos_3d.py:15-34): every boundary primitive
+// is Dirichlet (the full scene); one step is a closest-point query and a jump
+// to a uniform point on the largest empty ball.
+template <bool A>
+DWD bool wos_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk<2>& w, Rng& rng, const Adj& adj) {
+    const SubDev& F = sc.sub[0];
+    if (!(w.i < P.nsteps)) {
+        w.live = false;
+        return false;
+    }
+    CP c = closest2(F, w.ip);
+    float d = __builtin_fabsf(c.d);
+    w.i += 1;
+    if (d < P.eps) {
+        w.result += g2(bc, F, c);
+        if constexpr (A) adj_lin2(adj.dg, F, c.prim, c.t, adj.w);
+        w.term = c.prim;
+        w.live = false;
+        return false;
+    }
+    F2 dir = uniform_dir2(rng);
+    w.ip = w.ip + dir * d;
+    if (!(w.i < P.nsteps)) w.live = false;
+    return w.live;
+}
+template <bool A>
+DWD bool wos_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk<3>& w, Rng& rng, const Adj& adj) {
+    const SubDev& F = sc.sub[0];
+    if (!(w.i < P.nsteps)) {
+        w.live = false;
+        return false;
+    }
+    CP c = closest3(F, w.ip);
+    float d = __builtin_fabsf(c.d);
+    w.i += 1;
+    if (d < P.eps) {
+        w.result += g3(bc, F, c);
+        if constexpr (A) adj_lin3(adj.dg, F, c.prim, c.u, c.v, adj.w);
+        w.term = c.prim;
+        w.live = false;
+        return false;
+    }
+    F3 dir = uniform_sphere(rng);
+    w.ip = w.ip + dir * d;
+    if (!(w.i < P.nsteps)) w.live = false;
+    return w.live;
+}
+
 template <bool X, bool A = false>
 DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk<2>& w, Rng& rng,
                    const Adj& adj = Adj{0.0f, nullptr, nullptr}) {
+    if (P.flags & DW_SOLVE_WOS) return wos_step<A>(sc, bc, P, w, rng, adj);
     const SubDev& F = sc.sub[0];
     const SubDev& Dn = sc.sub[1];
     const SubDev& Nm = sc.sub[2];
@@ -311,6 +360,7 @@ DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk
 template <bool X, bool A = false>
 DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk<3>& w, Rng& rng,
                    const Adj& adj = Adj{0.0f, nullptr, nullptr}) {
+    if (P.flags & DW_SOLVE_WOS) return wos_step<A>(sc, bc, P, w, rng, adj);
     const SubDev& F = sc.sub[0];
     const SubDev& Dn = sc.sub[1];
     const SubDev& Nm = sc.sub[2];
@@ -402,11 +452,12 @@ DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk
 }
 
 // final value of a finished walk: 2D guard (wost.py:218-219); 3D has none
-DWD float walk_value(const Walk<2>& w) {
+DWD float walk_value(const Walk<2>& w, const ParamsDev& P) {
     float r = w.result;
+    if (P.flags & DW_SOLVE_WOS) return r;  // WoS.u has no guard
     return (bad(r) || __builtin_fabsf(r) > 100.0f) ? 0.0f : r;
 }
-DWD float walk_value(const Walk<3>& w) { return w.result; }
+DWD float walk_value(const Walk<3>& w, const ParamsDev&) { return w.result; }
 
 template <int D, bool X = true>
 DWD float walk_u(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, typename VecT<D>::T x, Rng& rng,
@@ -417,7 +468,7 @@ DWD float walk_u(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, typena
     }
     steps = w.i;
     term = w.term;
-    return walk_value(w);
+    return walk_value(w, P);
 }
 DWD float walk_u2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, F2 x, Rng& rng, int& steps, int& term) {
     return walk_u<2>(sc, bc, P, x, rng, steps, term);

@@ -375,7 +375,7 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
         }
         if (__ballot(!done) == 0) break;
         if (job >= 0 && !walk_step<X>(sc, bc, P, w, rng)) {
-            float v = walk_value(w);
+            float v = walk_value(w, P);
             if (v == w.result) kept |= 1u << (sub & 7);
             res += v;
             steps += w.i;
@@ -507,7 +507,7 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
         }
         if (__ballot(!done) == 0) break;
         if (job >= 0 && !walk_step<X>(*ITEM(sc), ITEM(bc), ITEM(P), w, rng)) {
-            float v = walk_value(w);
+            float v = walk_value(w, ITEM(P));
             steps += w.i;
             // walks the guards zero contribute nothing to the adjoint
             bool kept = D == 2 ? v == w.result : !(bad(v) || __builtin_fabsf(v) > 10000.0f);
@@ -770,7 +770,61 @@ __global__ void __launch_bounds__(kBlock) k_dude(SceneDev sc, BCDev bc, ParamsDe
     int wi = 0;
     unsigned long long steps = 0;
     int st, tm;
-    if (sc.dim == 2) {
+    if (P.flags & DW_SOLVE_WOS) {
+        // WoS.dude (wos.py:38-105) / WoS3D.dude (wos_3d.py:36-102): separate
+        // source direction, running-sum control variates, no outlier guards
+        const int D = sc.dim;
+        F3 p = f3(pts[D * i], pts[D * i + 1], D == 3 ? pts[D * i + 2] : 0.0f);
+        F3 ev = f3(e[D * i], e[D * i + 1], D == 3 ? e[D * i + 2] : 0.0f);
+        float u_sum = 0.0f, u_count = 0.0f, src_sum = 0.0f, src_count = 0.0f;
+        float gnorm = D == 2 ? R * R / 4.0f : R * R / 6.0f;
+        while ((wi < P.nwalks) && active) {
+            F3 dir, dir_src, p_src;
+            float r_src, pdf_src;
+            if (D == 2) {
+                F2 d2 = uniform_dir2(rng), s2 = uniform_dir2(rng), q2;
+                greens2_sample(f2(p.x, p.y), R, s2, rng, q2, r_src, pdf_src);
+                dir = f3(d2.x, d2.y, 0.0f), dir_src = f3(s2.x, s2.y, 0.0f), p_src = f3(q2.x, q2.y, 0.0f);
+            } else {
+                dir = uniform_sphere(rng);
+                dir_src = uniform_sphere(rng);
+                greens3_sample(p, R, dir_src, rng, p_src, r_src, pdf_src);
+            }
+            int niters = P.av ? 2 : 1;
+            uint64_t state = rng.state;
+            int it = 0;
+            while ((it < niters) && (wi < P.nwalks)) {
+                if (it == 1) {
+                    dir_src = -dir_src;
+                    p_src = p * 2.0f - p_src;
+                }
+                float src = gnorm * src_f(bc);
+                float src_grad;
+                if (D == 2) {
+                    F2 gr = greens2_gradient(f2(p.x, p.y), R, r_src, f2(p_src.x, p_src.y));
+                    src_grad = dot(gr, f2(ev.x, ev.y)) / (pdf_src * gnorm);
+                } else {
+                    src_grad = dot(greens3_gradient(p, R, r_src, p_src), ev) / (pdf_src * gnorm);
+                }
+                src_sum += src;
+                src_count += 1.0f;
+                float _src = P.cv ? src - (src_sum / src_count) : src;
+                result += _src * src_grad;
+                if (it == 1) dir = -dir;
+                F3 _p = p + dir * R;
+                rng.state = state;
+                float u = D == 2 ? walk_u2(sc, bc, P, f2(_p.x, _p.y), rng, st, tm) : walk_u3(sc, bc, P, _p, rng, st, tm);
+                steps += st;
+                u_sum += u;
+                u_count += 1.0f;
+                float _u = P.cv ? u - u_sum / u_count : u;
+                float de = D == 2 ? dot(f2(ev.x, ev.y), f2(dir.x, dir.y)) : dot(ev, dir);
+                result += (D == 2 ? 2.0f : 3.0f) / R * _u * de;
+                wi += 1;
+                it += 1;
+            }
+        }
+    } else if (sc.dim == 2) {
         F2 p = f2(pts[2 * i], pts[2 * i + 1]), ev = f2(e[2 * i], e[2 * i + 1]);
         float u_sum = 0.0f, u_count = 0.0f, src_sum = 0.0f, src_count = 0.0f;
         float gnorm = R * R / 4.0f;
@@ -1586,6 +1640,7 @@ int dw_solve_fused(const dw_scene* s, const float* pts, int64_t n, const dw_solv
     if ((rc = check_scene(s, 0))) return rc;
     if ((rc = check_params(params)) || (rc = check_bc(boundary))) return rc;
     if (n < 0 || (n > 0 && (!pts || !u || !grad))) return fail(DW_ERR_ARG, "bad arrays");
+    if (params->flags & DW_SOLVE_WOS) return fail(DW_ERR_ARG, "the fused estimator runs WoSt walks only");
     if (n == 0) return DW_OK;
     const int D = s->dev.dim;
     const int npairs = (params->nwalks + 1) / 2;
@@ -1655,6 +1710,7 @@ int dw_adjoint_fused(const dw_scene* s, const float* pts, int64_t n, const dw_so
     if ((rc = check_params(params)) || (rc = check_bc(boundary))) return rc;
     if (boundary->bc_code != DW_BC_PER_VERTEX)
         return fail(DW_ERR_ARG, "the adjoint needs per-vertex boundary data (DW_BC_PER_VERTEX)");
+    if (params->flags & DW_SOLVE_WOS) return fail(DW_ERR_ARG, "the fused estimator runs WoSt walks only");
     if (n < 0 || (n > 0 && (!pts || !dl_du || !dl_dgrad || !pair_flags || !dl_dg || !dl_dh)))
         return fail(DW_ERR_ARG, "bad arrays");
     if (n == 0) return DW_OK;
@@ -1725,6 +1781,8 @@ int dw_batch_create(const dw_batch_item* items, int32_t n_items, uint64_t* total
         if ((it.params.flags & DW_SOLVE_EXACT_CONES) != exact)
             return fail(DW_ERR_ARG, "batch item " + std::to_string(k) + ": all items need the same cone mode");
         if ((rc = check_params(&it.params)) || (rc = check_bc(&it.boundary))) return rc;
+        if (it.params.flags & DW_SOLVE_WOS)
+            return fail(DW_ERR_ARG, "batch item " + std::to_string(k) + ": the fused estimator runs WoSt walks only");
         if (it.n < 0 || (it.n > 0 && (!it.pts || !it.u || !it.grad)))
             return fail(DW_ERR_ARG, "batch item " + std::to_string(k) + ": bad arrays");
     }

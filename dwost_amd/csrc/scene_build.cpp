@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <unordered_map>
+#include <thread>
 #include <utility>
 
 namespace dwb {
@@ -619,13 +620,19 @@ std::string build_scene(int dim, const float* V, int64_t nv, const int32_t* I, c
     out.V.assign(V, V + nv * dim);
     out.I.assign(I, I + np * dim);
     out.types.assign(types, types + np);
-    for (int s = 0; s < 3; ++s) {
+    // The three sub-scenes (full, Dirichlet, Neumann) are independent builds
+    // over the same read-only input: one thread each.
+    auto build_one = [&out, dim](int s) {
         int want = s == 0 ? -1 : (s == 1 ? 1 : 0);
         if (dim == 2)
             make_sub<2>(out, want, out.sub[s]);
         else
             make_sub<3>(out, want, out.sub[s]);
-    }
+    };
+    std::thread t1(build_one, 1), t2(build_one, 2);
+    build_one(0);
+    t1.join();
+    t2.join();
     return "";
 }
 

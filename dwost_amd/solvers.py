@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass
+from typing import ClassVar
 
 import torch
 
@@ -46,10 +47,13 @@ class Solver:
     blocks_per_cu: int = 0
     spatial_order: int = 0  # 0 auto, 1 on, 2 off
 
+    _walk_flag: ClassVar[int] = 0  # walk family (WoSt; WoS sets DW_SOLVE_WOS)
+
     def _params(self, seed: int, point_base: int = 0) -> _lib.SolverParams:
+        flags = (_lib.SOLVE_EXACT_CONES if self.exact_cones else 0) | self._walk_flag
         return _lib.SolverParams(int(self.nwalks), int(self.nsteps), int(self.nsubwalks), int(self.control_variates),
                                  int(self.antithetic_variates), int(seed) & 0xFFFFFFFF, float(self.eps),
-                                 float(self.min_R), _lib.SOLVE_EXACT_CONES if self.exact_cones else 0,
+                                 float(self.min_R), flags,
                                  int(point_base), int(self.block_size), int(self.blocks_per_cu),
                                  int(self.spatial_order))
 
@@ -283,3 +287,25 @@ class WoSt3D(Solver):
     antithetic_variates: bool = False
     min_R: float = 1e-3
     dim: int = 3
+
+
+@dataclass
+class WoS(Solver):
+    """2D Walk on Spheres (diff_wost/solvers/wos.py:10-105): every boundary
+    primitive is Dirichlet; ``solve`` and ``solve_dude`` only."""
+
+    control_variates: bool = True
+    antithetic_variates: bool = True
+    use_bvh: bool = False  # reference field; the scene is always a BVH here
+    dim: int = 2
+    _walk_flag: ClassVar[int] = _lib.SOLVE_WOS
+
+
+@dataclass
+class WoS3D(Solver):
+    """3D Walk on Spheres (diff_wost/solvers/wos_3d.py:10-102)."""
+
+    control_variates: bool = True
+    antithetic_variates: bool = True
+    dim: int = 3
+    _walk_flag: ClassVar[int] = _lib.SOLVE_WOS

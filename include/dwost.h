@@ -104,6 +104,10 @@ typedef struct dw_solver_params { /* Solver / WoSt fields (solver.py:30-34, wost
  * with a 4e-6 conservative margin; the two differ only where the reference's
  * own rounding would prune a node holding the nearest silhouette. */
 #define DW_SOLVE_EXACT_CONES 1
+/* Plain Walk on Spheres (WoS.u / WoS.dude, wos.py:16-105; WoS3D, wos_3d.py:15-102):
+ * every boundary primitive is treated as Dirichlet, each step jumps to a uniform
+ * point on the largest empty ball.  dw_solve_u and dw_solve_dude only. */
+#define DW_SOLVE_WOS 2
 
 const char* dw_last_error(void);
 int dw_version(void);

@@ -145,9 +145,9 @@ class OracleScene:
         return lib().orc_ref_topology(self.h, sub, ref)
 
     @staticmethod
-    def _params(nwalks, nsteps, nsubwalks, cv, av, seed, eps, min_R, base=0):
-        ip = np.array([nwalks, nsteps, nsubwalks, int(cv), int(av), np.uint32(seed).astype(np.int32), int(base)],
-                      np.int32)
+    def _params(nwalks, nsteps, nsubwalks, cv, av, seed, eps, min_R, base=0, wos=False):
+        ip = np.array([nwalks, nsteps, nsubwalks, int(cv), int(av), np.uint32(seed).astype(np.int32), int(base),
+                       int(wos)], np.int32)
         fp = np.array([eps, min_R], np.float32)
         return ip, fp
 
@@ -160,10 +160,10 @@ class OracleScene:
 
     def solve_u(self, pts, bc_code, nwalks, nsteps=128, eps=1e-3, min_R=1e-3, cv=True, av=True, seed=0,
                 nsubwalks=1, coeffs=None, gv=None, hv=None, src_code=0, src_c=0.0, per_walk=False,
-                count_bytes=False, base=0):
+                count_bytes=False, base=0, wos=False):
         pts = _f32(pts)
         n = len(pts)
-        ip, fp = self._params(nwalks, nsteps, nsubwalks, cv, av, seed, eps, min_R, base)
+        ip, fp = self._params(nwalks, nsteps, nsubwalks, cv, av, seed, eps, min_R, base, wos)
         c, gv, hv = self._bc(bc_code, coeffs, gv, hv)
         mean = np.zeros(n, np.float32)
         wv = np.zeros(n * nwalks, np.float32) if per_walk else None
@@ -180,11 +180,11 @@ class OracleScene:
         return out
 
     def solve_dude(self, pts, e, bc_code, nwalks, nsteps=128, eps=1e-3, min_R=1e-3, cv=True, av=True, seed=0,
-                   coeffs=None, gv=None, hv=None, src_code=0, src_c=0.0, base=0):
+                   coeffs=None, gv=None, hv=None, src_code=0, src_c=0.0, base=0, wos=False):
         pts = _f32(pts)
         e = _f32(e)
         n = len(pts)
-        ip, fp = self._params(nwalks, nsteps, 1, cv, av, seed, eps, min_R, base)
+        ip, fp = self._params(nwalks, nsteps, 1, cv, av, seed, eps, min_R, base, wos)
         c, gv, hv = self._bc(bc_code, coeffs, gv, hv)
         out = np.zeros(n, np.float32)
         st = np.zeros(1, np.uint64)

@@ -1713,6 +1713,7 @@ struct Params {
     int cv, av;
     uint32_t seed;
     int64_t base;  // global index of point 0: streams are keyed by (base + i, walk)
+    int wos;       // plain Walk on Spheres (wos.py / wos_3d.py) instead of WoSt
 };
 struct WalkOut {
     float value;
@@ -1911,6 +1912,39 @@ static WalkOut wost_u3(const Scene& sc, const BC& bc, const Params& P, V3 x, Pcg
     return out;
 }
 
+// WoS.u (wos.py:16-36) / WoS3D.u (wos_3d.py:15-34): every boundary primitive
+// is Dirichlet; step to a uniform point on the largest empty ball.
+template <int D>
+static WalkOut wos_u(const Scene& sc, const BC& bc, const Params& P, const float* x, Pcg& rng) {
+    WalkOut out;
+    out.term_prim = -1;
+    float result = 0.0f;
+    int i = 0;
+    bool active = true;
+    V2 p2 = v2(x[0], x[1]);
+    V3 p3 = D == 3 ? v3(x[0], x[1], x[2]) : v3(0, 0, 0);
+    while ((i < P.nsteps) && active) {
+        CP cp = D == 2 ? bvh2_closest(sc.s2[0].tree, p2, nullptr) : snch3_closest(sc.s3[0].tree, p3, nullptr);
+        float d = std::fabs(cp.d);  // scene.distance
+        if (d < P.eps) {
+            // scene.closest_point: the same query
+            result += D == 2 ? g_of<2>(bc, sc.s2[0], cp) : g_of<3>(bc, sc.s3[0], cp);
+            out.term_prim = cp.prim;
+            active = false;
+        } else if (D == 2) {
+            V2 dir = uniform_dir2(rng);
+            p2 = add(p2, mul(dir, d));
+        } else {
+            V3 dir = uniform_sphere(rng);
+            p3 = add(p3, mul(dir, d));
+        }
+        i += 1;
+    }
+    out.value = result;
+    out.steps = i;
+    return out;
+}
+
 // Solver.walk (solver.py:73-92): nsubwalks sequential walks, averaged
 static WalkOut walk_u(const Scene& sc, const BC& bc, const Params& P, const float* x, Pcg& rng, bool count) {
     WalkOut acc;
@@ -1919,8 +1953,9 @@ static WalkOut walk_u(const Scene& sc, const BC& bc, const Params& P, const floa
     acc.term_prim = -1;
     float res = 0.0f;
     for (int s = 0; s < P.nsubwalks; ++s) {
-        WalkOut w = sc.dim == 2 ? wost_u2(sc, bc, P, v2(x[0], x[1]), rng, false, v2(0, 0), count)
-                                : wost_u3(sc, bc, P, v3(x[0], x[1], x[2]), rng, false, v3(0, 0, 0), count);
+        WalkOut w = P.wos ? (sc.dim == 2 ? wos_u<2>(sc, bc, P, x, rng) : wos_u<3>(sc, bc, P, x, rng))
+                    : sc.dim == 2 ? wost_u2(sc, bc, P, v2(x[0], x[1]), rng, false, v2(0, 0), count)
+                                  : wost_u3(sc, bc, P, v3(x[0], x[1], x[2]), rng, false, v3(0, 0, 0), count);
         res += w.value;
         acc.steps += w.steps;
         acc.term_prim = w.term_prim;
@@ -2027,6 +2062,70 @@ static float dude3(const Scene& sc, const BC& bc, const Params& P, V3 p, V3 e, P
         }
     }
     if (bad(result) || std::fabs(result) > 10000.0f) result = 0.0f;
+    return result / (float)P.nwalks;
+}
+
+// WoS.dude (wos.py:38-105) / WoS3D.dude (wos_3d.py:36-102): antithetic pairs
+// from the largest empty ball with running-sum control variates; the source
+// direction is a separate draw in both dimensions; no outlier guards.
+template <int D>
+static float wos_dude(const Scene& sc, const BC& bc, const Params& P, const float* x, const float* ev, Pcg& rng,
+                      uint64_t* steps) {
+    V3 p = v3(x[0], x[1], D == 3 ? x[2] : 0.0f), e = v3(ev[0], ev[1], D == 3 ? ev[2] : 0.0f);
+    float R = D == 2 ? std::fabs(bvh2_closest(sc.s2[0].tree, v2(p.x, p.y), nullptr).d)
+                     : std::fabs(snch3_closest(sc.s3[0].tree, p, nullptr).d);
+    bool active = D == 2 ? inside2(sc, v2(p.x, p.y)) : inside3(sc, p);
+    float u_sum = 0.0f, u_count = 0.0f, src_sum = 0.0f, src_count = 0.0f, result = 0.0f;
+    float gnorm = D == 2 ? R * R / 4.0f : R * R / 6.0f;
+    int i = 0;
+    while ((i < P.nwalks) && active) {
+        V3 dir, dir_src, p_src;
+        float r_src, pdf_src;
+        if (D == 2) {
+            V2 d2 = uniform_dir2(rng), s2 = uniform_dir2(rng), q2;
+            greens2_sample(v2(p.x, p.y), R, s2, rng, q2, r_src, pdf_src);
+            dir = v3(d2.x, d2.y, 0.0f), dir_src = v3(s2.x, s2.y, 0.0f), p_src = v3(q2.x, q2.y, 0.0f);
+        } else {
+            dir = uniform_sphere(rng);
+            dir_src = uniform_sphere(rng);
+            greens3_sample(p, R, dir_src, rng, p_src, r_src, pdf_src);
+        }
+        int niters = P.av ? 2 : 1;
+        uint64_t state = rng.state;
+        int it = 0;
+        while ((it < niters) && (i < P.nwalks)) {
+            if (it == 1) {
+                dir_src = neg(dir_src);
+                p_src = sub(mul(p, 2.0f), p_src);
+            }
+            float src = gnorm * src_f(bc);
+            float src_grad;
+            if (D == 2) {
+                V2 gr = greens2_gradient(v2(p.x, p.y), R, r_src, v2(p_src.x, p_src.y));
+                src_grad = dot(gr, v2(e.x, e.y)) / (pdf_src * gnorm);
+            } else {
+                src_grad = dot(greens3_gradient(p, R, r_src, p_src), e) / (pdf_src * gnorm);
+            }
+            src_sum += src;
+            src_count += 1.0f;
+            float _src = P.cv ? src - (src_sum / src_count) : src;
+            result += _src * src_grad;
+            if (it == 1) dir = neg(dir);
+            V3 _p = add(p, mul(dir, R));
+            rng.state = state;
+            float xp[3] = {_p.x, _p.y, _p.z};
+            WalkOut w = wos_u<D>(sc, bc, P, xp, rng);
+            if (steps) *steps += (uint64_t)w.steps;
+            float u = w.value;
+            u_sum += u;
+            u_count += 1.0f;
+            float _u = P.cv ? u - u_sum / u_count : u;
+            float de = D == 2 ? dot(v2(e.x, e.y), v2(dir.x, dir.y)) : dot(e, dir);
+            result += (D == 2 ? 2.0f : 3.0f) / R * _u * de;
+            i += 1;
+            it += 1;
+        }
+    }
     return result / (float)P.nwalks;
 }
 
@@ -2284,6 +2383,7 @@ static Params make_params(const int32_t* ip, const float* fp) {
     P.av = ip[4];
     P.seed = (uint32_t)ip[5];
     P.base = ip[6];
+    P.wos = ip[7];
     P.eps = fp[0];
     P.min_R = fp[1];
     return P;
@@ -2334,7 +2434,10 @@ void orc_solve_dude(void* s, const float* pts, const float* e, int64_t n, const 
     for (int64_t i = 0; i < n; ++i) {
         Pcg rng = pcg_for(P.seed, (uint32_t)(P.base + i), (uint64_t)(P.base + i));
         uint64_t st = 0;
-        if (D == 2)
+        if (P.wos)
+            out[i] = D == 2 ? wos_dude<2>(*sc, bc, P, pts + 2 * i, e + 2 * i, rng, &st)
+                            : wos_dude<3>(*sc, bc, P, pts + 3 * i, e + 3 * i, rng, &st);
+        else if (D == 2)
             out[i] = dude2(*sc, bc, P, v2(pts[2 * i], pts[2 * i + 1]), v2(e[2 * i], e[2 * i + 1]), rng, &st);
         else
             out[i] = dude3(*sc, bc, P, v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]),

@@ -116,6 +116,8 @@ struct SubDev {
     const float* cdf;
     const float* pmf;
     float pmf_sum, pmf_norm;
+    const int* cdf_guide;  // (1 << guide_shift) + 1 bracket starts for the CDF search
+    int guide_shift;
     const float* local_prims;  // dim*dim floats per primitive (a, b[, c])
     const int* local_vglobal;  // dim global vertex ids per primitive
     const float* vnormal;      // 2D vertex normals (local vertex order), full scene only

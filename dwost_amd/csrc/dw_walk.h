@@ -76,11 +76,14 @@ DWD void adj_lin3(float* buf, const SubDev& S, int prim, float u, float v, float
     atomicAdd(buf + S.local_vglobal[3 * prim + 2], w * v);
 }
 
-// DiscreteDistribution.sample_reuse_pmf (distr.py:52-107) with Dr.Jit's
-// binary_search over [0, n-1] (log2(n-1)+1 halvings).
+// DiscreteDistribution.sample_reuse_pmf (distr.py:52-107): Dr.Jit's
+// binary_search (the first index whose predicate fails, or n - 1), run inside
+// the bracket the host's guide table gives for the draw: the predicate is
+// monotone in the index and the draw, so the bracket holds that same index.
 DWD int cdf_sample(const SubDev& S, float value, float& reuse, float& pmf) {
     float sample = value * S.pmf_sum;
-    uint32_t start = 0, end = (uint32_t)(S.n_prims - 1);
+    int k = (int)(value * (float)(1 << S.guide_shift));  // exact scaling, value < 1
+    uint32_t start = (uint32_t)S.cdf_guide[k], end = (uint32_t)S.cdf_guide[k + 1];
     int iters = start < end ? (32 - __builtin_clz(end - start)) : 0;
     for (int it = 0; it < iters; ++it) {
         uint32_t mid = (start + end) >> 1;

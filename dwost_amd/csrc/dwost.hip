@@ -1131,6 +1131,9 @@ int upload_sub(dw_scene* s, const dwb::SubHost& H, SubDev& S) {
     S.cdf = (const float*)p;
     if ((rc = upload(s, H.pmf, &p))) return rc;
     S.pmf = (const float*)p;
+    if ((rc = upload(s, H.cdf_guide, &p))) return rc;
+    S.cdf_guide = (const int*)p;
+    S.guide_shift = H.guide_shift;
     std::vector<float> lp(np * D * D);
     std::vector<int> lvg(np * D), lvi(np * D);
     for (size_t j = 0; j < np; ++j)

@@ -431,6 +431,38 @@ void face_bounds3(SubHost& S) {
     }
 }
 
+// The index DiscreteDistribution.sample draws for a value u (distr.py:52-70):
+// the first i whose predicate ((cdf < u*sum) || cdf == 0) && cdf != sum fails,
+// n - 1 when none does.  The predicate is monotone in i (cdf is non-decreasing
+// and non-negative) and in u, so the index is non-decreasing in u; a table of
+// the indices at u = k/K therefore brackets the index of every u in
+// [k/K, (k+1)/K), and a search inside the bracket finds the same index.
+int cdf_first_false(const SubHost& S, float sample) {
+    int lo = 0, hi = (int)S.cdf.size() - 1;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        float cv = S.cdf[mid];
+        bool cond = ((cv < sample) || (cv == 0.0f)) && (cv != S.pmf_sum);
+        if (cond)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+void cdf_guide(SubHost& S) {
+    size_t n = S.cdf.size();
+    int shift = 0;
+    while (shift < 16 && ((size_t)1 << shift) * 4 < n) ++shift;
+    S.guide_shift = shift;
+    int K = 1 << shift;
+    S.cdf_guide.resize((size_t)K + 1);
+    for (int k = 0; k <= K; ++k) {
+        float u = (float)k / (float)K;  // exact: K is a power of two
+        S.cdf_guide[k] = n ? cdf_first_false(S, u * S.pmf_sum) : 0;
+    }
+}
+
 template <int D>
 void finish_sub(SubHost& S) {
     S.present = true;
@@ -467,6 +499,7 @@ void finish_sub(SubHost& S) {
     }
     S.pmf_sum = np ? S.cdf[np - 1] : 0.0f;
     S.pmf_norm = 1.0f / S.pmf_sum;
+    cdf_guide(S);
     if (D == 2) {
         // vertex normals (polyline.py:173-186): scatter edge normals to .x then .y
         S.vnormal.assign(nv * 2, 0.0f);

@@ -26,6 +26,10 @@ struct SubHost {
     bool py_sils = false;
     std::vector<float> pmf, cdf;
     float pmf_sum = 0.0f, pmf_norm = 0.0f;
+    // guide for the CDF search: a draw u in [k/K, (k+1)/K) selects an index in
+    // [cdf_guide[k], cdf_guide[k+1]] (K = 1 << guide_shift)
+    std::vector<int> cdf_guide;
+    int guide_shift = 0;
     std::vector<float> vnormal;  // 2D only
     // BVH / SNCH (node-major)
     int max_depth = 0;

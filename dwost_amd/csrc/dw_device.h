@@ -208,38 +208,34 @@ constexpr int kLdsStack = 16;
 constexpr int kStackLdsBytesPerThread = kLdsStack * 8;
 extern __shared__ int dw_lds[];
 
-struct Stk {
-    int* si;
-    float* sd;
-    int stride, sp;
+// Stack entries below the LDS part (per lane, private memory).  Kept apart
+// from the traversal state so that state stays in registers.
+struct Ovf {
     int oi[DW_MAX_STACK - kLdsStack];
     float od[DW_MAX_STACK - kLdsStack];
-    DWD void push(int r, float d) {
-        if (sp < kLdsStack) {
-            si[sp * stride] = r;
-            sd[sp * stride] = d;
-        } else {
-            oi[sp - kLdsStack] = r;
-            od[sp - kLdsStack] = d;
-        }
-        ++sp;
-    }
-    DWD void pop(int& r, float& d) {
-        --sp;
-        if (sp < kLdsStack) {
-            r = si[sp * stride];
-            d = sd[sp * stride];
-        } else {
-            r = oi[sp - kLdsStack];
-            d = od[sp - kLdsStack];
-        }
-    }
 };
-DWD void stack_init(Stk& st) {
-    st.stride = (int)blockDim.x;
-    st.si = dw_lds + threadIdx.x;
-    st.sd = (float*)(dw_lds + kLdsStack * blockDim.x) + threadIdx.x;
-    st.sp = 0;
+// LDS slot of entry k of this lane, addressed from dw_lds at each access so
+// the accesses stay LDS instructions
+DWD int stk_slot(int k) { return k * (int)blockDim.x + (int)threadIdx.x; }
+DWD void stk_push(int& sp, Ovf& o, int r, float d) {
+    if (sp < kLdsStack) {
+        dw_lds[stk_slot(sp)] = r;
+        ((float*)dw_lds)[stk_slot(sp + kLdsStack)] = d;
+    } else {
+        o.oi[sp - kLdsStack] = r;
+        o.od[sp - kLdsStack] = d;
+    }
+    ++sp;
+}
+DWD void stk_pop(int& sp, Ovf& o, int& r, float& d) {
+    --sp;
+    if (sp < kLdsStack) {
+        r = dw_lds[stk_slot(sp)];
+        d = ((float*)dw_lds)[stk_slot(sp + kLdsStack)];
+    } else {
+        r = o.oi[sp - kLdsStack];
+        d = o.od[sp - kLdsStack];
+    }
 }
 
 // leaf reference -> [off, off + cnt)
@@ -256,16 +252,124 @@ DWD void leaf_range(const SubDev& S, int ref, int& off, int& cnt) {
     }
 }
 
+template <int D>
+struct VecT;
+template <>
+struct VecT<2> {
+    using T = F2;
+};
+template <>
+struct VecT<3> {
+    using T = F3;
+};
 // ---------------------------------------------------------------------------
+// Resumable traversals.  A query starts with *_begin, advances by one loop
+// iteration (an interior record's two child tests, or one leaf) per *_iter
+// call, which returns true once the traversal has finished, and is read with
+// *_end.  The synchronous queries run the iterations back to back; the
+// persistent walk kernels interleave them with the other lanes' walk logic, so
+// a lane whose traversal ends early moves on instead of idling until the
+// wave's longest traversal ends.  During the traversal only the winning
+// primitive's slot is kept: *_end recomputes its record with the same
+// arithmetic (the primitive tests are pure functions of query and primitive).
+// ---------------------------------------------------------------------------
+enum : int { TQ_CLOSEST = 0, TQ_SIL = 1, TQ_ISECT = 2 };
+struct Trav {
+    const SubDev* S;
+    const void* rec;    // child-in-parent records of the tree walked (BVH or SNCH)
+    const void* prims;  // its primitives (segments / triangles) or silhouettes
+    F3 x;         // query point / ray origin (2D: z = 0)
+    F3 v;         // ray direction (intersection)
+    float r_max;  // pruning radius; silhouettes: the running nearest distance
+    float d;      // closest point: best distance so far
+    float cd;     // lower bound of the pending node
+    int ref;      // pending node (>= 0) or leaf (< 0) reference
+    int best;     // winning primitive / silhouette slot, -1: none
+    int kind;     // TQ_*
+    int sp;       // traversal stack depth
+};
+
+// Tree records and primitives are read through global-address-space pointers:
+// global loads wait on the vector-memory counter only, not on LDS traffic.
+#define DW_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+DWD T gload(const void* base, int i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const DW_GLOBAL T*)base)[i];
+#else  // host pass of the device code: never executed
+    return ((const T*)base)[i];
+#endif
+}
+
 // 2D: BVH closest point (bvh.py:111-188 + line_segment.py:74-85).  Visit order
 // and pruning are the reference's: both children tested in order with r_max
 // tightened by each d_max, nearer child first, `cd > r_max` skips.
-// ---------------------------------------------------------------------------
-DWD CP closest2(const SubDev& S, F2 p) {
-    const RBvh2* recs = (const RBvh2*)S.rec_bvh;
-    const DSeg* prims = (const DSeg*)S.prims;
-    Stk st;
-    stack_init(st);
+DWD void closest2_begin(Trav& t, const SubDev& S, F2 p) {
+    t.kind = TQ_CLOSEST;
+    t.S = &S;
+    t.rec = S.rec_bvh;
+    t.prims = S.prims;
+    t.x = f3(p.x, p.y, 0.0f);
+    t.d = kInf;
+    t.best = -1;
+    t.r_max = kInf;
+    t.ref = S.root_ref_bvh;
+    t.cd = kInf;
+    t.sp = 0;
+}
+DWD bool closest2_iter(Trav& t, Ovf& o) {
+    const SubDev& S = *t.S;
+    F2 p = f2(t.x.x, t.x.y);
+    if (!(t.cd > t.r_max)) {
+        if (t.ref < 0) {
+            int off, cnt;
+            leaf_range(S, t.ref, off, cnt);
+            for (int j = 0; j < cnt; ++j) {
+                DSeg sg = gload<DSeg>(t.prims, off + j);
+                F2 a = f2(sg.ax, sg.ay), b = f2(sg.bx, sg.by);
+                F2 pa = p - a, ba = b - a;
+                float h = clampf(dot(pa, ba) / dot(ba, ba), 0.0f, 1.0f);
+                F2 i = a + ba * h;
+                float d = norm(i - p);
+                if (d < t.d) {
+                    t.d = d;
+                    t.best = off + j;
+                }
+            }
+            t.r_max = fmn(t.r_max, t.d);
+        } else {
+            RBvh2 rc = gload<RBvh2>(t.rec, t.ref);
+            float dminL, dmaxL, dminR, dmaxR;
+            box_dist(f2(rc.lmnx, rc.lmny), f2(rc.lmxx, rc.lmxy), p, dminL, dmaxL);
+            bool hitL = dminL <= t.r_max;
+            t.r_max = fmn(t.r_max, dmaxL);
+            box_dist(f2(rc.rmnx, rc.rmny), f2(rc.rmxx, rc.rmxy), p, dminR, dmaxR);
+            bool hitR = dminR <= t.r_max;
+            t.r_max = fmn(t.r_max, dmaxR);
+            if (hitL && hitR) {
+                bool sw = dminR < dminL;  // `d_max_right < d_max_right` (bvh.py:163) never swaps
+                stk_push(t.sp, o, sw ? rc.ref_l : rc.ref_r, sw ? dminL : dminR);
+                t.ref = sw ? rc.ref_r : rc.ref_l;
+                t.cd = sw ? dminR : dminL;
+                return false;
+            } else if (hitL) {
+                t.ref = rc.ref_l;
+                t.cd = dminL;
+                return false;
+            } else if (hitR) {
+                t.ref = rc.ref_r;
+                t.cd = dminR;
+                return false;
+            }
+        }
+    }
+    if (t.sp == 0) return true;
+    stk_pop(t.sp, o, t.ref, t.cd);
+    return false;
+}
+DWD CP closest2_end(const Trav& t) {
+    const SubDev& S = *t.S;
+    F2 p = f2(t.x.x, t.x.y);
     CP its;
     its.p = f3(p.x, p.y, 0.0f);
     its.n = f3(0.0f, 0.0f, 0.0f);
@@ -273,143 +377,137 @@ DWD CP closest2(const SubDev& S, F2 p) {
     its.u = its.v = 0.0f;
     its.d = kInf;
     its.prim = -1;
-    int best = -1;
-    float r_max = kInf;
-    int ref = S.root_ref_bvh;
-    float cd = kInf;
-    while (true) {
-        if (!(cd > r_max)) {
-            if (ref < 0) {
-                int off, cnt;
-                leaf_range(S, ref, off, cnt);
-                for (int j = 0; j < cnt; ++j) {
-                    DSeg sg = prims[off + j];
-                    F2 a = f2(sg.ax, sg.ay), b = f2(sg.bx, sg.by);
-                    F2 pa = p - a, ba = b - a;
-                    float h = clampf(dot(pa, ba) / dot(ba, ba), 0.0f, 1.0f);
-                    F2 i = a + ba * h;
-                    float d = norm(i - p);
-                    if (d < its.d) {
-                        its.d = d;
-                        its.t = h;
-                        its.p = f3(i.x, i.y, 0.0f);
-                        best = off + j;
-                    }
-                }
-                r_max = fmn(r_max, its.d);
-            } else {
-                RBvh2 rc = recs[ref];
-                float dminL, dmaxL, dminR, dmaxR;
-                box_dist(f2(rc.lmnx, rc.lmny), f2(rc.lmxx, rc.lmxy), p, dminL, dmaxL);
-                bool hitL = dminL <= r_max;
-                r_max = fmn(r_max, dmaxL);
-                box_dist(f2(rc.rmnx, rc.rmny), f2(rc.rmxx, rc.rmxy), p, dminR, dmaxR);
-                bool hitR = dminR <= r_max;
-                r_max = fmn(r_max, dmaxR);
-                if (hitL && hitR) {
-                    bool sw = dminR < dminL;  // `d_max_right < d_max_right` (bvh.py:163) never swaps
-                    st.push(sw ? rc.ref_l : rc.ref_r, sw ? dminL : dminR);
-                    ref = sw ? rc.ref_r : rc.ref_l;
-                    cd = sw ? dminR : dminL;
-                    continue;
-                } else if (hitL) {
-                    ref = rc.ref_l;
-                    cd = dminL;
-                    continue;
-                } else if (hitR) {
-                    ref = rc.ref_r;
-                    cd = dminR;
-                    continue;
-                }
-            }
-        }
-        if (st.sp == 0) break;
-        st.pop(ref, cd);
-    }
-    if (best >= 0) {  // normal of the winning segment (LineSegment.closest_point)
-        DSeg sg = prims[best];
-        F2 t = normalize(f2(sg.bx - sg.ax, sg.by - sg.ay));
-        its.n = f3(t.y, -t.x, 0.0f);
-        its.prim = S.prim_index[best];
+    if (t.best >= 0) {  // the winner's record and normal (LineSegment.closest_point)
+        DSeg sg = ((const DSeg*)S.prims)[t.best];
+        F2 a = f2(sg.ax, sg.ay), b = f2(sg.bx, sg.by);
+        F2 pa = p - a, ba = b - a;
+        float h = clampf(dot(pa, ba) / dot(ba, ba), 0.0f, 1.0f);
+        F2 i = a + ba * h;
+        its.d = norm(i - p);
+        its.t = h;
+        its.p = f3(i.x, i.y, 0.0f);
+        F2 tg = normalize(f2(sg.bx - sg.ax, sg.by - sg.ay));
+        its.n = f3(tg.y, -tg.x, 0.0f);
+        its.prim = S.prim_index[t.best];
     }
     return its;
+}
+DWD CP closest2(const SubDev& S, F2 p) {
+    Trav t;
+    Ovf o;
+    closest2_begin(t, S, p);
+    while (!closest2_iter(t, o)) {
+    }
+    return closest2_end(t);
 }
 
 // BVH.intersect (bvh.py:190-257): the first valid hit in traversal order wins
 // (`r_max = its.d` reads the zero-initialised record, :216), so the traversal
-// stops at the first hit.
-DWD Hit intersect2(const SubDev& S, F2 x, F2 v, float r_max) {
-    const RBvh2* recs = (const RBvh2*)S.rec_bvh;
-    const DSeg* prims = (const DSeg*)S.prims;
+// stops at the first hit.  LineSegment.ray_intersect (line_segment.py:47-72).
+DWD bool seg_hit(const DSeg& sg, F2 x, F2 v, float r_max, float& s_, float& t_) {
+    F2 a = f2(sg.ax, sg.ay), b = f2(sg.bx, sg.by);
+    F2 u = a - x, w = b - a;
+    float dv = v.x * w.y - v.y * w.x;
+    if (!(__builtin_fabsf(dv) > kDrEps)) return false;
+    float ud = u.x * v.y - u.y * v.x;
+    s_ = ud / dv;
+    if (!((s_ >= 0.0f) && (s_ <= 1.0f))) return false;
+    float uv = u.x * w.y - u.y * w.x;
+    t_ = uv / dv;
+    if (!((t_ >= 0.0f) && (t_ <= r_max))) return false;
+    return __builtin_fabsf(t_) < r_max;
+}
+DWD bool isect2_begin(Trav& t, const SubDev& S, F2 x, F2 v, float r_max) {
+    t.kind = TQ_ISECT;
+    t.S = &S;
+    t.rec = S.rec_bvh;
+    t.prims = S.prims;
+    t.x = f3(x.x, x.y, 0.0f);
+    t.v = f3(v.x, v.y, 0.0f);
+    t.r_max = r_max;
+    t.best = -1;
+    float tmin;
+    if (!box_ray(f2(S.root_mn[0], S.root_mn[1]), f2(S.root_mx[0], S.root_mx[1]), x, v, r_max, tmin)) return false;
+    t.sp = 0;
+    t.ref = S.root_ref_bvh;
+    t.cd = tmin;
+    return true;
+}
+DWD bool isect2_iter(Trav& t, Ovf& o) {
+    const SubDev& S = *t.S;
+    F2 x = f2(t.x.x, t.x.y), v = f2(t.v.x, t.v.y);
+    if (t.cd <= t.r_max) {
+        if (t.ref < 0) {
+            int off, cnt;
+            leaf_range(S, t.ref, off, cnt);
+            for (int j = 0; j < cnt; ++j) {
+                float s_, t_;
+                if (seg_hit(gload<DSeg>(t.prims, off + j), x, v, t.r_max, s_, t_)) {
+                    t.best = off + j;
+                    return true;
+                }
+            }
+        } else {
+            RBvh2 rc = gload<RBvh2>(t.rec, t.ref);
+            float t0, t1;
+            bool h0 = box_ray(f2(rc.lmnx, rc.lmny), f2(rc.lmxx, rc.lmxy), x, v, t.r_max, t0);
+            bool h1 = box_ray(f2(rc.rmnx, rc.rmny), f2(rc.rmxx, rc.rmxy), x, v, t.r_max, t1);
+            if (h0 && h1) {
+                bool sw = t1 < t0;
+                stk_push(t.sp, o, sw ? rc.ref_l : rc.ref_r, sw ? t0 : t1);
+                t.ref = sw ? rc.ref_r : rc.ref_l;
+                t.cd = sw ? t1 : t0;
+                return false;
+            } else if (h0) {
+                t.ref = rc.ref_l;
+                t.cd = t0;
+                return false;
+            } else if (h1) {
+                t.ref = rc.ref_r;
+                t.cd = t1;
+                return false;
+            }
+        }
+    }
+    if (t.sp == 0) return true;
+    stk_pop(t.sp, o, t.ref, t.cd);
+    return false;
+}
+DWD Hit isect2_end(const Trav& t) {
+    const SubDev& S = *t.S;
     Hit its;
     its.valid = false;
     its.on_boundary = false;
     its.p = its.n = f3(0.0f, 0.0f, 0.0f);
     its.t = its.u = its.v = its.d = 0.0f;
     its.prim = 0;
-    float tmin;
-    if (!box_ray(f2(S.root_mn[0], S.root_mn[1]), f2(S.root_mx[0], S.root_mx[1]), x, v, r_max, tmin)) return its;
-    Stk st;
-    stack_init(st);
-    int ref = S.root_ref_bvh;
-    float cd = tmin;
-    while (true) {
-        if (cd <= r_max) {
-            if (ref < 0) {
-                int off, cnt;
-                leaf_range(S, ref, off, cnt);
-                for (int j = 0; j < cnt; ++j) {  // LineSegment.ray_intersect (line_segment.py:47-72)
-                    DSeg sg = prims[off + j];
-                    F2 a = f2(sg.ax, sg.ay), b = f2(sg.bx, sg.by);
-                    F2 u = a - x, w = b - a;
-                    float dv = v.x * w.y - v.y * w.x;
-                    if (!(__builtin_fabsf(dv) > kDrEps)) continue;
-                    float ud = u.x * v.y - u.y * v.x;
-                    float s_ = ud / dv;
-                    if (!((s_ >= 0.0f) && (s_ <= 1.0f))) continue;
-                    float uv = u.x * w.y - u.y * w.x;
-                    float t = uv / dv;
-                    if (!((t >= 0.0f) && (t <= r_max))) continue;
-                    float d = __builtin_fabsf(t);
-                    if (!(d < r_max)) continue;
-                    F2 n = normalize(f2(w.y, -w.x));
-                    F2 hp = x + v * t;
-                    its.valid = true;
-                    its.on_boundary = true;
-                    its.p = f3(hp.x, hp.y, 0.0f);
-                    its.n = f3(n.x, n.y, 0.0f);
-                    its.t = s_;
-                    its.d = d;
-                    its.prim = S.prim_index[off + j];
-                    return its;
-                }
-            } else {
-                RBvh2 rc = recs[ref];
-                float t0, t1;
-                bool h0 = box_ray(f2(rc.lmnx, rc.lmny), f2(rc.lmxx, rc.lmxy), x, v, r_max, t0);
-                bool h1 = box_ray(f2(rc.rmnx, rc.rmny), f2(rc.rmxx, rc.rmxy), x, v, r_max, t1);
-                if (h0 && h1) {
-                    bool sw = t1 < t0;
-                    st.push(sw ? rc.ref_l : rc.ref_r, sw ? t0 : t1);
-                    ref = sw ? rc.ref_r : rc.ref_l;
-                    cd = sw ? t1 : t0;
-                    continue;
-                } else if (h0) {
-                    ref = rc.ref_l;
-                    cd = t0;
-                    continue;
-                } else if (h1) {
-                    ref = rc.ref_r;
-                    cd = t1;
-                    continue;
-                }
-            }
-        }
-        if (st.sp == 0) break;
-        st.pop(ref, cd);
+    if (t.best >= 0) {
+        DSeg sg = ((const DSeg*)S.prims)[t.best];
+        F2 x = f2(t.x.x, t.x.y), v = f2(t.v.x, t.v.y);
+        float s_ = 0.0f, t_ = 0.0f;
+        seg_hit(sg, x, v, t.r_max, s_, t_);
+        F2 w = f2(sg.bx - sg.ax, sg.by - sg.ay);
+        F2 n = normalize(f2(w.y, -w.x));
+        F2 hp = x + v * t_;
+        its.valid = true;
+        its.on_boundary = true;
+        its.p = f3(hp.x, hp.y, 0.0f);
+        its.n = f3(n.x, n.y, 0.0f);
+        its.t = s_;
+        its.d = __builtin_fabsf(t_);
+        its.prim = S.prim_index[t.best];
     }
     return its;
 }
+DWD Hit intersect2(const SubDev& S, F2 x, F2 v, float r_max) {
+    Trav t;
+    Ovf o;
+    if (isect2_begin(t, S, x, v, r_max))
+        while (!isect2_iter(t, o)) {
+        }
+    return isect2_end(t);
+}
+
 
 // project_to_plane (bvh.py:16-20, snch3d.py:30-45)
 DWD float project_to_plane(F2 n, F2 e) { return __builtin_fabsf(dot(e, vabs(f2(-n.y, n.x)))); }
@@ -532,159 +630,207 @@ DWD bool snch3_child(F3 mn, F3 mx, F3 axis, float ha, float sin_ha, float cos_ha
     }
 }
 
-// SNCH.closest_silhouette 2D (snch.py:312-378 + silhouette_vertex.py:80-93)
-template <bool X = true>
-DWD bool closest_sil2(const SubDev& S, F2 x, float r_max, float& d_out, int& ref_out, F2& p_out) {
-    const RSnch2* recs = (const RSnch2*)S.rec_snch;
-    const DSilV* sils = (const DSilV*)S.sils;
-    bool found = false;
-    d_out = 0.0f;
-    ref_out = -1;
+// SNCH.closest_silhouette 2D (snch.py:312-378 + silhouette_vertex.py:80-93) and
+// SNCH3D.closest_silhouette (snch3d.py:594-663 + silhouette_edge.py:83-97).
+// A silhouette is accepted only below the running r_max, which then drops to
+// its distance, so the nearest distance found is r_max at the end.
+template <int D>
+DWD bool sil_begin(Trav& t, const SubDev& S, typename VecT<D>::T x, float r_max);
+template <>
+DWD bool sil_begin<2>(Trav& t, const SubDev& S, F2 x, float r_max) {
+    t.kind = TQ_SIL;
+    t.S = &S;
+    t.rec = S.rec_snch;
+    t.prims = S.sils;
+    t.x = f3(x.x, x.y, 0.0f);
+    t.r_max = r_max;
+    t.best = -1;
     float dmin = box_dmin(f2(S.root_mn[0], S.root_mn[1]), f2(S.root_mx[0], S.root_mx[1]), x);
     if (!(dmin <= r_max)) return false;
-    Stk st;
-    stack_init(st);
-    int ref = S.root_ref_snch;
-    float cd = dmin;
-    while (true) {
-        if (cd <= r_max) {
-            if (ref < 0) {
-                int off, cnt;
-                leaf_range(S, ref, off, cnt);
-                for (int j = 0; j < cnt; ++j) {
-                    DSilV sv = sils[off + j];
-                    F2 a = f2(sv.ax, sv.ay), b = f2(sv.bx, sv.by), c = f2(sv.cx, sv.cy);
-                    float d = norm(x - b);
-                    if (sv.both && (d < r_max)) {
-                        if (cross2(b - a, x - a) * cross2(c - b, x - b) < 0.0f) {
-                            r_max = fmn(r_max, d);
-                            found = true;
-                            d_out = d;
-                            ref_out = off + j;
-                            p_out = b;
-                        }
-                    }
-                }
-            } else {
-                RSnch2 rc = recs[ref];
-                float d0, d1;
-                bool h0, h1;
-                if constexpr (X) {
-                    h0 = snch_overlaps(f2(rc.lmnx, rc.lmny), f2(rc.lmxx, rc.lmxy), f2(rc.lax, rc.lay), rc.lha, rc.lrad,
-                                       x, r_max, d0);
-                    h1 = snch_overlaps(f2(rc.rmnx, rc.rmny), f2(rc.rmxx, rc.rmxy), f2(rc.rax, rc.ray), rc.rha, rc.rrad,
-                                       x, r_max, d1);
-                } else {
-                    h0 = snch_overlaps_fast(f2(rc.lmnx, rc.lmny), f2(rc.lmxx, rc.lmxy), f2(rc.lax, rc.lay), rc.lha,
-                                            rc.lsin, rc.lcos, rc.lrad, x, r_max, d0);
-                    h1 = snch_overlaps_fast(f2(rc.rmnx, rc.rmny), f2(rc.rmxx, rc.rmxy), f2(rc.rax, rc.ray), rc.rha,
-                                            rc.rsin, rc.rcos, rc.rrad, x, r_max, d1);
-                }
-                if (h0 && h1) {
-                    bool sw = d1 < d0;
-                    st.push(sw ? rc.ref_l : rc.ref_r, sw ? d0 : d1);
-                    ref = sw ? rc.ref_r : rc.ref_l;
-                    cd = sw ? d1 : d0;
-                    continue;
-                } else if (h0) {
-                    ref = rc.ref_l;
-                    cd = d0;
-                    continue;
-                } else if (h1) {
-                    ref = rc.ref_r;
-                    cd = d1;
-                    continue;
-                }
-            }
-        }
-        if (st.sp == 0) break;
-        st.pop(ref, cd);
-    }
-    return found;
+    t.sp = 0;
+    t.ref = S.root_ref_snch;
+    t.cd = dmin;
+    return true;
 }
-
-// SNCH3D.closest_silhouette (snch3d.py:594-663 + silhouette_edge.py:83-97)
-template <bool X = true>
-DWD bool closest_sil3(const SubDev& S, F3 x, float r_max, float& d_out, int& ref_out, F3& p_out) {
-    const RSnch3* recs = (const RSnch3*)S.rec_snch;
-    const DSilE* sils = (const DSilE*)S.sils;
-    bool found = false;
-    d_out = 0.0f;
-    ref_out = -1;
+template <>
+DWD bool sil_begin<3>(Trav& t, const SubDev& S, F3 x, float r_max) {
+    t.kind = TQ_SIL;
+    t.S = &S;
+    t.rec = S.rec_snch;
+    t.prims = S.sils;
+    t.x = x;
+    t.r_max = r_max;
+    t.best = -1;
     float dmin = box_dmin(f3(S.root_mn[0], S.root_mn[1], S.root_mn[2]), f3(S.root_mx[0], S.root_mx[1], S.root_mx[2]), x);
     if (!(dmin <= r_max)) return false;
-    Stk st;
-    stack_init(st);
-    int ref = S.root_ref_snch;
-    float cd = dmin;
-    while (true) {
-        if (cd <= r_max) {
-            if (ref < 0) {
-                int off, cnt;
-                leaf_range(S, ref, off, cnt);
-                for (int j = 0; j < cnt; ++j) {
-                    DSilE se = sils[off + j];
-                    F3 a = f3(se.ax, se.ay, se.az), b = f3(se.bx, se.by, se.bz);
-                    F3 u = b - a, w = x - a;
-                    float t = clampf(dot(w, u) / dot(u, u), 0.0f, 1.0f);
-                    F3 pt = lerp(a, b, t);
-                    float d = norm(pt - x);
-                    if (d < r_max) {
-                        bool is = se.boundary != 0;
-                        if (!is) {
-                            F3 c = f3(se.cx, se.cy, se.cz), dd = f3(se.dx, se.dy, se.dz);
-                            F3 n0 = normalize(cross(b - a, c - a));
-                            F3 n1 = normalize(cross(dd - a, b - a));
-                            F3 view = pt - x;
-                            is = dot(view, n0) * dot(view, n1) < 0.0f;
-                        }
-                        if (is) {
-                            r_max = fmn(r_max, d);
-                            found = true;
-                            d_out = d;
-                            ref_out = off + j;
-                            p_out = pt;
-                        }
+    t.sp = 0;
+    t.ref = S.root_ref_snch;
+    t.cd = dmin;
+    return true;
+}
+template <bool X>
+DWD bool sil2_iter(Trav& t, Ovf& o) {
+    const SubDev& S = *t.S;
+    F2 x = f2(t.x.x, t.x.y);
+    if (t.cd <= t.r_max) {
+        if (t.ref < 0) {
+            int off, cnt;
+            leaf_range(S, t.ref, off, cnt);
+            for (int j = 0; j < cnt; ++j) {
+                DSilV sv = gload<DSilV>(t.prims, off + j);
+                F2 a = f2(sv.ax, sv.ay), b = f2(sv.bx, sv.by), c = f2(sv.cx, sv.cy);
+                float d = norm(x - b);
+                if (sv.both && (d < t.r_max)) {
+                    if (cross2(b - a, x - a) * cross2(c - b, x - b) < 0.0f) {
+                        t.r_max = fmn(t.r_max, d);
+                        t.best = off + j;
                     }
                 }
+            }
+        } else {
+            RSnch2 rc = gload<RSnch2>(t.rec, t.ref);
+            float d0, d1;
+            bool h0, h1;
+            if constexpr (X) {
+                h0 = snch_overlaps(f2(rc.lmnx, rc.lmny), f2(rc.lmxx, rc.lmxy), f2(rc.lax, rc.lay), rc.lha, rc.lrad, x,
+                                   t.r_max, d0);
+                h1 = snch_overlaps(f2(rc.rmnx, rc.rmny), f2(rc.rmxx, rc.rmxy), f2(rc.rax, rc.ray), rc.rha, rc.rrad, x,
+                                   t.r_max, d1);
             } else {
-                RSnch3 rc = recs[ref];
-                F3 lmn = f3(rc.lmnx, rc.lmny, rc.lmnz), lmx = f3(rc.lmxx, rc.lmxy, rc.lmxz);
-                F3 rmn = f3(rc.rmnx, rc.rmny, rc.rmnz), rmx = f3(rc.rmxx, rc.rmxy, rc.rmxz);
-                float d0, d1;
-                bool h0 = snch3_child<X>(lmn, lmx, f3(rc.lax, rc.lay, rc.laz), rc.lha, rc.lsin, rc.lcos, rc.lrad,
-                                         rc.lrho, rc.lelo, rc.lehi, x, r_max, d0);
-                bool h1 = snch3_child<X>(rmn, rmx, f3(rc.rax, rc.ray, rc.raz), rc.rha, rc.rsin, rc.rcos, rc.rrad,
-                                         rc.rrho, rc.relo, rc.rehi, x, r_max, d1);
-                if (h0 && h1) {
-                    bool sw = d1 < d0;
-                    st.push(sw ? rc.ref_l : rc.ref_r, sw ? d0 : d1);
-                    ref = sw ? rc.ref_r : rc.ref_l;
-                    cd = sw ? d1 : d0;
-                    continue;
-                } else if (h0) {
-                    ref = rc.ref_l;
-                    cd = d0;
-                    continue;
-                } else if (h1) {
-                    ref = rc.ref_r;
-                    cd = d1;
-                    continue;
-                }
+                h0 = snch_overlaps_fast(f2(rc.lmnx, rc.lmny), f2(rc.lmxx, rc.lmxy), f2(rc.lax, rc.lay), rc.lha, rc.lsin,
+                                        rc.lcos, rc.lrad, x, t.r_max, d0);
+                h1 = snch_overlaps_fast(f2(rc.rmnx, rc.rmny), f2(rc.rmxx, rc.rmxy), f2(rc.rax, rc.ray), rc.rha, rc.rsin,
+                                        rc.rcos, rc.rrad, x, t.r_max, d1);
+            }
+            if (h0 && h1) {
+                bool sw = d1 < d0;
+                stk_push(t.sp, o, sw ? rc.ref_l : rc.ref_r, sw ? d0 : d1);
+                t.ref = sw ? rc.ref_r : rc.ref_l;
+                t.cd = sw ? d1 : d0;
+                return false;
+            } else if (h0) {
+                t.ref = rc.ref_l;
+                t.cd = d0;
+                return false;
+            } else if (h1) {
+                t.ref = rc.ref_r;
+                t.cd = d1;
+                return false;
             }
         }
-        if (st.sp == 0) break;
-        st.pop(ref, cd);
     }
-    return found;
+    if (t.sp == 0) return true;
+    stk_pop(t.sp, o, t.ref, t.cd);
+    return false;
+}
+// silhouette_edge.py:83-97: distance to the edge, then the two-face test
+DWD bool sile_test(const DSilE& se, F3 x, float r_max, float& d, F3& pt) {
+    F3 a = f3(se.ax, se.ay, se.az), b = f3(se.bx, se.by, se.bz);
+    F3 u = b - a, w = x - a;
+    float tt = clampf(dot(w, u) / dot(u, u), 0.0f, 1.0f);
+    pt = lerp(a, b, tt);
+    d = norm(pt - x);
+    if (!(d < r_max)) return false;
+    if (se.boundary != 0) return true;
+    F3 c = f3(se.cx, se.cy, se.cz), dd = f3(se.dx, se.dy, se.dz);
+    F3 n0 = normalize(cross(b - a, c - a));
+    F3 n1 = normalize(cross(dd - a, b - a));
+    F3 view = pt - x;
+    return dot(view, n0) * dot(view, n1) < 0.0f;
+}
+template <bool X>
+DWD bool sil3_iter(Trav& t, Ovf& o) {
+    const SubDev& S = *t.S;
+    F3 x = t.x;
+    if (t.cd <= t.r_max) {
+        if (t.ref < 0) {
+            int off, cnt;
+            leaf_range(S, t.ref, off, cnt);
+            for (int j = 0; j < cnt; ++j) {
+                float d;
+                F3 pt;
+                if (sile_test(gload<DSilE>(t.prims, off + j), x, t.r_max, d, pt)) {
+                    t.r_max = fmn(t.r_max, d);
+                    t.best = off + j;
+                }
+            }
+        } else {
+            RSnch3 rc = gload<RSnch3>(t.rec, t.ref);
+            F3 lmn = f3(rc.lmnx, rc.lmny, rc.lmnz), lmx = f3(rc.lmxx, rc.lmxy, rc.lmxz);
+            F3 rmn = f3(rc.rmnx, rc.rmny, rc.rmnz), rmx = f3(rc.rmxx, rc.rmxy, rc.rmxz);
+            float d0, d1;
+            bool h0 = snch3_child<X>(lmn, lmx, f3(rc.lax, rc.lay, rc.laz), rc.lha, rc.lsin, rc.lcos, rc.lrad, rc.lrho,
+                                     rc.lelo, rc.lehi, x, t.r_max, d0);
+            bool h1 = snch3_child<X>(rmn, rmx, f3(rc.rax, rc.ray, rc.raz), rc.rha, rc.rsin, rc.rcos, rc.rrad, rc.rrho,
+                                     rc.relo, rc.rehi, x, t.r_max, d1);
+            if (h0 && h1) {
+                bool sw = d1 < d0;
+                stk_push(t.sp, o, sw ? rc.ref_l : rc.ref_r, sw ? d0 : d1);
+                t.ref = sw ? rc.ref_r : rc.ref_l;
+                t.cd = sw ? d1 : d0;
+                return false;
+            } else if (h0) {
+                t.ref = rc.ref_l;
+                t.cd = d0;
+                return false;
+            } else if (h1) {
+                t.ref = rc.ref_r;
+                t.cd = d1;
+                return false;
+            }
+        }
+    }
+    if (t.sp == 0) return true;
+    stk_pop(t.sp, o, t.ref, t.cd);
+    return false;
+}
+template <bool X = true>
+DWD bool closest_sil2(const SubDev& S, F2 x, float r_max, float& d_out, int& ref_out, F2& p_out) {
+    Trav t;
+    Ovf o;
+    if (sil_begin<2>(t, S, x, r_max))
+        while (!sil2_iter<X>(t, o)) {
+        }
+    d_out = 0.0f;
+    ref_out = t.best;
+    if (t.best < 0) return false;
+    const DSilV& sv = ((const DSilV*)S.sils)[t.best];
+    d_out = t.r_max;
+    p_out = f2(sv.bx, sv.by);
+    return true;
+}
+template <bool X = true>
+DWD bool closest_sil3(const SubDev& S, F3 x, float r_max, float& d_out, int& ref_out, F3& p_out) {
+    Trav t;
+    Ovf o;
+    if (sil_begin<3>(t, S, x, r_max))
+        while (!sil3_iter<X>(t, o)) {
+        }
+    d_out = 0.0f;
+    ref_out = t.best;
+    if (t.best < 0) return false;
+    float d;
+    sile_test(((const DSilE*)S.sils)[t.best], x, kInf, d, p_out);
+    d_out = t.r_max;
+    return true;
 }
 
-// Polyline.star_radius (polyline.py:304-314) / Mesh.star_radius (mesh.py:337-346)
-template <bool X = true>
-DWD float star_radius2(const SubDev& S, F2 x, float r_max) {
+// Polyline.star_radius (polyline.py:304-314) / Mesh.star_radius (mesh.py:337-346):
+// the silhouette search runs inside min(|x - farthest box corner|, r_max).
+DWD float star_bound(const SubDev& S, F2 x, float r_max) {
     F2 m = f2(fmx(__builtin_fabsf(x.x - S.bb_min[0]), __builtin_fabsf(x.x - S.bb_max[0])),
               fmx(__builtin_fabsf(x.y - S.bb_min[1]), __builtin_fabsf(x.y - S.bb_max[1])));
-    float R = fmn(norm(m), r_max);
+    return fmn(norm(m), r_max);
+}
+DWD float star_bound(const SubDev& S, F3 x, float r_max) {
+    F3 m = f3(fmx(__builtin_fabsf(x.x - S.bb_min[0]), __builtin_fabsf(x.x - S.bb_max[0])),
+              fmx(__builtin_fabsf(x.y - S.bb_min[1]), __builtin_fabsf(x.y - S.bb_max[1])),
+              fmx(__builtin_fabsf(x.z - S.bb_min[2]), __builtin_fabsf(x.z - S.bb_max[2])));
+    return fmn(norm(m), r_max);
+}
+template <bool X = true>
+DWD float star_radius2(const SubDev& S, F2 x, float r_max) {
+    float R = star_bound(S, x, r_max);
     if (!S.py_sils) return R;
     float d;
     int ref;
@@ -693,16 +839,14 @@ DWD float star_radius2(const SubDev& S, F2 x, float r_max) {
 }
 template <bool X = true>
 DWD float star_radius3(const SubDev& S, F3 x, float r_max) {
-    F3 m = f3(fmx(__builtin_fabsf(x.x - S.bb_min[0]), __builtin_fabsf(x.x - S.bb_max[0])),
-              fmx(__builtin_fabsf(x.y - S.bb_min[1]), __builtin_fabsf(x.y - S.bb_max[1])),
-              fmx(__builtin_fabsf(x.z - S.bb_min[2]), __builtin_fabsf(x.z - S.bb_max[2])));
-    float R = fmn(norm(m), r_max);
+    float R = star_bound(S, x, r_max);
     if (!S.py_sils) return R;
     float d;
     int ref;
     F3 p;
     return closest_sil3<X>(S, x, R, d, ref, p) ? d : R;
 }
+
 
 // ---------------------------------------------------------------------------
 // 3D: closest point on triangles (math.py:257-322) and SNCH3D.closest_point
@@ -751,162 +895,189 @@ DWD void tri_closest(const DTri& tr, F3 p, F3& pt, float& u_out, float& v_out, f
     v_out = 1.0f - u0 - u1;
 }
 
-DWD CP closest3(const SubDev& S, F3 p) {
-    const RBvh3* recs = (const RBvh3*)S.rec_bvh;
-    const DTri* prims = (const DTri*)S.prims;
-    Stk st;
-    stack_init(st);
+DWD void closest3_begin(Trav& t, const SubDev& S, F3 p) {
+    t.kind = TQ_CLOSEST;
+    t.S = &S;
+    t.rec = S.rec_bvh;
+    t.prims = S.prims;
+    t.x = p;
+    t.d = kInf;
+    t.best = -1;
+    t.r_max = kInf;
+    t.ref = S.root_ref_bvh;
+    t.cd = kInf;
+    t.sp = 0;
+}
+DWD bool closest3_iter(Trav& t, Ovf& o) {
+    const SubDev& S = *t.S;
+    F3 p = t.x;
+    if (t.cd <= t.r_max) {
+        if (t.ref < 0) {
+            int off, cnt;
+            leaf_range(S, t.ref, off, cnt);
+            for (int j = 0; j < cnt; ++j) {
+                F3 pt;
+                float u, v, d;
+                tri_closest(gload<DTri>(t.prims, off + j), p, pt, u, v, d);
+                if (d < t.d) {
+                    t.d = d;
+                    t.best = off + j;
+                    t.r_max = fmn(t.r_max, d);
+                }
+            }
+        } else {
+            RBvh3 rc = gload<RBvh3>(t.rec, t.ref);
+            float dminL, dmaxL, dminR, dmaxR;
+            box_dist(f3(rc.lmnx, rc.lmny, rc.lmnz), f3(rc.lmxx, rc.lmxy, rc.lmxz), p, dminL, dmaxL);
+            bool hitL = dminL <= t.r_max;
+            t.r_max = fmn(t.r_max, dmaxL);
+            box_dist(f3(rc.rmnx, rc.rmny, rc.rmnz), f3(rc.rmxx, rc.rmxy, rc.rmxz), p, dminR, dmaxR);
+            bool hitR = dminR <= t.r_max;
+            t.r_max = fmn(t.r_max, dmaxR);
+            if (hitL && hitR) {
+                // snch3d.py:383-388: equal zero d_min -> nearer d_max first; then d_min order
+                int closer = rc.ref_l, other = rc.ref_r;
+                if ((dminL == 0.0f) && (dminR == 0.0f) && (dmaxR < dmaxL)) closer = rc.ref_r, other = rc.ref_l;
+                float dc = dminL, dO = dminR;
+                if (dminR < dminL) {
+                    int tmp = closer;
+                    closer = other;
+                    other = tmp;
+                    dc = dminR;
+                    dO = dminL;
+                }
+                stk_push(t.sp, o, other, dO);
+                t.ref = closer;
+                t.cd = dc;
+                return false;
+            } else if (hitL) {
+                t.ref = rc.ref_l;
+                t.cd = dminL;
+                return false;
+            } else if (hitR) {
+                t.ref = rc.ref_r;
+                t.cd = dminR;
+                return false;
+            }
+        }
+    }
+    if (t.sp == 0) return true;
+    stk_pop(t.sp, o, t.ref, t.cd);
+    return false;
+}
+DWD CP closest3_end(const Trav& t) {
     CP its;
     its.p = its.n = f3(0.0f, 0.0f, 0.0f);
     its.t = its.u = its.v = 0.0f;
     its.d = kInf;
     its.prim = -1;
-    int best = -1;
-    float r_max = kInf;
-    int ref = S.root_ref_bvh;
-    float cd = kInf;
-    while (true) {
-        if (cd <= r_max) {
-            if (ref < 0) {
-                int off, cnt;
-                leaf_range(S, ref, off, cnt);
-                for (int j = 0; j < cnt; ++j) {
-                    DTri tr = prims[off + j];
-                    F3 pt;
-                    float u, v, d;
-                    tri_closest(tr, p, pt, u, v, d);
-                    if (d < its.d) {
-                        its.d = d;
-                        its.p = pt;
-                        its.u = u;
-                        its.v = v;
-                        best = off + j;
-                        r_max = fmn(r_max, d);
-                    }
-                }
-            } else {
-                RBvh3 rc = recs[ref];
-                float dminL, dmaxL, dminR, dmaxR;
-                box_dist(f3(rc.lmnx, rc.lmny, rc.lmnz), f3(rc.lmxx, rc.lmxy, rc.lmxz), p, dminL, dmaxL);
-                bool hitL = dminL <= r_max;
-                r_max = fmn(r_max, dmaxL);
-                box_dist(f3(rc.rmnx, rc.rmny, rc.rmnz), f3(rc.rmxx, rc.rmxy, rc.rmxz), p, dminR, dmaxR);
-                bool hitR = dminR <= r_max;
-                r_max = fmn(r_max, dmaxR);
-                if (hitL && hitR) {
-                    // snch3d.py:383-388: equal zero d_min -> nearer d_max first; then d_min order
-                    int closer = rc.ref_l, other = rc.ref_r;
-                    if ((dminL == 0.0f) && (dminR == 0.0f) && (dmaxR < dmaxL)) closer = rc.ref_r, other = rc.ref_l;
-                    float dc = dminL, dO = dminR;
-                    if (dminR < dminL) {
-                        int t = closer;
-                        closer = other;
-                        other = t;
-                        dc = dminR;
-                        dO = dminL;
-                    }
-                    st.push(other, dO);
-                    ref = closer;
-                    cd = dc;
-                    continue;
-                } else if (hitL) {
-                    ref = rc.ref_l;
-                    cd = dminL;
-                    continue;
-                } else if (hitR) {
-                    ref = rc.ref_r;
-                    cd = dminR;
-                    continue;
-                }
-            }
-        }
-        if (st.sp == 0) break;
-        st.pop(ref, cd);
-    }
-    if (best >= 0) {
-        DTri tr = prims[best];
+    if (t.best >= 0) {
+        DTri tr = ((const DTri*)t.S->prims)[t.best];
+        tri_closest(tr, t.x, its.p, its.u, its.v, its.d);
         F3 a = f3(tr.ax, tr.ay, tr.az), b = f3(tr.bx, tr.by, tr.bz), c = f3(tr.cx, tr.cy, tr.cz);
         its.n = normalize(cross(b - a, c - a));
         its.prim = tr.index;
     }
     return its;
 }
+DWD CP closest3(const SubDev& S, F3 p) {
+    Trav t;
+    Ovf o;
+    closest3_begin(t, S, p);
+    while (!closest3_iter(t, o)) {
+    }
+    return closest3_end(t);
+}
 
 // SNCH3D.intersect (snch3d.py:411-476) + Triangle.ray_intersect (triangle.py:59-86);
-// prim = BVH-sorted index (triangle.py:82)
-DWD Hit intersect3(const SubDev& S, F3 x, F3 v, float r_max) {
-    const RBvh3* recs = (const RBvh3*)S.rec_bvh;
-    const DTri* prims = (const DTri*)S.prims;
+// the nearest hit wins (r_max drops to each accepted t); prim = BVH-sorted index
+// (triangle.py:82)
+DWD bool tri_hit(const DTri& tr, F3 x, F3 v, float r_max, float& bu, float& bw, float& t_) {
+    F3 a = f3(tr.ax, tr.ay, tr.az), b = f3(tr.bx, tr.by, tr.bz), c = f3(tr.cx, tr.cy, tr.cz);
+    F3 v1 = b - a, v2 = c - a;
+    F3 pp = cross(v, v2);
+    float det = dot(v1, pp);
+    if (!(__builtin_fabsf(det) > kDrEps)) return false;
+    float inv = 1.0f / det;
+    F3 s = x - a;
+    bu = dot(s, pp) * inv;
+    if (!((bu >= 0.0f) && (bu <= 1.0f))) return false;
+    F3 q = cross(s, v1);
+    bw = dot(v, q) * inv;
+    if (!((bw >= 0.0f) && (bu + bw <= 1.0f))) return false;
+    t_ = dot(v2, q) * inv;
+    if (!((t_ >= 0.0f) && (t_ <= r_max))) return false;
+    return t_ < r_max;
+}
+DWD bool isect3_begin(Trav& t, const SubDev& S, F3 x, F3 v, float r_max) {
+    t.kind = TQ_ISECT;
+    t.S = &S;
+    t.rec = S.rec_bvh;
+    t.prims = S.prims;
+    t.x = x;
+    t.v = v;
+    t.r_max = r_max;
+    t.best = -1;
+    float tmin;
+    if (!box_ray(f3(S.root_mn[0], S.root_mn[1], S.root_mn[2]), f3(S.root_mx[0], S.root_mx[1], S.root_mx[2]), x, v,
+                 r_max, tmin))
+        return false;
+    t.sp = 0;
+    t.ref = S.root_ref_bvh;
+    t.cd = tmin;
+    return true;
+}
+DWD bool isect3_iter(Trav& t, Ovf& o) {
+    const SubDev& S = *t.S;
+    F3 x = t.x, v = t.v;
+    if (t.cd <= t.r_max) {
+        if (t.ref < 0) {
+            int off, cnt;
+            leaf_range(S, t.ref, off, cnt);
+            for (int j = 0; j < cnt; ++j) {
+                float bu, bw, t_;
+                if (tri_hit(gload<DTri>(t.prims, off + j), x, v, t.r_max, bu, bw, t_)) {
+                    t.r_max = t_;
+                    t.best = off + j;
+                }
+            }
+        } else {
+            RBvh3 rc = gload<RBvh3>(t.rec, t.ref);
+            float t0, t1;
+            bool h0 = box_ray(f3(rc.lmnx, rc.lmny, rc.lmnz), f3(rc.lmxx, rc.lmxy, rc.lmxz), x, v, t.r_max, t0);
+            bool h1 = box_ray(f3(rc.rmnx, rc.rmny, rc.rmnz), f3(rc.rmxx, rc.rmxy, rc.rmxz), x, v, t.r_max, t1);
+            if (h0 && h1) {
+                bool sw = t1 < t0;
+                stk_push(t.sp, o, sw ? rc.ref_l : rc.ref_r, sw ? t0 : t1);
+                t.ref = sw ? rc.ref_r : rc.ref_l;
+                t.cd = sw ? t1 : t0;
+                return false;
+            } else if (h0) {
+                t.ref = rc.ref_l;
+                t.cd = t0;
+                return false;
+            } else if (h1) {
+                t.ref = rc.ref_r;
+                t.cd = t1;
+                return false;
+            }
+        }
+    }
+    if (t.sp == 0) return true;
+    stk_pop(t.sp, o, t.ref, t.cd);
+    return false;
+}
+DWD Hit isect3_end(const Trav& t) {
     Hit its;
     its.valid = false;
     its.on_boundary = false;
     its.p = its.n = f3(0.0f, 0.0f, 0.0f);
     its.t = its.u = its.v = its.d = 0.0f;
     its.prim = 0;
-    int best = -1;
-    float bv = 0.0f, bw = 0.0f;
-    float tmin;
-    if (!box_ray(f3(S.root_mn[0], S.root_mn[1], S.root_mn[2]), f3(S.root_mx[0], S.root_mx[1], S.root_mx[2]), x, v,
-                 r_max, tmin))
-        return its;
-    Stk st;
-    stack_init(st);
-    int ref = S.root_ref_bvh;
-    float cd = tmin;
-    while (true) {
-        if (cd <= r_max) {
-            if (ref < 0) {
-                int off, cnt;
-                leaf_range(S, ref, off, cnt);
-                for (int j = 0; j < cnt; ++j) {
-                    DTri tr = prims[off + j];
-                    F3 a = f3(tr.ax, tr.ay, tr.az), b = f3(tr.bx, tr.by, tr.bz), c = f3(tr.cx, tr.cy, tr.cz);
-                    F3 v1 = b - a, v2 = c - a;
-                    F3 pp = cross(v, v2);
-                    float det = dot(v1, pp);
-                    if (!(__builtin_fabsf(det) > kDrEps)) continue;
-                    float inv = 1.0f / det;
-                    F3 s = x - a;
-                    float bu = dot(s, pp) * inv;
-                    if (!((bu >= 0.0f) && (bu <= 1.0f))) continue;
-                    F3 q = cross(s, v1);
-                    float bw_ = dot(v, q) * inv;
-                    if (!((bw_ >= 0.0f) && (bu + bw_ <= 1.0f))) continue;
-                    float t = dot(v2, q) * inv;
-                    if (!((t >= 0.0f) && (t <= r_max))) continue;
-                    if (!(t < r_max)) continue;
-                    r_max = t;
-                    best = off + j;
-                    bv = bu;
-                    bw = bw_;
-                }
-            } else {
-                RBvh3 rc = recs[ref];
-                float t0, t1;
-                bool h0 = box_ray(f3(rc.lmnx, rc.lmny, rc.lmnz), f3(rc.lmxx, rc.lmxy, rc.lmxz), x, v, r_max, t0);
-                bool h1 = box_ray(f3(rc.rmnx, rc.rmny, rc.rmnz), f3(rc.rmxx, rc.rmxy, rc.rmxz), x, v, r_max, t1);
-                if (h0 && h1) {
-                    bool sw = t1 < t0;
-                    st.push(sw ? rc.ref_l : rc.ref_r, sw ? t0 : t1);
-                    ref = sw ? rc.ref_r : rc.ref_l;
-                    cd = sw ? t1 : t0;
-                    continue;
-                } else if (h0) {
-                    ref = rc.ref_l;
-                    cd = t0;
-                    continue;
-                } else if (h1) {
-                    ref = rc.ref_r;
-                    cd = t1;
-                    continue;
-                }
-            }
-        }
-        if (st.sp == 0) break;
-        st.pop(ref, cd);
-    }
-    if (best >= 0) {
-        DTri tr = prims[best];
+    if (t.best >= 0) {
+        DTri tr = ((const DTri*)t.S->prims)[t.best];
+        float bv = 0.0f, bw = 0.0f, t_;
+        tri_hit(tr, t.x, t.v, kInf, bv, bw, t_);
         F3 a = f3(tr.ax, tr.ay, tr.az), b = f3(tr.bx, tr.by, tr.bz), c = f3(tr.cx, tr.cy, tr.cz);
         F3 v1 = b - a, v2 = c - a;
         its.valid = true;
@@ -915,10 +1086,36 @@ DWD Hit intersect3(const SubDev& S, F3 x, F3 v, float r_max) {
         its.n = normalize(cross(v1, v2));
         its.u = 1.0f - bv - bw;
         its.v = bv;
-        its.d = r_max;
-        its.prim = best;
+        its.d = t.r_max;
+        its.prim = t.best;
     }
     return its;
 }
+DWD Hit intersect3(const SubDev& S, F3 x, F3 v, float r_max) {
+    Trav t;
+    Ovf o;
+    if (isect3_begin(t, S, x, v, r_max))
+        while (!isect3_iter(t, o)) {
+        }
+    return isect3_end(t);
+}
 
+// One traversal iteration of whichever query t holds (the walk kernels' three
+// query kinds); true when it has finished.
+template <int D, bool X>
+DWD bool trav_iter_as(Trav& t, Ovf& o, int kind) {
+    if constexpr (D == 2) {
+        if (kind == TQ_CLOSEST) return closest2_iter(t, o);
+        if (kind == TQ_SIL) return sil2_iter<X>(t, o);
+        return isect2_iter(t, o);
+    } else {
+        if (kind == TQ_CLOSEST) return closest3_iter(t, o);
+        if (kind == TQ_SIL) return sil3_iter<X>(t, o);
+        return isect3_iter(t, o);
+    }
+}
+template <int D, bool X>
+DWD bool trav_iter(Trav& t, Ovf& o) {
+    return trav_iter_as<D, X>(t, o, t.kind);
+}
 }  // namespace dwd

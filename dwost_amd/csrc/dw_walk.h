@@ -179,21 +179,29 @@ DWD bool inside3(const SceneDev& sc, F3 x) {
 }
 
 // --------------------------------------------------------------------------
-// WoSt.u (wost.py:145-221) / WoSt3D.u (wost_3d.py:43-106) as a step machine:
-// walk_init, then walk_step until it returns false, then walk_value.
-// Two exact shortcuts against the literal loop: the terminating step reuses the
-// Dirichlet closest-point record (the reference queries the same point twice,
-// wost.py:171,183) and skips the star radius, which has no side effects there.
+// WoSt.u (wost.py:145-221) / WoSt3D.u (wost_3d.py:43-106) and WoS.u (wos.py:16-36)
+// / WoS3D.u (wos_3d.py:15-34) as a step machine.  walk_init, then walk_advance
+// repeatedly: it runs the walk's logic from its current stage until it has
+// started the next boundary query in `t` (ADV_QUERY; the caller runs the
+// traversal to its end and calls walk_advance again), finished a step
+// (ADV_STEP) or finished the walk (ADV_END); then walk_value.
+// Stages of one WoSt step: Dirichlet closest point -> star radius (silhouette
+// query) -> [2D: Neumann sample + control-variate closest point] -> ray
+// intersection -> source term.  Two exact shortcuts against the literal loop:
+// the terminating step reuses the Dirichlet closest-point record (the
+// reference queries the same point twice, wost.py:171,183) and skips the star
+// radius, which has no side effects there.
 // --------------------------------------------------------------------------
+enum : int { WS_START = 0, WS_WOS, WS_DIR, WS_STAR, WS_SIL, WS_CV, WS_MOVE, WS_ISECT };
+enum : int { ADV_QUERY = 0, ADV_STEP = 1, ADV_END = 2 };
+
 template <int D>
-struct VecT;
+struct WalkAux {};
 template <>
-struct VecT<2> {
-    using T = F2;
-};
-template <>
-struct VecT<3> {
-    using T = F3;
+struct WalkAux<2> {  // Neumann sample held across the control-variate query
+    F2 ref, bp, bn, cv_n;
+    float rnd, bpdf, cv_t;
+    int idx, cv_prim;
 };
 
 template <int D>
@@ -202,7 +210,10 @@ struct Walk {
     float result;
     int i;
     int term;
+    int stage;
     bool ob, live;
+    float R, d_dir;  // ball radius and Dirichlet distance of the current step
+    WalkAux<D> aux;
 };
 
 DWD void set_zero(F2& v) { v = f2(0.0f, 0.0f); }
@@ -217,241 +228,300 @@ DWD void walk_init(Walk<D>& w, typename VecT<D>::T x) {
     w.i = 0;
     w.term = -1;
     w.live = true;
+    w.stage = WS_START;
 }
 
-// WoS.u (wos.py:16-36) / WoS3D.u (wos_3d.py:15-34): every boundary primitive
-// is Dirichlet (the full scene); one step is a closest-point query and a jump
-// to a uniform point on the largest empty ball.
-template <bool A>
-DWD bool wos_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk<2>& w, Rng& rng, const Adj& adj) {
-    const SubDev& F = sc.sub[0];
+// step finished: count it, end the walk at nsteps
+template <int D>
+DWD int step_done(Walk<D>& w, const ParamsDev& P) {
+    w.stage = WS_START;
     if (!(w.i < P.nsteps)) {
         w.live = false;
-        return false;
+        return ADV_END;
     }
-    CP c = closest2(F, w.ip);
-    float d = __builtin_fabsf(c.d);
-    w.i += 1;
-    if (d < P.eps) {
-        w.result += g2(bc, F, c);
-        if constexpr (A) adj_lin2(adj.dg, F, c.prim, c.t, adj.w);
-        w.term = c.prim;
-        w.live = false;
-        return false;
-    }
-    F2 dir = uniform_dir2(rng);
-    w.ip = w.ip + dir * d;
-    if (!(w.i < P.nsteps)) w.live = false;
-    return w.live;
-}
-template <bool A>
-DWD bool wos_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk<3>& w, Rng& rng, const Adj& adj) {
-    const SubDev& F = sc.sub[0];
-    if (!(w.i < P.nsteps)) {
-        w.live = false;
-        return false;
-    }
-    CP c = closest3(F, w.ip);
-    float d = __builtin_fabsf(c.d);
-    w.i += 1;
-    if (d < P.eps) {
-        w.result += g3(bc, F, c);
-        if constexpr (A) adj_lin3(adj.dg, F, c.prim, c.u, c.v, adj.w);
-        w.term = c.prim;
-        w.live = false;
-        return false;
-    }
-    F3 dir = uniform_sphere(rng);
-    w.ip = w.ip + dir * d;
-    if (!(w.i < P.nsteps)) w.live = false;
-    return w.live;
+    return ADV_STEP;
 }
 
-template <bool X, bool A = false>
-DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk<2>& w, Rng& rng,
-                   const Adj& adj = Adj{0.0f, nullptr, nullptr}) {
-    if (P.flags & DW_SOLVE_WOS) return wos_step<A>(sc, bc, P, w, rng, adj);
+template <bool X, bool A>
+DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk<2>& w, Rng& rng, Trav& t,
+                     const Adj& adj) {
     const SubDev& F = sc.sub[0];
     const SubDev& Dn = sc.sub[1];
     const SubDev& Nm = sc.sub[2];
-    if (!(w.i < P.nsteps)) {
-        w.live = false;
-        return false;
-    }
-    float R = kInf, d_dir = kInf;
-    CP cpd;
-    if (Dn.present) {
-        cpd = closest2(Dn, w.ip);
-        R = d_dir = __builtin_fabsf(cpd.d);
-    }
-    if (d_dir < P.eps) {
-        w.result += g2(bc, Dn, cpd);
-        if constexpr (A) adj_lin2(adj.dg, Dn, cpd.prim, cpd.t, adj.w);
-        w.term = cpd.prim;
+    if (w.stage == WS_START) {
+        if (!(w.i < P.nsteps)) {
+            w.live = false;
+            return ADV_END;
+        }
+        if (P.flags & DW_SOLVE_WOS) {
+            closest2_begin(t, F, w.ip);
+            w.stage = WS_WOS;
+            return ADV_QUERY;
+        }
+        if (Dn.present) {
+            closest2_begin(t, Dn, w.ip);
+            w.stage = WS_DIR;
+            return ADV_QUERY;
+        }
+        w.R = w.d_dir = kInf;
+        w.stage = WS_STAR;
+    } else if (w.stage == WS_WOS) {  // WoS: terminate in the eps shell or jump to the ball
+        CP c = closest2_end(t);
+        float d = __builtin_fabsf(c.d);
         w.i += 1;
-        w.live = false;
-        return false;
+        if (d < P.eps) {
+            w.result += g2(bc, F, c);
+            if constexpr (A) adj_lin2(adj.dg, F, c.prim, c.t, adj.w);
+            w.term = c.prim;
+            w.live = false;
+            return ADV_END;
+        }
+        F2 dir = uniform_dir2(rng);
+        w.ip = w.ip + dir * d;
+        return step_done(w, P);
+    } else if (w.stage == WS_DIR) {
+        CP cpd = closest2_end(t);
+        w.R = w.d_dir = __builtin_fabsf(cpd.d);
+        if (w.d_dir < P.eps) {
+            w.result += g2(bc, Dn, cpd);
+            if constexpr (A) adj_lin2(adj.dg, Dn, cpd.prim, cpd.t, adj.w);
+            w.term = cpd.prim;
+            w.i += 1;
+            w.live = false;
+            return ADV_END;
+        }
+        w.stage = WS_STAR;
     }
-    if (Nm.present) {
-        F2 p = w.ob ? w.ip - w.in * 1e-5f : w.ip;
-        float sr = star_radius2<X>(Nm, p, d_dir) * 0.98f;
-        sr = fmx(sr, P.min_R);
-        R = fmn(R, sr);
+    if (w.stage == WS_STAR) {
+        w.stage = WS_SIL;
+        if (Nm.present) {  // star radius on the Neumann part
+            F2 p = w.ob ? w.ip - w.in * 1e-5f : w.ip;
+            t.r_max = star_bound(Nm, p, w.d_dir);
+            if (Nm.py_sils && sil_begin<2>(t, Nm, p, t.r_max)) return ADV_QUERY;
+        }
     }
-    F2 ref = f2(0.0f, 0.0f);
-    int cv_prim = -1;
-    float cv_t = 0.0f;
-    F2 cv_n = f2(0.0f, 0.0f);
-    if (Nm.present) {
-        float rnd, pmf;
-        int idx = cdf_sample(Nm, rng.next(), rnd, pmf);
-        float4 e = ((const float4*)Nm.local_prims)[idx];
-        F2 a = f2(e.x, e.y), b = f2(e.z, e.w);
-        F2 tg = b - a;
-        F2 bn = normalize(f2(tg.y, -tg.x));
-        F2 bp = lerp(a, b, rnd);
-        float bpdf = pmf / norm(tg);
-        CP cr = closest2(F, w.ip);  // boundary_control_variates (wost.py:123-126)
+    if (w.stage == WS_SIL) {
+        w.aux.ref = f2(0.0f, 0.0f);
+        w.aux.cv_prim = -1;
+        w.stage = WS_MOVE;
+        if (Nm.present) {
+            float sr = t.r_max * 0.98f;
+            sr = fmx(sr, P.min_R);
+            w.R = fmn(w.R, sr);
+            float rnd, pmf;
+            int idx = cdf_sample(Nm, rng.next(), rnd, pmf);
+            float4 e = ((const float4*)Nm.local_prims)[idx];
+            F2 a = f2(e.x, e.y), b = f2(e.z, e.w);
+            F2 tg = b - a;
+            w.aux.bn = normalize(f2(tg.y, -tg.x));
+            w.aux.bp = lerp(a, b, rnd);
+            w.aux.bpdf = pmf / norm(tg);
+            w.aux.rnd = rnd;
+            w.aux.idx = idx;
+            closest2_begin(t, F, w.ip);  // boundary_control_variates (wost.py:123-126)
+            w.stage = WS_CV;
+            return ADV_QUERY;
+        }
+    } else if (w.stage == WS_CV) {
+        CP cr = closest2_end(t);
         float hc = h2(bc, F, cr.prim, cr.t, f2(cr.p.x, cr.p.y), f2(cr.n.x, cr.n.y));
-        ref = f2(hc * cr.n.x, hc * cr.n.y);
+        F2 ref = f2(hc * cr.n.x, hc * cr.n.y);
+        F2 bp = w.aux.bp, bn = w.aux.bn;
         float alpha = w.ob ? 2.0f : 1.0f;  // handle_boundary (wost.py:104-121)
         float hb = 0.0f;
-        float G = clampf(greens2_G(w.ip, R, w.ip, bp), -10.0f, 10.0f);
-        if (norm(w.ip - bp) <= R) {
-            float h = h2(bc, Nm, idx, rnd, bp, bn);
+        float G = clampf(greens2_G(w.ip, w.R, w.ip, bp), -10.0f, 10.0f);
+        if (norm(w.ip - bp) <= w.R) {
+            float h = h2(bc, Nm, w.aux.idx, w.aux.rnd, bp, bn);
             if (P.cv) h = h - dot(ref, bn);
-            hb = 0.0f - G * h / bpdf;
+            hb = 0.0f - G * h / w.aux.bpdf;
             if constexpr (A) {
-                float k = adj.w * alpha * (G / bpdf);  // d/dh of -alpha G h / pdf is -k
-                adj_lin2(adj.dh, Nm, idx, rnd, -k);
+                float k = adj.w * alpha * (G / w.aux.bpdf);  // d/dh of -alpha G h / pdf is -k
+                adj_lin2(adj.dh, Nm, w.aux.idx, w.aux.rnd, -k);
                 if (P.cv) adj_lin2(adj.dh, F, cr.prim, cr.t, k * dot(f2(cr.n.x, cr.n.y), bn));
             }
         }
         w.result += alpha * hb;
-        if constexpr (A) cv_prim = cr.prim, cv_t = cr.t, cv_n = f2(cr.n.x, cr.n.y);
+        w.aux.ref = ref;
+        if constexpr (A) w.aux.cv_prim = cr.prim, w.aux.cv_t = cr.t, w.aux.cv_n = f2(cr.n.x, cr.n.y);
+        w.stage = WS_MOVE;
     }
-    F2 dir = uniform_dir2(rng);
-    if (w.ob && (dot(dir, w.in) > 0.0f)) dir = -dir;
+    if (w.stage == WS_MOVE) {
+        F2 dir = uniform_dir2(rng);
+        if (w.ob && (dot(dir, w.in) > 0.0f)) dir = -dir;
+        F2 o = w.ob ? w.ip - w.in * 1e-5f : w.ip;  // Polyline.intersect (polyline.py:435-466)
+        w.stage = WS_ISECT;
+        if (isect2_begin(t, F, o, dir, w.R)) return ADV_QUERY;
+    }
+    // WS_ISECT: move to the hit or the ball, then the source term
+    Hit its = isect2_end(t);
+    F2 o = f2(t.x.x, t.x.y), dir = f2(t.v.x, t.v.y);
     F2 _p = w.ip;
-    F2 o = w.ob ? w.ip - w.in * 1e-5f : w.ip;  // Polyline.intersect (polyline.py:435-466)
-    Hit its = intersect2(F, o, dir, R);
     if (its.valid) {
         w.ip = f2(its.p.x, its.p.y);
         w.in = f2(its.n.x, its.n.y);
         w.ob = true;
     } else {
-        w.ip = o + dir * R;
+        w.ip = o + dir * w.R;
         w.in = dir;
-        its.d = R;
+        its.d = w.R;
         w.ob = false;
     }
-    if (P.cv) w.result += dot(_p - w.ip, ref);
+    if (P.cv) w.result += dot(_p - w.ip, w.aux.ref);
     if constexpr (A) {  // d/dhc of dot(_p - ip, hc n_c)
-        if (P.cv && cv_prim >= 0) adj_lin2(adj.dh, F, cv_prim, cv_t, adj.w * dot(_p - w.ip, cv_n));
+        if (P.cv && w.aux.cv_prim >= 0)
+            adj_lin2(adj.dh, F, w.aux.cv_prim, w.aux.cv_t, adj.w * dot(_p - w.ip, w.aux.cv_n));
     }
     F2 sp;  // handle_source (wost.py:128-143)
     float sr_, spdf;
-    greens2_sample(_p, R, dir, rng, sp, sr_, spdf);
+    greens2_sample(_p, w.R, dir, rng, sp, sr_, spdf);
     float src = 0.0f;
-    if (norm(sp - _p) < its.d) src += (R * R / 4.0f) * src_f(bc);
+    if (norm(sp - _p) < its.d) src += (w.R * w.R / 4.0f) * src_f(bc);
     w.result += src;
     w.i += 1;
-    if (!(w.i < P.nsteps)) w.live = false;
-    return w.live;
+    return step_done(w, P);
 }
 
-template <bool X, bool A = false>
-DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk<3>& w, Rng& rng,
-                   const Adj& adj = Adj{0.0f, nullptr, nullptr}) {
-    if (P.flags & DW_SOLVE_WOS) return wos_step<A>(sc, bc, P, w, rng, adj);
+template <bool X, bool A>
+DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk<3>& w, Rng& rng, Trav& t,
+                     const Adj& adj) {
     const SubDev& F = sc.sub[0];
     const SubDev& Dn = sc.sub[1];
     const SubDev& Nm = sc.sub[2];
-    if (!(w.i < P.nsteps)) {
-        w.live = false;
-        return false;
-    }
-    if (__builtin_isnan(w.ip.x) || __builtin_isnan(w.ip.y) || __builtin_isnan(w.ip.z)) {
-        // A walk that overflowed to infinity keeps a NaN coordinate from here on
-        // (ip' = ip + dir R), and every remaining step is then fixed: no
-        // closest-point or hit test can succeed (all distances are NaN), so it
-        // never terminates early, the Neumann and source terms are skipped
-        // (their |y - x| tests are false) and only `result += 0` remains.  The
-        // traversals of such steps visit whole trees (NaN box distances clamp
-        // to 0), so they are replaced by the outcome: the step count runs out
-        // and the stream advances by the draws of the skipped steps (cdf, two
-        // triangle coordinates, two direction and two source draws).
-        uint64_t left = (uint64_t)(P.nsteps - w.i);
-        rng.advance(left * (Nm.present ? 7u : 4u));
-        w.result += 0.0f;
-        w.i = P.nsteps;
-        w.live = false;
-        return false;
-    }
-    float R = kInf, d_dir = kInf;
-    CP cpd;
-    if (Dn.present) {
-        cpd = closest3(Dn, w.ip);
-        R = d_dir = __builtin_fabsf(cpd.d);
-    }
-    if (d_dir < P.eps) {
-        w.result += g3(bc, Dn, cpd);
-        if constexpr (A) adj_lin3(adj.dg, Dn, cpd.prim, cpd.u, cpd.v, adj.w);
-        w.term = cpd.prim;
-        w.i += 1;
-        w.live = false;
-        return false;
-    }
-    F3 p = w.ob ? w.ip - w.in * 1e-3f : w.ip;
-    float sr = star_radius3<X>(F, p, d_dir) * 0.98f;
-    sr = fmx(sr, P.min_R);
-    R = fmn(R, sr);
-    if (Nm.present) {  // Mesh.sample_boundary (mesh.py:474-490)
-        float rnd, pmf;
-        int idx = cdf_sample(Nm, rng.next(), rnd, pmf);
-        const float* t = Nm.local_prims + 9 * idx;
-        F3 a = f3(t[0], t[1], t[2]), b = f3(t[3], t[4], t[5]), c = f3(t[6], t[7], t[8]);
-        F3 cr = cross(b - a, c - a);
-        float area = norm(cr) / 2.0f;
-        float r1 = rng.next();
-        float r2 = rng.next();
-        float su = __builtin_sqrtf(r1) * (1.0f - r2);
-        float sv = __builtin_sqrtf(r1) * r2;
-        F3 bp = (b * su + c * sv) + a * ((1.0f - su) - sv);
-        F3 bn = normalize(cr);
-        float bpdf = pmf / area;
-        float G = clampf(greens3_G(w.ip, R, w.ip, bp), -100.0f, 100.0f);
-        float alpha = w.ob ? 2.0f : 1.0f;
-        if (norm(w.ip - bp) <= R) {
-            float h = h3(bc, Nm, idx, su, sv, bp, bn);
-            w.result -= G * h * alpha / bpdf;
-            if constexpr (A) adj_lin3(adj.dh, Nm, idx, su, sv, -adj.w * ((G * alpha) / bpdf));
+    if (w.stage == WS_START) {
+        if (!(w.i < P.nsteps)) {
+            w.live = false;
+            return ADV_END;
         }
+        if (P.flags & DW_SOLVE_WOS) {
+            closest3_begin(t, F, w.ip);
+            w.stage = WS_WOS;
+            return ADV_QUERY;
+        }
+        if (__builtin_isnan(w.ip.x) || __builtin_isnan(w.ip.y) || __builtin_isnan(w.ip.z)) {
+            // A walk that overflowed to infinity keeps a NaN coordinate from here on
+            // (ip' = ip + dir R), and every remaining step is then fixed: no
+            // closest-point or hit test can succeed (all distances are NaN), so it
+            // never terminates early, the Neumann and source terms are skipped
+            // (their |y - x| tests are false) and only `result += 0` remains.  The
+            // traversals of such steps visit whole trees (NaN box distances clamp
+            // to 0), so they are replaced by the outcome: the step count runs out
+            // and the stream advances by the draws of the skipped steps (cdf, two
+            // triangle coordinates, two direction and two source draws).
+            uint64_t left = (uint64_t)(P.nsteps - w.i);
+            rng.advance(left * (Nm.present ? 7u : 4u));
+            w.result += 0.0f;
+            w.i = P.nsteps;
+            w.live = false;
+            return ADV_END;
+        }
+        if (Dn.present) {
+            closest3_begin(t, Dn, w.ip);
+            w.stage = WS_DIR;
+            return ADV_QUERY;
+        }
+        w.R = w.d_dir = kInf;
+        w.stage = WS_STAR;
+    } else if (w.stage == WS_WOS) {
+        CP c = closest3_end(t);
+        float d = __builtin_fabsf(c.d);
+        w.i += 1;
+        if (d < P.eps) {
+            w.result += g3(bc, F, c);
+            if constexpr (A) adj_lin3(adj.dg, F, c.prim, c.u, c.v, adj.w);
+            w.term = c.prim;
+            w.live = false;
+            return ADV_END;
+        }
+        F3 dir = uniform_sphere(rng);
+        w.ip = w.ip + dir * d;
+        return step_done(w, P);
+    } else if (w.stage == WS_DIR) {
+        CP cpd = closest3_end(t);
+        w.R = w.d_dir = __builtin_fabsf(cpd.d);
+        if (w.d_dir < P.eps) {
+            w.result += g3(bc, Dn, cpd);
+            if constexpr (A) adj_lin3(adj.dg, Dn, cpd.prim, cpd.u, cpd.v, adj.w);
+            w.term = cpd.prim;
+            w.i += 1;
+            w.live = false;
+            return ADV_END;
+        }
+        w.stage = WS_STAR;
     }
-    F3 dir = uniform_sphere(rng);
-    if (w.ob && (dot(dir, w.in) > 0.0f)) dir = -dir;
+    if (w.stage == WS_STAR) {  // star radius on the full mesh
+        F3 p = w.ob ? w.ip - w.in * 1e-3f : w.ip;
+        t.r_max = star_bound(F, p, w.d_dir);
+        w.stage = WS_SIL;
+        if (F.py_sils && sil_begin<3>(t, F, p, t.r_max)) return ADV_QUERY;
+    }
+    if (w.stage == WS_SIL) {
+        float sr = t.r_max * 0.98f;
+        sr = fmx(sr, P.min_R);
+        w.R = fmn(w.R, sr);
+        if (Nm.present) {  // Mesh.sample_boundary (mesh.py:474-490)
+            float rnd, pmf;
+            int idx = cdf_sample(Nm, rng.next(), rnd, pmf);
+            const float* tp = Nm.local_prims + 9 * idx;
+            F3 a = f3(tp[0], tp[1], tp[2]), b = f3(tp[3], tp[4], tp[5]), c = f3(tp[6], tp[7], tp[8]);
+            F3 cr = cross(b - a, c - a);
+            float area = norm(cr) / 2.0f;
+            float r1 = rng.next();
+            float r2 = rng.next();
+            float su = __builtin_sqrtf(r1) * (1.0f - r2);
+            float sv = __builtin_sqrtf(r1) * r2;
+            F3 bp = (b * su + c * sv) + a * ((1.0f - su) - sv);
+            F3 bn = normalize(cr);
+            float bpdf = pmf / area;
+            float G = clampf(greens3_G(w.ip, w.R, w.ip, bp), -100.0f, 100.0f);
+            float alpha = w.ob ? 2.0f : 1.0f;
+            if (norm(w.ip - bp) <= w.R) {
+                float h = h3(bc, Nm, idx, su, sv, bp, bn);
+                w.result -= G * h * alpha / bpdf;
+                if constexpr (A) adj_lin3(adj.dh, Nm, idx, su, sv, -adj.w * ((G * alpha) / bpdf));
+            }
+        }
+        F3 dir = uniform_sphere(rng);
+        if (w.ob && (dot(dir, w.in) > 0.0f)) dir = -dir;
+        F3 o = w.ob ? w.ip - w.in * 1e-5f : w.ip;  // Mesh.intersect (mesh.py:383-415)
+        w.stage = WS_ISECT;
+        if (isect3_begin(t, F, o, dir, w.R)) return ADV_QUERY;
+    }
+    // WS_ISECT: move to the hit or the ball, then the source term
+    Hit its = isect3_end(t);
+    F3 o = t.x, dir = t.v;
     F3 _p = w.ip;
-    F3 o = w.ob ? w.ip - w.in * 1e-5f : w.ip;  // Mesh.intersect (mesh.py:383-415)
-    Hit its = intersect3(F, o, dir, R);
     if (its.valid) {
         w.ip = its.p;
         w.in = its.n;
         w.ob = true;
     } else {
-        w.ip = o + dir * R;
+        w.ip = o + dir * w.R;
         w.in = dir;
-        its.d = R;
+        its.d = w.R;
         w.ob = false;
     }
     F3 sp;
     float sr_, spdf;
-    greens3_sample(_p, R, dir, rng, sp, sr_, spdf);
+    greens3_sample(_p, w.R, dir, rng, sp, sr_, spdf);
     float src = 0.0f;
-    if (norm(sp - _p) < its.d) src += (R * R / 6.0f) * src_f(bc);
+    if (norm(sp - _p) < its.d) src += (w.R * w.R / 6.0f) * src_f(bc);
     w.result += src;
     w.i += 1;
-    if (!(w.i < P.nsteps)) w.live = false;
-    return w.live;
+    return step_done(w, P);
+}
+
+// One whole step (or the walk's end) with the traversals run back to back.
+template <bool X, bool A = false, int D>
+DWD bool walk_step(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Walk<D>& w, Rng& rng,
+                   const Adj& adj = Adj{0.0f, nullptr, nullptr}) {
+    Trav t;
+    Ovf o;
+    while (true) {
+        int r = walk_advance<X, A>(sc, bc, P, w, rng, t, adj);
+        if (r != ADV_QUERY) return r == ADV_STEP;
+        while (!trav_iter<D, X>(t, o)) {
+        }
+    }
 }
 
 // final value of a finished walk: 2D guard (wost.py:218-219); 3D has none

@@ -23,6 +23,13 @@ using namespace dwd;
 namespace {
 
 constexpr int kBlock = 256;
+// Waiting lanes of a wave that trigger a walk-logic pass in the interleaved
+// persistent loop (run_jobs); 16-32 measured within 2% on the icosphere.
+#ifndef DW_GATE
+#define DW_GATE 24
+#endif
+constexpr int kGate = DW_GATE;
+
 constexpr size_t kStackLds = (size_t)kBlock * kStackLdsBytesPerThread;  // traversal stacks
 // Minimum resident waves per SIMD for the persistent walk kernels: the
 // literal-cone kernels keep their fp64 acos/asin/atan2 without spilling at 2;
@@ -222,6 +229,7 @@ struct JobQ {
     const int* order;         // spatial position -> point index (nullptr: identity)
     int64_t n;                // points
     int64_t per;              // jobs per point
+    int gate;                 // waiting lanes that trigger a walk-logic pass (run_jobs)
 };
 struct QState {
     int part;  // current range (wave-uniform)
@@ -341,56 +349,124 @@ DWD F3 load_pt<3>(const float* pts, int64_t i) {
     return f3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
 }
 
+// Lane scheduling of the persistent walk kernels.  Each lane owns one job (one
+// or more walks, see the kernels) and runs its walk as the step machine of
+// walk_advance.  Two loops:
+//  * step-synchronous: each loop iteration advances every lane's walk by one
+//    whole step, its traversals run back to back;
+//  * interleaved: the loop body runs one traversal iteration for every lane
+//    with a query in flight.  A lane whose query has ended waits; once Q.gate
+//    lanes of the wave wait (or none is traversing) the waiting lanes run their
+//    walk logic up to their next query, idle lanes take new jobs, and all
+//    rejoin the traversal.  A wave is then held neither by its longest walk nor
+//    by its longest traversal, while the walk logic between queries (sampling,
+//    transcendentals) still runs for many lanes at once.
+// The interleaved loop pays off where traversal lengths diverge most: on the
+// 81,920-triangle icosphere (3D, 262,144 points) it measured 1.3x the
+// step-synchronous loop, on the 2D holes scene (65,536 points) 0.54x.
+// start(job, w, rng): set up the job's first walk (false: nothing to walk).
+// next(w, rng): the walk in w has ended; set up the job's next one (false: the
+// job is complete).  advance(w, rng, t): walk_advance with the job's scene.
+template <int D>
+constexpr bool kInterleave = D == 3;
+
+template <int D, bool X, typename Start, typename Next, typename Adv>
+DWD void run_jobs(const JobQ& Q, Start& start, Next& next, Adv& advance) {
+    Walk<D> w;
+    Rng rng;
+    Trav t;
+    Ovf o;
+    bool busy = false, trav = false, done = false;
+    QState qs;
+    q_start(qs);
+    if constexpr (!kInterleave<D>) {
+        // one whole step per lane and loop iteration, traversals back to back
+        while (true) {
+            int64_t got = q_next(!done && !busy, Q, qs);
+            if (got == -2) done = true;
+            if (got >= 0) busy = start(got, w, rng);
+            if (__ballot(!done) == 0) break;
+            if (busy) {
+                int r;
+                while ((r = advance(w, rng, t)) == ADV_QUERY) {
+                    while (!trav_iter<D, X>(t, o)) {
+                    }
+                }
+                if (r == ADV_END && !next(w, rng)) busy = false;
+            }
+        }
+        return;
+    }
+    while (true) {
+        if (__ballot(trav) == 0 || __popcll(__ballot(!trav && !done)) >= Q.gate) {
+            int64_t got = q_next(!done && !busy, Q, qs);
+            if (got == -2) done = true;
+            if (got >= 0) busy = start(got, w, rng);
+            if (busy && !trav) {
+                while (true) {
+                    int r = advance(w, rng, t);
+                    if (r == ADV_QUERY) {
+                        trav = true;
+                        break;
+                    }
+                    if (r == ADV_END && !next(w, rng)) {
+                        busy = false;
+                        break;
+                    }
+                }
+            }
+            if (__ballot(!done) == 0) break;
+        }
+        if (trav && trav_iter<D, X>(t, o)) trav = false;
+    }
+}
+#define DW_LAMBDA __attribute__((always_inline))
+
 // Solver.solve jobs: job = i*W + k, nsubwalks walks on the job's stream.
 template <int D, bool X>
-__global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVES_FAST) k_walk_u_pers(const SceneDev* __restrict__ scp, BCDev bc, ParamsDev P,
-                                                        const float* __restrict__ pts, int64_t n, float* walk_u,
-                                                        int* walk_steps, int* walk_term, uint8_t* walk_flags,
-                                                        JobQ Q) {
+__global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVES_FAST)
+    k_walk_u_pers(const SceneDev* __restrict__ scp, BCDev bc, ParamsDev P, const float* __restrict__ pts, int64_t n,
+                  float* walk_u, int* walk_steps, int* walk_term, uint8_t* walk_flags, JobQ Q) {
     const SceneDev& sc = *scp;
     using V = typename VecT<D>::T;
     int64_t job = -1;
-    bool done = false;
-    QState qs;
-    q_start(qs);
-    Walk<D> w;
-    Rng rng;
     V x;
     float res = 0.0f;
     int steps = 0, sub = 0;
     uint32_t kept = 0;
-    while (true) {
-        int64_t got = q_next(!done && job < 0, Q, qs);
-        if (got == -2) done = true;
-        if (got >= 0) {
-            job = got;
-            kept = 0;
-            int64_t i = job / P.nwalks;
-            seed_stream(rng, P, i, P.nwalks, job - i * P.nwalks);
-            x = load_pt<D>(pts, i);
+    auto start = [&](int64_t got, Walk<D>& w, Rng& rng) DW_LAMBDA -> bool {
+        job = got;
+        kept = 0;
+        int64_t i = job / P.nwalks;
+        seed_stream(rng, P, i, P.nwalks, job - i * P.nwalks);
+        x = load_pt<D>(pts, i);
+        walk_init<D>(w, x);
+        res = 0.0f;
+        steps = 0;
+        sub = 0;
+        return true;
+    };
+    auto next = [&](Walk<D>& w, Rng&) DW_LAMBDA -> bool {
+        float v = walk_value(w, P);
+        if (v == w.result) kept |= 1u << (sub & 7);
+        res += v;
+        steps += w.i;
+        if (++sub < P.nsubwalks) {
             walk_init<D>(w, x);
-            res = 0.0f;
-            steps = 0;
-            sub = 0;
+            return true;
         }
-        if (__ballot(!done) == 0) break;
-        if (job >= 0 && !walk_step<X>(sc, bc, P, w, rng)) {
-            float v = walk_value(w, P);
-            if (v == w.result) kept |= 1u << (sub & 7);
-            res += v;
-            steps += w.i;
-            if (++sub < P.nsubwalks) {
-                walk_init<D>(w, x);
-            } else {
-                walk_u[job] = res / (float)P.nsubwalks;
-                if (walk_steps) walk_steps[job] = steps;
-                if (walk_term) walk_term[job] = w.term;
-                if (walk_flags) walk_flags[job] = (uint8_t)kept;
-                job = -1;
-            }
-        }
-    }
+        walk_u[job] = res / (float)P.nsubwalks;
+        if (walk_steps) walk_steps[job] = steps;
+        if (walk_term) walk_term[job] = w.term;
+        if (walk_flags) walk_flags[job] = (uint8_t)kept;
+        return false;
+    };
+    auto advance = [&](Walk<D>& w, Rng& rng, Trav& t) DW_LAMBDA -> int {
+        return walk_advance<X, false>(sc, bc, P, w, rng, t, Adj{0.0f, nullptr, nullptr});
+    };
+    run_jobs<D, X>(Q, start, next, advance);
 }
+
 
 // Distance to the full boundary and the inside flag per point (used by the
 // gradient estimators, wost.py:43-50).
@@ -457,97 +533,93 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
 #define ITEM(f) (B ? items[it].f : one.f)
     int it = 0;
     int64_t job = -1;
-    bool done = false;
-    QState qs;
-    q_start(qs);
-    Walk<D> w;
-    Rng rng;
     V p, dir, gs;
     float R = 0.0f, up = 0.0f;
     uint64_t saved = 0;
     int phase = 0, steps = 0;
     bool kept_plus = false;
-    while (true) {
-        int64_t got = q_next(!done && job < 0, Q, qs);
-        if (got == -2) done = true;
-        if (got >= 0) {
-            if constexpr (B) {
-                it = find_item(n_items, got, [&](int k) { return items[k].job0; });
-                got -= items[it].job0;
-            }
-            const int npairs = ITEM(npairs);
-            int64_t i = got / npairs;
-            if (!ITEM(inside)[i]) {
-                float* o = ITEM(pair_vals) + got * (D + 1);
-                for (int q = 0; q <= D; ++q) o[q] = 0.0f;
-                ITEM(pair_steps)[got] = 0;
-                if (ITEM(pair_flags)) ITEM(pair_flags)[got] = 0;
-            } else {
-                job = got;
-                R = ITEM(R)[i];
-                p = load_pt<D>(ITEM(pts), i);
-                seed_stream(rng, ITEM(P), i, npairs, job - i * npairs);
-                V p_src;
-                float r_src, pdf_src;
-                if constexpr (D == 2) {
-                    dir = uniform_dir2(rng);
-                    F2 dir_src = uniform_dir2(rng);
-                    greens2_sample(p, R, dir_src, rng, p_src, r_src, pdf_src);
-                    gs = greens2_gradient(p, R, r_src, p_src) / (pdf_src * (R * R / 4.0f));
-                } else {
-                    dir = uniform_sphere(rng);
-                    greens3_sample(p, R, dir, rng, p_src, r_src, pdf_src);
-                    gs = greens3_gradient(p, R, r_src, p_src) / (pdf_src * (R * R / 6.0f));
-                }
-                saved = rng.state;
-                walk_init<D>(w, p + dir * R);
-                phase = 0;
-                steps = 0;
-            }
+    auto start = [&](int64_t got, Walk<D>& w, Rng& rng) DW_LAMBDA -> bool {
+        if constexpr (B) {
+            it = find_item(n_items, got, [&](int k) { return items[k].job0; });
+            got -= items[it].job0;
         }
-        if (__ballot(!done) == 0) break;
-        if (job >= 0 && !walk_step<X>(*ITEM(sc), ITEM(bc), ITEM(P), w, rng)) {
-            float v = walk_value(w, ITEM(P));
-            steps += w.i;
-            // walks the guards zero contribute nothing to the adjoint
-            bool kept = D == 2 ? v == w.result : !(bad(v) || __builtin_fabsf(v) > 10000.0f);
-            if (phase == 0) {
-                kept_plus = kept;
-                up = v;
-                rng.state = saved;
-                walk_init<D>(w, p - dir * R);
-                phase = 1;
-            } else {
-                float um = v;
-                float* o = ITEM(pair_vals) + job * (D + 1);
-                const float fsrc = src_f(ITEM(bc));
-                if constexpr (D == 2) {
-                    float s = (R * R / 4.0f) * fsrc;
-                    float k = 2.0f / R * (up - um);
-                    float g0 = k * dir.x + (s - s) * gs.x, g1 = k * dir.y + (s - s) * gs.y;
-                    o[0] = (up + um) + (s + s);
-                    o[1] = bad(g0) ? 0.0f : g0;
-                    o[2] = bad(g1) ? 0.0f : g1;
-                } else {
-                    float u1 = (bad(up) || __builtin_fabsf(up) > 10000.0f) ? 0.0f : up;
-                    float u2 = (bad(um) || __builtin_fabsf(um) > 10000.0f) ? 0.0f : um;
-                    float s = (R * R / 6.0f) * fsrc;
-                    float k = 3.0f / R * (u1 - u2);
-                    float g0 = k * dir.x + (s - s) * gs.x, g1 = k * dir.y + (s - s) * gs.y,
-                          g2_ = k * dir.z + (s - s) * gs.z;
-                    o[0] = (u1 + u2) + (s + s);
-                    o[1] = bad(g0) ? 0.0f : g0;
-                    o[2] = bad(g1) ? 0.0f : g1;
-                    o[3] = bad(g2_) ? 0.0f : g2_;
-                }
-                ITEM(pair_steps)[job] = steps;
-                if (ITEM(pair_flags)) ITEM(pair_flags)[job] = (uint8_t)((kept_plus ? 1 : 0) | (kept ? 2 : 0));
-                job = -1;
-            }
+        const int npairs = ITEM(npairs);
+        int64_t i = got / npairs;
+        if (!ITEM(inside)[i]) {
+            float* o = ITEM(pair_vals) + got * (D + 1);
+            for (int q = 0; q <= D; ++q) o[q] = 0.0f;
+            ITEM(pair_steps)[got] = 0;
+            if (ITEM(pair_flags)) ITEM(pair_flags)[got] = 0;
+            return false;
         }
-    }
+        job = got;
+        R = ITEM(R)[i];
+        p = load_pt<D>(ITEM(pts), i);
+        seed_stream(rng, ITEM(P), i, npairs, job - i * npairs);
+        V p_src;
+        float r_src, pdf_src;
+        if constexpr (D == 2) {
+            dir = uniform_dir2(rng);
+            F2 dir_src = uniform_dir2(rng);
+            greens2_sample(p, R, dir_src, rng, p_src, r_src, pdf_src);
+            gs = greens2_gradient(p, R, r_src, p_src) / (pdf_src * (R * R / 4.0f));
+        } else {
+            dir = uniform_sphere(rng);
+            greens3_sample(p, R, dir, rng, p_src, r_src, pdf_src);
+            gs = greens3_gradient(p, R, r_src, p_src) / (pdf_src * (R * R / 6.0f));
+        }
+        saved = rng.state;
+        walk_init<D>(w, p + dir * R);
+        phase = 0;
+        steps = 0;
+        return true;
+    };
+    auto next = [&](Walk<D>& w, Rng& rng) DW_LAMBDA -> bool {
+        float v = walk_value(w, ITEM(P));
+        steps += w.i;
+        // walks the guards zero contribute nothing to the adjoint
+        bool kept = D == 2 ? v == w.result : !(bad(v) || __builtin_fabsf(v) > 10000.0f);
+        if (phase == 0) {
+            kept_plus = kept;
+            up = v;
+            rng.state = saved;
+            walk_init<D>(w, p - dir * R);
+            phase = 1;
+            return true;
+        }
+        float um = v;
+        float* o = ITEM(pair_vals) + job * (D + 1);
+        const float fsrc = src_f(ITEM(bc));
+        if constexpr (D == 2) {
+            float s = (R * R / 4.0f) * fsrc;
+            float k = 2.0f / R * (up - um);
+            float g0 = k * dir.x + (s - s) * gs.x, g1 = k * dir.y + (s - s) * gs.y;
+            o[0] = (up + um) + (s + s);
+            o[1] = bad(g0) ? 0.0f : g0;
+            o[2] = bad(g1) ? 0.0f : g1;
+        } else {
+            float u1 = (bad(up) || __builtin_fabsf(up) > 10000.0f) ? 0.0f : up;
+            float u2 = (bad(um) || __builtin_fabsf(um) > 10000.0f) ? 0.0f : um;
+            float s = (R * R / 6.0f) * fsrc;
+            float k = 3.0f / R * (u1 - u2);
+            float g0 = k * dir.x + (s - s) * gs.x, g1 = k * dir.y + (s - s) * gs.y,
+                  g2_ = k * dir.z + (s - s) * gs.z;
+            o[0] = (u1 + u2) + (s + s);
+            o[1] = bad(g0) ? 0.0f : g0;
+            o[2] = bad(g1) ? 0.0f : g1;
+            o[3] = bad(g2_) ? 0.0f : g2_;
+        }
+        ITEM(pair_steps)[job] = steps;
+        if (ITEM(pair_flags)) ITEM(pair_flags)[job] = (uint8_t)((kept_plus ? 1 : 0) | (kept ? 2 : 0));
+        return false;
+    };
+    auto advance = [&](Walk<D>& w, Rng& rng, Trav& t) DW_LAMBDA -> int {
+        return walk_advance<X, false>(*ITEM(sc), ITEM(bc), ITEM(P), w, rng, t, Adj{0.0f, nullptr, nullptr});
+    };
+    run_jobs<D, X>(Q, start, next, advance);
 #undef ITEM
 }
+
 
 // Batched point preparation and reduction: thread (wave) per point of the
 // concatenated point space.
@@ -583,61 +655,51 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
     const SceneDev& sc = *scp;
     using V = typename VecT<D>::T;
     const float inv = 1.0f / (float)(2 * npairs);
-    int64_t job = -1;
-    bool done = false;
-    QState qs;
-    q_start(qs);
-    Walk<D> w;
-    Rng rng;
     V p, dir;
     float R = 0.0f, wm = 0.0f;
     Adj adj{0.0f, dl_dg, dl_dh};
     uint64_t saved = 0;
     int phase = 0;
-    while (true) {
-        int64_t got = q_next(!done && job < 0, Q, qs);
-        if (got == -2) done = true;
-        if (got >= 0) {
-            int64_t i = got / npairs;
-            uint8_t fl = pair_flags[got];
-            if (insidep[i] && fl) {
-                job = got;
-                R = Rp[i];
-                p = load_pt<D>(pts, i);
-                seed_stream(rng, P, i, npairs, job - i * npairs);
-                V p_src;
-                float r_src, pdf_src;
-                float proj;
-                if constexpr (D == 2) {
-                    dir = uniform_dir2(rng);
-                    F2 dir_src = uniform_dir2(rng);
-                    greens2_sample(p, R, dir_src, rng, p_src, r_src, pdf_src);
-                    proj = dl_dgrad[2 * i] * dir.x + dl_dgrad[2 * i + 1] * dir.y;
-                } else {
-                    dir = uniform_sphere(rng);
-                    greens3_sample(p, R, dir, rng, p_src, r_src, pdf_src);
-                    proj = (dl_dgrad[3 * i] * dir.x + dl_dgrad[3 * i + 1] * dir.y) + dl_dgrad[3 * i + 2] * dir.z;
-                }
-                float k = (float)D / R * proj;
-                adj.w = (fl & 1) ? (dl_du[i] + k) * inv : 0.0f;
-                wm = (fl & 2) ? (dl_du[i] - k) * inv : 0.0f;
-                saved = rng.state;
-                walk_init<D>(w, p + dir * R);
-                phase = 0;
-            }
+    auto start = [&](int64_t got, Walk<D>& w, Rng& rng) DW_LAMBDA -> bool {
+        int64_t i = got / npairs;
+        uint8_t fl = pair_flags[got];
+        if (!(insidep[i] && fl)) return false;
+        R = Rp[i];
+        p = load_pt<D>(pts, i);
+        seed_stream(rng, P, i, npairs, got - i * npairs);
+        V p_src;
+        float r_src, pdf_src;
+        float proj;
+        if constexpr (D == 2) {
+            dir = uniform_dir2(rng);
+            F2 dir_src = uniform_dir2(rng);
+            greens2_sample(p, R, dir_src, rng, p_src, r_src, pdf_src);
+            proj = dl_dgrad[2 * i] * dir.x + dl_dgrad[2 * i + 1] * dir.y;
+        } else {
+            dir = uniform_sphere(rng);
+            greens3_sample(p, R, dir, rng, p_src, r_src, pdf_src);
+            proj = (dl_dgrad[3 * i] * dir.x + dl_dgrad[3 * i + 1] * dir.y) + dl_dgrad[3 * i + 2] * dir.z;
         }
-        if (__ballot(!done) == 0) break;
-        if (job >= 0 && !walk_step<X, true>(sc, bc, P, w, rng, adj)) {
-            if (phase == 0) {
-                rng.state = saved;
-                walk_init<D>(w, p - dir * R);
-                adj.w = wm;
-                phase = 1;
-            } else {
-                job = -1;
-            }
-        }
-    }
+        float k = (float)D / R * proj;
+        adj.w = (fl & 1) ? (dl_du[i] + k) * inv : 0.0f;
+        wm = (fl & 2) ? (dl_du[i] - k) * inv : 0.0f;
+        saved = rng.state;
+        walk_init<D>(w, p + dir * R);
+        phase = 0;
+        return true;
+    };
+    auto next = [&](Walk<D>& w, Rng& rng) DW_LAMBDA -> bool {
+        if (phase != 0) return false;
+        rng.state = saved;
+        walk_init<D>(w, p - dir * R);
+        adj.w = wm;
+        phase = 1;
+        return true;
+    };
+    auto advance = [&](Walk<D>& w, Rng& rng, Trav& t) DW_LAMBDA -> int {
+        return walk_advance<X, true>(sc, bc, P, w, rng, t, adj);
+    };
+    run_jobs<D, X>(Q, start, next, advance);
 }
 
 // Adjoint of Solver.solve: job i*W + k replays its nsubwalks walks with weight
@@ -650,44 +712,35 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
     const SceneDev& sc = *scp;
     using V = typename VecT<D>::T;
     const float inv = 1.0f / ((float)P.nwalks * (float)P.nsubwalks);
-    int64_t job = -1;
-    bool done = false;
-    QState qs;
-    q_start(qs);
-    Walk<D> w;
-    Rng rng;
     V x;
     Adj adj{0.0f, dl_dg, dl_dh};
     int sub = 0;
     uint32_t fl = 0;
     float wi = 0.0f;
-    while (true) {
-        int64_t got = q_next(!done && job < 0, Q, qs);
-        if (got == -2) done = true;
-        if (got >= 0) {
-            int64_t i = got / P.nwalks;
-            fl = walk_flags[got];
-            if (fl) {
-                job = got;
-                seed_stream(rng, P, i, P.nwalks, job - i * P.nwalks);
-                x = load_pt<D>(pts, i);
-                walk_init<D>(w, x);
-                sub = 0;
-                wi = dl_du[i] * inv;
-                adj.w = (fl & 1u) ? wi : 0.0f;
-            }
-        }
-        if (__ballot(!done) == 0) break;
-        if (job >= 0 && !walk_step<X, true>(sc, bc, P, w, rng, adj)) {
-            if (++sub < P.nsubwalks) {
-                walk_init<D>(w, x);
-                adj.w = (fl & (1u << (sub & 7))) ? wi : 0.0f;
-            } else {
-                job = -1;
-            }
-        }
-    }
+    auto start = [&](int64_t got, Walk<D>& w, Rng& rng) DW_LAMBDA -> bool {
+        int64_t i = got / P.nwalks;
+        fl = walk_flags[got];
+        if (!fl) return false;
+        seed_stream(rng, P, i, P.nwalks, got - i * P.nwalks);
+        x = load_pt<D>(pts, i);
+        walk_init<D>(w, x);
+        sub = 0;
+        wi = dl_du[i] * inv;
+        adj.w = (fl & 1u) ? wi : 0.0f;
+        return true;
+    };
+    auto next = [&](Walk<D>& w, Rng&) DW_LAMBDA -> bool {
+        if (++sub >= P.nsubwalks) return false;
+        walk_init<D>(w, x);
+        adj.w = (fl & (1u << (sub & 7))) ? wi : 0.0f;
+        return true;
+    };
+    auto advance = [&](Walk<D>& w, Rng& rng, Trav& t) DW_LAMBDA -> int {
+        return walk_advance<X, true>(sc, bc, P, w, rng, t, adj);
+    };
+    run_jobs<D, X>(Q, start, next, advance);
 }
+
 
 // Fused u + grad u: one antithetic pair per lane (pair k of point i uses the
 // stream of lane i*npairs + k).  Writes (usum, g[dim]) per pair and steps.
@@ -1265,6 +1318,7 @@ int make_queue(const dw_scene* s, StreamCtx& c, const float* pts, int64_t n, int
     Q.order = nullptr;
     Q.n = n;
     Q.per = per;
+    Q.gate = kGate;
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kQueueCtrs), 0, st, c.ctr, kQueueCtrs, 0ull);
     DW_HIP(hipGetLastError());
     if (!ows) return DW_OK;
@@ -1395,6 +1449,7 @@ int batch_launch(dw_batch* b, hipStream_t st) {
     Q.order = nullptr;
     Q.n = b->total_jobs;
     Q.per = 1;
+    Q.gate = kGate;
     FusedItem none;
     std::memset(&none, 0, sizeof(none));
     const FusedItem* items = b->dev_items;

@@ -199,8 +199,8 @@ DWD F3 nmax(const DNode3& n) { return f3(n.mxx, n.mxy, n.mxz); }
 
 // ---------------------------------------------------------------------------
 // Traversal stack: the bottom kLdsStack entries of each lane live in LDS
-// (entry k of lane t at [k * blockDim.x + t], conflict-free), deeper entries in
-// a private overflow array.  Traversals descend into the nearer child and push
+// (stk_slot; blocks are whole waves), deeper entries in a private overflow
+// array.  Traversals descend into the nearer child and push
 // only the farther one, so the stack never holds more than the tree depth.
 // Kernels that traverse must launch with kStackLdsBytes of dynamic LDS.
 // ---------------------------------------------------------------------------
@@ -214,9 +214,15 @@ struct Ovf {
     int oi[DW_MAX_STACK - kLdsStack];
     float od[DW_MAX_STACK - kLdsStack];
 };
-// LDS slot of entry k of this lane, addressed from dw_lds at each access so
-// the accesses stay LDS instructions
-DWD int stk_slot(int k) { return k * (int)blockDim.x + (int)threadIdx.x; }
+// LDS slots of this lane: each wave owns 2 * kLdsStack rows of 64 dwords
+// (node references, then bounds), entry k of lane l in row k, column l, so a
+// wave's accesses are conflict-free.  The slot depends on threadIdx.x only (no
+// block-size read) and is addressed from dw_lds at each access so the accesses
+// stay LDS instructions.
+DWD int stk_slot(int k) {
+    int tid = (int)threadIdx.x;
+    return (tid & ~63) * (2 * kLdsStack) + k * 64 + (tid & 63);
+}
 DWD void stk_push(int& sp, Ovf& o, int r, float d) {
     if (sp < kLdsStack) {
         dw_lds[stk_slot(sp)] = r;

@@ -31,6 +31,7 @@ constexpr int kBlock = 256;
 constexpr int kGate = DW_GATE;
 
 constexpr size_t kStackLds = (size_t)kBlock * kStackLdsBytesPerThread;  // traversal stacks
+static_assert(kBlock % 64 == 0, "the LDS stack layout assumes whole waves per block");
 // Minimum resident waves per SIMD for the persistent walk kernels: the
 // literal-cone kernels keep their fp64 acos/asin/atan2 without spilling at 2;
 // the transcendental-free ones run best at 4 (a few VGPRs spill).

@@ -914,63 +914,6 @@ DWD void closest3_begin(Trav& t, const SubDev& S, F3 p) {
     t.cd = kInf;
     t.sp = 0;
 }
-DWD bool closest3_iter(Trav& t, Ovf& o) {
-    const SubDev& S = *t.S;
-    F3 p = t.x;
-    if (t.cd <= t.r_max) {
-        if (t.ref < 0) {
-            int off, cnt;
-            leaf_range(S, t.ref, off, cnt);
-            for (int j = 0; j < cnt; ++j) {
-                F3 pt;
-                float u, v, d;
-                tri_closest(gload<DTri>(t.prims, off + j), p, pt, u, v, d);
-                if (d < t.d) {
-                    t.d = d;
-                    t.best = off + j;
-                    t.r_max = fmn(t.r_max, d);
-                }
-            }
-        } else {
-            RBvh3 rc = gload<RBvh3>(t.rec, t.ref);
-            float dminL, dmaxL, dminR, dmaxR;
-            box_dist(f3(rc.lmnx, rc.lmny, rc.lmnz), f3(rc.lmxx, rc.lmxy, rc.lmxz), p, dminL, dmaxL);
-            bool hitL = dminL <= t.r_max;
-            t.r_max = fmn(t.r_max, dmaxL);
-            box_dist(f3(rc.rmnx, rc.rmny, rc.rmnz), f3(rc.rmxx, rc.rmxy, rc.rmxz), p, dminR, dmaxR);
-            bool hitR = dminR <= t.r_max;
-            t.r_max = fmn(t.r_max, dmaxR);
-            if (hitL && hitR) {
-                // snch3d.py:383-388: equal zero d_min -> nearer d_max first; then d_min order
-                int closer = rc.ref_l, other = rc.ref_r;
-                if ((dminL == 0.0f) && (dminR == 0.0f) && (dmaxR < dmaxL)) closer = rc.ref_r, other = rc.ref_l;
-                float dc = dminL, dO = dminR;
-                if (dminR < dminL) {
-                    int tmp = closer;
-                    closer = other;
-                    other = tmp;
-                    dc = dminR;
-                    dO = dminL;
-                }
-                stk_push(t.sp, o, other, dO);
-                t.ref = closer;
-                t.cd = dc;
-                return false;
-            } else if (hitL) {
-                t.ref = rc.ref_l;
-                t.cd = dminL;
-                return false;
-            } else if (hitR) {
-                t.ref = rc.ref_r;
-                t.cd = dminR;
-                return false;
-            }
-        }
-    }
-    if (t.sp == 0) return true;
-    stk_pop(t.sp, o, t.ref, t.cd);
-    return false;
-}
 DWD CP closest3_end(const Trav& t) {
     CP its;
     its.p = its.n = f3(0.0f, 0.0f, 0.0f);
@@ -986,15 +929,6 @@ DWD CP closest3_end(const Trav& t) {
     }
     return its;
 }
-DWD CP closest3(const SubDev& S, F3 p) {
-    Trav t;
-    Ovf o;
-    closest3_begin(t, S, p);
-    while (!closest3_iter(t, o)) {
-    }
-    return closest3_end(t);
-}
-
 // SNCH3D.intersect (snch3d.py:411-476) + Triangle.ray_intersect (triangle.py:59-86);
 // the nearest hit wins (r_max drops to each accepted t); prim = BVH-sorted index
 // (triangle.py:82)
@@ -1033,38 +967,69 @@ DWD bool isect3_begin(Trav& t, const SubDev& S, F3 x, F3 v, float r_max) {
     t.cd = tmin;
     return true;
 }
-DWD bool isect3_iter(Trav& t, Ovf& o) {
+// One iteration of a 3D BVH query: closest point (snch3d.py:349-409) or ray
+// intersection (snch3d.py:411-476).  Both read the same records and triangles
+// and share the stack logic, so lanes of the two kinds side by side in the
+// interleaved walk loop share one fetch per iteration.
+DWD bool bvh3_iter(Trav& t, Ovf& o) {
     const SubDev& S = *t.S;
-    F3 x = t.x, v = t.v;
+    const bool cp = t.kind == TQ_CLOSEST;
     if (t.cd <= t.r_max) {
         if (t.ref < 0) {
             int off, cnt;
             leaf_range(S, t.ref, off, cnt);
             for (int j = 0; j < cnt; ++j) {
-                float bu, bw, t_;
-                if (tri_hit(gload<DTri>(t.prims, off + j), x, v, t.r_max, bu, bw, t_)) {
-                    t.r_max = t_;
-                    t.best = off + j;
+                DTri tr = gload<DTri>(t.prims, off + j);
+                if (cp) {
+                    F3 pt;
+                    float u, v, d;
+                    tri_closest(tr, t.x, pt, u, v, d);
+                    if (d < t.d) {
+                        t.d = d;
+                        t.best = off + j;
+                        t.r_max = fmn(t.r_max, d);
+                    }
+                } else {
+                    float bu, bw, t_;
+                    if (tri_hit(tr, t.x, t.v, t.r_max, bu, bw, t_)) {
+                        t.r_max = t_;
+                        t.best = off + j;
+                    }
                 }
             }
         } else {
             RBvh3 rc = gload<RBvh3>(t.rec, t.ref);
-            float t0, t1;
-            bool h0 = box_ray(f3(rc.lmnx, rc.lmny, rc.lmnz), f3(rc.lmxx, rc.lmxy, rc.lmxz), x, v, t.r_max, t0);
-            bool h1 = box_ray(f3(rc.rmnx, rc.rmny, rc.rmnz), f3(rc.rmxx, rc.rmxy, rc.rmxz), x, v, t.r_max, t1);
+            F3 lmn = f3(rc.lmnx, rc.lmny, rc.lmnz), lmx = f3(rc.lmxx, rc.lmxy, rc.lmxz);
+            F3 rmn = f3(rc.rmnx, rc.rmny, rc.rmnz), rmx = f3(rc.rmxx, rc.rmxy, rc.rmxz);
+            float d0, d1;
+            bool h0, h1, sw;  // sw: right child first
+            if (cp) {
+                float dmaxL, dmaxR;
+                box_dist(lmn, lmx, t.x, d0, dmaxL);
+                h0 = d0 <= t.r_max;
+                t.r_max = fmn(t.r_max, dmaxL);
+                box_dist(rmn, rmx, t.x, d1, dmaxR);
+                h1 = d1 <= t.r_max;
+                t.r_max = fmn(t.r_max, dmaxR);
+                // snch3d.py:383-388: equal zero d_min -> nearer d_max first; then d_min order
+                sw = ((d0 == 0.0f) && (d1 == 0.0f) && (dmaxR < dmaxL)) || (d1 < d0);
+            } else {
+                h0 = box_ray(lmn, lmx, t.x, t.v, t.r_max, d0);
+                h1 = box_ray(rmn, rmx, t.x, t.v, t.r_max, d1);
+                sw = d1 < d0;
+            }
             if (h0 && h1) {
-                bool sw = t1 < t0;
-                stk_push(t.sp, o, sw ? rc.ref_l : rc.ref_r, sw ? t0 : t1);
+                stk_push(t.sp, o, sw ? rc.ref_l : rc.ref_r, sw ? d0 : d1);
                 t.ref = sw ? rc.ref_r : rc.ref_l;
-                t.cd = sw ? t1 : t0;
+                t.cd = sw ? d1 : d0;
                 return false;
             } else if (h0) {
                 t.ref = rc.ref_l;
-                t.cd = t0;
+                t.cd = d0;
                 return false;
             } else if (h1) {
                 t.ref = rc.ref_r;
-                t.cd = t1;
+                t.cd = d1;
                 return false;
             }
         }
@@ -1073,6 +1038,15 @@ DWD bool isect3_iter(Trav& t, Ovf& o) {
     stk_pop(t.sp, o, t.ref, t.cd);
     return false;
 }
+DWD CP closest3(const SubDev& S, F3 p) {
+    Trav t;
+    Ovf o;
+    closest3_begin(t, S, p);
+    while (!bvh3_iter(t, o)) {
+    }
+    return closest3_end(t);
+}
+
 DWD Hit isect3_end(const Trav& t) {
     Hit its;
     its.valid = false;
@@ -1101,7 +1075,7 @@ DWD Hit intersect3(const SubDev& S, F3 x, F3 v, float r_max) {
     Trav t;
     Ovf o;
     if (isect3_begin(t, S, x, v, r_max))
-        while (!isect3_iter(t, o)) {
+        while (!bvh3_iter(t, o)) {
         }
     return isect3_end(t);
 }
@@ -1115,9 +1089,8 @@ DWD bool trav_iter_as(Trav& t, Ovf& o, int kind) {
         if (kind == TQ_SIL) return sil2_iter<X>(t, o);
         return isect2_iter(t, o);
     } else {
-        if (kind == TQ_CLOSEST) return closest3_iter(t, o);
         if (kind == TQ_SIL) return sil3_iter<X>(t, o);
-        return isect3_iter(t, o);
+        return bvh3_iter(t, o);
     }
 }
 template <int D, bool X>

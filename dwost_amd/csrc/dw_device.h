@@ -744,52 +744,6 @@ DWD bool sile_test(const DSilE& se, F3 x, float r_max, float& d, F3& pt) {
     F3 view = pt - x;
     return dot(view, n0) * dot(view, n1) < 0.0f;
 }
-template <bool X>
-DWD bool sil3_iter(Trav& t, Ovf& o) {
-    const SubDev& S = *t.S;
-    F3 x = t.x;
-    if (t.cd <= t.r_max) {
-        if (t.ref < 0) {
-            int off, cnt;
-            leaf_range(S, t.ref, off, cnt);
-            for (int j = 0; j < cnt; ++j) {
-                float d;
-                F3 pt;
-                if (sile_test(gload<DSilE>(t.prims, off + j), x, t.r_max, d, pt)) {
-                    t.r_max = fmn(t.r_max, d);
-                    t.best = off + j;
-                }
-            }
-        } else {
-            RSnch3 rc = gload<RSnch3>(t.rec, t.ref);
-            F3 lmn = f3(rc.lmnx, rc.lmny, rc.lmnz), lmx = f3(rc.lmxx, rc.lmxy, rc.lmxz);
-            F3 rmn = f3(rc.rmnx, rc.rmny, rc.rmnz), rmx = f3(rc.rmxx, rc.rmxy, rc.rmxz);
-            float d0, d1;
-            bool h0 = snch3_child<X>(lmn, lmx, f3(rc.lax, rc.lay, rc.laz), rc.lha, rc.lsin, rc.lcos, rc.lrad, rc.lrho,
-                                     rc.lelo, rc.lehi, x, t.r_max, d0);
-            bool h1 = snch3_child<X>(rmn, rmx, f3(rc.rax, rc.ray, rc.raz), rc.rha, rc.rsin, rc.rcos, rc.rrad, rc.rrho,
-                                     rc.relo, rc.rehi, x, t.r_max, d1);
-            if (h0 && h1) {
-                bool sw = d1 < d0;
-                stk_push(t.sp, o, sw ? rc.ref_l : rc.ref_r, sw ? d0 : d1);
-                t.ref = sw ? rc.ref_r : rc.ref_l;
-                t.cd = sw ? d1 : d0;
-                return false;
-            } else if (h0) {
-                t.ref = rc.ref_l;
-                t.cd = d0;
-                return false;
-            } else if (h1) {
-                t.ref = rc.ref_r;
-                t.cd = d1;
-                return false;
-            }
-        }
-    }
-    if (t.sp == 0) return true;
-    stk_pop(t.sp, o, t.ref, t.cd);
-    return false;
-}
 template <bool X = true>
 DWD bool closest_sil2(const SubDev& S, F2 x, float r_max, float& d_out, int& ref_out, F2& p_out) {
     Trav t;
@@ -810,7 +764,7 @@ DWD bool closest_sil3(const SubDev& S, F3 x, float r_max, float& d_out, int& ref
     Trav t;
     Ovf o;
     if (sil_begin<3>(t, S, x, r_max))
-        while (!sil3_iter<X>(t, o)) {
+        while (!trav3_iter<X>(t, o)) {
         }
     d_out = 0.0f;
     ref_out = t.best;
@@ -967,43 +921,67 @@ DWD bool isect3_begin(Trav& t, const SubDev& S, F3 x, F3 v, float r_max) {
     t.cd = tmin;
     return true;
 }
-// One iteration of a 3D BVH query: closest point (snch3d.py:349-409) or ray
-// intersection (snch3d.py:411-476).  Both read the same records and triangles
-// and share the stack logic, so lanes of the two kinds side by side in the
-// interleaved walk loop share one fetch per iteration.
-DWD bool bvh3_iter(Trav& t, Ovf& o) {
+// One iteration of a 3D query: closest point (snch3d.py:349-409), closest
+// silhouette (snch3d.py:594-663) or ray intersection (snch3d.py:411-476).  An
+// interior step of any kind fetches the record head (children's boxes and
+// references; SNCH records add their cones), so lanes of different kinds side
+// by side in the interleaved walk loop share one fetch per iteration, and all
+// kinds share the stack logic.
+template <bool X>
+DWD bool trav3_iter(Trav& t, Ovf& o) {
     const SubDev& S = *t.S;
-    const bool cp = t.kind == TQ_CLOSEST;
+    const int kind = t.kind;
     if (t.cd <= t.r_max) {
         if (t.ref < 0) {
             int off, cnt;
             leaf_range(S, t.ref, off, cnt);
-            for (int j = 0; j < cnt; ++j) {
-                DTri tr = gload<DTri>(t.prims, off + j);
-                if (cp) {
+            if (kind == TQ_SIL) {
+                for (int j = 0; j < cnt; ++j) {
+                    float d;
                     F3 pt;
-                    float u, v, d;
-                    tri_closest(tr, t.x, pt, u, v, d);
-                    if (d < t.d) {
-                        t.d = d;
-                        t.best = off + j;
+                    if (sile_test(gload<DSilE>(t.prims, off + j), t.x, t.r_max, d, pt)) {
                         t.r_max = fmn(t.r_max, d);
-                    }
-                } else {
-                    float bu, bw, t_;
-                    if (tri_hit(tr, t.x, t.v, t.r_max, bu, bw, t_)) {
-                        t.r_max = t_;
                         t.best = off + j;
+                    }
+                }
+            } else {
+                for (int j = 0; j < cnt; ++j) {
+                    DTri tr = gload<DTri>(t.prims, off + j);
+                    if (kind == TQ_CLOSEST) {
+                        F3 pt;
+                        float u, v, d;
+                        tri_closest(tr, t.x, pt, u, v, d);
+                        if (d < t.d) {
+                            t.d = d;
+                            t.best = off + j;
+                            t.r_max = fmn(t.r_max, d);
+                        }
+                    } else {
+                        float bu, bw, t_;
+                        if (tri_hit(tr, t.x, t.v, t.r_max, bu, bw, t_)) {
+                            t.r_max = t_;
+                            t.best = off + j;
+                        }
                     }
                 }
             }
         } else {
-            RBvh3 rc = gload<RBvh3>(t.rec, t.ref);
+            const bool sn = kind == TQ_SIL;
+            const char* rec = (const char*)t.rec + (size_t)t.ref * (sn ? sizeof(RSnch3) : sizeof(RHead3));
+            RHead3 rc = gload<RHead3>(rec, 0);
+            RSnchTail3 cn;
+            if (sn) cn = gload<RSnchTail3>(rec + sizeof(RHead3), 0);
             F3 lmn = f3(rc.lmnx, rc.lmny, rc.lmnz), lmx = f3(rc.lmxx, rc.lmxy, rc.lmxz);
             F3 rmn = f3(rc.rmnx, rc.rmny, rc.rmnz), rmx = f3(rc.rmxx, rc.rmxy, rc.rmxz);
             float d0, d1;
             bool h0, h1, sw;  // sw: right child first
-            if (cp) {
+            if (sn) {
+                h0 = snch3_child<X>(lmn, lmx, f3(cn.lax, cn.lay, cn.laz), rc.lha, cn.lsin, cn.lcos, cn.lrad, cn.lrho,
+                                    cn.lelo, cn.lehi, t.x, t.r_max, d0);
+                h1 = snch3_child<X>(rmn, rmx, f3(cn.rax, cn.ray, cn.raz), rc.rha, cn.rsin, cn.rcos, cn.rrad, cn.rrho,
+                                    cn.relo, cn.rehi, t.x, t.r_max, d1);
+                sw = d1 < d0;
+            } else if (kind == TQ_CLOSEST) {
                 float dmaxL, dmaxR;
                 box_dist(lmn, lmx, t.x, d0, dmaxL);
                 h0 = d0 <= t.r_max;
@@ -1042,7 +1020,7 @@ DWD CP closest3(const SubDev& S, F3 p) {
     Trav t;
     Ovf o;
     closest3_begin(t, S, p);
-    while (!bvh3_iter(t, o)) {
+    while (!trav3_iter<true>(t, o)) {
     }
     return closest3_end(t);
 }
@@ -1075,7 +1053,7 @@ DWD Hit intersect3(const SubDev& S, F3 x, F3 v, float r_max) {
     Trav t;
     Ovf o;
     if (isect3_begin(t, S, x, v, r_max))
-        while (!bvh3_iter(t, o)) {
+        while (!trav3_iter<true>(t, o)) {
         }
     return isect3_end(t);
 }
@@ -1083,18 +1061,13 @@ DWD Hit intersect3(const SubDev& S, F3 x, F3 v, float r_max) {
 // One traversal iteration of whichever query t holds (the walk kernels' three
 // query kinds); true when it has finished.
 template <int D, bool X>
-DWD bool trav_iter_as(Trav& t, Ovf& o, int kind) {
+DWD bool trav_iter(Trav& t, Ovf& o) {
     if constexpr (D == 2) {
-        if (kind == TQ_CLOSEST) return closest2_iter(t, o);
-        if (kind == TQ_SIL) return sil2_iter<X>(t, o);
+        if (t.kind == TQ_CLOSEST) return closest2_iter(t, o);
+        if (t.kind == TQ_SIL) return sil2_iter<X>(t, o);
         return isect2_iter(t, o);
     } else {
-        if (kind == TQ_SIL) return sil3_iter<X>(t, o);
-        return bvh3_iter(t, o);
+        return trav3_iter<X>(t, o);
     }
-}
-template <int D, bool X>
-DWD bool trav_iter(Trav& t, Ovf& o) {
-    return trav_iter_as<D, X>(t, o, t.kind);
 }
 }  // namespace dwd

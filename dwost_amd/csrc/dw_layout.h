@@ -82,24 +82,29 @@ struct alignas(16) RSnch2 {  // 96 B
     float lsin, lcos, rsin, rcos;
     int ref_l, ref_r, pad0, pad1;
 };
-struct alignas(16) RBvh3 {  // 64 B
+// 3D records share a 64 B head (both children's boxes and references), so one
+// fetch serves every query kind; an SNCH record appends its cones and
+// face-plane bounds.
+struct alignas(16) RHead3 {  // 64 B
     float lmnx, lmny, lmnz, lmxx;
     float lmxy, lmxz, rmnx, rmny;
     float rmnz, rmxx, rmxy, rmxz;
-    int ref_l, ref_r, pad0, pad1;
-};
-struct alignas(16) RSnch3 {  // 144 B
-    float lmnx, lmny, lmnz, lmxx;
-    float lmxy, lmxz, rmnx, rmny;
-    float rmnz, rmxx, rmxy, rmxz;
-    float lax, lay, laz, lha;
-    float rax, ray, raz, rha;
-    float lrad, rrad, lsin, lcos;
-    float rsin, rcos;
     int ref_l, ref_r;
+    float lha, rha;  // SNCH: cone half angles (BVH: 0)
+};
+typedef RHead3 RBvh3;
+struct alignas(16) RSnchTail3 {  // 80 B
+    float lax, lay, laz, lrad;
+    float rax, ray, raz, rrad;
+    float lsin, lcos, rsin, rcos;
     float lrho, lelo, lehi, rrho;  // face-plane bounds (rho, e_lo, e_hi) per child
     float relo, rehi, pad0, pad1;
 };
+struct alignas(16) RSnch3 {  // 144 B
+    RHead3 h;
+    RSnchTail3 c;
+};
+static_assert(sizeof(RHead3) == 64 && sizeof(RSnch3) == 144, "3D record layout");
 
 // Everything one sub-scene exposes to the kernels (device pointers).
 struct SubDev {

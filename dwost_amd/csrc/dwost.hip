@@ -1070,15 +1070,15 @@ int upload_records(dw_scene* s, const dwb::SubHost& H, SubDev& S) {
             RBvh3& b = rb[i];
             std::memcpy(&b, v, sizeof(v));
             b.ref_l = bvh_ref(L), b.ref_r = bvh_ref(R);
-            RSnch3& c = rs[i];
-            std::memcpy(&c, v, sizeof(v));
+            RSnch3& sn = rs[i];
+            std::memcpy(&sn.h, v, sizeof(v));
+            sn.h.ref_l = snch_ref(L), sn.h.ref_r = snch_ref(R);
+            sn.h.lha = H.node_half_angle[L], sn.h.rha = H.node_half_angle[R];
+            RSnchTail3& c = sn.c;
             c.lax = H.node_axis[3 * L], c.lay = H.node_axis[3 * L + 1], c.laz = H.node_axis[3 * L + 2];
-            c.lha = H.node_half_angle[L];
             c.rax = H.node_axis[3 * R], c.ray = H.node_axis[3 * R + 1], c.raz = H.node_axis[3 * R + 2];
-            c.rha = H.node_half_angle[R];
             c.lrad = H.node_radius[L], c.rrad = H.node_radius[R];
             c.lsin = cs(L, true), c.lcos = cs(L, false), c.rsin = cs(R, true), c.rcos = cs(R, false);
-            c.ref_l = snch_ref(L), c.ref_r = snch_ref(R);
             c.lrho = H.node_pb[3 * L], c.lelo = H.node_pb[3 * L + 1], c.lehi = H.node_pb[3 * L + 2];
             c.rrho = H.node_pb[3 * R], c.relo = H.node_pb[3 * R + 1], c.rehi = H.node_pb[3 * R + 2];
         }

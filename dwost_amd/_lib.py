@@ -40,7 +40,8 @@ class SolverParams(C.Structure):
     _fields_ = [("nwalks", C.c_int32), ("nsteps", C.c_int32), ("nsubwalks", C.c_int32),
                 ("control_variates", C.c_int32), ("antithetic_variates", C.c_int32), ("seed", C.c_uint32),
                 ("eps", C.c_float), ("min_R", C.c_float), ("flags", C.c_int32), ("point_base", C.c_int64),
-                ("block_size", C.c_int32), ("blocks_per_cu", C.c_int32), ("spatial_order", C.c_int32)]
+                ("block_size", C.c_int32), ("blocks_per_cu", C.c_int32), ("spatial_order", C.c_int32),
+                ("schedule", C.c_int32), ("gate", C.c_int32)]
 
 SOLVE_EXACT_CONES = 1
 SOLVE_WOS = 2

@@ -1,12 +1,15 @@
 """Launch-shape autotuner for the persistent walk kernels.
 
-The walk kernels take three knobs that never change a result (each walk's
+The walk kernels take four knobs that never change a result (each walk's
 random stream is keyed by its (point, walk) index, not by the lane that runs
 it): threads per block (64 / 128 / 256), a cap on resident blocks per CU (the
-occupancy: fewer waves per SIMD means less L2 pressure per wave) and whether
-the points are sorted by Morton cell and split into one range per XCD.
-``tune`` times every combination on a sample of the workload, checks that all
-of them return the same bits, and stores the fastest on the solver.
+occupancy: fewer waves per SIMD means less L2 pressure per wave), whether the
+points are sorted by Morton cell and split into one range per XCD, and the
+lane scheduling (interleaved traversals or step-synchronous walks; the gate
+threshold of the interleaved loop is left at its default, flat within 2% over
+16-32 on the icosphere).  ``tune`` times every combination on a sample of the
+workload, checks that all of them return the same bits, and stores the fastest
+on the solver.
 """
 
 from __future__ import annotations
@@ -23,6 +26,7 @@ from .solvers import Solver
 BLOCK_SIZES = (64, 128, 256)
 BLOCKS_PER_CU = (0, 1, 2)  # 0: as many as the occupancy allows
 ORDERS = (1, 2)  # spatial order on / off
+SCHEDULES = (1, 2)  # interleaved / step-synchronous
 
 _cache: dict = {}
 
@@ -39,29 +43,29 @@ def tune(solver: Solver, scene: _Scene, points, seed: int = 0, sample: int = 819
     key = _key(solver, scene, p.shape[0])
     if use_cache and key in _cache:
         best = _cache[key]
-        solver.block_size, solver.blocks_per_cu, solver.spatial_order = best["config"]
+        solver.block_size, solver.blocks_per_cu, solver.spatial_order, solver.schedule = best["config"]
         return best
     q = p[:sample].contiguous()
     results, ref = [], None
-    for bs, pc, order in itertools.product(BLOCK_SIZES, BLOCKS_PER_CU, ORDERS):
-        s = replace(solver, block_size=bs, blocks_per_cu=pc, spatial_order=order)
+    for bs, pc, order, sched in itertools.product(BLOCK_SIZES, BLOCKS_PER_CU, ORDERS, SCHEDULES):
+        s = replace(solver, block_size=bs, blocks_per_cu=pc, spatial_order=order, schedule=sched)
         run = (lambda: s.solve_grad(q, scene, seed=seed)["u"]) if fused else (lambda: s.solve(q, scene, seed=seed))
         out = run()  # warm-up (allocations, first launch)
         torch.cuda.synchronize()
         if ref is None:
             ref = out.clone()
         elif not torch.equal(out.view(torch.int32), ref.view(torch.int32)):
-            raise RuntimeError(f"launch shape {(bs, pc, order)} changed the result")
+            raise RuntimeError(f"launch shape {(bs, pc, order, sched)} changed the result")
         times = []
         for _ in range(reps):
             t0 = time.perf_counter()
             run()
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t0)
-        results.append(((bs, pc, order), min(times)))
+        results.append(((bs, pc, order, sched), min(times)))
     results.sort(key=lambda r: r[1])
-    best = dict(config=results[0][0], seconds=results[0][1], seconds_256_full_ordered=dict(results)[(256, 0, 1)],
-                table=results)
-    solver.block_size, solver.blocks_per_cu, solver.spatial_order = best["config"]
+    default = (256, 0, 1, 1 if scene.dim == 3 else 2)
+    best = dict(config=results[0][0], seconds=results[0][1], seconds_default=dict(results)[default], table=results)
+    solver.block_size, solver.blocks_per_cu, solver.spatial_order, solver.schedule = best["config"]
     _cache[key] = best
     return best

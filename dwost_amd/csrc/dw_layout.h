@@ -159,4 +159,5 @@ struct ParamsDev {
     int flags;  // DW_SOLVE_*
     int64_t point_base;  // global index of point 0 (stream keys)
     int block_size, blocks_per_cu;  // launch shape (0: defaults)
+    int schedule, gate;             // lane scheduling of the persistent kernels
 };

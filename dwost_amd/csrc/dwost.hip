@@ -368,10 +368,7 @@ DWD F3 load_pt<3>(const float* pts, int64_t i) {
 // start(job, w, rng): set up the job's first walk (false: nothing to walk).
 // next(w, rng): the walk in w has ended; set up the job's next one (false: the
 // job is complete).  advance(w, rng, t): walk_advance with the job's scene.
-template <int D>
-constexpr bool kInterleave = D == 3;
-
-template <int D, bool X, typename Start, typename Next, typename Adv>
+template <int D, bool X, bool I, typename Start, typename Next, typename Adv>
 DWD void run_jobs(const JobQ& Q, Start& start, Next& next, Adv& advance) {
     Walk<D> w;
     Rng rng;
@@ -380,7 +377,7 @@ DWD void run_jobs(const JobQ& Q, Start& start, Next& next, Adv& advance) {
     bool busy = false, trav = false, done = false;
     QState qs;
     q_start(qs);
-    if constexpr (!kInterleave<D>) {
+    if constexpr (!I) {
         // one whole step per lane and loop iteration, traversals back to back
         while (true) {
             int64_t got = q_next(!done && !busy, Q, qs);
@@ -424,7 +421,7 @@ DWD void run_jobs(const JobQ& Q, Start& start, Next& next, Adv& advance) {
 #define DW_LAMBDA __attribute__((always_inline))
 
 // Solver.solve jobs: job = i*W + k, nsubwalks walks on the job's stream.
-template <int D, bool X>
+template <int D, bool X, bool I>
 __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVES_FAST)
     k_walk_u_pers(const SceneDev* __restrict__ scp, BCDev bc, ParamsDev P, const float* __restrict__ pts, int64_t n,
                   float* walk_u, int* walk_steps, int* walk_term, uint8_t* walk_flags, JobQ Q) {
@@ -465,7 +462,7 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
     auto advance = [&](Walk<D>& w, Rng& rng, Trav& t) DW_LAMBDA -> int {
         return walk_advance<X, false>(sc, bc, P, w, rng, t, Adj{0.0f, nullptr, nullptr});
     };
-    run_jobs<D, X>(Q, start, next, advance);
+    run_jobs<D, X, I>(Q, start, next, advance);
 }
 
 
@@ -527,7 +524,7 @@ DWD int find_item(int n_items, int64_t g, F first) {
 // B = false: one item (kernel argument), jobs from the spatial queue.
 // B = true: many items (device array); the queue runs over the concatenated
 // job space and each lane looks up the item of its job.
-template <int D, bool X, bool B>
+template <int D, bool B, bool X, bool I>
 __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVES_FAST)
     k_fused_pers(FusedItem one, const FusedItem* __restrict__ items, int n_items, JobQ Q) {
     using V = typename VecT<D>::T;
@@ -617,7 +614,7 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
     auto advance = [&](Walk<D>& w, Rng& rng, Trav& t) DW_LAMBDA -> int {
         return walk_advance<X, false>(*ITEM(sc), ITEM(bc), ITEM(P), w, rng, t, Adj{0.0f, nullptr, nullptr});
     };
-    run_jobs<D, X>(Q, start, next, advance);
+    run_jobs<D, X, I>(Q, start, next, advance);
 #undef ITEM
 }
 
@@ -647,7 +644,7 @@ __global__ void __launch_bounds__(kBlock) k_reduce_batch(const FusedItem* __rest
 // / (2 npairs), and scatter-adds dL/dg, dL/dh per vertex.  The walks are linear
 // in the per-vertex boundary data for fixed random numbers; walks zeroed by the
 // outlier guards in the forward pass (pair_flags) are skipped.
-template <int D, bool X>
+template <int D, bool X, bool I>
 __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVES_FAST)
     k_adjoint_pers(const SceneDev* __restrict__ scp, BCDev bc, ParamsDev P, const float* __restrict__ pts,
                    const float* __restrict__ Rp, const int* __restrict__ insidep, int64_t n, int npairs,
@@ -700,12 +697,12 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
     auto advance = [&](Walk<D>& w, Rng& rng, Trav& t) DW_LAMBDA -> int {
         return walk_advance<X, true>(sc, bc, P, w, rng, t, adj);
     };
-    run_jobs<D, X>(Q, start, next, advance);
+    run_jobs<D, X, I>(Q, start, next, advance);
 }
 
 // Adjoint of Solver.solve: job i*W + k replays its nsubwalks walks with weight
 // dL/du[i] / (W * nsubwalks) each (walks zeroed by the guard are skipped).
-template <int D, bool X>
+template <int D, bool X, bool I>
 __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVES_FAST)
     k_adjoint_u_pers(const SceneDev* __restrict__ scp, BCDev bc, ParamsDev P, const float* __restrict__ pts,
                      int64_t n, const float* __restrict__ dl_du, const uint8_t* __restrict__ walk_flags, float* dl_dg,
@@ -739,7 +736,7 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
     auto advance = [&](Walk<D>& w, Rng& rng, Trav& t) DW_LAMBDA -> int {
         return walk_advance<X, true>(sc, bc, P, w, rng, t, adj);
     };
-    run_jobs<D, X>(Q, start, next, advance);
+    run_jobs<D, X, I>(Q, start, next, advance);
 }
 
 
@@ -1250,6 +1247,8 @@ ParamsDev to_params(const dw_solver_params* p) {
     P.point_base = p->point_base;
     P.block_size = p->block_size;
     P.blocks_per_cu = p->blocks_per_cu;
+    P.schedule = p->schedule;
+    P.gate = p->gate;
     return P;
 }
 int check_params(const dw_solver_params* p) {
@@ -1262,6 +1261,8 @@ int check_params(const dw_solver_params* p) {
         return fail(DW_ERR_ARG, "block_size must be 0, 64, 128 or 256");
     if (p->blocks_per_cu < 0) return fail(DW_ERR_ARG, "blocks_per_cu must be >= 0");
     if (p->spatial_order < 0 || p->spatial_order > 2) return fail(DW_ERR_ARG, "spatial_order must be 0, 1 or 2");
+    if (p->schedule < 0 || p->schedule > 2) return fail(DW_ERR_ARG, "schedule must be 0, 1 or 2");
+    if (p->gate < 0 || p->gate > 64) return fail(DW_ERR_ARG, "gate must be in 0..64");
     return DW_OK;
 }
 
@@ -1351,11 +1352,18 @@ int make_queue(const dw_scene* s, StreamCtx& c, const float* pts, int64_t n, int
 struct LaunchCfg {
     int block = kBlock;
     int per_cu = 0;
+    bool interleave = false;  // lane scheduling (DW_SCHED_*)
+    int gate = kGate;
 };
-LaunchCfg launch_cfg(const ParamsDev& P) {
+// DW_SCHED_AUTO: interleaved where traversal lengths diverge most.  On the
+// 81,920-triangle icosphere (3D, 262,144 points) it measured 1.48x the
+// step-synchronous loop, on the 2D holes scene (65,536 points) 0.54x.
+LaunchCfg launch_cfg(const ParamsDev& P, int dim) {
     LaunchCfg c;
     if (P.block_size) c.block = P.block_size;
     c.per_cu = P.blocks_per_cu;
+    c.interleave = P.schedule == DW_SCHED_INTERLEAVED || (P.schedule == DW_SCHED_AUTO && dim == 3);
+    if (P.gate) c.gate = P.gate;
     return c;
 }
 template <typename K>
@@ -1371,19 +1379,25 @@ int persistent_grid(K kernel, const LaunchCfg& cfg, unsigned& blocks) {
     return DW_OK;
 }
 
-// Launches the exact- or fast-cone instance of a persistent kernel on its grid.
-template <typename KE, typename KF, typename... A>
-int launch_pers(bool exact, const LaunchCfg& cfg, KE ke, KF kf, hipStream_t st, A... args) {
+// The four instances of a persistent kernel: exact / fast cones x interleaved /
+// step-synchronous scheduling.
+template <typename K>
+struct Pers {
+    K exact_i, fast_i, exact_s, fast_s;
+};
+#define DW_PERS(k, ...) \
+    Pers<decltype(&k<__VA_ARGS__, true, true>)> { \
+        k<__VA_ARGS__, true, true>, k<__VA_ARGS__, false, true>, k<__VA_ARGS__, true, false>, k<__VA_ARGS__, false, false> \
+    }
+// Launches the instance the cone mode and the schedule select on its grid.
+template <typename K, typename... A>
+int launch_pers(bool exact, const LaunchCfg& cfg, const Pers<K>& ks, hipStream_t st, A... args) {
     unsigned blocks = 0;
     int rc;
     const size_t lds = (size_t)cfg.block * kStackLdsBytesPerThread;
-    if (exact) {
-        if ((rc = persistent_grid(ke, cfg, blocks))) return rc;
-        hipLaunchKernelGGL(ke, dim3(blocks), dim3(cfg.block), lds, st, args...);
-    } else {
-        if ((rc = persistent_grid(kf, cfg, blocks))) return rc;
-        hipLaunchKernelGGL(kf, dim3(blocks), dim3(cfg.block), lds, st, args...);
-    }
+    K k = cfg.interleave ? (exact ? ks.exact_i : ks.fast_i) : (exact ? ks.exact_s : ks.fast_s);
+    if ((rc = persistent_grid(k, cfg, blocks))) return rc;
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(cfg.block), lds, st, args...);
     DW_HIP(hipGetLastError());
     return DW_OK;
 }
@@ -1450,15 +1464,15 @@ int batch_launch(dw_batch* b, hipStream_t st) {
     Q.order = nullptr;
     Q.n = b->total_jobs;
     Q.per = 1;
-    Q.gate = kGate;
+    Q.gate = b->cfg.gate;
     FusedItem none;
     std::memset(&none, 0, sizeof(none));
     const FusedItem* items = b->dev_items;
     if (b->dim == 2)
-        rc = launch_pers(b->exact, b->cfg, k_fused_pers<2, true, true>, k_fused_pers<2, false, true>, st, none, items,
+        rc = launch_pers(b->exact, b->cfg, DW_PERS(k_fused_pers, 2, true), st, none, items,
                          b->n_items, Q);
     else
-        rc = launch_pers(b->exact, b->cfg, k_fused_pers<3, true, true>, k_fused_pers<3, false, true>, st, none, items,
+        rc = launch_pers(b->exact, b->cfg, DW_PERS(k_fused_pers, 3, true), st, none, items,
                          b->n_items, Q);
     if (rc) return rc;
     if (b->total_steps && (rc = zero_u64(b->total_steps, st))) return rc;
@@ -1652,12 +1666,14 @@ int dw_solve_u(const dw_scene* s, const float* pts, int64_t n, const dw_solver_p
     } else {
         JobQ Q;
         if ((rc = make_queue(s, *c, pts, n, params->nwalks, obytes ? cur : nullptr, st, Q))) return rc;
+        const LaunchCfg cfg = launch_cfg(to_params(params), s->dev.dim);
+        Q.gate = cfg.gate;
         const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
         if (s->dev.dim == 2)
-            rc = launch_pers(exact, launch_cfg(to_params(params)), k_walk_u_pers<2, true>, k_walk_u_pers<2, false>, st, s->dev_ptr, to_bc(boundary),
+            rc = launch_pers(exact, cfg, DW_PERS(k_walk_u_pers, 2), st, s->dev_ptr, to_bc(boundary),
                              to_params(params), pts, n, wv, ws, walk_term, walk_flags, Q);
         else
-            rc = launch_pers(exact, launch_cfg(to_params(params)), k_walk_u_pers<3, true>, k_walk_u_pers<3, false>, st, s->dev_ptr, to_bc(boundary),
+            rc = launch_pers(exact, cfg, DW_PERS(k_walk_u_pers, 3), st, s->dev_ptr, to_bc(boundary),
                              to_params(params), pts, n, wv, ws, walk_term, walk_flags, Q);
         if (rc) return rc;
     }
@@ -1731,6 +1747,8 @@ int dw_solve_fused(const dw_scene* s, const float* pts, int64_t n, const dw_solv
     } else {
         JobQ Q;
         if ((rc = make_queue(s, *c, pts, n, npairs, obytes ? cur : nullptr, st, Q))) return rc;
+        const LaunchCfg cfg = launch_cfg(to_params(params), s->dev.dim);
+        Q.gate = cfg.gate;
         const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
         FusedItem one;
         std::memset(&one, 0, sizeof(one));
@@ -1747,9 +1765,9 @@ int dw_solve_fused(const dw_scene* s, const float* pts, int64_t n, const dw_solv
         one.npairs = npairs;
         const FusedItem* none = nullptr;
         if (D == 2)
-            rc = launch_pers(exact, launch_cfg(to_params(params)), k_fused_pers<2, true, false>, k_fused_pers<2, false, false>, st, one, none, 1, Q);
+            rc = launch_pers(exact, cfg, DW_PERS(k_fused_pers, 2, false), st, one, none, 1, Q);
         else
-            rc = launch_pers(exact, launch_cfg(to_params(params)), k_fused_pers<3, true, false>, k_fused_pers<3, false, false>, st, one, none, 1, Q);
+            rc = launch_pers(exact, cfg, DW_PERS(k_fused_pers, 3, false), st, one, none, 1, Q);
         if (rc) return rc;
     }
     if (total_steps && (rc = zero_u64((unsigned long long*)total_steps, st))) return rc;
@@ -1787,11 +1805,13 @@ int dw_adjoint_fused(const dw_scene* s, const float* pts, int64_t n, const dw_so
     DW_HIP(hipGetLastError());
     JobQ Q;
     if ((rc = make_queue(s, *c, pts, n, npairs, obytes ? (char*)p + (size_t)n * 8 : nullptr, st, Q))) return rc;
+    const LaunchCfg cfg = launch_cfg(to_params(params), s->dev.dim);
+    Q.gate = cfg.gate;
     const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
     if (D == 2)
-        return launch_pers(exact, launch_cfg(to_params(params)), k_adjoint_pers<2, true>, k_adjoint_pers<2, false>, st, s->dev_ptr, to_bc(boundary),
+        return launch_pers(exact, cfg, DW_PERS(k_adjoint_pers, 2), st, s->dev_ptr, to_bc(boundary),
                            to_params(params), pts, R, ins, n, npairs, dl_du, dl_dgrad, pair_flags, dl_dg, dl_dh, Q);
-    return launch_pers(exact, launch_cfg(to_params(params)), k_adjoint_pers<3, true>, k_adjoint_pers<3, false>, st, s->dev_ptr, to_bc(boundary),
+    return launch_pers(exact, cfg, DW_PERS(k_adjoint_pers, 3), st, s->dev_ptr, to_bc(boundary),
                        to_params(params), pts, R, ins, n, npairs, dl_du, dl_dgrad, pair_flags, dl_dg, dl_dh, Q);
 }
 
@@ -1814,11 +1834,13 @@ int dw_adjoint_u(const dw_scene* s, const float* pts, int64_t n, const dw_solver
     if (obytes && (rc = workspace(*c, obytes, st, &p))) return rc;
     JobQ Q;
     if ((rc = make_queue(s, *c, pts, n, params->nwalks, p, st, Q))) return rc;
+    const LaunchCfg cfg = launch_cfg(to_params(params), s->dev.dim);
+    Q.gate = cfg.gate;
     const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
     if (s->dev.dim == 2)
-        return launch_pers(exact, launch_cfg(to_params(params)), k_adjoint_u_pers<2, true>, k_adjoint_u_pers<2, false>, st, s->dev_ptr,
+        return launch_pers(exact, cfg, DW_PERS(k_adjoint_u_pers, 2), st, s->dev_ptr,
                            to_bc(boundary), to_params(params), pts, n, dl_du, walk_flags, dl_dg, dl_dh, Q);
-    return launch_pers(exact, launch_cfg(to_params(params)), k_adjoint_u_pers<3, true>, k_adjoint_u_pers<3, false>, st, s->dev_ptr, to_bc(boundary),
+    return launch_pers(exact, cfg, DW_PERS(k_adjoint_u_pers, 3), st, s->dev_ptr, to_bc(boundary),
                        to_params(params), pts, n, dl_du, walk_flags, dl_dg, dl_dh, Q);
 }
 
@@ -1857,7 +1879,7 @@ int dw_batch_create(const dw_batch_item* items, int32_t n_items, uint64_t* total
     b->dim = dim;
     b->exact = exact != 0;
     b->use_graph = (flags & DW_BATCH_GRAPH) != 0;
-    b->cfg = launch_cfg(to_params(&items[0].params));
+    b->cfg = launch_cfg(to_params(&items[0].params), b->dim);
     b->n_items = (int)order.size();
     b->total_steps = (unsigned long long*)total_steps;
     size_t bytes = 0;

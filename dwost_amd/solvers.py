@@ -46,6 +46,8 @@ class Solver:
     block_size: int = 0
     blocks_per_cu: int = 0
     spatial_order: int = 0  # 0 auto, 1 on, 2 off
+    schedule: int = 0  # lane scheduling: 0 auto, 1 interleaved, 2 step-synchronous
+    gate: int = 0  # interleaved schedule: waiting lanes per walk-logic pass (0: 24)
 
     _walk_flag: ClassVar[int] = 0  # walk family (WoSt; WoS sets DW_SOLVE_WOS)
 
@@ -55,7 +57,7 @@ class Solver:
                                  int(self.antithetic_variates), int(seed) & 0xFFFFFFFF, float(self.eps),
                                  float(self.min_R), flags,
                                  int(point_base), int(self.block_size), int(self.blocks_per_cu),
-                                 int(self.spatial_order))
+                                 int(self.spatial_order), int(self.schedule), int(self.gate))
 
     def _check(self, scene: _Scene):
         if scene.dim != self.dim:

@@ -93,11 +93,24 @@ typedef struct dw_solver_params { /* Solver / WoSt fields (solver.py:30-34, wost
     int32_t blocks_per_cu;  /* cap on resident blocks per CU (0: occupancy limit)  */
     int32_t spatial_order;  /* DW_ORDER_AUTO / _ON / _OFF: sort points by Morton
                                cell and give each XCD a contiguous range         */
+    int32_t schedule;       /* DW_SCHED_*: lane scheduling of the walk kernels     */
+    int32_t gate;           /* interleaved schedule: waiting lanes of a wave that
+                               trigger a walk-logic pass, 1..64 (0: 24)           */
 } dw_solver_params;
 
 #define DW_ORDER_AUTO 0 /* on from 8192 points */
 #define DW_ORDER_ON 1
 #define DW_ORDER_OFF 2
+
+/* Lane scheduling of the persistent walk kernels (results never depend on it).
+ * STEP: every lane advances its walk by one whole step per loop iteration, the
+ * step's traversals back to back.  INTERLEAVED: lanes run traversal iterations
+ * of whatever query each has in flight side by side and run the walk logic
+ * between queries in gated passes, so no lane waits for a wave's longest
+ * traversal.  AUTO: interleaved in 3D, step-synchronous in 2D. */
+#define DW_SCHED_AUTO 0
+#define DW_SCHED_INTERLEAVED 1
+#define DW_SCHED_STEP 2
 
 /* Evaluate the SNCH cone test literally (acos/asin/atan2 per child, snch.py:46-97)
  * instead of the transcendental-free form |cos| <= sin(half angle + view angle)

@@ -1,0 +1,90 @@
+"""Lane scheduling of the persistent walk kernels never changes a result: the
+interleaved and the step-synchronous loops (and any gate threshold) return the
+same bits for u, grad u and per-walk values in 2D and 3D, and the same adjoint
+up to the order of its atomic scatter-adds.  Each
+schedule is also checked against the oracle walk by walk in the parity tests
+(AUTO picks interleaved in 3D, step-synchronous in 2D); here the opposite
+schedule of each dimension is compared with the default."""
+
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import dwost_amd
+
+    dwost_amd._lib.lib()
+    return dwost_amd
+
+
+def _bits(t):
+    return t.detach().cpu().numpy().view(np.uint32)
+
+
+def _case(gpu, dim):
+    from dwost_amd import scenes
+
+    if dim == 2:
+        spec = scenes.holes_scene_2d(n_query=400, eps=1e-3)
+        sc = gpu.Scene2D(spec.vertices, spec.indices, spec.types, gpu.BoundaryCondition(spec.bc_code))
+        sol = gpu.WoSt(eps=1e-3, nwalks=64, nsteps=1024)
+    else:
+        spec = scenes.icosphere_scene(level=3, n_query=300, eps=1e-3)
+        sc = gpu.Scene3D(spec.vertices, spec.indices, spec.types, gpu.BoundaryCondition(spec.bc_code))
+        sol = gpu.WoSt3D(eps=1e-3, nwalks=32, nsteps=1024)
+    return sc, sol, torch.as_tensor(spec.queries).cuda()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_schedules_agree(gpu, dim):
+    sc, sol, q = _case(gpu, dim)
+    ref_g = sol.solve_grad(q, sc, seed=7, per_pair=True, count_steps=True)
+    ref_u = sol.solve(q, sc, seed=7, per_walk=True)
+    for sched, gate in ((1, 0), (2, 0), (1, 1), (1, 64)):
+        s = replace(sol, schedule=sched, gate=gate)
+        g = s.solve_grad(q, sc, seed=7, per_pair=True, count_steps=True)
+        u = s.solve(q, sc, seed=7, per_walk=True)
+        for k in ("u", "grad", "pair_vals", "pair_steps"):
+            assert np.array_equal(_bits(g[k]), _bits(ref_g[k])), (sched, gate, k)
+        assert int(g["steps"][0]) == int(ref_g["steps"][0])
+        for k in ("u", "walk_u", "walk_steps", "walk_term"):
+            assert np.array_equal(_bits(u[k]), _bits(ref_u[k])), (sched, gate, k)
+
+
+def test_adjoint_schedules_agree(gpu):
+    from dwost_amd import scenes
+
+    prob = scenes.adjoint_problem_3d(level=3, n_query=128)
+    spec = prob.spec
+    q = torch.as_tensor(spec.queries).cuda()
+    ut, gt = torch.as_tensor(prob.u_target).cuda(), torch.as_tensor(prob.grad_target).cuda()
+    grads = []
+    for sched in (1, 2):
+        g = torch.as_tensor(prob.g0).cuda().requires_grad_(True)
+        h = torch.as_tensor(prob.h0).cuda().requires_grad_(True)
+        scene = gpu.Scene3D(spec.vertices, spec.indices, spec.types, gpu.BoundaryCondition.per_vertex(g, h))
+        cs = gpu.CustomSolver(gpu.WoSt3D(eps=1e-3, nwalks=32, nsteps=1024, schedule=sched))
+        u, gr = cs.solve_grad(q, scene, seed=11)
+        (torch.mean((u - ut) ** 2) + torch.mean(torch.sum((gr - gt) ** 2, 1))).backward()
+        grads.append((g.grad.cpu().numpy(), h.grad.cpu().numpy(), u.detach().cpu().numpy()))
+    (g1, h1, u1), (g2, h2, u2) = grads
+    assert np.array_equal(u1.view(np.uint32), u2.view(np.uint32))
+    for a, b in ((g1, g2), (h1, h2)):
+        assert np.allclose(a, b, rtol=1e-4, atol=1e-6 * float(np.abs(a).max()))
+
+
+def test_schedule_rejects_bad_values(gpu):
+    sc, sol, q = _case(gpu, 2)
+    with pytest.raises(Exception):
+        replace(sol, schedule=3).solve(q[:4], sc)
+    with pytest.raises(Exception):
+        replace(sol, gate=65).solve(q[:4], sc)

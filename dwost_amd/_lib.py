@@ -76,6 +76,8 @@ def lib():
     L.dw_star_radius.argtypes = [vp, i32, vp, vp, i64, vp, vp]
     L.dw_solve_u.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp, vp, vp, vp, vp]
     L.dw_solve_dude.argtypes = [vp, vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp]
+    L.dw_trace_step.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), C.c_int32, C.c_float, vp,
+                                vp, vp]
     L.dw_solve_fused.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp, vp, vp, vp,
                                  vp]
     L.dw_batch_create.argtypes = [C.POINTER(BatchItem), i32, vp, i32, C.POINTER(vp)]
@@ -87,6 +89,7 @@ def lib():
                                    vp]
     for name in ("dw_scene_create", "dw_scene_destroy", "dw_scene_info", "dw_scene_export", "dw_closest_point",
                  "dw_intersect", "dw_closest_silhouette", "dw_star_radius", "dw_solve_u", "dw_solve_dude",
+                 "dw_trace_step",
                  "dw_solve_fused", "dw_adjoint_u", "dw_adjoint_fused", "dw_batch_create", "dw_batch_solve"):
         getattr(L, name).restype = C.c_int
     _lib = L

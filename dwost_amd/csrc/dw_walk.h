@@ -162,21 +162,25 @@ DWD F3 uniform_sphere(Rng& rng) {  // math.py:348-353
     return f3(r * cr_cos(theta), r * cr_sin(theta), z);
 }
 
-// sdf / inside on the full scene (polyline.py:188-234, mesh.py:290-297)
-DWD bool inside2(const SceneDev& sc, F2 x) {
+// Polyline.sdf / Mesh.sdf and inside (polyline.py:188-234, mesh.py:290-297) on
+// the full scene: distance signed by the side of the interpolated vertex normal
+// (2D) or the face normal (3D), negative inside; inside is sdf < 0.
+DWD float sdf2(const SceneDev& sc, F2 x) {
     const SubDev& F = sc.sub[0];
     CP c = closest2(F, x);
     const float2* vn = (const float2*)F.vnormal;
     float2 a = vn[F.local_vidx[2 * c.prim]], b = vn[F.local_vidx[2 * c.prim + 1]];
     F2 n = lerp(f2(a.x, a.y), f2(b.x, b.y), c.t);
     float s = dot(n, x - f2(c.p.x, c.p.y)) >= 0.0f ? 1.0f : -1.0f;
-    return c.d * s < 0.0f;
+    return c.d * s;
 }
-DWD bool inside3(const SceneDev& sc, F3 x) {
+DWD float sdf3(const SceneDev& sc, F3 x) {
     CP c = closest3(sc.sub[0], x);
     float s = dot(c.n, x - c.p) >= 0.0f ? 1.0f : -1.0f;
-    return c.d * s < 0.0f;
+    return c.d * s;
 }
+DWD bool inside2(const SceneDev& sc, F2 x) { return sdf2(sc, x) < 0.0f; }
+DWD bool inside3(const SceneDev& sc, F3 x) { return sdf3(sc, x) < 0.0f; }
 
 // --------------------------------------------------------------------------
 // WoSt.u (wost.py:145-221) / WoSt3D.u (wost_3d.py:43-106) and WoS.u (wos.py:16-36)

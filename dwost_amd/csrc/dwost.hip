@@ -805,6 +805,95 @@ __global__ void __launch_bounds__(kBlock) k_fused(SceneDev sc, BCDev bc, ParamsD
     pair_steps[lane] = st1 + st2;
 }
 
+// WoSt.dude (wost.py:30-102) / WoSt3D.dude (wost_3d.py:108-175) at one point
+// for one direction e: nwalks walks (antithetic pairs) from the first ball of
+// radius R on the caller's stream; returns the sum, the caller divides by nwalks.
+DWD float dude2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, F2 p, F2 ev, float R, bool active, Rng& rng,
+                unsigned long long& steps) {
+    float result = 0.0f;
+    int wi = 0, st, tm;
+    float u_sum = 0.0f, u_count = 0.0f, src_sum = 0.0f, src_count = 0.0f;
+    float gnorm = R * R / 4.0f;
+    while ((wi < P.nwalks) && active) {
+        F2 dir = uniform_dir2(rng);
+        F2 dir_src = uniform_dir2(rng);
+        F2 p_src;
+        float r_src, pdf_src;
+        greens2_sample(p, R, dir_src, rng, p_src, r_src, pdf_src);
+        int niters = P.av ? 2 : 1;
+        uint64_t state = rng.state;
+        int it = 0;
+        while ((it < niters) && (wi < P.nwalks)) {
+            if (it == 1) {
+                dir_src = -dir_src;
+                p_src = p * 2.0f - p_src;
+            }
+            float src = gnorm * src_f(bc);
+            float src_grad = dot(greens2_gradient(p, R, r_src, p_src), ev) / (pdf_src * gnorm);
+            src_sum += src;
+            src_count += 1.0f;
+            float _src = P.cv ? src - (src_sum / src_count) : src;
+            result += _src * src_grad;
+            if (it == 1) dir = -dir;
+            F2 _p = p + dir * R;
+            rng.state = state;
+            float u = walk_u2(sc, bc, P, _p, rng, st, tm);
+            steps += st;
+            u_sum += u;
+            u_count += 1.0f;
+            float _u = P.cv ? u - u_sum / u_count : u;
+            float r2 = 2.0f / R * _u * dot(ev, dir);
+            if ((__builtin_fabsf(r2) > 1e6f) || bad(r2)) r2 = 0.0f;
+            result += r2;
+            wi += 1;
+            it += 1;
+        }
+    }
+    return result;
+}
+DWD float dude3(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, F3 p, F3 ev, float R, bool active, Rng& rng,
+                unsigned long long& steps) {
+    float result = 0.0f;
+    int wi = 0, st, tm;
+    float u_mean = 0.0f, u_count = 0.0f, src_mean = 0.0f, src_count = 0.0f;
+    float gnorm = R * R / 6.0f;
+    while ((wi < P.nwalks) && active) {
+        F3 dir = uniform_sphere(rng);
+        F3 p_src;
+        float r_src, pdf_src;
+        greens3_sample(p, R, dir, rng, p_src, r_src, pdf_src);
+        int niters = P.av ? 2 : 1;
+        uint64_t state = rng.state;
+        int it = 0;
+        while ((it < niters) && (wi < P.nwalks)) {
+            if (it == 1) {
+                dir = -dir;
+                p_src = p * 2.0f - p_src;
+            }
+            float src = gnorm * src_f(bc);
+            float src_grad = dot(greens3_gradient(p, R, r_src, p_src), ev) / (pdf_src * gnorm);
+            src_mean = src_mean * (src_count / (src_count + 1.0f)) + src / (src_count + 1.0f);
+            src_count += 1.0f;
+            float _src = P.cv ? src - src_mean : src;
+            result += _src * src_grad;
+            if (it == 1) dir = -dir;  // second flip (wost_3d.py:152-153)
+            F3 _p = p + dir * R;
+            rng.state = state;
+            float u = walk_u3(sc, bc, P, _p, rng, st, tm);
+            steps += st;
+            if (bad(u) || __builtin_fabsf(u) > 10000.0f) u = 0.0f;
+            u_mean = u_mean * (u_count / (u_count + 1.0f)) + u / (u_count + 1.0f);
+            u_count += 1.0f;
+            float _u = P.cv ? u - u_mean : u;
+            result += 3.0f / R * _u * dot(ev, dir);
+            wi += 1;
+            it += 1;
+        }
+    }
+    if (bad(result) || __builtin_fabsf(result) > 10000.0f) result = 0.0f;
+    return result;
+}
+
 // WoSt.solve_dude / WoSt3D.solve_dude (wost.py:21-102, wost_3d.py:108-175):
 // one PCG32 stream per point, walks run serially inside the lane.
 __global__ void __launch_bounds__(kBlock) k_dude(SceneDev sc, BCDev bc, ParamsDev P, const float* __restrict__ pts,
@@ -876,85 +965,85 @@ __global__ void __launch_bounds__(kBlock) k_dude(SceneDev sc, BCDev bc, ParamsDe
             }
         }
     } else if (sc.dim == 2) {
-        F2 p = f2(pts[2 * i], pts[2 * i + 1]), ev = f2(e[2 * i], e[2 * i + 1]);
-        float u_sum = 0.0f, u_count = 0.0f, src_sum = 0.0f, src_count = 0.0f;
-        float gnorm = R * R / 4.0f;
-        while ((wi < P.nwalks) && active) {
-            F2 dir = uniform_dir2(rng);
-            F2 dir_src = uniform_dir2(rng);
-            F2 p_src;
-            float r_src, pdf_src;
-            greens2_sample(p, R, dir_src, rng, p_src, r_src, pdf_src);
-            int niters = P.av ? 2 : 1;
-            uint64_t state = rng.state;
-            int it = 0;
-            while ((it < niters) && (wi < P.nwalks)) {
-                if (it == 1) {
-                    dir_src = -dir_src;
-                    p_src = p * 2.0f - p_src;
-                }
-                float src = gnorm * src_f(bc);
-                float src_grad = dot(greens2_gradient(p, R, r_src, p_src), ev) / (pdf_src * gnorm);
-                src_sum += src;
-                src_count += 1.0f;
-                float _src = P.cv ? src - (src_sum / src_count) : src;
-                result += _src * src_grad;
-                if (it == 1) dir = -dir;
-                F2 _p = p + dir * R;
-                rng.state = state;
-                float u = walk_u2(sc, bc, P, _p, rng, st, tm);
-                steps += st;
-                u_sum += u;
-                u_count += 1.0f;
-                float _u = P.cv ? u - u_sum / u_count : u;
-                float r2 = 2.0f / R * _u * dot(ev, dir);
-                if ((__builtin_fabsf(r2) > 1e6f) || bad(r2)) r2 = 0.0f;
-                result += r2;
-                wi += 1;
-                it += 1;
-            }
-        }
+        result = dude2(sc, bc, P, f2(pts[2 * i], pts[2 * i + 1]), f2(e[2 * i], e[2 * i + 1]), R, active, rng, steps);
     } else {
-        F3 p = f3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]), ev = f3(e[3 * i], e[3 * i + 1], e[3 * i + 2]);
-        float u_mean = 0.0f, u_count = 0.0f, src_mean = 0.0f, src_count = 0.0f;
-        float gnorm = R * R / 6.0f;
-        while ((wi < P.nwalks) && active) {
-            F3 dir = uniform_sphere(rng);
-            F3 p_src;
-            float r_src, pdf_src;
-            greens3_sample(p, R, dir, rng, p_src, r_src, pdf_src);
-            int niters = P.av ? 2 : 1;
-            uint64_t state = rng.state;
-            int it = 0;
-            while ((it < niters) && (wi < P.nwalks)) {
-                if (it == 1) {
-                    dir = -dir;
-                    p_src = p * 2.0f - p_src;
-                }
-                float src = gnorm * src_f(bc);
-                float src_grad = dot(greens3_gradient(p, R, r_src, p_src), ev) / (pdf_src * gnorm);
-                src_mean = src_mean * (src_count / (src_count + 1.0f)) + src / (src_count + 1.0f);
-                src_count += 1.0f;
-                float _src = P.cv ? src - src_mean : src;
-                result += _src * src_grad;
-                if (it == 1) dir = -dir;  // second flip (wost_3d.py:152-153)
-                F3 _p = p + dir * R;
-                rng.state = state;
-                float u = walk_u3(sc, bc, P, _p, rng, st, tm);
-                steps += st;
-                if (bad(u) || __builtin_fabsf(u) > 10000.0f) u = 0.0f;
-                u_mean = u_mean * (u_count / (u_count + 1.0f)) + u / (u_count + 1.0f);
-                u_count += 1.0f;
-                float _u = P.cv ? u - u_mean : u;
-                result += 3.0f / R * _u * dot(ev, dir);
-                wi += 1;
-                it += 1;
-            }
-        }
-        if (bad(result) || __builtin_fabsf(result) > 10000.0f) result = 0.0f;
+        result = dude3(sc, bc, P, f3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]), f3(e[3 * i], e[3 * i + 1], e[3 * i + 2]),
+                       R, active, rng, steps);
     }
     out[i] = result / (float)P.nwalks;
     if (step_total) atomicAdd(step_total, steps);
+}
+
+// StreamlineTracer.next_position (examples/example_2d/trace.py:19-68,
+// examples/example_3d/trace.py:21-60) with a WoSt / WoSt3D solver: sample
+// j = i * tn + k of point i runs on the PCG32 stream of lane j (TEA(seed, i)),
+// and the gradient components are Solver.walk_dude (solver.py:148-166) along
+// the axes, one after the other on that stream.  2D steps only from inside
+// (sdf <= 0); when the stepped point is inside too it keeps half the step (the
+// reference averages fy with fy_tilde = current_p - y_tilde, which is zero
+// there).  3D takes the full step.  samples[j] = the sample's new position.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_trace(SceneDev sc, BCDev bc, ParamsDev P, const float* __restrict__ pts,
+                                                  int64_t n, int tn, float step, float* samples,
+                                                  unsigned long long* step_total) {
+    using V = typename VecT<D>::T;
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n * tn) return;
+    int64_t i = j / tn;
+    Rng rng;
+    seed_stream(rng, P, i, tn, j - i * tn);
+    unsigned long long steps = 0;
+    V p = load_pt<D>(pts, i);
+    auto walk_dude = [&](V ev) -> float {  // nsubwalks WoSt.dude estimates, each / nwalks
+        float r = 0.0f;
+        if constexpr (D == 2) {
+            float R = __builtin_fabsf(closest2(sc.sub[0], p).d);
+            bool active = inside2(sc, p);
+            for (int s = 0; s < P.nsubwalks; ++s)
+                r += dude2(sc, bc, P, p, ev, R, active, rng, steps) / (float)P.nwalks;
+        } else {
+            float R = __builtin_fabsf(closest3(sc.sub[0], p).d);
+            bool active = inside3(sc, p);
+            for (int s = 0; s < P.nsubwalks; ++s)
+                r += dude3(sc, bc, P, p, ev, R, active, rng, steps) / (float)P.nwalks;
+        }
+        return r / (float)P.nsubwalks;
+    };
+    V cur = p;
+    if constexpr (D == 2) {
+        if (!(sdf2(sc, p) > 0.0f)) {
+            float gx = walk_dude(f2(1.0f, 0.0f));
+            float gy = walk_dude(f2(0.0f, 1.0f));
+            cur = p + f2(gx, gy) * step;
+            F2 yt = cur;
+            if (!(sdf2(sc, cur) > 0.0f)) {
+                F2 fy = yt - p, fyt = cur - yt;
+                cur = p + (fy + fyt) * 0.5f;
+            }
+        }
+        samples[2 * j] = cur.x;
+        samples[2 * j + 1] = cur.y;
+    } else {
+        float gx = walk_dude(f3(1.0f, 0.0f, 0.0f));
+        float gy = walk_dude(f3(0.0f, 1.0f, 0.0f));
+        float gz = walk_dude(f3(0.0f, 0.0f, 1.0f));
+        cur = p + f3(gx, gy, gz) * step;
+        samples[3 * j] = cur.x;
+        samples[3 * j + 1] = cur.y;
+        samples[3 * j + 2] = cur.z;
+    }
+    if (step_total) atomicAdd(step_total, steps);
+}
+// Mean of each point's samples, summed in sample order.
+__global__ void __launch_bounds__(kBlock) k_trace_mean(const float* __restrict__ samples, int64_t n, int tn, int dim,
+                                                       float* out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int d = 0; d < dim; ++d) {
+        float s = 0.0f;
+        for (int k = 0; k < tn; ++k) s += samples[((int64_t)i * tn + k) * dim + d];
+        out[(int64_t)i * dim + d] = tn > 1 ? s / (float)tn : s;
+    }
 }
 
 }  // namespace
@@ -1704,6 +1793,40 @@ int dw_solve_dude(const dw_scene* s, const float* pts, const float* e, int64_t n
     if (total_steps && (rc = zero_u64((unsigned long long*)total_steps, st))) return rc;
     hipLaunchKernelGGL(k_dude, dim3(blocks_for(n)), dim3(kBlock), kStackLds, st, s->dev, to_bc(boundary), to_params(params),
                        pts, e, R, ins, n, out, (unsigned long long*)total_steps);
+    DW_HIP(hipGetLastError());
+    return DW_OK;
+}
+
+int dw_trace_step(const dw_scene* s, const float* pts, int64_t n, const dw_solver_params* params,
+                  const dw_boundary* boundary, int32_t samples_per_point, float step_size, float* out,
+                  uint64_t* total_steps, void* stream) {
+    int rc;
+    if ((rc = check_scene(s, 0))) return rc;
+    if ((rc = check_params(params)) || (rc = check_bc(boundary))) return rc;
+    if (n < 0 || (n > 0 && (!pts || !out))) return fail(DW_ERR_ARG, "bad arrays");
+    if (samples_per_point < 1) return fail(DW_ERR_ARG, "samples_per_point must be >= 1");
+    if (params->flags & DW_SOLVE_WOS) return fail(DW_ERR_ARG, "the tracer runs WoSt walks only");
+    if (n == 0) return DW_OK;
+    const int D = s->dev.dim;
+    const int64_t ns = n * (int64_t)samples_per_point;
+    hipStream_t st = (hipStream_t)stream;
+    StreamCtx* c;
+    if ((rc = stream_ctx(st, &c))) return rc;
+    void* p;
+    if ((rc = workspace(*c, (size_t)ns * D * 4, st, &p))) return rc;
+    float* samples = (float*)p;
+    if (total_steps && (rc = zero_u64((unsigned long long*)total_steps, st))) return rc;
+    if (D == 2)
+        hipLaunchKernelGGL(k_trace<2>, dim3(blocks_for(ns)), dim3(kBlock), kStackLds, st, s->dev, to_bc(boundary),
+                           to_params(params), pts, n, (int)samples_per_point, step_size, samples,
+                           (unsigned long long*)total_steps);
+    else
+        hipLaunchKernelGGL(k_trace<3>, dim3(blocks_for(ns)), dim3(kBlock), kStackLds, st, s->dev, to_bc(boundary),
+                           to_params(params), pts, n, (int)samples_per_point, step_size, samples,
+                           (unsigned long long*)total_steps);
+    DW_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_trace_mean, dim3(blocks_for(n)), dim3(kBlock), 0, st, samples, n, (int)samples_per_point, D,
+                       out);
     DW_HIP(hipGetLastError());
     return DW_OK;
 }

@@ -188,6 +188,16 @@ int dw_adjoint_u(const dw_scene* scene, const float* pts, int64_t n, const dw_so
 int dw_solve_dude(const dw_scene* scene, const float* pts, const float* e, int64_t n, const dw_solver_params* params,
                   const dw_boundary* boundary, float* out, uint64_t* total_steps, void* stream);
 
+/* StreamlineTracer.next_position (examples/example_2d/trace.py:19-68,
+ * examples/example_3d/trace.py:21-60) with a WoSt / WoSt3D solver
+ * (Solver.walk_dude, solver.py:148-166, along each axis): out[n*dim] = the mean
+ * of samples_per_point streamline steps of length step_size from pts; sample j
+ * of point i runs on the stream of lane i * samples_per_point + j, seeded with
+ * params->seed (the iteration index in the reference example). */
+int dw_trace_step(const dw_scene* scene, const float* pts, int64_t n, const dw_solver_params* params,
+                  const dw_boundary* boundary, int32_t samples_per_point, float step_size, float* out,
+                  uint64_t* total_steps, void* stream);
+
 /* Fused u + grad u (the throughput form; no single reference counterpart):
  * per antithetic pair, the WoSt.dude first-ball step (wost.py:51-99) yields
  * both u(p) (mean value over the ball) and grad u.  u[n], grad[n][dim];

@@ -48,6 +48,8 @@ def lib():
         L.orc_solve_u.argtypes = [C.c_void_p, C.c_void_p, C.c_int64] + common + [C.c_void_p] * 5
         L.orc_solve_dude.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64] + common + [C.c_void_p] * 2
         L.orc_solve_fused.argtypes = [C.c_void_p, C.c_void_p, C.c_int64] + common + [C.c_void_p] * 6
+        L.orc_trace_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int64] + common + [C.c_int32, C.c_float] + [
+            C.c_void_p] * 3
         L.orc_tea.argtypes = [C.c_uint32, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p]
         L.orc_pcg_seq.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_void_p]
         L.orc_pcg_lane_floats.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_void_p]
@@ -191,6 +193,22 @@ class OracleScene:
         lib().orc_solve_dude(self.h, _p(pts), _p(e), n, _p(ip), _p(fp), bc_code, _p(c), _p(gv), _p(hv), src_code,
                              src_c, _p(out), _p(st))
         return dict(dude=out, steps=int(st[0]))
+
+    def trace_step(self, pts, bc_code, nwalks, samples_per_point, step_size, nsubwalks=1, nsteps=128, eps=1e-3,
+                   min_R=1e-3, cv=True, av=True, seed=0, coeffs=None, gv=None, hv=None, src_code=0, src_c=0.0, base=0):
+        """StreamlineTracer.next_position with a WoSt solver of nwalks walks per
+        dude and nsubwalks dudes per walk_dude; returns the mean new positions
+        and the per-sample ones."""
+        pts = _f32(pts)
+        n, D = len(pts), self.dim
+        ip, fp = self._params(nwalks, nsteps, nsubwalks, cv, av, seed, eps, min_R, base)
+        c, gv, hv = self._bc(bc_code, coeffs, gv, hv)
+        out = np.zeros((n, D), np.float32)
+        smp = np.zeros((n * samples_per_point, D), np.float32)
+        st = np.zeros(1, np.uint64)
+        lib().orc_trace_step(self.h, _p(pts), n, _p(ip), _p(fp), bc_code, _p(c), _p(gv), _p(hv), src_code, src_c,
+                             int(samples_per_point), float(step_size), _p(out), _p(smp), _p(st))
+        return dict(pos=out, samples=smp, steps=int(st[0]))
 
     def solve_fused(self, pts, bc_code, nwalks, nsteps=128, eps=1e-3, min_R=1e-3, seed=0, coeffs=None, gv=None,
                     hv=None, src_code=0, src_c=0.0, per_pair=False, cv=True, base=0):

@@ -1723,20 +1723,22 @@ struct WalkOut {
 };
 
 // sdf / inside (polyline.py:188-234, mesh.py:290-297) on the full scene
-static bool inside2(const Scene& sc, V2 x) {
+static float sdf2(const Scene& sc, V2 x) {
     const Sub<2>& F = sc.s2[0];
     CP its = bvh2_closest(F.tree, x, nullptr);
     int i0 = F.I[2 * its.prim], i1 = F.I[2 * its.prim + 1];
     V2 n1 = v2(F.vnormal[2 * i0], F.vnormal[2 * i0 + 1]), n2 = v2(F.vnormal[2 * i1], F.vnormal[2 * i1 + 1]);
     V2 n = lerp(n1, n2, its.t);
-    float s = dot(n, sub(x, v2(its.p.x, its.p.y))) >= 0.0f ? 1.0f : -1.0f;
-    return its.d * s < 0.0f;
+    float s = dot(n, sub(x, v2(its.p.x, its.p.y))) >= 0.0f ? 1.0f : -1.0f;  // dr.sign: +1 at 0
+    return its.d * s;
 }
-static bool inside3(const Scene& sc, V3 x) {
+static float sdf3(const Scene& sc, V3 x) {
     CP c = snch3_closest(sc.s3[0].tree, x, nullptr);
     float s = dot(c.n, sub(x, c.p)) >= 0.0f ? 1.0f : -1.0f;
-    return c.d * s < 0.0f;
+    return c.d * s;
 }
+static bool inside2(const Scene& sc, V2 x) { return sdf2(sc, x) < 0.0f; }
+static bool inside3(const Scene& sc, V3 x) { return sdf3(sc, x) < 0.0f; }
 
 // WoSt.u (wost.py:145-221)
 static WalkOut wost_u2(const Scene& sc, const BC& bc, const Params& P, V2 x, Pcg& rng, bool ob0, V2 n0,
@@ -2444,6 +2446,75 @@ void orc_solve_dude(void* s, const float* pts, const float* e, int64_t n, const 
                            v3(e[3 * i], e[3 * i + 1], e[3 * i + 2]), rng, &st);
         steps += st;
     }
+    if (steps_total) *steps_total = steps;
+}
+
+// StreamlineTracer.next_position (examples/example_2d/trace.py:19-68,
+// examples/example_3d/trace.py:21-60) with WoSt / WoSt3D: sample j of point i
+// runs on the lane-(base+i)*tn+j stream of TEA(seed, base+i); each gradient
+// component is Solver.walk_dude (solver.py:148-166): the mean of nsubwalks
+// dude estimates, the axes one after the other on the sample's stream.  2D:
+// _next_position's two should_increment tests (sdf <= 0); the corrector term
+// fy_tilde = current_p - y_tilde is zero as written there, so a step that stays
+// inside keeps half its length.  3D: one Euler step.  The samples of a point
+// are averaged in sample order (the reference's scatter_reduce leaves the order
+// open).  samples[n*tn*dim] optional.
+void orc_trace_step(void* s, const float* pts, int64_t n, const int32_t* ip, const float* fp, int bc_code,
+                    const float* bc_c, const float* gv, const float* hv, int src_code, float src_c, int32_t tn,
+                    float step, float* out, float* samples, uint64_t* steps_total) {
+    Scene* sc = (Scene*)s;
+    Params P = make_params(ip, fp);
+    BC bc = make_bc(bc_code, bc_c, gv, hv, src_code, src_c);
+    const int D = sc->dim;
+    std::vector<float> smp((size_t)n * tn * D);
+    uint64_t steps = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : steps)
+    for (int64_t j = 0; j < n * (int64_t)tn; ++j) {
+        int64_t i = j / tn;
+        Pcg rng = pcg_for(P.seed, (uint32_t)(P.base + i), (uint64_t)((P.base + i) * tn + (j - i * tn)));
+        uint64_t st = 0;
+        float* o = &smp[(size_t)j * D];
+        if (D == 2) {
+            V2 p = v2(pts[2 * i], pts[2 * i + 1]);
+            auto walk_dude = [&](V2 e) {
+                float r = 0.0f;
+                for (int k = 0; k < P.nsubwalks; ++k) r += dude2(*sc, bc, P, p, e, rng, &st);
+                return r / (float)P.nsubwalks;
+            };
+            V2 y = p, cur = p;
+            if (!(sdf2(*sc, cur) > 0.0f)) {
+                float gx = walk_dude(v2(1.0f, 0.0f));
+                float gy = walk_dude(v2(0.0f, 1.0f));
+                cur = add(cur, mul(v2(gx, gy), step));
+                V2 yt = cur;
+                if (!(sdf2(*sc, cur) > 0.0f)) {
+                    V2 fy = sub(yt, y), fyt = sub(cur, yt);
+                    cur = add(y, mul(add(fy, fyt), 0.5f));
+                }
+            }
+            o[0] = cur.x, o[1] = cur.y;
+        } else {
+            V3 p = v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+            auto walk_dude = [&](V3 e) {
+                float r = 0.0f;
+                for (int k = 0; k < P.nsubwalks; ++k) r += dude3(*sc, bc, P, p, e, rng, &st);
+                return r / (float)P.nsubwalks;
+            };
+            float gx = walk_dude(v3(1.0f, 0.0f, 0.0f));
+            float gy = walk_dude(v3(0.0f, 1.0f, 0.0f));
+            float gz = walk_dude(v3(0.0f, 0.0f, 1.0f));
+            V3 cur = add(p, mul(v3(gx, gy, gz), step));
+            o[0] = cur.x, o[1] = cur.y, o[2] = cur.z;
+        }
+        steps += st;
+    }
+    for (int64_t i = 0; i < n; ++i)
+        for (int d = 0; d < D; ++d) {
+            float acc = 0.0f;
+            for (int k = 0; k < tn; ++k) acc += smp[((size_t)i * tn + k) * D + d];
+            out[i * D + d] = tn > 1 ? acc / (float)tn : acc;
+        }
+    if (samples) std::memcpy(samples, smp.data(), smp.size() * sizeof(float));
     if (steps_total) *steps_total = steps;
 }
 

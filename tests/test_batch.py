@@ -70,6 +70,30 @@ def test_batch_graph_replays(gpu, batch_items):
     b.close()
 
 
+def test_batch_3d_matches_per_scene(gpu):
+    """3D items (two noisy icospheres of different resolution and noise, and
+    different walk counts) in one interleaved persistent launch, eager and
+    graph-replayed, against their own solves."""
+    from dwost_amd import scenes
+
+    items = []
+    for k, (level, n, W) in enumerate(((3, 150, 32), (4, 90, 16))):
+        spec = scenes.icosphere_scene(level=level, n_query=n, scene_seed=3 + k)
+        sc = gpu.Scene3D(spec.vertices, spec.indices, spec.types, gpu.BoundaryCondition(spec.bc_code))
+        items.append(gpu.BatchItem(gpu.WoSt3D(eps=1e-3, nwalks=W, nsteps=1024), sc,
+                                   torch.as_tensor(spec.queries).cuda(), seed=40 + k))
+    refs = [it.solver.solve_grad(it.points, it.scene, seed=it.seed) for it in items]
+    torch.cuda.synchronize()
+    refs = [(r["u"].cpu().numpy(), r["grad"].cpu().numpy()) for r in refs]
+    for graph in (False, True):
+        b = gpu.BatchSolver(items, graph=graph)
+        for _ in range(2 if graph else 1):
+            outs = b.solve()
+            torch.cuda.synchronize()
+            _check(outs, refs)
+        b.close()
+
+
 def test_batch_rejects_mixed_dims(gpu, batch_items, ico_spec):
     items, _ = batch_items
     sc3 = gpu.Scene3D(ico_spec.vertices, ico_spec.indices, ico_spec.types)

@@ -88,3 +88,30 @@ def test_schedule_rejects_bad_values(gpu):
         replace(sol, schedule=3).solve(q[:4], sc)
     with pytest.raises(Exception):
         replace(sol, gate=65).solve(q[:4], sc)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_short_walks_match_oracle(gpu, oracle, dim):
+    """Step limits 0, 1, 2 and 3 subwalks (the step machine's edge transitions:
+    a walk that ends before its first query, at its first step boundary, and
+    walks restarting inside one job) under both schedules, walk by walk against
+    the oracle."""
+    from dwost_amd import scenes
+
+    if dim == 2:
+        spec = scenes.holes_scene_2d(n_query=64, eps=1e-3)
+        sc = gpu.Scene2D(spec.vertices, spec.indices, spec.types, gpu.BoundaryCondition(spec.bc_code))
+        Sol = gpu.WoSt
+    else:
+        spec = scenes.icosphere_scene(level=3, n_query=64, eps=1e-3)
+        sc = gpu.Scene3D(spec.vertices, spec.indices, spec.types, gpu.BoundaryCondition(spec.bc_code))
+        Sol = gpu.WoSt3D
+    os_ = oracle.OracleScene(spec.vertices, spec.indices, spec.types)
+    q = np.ascontiguousarray(spec.queries)
+    for nsteps in (0, 1, 2):
+        o = os_.solve_u(q, spec.bc_code, 8, nsteps=nsteps, eps=1e-3, seed=9, nsubwalks=3, per_walk=True)
+        for sched in (1, 2):
+            sol = Sol(eps=1e-3, nwalks=8, nsteps=nsteps, nsubwalks=3, schedule=sched)
+            g = sol.solve(torch.as_tensor(q).cuda(), sc, seed=9, per_walk=True)
+            assert np.array_equal(_bits(g["walk_u"]), o["walk_u"].view(np.uint32)), (nsteps, sched)
+            assert np.array_equal(g["walk_steps"].cpu().numpy(), o["walk_steps"]), (nsteps, sched)

@@ -224,7 +224,8 @@ int dw_adjoint_fused(const dw_scene* scene, const float* pts, int64_t n, const d
  * solver.py:36-71); each item's results are bit-identical to dw_solve_fused
  * with the same arguments, since walk streams are keyed by (point_base + i,
  * pair) per item.  All items share the dimension and the DW_SOLVE_EXACT_CONES
- * bit; pts / u / grad are device arrays that must outlive the batch (their
+ * bit; the launch shape and schedule (block_size, blocks_per_cu, schedule,
+ * gate) are the first item's; pts / u / grad are device arrays that must outlive the batch (their
  * contents may change between solves).  total_steps (device uint64, optional)
  * receives the walk steps of each solve. */
 typedef struct dw_batch_item {

@@ -215,7 +215,7 @@ struct Walk {
     int i;
     int term;
     int stage;
-    bool ob, live;
+    bool ob;
     float R, d_dir;  // ball radius and Dirichlet distance of the current step
     WalkAux<D> aux;
 };
@@ -231,7 +231,6 @@ DWD void walk_init(Walk<D>& w, typename VecT<D>::T x) {
     w.result = 0.0f;
     w.i = 0;
     w.term = -1;
-    w.live = true;
     w.stage = WS_START;
 }
 
@@ -239,11 +238,7 @@ DWD void walk_init(Walk<D>& w, typename VecT<D>::T x) {
 template <int D>
 DWD int step_done(Walk<D>& w, const ParamsDev& P) {
     w.stage = WS_START;
-    if (!(w.i < P.nsteps)) {
-        w.live = false;
-        return ADV_END;
-    }
-    return ADV_STEP;
+    return w.i < P.nsteps ? ADV_STEP : ADV_END;
 }
 
 template <bool X, bool A>
@@ -253,10 +248,7 @@ DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Wa
     const SubDev& Dn = sc.sub[1];
     const SubDev& Nm = sc.sub[2];
     if (w.stage == WS_START) {
-        if (!(w.i < P.nsteps)) {
-            w.live = false;
-            return ADV_END;
-        }
+        if (!(w.i < P.nsteps)) return ADV_END;
         if (P.flags & DW_SOLVE_WOS) {
             closest2_begin(t, F, w.ip);
             w.stage = WS_WOS;
@@ -277,7 +269,6 @@ DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Wa
             w.result += g2(bc, F, c);
             if constexpr (A) adj_lin2(adj.dg, F, c.prim, c.t, adj.w);
             w.term = c.prim;
-            w.live = false;
             return ADV_END;
         }
         F2 dir = uniform_dir2(rng);
@@ -291,7 +282,6 @@ DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Wa
             if constexpr (A) adj_lin2(adj.dg, Dn, cpd.prim, cpd.t, adj.w);
             w.term = cpd.prim;
             w.i += 1;
-            w.live = false;
             return ADV_END;
         }
         w.stage = WS_STAR;
@@ -392,10 +382,7 @@ DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Wa
     const SubDev& Dn = sc.sub[1];
     const SubDev& Nm = sc.sub[2];
     if (w.stage == WS_START) {
-        if (!(w.i < P.nsteps)) {
-            w.live = false;
-            return ADV_END;
-        }
+        if (!(w.i < P.nsteps)) return ADV_END;
         if (P.flags & DW_SOLVE_WOS) {
             closest3_begin(t, F, w.ip);
             w.stage = WS_WOS;
@@ -415,7 +402,6 @@ DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Wa
             rng.advance(left * (Nm.present ? 7u : 4u));
             w.result += 0.0f;
             w.i = P.nsteps;
-            w.live = false;
             return ADV_END;
         }
         if (Dn.present) {
@@ -433,7 +419,6 @@ DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Wa
             w.result += g3(bc, F, c);
             if constexpr (A) adj_lin3(adj.dg, F, c.prim, c.u, c.v, adj.w);
             w.term = c.prim;
-            w.live = false;
             return ADV_END;
         }
         F3 dir = uniform_sphere(rng);
@@ -447,7 +432,6 @@ DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Wa
             if constexpr (A) adj_lin3(adj.dg, Dn, cpd.prim, cpd.u, cpd.v, adj.w);
             w.term = cpd.prim;
             w.i += 1;
-            w.live = false;
             return ADV_END;
         }
         w.stage = WS_STAR;

@@ -740,6 +740,124 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
 }
 
 
+// WoSt.solve_dude / WoSt3D.solve_dude (wost.py:21-102, wost_3d.py:108-175) as
+// persistent jobs: job = point, whose nwalks walks run in order on the point's
+// one PCG32 stream (each antithetic pair restarts from the state its first-ball
+// draws leave, the next pair continues where the second walk ended), exactly
+// as dude2 / dude3; the lane scheduler refills lanes as points finish.
+template <int D, bool X, bool I>
+__global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVES_FAST)
+    k_dude_pers(const SceneDev* __restrict__ scp, BCDev bc, ParamsDev P, const float* __restrict__ pts,
+                const float* __restrict__ e, const float* __restrict__ Rp, const int* __restrict__ insidep, float* out,
+                unsigned long long* step_total, JobQ Q) {
+    const SceneDev& sc = *scp;
+    using V = typename VecT<D>::T;
+    int64_t job = -1;
+    V p, ev, dir, dir_src, p_src;
+    float R = 0.0f, r_src = 0.0f, pdf_src = 0.0f, result = 0.0f;
+    float u_acc = 0.0f, u_count = 0.0f, src_acc = 0.0f, src_count = 0.0f;  // 2D running sums, 3D running means
+    uint64_t state = 0;
+    int wi = 0, it = 0, niters = 1;
+    unsigned long long steps = 0;
+    auto begin_pair = [&](Rng& rng) DW_LAMBDA -> bool {
+        if (!(wi < P.nwalks)) return false;
+        if constexpr (D == 2) {
+            dir = uniform_dir2(rng);
+            dir_src = uniform_dir2(rng);
+            greens2_sample(p, R, dir_src, rng, p_src, r_src, pdf_src);
+        } else {
+            dir = uniform_sphere(rng);
+            greens3_sample(p, R, dir, rng, p_src, r_src, pdf_src);
+        }
+        niters = P.av ? 2 : 1;
+        state = rng.state;
+        it = 0;
+        return true;
+    };
+    auto begin_walk = [&](Walk<D>& w, Rng& rng) DW_LAMBDA {
+        if constexpr (D == 2) {
+            const float gnorm = R * R / 4.0f;
+            if (it == 1) {
+                dir_src = -dir_src;
+                p_src = p * 2.0f - p_src;
+            }
+            float src = gnorm * src_f(bc);
+            float src_grad = dot(greens2_gradient(p, R, r_src, p_src), ev) / (pdf_src * gnorm);
+            src_acc += src;
+            src_count += 1.0f;
+            float _src = P.cv ? src - (src_acc / src_count) : src;
+            result += _src * src_grad;
+        } else {
+            const float gnorm = R * R / 6.0f;
+            if (it == 1) {
+                dir = -dir;
+                p_src = p * 2.0f - p_src;
+            }
+            float src = gnorm * src_f(bc);
+            float src_grad = dot(greens3_gradient(p, R, r_src, p_src), ev) / (pdf_src * gnorm);
+            src_acc = src_acc * (src_count / (src_count + 1.0f)) + src / (src_count + 1.0f);
+            src_count += 1.0f;
+            float _src = P.cv ? src - src_acc : src;
+            result += _src * src_grad;
+        }
+        if (it == 1) dir = -dir;  // 3D: the second flip (wost_3d.py:152-153)
+        rng.state = state;
+        walk_init<D>(w, p + dir * R);
+    };
+    auto finish = [&]() DW_LAMBDA {
+        if constexpr (D == 3) {
+            if (bad(result) || __builtin_fabsf(result) > 10000.0f) result = 0.0f;
+        }
+        out[job] = result / (float)P.nwalks;
+    };
+    auto start = [&](int64_t got, Walk<D>& w, Rng& rng) DW_LAMBDA -> bool {
+        job = got;
+        p = load_pt<D>(pts, got);
+        ev = load_pt<D>(e, got);
+        R = Rp[got];
+        seed_stream(rng, P, got, 1, 0);
+        result = u_acc = u_count = src_acc = src_count = 0.0f;
+        wi = 0;
+        if (!insidep[got] || !begin_pair(rng)) {
+            finish();
+            return false;
+        }
+        begin_walk(w, rng);
+        return true;
+    };
+    auto next = [&](Walk<D>& w, Rng& rng) DW_LAMBDA -> bool {
+        float u = walk_value(w, P);
+        steps += w.i;
+        if constexpr (D == 2) {
+            u_acc += u;
+            u_count += 1.0f;
+            float _u = P.cv ? u - u_acc / u_count : u;
+            float r2 = 2.0f / R * _u * dot(ev, dir);
+            if ((__builtin_fabsf(r2) > 1e6f) || bad(r2)) r2 = 0.0f;
+            result += r2;
+        } else {
+            if (bad(u) || __builtin_fabsf(u) > 10000.0f) u = 0.0f;
+            u_acc = u_acc * (u_count / (u_count + 1.0f)) + u / (u_count + 1.0f);
+            u_count += 1.0f;
+            float _u = P.cv ? u - u_acc : u;
+            result += 3.0f / R * _u * dot(ev, dir);
+        }
+        wi += 1;
+        it += 1;
+        if ((it < niters && wi < P.nwalks) || begin_pair(rng)) {
+            begin_walk(w, rng);
+            return true;
+        }
+        finish();
+        return false;
+    };
+    auto advance = [&](Walk<D>& w, Rng& rng, Trav& t) DW_LAMBDA -> int {
+        return walk_advance<X, false>(sc, bc, P, w, rng, t, Adj{0.0f, nullptr, nullptr});
+    };
+    run_jobs<D, X, I>(Q, start, next, advance);
+    if (step_total && steps) atomicAdd(step_total, steps);
+}
+
 // Fused u + grad u: one antithetic pair per lane (pair k of point i uses the
 // stream of lane i*npairs + k).  Writes (usum, g[dim]) per pair and steps.
 __global__ void __launch_bounds__(kBlock) k_fused(SceneDev sc, BCDev bc, ParamsDev P, const float* __restrict__ pts,
@@ -1784,13 +1902,29 @@ int dw_solve_dude(const dw_scene* s, const float* pts, const float* e, int64_t n
     hipStream_t st = (hipStream_t)stream;
     StreamCtx* c;
     if ((rc = stream_ctx(st, &c))) return rc;
+    // WoSt dudes run as persistent point jobs; WoS dudes (and DWOST_KERNEL=simple) one point per thread
+    const bool persistent = !(params->flags & DW_SOLVE_WOS) && !use_simple_kernels();
+    const size_t obytes = persistent ? order_bytes(n, params) : 0;
     void* p;
-    if ((rc = workspace(*c, (size_t)n * 8, st, &p))) return rc;
+    if ((rc = workspace(*c, (size_t)n * 8 + obytes, st, &p))) return rc;
     float* R = (float*)p;
     int* ins = (int*)((char*)p + (size_t)n * 4);
     hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(n)), dim3(kBlock), kStackLds, st, s->dev, pts, n, R, ins);
     DW_HIP(hipGetLastError());
     if (total_steps && (rc = zero_u64((unsigned long long*)total_steps, st))) return rc;
+    if (persistent) {
+        JobQ Q;
+        if ((rc = make_queue(s, *c, pts, n, 1, obytes ? (char*)p + (size_t)n * 8 : nullptr, st, Q))) return rc;
+        const LaunchCfg cfg = launch_cfg(to_params(params), s->dev.dim);
+        Q.gate = cfg.gate;
+        const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
+        if (s->dev.dim == 2)
+            return launch_pers(exact, cfg, DW_PERS(k_dude_pers, 2), st, s->dev_ptr, to_bc(boundary),
+                               to_params(params), pts, e, (const float*)R, (const int*)ins, out,
+                               (unsigned long long*)total_steps, Q);
+        return launch_pers(exact, cfg, DW_PERS(k_dude_pers, 3), st, s->dev_ptr, to_bc(boundary), to_params(params),
+                           pts, e, (const float*)R, (const int*)ins, out, (unsigned long long*)total_steps, Q);
+    }
     hipLaunchKernelGGL(k_dude, dim3(blocks_for(n)), dim3(kBlock), kStackLds, st, s->dev, to_bc(boundary), to_params(params),
                        pts, e, R, ins, n, out, (unsigned long long*)total_steps);
     DW_HIP(hipGetLastError());

@@ -115,3 +115,23 @@ def test_short_walks_match_oracle(gpu, oracle, dim):
             g = sol.solve(torch.as_tensor(q).cuda(), sc, seed=9, per_walk=True)
             assert np.array_equal(_bits(g["walk_u"]), o["walk_u"].view(np.uint32)), (nsteps, sched)
             assert np.array_equal(g["walk_steps"].cpu().numpy(), o["walk_steps"]), (nsteps, sched)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_persistent_dude_matches_simple(gpu, dim, monkeypatch):
+    """solve_dude runs WoSt dudes as persistent point jobs; with the literal cone
+    test both schedules return the bits of the one-point-per-thread kernel."""
+    sc, sol, q = _case(gpu, dim)
+    sol = replace(sol, exact_cones=True, nwalks=12)
+    rng = np.random.default_rng(3)
+    e = rng.standard_normal((q.shape[0], dim)).astype(np.float32)
+    e /= np.linalg.norm(e, axis=1, keepdims=True)
+    e = torch.as_tensor(e).cuda()
+    monkeypatch.setenv("DWOST_KERNEL", "simple")
+    ref, ref_steps = sol.solve_dude(q, e, sc, seed=4, count_steps=True)
+    torch.cuda.synchronize()
+    monkeypatch.delenv("DWOST_KERNEL")
+    for sched in (1, 2):
+        got, steps = replace(sol, schedule=sched).solve_dude(q, e, sc, seed=4, count_steps=True)
+        assert np.array_equal(_bits(got), _bits(ref)), sched
+        assert int(steps[0]) == int(ref_steps[0])

@@ -740,26 +740,66 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
 }
 
 
-// WoSt.solve_dude / WoSt3D.solve_dude (wost.py:21-102, wost_3d.py:108-175) as
-// persistent jobs: job = point, whose nwalks walks run in order on the point's
-// one PCG32 stream (each antithetic pair restarts from the state its first-ball
-// draws leave, the next pair continues where the second walk ended), exactly
-// as dude2 / dude3; the lane scheduler refills lanes as points finish.
-template <int D, bool X, bool I>
-__global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVES_FAST)
-    k_dude_pers(const SceneDev* __restrict__ scp, BCDev bc, ParamsDev P, const float* __restrict__ pts,
-                const float* __restrict__ e, const float* __restrict__ Rp, const int* __restrict__ insidep, float* out,
-                unsigned long long* step_total, JobQ Q) {
-    const SceneDev& sc = *scp;
+// The walk sequence of WoSt.dude / WoSt3D.dude (wost.py:30-102,
+// wost_3d.py:108-175) at one point for one direction, as a resumable machine:
+// nwalks walks in order on the caller's stream, each antithetic pair restarting
+// from the state its first-ball draws leave and the next pair continuing where
+// the second walk ended (the arithmetic of dude2 / dude3).
+template <int D>
+struct DudeRun {
     using V = typename VecT<D>::T;
-    int64_t job = -1;
     V p, ev, dir, dir_src, p_src;
-    float R = 0.0f, r_src = 0.0f, pdf_src = 0.0f, result = 0.0f;
-    float u_acc = 0.0f, u_count = 0.0f, src_acc = 0.0f, src_count = 0.0f;  // 2D running sums, 3D running means
-    uint64_t state = 0;
-    int wi = 0, it = 0, niters = 1;
-    unsigned long long steps = 0;
-    auto begin_pair = [&](Rng& rng) DW_LAMBDA -> bool {
+    float R, r_src, pdf_src, result, u_acc, u_count, src_acc, src_count;  // 2D running sums, 3D running means
+    uint64_t state;
+    int wi, it, niters;
+
+    // Starts the estimate; true when its first walk is set up in w.
+    DWD bool begin(V p_, V e_, float R_, bool active, const ParamsDev& P, const BCDev& bc, Walk<D>& w, Rng& rng) {
+        p = p_;
+        ev = e_;
+        R = R_;
+        result = u_acc = u_count = src_acc = src_count = 0.0f;
+        wi = 0;
+        if (!active || !begin_pair(P, rng)) return false;
+        begin_walk(P, bc, w, rng);
+        return true;
+    }
+    // The walk in w has ended; true when the next walk is set up.
+    DWD bool walk_ended(const ParamsDev& P, const BCDev& bc, Walk<D>& w, Rng& rng, unsigned long long& steps) {
+        float u = walk_value(w, P);
+        steps += w.i;
+        if constexpr (D == 2) {
+            u_acc += u;
+            u_count += 1.0f;
+            float _u = P.cv ? u - u_acc / u_count : u;
+            float r2 = 2.0f / R * _u * dot(ev, dir);
+            if ((__builtin_fabsf(r2) > 1e6f) || bad(r2)) r2 = 0.0f;
+            result += r2;
+        } else {
+            if (bad(u) || __builtin_fabsf(u) > 10000.0f) u = 0.0f;
+            u_acc = u_acc * (u_count / (u_count + 1.0f)) + u / (u_count + 1.0f);
+            u_count += 1.0f;
+            float _u = P.cv ? u - u_acc : u;
+            result += 3.0f / R * _u * dot(ev, dir);
+        }
+        wi += 1;
+        it += 1;
+        if ((it < niters && wi < P.nwalks) || begin_pair(P, rng)) {
+            begin_walk(P, bc, w, rng);
+            return true;
+        }
+        return false;
+    }
+    // The estimate: sum / nwalks (3D: outlier guard first).
+    DWD float value(const ParamsDev& P) const {
+        float r = result;
+        if constexpr (D == 3) {
+            if (bad(r) || __builtin_fabsf(r) > 10000.0f) r = 0.0f;
+        }
+        return r / (float)P.nwalks;
+    }
+
+    DWD bool begin_pair(const ParamsDev& P, Rng& rng) {
         if (!(wi < P.nwalks)) return false;
         if constexpr (D == 2) {
             dir = uniform_dir2(rng);
@@ -773,8 +813,8 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
         state = rng.state;
         it = 0;
         return true;
-    };
-    auto begin_walk = [&](Walk<D>& w, Rng& rng) DW_LAMBDA {
+    }
+    DWD void begin_walk(const ParamsDev& P, const BCDev& bc, Walk<D>& w, Rng& rng) {
         if constexpr (D == 2) {
             const float gnorm = R * R / 4.0f;
             if (it == 1) {
@@ -803,53 +843,135 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
         if (it == 1) dir = -dir;  // 3D: the second flip (wost_3d.py:152-153)
         rng.state = state;
         walk_init<D>(w, p + dir * R);
+    }
+};
+
+// WoSt.solve_dude / WoSt3D.solve_dude (wost.py:21-102, wost_3d.py:108-175) as
+// persistent jobs: job = point, its walks on the point's one stream (DudeRun);
+// the lane scheduler refills lanes as points finish.
+template <int D, bool X, bool I>
+__global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVES_FAST)
+    k_dude_pers(const SceneDev* __restrict__ scp, BCDev bc, ParamsDev P, const float* __restrict__ pts,
+                const float* __restrict__ e, const float* __restrict__ Rp, const int* __restrict__ insidep, float* out,
+                unsigned long long* step_total, JobQ Q) {
+    const SceneDev& sc = *scp;
+    int64_t job = -1;
+    DudeRun<D> dude;
+    unsigned long long steps = 0;
+    auto start = [&](int64_t got, Walk<D>& w, Rng& rng) DW_LAMBDA -> bool {
+        job = got;
+        seed_stream(rng, P, got, 1, 0);
+        if (dude.begin(load_pt<D>(pts, got), load_pt<D>(e, got), Rp[got], insidep[got] != 0, P, bc, w, rng))
+            return true;
+        out[job] = dude.value(P);
+        return false;
     };
-    auto finish = [&]() DW_LAMBDA {
+    auto next = [&](Walk<D>& w, Rng& rng) DW_LAMBDA -> bool {
+        if (dude.walk_ended(P, bc, w, rng, steps)) return true;
+        out[job] = dude.value(P);
+        return false;
+    };
+    auto advance = [&](Walk<D>& w, Rng& rng, Trav& t) DW_LAMBDA -> int {
+        return walk_advance<X, false>(sc, bc, P, w, rng, t, Adj{0.0f, nullptr, nullptr});
+    };
+    run_jobs<D, X, I>(Q, start, next, advance);
+    if (step_total && steps) atomicAdd(step_total, steps);
+}
+
+// StreamlineTracer.next_position (examples/example_2d/trace.py:19-68,
+// examples/example_3d/trace.py:21-60) as persistent jobs: job j = i * tn + k is
+// sample k of point i (the stream of lane j, TEA(seed, i)); its gradient
+// components are walk_dude (nsubwalks DudeRuns) along each axis in turn on that
+// stream, then the step rule of k_trace.  samples[j] = the sample's position.
+template <int D, bool X, bool I>
+__global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVES_FAST)
+    k_trace_pers(const SceneDev* __restrict__ scp, BCDev bc, ParamsDev P, const float* __restrict__ pts,
+                 const float* __restrict__ Rp, const int* __restrict__ insidep, int tn, float step, float* samples,
+                 unsigned long long* step_total, JobQ Q) {
+    using V = typename VecT<D>::T;
+    const SceneDev& sc = *scp;
+    int64_t job = -1;
+    V p, g;
+    float R = 0.0f, comp = 0.0f;
+    bool active = false;
+    int axis = 0, sub = 0;
+    DudeRun<D> dude;
+    unsigned long long steps = 0;
+    auto put = [&](V q) DW_LAMBDA {
+        float* o = samples + job * D;
+        o[0] = q.x;
+        o[1] = q.y;
+        if constexpr (D == 3) o[2] = q.z;
+    };
+    auto unit = [&](int a) DW_LAMBDA -> V {
+        V u;
+        set_zero(u);
+        if (a == 0) u.x = 1.0f;
+        if (a == 1) u.y = 1.0f;
         if constexpr (D == 3) {
-            if (bad(result) || __builtin_fabsf(result) > 10000.0f) result = 0.0f;
+            if (a == 2) u.z = 1.0f;
         }
-        out[job] = result / (float)P.nwalks;
+        return u;
+    };
+    auto finish_sample = [&]() DW_LAMBDA {
+        V cur = p + g * step;
+        if constexpr (D == 2) {
+            F2 yt = cur;
+            if (!(sdf2(sc, cur) > 0.0f)) {
+                F2 fy = yt - p, fyt = cur - yt;
+                cur = p + (fy + fyt) * 0.5f;
+            }
+        }
+        put(cur);
+    };
+    // Dude value done: fold it into the axis component; starts dudes until one
+    // has a walk (true) or the sample is complete (false).
+    auto next_dude = [&](Walk<D>& w, Rng& rng, bool have_value) DW_LAMBDA -> bool {
+        while (true) {
+            if (have_value) {
+                comp += dude.value(P);
+                if (++sub == P.nsubwalks) {
+                    float c = comp / (float)P.nsubwalks;
+                    if (axis == 0) g.x = c;
+                    if (axis == 1) g.y = c;
+                    if constexpr (D == 3) {
+                        if (axis == 2) g.z = c;
+                    }
+                    ++axis;
+                    sub = 0;
+                    comp = 0.0f;
+                }
+            }
+            if (axis == D) {
+                finish_sample();
+                return false;
+            }
+            if (dude.begin(p, unit(axis), R, active, P, bc, w, rng)) return true;
+            have_value = true;
+        }
     };
     auto start = [&](int64_t got, Walk<D>& w, Rng& rng) DW_LAMBDA -> bool {
         job = got;
-        p = load_pt<D>(pts, got);
-        ev = load_pt<D>(e, got);
-        R = Rp[got];
-        seed_stream(rng, P, got, 1, 0);
-        result = u_acc = u_count = src_acc = src_count = 0.0f;
-        wi = 0;
-        if (!insidep[got] || !begin_pair(rng)) {
-            finish();
-            return false;
+        int64_t i = got / tn;
+        seed_stream(rng, P, i, tn, got - i * tn);
+        p = load_pt<D>(pts, i);
+        R = Rp[i];
+        active = insidep[i] != 0;
+        set_zero(g);
+        if constexpr (D == 2) {
+            if (sdf2(sc, p) > 0.0f) {  // should_increment: only from inside
+                put(p);
+                return false;
+            }
         }
-        begin_walk(w, rng);
-        return true;
+        axis = 0;
+        sub = 0;
+        comp = 0.0f;
+        return next_dude(w, rng, false);
     };
     auto next = [&](Walk<D>& w, Rng& rng) DW_LAMBDA -> bool {
-        float u = walk_value(w, P);
-        steps += w.i;
-        if constexpr (D == 2) {
-            u_acc += u;
-            u_count += 1.0f;
-            float _u = P.cv ? u - u_acc / u_count : u;
-            float r2 = 2.0f / R * _u * dot(ev, dir);
-            if ((__builtin_fabsf(r2) > 1e6f) || bad(r2)) r2 = 0.0f;
-            result += r2;
-        } else {
-            if (bad(u) || __builtin_fabsf(u) > 10000.0f) u = 0.0f;
-            u_acc = u_acc * (u_count / (u_count + 1.0f)) + u / (u_count + 1.0f);
-            u_count += 1.0f;
-            float _u = P.cv ? u - u_acc : u;
-            result += 3.0f / R * _u * dot(ev, dir);
-        }
-        wi += 1;
-        it += 1;
-        if ((it < niters && wi < P.nwalks) || begin_pair(rng)) {
-            begin_walk(w, rng);
-            return true;
-        }
-        finish();
-        return false;
+        if (dude.walk_ended(P, bc, w, rng, steps)) return true;
+        return next_dude(w, rng, true);
     };
     auto advance = [&](Walk<D>& w, Rng& rng, Trav& t) DW_LAMBDA -> int {
         return walk_advance<X, false>(sc, bc, P, w, rng, t, Adj{0.0f, nullptr, nullptr});
@@ -1946,11 +2068,32 @@ int dw_trace_step(const dw_scene* s, const float* pts, int64_t n, const dw_solve
     hipStream_t st = (hipStream_t)stream;
     StreamCtx* c;
     if ((rc = stream_ctx(st, &c))) return rc;
+    const bool persistent = !use_simple_kernels();
+    const size_t obytes = persistent ? order_bytes(n, params) : 0;
     void* p;
-    if ((rc = workspace(*c, (size_t)ns * D * 4, st, &p))) return rc;
+    if ((rc = workspace(*c, (size_t)ns * D * 4 + (size_t)n * 8 + obytes, st, &p))) return rc;
     float* samples = (float*)p;
+    float* R = (float*)((char*)p + (size_t)ns * D * 4);
+    int* ins = (int*)(R + n);
     if (total_steps && (rc = zero_u64((unsigned long long*)total_steps, st))) return rc;
-    if (D == 2)
+    if (persistent) {  // samples as persistent jobs, grouped by point in the queue
+        hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(n)), dim3(kBlock), kStackLds, st, s->dev, pts, n, R, ins);
+        DW_HIP(hipGetLastError());
+        JobQ Q;
+        if ((rc = make_queue(s, *c, pts, n, samples_per_point, obytes ? (char*)(ins + n) : nullptr, st, Q))) return rc;
+        const LaunchCfg cfg = launch_cfg(to_params(params), D);
+        Q.gate = cfg.gate;
+        const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
+        if (D == 2)
+            rc = launch_pers(exact, cfg, DW_PERS(k_trace_pers, 2), st, s->dev_ptr, to_bc(boundary), to_params(params),
+                             pts, (const float*)R, (const int*)ins, (int)samples_per_point, step_size, samples,
+                             (unsigned long long*)total_steps, Q);
+        else
+            rc = launch_pers(exact, cfg, DW_PERS(k_trace_pers, 3), st, s->dev_ptr, to_bc(boundary), to_params(params),
+                             pts, (const float*)R, (const int*)ins, (int)samples_per_point, step_size, samples,
+                             (unsigned long long*)total_steps, Q);
+        if (rc) return rc;
+    } else if (D == 2)
         hipLaunchKernelGGL(k_trace<2>, dim3(blocks_for(ns)), dim3(kBlock), kStackLds, st, s->dev, to_bc(boundary),
                            to_params(params), pts, n, (int)samples_per_point, step_size, samples,
                            (unsigned long long*)total_steps);

@@ -100,3 +100,33 @@ def test_trace_parity(oracle, which, star_spec, ico_spec):
         o = g  # continue both from the GPU positions
     with pytest.raises(Exception):
         gpu.StreamlineTracer(scene, gpu.WoS(eps=1e-3) if spec.dim == 2 else gpu.WoS3D(eps=1e-3)).next_position(p)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("which", ["star", "ico"])
+def test_persistent_trace_matches_simple(which, star_spec, ico_spec, monkeypatch):
+    """The tracer runs its samples as persistent jobs; with the literal cone test
+    both schedules return the bits of the one-sample-per-thread kernel."""
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from dataclasses import replace
+
+    import dwost_amd as gpu
+
+    spec = dict(star=star_spec, ico=ico_spec)[which]
+    cls = gpu.Scene2D if spec.dim == 2 else gpu.Scene3D
+    scene = cls(spec.vertices, spec.indices, spec.types, gpu.BoundaryCondition(spec.bc_code))
+    sol = (gpu.WoSt if spec.dim == 2 else gpu.WoSt3D)(eps=1e-3, nwalks=8, nsubwalks=2, nsteps=1024,
+                                                      exact_cones=True)
+    q = torch.as_tensor(np.ascontiguousarray(spec.queries[:40])).cuda()
+    monkeypatch.setenv("DWOST_KERNEL", "simple")
+    ref, ref_steps = gpu.StreamlineTracer(scene, sol, step_size=0.03, nwalks=3).next_position(q, seed=6,
+                                                                                            count_steps=True)
+    torch.cuda.synchronize()
+    monkeypatch.delenv("DWOST_KERNEL")
+    for sched in (1, 2):
+        tr = gpu.StreamlineTracer(scene, replace(sol, schedule=sched), step_size=0.03, nwalks=3)
+        got, steps = tr.next_position(q, seed=6, count_steps=True)
+        assert np.array_equal(got.cpu().numpy().view(np.uint32), ref.cpu().numpy().view(np.uint32)), sched
+        assert int(steps[0]) == int(ref_steps[0])

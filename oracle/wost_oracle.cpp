@@ -18,7 +18,8 @@
 //     once to fp32 (correctly rounded except in ~2^-28 of inputs).
 //   * Dr.Jit's own bit-level choices (its transcendental polynomials, its
 //     fmadd usage inside dot, PCG32 lane offsets) are not observable in this
-//     container; the choices above are documented in DESIGN.md.
+//     container; the choices above are this restatement's own, shared with the
+//     HIP kernels.
 //
 // The C++ builder restates include/bvh.h + include/snch.h (Eigen reductions
 // written out in Eigen's unrolled order: x0 op (x1 op x2) for 3-vectors).

@@ -741,17 +741,22 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
 
 
 // The walk sequence of WoSt.dude / WoSt3D.dude (wost.py:30-102,
-// wost_3d.py:108-175) at one point for one direction, as a resumable machine:
-// nwalks walks in order on the caller's stream, each antithetic pair restarting
-// from the state its first-ball draws leave and the next pair continuing where
-// the second walk ended (the arithmetic of dude2 / dude3).
+// wost_3d.py:108-175) and, with DW_SOLVE_WOS, WoS.dude / WoS3D.dude
+// (wos.py:38-105, wos_3d.py:36-102) at one point for one direction, as a
+// resumable machine: nwalks walks in order on the caller's stream, each
+// antithetic pair restarting from the state its first-ball draws leave and the
+// next pair continuing where the second walk ended (the arithmetic of dude2 /
+// dude3 and of k_dude's WoS branch).  Running sums in 2D and for WoS, running
+// means in WoSt3D; WoS has no outlier guards.
 template <int D>
 struct DudeRun {
     using V = typename VecT<D>::T;
     V p, ev, dir, dir_src, p_src;
-    float R, r_src, pdf_src, result, u_acc, u_count, src_acc, src_count;  // 2D running sums, 3D running means
+    float R, r_src, pdf_src, result, u_acc, u_count, src_acc, src_count;
     uint64_t state;
     int wi, it, niters;
+
+    DWD static bool sums(const ParamsDev& P) { return D == 2 || (P.flags & DW_SOLVE_WOS); }
 
     // Starts the estimate; true when its first walk is set up in w.
     DWD bool begin(V p_, V e_, float R_, bool active, const ParamsDev& P, const BCDev& bc, Walk<D>& w, Rng& rng) {
@@ -768,12 +773,13 @@ struct DudeRun {
     DWD bool walk_ended(const ParamsDev& P, const BCDev& bc, Walk<D>& w, Rng& rng, unsigned long long& steps) {
         float u = walk_value(w, P);
         steps += w.i;
-        if constexpr (D == 2) {
+        if (sums(P)) {
+            const bool wos = (P.flags & DW_SOLVE_WOS) != 0;
             u_acc += u;
             u_count += 1.0f;
             float _u = P.cv ? u - u_acc / u_count : u;
-            float r2 = 2.0f / R * _u * dot(ev, dir);
-            if ((__builtin_fabsf(r2) > 1e6f) || bad(r2)) r2 = 0.0f;
+            float r2 = (float)D / R * _u * dot(ev, dir);
+            if (!wos && ((__builtin_fabsf(r2) > 1e6f) || bad(r2))) r2 = 0.0f;
             result += r2;
         } else {
             if (bad(u) || __builtin_fabsf(u) > 10000.0f) u = 0.0f;
@@ -790,10 +796,10 @@ struct DudeRun {
         }
         return false;
     }
-    // The estimate: sum / nwalks (3D: outlier guard first).
+    // The estimate: sum / nwalks (WoSt3D: outlier guard first).
     DWD float value(const ParamsDev& P) const {
         float r = result;
-        if constexpr (D == 3) {
+        if (!sums(P)) {
             if (bad(r) || __builtin_fabsf(r) > 10000.0f) r = 0.0f;
         }
         return r / (float)P.nwalks;
@@ -805,6 +811,10 @@ struct DudeRun {
             dir = uniform_dir2(rng);
             dir_src = uniform_dir2(rng);
             greens2_sample(p, R, dir_src, rng, p_src, r_src, pdf_src);
+        } else if (P.flags & DW_SOLVE_WOS) {  // its own source direction
+            dir = uniform_sphere(rng);
+            dir_src = uniform_sphere(rng);
+            greens3_sample(p, R, dir_src, rng, p_src, r_src, pdf_src);
         } else {
             dir = uniform_sphere(rng);
             greens3_sample(p, R, dir, rng, p_src, r_src, pdf_src);
@@ -815,39 +825,43 @@ struct DudeRun {
         return true;
     }
     DWD void begin_walk(const ParamsDev& P, const BCDev& bc, Walk<D>& w, Rng& rng) {
-        if constexpr (D == 2) {
-            const float gnorm = R * R / 4.0f;
+        const float gnorm = D == 2 ? R * R / 4.0f : R * R / 6.0f;
+        float src = gnorm * src_f(bc), src_grad;
+        if (sums(P)) {
             if (it == 1) {
                 dir_src = -dir_src;
                 p_src = p * 2.0f - p_src;
             }
-            float src = gnorm * src_f(bc);
-            float src_grad = dot(greens2_gradient(p, R, r_src, p_src), ev) / (pdf_src * gnorm);
+            if constexpr (D == 2)
+                src_grad = dot(greens2_gradient(p, R, r_src, p_src), ev) / (pdf_src * gnorm);
+            else
+                src_grad = dot(greens3_gradient(p, R, r_src, p_src), ev) / (pdf_src * gnorm);
             src_acc += src;
             src_count += 1.0f;
             float _src = P.cv ? src - (src_acc / src_count) : src;
             result += _src * src_grad;
         } else {
-            const float gnorm = R * R / 6.0f;
             if (it == 1) {
                 dir = -dir;
                 p_src = p * 2.0f - p_src;
             }
-            float src = gnorm * src_f(bc);
-            float src_grad = dot(greens3_gradient(p, R, r_src, p_src), ev) / (pdf_src * gnorm);
-            src_acc = src_acc * (src_count / (src_count + 1.0f)) + src / (src_count + 1.0f);
-            src_count += 1.0f;
-            float _src = P.cv ? src - src_acc : src;
-            result += _src * src_grad;
+            if constexpr (D == 3) {
+                src_grad = dot(greens3_gradient(p, R, r_src, p_src), ev) / (pdf_src * gnorm);
+                src_acc = src_acc * (src_count / (src_count + 1.0f)) + src / (src_count + 1.0f);
+                src_count += 1.0f;
+                float _src = P.cv ? src - src_acc : src;
+                result += _src * src_grad;
+            }
         }
-        if (it == 1) dir = -dir;  // 3D: the second flip (wost_3d.py:152-153)
+        if (it == 1) dir = -dir;  // WoSt3D: the second flip (wost_3d.py:152-153)
         rng.state = state;
         walk_init<D>(w, p + dir * R);
     }
 };
 
-// WoSt.solve_dude / WoSt3D.solve_dude (wost.py:21-102, wost_3d.py:108-175) as
-// persistent jobs: job = point, its walks on the point's one stream (DudeRun);
+// WoSt.solve_dude / WoSt3D.solve_dude (wost.py:21-102, wost_3d.py:108-175), and
+// the WoS dudes, as persistent jobs: job = point, its walks on the point's one
+// stream (DudeRun);
 // the lane scheduler refills lanes as points finish.
 template <int D, bool X, bool I>
 __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVES_FAST)
@@ -2024,8 +2038,8 @@ int dw_solve_dude(const dw_scene* s, const float* pts, const float* e, int64_t n
     hipStream_t st = (hipStream_t)stream;
     StreamCtx* c;
     if ((rc = stream_ctx(st, &c))) return rc;
-    // WoSt dudes run as persistent point jobs; WoS dudes (and DWOST_KERNEL=simple) one point per thread
-    const bool persistent = !(params->flags & DW_SOLVE_WOS) && !use_simple_kernels();
+    // dudes run as persistent point jobs (DWOST_KERNEL=simple: one point per thread)
+    const bool persistent = !use_simple_kernels();
     const size_t obytes = persistent ? order_bytes(n, params) : 0;
     void* p;
     if ((rc = workspace(*c, (size_t)n * 8 + obytes, st, &p))) return rc;

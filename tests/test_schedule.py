@@ -117,11 +117,15 @@ def test_short_walks_match_oracle(gpu, oracle, dim):
             assert np.array_equal(g["walk_steps"].cpu().numpy(), o["walk_steps"]), (nsteps, sched)
 
 
+@pytest.mark.parametrize("wos", [False, True])
 @pytest.mark.parametrize("dim", [2, 3])
-def test_persistent_dude_matches_simple(gpu, dim, monkeypatch):
-    """solve_dude runs WoSt dudes as persistent point jobs; with the literal cone
-    test both schedules return the bits of the one-point-per-thread kernel."""
+def test_persistent_dude_matches_simple(gpu, dim, wos, monkeypatch):
+    """solve_dude runs its dudes (WoSt and WoS) as persistent point jobs; with the
+    literal cone test both schedules return the bits of the one-point-per-thread
+    kernel."""
     sc, sol, q = _case(gpu, dim)
+    if wos:
+        sol = (gpu.WoS if dim == 2 else gpu.WoS3D)(eps=1e-3, nwalks=12, nsteps=1024)
     sol = replace(sol, exact_cones=True, nwalks=12)
     rng = np.random.default_rng(3)
     e = rng.standard_normal((q.shape[0], dim)).astype(np.float32)

@@ -529,13 +529,20 @@ DWD float project_to_plane(F3 n, F3 e) {
 }
 DWD bool in_range(float x, float a, float b) { return (x >= a) && (x <= b); }
 
-// SNCHNode.overlaps (snch.py:46-97, snch3d.py:65-115)
+// SNCHNode.overlaps (snch.py:46-97, snch3d.py:65-115): the box test, then for a
+// valid cone (ha >= 0) the literal cone test below.
+template <typename V>
+DWD bool snch_cone_literal(V mn, V mx, V axis, float ha, float radius, V x, float d_min);
 template <typename V>
 DWD bool snch_overlaps(V mn, V mx, V axis, float ha, float radius, V x, float r_max, float& d_min) {
     float d_max;
     box_dist(mn, mx, x, d_min, d_max);
-    bool ov = d_min <= r_max;
-    if ((ha >= 0.0f) && ov) {
+    return (d_min <= r_max) && (!(ha >= 0.0f) || snch_cone_literal(mn, mx, axis, ha, radius, x, d_min));
+}
+template <typename V>
+DWD bool snch_cone_literal(V mn, V mx, V axis, float ha, float radius, V x, float d_min) {
+    bool ov;
+    {
         ov = (ha > kPi2) && (d_min < kDrEps);
         if (!ov) {
             V cen = (mn + mx) / 2.0f;
@@ -619,16 +626,15 @@ DWD bool face_pruned(F3 mn, F3 mx, F3 axis, float rho, float elo, float ehi, F3 
     float sl = rho * norm(v);
     return ((cv - sl) - ehi > 0.0f) || ((cv + sl) - elo < 0.0f);
 }
-// One child of a 3D SNCH record: box overlap, face-plane bound, cone test.
+// One child of a 3D SNCH record: box overlap (d_min from box_dist), face-plane
+// bound, cone test.
 template <bool X>
 DWD bool snch3_child(F3 mn, F3 mx, F3 axis, float ha, float sin_ha, float cos_ha, float radius, float rho, float elo,
-                     float ehi, F3 x, float r_max, float& d_min) {
+                     float ehi, F3 x, float r_max, float d_min) {
     if constexpr (X) {
-        return snch_overlaps(mn, mx, axis, ha, radius, x, r_max, d_min) &&
+        return (d_min <= r_max) && (!(ha >= 0.0f) || snch_cone_literal(mn, mx, axis, ha, radius, x, d_min)) &&
                !face_pruned(mn, mx, axis, rho, elo, ehi, x);
     } else {
-        float d_max;
-        box_dist(mn, mx, x, d_min, d_max);
         if (!(d_min <= r_max)) return false;
         if (face_pruned(mn, mx, axis, rho, elo, ehi, x)) return false;
         if (!(ha >= 0.0f)) return true;
@@ -973,8 +979,14 @@ DWD bool trav3_iter(Trav& t, Ovf& o) {
             if (sn) cn = gload<RSnchTail3>(rec + sizeof(RHead3), 0);
             F3 lmn = f3(rc.lmnx, rc.lmny, rc.lmnz), lmx = f3(rc.lmxx, rc.lmxy, rc.lmxz);
             F3 rmn = f3(rc.rmnx, rc.rmny, rc.rmnz), rmx = f3(rc.rmxx, rc.rmxy, rc.rmxz);
-            float d0, d1;
+            float d0, d1, dmaxL, dmaxR;
             bool h0, h1, sw;  // sw: right child first
+            // Box distances serve both the silhouette and the closest-point
+            // kinds: one copy for the lanes of both.
+            if (kind != TQ_ISECT) {
+                box_dist(lmn, lmx, t.x, d0, dmaxL);
+                box_dist(rmn, rmx, t.x, d1, dmaxR);
+            }
             if (sn) {
                 h0 = snch3_child<X>(lmn, lmx, f3(cn.lax, cn.lay, cn.laz), rc.lha, cn.lsin, cn.lcos, cn.lrad, cn.lrho,
                                     cn.lelo, cn.lehi, t.x, t.r_max, d0);
@@ -982,11 +994,8 @@ DWD bool trav3_iter(Trav& t, Ovf& o) {
                                     cn.relo, cn.rehi, t.x, t.r_max, d1);
                 sw = d1 < d0;
             } else if (kind == TQ_CLOSEST) {
-                float dmaxL, dmaxR;
-                box_dist(lmn, lmx, t.x, d0, dmaxL);
                 h0 = d0 <= t.r_max;
                 t.r_max = fmn(t.r_max, dmaxL);
-                box_dist(rmn, rmx, t.x, d1, dmaxR);
                 h1 = d1 <= t.r_max;
                 t.r_max = fmn(t.r_max, dmaxR);
                 // snch3d.py:383-388: equal zero d_min -> nearer d_max first; then d_min order

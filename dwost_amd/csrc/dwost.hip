@@ -2357,7 +2357,7 @@ int dw_batch_solve(dw_batch* b, void* stream) {
     }
     if (!b->use_graph) return batch_launch(b, st);
     if (!b->exec) {
-        DW_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+        if (!b->cap) DW_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
         DW_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeRelaxed));
         int rc = batch_launch(b, b->cap);
         hipGraph_t g = nullptr;

@@ -107,8 +107,9 @@ class OracleScene:
         self.h = lib().orc_scene_create(self.dim, _p(self.V), len(self.V), _p(self.I), _p(self.T), len(self.I))
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().orc_scene_destroy(self.h)
+        # at interpreter exit the module globals may already be gone
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_scene_destroy(self.h)
             self.h = None
 
     def info(self, sub: int) -> dict:

@@ -71,7 +71,7 @@ def test_fused_unbiased(oracle, case):
     pg = f["pair_g"].reshape(len(q), P, spec.dim).astype(np.float64) / 2
     zu = _z(pu, scenes.analytic_u(code, q))
     ga = scenes.analytic_grad(code, q)
-    # grad u: (dim / R) u dir has variance ~ 1/R^2; judge the points off the shell
+    # grad u: (dim / R) u dir has variance ~ 1/R^2; score the points off the shell
     far = sc.query(oracle.SUB_FULL, oracle.KIND_CLOSEST, q)["d"] > 0.05
     # a point passes when every component of its gradient lies within 3 sigma
     ok = np.all(np.abs(_z(pg[far], ga[far])) < 3, axis=1)

@@ -204,8 +204,8 @@ def main():
         print("[bench] copied 3 x 256 MiB", file=sys.stderr, flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: every box of this build has a single MI355X "
-              "(multi-GPU sharding is a design note, not a measured path)", file=sys.stderr)
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} asked): the walks shard by point "
+              "range across GPUs, but that path is not implemented or measured here", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -2132,8 +2132,8 @@ static float wos_dude(const Scene& sc, const BC& bc, const Params& P, const floa
     return result / (float)P.nwalks;
 }
 
-// Fused u + grad u estimator (this build's throughput form; not a reference
-// function): per antithetic pair k of point i, a PCG32 stream seeded like
+// Fused u + grad u estimator (the product's throughput form of u and the
+// first-ball gradient; no single reference function): per antithetic pair k of point i, a PCG32 stream seeded like
 // Solver.solve with lane = i*npairs + k; direction (and the source radius) are
 // drawn as in WoSt.dude / WoSt3D.dude, the state is saved, and the two walks
 // from p +- R*dir replay the same stream.

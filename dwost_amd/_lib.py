@@ -71,7 +71,7 @@ def lib():
     L.dw_scene_info.argtypes = [vp, i32, vp]
     L.dw_scene_export.argtypes = [vp, i32, vp, vp, vp, vp, vp]
     L.dw_closest_point.argtypes = [vp, i32, vp, i64, vp, vp, vp, vp, vp, vp]
-    L.dw_intersect.argtypes = [vp, i32, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp]
+    L.dw_intersect.argtypes = [vp, i32, vp, vp, vp, i64, i32, vp, vp, vp, vp, vp, vp]
     L.dw_closest_silhouette.argtypes = [vp, i32, vp, vp, i64, vp, vp, vp, vp]
     L.dw_star_radius.argtypes = [vp, i32, vp, vp, i64, vp, vp]
     L.dw_solve_u.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp, vp, vp, vp, vp]

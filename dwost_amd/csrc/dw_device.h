@@ -514,6 +514,61 @@ DWD Hit intersect2(const SubDev& S, F2 x, F2 v, float r_max) {
     return isect2_end(t);
 }
 
+// Nearest-hit form of the 2D ray query: the answer of the reference's
+// brute-force Polyline.intersect_baseline (polyline.py:468-496), found through
+// the BVH the way SNCH3D.intersect does in 3D (snch3d.py:435): each hit below
+// the running r_max becomes the new r_max, so farther nodes are pruned and the
+// first of equally near hits in traversal order is kept.
+DWD bool isect2n_iter(Trav& t, Ovf& o) {
+    const SubDev& S = *t.S;
+    F2 x = f2(t.x.x, t.x.y), v = f2(t.v.x, t.v.y);
+    if (t.cd <= t.r_max) {
+        if (t.ref < 0) {
+            int off, cnt;
+            leaf_range(S, t.ref, off, cnt);
+            for (int j = 0; j < cnt; ++j) {
+                float s_, t_;
+                if (seg_hit(gload<DSeg>(t.prims, off + j), x, v, t.r_max, s_, t_)) {
+                    t.r_max = t_;
+                    t.best = off + j;
+                }
+            }
+        } else {
+            RBvh2 rc = gload<RBvh2>(t.rec, t.ref);
+            float t0, t1;
+            bool h0 = box_ray(f2(rc.lmnx, rc.lmny), f2(rc.lmxx, rc.lmxy), x, v, t.r_max, t0);
+            bool h1 = box_ray(f2(rc.rmnx, rc.rmny), f2(rc.rmxx, rc.rmxy), x, v, t.r_max, t1);
+            if (h0 && h1) {
+                bool sw = t1 < t0;
+                stk_push(t.sp, o, sw ? rc.ref_l : rc.ref_r, sw ? t0 : t1);
+                t.ref = sw ? rc.ref_r : rc.ref_l;
+                t.cd = sw ? t1 : t0;
+                return false;
+            } else if (h0) {
+                t.ref = rc.ref_l;
+                t.cd = t0;
+                return false;
+            } else if (h1) {
+                t.ref = rc.ref_r;
+                t.cd = t1;
+                return false;
+            }
+        }
+    }
+    if (t.sp == 0) return true;
+    stk_pop(t.sp, o, t.ref, t.cd);
+    return false;
+}
+DWD Hit intersect2_nearest(const SubDev& S, F2 x, F2 v, float r_max) {
+    Trav t;
+    Ovf o;
+    if (isect2_begin(t, S, x, v, r_max))
+        while (!isect2n_iter(t, o)) {
+        }
+    t.r_max = r_max;  // the record re-tests the winner against the caller's r_max
+    return isect2_end(t);
+}
+
 
 // project_to_plane (bvh.py:16-20, snch3d.py:30-45)
 DWD float project_to_plane(F2 n, F2 e) { return __builtin_fabsf(dot(e, vabs(f2(-n.y, n.x)))); }

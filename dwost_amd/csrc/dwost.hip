@@ -72,15 +72,17 @@ __global__ void __launch_bounds__(kBlock) k_closest(SceneDev sc, int sub, const 
 
 __global__ void __launch_bounds__(kBlock) k_intersect(SceneDev sc, int sub, const float* __restrict__ org,
                                                       const float* __restrict__ dir, const float* __restrict__ rmax,
-                                                      int64_t n, int* prim, float* dist, float* point, float* normal,
-                                                      float* tuv) {
+                                                      int64_t n, int mode, int* prim, float* dist, float* point,
+                                                      float* normal, float* tuv) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const SubDev& S = sc.sub[sub];
     int D = sc.dim;
     Hit h;
     if (D == 2)
-        h = intersect2(S, f2(org[2 * q], org[2 * q + 1]), f2(dir[2 * q], dir[2 * q + 1]), rmax[q]);
+        h = mode == DW_RAY_NEAREST
+                ? intersect2_nearest(S, f2(org[2 * q], org[2 * q + 1]), f2(dir[2 * q], dir[2 * q + 1]), rmax[q])
+                : intersect2(S, f2(org[2 * q], org[2 * q + 1]), f2(dir[2 * q], dir[2 * q + 1]), rmax[q]);
     else
         h = intersect3(S, f3(org[3 * q], org[3 * q + 1], org[3 * q + 2]),
                        f3(dir[3 * q], dir[3 * q + 1], dir[3 * q + 2]), rmax[q]);
@@ -1942,13 +1944,14 @@ int dw_closest_point(const dw_scene* s, int32_t sub, const float* pts, int64_t n
 }
 
 int dw_intersect(const dw_scene* s, int32_t sub, const float* origin, const float* dir, const float* rmax, int64_t n,
-                 int32_t* prim, float* dist, float* point, float* normal, float* tuv, void* stream) {
+                 int32_t mode, int32_t* prim, float* dist, float* point, float* normal, float* tuv, void* stream) {
     int rc = check_scene(s, sub);
     if (rc) return rc;
+    if (mode != DW_RAY_REFERENCE && mode != DW_RAY_NEAREST) return fail(DW_ERR_ARG, "ray mode must be 0 or 1");
     if (n < 0 || (n > 0 && (!origin || !dir || !rmax))) return fail(DW_ERR_ARG, "bad ray arrays");
     if (n == 0) return DW_OK;
     hipLaunchKernelGGL(k_intersect, dim3(blocks_for(n)), dim3(kBlock), kStackLds, (hipStream_t)stream, s->dev, sub, origin,
-                       dir, rmax, n, prim, dist, point, normal, tuv);
+                       dir, rmax, n, mode, prim, dist, point, normal, tuv);
     DW_HIP(hipGetLastError());
     return DW_OK;
 }

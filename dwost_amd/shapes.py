@@ -184,8 +184,13 @@ class _Scene:
     def distance(self, p, sub: int = SUB_FULL) -> torch.Tensor:
         return self.closest_point(p, sub)["d"]
 
-    def intersect(self, origin, direction, r_max, sub: int = SUB_FULL) -> dict:
-        """BVH.intersect (bvh.py:190-257) / SNCH3D.intersect (snch3d.py:411-476); prim = -1 on a miss."""
+    def intersect(self, origin, direction, r_max, sub: int = SUB_FULL, nearest: bool = False) -> dict:
+        """BVH.intersect (bvh.py:190-257) / SNCH3D.intersect (snch3d.py:411-476); prim = -1 on a miss.
+
+        2D rays return the first valid hit in traversal order, as the reference's
+        BVH does (bvh.py:215-217); ``nearest=True`` returns the nearest hit
+        (Polyline.intersect_baseline, polyline.py:468-496).  3D rays are nearest
+        either way."""
         o = _as_points(origin, self.dim)
         v = _as_points(direction, self.dim)
         n = o.shape[0]
@@ -194,8 +199,8 @@ class _Scene:
         out = dict(prim=torch.empty(n, dtype=torch.int32, device=dev), d=torch.empty(n, device=dev),
                    p=torch.empty(n, self.dim, device=dev), n=torch.empty(n, self.dim, device=dev),
                    tuv=torch.empty(n, 2, device=dev))
-        check(lib().dw_intersect(self._h, sub, ptr(o), ptr(v), ptr(r), n, ptr(out["prim"]), ptr(out["d"]),
-                                 ptr(out["p"]), ptr(out["n"]), ptr(out["tuv"]), stream_handle(dev)))
+        check(lib().dw_intersect(self._h, sub, ptr(o), ptr(v), ptr(r), n, int(bool(nearest)), ptr(out["prim"]),
+                                 ptr(out["d"]), ptr(out["p"]), ptr(out["n"]), ptr(out["tuv"]), stream_handle(dev)))
         return out
 
     def closest_silhouette(self, p, r_max=float("inf"), sub: int = SUB_FULL) -> dict:

@@ -150,11 +150,18 @@ int dw_scene_export(const dw_scene* scene, int32_t sub, float* box, int32_t* int
 int dw_closest_point(const dw_scene* scene, int32_t sub, const float* pts, int64_t n, int32_t* prim, float* dist,
                      float* point, float* normal, float* tuv, void* stream);
 
+/* ray query modes of dw_intersect */
+#define DW_RAY_REFERENCE 0 /* the reference's BVH query: 2D first valid hit in traversal order (bvh.py:215-217,
+                              what the walks use), 3D nearest hit                                              */
+#define DW_RAY_NEAREST 1   /* nearest hit in 2D too: the answer of Polyline.intersect_baseline
+                              (polyline.py:468-496), through the BVH; 3D is unchanged                          */
+
 /* BVH.intersect (bvh.py:190-257, first hit in traversal order) /
  * SNCH3D.intersect (snch3d.py:411-476, nearest hit; prim = BVH-sorted index).
- * origin[n][dim], dir[n][dim], rmax[n]; prim[n] = -1 on a miss. */
+ * origin[n][dim], dir[n][dim], rmax[n]; mode: DW_RAY_*; prim[n] = -1 on a miss. */
 int dw_intersect(const dw_scene* scene, int32_t sub, const float* origin, const float* dir, const float* rmax,
-                 int64_t n, int32_t* prim, float* dist, float* point, float* normal, float* tuv, void* stream);
+                 int64_t n, int32_t mode, int32_t* prim, float* dist, float* point, float* normal, float* tuv,
+                 void* stream);
 
 /* SNCH.closest_silhouette (snch.py:312-378, snch3d.py:594-663).  sil[n] is the
  * silhouette reference index (-1: none within rmax). */

@@ -110,6 +110,38 @@ def test_queries_bit_exact(gpu, oracle, which, star_spec, holes_spec, ico_spec):
             assert _same_bits(_np(g[k])[hit], ok_), ("ray", k, rmax)
 
 
+@pytest.mark.parametrize("which", ["star", "holes"])
+def test_ray_nearest_2d_vs_brute_force(gpu, oracle, which, star_spec, holes_spec):
+    """DW_RAY_NEAREST: the nearest hit (Polyline.intersect_baseline,
+    polyline.py:468-496), checked against the brute-force loop over all
+    segments with long rays that cross several boundaries."""
+    spec = dict(star=star_spec, holes=holes_spec)[which]
+    gs, os_ = _scenes(gpu, oracle, spec)
+    q = spec.queries[:2048]
+    v = _dirs(len(q), 2, 21)
+    rm = np.full(len(q), 10.0, np.float32)
+    g = gs.intersect(_t(q), _t(v), _t(rm), sub=0, nearest=True)
+    f = gs.intersect(_t(q), _t(v), _t(rm), sub=0)
+    b = os_.query(0, oracle.KIND_BRUTE_RAY, q, v, rm)
+    gp, gd, fd = _np(g["prim"]), _np(g["d"]), _np(f["d"])
+    hit = b["prim"] >= 0
+    assert hit.mean() > 0.99
+    assert np.array_equal(gp >= 0, hit)
+    # the same segment, or an equally near one (a ray through a shared vertex)
+    same = (gp == b["prim"]) | np.isclose(gd, b["d"], rtol=1e-6, atol=0.0)
+    assert same[hit].all()
+    assert np.allclose(gd[hit], b["d"][hit], rtol=1e-5)
+    # the hit point lies on the ray at the reported distance
+    assert np.allclose(_np(g["p"])[hit], q[hit] + v[hit] * gd[hit, None], atol=1e-5)
+    # the reference mode returns a valid hit that is never nearer
+    assert np.all(fd[hit] >= gd[hit] * (1 - 1e-6))
+    print(f"{which}: first hit in traversal order farther than the nearest for "
+          f"{np.mean(fd[hit] > gd[hit] * (1 + 1e-4)):.4f} of the rays")
+    # an unknown mode is an argument error (checked before any launch)
+    rc = gpu._lib.lib().dw_intersect(gs.handle, 0, None, None, None, 0, 2, None, None, None, None, None, None)
+    assert rc != 0
+
+
 def _walk_parity(gpu, oracle, spec, n, W, nsteps, eps, solver_cls, exact_cones=True, **kw):
     gs, os_ = _scenes(gpu, oracle, spec)
     gs.boundary = gpu.BoundaryCondition(spec.bc_code)

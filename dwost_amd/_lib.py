@@ -45,6 +45,7 @@ class SolverParams(C.Structure):
 
 SOLVE_EXACT_CONES = 1
 SOLVE_WOS = 2
+RAY_REFERENCE, RAY_NEAREST = 0, 1  # dw_intersect modes
 
 
 class BatchItem(C.Structure):

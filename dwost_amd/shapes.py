@@ -199,8 +199,9 @@ class _Scene:
         out = dict(prim=torch.empty(n, dtype=torch.int32, device=dev), d=torch.empty(n, device=dev),
                    p=torch.empty(n, self.dim, device=dev), n=torch.empty(n, self.dim, device=dev),
                    tuv=torch.empty(n, 2, device=dev))
-        check(lib().dw_intersect(self._h, sub, ptr(o), ptr(v), ptr(r), n, int(bool(nearest)), ptr(out["prim"]),
-                                 ptr(out["d"]), ptr(out["p"]), ptr(out["n"]), ptr(out["tuv"]), stream_handle(dev)))
+        mode = _lib.RAY_NEAREST if nearest else _lib.RAY_REFERENCE
+        check(lib().dw_intersect(self._h, sub, ptr(o), ptr(v), ptr(r), n, mode, ptr(out["prim"]), ptr(out["d"]),
+                                 ptr(out["p"]), ptr(out["n"]), ptr(out["tuv"]), stream_handle(dev)))
         return out
 
     def closest_silhouette(self, p, r_max=float("inf"), sub: int = SUB_FULL) -> dict:

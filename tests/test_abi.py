@@ -48,3 +48,44 @@ def test_host_only_scene_rejects_queries():
 
     rc = _lib.lib().dw_closest_point(L, 0, None, 0, None, None, None, None, None, None)
     assert rc == -1 and b"HOST_ONLY" in _lib.lib().dw_last_error()
+
+
+def test_ctypes_structs_and_constants_match_header(tmp_path):
+    """The ctypes mirror in dwost_amd/_lib.py has the C layout of every struct
+    (field offsets and sizes as gcc lays them out) and the header's constants."""
+    import ctypes as C
+
+    from dwost_amd import _lib, shapes
+
+    structs = {"dw_scene_desc": _lib.SceneDesc, "dw_boundary": _lib.Boundary, "dw_solver_params": _lib.SolverParams,
+               "dw_batch_item": _lib.BatchItem}
+    consts = {"DW_OK": _lib.DW_OK, "DW_SCENE_HOST_ONLY": _lib.SCENE_HOST_ONLY,
+              "DW_SOLVE_EXACT_CONES": _lib.SOLVE_EXACT_CONES, "DW_SOLVE_WOS": _lib.SOLVE_WOS,
+              "DW_BATCH_GRAPH": _lib.BATCH_GRAPH, "DW_SUB_FULL": _lib.SUB_FULL, "DW_SUB_DIRICHLET": _lib.SUB_DIRICHLET,
+              "DW_SUB_NEUMANN": _lib.SUB_NEUMANN, "DW_BC_EXPCOS": shapes.BC_EXPCOS, "DW_BC_QUAD": shapes.BC_QUAD,
+              "DW_BC_LINEAR": shapes.BC_LINEAR, "DW_BC_PER_VERTEX": shapes.BC_PER_VERTEX,
+              "DW_SRC_ZERO": shapes.SRC_ZERO, "DW_SRC_CONST": shapes.SRC_CONST,
+              "DW_RAY_REFERENCE": _lib.RAY_REFERENCE, "DW_RAY_NEAREST": _lib.RAY_NEAREST}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "dwost.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    for k in consts:
+        lines.append(f'printf("const {k} %d\\n", (int)({k}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.dirname(HEADER), str(src), "-o",
+                           str(exe)])
+    got = {}
+    for line in subprocess.check_output([str(exe)], text=True).splitlines():
+        a, b, v = line.split()
+        got[(a, b)] = int(v)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+    for k, v in consts.items():
+        assert got[("const", k)] == v, k

@@ -55,6 +55,7 @@ def lib():
         L.orc_pcg_lane_floats.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_void_p]
         L.orc_math.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
         L.orc_num_threads.restype = C.c_int
+        L.orc_cv_bound_check.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
     return _lib
 
 
@@ -143,6 +144,16 @@ class OracleScene:
         tuv = np.zeros((n, 2), np.float32)
         lib().orc_query(self.h, sub, kind, _p(pts), _p(dirs), _p(rmax), n, _p(prim), _p(d), _p(p), _p(nrm), _p(tuv))
         return dict(prim=prim, d=d, p=p, n=nrm, tuv=tuv)
+
+    def cv_bound_check(self, pts):
+        """2D: the full-scene closest point queried with its pruning radius
+        started just above the Dirichlet distance, against the unbounded query.
+        Returns (per-point bitwise agreement, nodes visited unbounded, bounded)."""
+        pts = _f32(pts)
+        same = np.zeros(len(pts), np.int32)
+        nodes = np.zeros(2, np.uint64)
+        lib().orc_cv_bound_check(self.h, _p(pts), len(pts), _p(same), _p(nodes))
+        return same.astype(bool), int(nodes[0]), int(nodes[1])
 
     def ref_topology(self, sub: int, ref: int) -> int:
         return lib().orc_ref_topology(self.h, sub, ref)

@@ -907,7 +907,9 @@ static inline void seg_closest(const Prim<2>& s, V2 p, CP& r) {
 }
 
 // BVH.closest_point (bvh.py:111-188)
-static CP bvh2_closest(const Tree<2>& T, V2 p, Cnt* c) {
+// r0 < inf starts the pruning radius below infinity (a bound the caller knows
+// on the answer); the reference query is r0 = inf.
+static CP bvh2_closest(const Tree<2>& T, V2 p, Cnt* c, float r0 = kInf) {
     CP its;
     its.p = v3(p.x, p.y, 0.0f);
     its.n = v3(0.0f, 0.0f, 0.0f);
@@ -915,12 +917,12 @@ static CP bvh2_closest(const Tree<2>& T, V2 p, Cnt* c) {
     its.u = its.v = 0.0f;
     its.d = kInf;
     its.prim = -1;
-    float r_max = kInf;
+    float r_max = r0;
     int st_i[64];
     float st_d[64];
     int sp = 0;
     st_i[0] = 0;
-    st_d[0] = r_max;
+    st_d[0] = r0 < kInf ? 0.0f : r_max;
     while (sp >= 0) {
         int ni = st_i[sp];
         float cd = st_d[sp];
@@ -2284,6 +2286,37 @@ void orc_sub_export(void* s, int sub, float* box, int32_t* ints, float* cone, in
 // Queries.  kind: 0 BVH closest point, 1 ray intersect (BVH), 2 closest silhouette,
 // 3 star radius, 10/11/12 brute force closest point / ray / silhouette.
 // in: pts[n*dim], dirs[n*dim] (ray), rmax[n].  out: prim[n], d[n], p[n*dim], nrm[n*dim], tuv[n*2]
+// The 2D boundary control variate's full-scene closest point (wost.py:123-126)
+// queried with its pruning radius started at the Dirichlet distance of the same
+// point, widened by relative 2^-16 plus 1e-6 of the coordinate scale (the root
+// box), against the unbounded reference query: same[i] = 1 when the two records
+// agree bit for bit; nodes[0..1] = nodes visited (unbounded, bounded).
+void orc_cv_bound_check(void* s, const float* pts, int64_t n, int32_t* same, uint64_t* nodes) {
+    Scene* sc = (Scene*)s;
+    const Tree<2>& F = sc->s2[0].tree;
+    const Node<2>& root = F.nodes[0];
+    V2 mn = node_min2(root), mx = node_max2(root);
+    float scale = fmaxf_(fmaxf_(std::fabs(mn.x), std::fabs(mx.x)), fmaxf_(std::fabs(mn.y), std::fabs(mx.y)));
+    uint64_t n0 = 0, n1 = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : n0, n1)
+    for (int64_t i = 0; i < n; ++i) {
+        V2 p = v2(pts[2 * i], pts[2 * i + 1]);
+        float dd = kInf;
+        if (sc->s2[1].present) dd = std::fabs(bvh2_closest(sc->s2[1].tree, p, nullptr).d);
+        float r0 = dd < kInf ? dd * (1.0f + 0x1p-16f) + 1e-6f * (1.0f + scale) : kInf;
+        Cnt ca, cb;
+        CP a = bvh2_closest(F, p, &ca);
+        CP b = bvh2_closest(F, p, &cb, r0);
+        bool eq = a.prim == b.prim && std::memcmp(&a.d, &b.d, 4) == 0 && std::memcmp(&a.t, &b.t, 4) == 0 &&
+                  std::memcmp(&a.p, &b.p, sizeof(a.p)) == 0 && std::memcmp(&a.n, &b.n, sizeof(a.n)) == 0;
+        same[i] = eq ? 1 : 0;
+        n0 += ca.nodes;
+        n1 += cb.nodes;
+    }
+    nodes[0] = n0;
+    nodes[1] = n1;
+}
+
 void orc_query(void* s, int sub, int kind, const float* pts, const float* dirs, const float* rmax, int64_t n,
                int32_t* prim, float* d, float* p, float* nrm, float* tuv) {
     Scene* sc = (Scene*)s;

@@ -123,3 +123,33 @@ def test_star_radius_bounds(oracle, star_spec):
     has = s["prim"] >= 0
     assert np.array_equal(R[has], s["d"][has])
     assert np.all(R[~has] == np.float32(0.3))
+
+
+@pytest.mark.parametrize("which", ["star", "holes"])
+def test_cv_closest_point_bounded_by_dirichlet_distance(oracle, which, star_spec, holes_spec):
+    """The 2D walk's control-variate closest point (full scene, wost.py:123-126)
+    is no farther than the Dirichlet distance found at the same point.  Started
+    with its pruning radius just above that distance (relative 2^-16 plus 1e-6
+    of the coordinate scale) the traversal only skips subtrees farther than the
+    answer, so the record, including which of equally near segments wins, is
+    the unbounded query's bit for bit: checked on interior points, points on and
+    beside the boundary, and points around the vertices where segments meet."""
+    spec = dict(star=star_spec, holes=holes_spec)[which]
+    sc = oracle.OracleScene(spec.vertices, spec.indices, spec.types)
+    rng = np.random.default_rng(17)
+    V, I = spec.vertices.astype(np.float64), spec.indices
+    f = rng.integers(0, len(I), 4096)
+    t = rng.choice([0.0, 1.0, 0.5, 1e-6, 1 - 1e-6], 4096)[:, None] * np.ones((1, 1))
+    t = np.where(rng.random((4096, 1)) < 0.5, t, rng.random((4096, 1)))
+    a, b = V[I[f, 0]], V[I[f, 1]]
+    n = np.stack([(b - a)[:, 1], -(b - a)[:, 0]], 1)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    off = rng.choice([0.0, 1e-6, -1e-6, 1e-4, -1e-4, 1e-2, -1e-2], 4096)[:, None]
+    near = a + (b - a) * t + off * n
+    corner = V[rng.integers(0, len(V), 2048)] + rng.normal(0, 1e-3, (2048, 2))
+    pts = np.concatenate([spec.queries[:4096].astype(np.float64), near, corner]).astype(np.float32)
+    same, n_full, n_bound = sc.cv_bound_check(pts)
+    print(f"{which}: {same.mean():.6f} identical; nodes visited {n_full} -> {n_bound} "
+          f"({n_bound / n_full:.3f})")
+    assert same.all()
+    assert n_bound < n_full

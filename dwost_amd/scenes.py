@@ -144,6 +144,36 @@ def _interior_2d(pts, V, I, clearance):
     return ins & (d > clearance)
 
 
+def _interior_lattice_2d(xs, ys, V, I, clearance):
+    """_interior_2d for the lattice xs x ys (float64 values already rounded to
+    float32), row by row in meshgrid "xy" order, with the same per-segment float64
+    arithmetic: a row's crossings come from the segments that straddle its y
+    (counted by a sorted search), and its distances from the segments whose y
+    range lies within the clearance (plus 1e-9) of it, the only ones that can be
+    that near.  Returns exactly what _interior_2d returns on those points."""
+    a, b = V[I[:, 0]], V[I[:, 1]]
+    ba = b - a
+    bb = np.maximum((ba * ba).sum(1), 1e-300)
+    ylo, yhi = np.minimum(a[:, 1], b[:, 1]), np.maximum(a[:, 1], b[:, 1])
+    band = clearance + 1e-9
+    out = np.empty((len(ys), len(xs)), dtype=bool)
+    for r, py in enumerate(ys):
+        ay, by, ax, bx = a[:, 1], b[:, 1], a[:, 0], b[:, 0]
+        cond = (ay > py) != (by > py)
+        xi = np.sort(ax[cond] + (py - ay[cond]) * (bx[cond] - ax[cond]) / (by[cond] - ay[cond]))
+        inside = ((len(xi) - np.searchsorted(xi, xs, side="right")) % 2) == 1
+        near = (ylo - band <= py) & (py <= yhi + band)
+        far = np.ones(len(xs), dtype=bool)
+        if near.any():
+            p = np.stack([xs, np.full(len(xs), py)], 1)[:, None, :]
+            pa = p - a[near][None]
+            h = np.clip((pa * ba[near][None]).sum(2) / bb[near][None], 0.0, 1.0)
+            d = pa - ba[near][None] * h[..., None]
+            far = np.sqrt((d * d).sum(2)).min(1) > clearance
+        out[r] = inside & far
+    return out.ravel()
+
+
 def star_scene_2d(n_seg: int = 256, n_query: int = 1024, eps: float = 1e-3, scene_seed: int = 1,
                   query_seed: int = 2) -> SceneSpec:
     """Star polygon r = 1 + 0.3 sin(5 theta + phi) + 0.02 jitter; x<0 midpoints Dirichlet."""
@@ -200,7 +230,8 @@ def holes_scene_2d(n_outer: int = 4096, n_holes: int = 8, hole_seg: int = 64, n_
             ys = np.linspace(lo[1], hi[1], k + 2)[1:-1]
             X, Y = np.meshgrid(xs, ys, indexing="xy")
             cand = np.stack([X.ravel(), Y.ravel()], 1).astype(np.float32).astype(np.float64)
-            ok = _interior_2d(cand, Vf, I, 2 * eps)
+            ok = _interior_lattice_2d(xs.astype(np.float32).astype(np.float64),
+                                      ys.astype(np.float32).astype(np.float64), Vf, I, 2 * eps)
             if ok.sum() >= n_query:
                 pts = cand[ok][:n_query]
                 break

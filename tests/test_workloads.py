@@ -1,7 +1,8 @@
 """The benchmark workloads (bench.py CONFIGS) have the frozen sizes and rules:
 segment / triangle counts, the Dirichlet / Neumann split, query counts, and
-queries inside the domain with their clearance.  The row-wise lattice interior
-test used for the holes workload returns exactly the dense test's mask."""
+queries inside the domain with their clearance.  The row-wise lattice and the
+binned interior tests that build the query sets return exactly the dense test's
+mask."""
 
 import numpy as np
 import pytest
@@ -29,6 +30,26 @@ def test_lattice_interior_matches_dense_test():
                 dense = S._interior_2d(cand, V, spec.indices, clearance)
                 rows = S._interior_lattice_2d(xs, ys, V, spec.indices, clearance)
                 assert np.array_equal(dense, rows), (spec.meta["name"], k, clearance)
+
+
+def test_binned_interior_matches_dense_test():
+    """Random points, points on and beside segments and at vertices: the binned
+    test used for random query sets returns the dense test's mask."""
+    from dwost_amd import scenes as S
+
+    rng = np.random.default_rng(4)
+    for spec in (S.star_scene_2d(n_query=64), S.holes_scene_2d(n_outer=1024, n_holes=3, n_query=256)):
+        V = spec.vertices.astype(np.float64)
+        I = spec.indices
+        lo, hi = V.min(0), V.max(0)
+        for clearance in (2e-4, 2e-3, 5e-2):
+            f = rng.integers(0, len(I), 3000)
+            a, b = V[I[f, 0]], V[I[f, 1]]
+            t = rng.choice([0.0, 1.0, 0.5, 0.25], 3000)[:, None]
+            pts = np.concatenate([rng.uniform(lo - 0.05, hi + 0.05, (8000, 2)),
+                                  a + (b - a) * t + rng.normal(0, clearance, (3000, 2)), V])
+            pts = pts.astype(np.float32).astype(np.float64)
+            assert np.array_equal(S._interior_2d(pts, V, I, clearance), S._interior_binned_2d(pts, V, I, clearance))
 
 
 def test_star_workload(bench_mod):

@@ -1,12 +1,21 @@
 """Walk-on-Stars benchmark on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config holes|star|ico]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config ico|holes|star]
+    python bench.py --alg-counts [--config ...]  # recount the algorithmic bytes / flops per walk step (CPU)
+    python bench.py --copy-calibration           # the copy that scales the FETCH_SIZE / WRITE_SIZE counters
 
-A step is one fused u + grad u solve over the whole query set of the chosen
-workload (default: 2D star polygon with 4096 Dirichlet segments and eight
-64-segment Neumann holes, 65,536 lattice queries, 1024 walks per point,
-eps 1e-4, at most 1024 steps per walk, g = x^2 - y^2).  Inputs are resident in
-HBM before timing starts.  Prints one JSON line.
+Headline workload (default ``ico``): the noisy icosphere (81,920 triangles,
+faces with centroid y > 0 Dirichlet, g = e^x cos y, exact Neumann data on the
+rest), 262,144 interior query points, 512 walks per point, eps 1e-3, at most
+1024 steps per walk, fp32, fused u + grad u estimator.  A step is one solve of
+one quarter of the query set (65,536 points, passed with point_base = its
+offset, so their walk streams are those of the whole-set solve); the seed
+changes every 4 steps, so each 4 consecutive steps are one complete solve of
+all 262,144 points, bit-identical to a single launch over all of them.
+Inputs are resident in HBM before timing starts.  Prints one JSON line; its
+extra keys measure the 2D holes workload (4096-segment star with eight
+64-segment Neumann holes, 65,536 lattice points x 1024 walks, eps 1e-4), the
+3D adjoint (fwd+bwd / fwd) and the 64-scene batch.
 """
 
 from __future__ import annotations
@@ -20,46 +29,82 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Algorithmic bytes per walk step, counted by the instrumented CPU restatement
-# in the reference traversal order with the reference SoA record sizes
-# (2D BVH node 24 B, 2D SNCH node 48 B, 3D SNCH node 60 B, segment 20 B,
-# triangle 40 B, silhouette vertex 40 B, silhouette edge 68 B, CDF gathers
-# 4 B per binary-search halving + 8 B) on 64 seeded points of each workload.
-B_ALG = {"star": 3350.8, "holes": 4641.3, "ico": 39510.1}
+# Algorithmic bytes and fp32 operations per walk step of the fused estimator,
+# counted by the instrumented CPU restatement in the reference traversal order
+# with the reference SoA record sizes (2D BVH node 24 B, 2D SNCH node 48 B, 3D
+# SNCH node 60 B, segment 20 B, triangle 40 B, silhouette vertex 40 B,
+# silhouette edge 68 B, CDF gathers 4 B per binary-search halving + 8 B) on
+# 1024 seeded points of each workload at its walk count:
+#     python bench.py --alg-counts --config <name>
+# Steps from a NaN position (3D walks that left the domain) fetch nothing and
+# count with 0 bytes; ff_share is their share of all steps.
+ALG = {
+    "star": dict(b_alg=3397.8, f_alg=2199.8, steps_per_walk=84.46, ff_share=0.0),
+    "holes": dict(b_alg=4650.5, f_alg=2920.8, steps_per_walk=41.61, ff_share=0.0),
+    "ico": dict(b_alg=40593.4, f_alg=28821.9, steps_per_walk=26.69, ff_share=5.5e-05),
+}
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 
 CONFIGS = {
-    "star": dict(workload="star polygon 256 segments (mixed D/N), g=e^x cos y", n_query=1024, nwalks=256, eps=1e-3),
-    "holes": dict(workload="star polygon 4096 segments + 8 Neumann holes x 64, g=x^2-y^2, 256x256-lattice queries",
-                  n_query=65536, nwalks=1024, eps=1e-4),
-    "ico": dict(workload="noisy icosphere 81920 triangles, y>0 Dirichlet, g=e^x cos y", n_query=262144, nwalks=512,
-                eps=1e-3),
+    "star": dict(workload="star polygon 256 segments (mixed D/N), g=e^x cos y, 1024 interior points", n_query=1024,
+                 nwalks=256, eps=1e-3, shards=1),
+    "holes": dict(workload="star polygon 4096 segments + 8 Neumann holes x 64, g=x^2-y^2, 256x256-lattice points",
+                  n_query=65536, nwalks=1024, eps=1e-4, shards=1),
+    "ico": dict(workload="noisy icosphere 81920 triangles (level 6), centroid y>0 Dirichlet, g=e^x cos y, "
+                         "262144 interior points", n_query=262144, nwalks=512, eps=1e-3, shards=4),
 }
 
 
-def make_spec(name: str):
+def make_spec(name: str, n_query: int | None = None):
     from dwost_amd import scenes
 
     c = CONFIGS[name]
+    n = n_query or c["n_query"]
     if name == "star":
-        return scenes.star_scene_2d(n_query=c["n_query"], eps=c["eps"])
+        return scenes.star_scene_2d(n_query=n, eps=c["eps"])
     if name == "holes":
-        return scenes.holes_scene_2d(n_query=c["n_query"], eps=c["eps"])
-    return scenes.icosphere_scene(level=6, n_query=c["n_query"], eps=c["eps"])
+        return scenes.holes_scene_2d(n_query=n, eps=c["eps"])
+    return scenes.icosphere_scene(level=6, n_query=n, eps=c["eps"])
 
 
 def profile_figures(name: str) -> dict:
     """Figures of the committed rocprofv3 summary of this workload's kernel
     (profiles/k_fused_pers_<name>.md): HBM traffic per walk step from
-    FETCH_SIZE / WRITE_SIZE (copy-calibrated), L2 hit rate, VALU busy."""
+    FETCH_SIZE / WRITE_SIZE (copy-calibrated), L2 hit rate, VALU busy and the
+    active-lane fraction of issued VALU instructions."""
     path = os.path.join(ROOT, "profiles", f"k_fused_pers_{name}.md")
     out = {}
     if os.path.exists(path):
         for line in open(path):
             k, sep, v = line.strip().partition(": ")
-            if sep and k in ("traffic_bytes_per_walk_step", "l2_hit_rate", "valu_busy"):
+            if sep and k in ("traffic_bytes_per_walk_step", "l2_hit_rate", "valu_busy", "valu_lane_activity"):
                 out[k] = float(v)
     return out
+
+
+def alg_counts(name: str, n_points: int = 1024) -> dict:
+    """B_alg / F_alg of the fused estimator: the oracle's walk counters on
+    n_points seeded points of the workload at its walk count (CPU only)."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    O.build()
+    c = CONFIGS[name]
+    spec = make_spec(name)
+    sc = O.OracleScene(spec.vertices, spec.indices, spec.types)
+    idx = np.sort(np.random.default_rng(0).choice(len(spec.queries), min(n_points, len(spec.queries)), replace=False))
+    t0 = time.perf_counter()
+    o = sc.solve_fused(spec.queries[idx], spec.bc_code, c["nwalks"], nsteps=1024, eps=c["eps"], seed=0, count=True)
+    dt = time.perf_counter() - t0
+    npairs = (c["nwalks"] + 1) // 2
+    st = o["steps"]
+    return dict(config=name, points=len(idx), nwalks=c["nwalks"], steps=st, b_alg=round(o["bytes"] / st, 1),
+                f_alg=round(o["flops"] / st, 1), steps_per_walk=round(st / (len(idx) * 2 * npairs), 2),
+                ff_share=round(o["ff"] / st, 6), nodes_per_step=round(o["nodes"] / st, 1),
+                prims_per_step=round(o["prims"] / st, 2), sils_per_step=round(o["sils"] / st, 2),
+                cpu_seconds=round(dt, 1), threads=O.num_threads())
 
 
 def cpu_baseline(spec, cfg, target_s: float = 12.0):
@@ -88,6 +133,87 @@ def cpu_baseline(spec, cfg, target_s: float = 12.0):
                        f"({total_steps} walk steps, {total_t:.1f} s), fused u+grad estimator, "
                        f"C++ restatement of the reference algorithm, OpenMP",
                 points_per_s=npts / total_t)
+
+
+def timed_solves(solver, scene, q, steps: int, shards: int, seed0: int, count: bool, progress: bool, tag: str):
+    """`steps` fused solves, step k over shard k % shards with seed seed0 + k // shards.
+    Returns (outputs, per-launch HIP-event ms, wall seconds)."""
+    import torch
+
+    n = q.shape[0]
+    m = (n + shards - 1) // shards
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    outs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        s = k % shards
+        lo, hi = s * m, min(n, (s + 1) * m)
+        starts[k].record()
+        o = solver.solve_grad(q[lo:hi], scene, seed=seed0 + k // shards, count_steps=count, point_base=lo)
+        ends[k].record()
+        outs.append((lo, hi, o))
+        if progress:
+            ends[k].synchronize()
+            print(f"[bench] {tag} step {k} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return outs, [a.elapsed_time(b) for a, b in zip(starts, ends)], wall
+
+
+def accuracy(spec, scene, q, outs, shards: int) -> dict:
+    """RMSE of u and grad u against the FP64 analytic field over the first
+    complete solve (steps 0 .. shards-1: every point once, one seed)."""
+    import numpy as np
+
+    from dwost_amd import scenes
+
+    n = q.shape[0]
+    u = np.full(n, np.nan)
+    g = np.full((n, spec.dim), np.nan)
+    for lo, hi, o in outs[:shards]:
+        u[lo:hi] = o["u"].cpu().numpy()
+        g[lo:hi] = o["grad"].cpu().numpy()
+    cov = np.isfinite(u)
+    p = spec.queries[cov]
+    ua = scenes.analytic_u(spec.bc_code, p)
+    ga = scenes.analytic_grad(spec.bc_code, p)
+    R = scene.distance(q).cpu().numpy()[cov]
+    far = R > 0.05
+    return dict(rmse_points=int(cov.sum()), rmse_u=float(np.sqrt(np.mean((u[cov] - ua) ** 2))),
+                rmse_grad=float(np.sqrt(np.mean((g[cov] - ga) ** 2))),
+                rmse_grad_far=float(np.sqrt(np.mean((g[cov][far] - ga[far]) ** 2))))
+
+
+def secondary_2d(progress: bool, reps: int = 2) -> dict:
+    """The 2D holes workload (65,536 lattice points x 1024 walks, eps 1e-4):
+    `reps` full fused solves after one warm-up."""
+    import numpy as np
+
+    import dwost_amd
+
+    c = CONFIGS["holes"]
+    spec = make_spec("holes")
+    import torch
+
+    scene = dwost_amd.Scene2D(spec.vertices, spec.indices, spec.types, dwost_amd.BoundaryCondition(spec.bc_code))
+    solver = dwost_amd.WoSt(eps=c["eps"], nwalks=c["nwalks"], nsteps=1024)
+    q = torch.as_tensor(spec.queries).cuda()
+    solver.solve_grad(q, scene, seed=999)
+    outs, ms, wall = timed_solves(solver, scene, q, reps, 1, 0, True, progress, "holes")
+    steps = sum(int(o["steps"][0]) for _, _, o in outs)
+    avg_s = float(np.mean(ms)) / 1000.0
+    prof = profile_figures("holes")
+    ach = ALG["holes"]["b_alg"] * steps / reps / avg_s / 1e9
+    acc = accuracy(spec, scene, q, outs, 1)
+    return dict(workload=c["workload"], n_points=len(q), nwalks=c["nwalks"], eps=c["eps"],
+                walk_steps_per_s=round(steps / wall, 1), grad_points_per_s=round(reps * len(q) / wall, 1),
+                ms_per_solve=round(1000 * wall / reps, 3), avg_launch_ms=round(1000 * avg_s, 3),
+                roofline_frac=round(ach / HBM_PEAK_GBS, 4), achieved_gbs=round(ach, 1),
+                b_alg_bytes_per_step=ALG["holes"]["b_alg"], l2_hit_rate=prof.get("l2_hit_rate"),
+                valu_busy=prof.get("valu_busy"), valu_lane_activity=prof.get("valu_lane_activity"),
+                mean_steps_per_walk=round(steps / (reps * len(q) * c["nwalks"]), 3), **acc)
 
 
 def adjoint_ratio(points: int, reps: int = 2):
@@ -129,15 +255,15 @@ def adjoint_ratio(points: int, reps: int = 2):
         fwd_bwd(k)
     torch.cuda.synchronize()
     t_fb = (time.perf_counter() - t0) / reps
-    return dict(workload=f"noisy icosphere 81920 triangles, per-vertex g/h, {points} points x 256 walks, "
-                         "loss MSE(u)+MSE(grad u)", fwd_ms=round(1000 * t_f, 3), fwd_bwd_ms=round(1000 * t_fb, 3),
-                ratio=round(t_fb / t_f, 3))
+    return dict(workload=f"noisy icosphere 81920 triangles, 40962 vertices with per-vertex g/h, {points} points x "
+                         "256 walks, loss MSE(u)+MSE(grad u)", fwd_ms=round(1000 * t_f, 3),
+                fwd_bwd_ms=round(1000 * t_fb, 3), ratio=round(t_fb / t_f, 3))
 
 
-def batch_speedup(max_queries: int, reps: int = 3):
+def batch_speedup(max_queries: int | None, reps: int = 1):
     """One persistent launch over the 64-scene uneven batch (segments
-    log-U[64, 16384], 0-4 holes, walks {64, 256, 1024}, queries log-U[100, 1e5]
-    capped at max_queries) against the same solves as per-scene launches."""
+    log-U[64, 16384], 0-4 holes, walks {64, 256, 1024}, queries log-U[100, 1e5],
+    optionally capped) against the same solves as per-scene launches."""
     import torch
 
     import dwost_amd
@@ -170,8 +296,9 @@ def batch_speedup(max_queries: int, reps: int = 3):
     t_b = (time.perf_counter() - t0) / reps
     steps = int(b.steps[0])
     b.close()
-    return dict(workload=f"64 uneven 2D scenes, {sum(len(s.queries) for s in specs)} points "
-                         f"(queries capped at {max_queries}), walks per point from {{64, 256, 1024}}",
+    cap = "uncapped" if max_queries is None else f"capped at {max_queries}"
+    return dict(workload=f"64 uneven 2D scenes, {sum(len(s.queries) for s in specs)} points (queries {cap}), "
+                         "walks per point from {64, 256, 1024}",
                 per_scene_ms=round(1000 * t_s, 3), batched_graph_ms=round(1000 * t_b, 3),
                 speedup=round(t_s / t_b, 3), walk_steps_per_s=round(steps / t_b, 1))
 
@@ -179,20 +306,29 @@ def batch_speedup(max_queries: int, reps: int = 3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="holes", choices=sorted(CONFIGS))
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="ico", choices=sorted(CONFIGS))
+    ap.add_argument("--shards", type=int, default=0, help="steps per complete solve (0: the workload's default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--points", type=int, default=0, help="use only the first N query points (0: all)")
     ap.add_argument("--progress", action="store_true", help="print a line after each timed step")
-    ap.add_argument("--batch-queries", type=int, default=4000,
-                    help="query cap per scene of the 64-scene batched-vs-per-scene measurement (0: skip)")
-    ap.add_argument("--adjoint-points", type=int, default=2048,
+    ap.add_argument("--no-2d", action="store_true", help="skip the 2D holes measurement")
+    ap.add_argument("--batch-queries", type=int, default=-1,
+                    help="query cap per scene of the 64-scene batched-vs-per-scene measurement "
+                         "(-1: uncapped, 0: skip)")
+    ap.add_argument("--adjoint-points", type=int, default=16384,
                     help="points of the fwd+bwd/fwd measurement on the 3D adjoint workload (0: skip)")
+    ap.add_argument("--alg-counts", action="store_true",
+                    help="count algorithmic bytes / flops per walk step with the oracle on 1024 points (CPU), print "
+                         "them and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 256 MiB device-to-device copies (scales the FETCH_SIZE / WRITE_SIZE "
                          "counters) and exit")
     args = ap.parse_args()
+    if args.alg_counts:
+        print(json.dumps(alg_counts(args.config)), flush=True)
+        return
     if args.copy_calibration:
         import torch
 
@@ -212,70 +348,60 @@ def main():
     import torch
 
     import dwost_amd
-    from dwost_amd import scenes
 
     cfg = CONFIGS[args.config]
+    shards = args.shards or cfg["shards"]
     spec = make_spec(args.config)
+    if args.points:
+        spec.queries = spec.queries[: args.points]
     Scene = dwost_amd.Scene2D if spec.dim == 2 else dwost_amd.Scene3D
     scene = Scene(spec.vertices, spec.indices, spec.types, dwost_amd.BoundaryCondition(spec.bc_code))
     Solver = dwost_amd.WoSt if spec.dim == 2 else dwost_amd.WoSt3D
     solver = Solver(eps=cfg["eps"], nwalks=cfg["nwalks"], nsteps=1024)
-    if args.points:
-        spec.queries = spec.queries[: args.points]
     q = torch.as_tensor(spec.queries).cuda()
     torch.cuda.synchronize()
-    print(f"[bench] scene ready: {args.config}, {len(q)} points", file=sys.stderr, flush=True)
+    print(f"[bench] scene ready: {args.config}, {len(q)} points, {shards} shards", file=sys.stderr, flush=True)
 
-    for w in range(args.warmup):
-        solver.solve_grad(q, scene, seed=1000 + w)
-    torch.cuda.synchronize()
+    timed_solves(solver, scene, q, args.warmup, shards, 1000, False, args.progress, "warmup")
     print(f"[bench] warmup done ({args.warmup})", file=sys.stderr, flush=True)
-
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    outs = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        starts[k].record()
-        outs.append(solver.solve_grad(q, scene, seed=k, count_steps=True))
-        ends[k].record()
-        if args.progress:
-            ends[k].synchronize()
-            print(f"[bench] step {k} done", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kernel_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
-    total_steps = sum(int(o["steps"][0]) for o in outs)
+    outs, kernel_ms, wall = timed_solves(solver, scene, q, args.steps, shards, 0, True, args.progress, "timed")
+    total_steps = sum(int(o["steps"][0]) for _, _, o in outs)
     value = total_steps / wall
     ms_per_step = 1000.0 * wall / args.steps
+    pts_per_step = sum(hi - lo for lo, hi, _ in outs) / args.steps
     print(f"[bench] {args.steps} steps in {wall:.3f}s", file=sys.stderr, flush=True)
 
-    # accuracy of the last step against the FP64 analytic field
-    u = outs[-1]["u"].cpu().numpy()
-    g = outs[-1]["grad"].cpu().numpy()
-    ua = scenes.analytic_u(spec.bc_code, spec.queries)
-    ga = scenes.analytic_grad(spec.bc_code, spec.queries)
-    R = scene.distance(q).cpu().numpy()
-    far = R > 0.05
-    acc = dict(rmse_u=float(np.sqrt(np.mean((u - ua) ** 2))), rmse_grad=float(np.sqrt(np.mean((g - ga) ** 2))),
-               rmse_grad_far=float(np.sqrt(np.mean((g[far] - ga[far]) ** 2))),
-               mean_steps_per_walk=total_steps / (args.steps * len(q) * 2 * ((cfg["nwalks"] + 1) // 2)))
+    acc = accuracy(spec, scene, q, outs, shards)
+    npairs = (cfg["nwalks"] + 1) // 2
+    acc["mean_steps_per_walk"] = total_steps / (args.steps * pts_per_step * 2 * npairs)
 
+    alg = ALG[args.config]
     steps_per_launch = total_steps / args.steps
     avg_kernel_s = float(np.mean(kernel_ms)) / 1000.0
-    achieved = B_ALG[args.config] * steps_per_launch / avg_kernel_s / 1e9
+    achieved = alg["b_alg"] * steps_per_launch / avg_kernel_s / 1e9
     prof = profile_figures(args.config)
     traffic = prof.get("traffic_bytes_per_walk_step")
+    kname = f"k_fused_pers<{spec.dim}, false, false, {'true' if spec.dim == 3 else 'false'}>"
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4),
                 traffic=None if traffic is None else round(traffic * steps_per_launch),
-                kernel="k_fused_pers (HIP events bracket point prep + kernel + reduction; the kernel is >99.6%)",
-                b_alg_bytes_per_step=B_ALG[args.config], avg_launch_ms=round(1000 * avg_kernel_s, 3),
+                kernel=f"{kname} (HIP events bracket point prep + kernel + reduction; the kernel is >99% of it)",
+                b_alg_bytes_per_step=alg["b_alg"], avg_launch_ms=round(1000 * avg_kernel_s, 3),
+                f_alg_flops_per_step=alg["f_alg"],
+                achieved_tflops=round(alg["f_alg"] * steps_per_launch / avg_kernel_s / 1e12, 3),
+                fp32_peak_tflops=FP32_PEAK_TFLOPS, ff_share=alg["ff_share"],
                 l2_hit_rate=prof.get("l2_hit_rate"), valu_busy=prof.get("valu_busy"),
-                note="achieved = algorithmic bytes (reference record sizes) / launch time; traffic = measured HBM "
-                     "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE per walk step x steps); the scene is "
-                     "cache-resident, so VALU issue or fetch latency binds, not HBM")
+                valu_lane_activity=prof.get("valu_lane_activity"),
+                limiter="dependent node-fetch latency and VALU issue with divergent lanes (the scene is L2/MALL "
+                        "resident), not HBM bandwidth",
+                note="achieved = algorithmic bytes (reference record sizes, reference traversal order) / launch "
+                     "time; traffic = measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE per walk "
+                     "step x steps)")
+
+    holes = None
+    if not args.no_2d and args.config != "holes":
+        holes = secondary_2d(args.progress)
+        print(f"[bench] holes {holes['walk_steps_per_s']:.3e} walk-steps/s", file=sys.stderr, flush=True)
 
     adj = None
     if args.adjoint_points:
@@ -284,15 +410,16 @@ def main():
 
     bat = None
     if args.batch_queries:
-        bat = batch_speedup(args.batch_queries)
+        bat = batch_speedup(None if args.batch_queries < 0 else args.batch_queries)
         print(f"[bench] batch speedup {bat['speedup']}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(spec, cfg)
+        cpu["gpu_over_cpu"] = round(value / cpu["value"], 1)
 
     line = {
-        "metric": "walk-steps/s (fused u + grad u) at fixed RMSE vs analytic",
+        "metric": "walk-steps/s + grad-u points/s at fixed RMSE vs analytic; fwd+bwd/fwd time ratio",
         "value": round(value, 1),
         "unit": "walk-steps/s",
         "n_gpus": 1,
@@ -305,10 +432,14 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded scene + queries)",
         "config": dict(workload=cfg["workload"], config=args.config, n_points=len(q), nwalks=cfg["nwalks"],
-                       eps=cfg["eps"], nsteps=1024, grad_points_per_s=round(len(q) * args.steps / wall, 1),
+                       eps=cfg["eps"], nsteps=1024, points_per_step=int(pts_per_step), steps_per_solve=shards,
+                       grad_points_per_s=round(pts_per_step * args.steps / wall, 1),
+                       rmse_note="every walk equals the reference estimator's walk bit for bit (GPU parity "
+                                 "tests), so at the same walks per point the RMSE is the reference algorithm's",
                        **acc),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "holes_2d": holes,
         "adjoint": adj,
         "batch": bat,
     }

@@ -18,22 +18,29 @@ import torch
 
 from . import _lib
 from ._lib import check, lib, ptr, stream_handle
-from .shapes import _as_points, _Scene
+from .shapes import BoundaryCondition, _as_points, _Scene
 from .solvers import Solver
 
 
 @dataclass
 class BatchItem:
+    """One (solver, scene, points, seed) solve of a batch.  ``boundary``
+    (optional) replaces the scene's boundary data for this item only, so items
+    can share one uploaded scene with different g / h (e.g. perturbed per-vertex
+    values)."""
+
     solver: Solver
     scene: _Scene
     points: torch.Tensor
     seed: int = 0
     point_base: int = 0
+    boundary: BoundaryCondition | None = None
 
 
 class BatchSolver:
-    """Prepared multi-scene solve.  ``points`` tensors are captured by address:
-    update them in place between ``solve()`` calls, never reallocate them.
+    """Prepared multi-scene solve.  ``points`` must be contiguous float32
+    (N, dim) CUDA tensors on one device; they are captured by address: update
+    them in place between ``solve()`` calls, never reallocate them.
     ``graph=True`` records the launch sequence into a hipGraph on the first
     solve and replays it afterwards."""
 
@@ -51,6 +58,12 @@ class BatchSolver:
         for k, it in enumerate(items):
             it.solver._check(it.scene)
             p = _as_points(it.points, self.dim)
+            if not isinstance(it.points, torch.Tensor) or p.data_ptr() != it.points.data_ptr() or \
+                    p.shape != it.points.shape:
+                # the batch reads the points by address on every solve: a converted
+                # copy would keep solving the values it held at creation
+                raise ValueError(f"batch item {k}: points must be a contiguous float32 (N, {self.dim}) CUDA tensor "
+                                 "(the batch solves whatever it holds at each solve)")
             if dev is None:
                 dev = p.device
             elif p.device != dev:
@@ -58,10 +71,11 @@ class BatchSolver:
             n = p.shape[0]
             u = torch.empty(n, device=p.device)
             g = torch.empty(n, self.dim, device=p.device)
-            bc = it.scene.boundary.to_c(it.scene.n_vertices)
+            bnd = it.boundary if it.boundary is not None else it.scene.boundary
+            bc = bnd.to_c(it.scene.n_vertices)
             arr[k] = _lib.BatchItem(it.scene.handle, ptr(p), n, it.solver._params(it.seed, it.point_base), bc,
                                     ptr(u), ptr(g))
-            self._keep.append((p, it.scene, it.scene.boundary))
+            self._keep.append((p, it.scene, bnd))
             self.outputs.append(dict(u=u, grad=g))
         self.device = dev
         self.steps = torch.zeros(1, dtype=torch.int64, device=dev)

@@ -87,7 +87,7 @@ DWD float cr_cbrt(float x) { return (float)cbrt((double)x); }
 #endif
 
 // ---------------------------------------------------------------------------
-// TEA seeding (diff_wost/core/math.py:137-157) and PCG32 (Dr.Jit PCG32)
+// TEA seeding (diff_wost/core/math.py:87-107) and PCG32 (Dr.Jit PCG32)
 // ---------------------------------------------------------------------------
 DWD void tea4(uint32_t v0, uint32_t v1, uint32_t& o0, uint32_t& o1) {
     uint32_t s = 0;
@@ -849,6 +849,35 @@ DWD float star_bound(const SubDev& S, F3 x, float r_max) {
               fmx(__builtin_fabsf(x.z - S.bb_min[2]), __builtin_fabsf(x.z - S.bb_max[2])));
     return fmn(norm(m), r_max);
 }
+// Bounds [lo, hi] on every fp32 distance the queries compute from x to a point
+// of the sub-scene: its vertex box widened by 1e-3 (1 + max |coordinate|), which
+// holds every node box, closest point and silhouette point of the sub-scene;
+// lo / hi take the per-axis nearest / farthest offsets through the same norm.
+// fp32 subtraction, fma, sqrt, min and max round monotonically, so the box
+// distances d_min / d_max of every node and the distance to every primitive
+// or silhouette point lie in [lo, hi].  Only for |x_i| <= 2^59 / (1 + max
+// |coordinate|): there no product of an edge with an offset from x, no square
+// of such a product and no squared offset overflows (each stays below 2^126);
+// otherwise returns false.
+DWD bool dist_bounds(const SubDev& S, F3 x, float& lo, float& hi) {
+    float big = fmx(fmx(fmx(__builtin_fabsf(S.bb_min[0]), __builtin_fabsf(S.bb_max[0])),
+                        fmx(__builtin_fabsf(S.bb_min[1]), __builtin_fabsf(S.bb_max[1]))),
+                    fmx(__builtin_fabsf(S.bb_min[2]), __builtin_fabsf(S.bb_max[2])));
+    const float lim = 0x1p59f / (1.0f + big);
+    if (!(__builtin_fabsf(x.x) <= lim && __builtin_fabsf(x.y) <= lim && __builtin_fabsf(x.z) <= lim)) return false;
+    float m = 1e-3f * (1.0f + big);
+    const float xs[3] = {x.x, x.y, x.z};
+    float l[3], h[3];
+    for (int k = 0; k < 3; ++k) {
+        float a = S.bb_min[k] - m, b = S.bb_max[k] + m;
+        l[k] = xs[k] < a ? a - xs[k] : (xs[k] > b ? xs[k] - b : 0.0f);
+        h[k] = fmx(__builtin_fabsf(xs[k] - a), __builtin_fabsf(xs[k] - b));
+    }
+    lo = norm(f3(l[0], l[1], l[2]));
+    hi = norm(f3(h[0], h[1], h[2]));
+    return true;
+}
+
 template <bool X = true>
 DWD float star_radius2(const SubDev& S, F2 x, float r_max) {
     float R = star_bound(S, x, r_max);
@@ -870,7 +899,7 @@ DWD float star_radius3(const SubDev& S, F3 x, float r_max) {
 
 
 // ---------------------------------------------------------------------------
-// 3D: closest point on triangles (math.py:257-322) and SNCH3D.closest_point
+// 3D: closest point on triangles (math.py:207-272) and SNCH3D.closest_point
 // ---------------------------------------------------------------------------
 DWD void tri_closest(const DTri& tr, F3 p, F3& pt, float& u_out, float& v_out, float& d) {
     F3 a = f3(tr.ax, tr.ay, tr.az), b = f3(tr.bx, tr.by, tr.bz), c = f3(tr.cx, tr.cy, tr.cz);

@@ -8,7 +8,7 @@ namespace dwd {
 // Boundary data (diff_wost/shapes/polyline.py:52-58, mesh.py:59-63): g = u(p),
 // h = n . grad u(p) for an analytic harmonic u, or per-vertex g/h buffers
 // interpolated linearly (2D: lerp by t; 3D: barycentric (u, v) as in
-// math.py:356-357).
+// math.py:306-307).
 // --------------------------------------------------------------------------
 DWD float bc_u(const BCDev& bc, F3 p) {
     if (bc.code == DW_BC_EXPCOS) return cr_exp(p.x) * cr_cos(p.y);
@@ -149,11 +149,11 @@ DWD F3 greens3_gradient(F3 c, float R, float r, F3 y) {
     float gn = (1.0f / r3 - 1.0f / (R * R * R)) / kFourPi;
     return (y - c) * gn;
 }
-DWD F2 uniform_dir2(Rng& rng) {  // uniform_angle (math.py:325-327)
+DWD F2 uniform_dir2(Rng& rng) {  // uniform_angle (math.py:275-277)
     float theta = (rng.next() * 2.0f) * kPi;
     return f2(cr_cos(theta), cr_sin(theta));
 }
-DWD F3 uniform_sphere(Rng& rng) {  // math.py:348-353
+DWD F3 uniform_sphere(Rng& rng) {  // math.py:298-303
     float u0 = rng.next();
     float u1 = rng.next();
     float z = 1.0f - 2.0f * u0;
@@ -405,12 +405,24 @@ DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Wa
             return ADV_END;
         }
         if (Dn.present) {
-            closest3_begin(t, Dn, w.ip);
-            w.stage = WS_DIR;
-            return ADV_QUERY;
+            // Far from the scene every distance to the Dirichlet boundary rounds to
+            // one value: when the bounds on all of them coincide, the closest-point
+            // query (which would visit the whole tree, every test a tie) returns
+            // exactly that distance, and a step that does not terminate uses
+            // nothing else of its record.
+            float lo, hi;
+            if (dist_bounds(Dn, w.ip, lo, hi) && lo == hi && !(lo < P.eps)) {
+                w.R = w.d_dir = lo;
+                w.stage = WS_STAR;
+            } else {
+                closest3_begin(t, Dn, w.ip);
+                w.stage = WS_DIR;
+                return ADV_QUERY;
+            }
+        } else {
+            w.R = w.d_dir = kInf;
+            w.stage = WS_STAR;
         }
-        w.R = w.d_dir = kInf;
-        w.stage = WS_STAR;
     } else if (w.stage == WS_WOS) {
         CP c = closest3_end(t);
         float d = __builtin_fabsf(c.d);
@@ -440,7 +452,12 @@ DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Wa
         F3 p = w.ob ? w.ip - w.in * 1e-3f : w.ip;
         t.r_max = star_bound(F, p, w.d_dir);
         w.stage = WS_SIL;
-        if (F.py_sils && sil_begin<3>(t, F, p, t.r_max)) return ADV_QUERY;
+        // a silhouette counts only nearer than r_max: none can when every
+        // distance to the scene is at least r_max, and the query then returns
+        // r_max unchanged
+        float lo, hi;
+        bool none = dist_bounds(F, p, lo, hi) && t.r_max <= lo;
+        if (F.py_sils && !none && sil_begin<3>(t, F, p, t.r_max)) return ADV_QUERY;
     }
     if (w.stage == WS_SIL) {
         float sr = t.r_max * 0.98f;

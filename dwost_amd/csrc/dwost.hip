@@ -1753,9 +1753,13 @@ int zero_u64(unsigned long long* p, hipStream_t st) {
     return DW_OK;
 }
 
-bool use_simple_kernels() {
+// DWOST_KERNEL=simple selects the one-job-per-thread kernels (a cross-check
+// of the persistent ones).  They evaluate the literal cone test and write no
+// guard flags, so they serve only exact-cone solves that ask for no flags;
+// every other call keeps the persistent kernels.
+bool use_simple_kernels(const dw_solver_params* p, const void* flags_out) {
     const char* e = getenv("DWOST_KERNEL");
-    return e && strcmp(e, "simple") == 0;
+    return e && strcmp(e, "simple") == 0 && (p->flags & DW_SOLVE_EXACT_CONES) && !flags_out;
 }
 
 }  // namespace
@@ -1994,7 +1998,7 @@ int dw_solve_u(const dw_scene* s, const float* pts, int64_t n, const dw_solver_p
     hipStream_t st = (hipStream_t)stream;
     StreamCtx* c;
     if ((rc = stream_ctx(st, &c))) return rc;
-    const bool simple = use_simple_kernels() && !walk_flags;
+    const bool simple = use_simple_kernels(params, walk_flags);
     int64_t lanes = n * (int64_t)params->nwalks;
     size_t obytes = simple ? 0 : order_bytes(n, params);
     size_t wbytes = (walk_u ? 0 : (size_t)lanes * 4) + ((total_steps && !walk_steps) ? (size_t)lanes * 4 : 0);
@@ -2042,7 +2046,7 @@ int dw_solve_dude(const dw_scene* s, const float* pts, const float* e, int64_t n
     StreamCtx* c;
     if ((rc = stream_ctx(st, &c))) return rc;
     // dudes run as persistent point jobs (DWOST_KERNEL=simple: one point per thread)
-    const bool persistent = !use_simple_kernels();
+    const bool persistent = !use_simple_kernels(params, nullptr);
     const size_t obytes = persistent ? order_bytes(n, params) : 0;
     void* p;
     if ((rc = workspace(*c, (size_t)n * 8 + obytes, st, &p))) return rc;
@@ -2085,7 +2089,7 @@ int dw_trace_step(const dw_scene* s, const float* pts, int64_t n, const dw_solve
     hipStream_t st = (hipStream_t)stream;
     StreamCtx* c;
     if ((rc = stream_ctx(st, &c))) return rc;
-    const bool persistent = !use_simple_kernels();
+    const bool persistent = !use_simple_kernels(params, nullptr);
     const size_t obytes = persistent ? order_bytes(n, params) : 0;
     void* p;
     if ((rc = workspace(*c, (size_t)ns * D * 4 + (size_t)n * 8 + obytes, st, &p))) return rc;
@@ -2140,7 +2144,7 @@ int dw_solve_fused(const dw_scene* s, const float* pts, int64_t n, const dw_solv
     hipStream_t st = (hipStream_t)stream;
     StreamCtx* c;
     if ((rc = stream_ctx(st, &c))) return rc;
-    const bool simple = use_simple_kernels();
+    const bool simple = use_simple_kernels(params, pair_flags);
     size_t obytes = simple ? 0 : order_bytes(n, params);
     size_t need = (size_t)n * 8 + (pair_vals ? 0 : (size_t)lanes * (D + 1) * 4) +
                   (pair_steps ? 0 : (size_t)lanes * 4) + obytes;

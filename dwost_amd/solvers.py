@@ -148,7 +148,7 @@ def _check_flags(t: torch.Tensor, numel: int, dev) -> None:
 
 def _same_walks(a: Solver, b: Solver) -> bool:
     """True when two solvers draw identical walks for the same seed."""
-    return (a.dim == b.dim and a.nwalks == b.nwalks and a.nsubwalks == b.nsubwalks and a.nsteps == b.nsteps
+    return (a._walk_flag == b._walk_flag and a.dim == b.dim and a.nwalks == b.nwalks and a.nsubwalks == b.nsubwalks and a.nsteps == b.nsteps
             and a.control_variates == b.control_variates and a.antithetic_variates == b.antithetic_variates
             and a.eps == b.eps and a.min_R == b.min_R and a.exact_cones == b.exact_cones)
 

@@ -47,7 +47,7 @@ def lib():
         common = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_float]
         L.orc_solve_u.argtypes = [C.c_void_p, C.c_void_p, C.c_int64] + common + [C.c_void_p] * 5
         L.orc_solve_dude.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64] + common + [C.c_void_p] * 2
-        L.orc_solve_fused.argtypes = [C.c_void_p, C.c_void_p, C.c_int64] + common + [C.c_void_p] * 6
+        L.orc_solve_fused.argtypes = [C.c_void_p, C.c_void_p, C.c_int64] + common + [C.c_void_p] * 7
         L.orc_trace_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int64] + common + [C.c_int32, C.c_float] + [
             C.c_void_p] * 3
         L.orc_tea.argtypes = [C.c_uint32, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p]
@@ -56,6 +56,7 @@ def lib():
         L.orc_math.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
         L.orc_num_threads.restype = C.c_int
         L.orc_cv_bound_check.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+        L.orc_far_check.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
     return _lib
 
 
@@ -155,6 +156,14 @@ class OracleScene:
         lib().orc_cv_bound_check(self.h, _p(pts), len(pts), _p(same), _p(nodes))
         return same.astype(bool), int(nodes[0]), int(nodes[1])
 
+    def far_check(self, pts):
+        """3D: the distance bounds the walk kernels use far from the scene,
+        checked against the reference queries (see orc_far_check)."""
+        pts = _f32(pts)
+        out = np.zeros((len(pts), 4), np.int32)
+        lib().orc_far_check(self.h, _p(pts), len(pts), _p(out))
+        return out
+
     def ref_topology(self, sub: int, ref: int) -> int:
         return lib().orc_ref_topology(self.h, sub, ref)
 
@@ -223,7 +232,10 @@ class OracleScene:
         return dict(pos=out, samples=smp, steps=int(st[0]))
 
     def solve_fused(self, pts, bc_code, nwalks, nsteps=128, eps=1e-3, min_R=1e-3, seed=0, coeffs=None, gv=None,
-                    hv=None, src_code=0, src_c=0.0, per_pair=False, cv=True, base=0):
+                    hv=None, src_code=0, src_c=0.0, per_pair=False, cv=True, base=0, count=False):
+        """count=True adds the algorithmic counters of the walks (reference
+        binary-tree order, reference record sizes): bytes, flops, nodes,
+        prims, sils and ff (steps from a NaN position, which fetch nothing)."""
         pts = _f32(pts)
         n = len(pts)
         D = self.dim
@@ -236,9 +248,12 @@ class OracleScene:
         pg = np.zeros((n * npairs, D), np.float32) if per_pair else None
         ps = np.zeros(n * npairs, np.int32) if per_pair else None
         st = np.zeros(1, np.uint64)
+        cnt = np.zeros(6, np.uint64) if count else None
         lib().orc_solve_fused(self.h, _p(pts), n, _p(ip), _p(fp), bc_code, _p(c), _p(gv), _p(hv), src_code, src_c,
-                              _p(u), _p(g), _p(pu), _p(pg), _p(ps), _p(st))
+                              _p(u), _p(g), _p(pu), _p(pg), _p(ps), _p(st), _p(cnt))
         out = dict(u=u, grad=g, steps=int(st[0]))
+        if count:
+            out.update({k: int(v) for k, v in zip(("bytes", "flops", "nodes", "prims", "sils", "ff"), cnt)})
         if per_pair:
             out.update(pair_u=pu, pair_g=pg, pair_steps=ps)
         return out

@@ -92,7 +92,7 @@ static inline V3 vmin(V3 a, V3 b) { return v3(fminf_(a.x, b.x), fminf_(a.y, b.y)
 static inline V3 vmax(V3 a, V3 b) { return v3(fmaxf_(a.x, b.x), fmaxf_(a.y, b.y), fmaxf_(a.z, b.z)); }
 static inline V2 vabs(V2 a) { return v2(std::fabs(a.x), std::fabs(a.y)); }
 static inline V3 vabs(V3 a) { return v3(std::fabs(a.x), std::fabs(a.y), std::fabs(a.z)); }
-// 2D cross as written in diff_wost/core/math.py:85-95 (plain ops)
+// 2D cross as written in diff_wost/core/math.py:35-45 (plain ops)
 static inline float cross2(V2 u, V2 v) { return u.x * v.y - u.y * v.x; }
 
 // correctly rounded (via fp64) transcendentals
@@ -108,7 +108,7 @@ static inline float cr_cbrt(float x) { return (float)std::cbrt((double)x); }
 static inline bool bad(float v) { return std::isnan(v) || std::isinf(v); }
 
 // ----------------------------------------------------------------------------
-// TEA (diff_wost/core/math.py:137-157) and PCG32 (Dr.Jit PCG32; O'Neill's
+// TEA (diff_wost/core/math.py:87-107) and PCG32 (Dr.Jit PCG32; O'Neill's
 // pcg32_srandom_r / pcg32_random_r; next_float32 = bits(u>>9 | 1.0f) - 1)
 // ----------------------------------------------------------------------------
 static inline void tea(uint32_t v0, uint32_t v1, int rounds, const uint32_t key[4], uint32_t* o0,
@@ -820,9 +820,32 @@ static void make_sub(const Scene& sc, int want_type, Sub<D>& S) {
 // Counters for algorithmic bytes (reference SoA record sizes)
 // ----------------------------------------------------------------------------
 struct Cnt {
-    uint64_t bytes = 0, nodes = 0, prims = 0, sils = 0;
+    uint64_t bytes = 0, nodes = 0, prims = 0, sils = 0, flops = 0;
+    uint64_t ff = 0;  // steps taken from a NaN position (no record fetched)
+    void add(const Cnt& o) {
+        bytes += o.bytes, nodes += o.nodes, prims += o.prims, sils += o.sils, flops += o.flops, ff += o.ff;
+    }
 };
 enum { NODE2 = 24, SNODE2 = 48, SNODE3 = 60, SEG = 20, TRI = 40, SILV = 40, SILE = 68 };
+// Algorithmic fp32 operations per record test, counted from the routines below
+// (add, sub, mul, div, min, max, compare, sqrt and each transcendental count 1;
+// a dot product of n terms counts 2n-1):
+//   FB2/FB3    box_dist (2 sub-vectors, 2 max, norm, min, norm) + the r_max update and test
+//   FR2/FR3    box_ray (2 x (sub, div) per axis, min/max per axis, 2 folds of dim)
+//   FC2/FC3    snch_overlaps past the box: centre, view vector, norm, acos of the
+//              axis angle, range tests, one asin or atan2 branch
+//   FSEG/FTRI  seg_closest / tri_closest (Ericson's region tests: 6 dots, 3 combos,
+//              barycentric divide and lerp, norm)
+//   FSEGR/FTRIR seg_ray / tri_ray (Moller-Trumbore: 2 edges, 2 crosses, 3 dots, 3 divides)
+//   FSILV/FSILE silv_closest (norm, 2 crosses) / sile_closest (segment point, norm,
+//              half of the time two face normals and two dots)
+//   FSTEP2/FSTEP3 the scalar logic of one walk step (ball radius, direction sample,
+//              Green's function and its rejection sampler, Neumann and control-variate
+//              terms, ray set-up)
+enum {
+    FB2 = 20, FB3 = 29, FR2 = 17, FR3 = 25, FC2 = 30, FC3 = 42, FSEG = 29, FTRI = 70, FSEGR = 30, FTRIR = 50,
+    FSILV = 21, FSILE = 70, FSTEP2 = 90, FSTEP3 = 80
+};
 
 // ----------------------------------------------------------------------------
 // 2D queries (diff_wost/shapes/bvh.py, snch.py, line_segment.py, silhouette_vertex.py)
@@ -934,7 +957,7 @@ static CP bvh2_closest(const Tree<2>& T, V2 p, Cnt* c, float r0 = kInf) {
             for (int j = 0; j < node.nref; ++j) {
                 CP r;
                 seg_closest(T.prims[node.off + j], p, r);
-                if (c) c->bytes += SEG, c->prims++;
+                if (c) c->bytes += SEG, c->prims++, c->flops += FSEG + 1;
                 if (r.d < its.d) its = r;
             }
             r_max = fminf_(r_max, its.d);
@@ -947,7 +970,7 @@ static CP bvh2_closest(const Tree<2>& T, V2 p, Cnt* c, float r0 = kInf) {
             box_dist(node_min2(T.nodes[R]), node_max2(T.nodes[R]), p, dminR, dmaxR);
             bool hitR = dminR <= r_max;
             r_max = fminf_(r_max, dmaxR);
-            if (c) c->bytes += 2 * NODE2, c->nodes += 2;
+            if (c) c->bytes += 2 * NODE2, c->nodes += 2, c->flops += 2 * FB2 + 1;
             if (hitL && hitR) {
                 int closer = L, other = R;
                 // `d_max_right < d_max_right` at bvh.py:163 never swaps
@@ -1003,7 +1026,7 @@ static Its bvh2_intersect(const Tree<2>& T, V2 x, V2 v, float r_max, Cnt* c) {
     std::memset(&its, 0, sizeof(its));
     its.prim = 0;
     float tmin;
-    if (c) c->bytes += NODE2, c->nodes++;
+    if (c) c->bytes += NODE2, c->nodes++, c->flops += FR2;
     if (!box_ray(node_min2(T.nodes[0]), node_max2(T.nodes[0]), x, v, r_max, tmin)) return its;
     int st_i[64];
     float st_d[64];
@@ -1020,7 +1043,7 @@ static Its bvh2_intersect(const Tree<2>& T, V2 x, V2 v, float r_max, Cnt* c) {
         if (node.nref > 0) {
             for (int j = 0; j < node.nref; ++j) {
                 Its r;
-                if (c) c->bytes += SEG, c->prims++;
+                if (c) c->bytes += SEG, c->prims++, c->flops += FSEGR;
                 if (seg_ray(T.prims[node.off + j], x, v, r_max, r) && r.d < r_max) {
                     r_max = its.d;
                     its = r;
@@ -1031,7 +1054,7 @@ static Its bvh2_intersect(const Tree<2>& T, V2 x, V2 v, float r_max, Cnt* c) {
             float t0, t1;
             bool h0 = box_ray(node_min2(T.nodes[L]), node_max2(T.nodes[L]), x, v, r_max, t0);
             bool h1 = box_ray(node_min2(T.nodes[R]), node_max2(T.nodes[R]), x, v, r_max, t1);
-            if (c) c->bytes += 2 * 16, c->nodes += 2;
+            if (c) c->bytes += 2 * 16, c->nodes += 2, c->flops += 2 * FR2 + 1;
             if (h0 && h1) {
                 int closer = L, other = R;
                 if (t1 < t0) {
@@ -1112,7 +1135,7 @@ static inline bool snch_overlaps(VT mn, VT mx, const float* axis_, float ha, flo
     return ov;
 }
 
-// is_silhouette (math.py:194-209) + SilhouetteVertex.closest_silhouette (silhouette_vertex.py:80-93)
+// is_silhouette (math.py:144-160) + SilhouetteVertex.closest_silhouette (silhouette_vertex.py:80-93)
 static inline bool silv_closest(const Sil2& s, int ref, V2 x, float r_max, SilRec& r) {
     V2 a = v2(s.a[0], s.a[1]), b = v2(s.b[0], s.b[1]), c = v2(s.c[0], s.c[1]);
     V2 view = sub(x, b);
@@ -1131,7 +1154,7 @@ static inline bool silv_closest(const Sil2& s, int ref, V2 x, float r_max, SilRe
     return false;
 }
 
-// closest_point_line_segment (math.py:174-191) in 3D + silhouette_edge.py:8-13,83-97
+// closest_point_line_segment (math.py:124-141) in 3D + silhouette_edge.py:8-13,83-97
 static inline bool sile_closest(const Sil3& s, int ref, V3 x, float r_max, SilRec& r) {
     V3 a = v3(s.a[0], s.a[1], s.a[2]), b = v3(s.b[0], s.b[1], s.b[2]);
     V3 u = sub(b, a), v = sub(x, a);
@@ -1173,7 +1196,7 @@ static SilRec snch_closest_sil(const Sub<D>& S, const float* xq, float r_max, Cn
     std::memcpy(&mn, T.nodes[0].box.mn, sizeof(VT));
     std::memcpy(&mx, T.nodes[0].box.mx, sizeof(VT));
     box_dist(mn, mx, x, dmin, dmax);
-    if (c) c->bytes += NB, c->nodes++;
+    if (c) c->bytes += NB, c->nodes++, c->flops += D == 2 ? FB2 : FB3;
     if (!(dmin <= r_max)) return rec;
     int st_i[64];
     float st_d[64];
@@ -1194,10 +1217,10 @@ static SilRec snch_closest_sil(const Sub<D>& S, const float* xq, float r_max, Cn
                 bool ok;
                 if constexpr (D == 2) {
                     ok = silv_closest(S.ref2[ref], ref, v2(xq[0], xq[1]), r_max, r);
-                    if (c) c->bytes += SILV, c->sils++;
+                    if (c) c->bytes += SILV, c->sils++, c->flops += FSILV;
                 } else {
                     ok = sile_closest(S.ref3[ref], ref, v3(xq[0], xq[1], xq[2]), r_max, r);
-                    if (c) c->bytes += SILE, c->sils++;
+                    if (c) c->bytes += SILE, c->sils++, c->flops += FSILE;
                 }
                 if (ok && r.d < r_max) {
                     r_max = fminf_(r_max, r.d);
@@ -1216,7 +1239,18 @@ static SilRec snch_closest_sil(const Sub<D>& S, const float* xq, float r_max, Cn
             float d0, d1;
             bool h0 = snch_overlaps(mnl, mxl, nl.axis, nl.half_angle, nl.radius, x, r_max, d0);
             bool h1 = snch_overlaps(mnr, mxr, nr.axis, nr.half_angle, nr.radius, x, r_max, d1);
-            if (c) c->bytes += 2 * NB, c->nodes += 2;
+            if (c) {
+                c->bytes += 2 * NB, c->nodes += 2;
+                // the cone part runs where the box overlaps and the cone is not trivially open
+                const int fb = D == 2 ? FB2 : FB3, fc = D == 2 ? FC2 : FC3;
+                const Node<D>* ch[2] = {&nl, &nr};
+                const float dd[2] = {d0, d1};
+                for (int k = 0; k < 2; ++k) {
+                    float ha = ch[k]->half_angle;
+                    bool cone = ha >= 0.0f && dd[k] <= r_max && !(ha > kPi2 && dd[k] < kDrEps);
+                    c->flops += fb + (cone ? fc : 0);
+                }
+            }
             if (h0 && h1) {
                 int closer = L, other = R;
                 if (d1 < d0) {
@@ -1256,7 +1290,7 @@ static float star_radius(const Sub<D>& S, const float* x, float r_max, Cnt* c) {
 }
 
 // ----------------------------------------------------------------------------
-// 3D queries (snch3d.py:334-476, triangle.py, math.py:257-322)
+// 3D queries (snch3d.py:334-476, triangle.py, math.py:207-272)
 // ----------------------------------------------------------------------------
 static inline void tri_closest(const Prim<3>& tr, V3 p, CP& r) {  // closest_point_triangle
     V3 a = v3(tr.v[0][0], tr.v[0][1], tr.v[0][2]);
@@ -1330,7 +1364,7 @@ static CP snch3_closest(const Tree<3>& T, V3 p, Cnt* c) {
             for (int j = 0; j < node.nref; ++j) {
                 CP r;
                 tri_closest(T.prims[node.off + j], p, r);
-                if (c) c->bytes += TRI, c->prims++;
+                if (c) c->bytes += TRI, c->prims++, c->flops += FTRI + 1;
                 if (r.d < its.d) {
                     its = r;
                     r_max = fminf_(r_max, its.d);
@@ -1345,7 +1379,7 @@ static CP snch3_closest(const Tree<3>& T, V3 p, Cnt* c) {
             box_dist(node_min3(T.nodes[R]), node_max3(T.nodes[R]), p, dminR, dmaxR);
             bool hitR = dminR <= r_max;
             r_max = fminf_(r_max, dmaxR);
-            if (c) c->bytes += 2 * SNODE3, c->nodes += 2;
+            if (c) c->bytes += 2 * SNODE3, c->nodes += 2, c->flops += 2 * FB3 + 1;
             if (hitL && hitR) {
                 int closer = L, other = R;
                 if ((dminL == 0.0f) && (dminR == 0.0f)) {
@@ -1406,7 +1440,7 @@ static Its snch3_intersect(const Tree<3>& T, V3 x, V3 v, float r_max, Cnt* c) {
     Its its;
     std::memset(&its, 0, sizeof(its));
     float tmin;
-    if (c) c->bytes += SNODE3, c->nodes++;
+    if (c) c->bytes += SNODE3, c->nodes++, c->flops += FR3;
     if (!box_ray(node_min3(T.nodes[0]), node_max3(T.nodes[0]), x, v, r_max, tmin)) return its;
     int st_i[64];
     float st_d[64];
@@ -1423,7 +1457,7 @@ static Its snch3_intersect(const Tree<3>& T, V3 x, V3 v, float r_max, Cnt* c) {
         if (node.nref > 0) {
             for (int j = 0; j < node.nref; ++j) {
                 Its r;
-                if (c) c->bytes += TRI, c->prims++;
+                if (c) c->bytes += TRI, c->prims++, c->flops += FTRIR;
                 if (tri_ray(T.prims[node.off + j], node.off + j, x, v, r_max, r) && r.d < r_max) {
                     r_max = r.d;
                     its = r;
@@ -1434,7 +1468,7 @@ static Its snch3_intersect(const Tree<3>& T, V3 x, V3 v, float r_max, Cnt* c) {
             float t0, t1;
             bool h0 = box_ray(node_min3(T.nodes[L]), node_max3(T.nodes[L]), x, v, r_max, t0);
             bool h1 = box_ray(node_min3(T.nodes[R]), node_max3(T.nodes[R]), x, v, r_max, t1);
-            if (c) c->bytes += 2 * 24, c->nodes += 2;
+            if (c) c->bytes += 2 * 24, c->nodes += 2, c->flops += 2 * FR3 + 1;
             if (h0 && h1) {
                 int closer = L, other = R;
                 if (t1 < t0) {
@@ -1610,7 +1644,7 @@ static inline float h_of(const BC& bc, const Sub<D>& S, int prim, float t, float
 }
 
 // ----------------------------------------------------------------------------
-// Sampling (distr.py, greens_fn.py, greens_fn_3d.py, math.py:325-353)
+// Sampling (distr.py, greens_fn.py, greens_fn_3d.py, math.py:275-303)
 // ----------------------------------------------------------------------------
 // DiscreteDistribution.sample_reuse_pmf (distr.py:52-107) with Dr.Jit binary_search
 template <int D>
@@ -1633,7 +1667,7 @@ static inline int cdf_sample(const Sub<D>& S, float value, float& reuse, float& 
         else
             end = mid;
     }
-    if (c) c->bytes += 4 * iters + 8;
+    if (c) c->bytes += 4 * iters + 8, c->flops += 2 * iters + 4;
     int idx = (int)start;
     pmf = S.pmf[idx] * S.pmf_norm;
     float cdf = idx > 0 ? S.cdf[idx - 1] * S.pmf_norm : 0.0f;
@@ -1694,11 +1728,11 @@ static inline V3 greens3_gradient(V3 c, float R, float r, V3 y) {
     float gn = (1.0f / r3 - 1.0f / (R * R * R)) / kFourPi;
     return mul(sub(y, c), gn);
 }
-static inline V2 uniform_dir2(Pcg& rng) {  // uniform_angle (math.py:325-327)
+static inline V2 uniform_dir2(Pcg& rng) {  // uniform_angle (math.py:275-277)
     float theta = (rng.next_float32() * 2.0f) * kPi;
     return v2(cr_cos(theta), cr_sin(theta));
 }
-static inline V3 uniform_sphere(Pcg& rng) {  // math.py:348-353
+static inline V3 uniform_sphere(Pcg& rng) {  // math.py:298-303
     float u0 = rng.next_float32();
     float u1 = rng.next_float32();
     float z = 1.0f - 2.0f * u0;
@@ -1758,6 +1792,7 @@ static WalkOut wost_u2(const Scene& sc, const BC& bc, const Params& P, V2 x, Pcg
     int i = 0;
     bool active = true;
     while ((i < P.nsteps) && active) {
+        if (c) c->flops += FSTEP2;
         float R = kInf, d_dir = kInf;
         if (Dn.present) {
             CP cp = bvh2_closest(Dn.tree, ip, c);
@@ -1839,7 +1874,7 @@ static WalkOut wost_u3(const Scene& sc, const BC& bc, const Params& P, V3 x, Pcg
                        bool count) {
     WalkOut out;
     out.term_prim = -1;
-    Cnt* c = count ? &out.cnt : nullptr;
+    Cnt* c0 = count ? &out.cnt : nullptr;
     const Sub<3>& F = sc.s3[0];
     const Sub<3>& Dn = sc.s3[1];
     const Sub<3>& Nm = sc.s3[2];
@@ -1849,6 +1884,17 @@ static WalkOut wost_u3(const Scene& sc, const BC& bc, const Params& P, V3 x, Pcg
     int i = 0;
     bool active = true;
     while ((i < P.nsteps) && active) {
+        // A step from a NaN position (a walk that overflowed to infinity) has a
+        // fixed outcome and needs no record: it counts as a step with no bytes
+        // and is tallied in `ff`.
+        const bool ff = std::isnan(ip.x) || std::isnan(ip.y) || std::isnan(ip.z);
+        Cnt* c = ff ? nullptr : c0;
+        if (c0) {
+            if (ff)
+                c0->ff++;
+            else
+                c0->flops += FSTEP3;
+        }
         float R = kInf, d_dir = kInf;
         if (Dn.present) {
             CP cp = snch3_closest(Dn.tree, ip, c);
@@ -1964,10 +2010,7 @@ static WalkOut walk_u(const Scene& sc, const BC& bc, const Params& P, const floa
         res += w.value;
         acc.steps += w.steps;
         acc.term_prim = w.term_prim;
-        acc.cnt.bytes += w.cnt.bytes;
-        acc.cnt.nodes += w.cnt.nodes;
-        acc.cnt.prims += w.cnt.prims;
-        acc.cnt.sils += w.cnt.sils;
+        acc.cnt.add(w.cnt);
     }
     acc.value = res / (float)P.nsubwalks;
     return acc;
@@ -2145,7 +2188,7 @@ struct PairOut {
     int steps;
 };
 static PairOut fused_pair(const Scene& sc, const BC& bc, const Params& P, const float* x, float R, bool inside,
-                          Pcg& rng) {
+                          Pcg& rng, Cnt* cnt = nullptr) {
     PairOut o;
     std::memset(&o, 0, sizeof(o));
     if (!inside) return o;
@@ -2159,9 +2202,10 @@ static PairOut fused_pair(const Scene& sc, const BC& bc, const Params& P, const 
         greens2_sample(p, R, dir_src, rng, p_src, r_src, pdf_src);
         float gnorm = R * R / 4.0f;
         uint64_t state = rng.state;
-        WalkOut wp = wost_u2(sc, bc, P, add(p, mul(dir, R)), rng, false, v2(0, 0), false);
+        WalkOut wp = wost_u2(sc, bc, P, add(p, mul(dir, R)), rng, false, v2(0, 0), cnt != nullptr);
         rng.state = state;
-        WalkOut wm = wost_u2(sc, bc, P, sub(p, mul(dir, R)), rng, false, v2(0, 0), false);
+        WalkOut wm = wost_u2(sc, bc, P, sub(p, mul(dir, R)), rng, false, v2(0, 0), cnt != nullptr);
+        if (cnt) cnt->add(wp.cnt), cnt->add(wm.cnt);
         float fsrc = src_f(bc);
         float s_p = gnorm * fsrc, s_m = gnorm * fsrc;
         V2 gs = divs(greens2_gradient(p, R, r_src, p_src), pdf_src * gnorm);
@@ -2179,9 +2223,10 @@ static PairOut fused_pair(const Scene& sc, const BC& bc, const Params& P, const 
         greens3_sample(p, R, dir, rng, p_src, r_src, pdf_src);
         float gnorm = R * R / 6.0f;
         uint64_t state = rng.state;
-        WalkOut wp = wost_u3(sc, bc, P, add(p, mul(dir, R)), rng, false, v3(0, 0, 0), false);
+        WalkOut wp = wost_u3(sc, bc, P, add(p, mul(dir, R)), rng, false, v3(0, 0, 0), cnt != nullptr);
         rng.state = state;
-        WalkOut wm = wost_u3(sc, bc, P, sub(p, mul(dir, R)), rng, false, v3(0, 0, 0), false);
+        WalkOut wm = wost_u3(sc, bc, P, sub(p, mul(dir, R)), rng, false, v3(0, 0, 0), cnt != nullptr);
+        if (cnt) cnt->add(wp.cnt), cnt->add(wm.cnt);
         float up = wp.value, um = wm.value;
         if (bad(up) || std::fabs(up) > 10000.0f) up = 0.0f;
         if (bad(um) || std::fabs(um) > 10000.0f) um = 0.0f;
@@ -2281,6 +2326,64 @@ void orc_sub_export(void* s, int sub, float* box, int32_t* ints, float* cone, in
         go(sc->s2[sub], 2);
     else
         go(sc->s3[sub], 3);
+}
+
+// Far-field check (3D).  The walk kernels skip two traversals from points far
+// outside the scene on the strength of distance bounds: lo / hi = the norms of
+// the per-axis nearest / farthest offsets from x to the sub-scene's vertex box
+// widened by 1e-3 (1 + max |coordinate|), for |x_i| <= 2^59 / (1 + max
+// |coordinate|).  Claim: every
+// distance the queries compute lies in [lo, hi].  For each point this runs the
+// reference queries and reports: out[4i+0] = 1 if the bounds apply (2: and
+// the Dirichlet box's lo == hi); +1 = the
+// Dirichlet closest-point distance lies in [lo, hi] of the Dirichlet box (and
+// equals lo when lo == hi); +2 = with r_max = star_bound(full, x, that distance)
+// <= lo of the full box, the closest-silhouette search finds nothing (1), or
+// the shortcut does not apply (2); +3 = every silhouette distance found lies in
+// the full box's [lo, hi] (1) or no silhouette was found (2).
+static bool far_bounds(const Sub<3>& S, V3 x, float& lo, float& hi) {
+    float big = 0.0f;
+    for (int k = 0; k < 3; ++k) big = fmaxf_(big, fmaxf_(std::fabs(S.bb_min[k]), std::fabs(S.bb_max[k])));
+    const float lim = 576460752303423488.0f / (1.0f + big);  // 2^59 / (1 + max |coordinate|)
+    if (!(std::fabs(x.x) <= lim && std::fabs(x.y) <= lim && std::fabs(x.z) <= lim)) return false;
+    float m = 1e-3f * (1.0f + big);
+    float xs[3] = {x.x, x.y, x.z}, l[3], h[3];
+    for (int k = 0; k < 3; ++k) {
+        float a = S.bb_min[k] - m, b = S.bb_max[k] + m;
+        l[k] = xs[k] < a ? a - xs[k] : (xs[k] > b ? xs[k] - b : 0.0f);
+        h[k] = fmaxf_(std::fabs(xs[k] - a), std::fabs(xs[k] - b));
+    }
+    lo = norm(v3(l[0], l[1], l[2]));
+    hi = norm(v3(h[0], h[1], h[2]));
+    return true;
+}
+
+void orc_far_check(void* s, const float* pts, int64_t n, int32_t* out) {
+    Scene* sc = (Scene*)s;
+    const Sub<3>& F = sc->s3[0];
+    const Sub<3>& Dn = sc->s3[1];
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t i = 0; i < n; ++i) {
+        V3 x = v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+        int32_t* o = out + 4 * i;
+        float lo, hi, flo, fhi;
+        o[0] = far_bounds(Dn, x, lo, hi) && far_bounds(F, x, flo, fhi);
+        o[1] = o[2] = o[3] = 0;
+        if (!o[0]) continue;
+        if (lo == hi) o[0] = 2;  // the Dirichlet bounds coincide
+        CP cp = snch3_closest(Dn.tree, x, nullptr);
+        float d = std::fabs(cp.d);
+        o[1] = (d >= lo && d <= hi && (lo != hi || d == lo)) ? 1 : 0;
+        float m3[3];
+        for (int k = 0; k < 3; ++k) m3[k] = fmaxf_(std::fabs((&x.x)[k] - F.bb_min[k]), std::fabs((&x.x)[k] - F.bb_max[k]));
+        float r_max = fminf_(norm(v3(m3[0], m3[1], m3[2])), d);  // star_radius's bound (mesh.py:337-346)
+        SilRec r = snch_closest_sil<3>(F, &x.x, r_max, nullptr);
+        if (r_max <= flo)
+            o[2] = r.valid ? 0 : 1;
+        else
+            o[2] = 2;
+        o[3] = r.valid ? ((r.d >= flo && r.d <= fhi) ? 1 : 0) : 2;
+    }
 }
 
 // Queries.  kind: 0 BVH closest point, 1 ray intersect (BVH), 2 closest silhouette,
@@ -2555,14 +2658,15 @@ void orc_trace_step(void* s, const float* pts, int64_t n, const int32_t* ip, con
 // Fused u + grad u.  pair_u[n*np], pair_g[n*np*dim], pair_steps[n*np] optional.
 void orc_solve_fused(void* s, const float* pts, int64_t n, const int32_t* ip, const float* fp, int bc_code,
                      const float* bc_c, const float* gv, const float* hv, int src_code, float src_c, float* u,
-                     float* grad, float* pair_u, float* pair_g, int32_t* pair_steps, uint64_t* steps_total) {
+                     float* grad, float* pair_u, float* pair_g, int32_t* pair_steps, uint64_t* steps_total,
+                     uint64_t* counts) {
     Scene* sc = (Scene*)s;
     Params P = make_params(ip, fp);
     BC bc = make_bc(bc_code, bc_c, gv, hv, src_code, src_c);
     int D = sc->dim;
     int64_t NP = (P.nwalks + 1) / 2;
-    uint64_t steps = 0;
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : steps)
+    uint64_t steps = 0, cb = 0, cf = 0, cn = 0, cp_ = 0, cs = 0, cff = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : steps, cb, cf, cn, cp_, cs, cff)
     for (int64_t i = 0; i < n; ++i) {
         const float* x = pts + i * D;
         float R;
@@ -2580,7 +2684,9 @@ void orc_solve_fused(void* s, const float* pts, int64_t n, const int32_t* ip, co
         for (int64_t k = 0; k < NP; ++k) {
             uint64_t lane = (uint64_t)((P.base + i) * NP + k);
             Pcg rng = pcg_for_lane(a, b, lane);
-            PairOut o = fused_pair(*sc, bc, P, x, R, ins, rng);
+            Cnt k_;
+            PairOut o = fused_pair(*sc, bc, P, x, R, ins, rng, counts ? &k_ : nullptr);
+            cb += k_.bytes, cf += k_.flops, cn += k_.nodes, cp_ += k_.prims, cs += k_.sils, cff += k_.ff;
             su += o.usum;
             for (int q = 0; q < D; ++q) sg[q] += o.g[q];
             steps += (uint64_t)o.steps;
@@ -2596,6 +2702,8 @@ void orc_solve_fused(void* s, const float* pts, int64_t n, const int32_t* ip, co
             for (int q = 0; q < D; ++q) grad[i * D + q] = (float)(sg[q] / nw);
     }
     if (steps_total) *steps_total = steps;
+    // counts[6] = {bytes, flops, nodes, primitives, silhouettes, NaN-position steps}
+    if (counts) counts[0] = cb, counts[1] = cf, counts[2] = cn, counts[3] = cp_, counts[4] = cs, counts[5] = cff;
 }
 
 // RNG / hashing exports for known-answer tests

@@ -89,3 +89,15 @@ def test_ctypes_structs_and_constants_match_header(tmp_path):
             assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
     for k, v in consts.items():
         assert got[("const", k)] == v, k
+
+
+def test_replay_solver_must_be_same_walk_family():
+    """CustomSolver reuses the forward's guard flags only when the backward
+    solver draws the same walks: same fields and the same walk family."""
+    import dwost_amd as dw
+    from dwost_amd.solvers import _same_walks
+
+    assert _same_walks(dw.WoSt(), dw.WoSt())
+    assert not _same_walks(dw.WoSt(), dw.WoS())  # equal fields, other walk
+    assert not _same_walks(dw.WoSt3D(), dw.WoS3D())
+    assert not _same_walks(dw.WoSt(), dw.WoSt(nwalks=7))

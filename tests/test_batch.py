@@ -116,3 +116,41 @@ def test_autotune_keeps_results(gpu, holes_spec):
     assert (sol.block_size, sol.blocks_per_cu, sol.spatial_order, sol.schedule) == rep["config"]
     assert len(rep["table"]) == 36
     assert torch.equal(sol.solve_grad(q, sc, seed=3)["u"].view(torch.int32), ref.view(torch.int32))
+
+
+def test_batch_points_updated_in_place(gpu, batch_items):
+    """The batch reads each item's points by address: new values written in place
+    between solves (eager and graph-replayed) give the per-scene results on the
+    new points, bit for bit."""
+    items, _ = batch_items
+    keep = [it.points.clone() for it in items]
+    try:
+        for graph in (False, True):
+            b = gpu.BatchSolver(items, graph=graph)
+            b.solve()
+            torch.cuda.synchronize()
+            for k, it in enumerate(items):  # reversed point order: other values at every address
+                it.points.copy_(keep[k].flip(0))
+            outs = b.solve()
+            torch.cuda.synchronize()
+            refs = [it.solver.solve_grad(it.points, it.scene, seed=it.seed) for it in items]
+            torch.cuda.synchronize()
+            _check(outs, [(r["u"].cpu().numpy(), r["grad"].cpu().numpy()) for r in refs])
+            for k, it in enumerate(items):
+                it.points.copy_(keep[k])
+            b.close()
+    finally:
+        for k, it in enumerate(items):
+            it.points.copy_(keep[k])
+        torch.cuda.synchronize()
+
+
+def test_batch_rejects_converted_points(gpu, batch_items):
+    """Points the batch would have to copy (host arrays, (dim, N) layout, float64,
+    non-contiguous views) are refused: the copy would never see in-place updates."""
+    items, _ = batch_items
+    it = items[0]
+    p = it.points
+    for bad in (p.cpu().numpy(), p.t(), p.double(), torch.cat([p, p], 1)[:, :2]):
+        with pytest.raises(ValueError):
+            gpu.BatchSolver([gpu.BatchItem(it.solver, it.scene, bad, seed=it.seed)])

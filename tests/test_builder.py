@@ -96,14 +96,8 @@ def test_snch_cones_contain_face_normals(oracle, which, star_spec, ico_spec):
         end[i] = j
     # face normals of a topology silhouette
     if D == 2:
-        fn = {}
-        for v in range(len(V)):
-            pass
-        edges = I
-        prev = {}
-        nxt = {}
         adj = [[] for _ in range(len(V))]
-        for a, b in edges:
+        for a, b in I:
             adj[a].append(b)
             adj[b].append(a)
 

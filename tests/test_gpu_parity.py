@@ -38,6 +38,21 @@ def _same_bits(a, b):
     return np.array_equal(a, b)
 
 
+# Fast-cone mode (the transcendental-free SNCH cone test) may differ from the
+# literal test only where the literal test's own rounding prunes a node holding
+# the nearest silhouette: a branch flip.  The stated bound is 0.1% of the walks
+# (floor: none at all below 1,000 walks); exact-cone runs must match every walk.
+FAST_CONE_FLIPS = 1e-3
+
+
+def assert_parity(same, exact, label):
+    same = np.asarray(same, bool).ravel()
+    bad = np.flatnonzero(~same)
+    allowed = 0 if exact else int(FAST_CONE_FLIPS * same.size)
+    print(f"{label}: {same.size - bad.size} of {same.size} identical ({bad.size} branch flips, {allowed} allowed)")
+    assert bad.size <= allowed, f"{label}: {bad.size} mismatches at {bad[:20].tolist()}"
+
+
 def _scenes(gpu, oracle, spec):
     cls = gpu.Scene2D if spec.dim == 2 else gpu.Scene3D
     return cls(spec.vertices, spec.indices, spec.types), oracle.OracleScene(spec.vertices, spec.indices, spec.types)
@@ -152,9 +167,8 @@ def _walk_parity(gpu, oracle, spec, n, W, nsteps, eps, solver_cls, exact_cones=T
                     av=sol.antithetic_variates, seed=7, per_walk=True)
     gu, gst, gtm = _np(g["walk_u"]), _np(g["walk_steps"]), _np(g["walk_term"])
     same = (gu.view(np.uint32) == o["walk_u"].view(np.uint32)) & (gst == o["walk_steps"]) & (gtm == o["walk_term"])
-    print(f"walk parity (exact_cones={exact_cones}) {same.mean():.6f} ({(~same).sum()} of {same.size} differ); "
-          f"mean steps {gst.mean():.1f}")
-    assert same.mean() >= 0.999
+    print(f"mean steps {gst.mean():.1f}")
+    assert_parity(same, exact_cones, f"walk parity (exact_cones={exact_cones})")
     assert int(_np(g["steps"])[0]) == int(gst.astype(np.int64).sum())
     # per-point means: Kahan fp32 on the GPU vs fp64 accumulation on the CPU
     assert np.allclose(_np(g["u"]), o["u"], rtol=2e-6, atol=2e-6)
@@ -180,21 +194,20 @@ def test_walk_u_parity_3d(gpu, oracle, ico_spec, exact):
     _walk_parity(gpu, oracle, ico_spec, 32, 32, 1024, 1e-3, gpu.WoSt3D, exact)
 
 
+@pytest.mark.parametrize("exact", [True, False])
 @pytest.mark.parametrize("which", ["star", "ico"])
-def test_dude_parity(gpu, oracle, which, star_spec, ico_spec):
+def test_dude_parity(gpu, oracle, which, exact, star_spec, ico_spec):
     spec = dict(star=star_spec, ico=ico_spec)[which]
     gs, os_ = _scenes(gpu, oracle, spec)
     gs.boundary = gpu.BoundaryCondition(spec.bc_code)
     n, W = 32, 32
     q = spec.queries[:n]
     e = _dirs(n, spec.dim, 13)
-    sol = (gpu.WoSt if spec.dim == 2 else gpu.WoSt3D)(eps=1e-3, nwalks=W, nsteps=1024)
+    sol = (gpu.WoSt if spec.dim == 2 else gpu.WoSt3D)(eps=1e-3, nwalks=W, nsteps=1024, exact_cones=exact)
     g = _np(sol.solve_dude(_t(q), _t(e), gs, seed=3))
     o = os_.solve_dude(q, e, spec.bc_code, W, nsteps=1024, eps=1e-3, cv=sol.control_variates,
                        av=sol.antithetic_variates, seed=3)["dude"]
-    same = g.view(np.uint32) == o.view(np.uint32)
-    print(f"dude parity {same.mean():.4f}")
-    assert same.mean() >= 0.97
+    assert_parity(g.view(np.uint32) == o.view(np.uint32), exact, f"dude parity (exact_cones={exact})")
 
 
 @pytest.mark.parametrize("exact", [True, False])
@@ -212,8 +225,7 @@ def test_fused_parity(gpu, oracle, which, exact, star_spec, holes_spec, ico_spec
     pv = _np(g["pair_vals"])
     same = (pv[:, 0].view(np.uint32) == o["pair_u"].view(np.uint32)) & np.all(
         pv[:, 1:].view(np.uint32) == o["pair_g"].view(np.uint32), axis=1) & (_np(g["pair_steps"]) == o["pair_steps"])
-    print(f"fused pair parity (exact_cones={exact}) {same.mean():.6f}")
-    assert same.mean() >= 0.999
+    assert_parity(same, exact, f"fused pair parity (exact_cones={exact})")
     assert int(_np(g["steps"])[0]) == o["steps"]
     assert np.allclose(_np(g["u"]), o["u"], rtol=1e-5, atol=1e-5)
     assert np.allclose(_np(g["grad"]), o["grad"], rtol=1e-4, atol=1e-4)

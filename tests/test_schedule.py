@@ -139,3 +139,35 @@ def test_persistent_dude_matches_simple(gpu, dim, wos, monkeypatch):
         got, steps = replace(sol, schedule=sched).solve_dude(q, e, sc, seed=4, count_steps=True)
         assert np.array_equal(_bits(got), _bits(ref)), sched
         assert int(steps[0]) == int(ref_steps[0])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_simple_switch_fused_and_flags(gpu, dim, monkeypatch):
+    """DWOST_KERNEL=simple: an exact-cone fused solve without guard flags runs the
+    one-pair-per-thread kernel and returns the persistent kernel's bits; a solve
+    that asks for guard flags (the CustomSolver forward) or for the fast cone
+    test keeps the persistent kernel, so the flags are written and the cone mode
+    is the one asked for."""
+    sc, sol, q = _case(gpu, dim)
+    sol = replace(sol, nwalks=16)
+    n, npairs = q.shape[0], 8
+    exact = replace(sol, exact_cones=True)
+    ref = exact.solve_grad(q, sc, seed=9, per_pair=True)
+    fast_ref = sol.solve_grad(q, sc, seed=9, per_pair=True)
+    flags_ref = torch.full((n * npairs,), 0xEE, dtype=torch.uint8, device="cuda")
+    exact.solve_grad(q, sc, seed=9, pair_flags=flags_ref)
+    torch.cuda.synchronize()
+    monkeypatch.setenv("DWOST_KERNEL", "simple")
+    got = exact.solve_grad(q, sc, seed=9, per_pair=True)
+    got_fast = sol.solve_grad(q, sc, seed=9, per_pair=True)
+    flags = torch.full((n * npairs,), 0xEE, dtype=torch.uint8, device="cuda")
+    with_flags = exact.solve_grad(q, sc, seed=9, per_pair=True, pair_flags=flags)
+    torch.cuda.synchronize()
+    monkeypatch.delenv("DWOST_KERNEL")
+    for k in ("u", "grad", "pair_vals", "pair_steps"):
+        assert np.array_equal(_bits(got[k]), _bits(ref[k])), k
+        assert np.array_equal(_bits(got_fast[k]), _bits(fast_ref[k])), k
+        assert np.array_equal(_bits(with_flags[k]), _bits(ref[k])), k
+    f = flags.cpu().numpy()
+    assert np.array_equal(f, flags_ref.cpu().numpy())
+    assert np.all(f <= 3)  # written: two guard bits per pair, never the fill value

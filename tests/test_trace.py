@@ -72,8 +72,9 @@ def test_oracle_trace_2d_half_step_and_outside(oracle):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("exact", [True, False])
 @pytest.mark.parametrize("which", ["star", "ico"])
-def test_trace_parity(oracle, which, star_spec, ico_spec):
+def test_trace_parity(oracle, which, exact, star_spec, ico_spec):
     torch = pytest.importorskip("torch")
     if not torch.cuda.is_available():
         pytest.skip("no HIP device")
@@ -83,7 +84,7 @@ def test_trace_parity(oracle, which, star_spec, ico_spec):
     cls = gpu.Scene2D if spec.dim == 2 else gpu.Scene3D
     scene = cls(spec.vertices, spec.indices, spec.types, gpu.BoundaryCondition(spec.bc_code))
     os_ = oracle.OracleScene(spec.vertices, spec.indices, spec.types)
-    sol = (gpu.WoSt if spec.dim == 2 else gpu.WoSt3D)(eps=1e-3, nwalks=16, nsteps=1024)
+    sol = (gpu.WoSt if spec.dim == 2 else gpu.WoSt3D)(eps=1e-3, nwalks=16, nsteps=1024, exact_cones=exact)
     tr = gpu.StreamlineTracer(scene, sol, step_size=0.02, nwalks=4)
     q = np.ascontiguousarray(spec.queries[:24])
     p = torch.as_tensor(q).cuda()
@@ -94,8 +95,11 @@ def test_trace_parity(oracle, which, star_spec, ico_spec):
                            av=sol.antithetic_variates, seed=it)["pos"]
         g = p.cpu().numpy()
         same = np.all(g.view(np.uint32) == o.view(np.uint32), axis=1)
-        print(f"trace step {it}: {same.mean():.4f} of points bit-identical")
-        assert same.mean() >= 0.9
+        bad = np.flatnonzero(~same)
+        print(f"trace step {it} (exact_cones={exact}): {same.sum()} of {same.size} points bit-identical")
+        # exact cones: every point; fast cones: a branch flip in one of the
+        # 4 x 16 x dim walks behind a point moves it, at most one point allowed
+        assert bad.size <= (0 if exact else 1), bad.tolist()
         assert np.allclose(g, o, rtol=1e-3, atol=1e-3)
         o = g  # continue both from the GPU positions
     with pytest.raises(Exception):

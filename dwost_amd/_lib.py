@@ -43,6 +43,7 @@ class SolverParams(C.Structure):
                 ("block_size", C.c_int32), ("blocks_per_cu", C.c_int32), ("spatial_order", C.c_int32),
                 ("schedule", C.c_int32), ("gate", C.c_int32)]
 
+NOALLOC_SCRATCH_BYTES = 64 << 20  # DW_NOALLOC_SCRATCH_BYTES
 SOLVE_EXACT_CONES = 1
 SOLVE_WOS = 2
 RAY_REFERENCE, RAY_NEAREST = 0, 1  # dw_intersect modes
@@ -81,6 +82,7 @@ def lib():
                                 vp, vp]
     L.dw_solve_fused.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp, vp, vp, vp,
                                  vp]
+    L.dw_solve_fused_noalloc.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp, vp]
     L.dw_batch_create.argtypes = [C.POINTER(BatchItem), i32, vp, i32, C.POINTER(vp)]
     L.dw_batch_solve.argtypes = [vp, vp]
     L.dw_batch_destroy.argtypes = [vp]
@@ -91,7 +93,7 @@ def lib():
     for name in ("dw_scene_create", "dw_scene_destroy", "dw_scene_info", "dw_scene_export", "dw_closest_point",
                  "dw_intersect", "dw_closest_silhouette", "dw_star_radius", "dw_solve_u", "dw_solve_dude",
                  "dw_trace_step",
-                 "dw_solve_fused", "dw_adjoint_u", "dw_adjoint_fused", "dw_batch_create", "dw_batch_solve"):
+                 "dw_solve_fused", "dw_solve_fused_noalloc", "dw_adjoint_u", "dw_adjoint_fused", "dw_batch_create", "dw_batch_solve"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -1659,14 +1659,14 @@ size_t order_bytes(int64_t n, const dw_solver_params* p) {
 // Builds the job queue of a persistent launch: zeroes the range counters (a
 // fill kernel, so the sequence can be captured in a graph) and, when
 // order_bytes(n) > 0, sorts the points by Morton cell into `ows`.
-int make_queue(const dw_scene* s, StreamCtx& c, const float* pts, int64_t n, int64_t per, void* ows, hipStream_t st,
-               JobQ& Q) {
-    Q.ctr = c.ctr;
+int make_queue(const dw_scene* s, unsigned long long* ctr, const float* pts, int64_t n, int64_t per, void* ows,
+               hipStream_t st, JobQ& Q) {
+    Q.ctr = ctr;
     Q.order = nullptr;
     Q.n = n;
     Q.per = per;
     Q.gate = kGate;
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kQueueCtrs), 0, st, c.ctr, kQueueCtrs, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kQueueCtrs), 0, st, ctr, kQueueCtrs, 0ull);
     DW_HIP(hipGetLastError());
     if (!ows) return DW_OK;
     const dwb::SubHost& F = s->host.sub[0];
@@ -1760,6 +1760,110 @@ int zero_u64(unsigned long long* p, hipStream_t st) {
 bool use_simple_kernels(const dw_solver_params* p, const void* flags_out) {
     const char* e = getenv("DWOST_KERNEL");
     return e && strcmp(e, "simple") == 0 && (p->flags & DW_SOLVE_EXACT_CONES) && !flags_out;
+}
+
+// Argument checks shared by the fused entry points.
+int check_fused(const dw_scene* s, const float* pts, int64_t n, const dw_solver_params* params,
+                const dw_boundary* boundary, const float* u, const float* grad) {
+    int rc;
+    if ((rc = check_scene(s, 0))) return rc;
+    if ((rc = check_params(params)) || (rc = check_bc(boundary))) return rc;
+    if (n < 0 || (n > 0 && (!pts || !u || !grad))) return fail(DW_ERR_ARG, "bad arrays");
+    if (params->flags & DW_SOLVE_WOS) return fail(DW_ERR_ARG, "the fused estimator runs WoSt walks only");
+    return DW_OK;
+}
+
+// Device buffers of one fused solve: queue counters, first-ball radius and
+// inside flag per point, per-pair partial results and step counts, and the
+// spatial order (nullptr: identity order).
+struct FusedBufs {
+    unsigned long long* ctr;
+    float* R;
+    int* ins;
+    float* pv;
+    int32_t* ps;
+    void* order;
+};
+
+// The launch sequence of a fused solve: point preparation, the walk kernel
+// (persistent, or the one-pair-per-thread form), per-point reduction.
+int fused_launch(const dw_scene* s, const float* pts, int64_t n, const dw_solver_params* params,
+                 const dw_boundary* boundary, float* u, float* grad, uint8_t* pair_flags, uint64_t* total_steps,
+                 bool simple, const FusedBufs& b, hipStream_t st) {
+    int rc;
+    const int D = s->dev.dim;
+    const int npairs = (params->nwalks + 1) / 2;
+    const int64_t lanes = n * (int64_t)npairs;
+    hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(n)), dim3(kBlock), kStackLds, st, s->dev, pts, n, b.R, b.ins);
+    DW_HIP(hipGetLastError());
+    if (simple) {
+        hipLaunchKernelGGL(k_fused, dim3(blocks_for(lanes)), dim3(kBlock), kStackLds, st, s->dev, to_bc(boundary),
+                           to_params(params), pts, b.R, b.ins, n, npairs, b.pv, b.ps);
+        DW_HIP(hipGetLastError());
+    } else {
+        JobQ Q;
+        if ((rc = make_queue(s, b.ctr, pts, n, npairs, b.order, st, Q))) return rc;
+        const LaunchCfg cfg = launch_cfg(to_params(params), s->dev.dim);
+        Q.gate = cfg.gate;
+        const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
+        FusedItem one;
+        std::memset(&one, 0, sizeof(one));
+        one.sc = s->dev_ptr;
+        one.bc = to_bc(boundary);
+        one.P = to_params(params);
+        one.pts = pts;
+        one.R = b.R;
+        one.inside = b.ins;
+        one.pair_vals = b.pv;
+        one.pair_steps = b.ps;
+        one.pair_flags = pair_flags;
+        one.n = n;
+        one.npairs = npairs;
+        const FusedItem* none = nullptr;
+        if (D == 2)
+            rc = launch_pers(exact, cfg, DW_PERS(k_fused_pers, 2, false), st, one, none, 1, Q);
+        else
+            rc = launch_pers(exact, cfg, DW_PERS(k_fused_pers, 3, false), st, one, none, 1, Q);
+        if (rc) return rc;
+    }
+    if (total_steps && (rc = zero_u64((unsigned long long*)total_steps, st))) return rc;
+    // u = sum(usum) / (2 npairs); grad = sum(g) / (2 npairs)
+    float scale = 1.0f / (float)(2 * npairs);
+    hipLaunchKernelGGL(k_reduce_mean, dim3(blocks_for(n * 64)), dim3(kBlock), 0, st, b.pv, n, npairs, D + 1, scale, u,
+                       grad, total_steps ? b.ps : nullptr, (unsigned long long*)total_steps);
+    DW_HIP(hipGetLastError());
+    return DW_OK;
+}
+
+}  // namespace
+
+// Scratch of dw_solve_fused_noalloc, for callers that cannot allocate: one
+// static device array per GPU, held until the process ends.  Host code never
+// uses the array's name as an address: the device address comes from
+// hipGetSymbolAddress, once per device with that device current (scratch_base).
+static __device__ __align__(256) unsigned char scratch[DW_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr size_t kScratchBytes = (size_t)DW_NOALLOC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+
+int scratch_base(void** out) {
+    int dev = 0;
+    DW_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(DW_ERR_ARG, "device index out of range for the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(DW_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_addr[dev] = p;
+    }
+    *out = g_scratch_addr[dev];
+    return DW_OK;
 }
 
 }  // namespace
@@ -2015,7 +2119,7 @@ int dw_solve_u(const dw_scene* s, const float* pts, int64_t n, const dw_solver_p
         DW_HIP(hipGetLastError());
     } else {
         JobQ Q;
-        if ((rc = make_queue(s, *c, pts, n, params->nwalks, obytes ? cur : nullptr, st, Q))) return rc;
+        if ((rc = make_queue(s, c->ctr, pts, n, params->nwalks, obytes ? cur : nullptr, st, Q))) return rc;
         const LaunchCfg cfg = launch_cfg(to_params(params), s->dev.dim);
         Q.gate = cfg.gate;
         const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
@@ -2057,7 +2161,7 @@ int dw_solve_dude(const dw_scene* s, const float* pts, const float* e, int64_t n
     if (total_steps && (rc = zero_u64((unsigned long long*)total_steps, st))) return rc;
     if (persistent) {
         JobQ Q;
-        if ((rc = make_queue(s, *c, pts, n, 1, obytes ? (char*)p + (size_t)n * 8 : nullptr, st, Q))) return rc;
+        if ((rc = make_queue(s, c->ctr, pts, n, 1, obytes ? (char*)p + (size_t)n * 8 : nullptr, st, Q))) return rc;
         const LaunchCfg cfg = launch_cfg(to_params(params), s->dev.dim);
         Q.gate = cfg.gate;
         const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
@@ -2101,7 +2205,8 @@ int dw_trace_step(const dw_scene* s, const float* pts, int64_t n, const dw_solve
         hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(n)), dim3(kBlock), kStackLds, st, s->dev, pts, n, R, ins);
         DW_HIP(hipGetLastError());
         JobQ Q;
-        if ((rc = make_queue(s, *c, pts, n, samples_per_point, obytes ? (char*)(ins + n) : nullptr, st, Q))) return rc;
+        if ((rc = make_queue(s, c->ctr, pts, n, samples_per_point, obytes ? (char*)(ins + n) : nullptr, st, Q)))
+            return rc;
         const LaunchCfg cfg = launch_cfg(to_params(params), D);
         Q.gate = cfg.gate;
         const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
@@ -2133,71 +2238,62 @@ int dw_solve_fused(const dw_scene* s, const float* pts, int64_t n, const dw_solv
                    const dw_boundary* boundary, float* u, float* grad, float* pair_vals, int32_t* pair_steps,
                    uint8_t* pair_flags, uint64_t* total_steps, void* stream) {
     int rc;
-    if ((rc = check_scene(s, 0))) return rc;
-    if ((rc = check_params(params)) || (rc = check_bc(boundary))) return rc;
-    if (n < 0 || (n > 0 && (!pts || !u || !grad))) return fail(DW_ERR_ARG, "bad arrays");
-    if (params->flags & DW_SOLVE_WOS) return fail(DW_ERR_ARG, "the fused estimator runs WoSt walks only");
+    if ((rc = check_fused(s, pts, n, params, boundary, u, grad))) return rc;
     if (n == 0) return DW_OK;
     const int D = s->dev.dim;
-    const int npairs = (params->nwalks + 1) / 2;
-    const int64_t lanes = n * (int64_t)npairs;
+    const int64_t lanes = n * (int64_t)((params->nwalks + 1) / 2);
     hipStream_t st = (hipStream_t)stream;
     StreamCtx* c;
     if ((rc = stream_ctx(st, &c))) return rc;
     const bool simple = use_simple_kernels(params, pair_flags);
+    FusedBufs b;
+    b.ctr = c->ctr;
     size_t obytes = simple ? 0 : order_bytes(n, params);
     size_t need = (size_t)n * 8 + (pair_vals ? 0 : (size_t)lanes * (D + 1) * 4) +
                   (pair_steps ? 0 : (size_t)lanes * 4) + obytes;
     void* p;
     if ((rc = workspace(*c, need, st, &p))) return rc;
     char* cur = (char*)p;
-    float* R = (float*)cur;
+    b.R = (float*)cur;
     cur += (size_t)n * 4;
-    int* ins = (int*)cur;
+    b.ins = (int*)cur;
     cur += (size_t)n * 4;
-    float* pv = pair_vals;
-    if (!pv) pv = (float*)cur, cur += (size_t)lanes * (D + 1) * 4;
-    int32_t* ps = pair_steps;
-    if (!ps) ps = (int32_t*)cur, cur += (size_t)lanes * 4;
-    hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(n)), dim3(kBlock), kStackLds, st, s->dev, pts, n, R, ins);
-    DW_HIP(hipGetLastError());
-    if (simple) {
-        hipLaunchKernelGGL(k_fused, dim3(blocks_for(lanes)), dim3(kBlock), kStackLds, st, s->dev, to_bc(boundary),
-                           to_params(params), pts, R, ins, n, npairs, pv, ps);
-        DW_HIP(hipGetLastError());
-    } else {
-        JobQ Q;
-        if ((rc = make_queue(s, *c, pts, n, npairs, obytes ? cur : nullptr, st, Q))) return rc;
-        const LaunchCfg cfg = launch_cfg(to_params(params), s->dev.dim);
-        Q.gate = cfg.gate;
-        const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
-        FusedItem one;
-        std::memset(&one, 0, sizeof(one));
-        one.sc = s->dev_ptr;
-        one.bc = to_bc(boundary);
-        one.P = to_params(params);
-        one.pts = pts;
-        one.R = R;
-        one.inside = ins;
-        one.pair_vals = pv;
-        one.pair_steps = ps;
-        one.pair_flags = pair_flags;
-        one.n = n;
-        one.npairs = npairs;
-        const FusedItem* none = nullptr;
-        if (D == 2)
-            rc = launch_pers(exact, cfg, DW_PERS(k_fused_pers, 2, false), st, one, none, 1, Q);
-        else
-            rc = launch_pers(exact, cfg, DW_PERS(k_fused_pers, 3, false), st, one, none, 1, Q);
-        if (rc) return rc;
-    }
-    if (total_steps && (rc = zero_u64((unsigned long long*)total_steps, st))) return rc;
-    // u = sum(usum) / (2 npairs); grad = sum(g) / (2 npairs)
-    float scale = 1.0f / (float)(2 * npairs);
-    hipLaunchKernelGGL(k_reduce_mean, dim3(blocks_for(n * 64)), dim3(kBlock), 0, st, pv, n, npairs, D + 1, scale, u,
-                       grad, total_steps ? ps : nullptr, (unsigned long long*)total_steps);
-    DW_HIP(hipGetLastError());
-    return DW_OK;
+    b.pv = pair_vals;
+    if (!b.pv) b.pv = (float*)cur, cur += (size_t)lanes * (D + 1) * 4;
+    b.ps = pair_steps;
+    if (!b.ps) b.ps = (int32_t*)cur, cur += (size_t)lanes * 4;
+    b.order = obytes ? cur : nullptr;
+    return fused_launch(s, pts, n, params, boundary, u, grad, pair_flags, total_steps, simple, b, st);
+}
+
+int dw_solve_fused_noalloc(const dw_scene* s, const float* pts, int64_t n, const dw_solver_params* params,
+                           const dw_boundary* boundary, float* u, float* grad, uint64_t* total_steps, void* stream) {
+    int rc;
+    if ((rc = check_fused(s, pts, n, params, boundary, u, grad))) return rc;
+    const int D = s->dev.dim;
+    const int64_t lanes = n * (int64_t)((params->nwalks + 1) / 2);
+    // scratch layout (each part 256-byte aligned): queue counters, first-ball
+    // radii, inside flags, per-pair partial results and step counts, order
+    const size_t obytes = order_bytes(n, params);
+    const size_t parts[6] = {(size_t)kQueueCtrs * 8, (size_t)n * 4, (size_t)n * 4, (size_t)lanes * (D + 1) * 4,
+                             (size_t)lanes * 4, obytes};
+    size_t off[6], need = 0;
+    for (int k = 0; k < 6; ++k) off[k] = need, need += (parts[k] + 255) & ~(size_t)255;
+    if (need > kScratchBytes)
+        return fail(DW_ERR_ARG, "dw_solve_fused_noalloc: n * ceil(nwalks / 2) needs " + std::to_string(need) +
+                                    " bytes of scratch, above its fixed " + std::to_string(kScratchBytes) +
+                                    " (DW_NOALLOC_SCRATCH_BYTES); use dw_solve_fused or fewer points per call");
+    if (n == 0) return DW_OK;
+    char* base;
+    if ((rc = scratch_base((void**)&base))) return rc;
+    FusedBufs b;
+    b.ctr = (unsigned long long*)(base + off[0]);
+    b.R = (float*)(base + off[1]);
+    b.ins = (int*)(base + off[2]);
+    b.pv = (float*)(base + off[3]);
+    b.ps = (int32_t*)(base + off[4]);
+    b.order = obytes ? base + off[5] : nullptr;
+    return fused_launch(s, pts, n, params, boundary, u, grad, nullptr, total_steps, false, b, (hipStream_t)stream);
 }
 
 int dw_adjoint_fused(const dw_scene* s, const float* pts, int64_t n, const dw_solver_params* params,
@@ -2225,7 +2321,7 @@ int dw_adjoint_fused(const dw_scene* s, const float* pts, int64_t n, const dw_so
     hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(n)), dim3(kBlock), kStackLds, st, s->dev, pts, n, R, ins);
     DW_HIP(hipGetLastError());
     JobQ Q;
-    if ((rc = make_queue(s, *c, pts, n, npairs, obytes ? (char*)p + (size_t)n * 8 : nullptr, st, Q))) return rc;
+    if ((rc = make_queue(s, c->ctr, pts, n, npairs, obytes ? (char*)p + (size_t)n * 8 : nullptr, st, Q))) return rc;
     const LaunchCfg cfg = launch_cfg(to_params(params), s->dev.dim);
     Q.gate = cfg.gate;
     const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
@@ -2254,7 +2350,7 @@ int dw_adjoint_u(const dw_scene* s, const float* pts, int64_t n, const dw_solver
     void* p = nullptr;
     if (obytes && (rc = workspace(*c, obytes, st, &p))) return rc;
     JobQ Q;
-    if ((rc = make_queue(s, *c, pts, n, params->nwalks, p, st, Q))) return rc;
+    if ((rc = make_queue(s, c->ctr, pts, n, params->nwalks, p, st, Q))) return rc;
     const LaunchCfg cfg = launch_cfg(to_params(params), s->dev.dim);
     Q.gate = cfg.gate;
     const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;

@@ -215,6 +215,23 @@ int dw_solve_fused(const dw_scene* scene, const float* pts, int64_t n, const dw_
                    const dw_boundary* boundary, float* u, float* grad, float* pair_vals, int32_t* pair_steps,
                    uint8_t* pair_flags, uint64_t* total_steps, void* stream);
 
+/* dw_solve_fused for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same kernels and the same bits as
+ * dw_solve_fused without per-pair outputs: u[n], grad[n][dim], total_steps
+ * (DEVICE uint64, optional).  Its scratch (queue counters, per-point first-ball
+ * radius and inside flag, per-pair partial results, spatial order) is one
+ * static device array of DW_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends; it calls no allocating routine.  A call whose scratch would
+ * exceed that size fails with DW_ERR_ARG before any launch: 3D needs about
+ * 20 B and 2D 16 B per (point, walk pair), plus 16 B per point (e.g. 3D
+ * n * ceil(nwalks / 2) <= ~3.3 million).  All calls on one GPU share its array
+ * and each leaves its contents behind: calls that overlap on one GPU (other
+ * streams, other threads) race and must be serialised by the caller. */
+#define DW_NOALLOC_SCRATCH_BYTES (64u << 20)
+int dw_solve_fused_noalloc(const dw_scene* scene, const float* pts, int64_t n, const dw_solver_params* params,
+                           const dw_boundary* boundary, float* u, float* grad, uint64_t* total_steps, void* stream);
+
 /* Adjoint of dw_solve_fused for per-vertex boundary data (the reference's
  * CustomSolver backward pass, solver.py:202-252, which replays the solver with
  * the same seed under Dr.Jit AD).  Replays every walk pair from its seed and

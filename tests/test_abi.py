@@ -65,7 +65,8 @@ def test_ctypes_structs_and_constants_match_header(tmp_path):
               "DW_SUB_NEUMANN": _lib.SUB_NEUMANN, "DW_BC_EXPCOS": shapes.BC_EXPCOS, "DW_BC_QUAD": shapes.BC_QUAD,
               "DW_BC_LINEAR": shapes.BC_LINEAR, "DW_BC_PER_VERTEX": shapes.BC_PER_VERTEX,
               "DW_SRC_ZERO": shapes.SRC_ZERO, "DW_SRC_CONST": shapes.SRC_CONST,
-              "DW_RAY_REFERENCE": _lib.RAY_REFERENCE, "DW_RAY_NEAREST": _lib.RAY_NEAREST}
+              "DW_RAY_REFERENCE": _lib.RAY_REFERENCE, "DW_RAY_NEAREST": _lib.RAY_NEAREST,
+              "DW_NOALLOC_SCRATCH_BYTES": _lib.NOALLOC_SCRATCH_BYTES}
     lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "dwost.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')

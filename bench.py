@@ -71,14 +71,15 @@ def make_spec(name: str, n_query: int | None = None):
 def profile_figures(name: str) -> dict:
     """Figures of the committed rocprofv3 summary of this workload's kernel
     (profiles/k_fused_pers_<name>.md): HBM traffic per walk step from
-    FETCH_SIZE / WRITE_SIZE (copy-calibrated), L2 hit rate, VALU busy and the
-    active-lane fraction of issued VALU instructions."""
+    FETCH_SIZE / WRITE_SIZE (copy-calibrated), L2 hit rate, the share of wave
+    time issuing VALU instructions and the active-lane fraction of issued VALU
+    instructions."""
     path = os.path.join(ROOT, "profiles", f"k_fused_pers_{name}.md")
     out = {}
     if os.path.exists(path):
         for line in open(path):
             k, sep, v = line.strip().partition(": ")
-            if sep and k in ("traffic_bytes_per_walk_step", "l2_hit_rate", "valu_busy", "valu_lane_activity"):
+            if sep and k in ("traffic_bytes_per_walk_step", "l2_hit_rate", "valu_issue_share", "valu_lane_activity"):
                 out[k] = float(v)
     return out
 
@@ -212,7 +213,7 @@ def secondary_2d(progress: bool, reps: int = 2) -> dict:
                 ms_per_solve=round(1000 * wall / reps, 3), avg_launch_ms=round(1000 * avg_s, 3),
                 roofline_frac=round(ach / HBM_PEAK_GBS, 4), achieved_gbs=round(ach, 1),
                 b_alg_bytes_per_step=ALG["holes"]["b_alg"], l2_hit_rate=prof.get("l2_hit_rate"),
-                valu_busy=prof.get("valu_busy"), valu_lane_activity=prof.get("valu_lane_activity"),
+                valu_issue_share=prof.get("valu_issue_share"), valu_lane_activity=prof.get("valu_lane_activity"),
                 mean_steps_per_walk=round(steps / (reps * len(q) * c["nwalks"]), 3), **acc)
 
 
@@ -390,7 +391,7 @@ def main():
                 f_alg_flops_per_step=alg["f_alg"],
                 achieved_tflops=round(alg["f_alg"] * steps_per_launch / avg_kernel_s / 1e12, 3),
                 fp32_peak_tflops=FP32_PEAK_TFLOPS, ff_share=alg["ff_share"],
-                l2_hit_rate=prof.get("l2_hit_rate"), valu_busy=prof.get("valu_busy"),
+                l2_hit_rate=prof.get("l2_hit_rate"), valu_issue_share=prof.get("valu_issue_share"),
                 valu_lane_activity=prof.get("valu_lane_activity"),
                 limiter="dependent node-fetch latency and VALU issue with divergent lanes (the scene is L2/MALL "
                         "resident), not HBM bandwidth",

@@ -16,14 +16,15 @@ def test_more_than_one_gpu_is_refused():
 
 
 def test_profile_figures_parse():
-    """bench.py reads the traffic, L2 hit rate and VALU figures of each workload's
+    """bench.py reads the traffic, L2 hit rate and VALU issue / lane figures of each workload's
     committed rocprofv3 summary; every summary names the command that made it."""
     sys.path.insert(0, ROOT)
     import bench
 
     for name in ("holes", "ico"):
         f = bench.profile_figures(name)
-        assert set(f) == {"traffic_bytes_per_walk_step", "l2_hit_rate", "valu_busy"}, name
-        assert f["traffic_bytes_per_walk_step"] > 0 and 0 < f["l2_hit_rate"] <= 1 and 0 < f["valu_busy"] <= 1
+        assert set(f) == {"traffic_bytes_per_walk_step", "l2_hit_rate", "valu_issue_share", "valu_lane_activity"}, name
+        assert f["traffic_bytes_per_walk_step"] > 0 and 0 < f["l2_hit_rate"] <= 1
+        assert 0 < f["valu_issue_share"] <= 1 and 0 < f["valu_lane_activity"] <= 1
         text = open(os.path.join(ROOT, "profiles", f"k_fused_pers_{name}.md")).read()
         assert "rocprofv3 --kernel-trace --stats" in text and "rocprofv3 --pmc FETCH_SIZE" in text

@@ -47,6 +47,7 @@ NOALLOC_SCRATCH_BYTES = 64 << 20  # DW_NOALLOC_SCRATCH_BYTES
 SOLVE_EXACT_CONES = 1
 SOLVE_WOS = 2
 RAY_REFERENCE, RAY_NEAREST = 0, 1  # dw_intersect modes
+BASE_CLOSEST, BASE_RAY, BASE_SILHOUETTE = 0, 1, 2  # dw_query_baseline kinds
 
 
 class BatchItem(C.Structure):
@@ -76,6 +77,10 @@ def lib():
     L.dw_intersect.argtypes = [vp, i32, vp, vp, vp, i64, i32, vp, vp, vp, vp, vp, vp]
     L.dw_closest_silhouette.argtypes = [vp, i32, vp, vp, i64, vp, vp, vp, vp]
     L.dw_star_radius.argtypes = [vp, i32, vp, vp, i64, vp, vp]
+    L.dw_sdf.argtypes = [vp, vp, i64, vp, vp]
+    L.dw_sample_boundary.argtypes = [vp, i32, vp, i64, vp, vp, vp, vp, vp, vp]
+    L.dw_query_baseline.argtypes = [vp, i32, i32, vp, vp, vp, i64, vp, vp, vp]
+    L.dw_scene_prim_map.argtypes = [vp, i32, vp]
     L.dw_solve_u.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp, vp, vp, vp, vp]
     L.dw_solve_dude.argtypes = [vp, vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp]
     L.dw_trace_step.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), C.c_int32, C.c_float, vp,
@@ -91,7 +96,8 @@ def lib():
     L.dw_adjoint_fused.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp, vp, vp,
                                    vp]
     for name in ("dw_scene_create", "dw_scene_destroy", "dw_scene_info", "dw_scene_export", "dw_closest_point",
-                 "dw_intersect", "dw_closest_silhouette", "dw_star_radius", "dw_solve_u", "dw_solve_dude",
+                 "dw_intersect", "dw_closest_silhouette", "dw_star_radius", "dw_sdf", "dw_sample_boundary",
+                 "dw_query_baseline", "dw_scene_prim_map", "dw_solve_u", "dw_solve_dude",
                  "dw_trace_step",
                  "dw_solve_fused", "dw_solve_fused_noalloc", "dw_adjoint_u", "dw_adjoint_fused", "dw_batch_create", "dw_batch_solve"):
         getattr(L, name).restype = C.c_int

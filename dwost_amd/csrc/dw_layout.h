@@ -117,6 +117,7 @@ struct SubDev {
     const void* prims;  // DSeg / DTri (BVH order)
     const int* prim_index;  // BVH order -> local primitive index (2D)
     const void* sils;   // DSilV / DSilE (reference ref order)
+    const int* ref_topo;  // silhouette ref -> topology silhouette id (2D: local vertex, 3D: edge)
     // sampling (local primitive order)
     const float* cdf;
     const float* pmf;

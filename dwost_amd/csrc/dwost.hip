@@ -352,6 +352,156 @@ DWD F3 load_pt<3>(const float* pts, int64_t i) {
     return f3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
 }
 
+// Polyline.sdf / Mesh.sdf (polyline.py:188-194, mesh.py:290-293) on the full
+// scene: the closest-point distance signed by the side of the interpolated
+// vertex normal (2D) or the face normal (3D); inside is sdf < 0.
+__global__ void __launch_bounds__(kBlock) k_sdf(SceneDev sc, const float* __restrict__ pts, int64_t n, float* out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = sc.dim == 2 ? sdf2(sc, load_pt<2>(pts, i)) : sdf3(sc, load_pt<3>(pts, i));
+}
+
+// Polyline.sample_boundary (polyline.py:498-512) / Mesh.sample_boundary
+// (mesh.py:474-490) on one sub-scene, from caller-supplied uniform draws in
+// [0, 1): 2D one per sample (the edge CDF; its reused remainder is t), 3D three
+// (face CDF, r1, r2).  A draw outside [0, 1) yields prim -1 and NaN values.
+__global__ void __launch_bounds__(kBlock) k_sample_boundary(SceneDev sc, int sub, const float* __restrict__ u,
+                                                            int64_t n, float* p, float* nrm, float* tuv, float* pdf,
+                                                            int* prim) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const SubDev& S = sc.sub[sub];
+    const int D = sc.dim;
+    const float nan = __builtin_nanf("");
+    float q[3] = {nan, nan, nan}, nn[3] = {nan, nan, nan}, uv[2] = {nan, nan}, pd = nan;
+    int idx = -1;
+    const float u0 = u[D == 2 ? i : 3 * i];
+    bool ok = u0 >= 0.0f && u0 < 1.0f;
+    if (D == 3) {
+        const float r1 = u[3 * i + 1], r2 = u[3 * i + 2];
+        ok = ok && r1 >= 0.0f && r1 < 1.0f && r2 >= 0.0f && r2 < 1.0f;
+    }
+    if (ok) {
+        float rnd, pmf;
+        idx = cdf_sample(S, u0, rnd, pmf);
+        if (D == 2) {
+            const float* lp = S.local_prims + 4 * idx;
+            F2 a = f2(lp[0], lp[1]), b = f2(lp[2], lp[3]);
+            F2 tg = b - a;
+            F2 nv = normalize(f2(tg.y, -tg.x));
+            F2 x = lerp(a, b, rnd);
+            q[0] = x.x, q[1] = x.y, nn[0] = nv.x, nn[1] = nv.y;
+            uv[0] = rnd, uv[1] = 0.0f;
+            pd = pmf / norm(tg);
+        } else {
+            const float* lp = S.local_prims + 9 * idx;
+            F3 a = f3(lp[0], lp[1], lp[2]), b = f3(lp[3], lp[4], lp[5]), c = f3(lp[6], lp[7], lp[8]);
+            F3 cr = cross(b - a, c - a);
+            float area = norm(cr) / 2.0f;
+            float r1 = u[3 * i + 1], r2 = u[3 * i + 2];
+            float su = __builtin_sqrtf(r1) * (1.0f - r2);
+            float sv = __builtin_sqrtf(r1) * r2;
+            F3 x = (b * su + c * sv) + a * ((1.0f - su) - sv);  // interpolate (math.py:306-307)
+            F3 nv = normalize(cr);
+            q[0] = x.x, q[1] = x.y, q[2] = x.z, nn[0] = nv.x, nn[1] = nv.y, nn[2] = nv.z;
+            uv[0] = su, uv[1] = sv;
+            pd = pmf / area;
+        }
+    }
+    for (int d = 0; d < D; ++d) {
+        if (p) p[D * i + d] = q[d];
+        if (nrm) nrm[D * i + d] = nn[d];
+    }
+    if (tuv) tuv[2 * i] = uv[0], tuv[2 * i + 1] = uv[1];
+    if (pdf) pdf[i] = pd;
+    if (prim) prim[i] = idx;
+}
+
+// The reference's brute-force queries, one thread per query looping over the
+// whole sub-scene in the reference's order: closest_point_baseline
+// (polyline.py:246-279, mesh.py:312-335), intersect_baseline (polyline.py:468-496,
+// mesh.py:440-472) and closest_silhouette_baseline (polyline.py:289-302,
+// mesh.py:558-570) over the topology silhouettes.  The first of equally near
+// candidates in that order wins (strict <); silhouettes come in reference-ref
+// order and tie on their topology id.  prim = -1 when nothing qualifies.
+__global__ void __launch_bounds__(kBlock) k_baseline(SceneDev sc, int sub, int kind, const float* __restrict__ pts,
+                                                     const float* __restrict__ dir, const float* __restrict__ rmax,
+                                                     int64_t n, int* prim, float* dist) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const SubDev& S = sc.sub[sub];
+    const int D = sc.dim;
+    const float r0 = rmax ? rmax[i] : kInf;
+    float dmin = kind == DW_BASE_CLOSEST ? kInf : r0;
+    int best = -1;
+    if (D == 2) {
+        F2 x = load_pt<2>(pts, i);
+        if (kind == DW_BASE_CLOSEST || kind == DW_BASE_RAY) {
+            F2 v = kind == DW_BASE_RAY ? load_pt<2>(dir, i) : f2(0.0f, 0.0f);
+            for (int j = 0; j < S.n_prims; ++j) {
+                const float* lp = S.local_prims + 4 * j;
+                F2 a = f2(lp[0], lp[1]), b = f2(lp[2], lp[3]);
+                float d;
+                if (kind == DW_BASE_CLOSEST) {
+                    F2 pa = x - a, ba = b - a;
+                    float h = clampf(dot(pa, ba) / dot(ba, ba), 0.0f, 1.0f);
+                    d = norm(pa - ba * h);
+                } else {  // ray_intersection (math.py:163-171)
+                    F2 uu = b - a, w = x - a;
+                    float dd = cross2(v, uu);
+                    float s_ = cross2(v, w) / dd;
+                    float t_ = cross2(uu, w) / dd;
+                    d = ((t_ > 0.0f) && (s_ >= 0.0f) && (s_ <= 1.0f)) ? t_ : kInf;
+                }
+                if (d < dmin) dmin = d, best = j;
+            }
+        } else {
+            const DSilV* sv = (const DSilV*)S.sils;
+            for (int r = 0; r < S.n_refs; ++r) {
+                F2 a = f2(sv[r].ax, sv[r].ay), b = f2(sv[r].bx, sv[r].by), c = f2(sv[r].cx, sv[r].cy);
+                float d = norm(x - b);
+                if (!(sv[r].both && d < r0)) continue;
+                if (!(cross2(b - a, x - a) * cross2(c - b, x - b) < 0.0f)) continue;
+                int topo = S.ref_topo[r];
+                if (d < dmin || (d == dmin && topo < best)) dmin = d, best = topo;
+            }
+        }
+    } else {
+        F3 x = load_pt<3>(pts, i);
+        if (kind == DW_BASE_CLOSEST || kind == DW_BASE_RAY) {
+            F3 v = kind == DW_BASE_RAY ? load_pt<3>(dir, i) : f3(0.0f, 0.0f, 0.0f);
+            for (int j = 0; j < S.n_prims; ++j) {
+                const float* lp = S.local_prims + 9 * j;
+                DTri tr;
+                std::memset(&tr, 0, sizeof(tr));
+                tr.ax = lp[0], tr.ay = lp[1], tr.az = lp[2], tr.bx = lp[3], tr.by = lp[4], tr.bz = lp[5];
+                tr.cx = lp[6], tr.cy = lp[7], tr.cz = lp[8];
+                float d;
+                if (kind == DW_BASE_CLOSEST) {
+                    F3 pt;
+                    float cu, cv;
+                    tri_closest(tr, x, pt, cu, cv, d);
+                } else {
+                    float bu, bw, t_;
+                    d = tri_hit(tr, x, v, kInf, bu, bw, t_) ? t_ : kInf;
+                }
+                if (d < dmin) dmin = d, best = j;
+            }
+        } else {
+            const DSilE* se = (const DSilE*)S.sils;
+            for (int r = 0; r < S.n_refs; ++r) {
+                float d;
+                F3 pt;
+                if (!sile_test(se[r], x, r0, d, pt)) continue;
+                int topo = S.ref_topo[r];
+                if (d < dmin || (d == dmin && topo < best)) dmin = d, best = topo;
+            }
+        }
+    }
+    prim[i] = best;
+    if (dist) dist[i] = best >= 0 ? dmin : kInf;
+}
+
 // Lane scheduling of the persistent walk kernels.  Each lane owns one job (one
 // or more walks, see the kernels) and runs its walk as the step machine of
 // walk_advance.  Two loops:
@@ -1523,6 +1673,8 @@ int upload_sub(dw_scene* s, const dwb::SubHost& H, SubDev& S) {
     if ((rc = upload_records(s, H, S))) return rc;
     if ((rc = upload(s, H.prim_order, &p))) return rc;
     S.prim_index = (const int*)p;
+    if ((rc = upload(s, H.refs, &p))) return rc;
+    S.ref_topo = (const int*)p;
     if ((rc = upload(s, H.cdf, &p))) return rc;
     S.cdf = (const float*)p;
     if ((rc = upload(s, H.pmf, &p))) return rc;
@@ -2010,6 +2162,16 @@ int dw_scene_info(const dw_scene* s, int32_t sub, int64_t* out8) {
     return DW_OK;
 }
 
+int dw_scene_prim_map(const dw_scene* s, int32_t sub, int32_t* out) {
+    if (!s) return fail(DW_ERR_ARG, "null scene");
+    if (sub < 0 || sub > 2) return fail(DW_ERR_ARG, "sub must be 0 (full), 1 (Dirichlet) or 2 (Neumann)");
+    const dwb::SubHost& H = s->host.sub[sub];
+    if (!H.present) return DW_OK;
+    if (!out) return fail(DW_ERR_ARG, "null output");
+    std::copy(H.pglobal.begin(), H.pglobal.end(), out);
+    return DW_OK;
+}
+
 int dw_scene_export(const dw_scene* s, int32_t sub, float* box, int32_t* ints, float* cone, int32_t* prim_index,
                     int32_t* refs) {
     if (!s || sub < 0 || sub > 2) return fail(DW_ERR_ARG, "bad argument");
@@ -2047,6 +2209,42 @@ int dw_closest_point(const dw_scene* s, int32_t sub, const float* pts, int64_t n
     if (n == 0) return DW_OK;
     hipLaunchKernelGGL(k_closest, dim3(blocks_for(n)), dim3(kBlock), kStackLds, (hipStream_t)stream, s->dev, sub, pts, n,
                        prim, dist, point, normal, tuv);
+    DW_HIP(hipGetLastError());
+    return DW_OK;
+}
+
+int dw_sdf(const dw_scene* s, const float* pts, int64_t n, float* sdf, void* stream) {
+    int rc = check_scene(s, 0);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!pts || !sdf))) return fail(DW_ERR_ARG, "bad arrays");
+    if (n == 0) return DW_OK;
+    hipLaunchKernelGGL(k_sdf, dim3(blocks_for(n)), dim3(kBlock), kStackLds, (hipStream_t)stream, s->dev, pts, n, sdf);
+    DW_HIP(hipGetLastError());
+    return DW_OK;
+}
+
+int dw_sample_boundary(const dw_scene* s, int32_t sub, const float* u, int64_t n, float* point, float* normal,
+                       float* tuv, float* pdf, int32_t* prim, void* stream) {
+    int rc = check_scene(s, sub);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && !u)) return fail(DW_ERR_ARG, "bad draws");
+    if (n == 0) return DW_OK;
+    hipLaunchKernelGGL(k_sample_boundary, dim3(blocks_for(n)), dim3(kBlock), 0, (hipStream_t)stream, s->dev, sub, u,
+                       n, point, normal, tuv, pdf, prim);
+    DW_HIP(hipGetLastError());
+    return DW_OK;
+}
+
+int dw_query_baseline(const dw_scene* s, int32_t sub, int32_t kind, const float* pts, const float* dir,
+                      const float* rmax, int64_t n, int32_t* prim, float* dist, void* stream) {
+    int rc = check_scene(s, sub);
+    if (rc) return rc;
+    if (kind != DW_BASE_CLOSEST && kind != DW_BASE_RAY && kind != DW_BASE_SILHOUETTE)
+        return fail(DW_ERR_ARG, "kind must be DW_BASE_CLOSEST, DW_BASE_RAY or DW_BASE_SILHOUETTE");
+    if (n < 0 || (n > 0 && (!pts || !prim || (kind == DW_BASE_RAY && !dir)))) return fail(DW_ERR_ARG, "bad arrays");
+    if (n == 0) return DW_OK;
+    hipLaunchKernelGGL(k_baseline, dim3(blocks_for(n)), dim3(kBlock), 0, (hipStream_t)stream, s->dev, sub, kind, pts,
+                       dir, rmax, n, prim, dist);
     DW_HIP(hipGetLastError());
     return DW_OK;
 }

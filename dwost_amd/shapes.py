@@ -66,6 +66,13 @@ class BoundaryCondition:
     def per_vertex(g: torch.Tensor, h: torch.Tensor) -> "BoundaryCondition":
         return BoundaryCondition(BC_PER_VERTEX, g=g, h=h)
 
+    def op_args(self) -> tuple:
+        """(bc_code, coeffs, g, h, source): the boundary arguments of the
+        torch.ops.dwost solvers."""
+        c = [float(self.coeffs[k]) if k < len(self.coeffs) else 0.0 for k in range(4)]
+        pv = self.code == BC_PER_VERTEX
+        return int(self.code), c, self.g if pv else None, self.h if pv else None, float(self.source)
+
     def to_c(self, n_vertices: int | None = None) -> _lib.Boundary:
         b = _lib.Boundary()
         b.bc_code = int(self.code)
@@ -215,6 +222,77 @@ class _Scene:
         check(lib().dw_closest_silhouette(self._h, sub, ptr(p), ptr(r), n, ptr(out["sil"]), ptr(out["d"]),
                                           ptr(out["p"]), stream_handle(dev)))
         return out
+
+    def sdf(self, p) -> torch.Tensor:
+        """Polyline.sdf / Mesh.sdf (polyline.py:188-194, mesh.py:290-293): the
+        full-scene distance, negative inside."""
+        p = _as_points(p, self.dim)
+        out = torch.empty(p.shape[0], device=p.device)
+        check(lib().dw_sdf(self._h, ptr(p), p.shape[0], ptr(out), stream_handle(p.device)))
+        return out
+
+    def inside(self, p) -> torch.Tensor:
+        """Polyline.inside / Mesh.inside (polyline.py:232-234, mesh.py:295-297): sdf < 0."""
+        return self.sdf(p) < 0
+
+    def sample_boundary(self, u, sub: int = SUB_FULL) -> dict:
+        """Polyline.sample_boundary / Mesh.sample_boundary (polyline.py:498-512,
+        mesh.py:474-490) with the sampler's uniform draws given: ``u`` of shape
+        (N,) in 2D (the edge CDF draw) or (N, 3) in 3D (face CDF draw, r1, r2),
+        values in [0, 1).  Returns p, n, t (2D) / uv (3D), pdf and prim (the
+        sub-scene's primitive id)."""
+        u = torch.as_tensor(u, dtype=torch.float32, device="cuda").contiguous()
+        want = (u.shape[0],) if self.dim == 2 else (u.shape[0], 3)
+        if tuple(u.shape) != want:
+            raise ValueError(f"draws must have shape {'(N,)' if self.dim == 2 else '(N, 3)'}")
+        n, dev, D = u.shape[0], u.device, self.dim
+        out = dict(p=torch.empty(n, D, device=dev), n=torch.empty(n, D, device=dev), tuv=torch.empty(n, 2, device=dev),
+                   pdf=torch.empty(n, device=dev), prim=torch.empty(n, dtype=torch.int32, device=dev))
+        check(lib().dw_sample_boundary(self._h, sub, ptr(u), n, ptr(out["p"]), ptr(out["n"]), ptr(out["tuv"]),
+                                       ptr(out["pdf"]), ptr(out["prim"]), stream_handle(dev)))
+        if D == 2:
+            out["t"] = out["tuv"][:, 0]
+        else:
+            out["uv"] = out["tuv"]
+        return out
+
+    def _baseline(self, kind, p, v=None, r_max=None, sub=SUB_FULL) -> dict:
+        p = _as_points(p, self.dim)
+        n, dev = p.shape[0], p.device
+        v = None if v is None else _as_points(v, self.dim)
+        r = None if r_max is None else torch.as_tensor(r_max, dtype=torch.float32, device=dev).expand(n).contiguous()
+        out = dict(prim=torch.empty(n, dtype=torch.int32, device=dev), d=torch.empty(n, device=dev))
+        check(lib().dw_query_baseline(self._h, sub, kind, ptr(p), ptr(v), ptr(r), n, ptr(out["prim"]),
+                                      ptr(out["d"]), stream_handle(dev)))
+        return out
+
+    def closest_point_baseline(self, p, sub: int = SUB_FULL) -> dict:
+        """Brute-force closest point (polyline.py:246-279, mesh.py:312-335): prim, d."""
+        return self._baseline(_lib.BASE_CLOSEST, p, sub=sub)
+
+    def intersect_baseline(self, origin, direction, r_max=float("inf"), sub: int = SUB_FULL) -> dict:
+        """Brute-force nearest ray hit (polyline.py:468-496, mesh.py:440-472): prim (-1: miss), d."""
+        return self._baseline(_lib.BASE_RAY, origin, direction, r_max, sub)
+
+    def closest_silhouette_baseline(self, p, r_max=float("inf"), sub: int = SUB_FULL) -> dict:
+        """Brute-force closest silhouette (polyline.py:289-302, mesh.py:558-570)
+        over the topology silhouettes: prim = topology id (-1: none), d."""
+        return self._baseline(_lib.BASE_SILHOUETTE, p, None, r_max, sub)
+
+    def global_prim_ids(self, sub: int) -> np.ndarray:
+        """Sub-scene primitive id -> scene primitive id (polyline.py:636-645)."""
+        inf = self.info(sub)
+        out = np.zeros(max(inf["nprims"], 1), np.int32)
+        check(lib().dw_scene_prim_map(self._h, sub, out.ctypes.data_as(C.c_void_p)))
+        return out[: inf["nprims"]]
+
+    @property
+    def dirichlet_2_global_indices(self) -> np.ndarray:
+        return self.global_prim_ids(SUB_DIRICHLET)
+
+    @property
+    def neumann_2_global_indices(self) -> np.ndarray:
+        return self.global_prim_ids(SUB_NEUMANN)
 
     def star_radius(self, p, r_max=float("inf"), sub: int = SUB_FULL) -> torch.Tensor:
         """Polyline.star_radius (polyline.py:304-314) / Mesh.star_radius (mesh.py:337-346)."""

@@ -21,7 +21,7 @@ from typing import ClassVar
 
 import torch
 
-from . import _lib
+from . import _lib, ops
 from ._lib import check, lib, ptr, stream_handle
 from .shapes import BC_PER_VERTEX, _as_points, _Scene
 
@@ -58,6 +58,11 @@ class Solver:
                                  float(self.min_R), flags,
                                  int(point_base), int(self.block_size), int(self.blocks_per_cu),
                                  int(self.spatial_order), int(self.schedule), int(self.gate))
+
+    def _op_params(self, seed: int, point_base: int = 0) -> tuple[list[int], list[float]]:
+        """The int and float parameter lists of the torch.ops.dwost solvers."""
+        prm = self._params(seed, point_base)
+        return [getattr(prm, f) for f in ops.IP_FIELDS], [float(getattr(prm, f)) for f in ops.FP_FIELDS]
 
     def _check(self, scene: _Scene):
         if scene.dim != self.dim:
@@ -160,82 +165,6 @@ def _per_vertex(scene: _Scene):
     return bc.g, bc.h
 
 
-class _SolveU(torch.autograd.Function):
-    """u = fwd_solver.solve(p); backward replays bwd_solver's walks from the
-    same seed and scatter-adds dL/dg, dL/dh (solver.py:202-252)."""
-
-    @staticmethod
-    def forward(ctx, g, h, cs, p, scene, seed):
-        fwd, bwd = cs.fwd_solver, cs.bwd_solver
-        n = p.shape[0]
-        flags = None
-        if _same_walks(fwd, bwd):
-            flags = torch.empty(n * fwd.nwalks, dtype=torch.uint8, device=p.device)
-        u = fwd.solve(p, scene, seed, walk_flags=flags)
-        ctx.cs, ctx.p, ctx.scene, ctx.seed, ctx.flags = cs, p, scene, seed, flags
-        return u
-
-    @staticmethod
-    def backward(ctx, dl_du):
-        cs, p, scene, seed = ctx.cs, ctx.p, ctx.scene, ctx.seed
-        bwd = cs.bwd_solver
-        n, dev = p.shape[0], p.device
-        flags = ctx.flags
-        if flags is None:  # the replay solver draws other walks: find its guard flags first
-            flags = torch.empty(n * bwd.nwalks, dtype=torch.uint8, device=dev)
-            bwd.solve(p, scene, seed, walk_flags=flags)
-        V = scene.n_vertices
-        dl_dg = torch.zeros(V, device=dev)
-        dl_dh = torch.zeros(V, device=dev)
-        prm = bwd._params(seed)
-        bc = scene.boundary.to_c(V)
-        w = dl_du.detach().to(torch.float32).contiguous()
-        check(lib().dw_adjoint_u(scene.handle, ptr(p), n, C.byref(prm), C.byref(bc), ptr(w), ptr(flags), ptr(dl_dg),
-                                 ptr(dl_dh), stream_handle(dev)))
-        return (dl_dg if ctx.needs_input_grad[0] else None, dl_dh if ctx.needs_input_grad[1] else None,
-                None, None, None, None)
-
-
-class _SolveGrad(torch.autograd.Function):
-    """(u, grad u) from fused antithetic pairs; backward replays the pairs."""
-
-    @staticmethod
-    def forward(ctx, g, h, cs, p, scene, seed):
-        fwd, bwd = cs.fwd_solver, cs.bwd_solver
-        n = p.shape[0]
-        flags = None
-        if _same_walks(fwd, bwd):
-            flags = torch.empty(n * ((fwd.nwalks + 1) // 2), dtype=torch.uint8, device=p.device)
-        out = fwd.solve_grad(p, scene, seed, pair_flags=flags)
-        ctx.cs, ctx.p, ctx.scene, ctx.seed, ctx.flags = cs, p, scene, seed, flags
-        return out["u"], out["grad"]
-
-    @staticmethod
-    def backward(ctx, dl_du, dl_dgrad):
-        cs, p, scene, seed = ctx.cs, ctx.p, ctx.scene, ctx.seed
-        bwd = cs.bwd_solver
-        n, dev, D = p.shape[0], p.device, scene.dim
-        flags = ctx.flags
-        if flags is None:
-            flags = torch.empty(n * ((bwd.nwalks + 1) // 2), dtype=torch.uint8, device=dev)
-            bwd.solve_grad(p, scene, seed, pair_flags=flags)
-        if dl_du is None:
-            dl_du = torch.zeros(n, device=dev)
-        if dl_dgrad is None:
-            dl_dgrad = torch.zeros(n, D, device=dev)
-        V = scene.n_vertices
-        dl_dg = torch.zeros(V, device=dev)
-        dl_dh = torch.zeros(V, device=dev)
-        prm = bwd._params(seed)
-        bc = scene.boundary.to_c(V)
-        wu = dl_du.detach().to(torch.float32).contiguous()
-        wg = dl_dgrad.detach().to(torch.float32).contiguous()
-        check(lib().dw_adjoint_fused(scene.handle, ptr(p), n, C.byref(prm), C.byref(bc), ptr(wu), ptr(wg), ptr(flags),
-                                     ptr(dl_dg), ptr(dl_dh), stream_handle(dev)))
-        return (dl_dg if ctx.needs_input_grad[0] else None, dl_dh if ctx.needs_input_grad[1] else None,
-                None, None, None, None)
-
-
 @dataclass
 class CustomSolver:
     """Differentiable solve (diff_wost/solvers/solver.py:255-293).
@@ -243,7 +172,9 @@ class CustomSolver:
     The forward value comes from ``fwd_solver``; the backward pass replays
     ``bwd_solver``'s walks from the same seed and scatter-adds dL/dg and dL/dh
     into the scene's per-vertex boundary tensors (``BoundaryCondition.per_vertex``
-    with ``requires_grad``).  Nothing is stored per walk but one flag byte."""
+    with ``requires_grad``).  Both run as the autograd-registered operators
+    ``torch.ops.dwost.solve_u`` / ``solve_fused``; nothing is stored per walk
+    but one guard-flag byte."""
 
     fwd_solver: Solver = None
     bwd_solver: Solver = None
@@ -252,23 +183,26 @@ class CustomSolver:
         if self.bwd_solver is None:
             self.bwd_solver = self.fwd_solver
 
-    def _prep(self, p, scene):
+    def _prep(self, p, scene, seed):
         if self.fwd_solver is None:
             raise ValueError("CustomSolver needs a fwd_solver")
         self.fwd_solver._check(scene)
         self.bwd_solver._check(scene)
-        g, h = _per_vertex(scene)
-        return g, h, _as_points(p, scene.dim)
+        _per_vertex(scene)
+        ip, fp = self.fwd_solver._op_params(seed)
+        bip, bfp = self.bwd_solver._op_params(seed)
+        if _same_walks(self.fwd_solver, self.bwd_solver):
+            bip, bfp = ip, fp  # identical walks: the backward reuses the forward's guard flags
+        return (scene.handle.value, _as_points(p, scene.dim), ip, fp, bip, bfp) + scene.boundary.op_args()
 
     def solve(self, p, scene: _Scene, seed: int = 0) -> torch.Tensor:
         """u at each point, differentiable w.r.t. the per-vertex g and h."""
-        g, h, p = self._prep(p, scene)
-        return _SolveU.apply(g, h, self, p, scene, seed)
+        return torch.ops.dwost.solve_u(*self._prep(p, scene, seed))[0]
 
     def solve_grad(self, p, scene: _Scene, seed: int = 0):
         """(u, grad u) from the fused estimator, both differentiable w.r.t. g and h."""
-        g, h, p = self._prep(p, scene)
-        return _SolveGrad.apply(g, h, self, p, scene, seed)
+        out = torch.ops.dwost.solve_fused(*self._prep(p, scene, seed))
+        return out[0], out[1]
 
 
 @dataclass

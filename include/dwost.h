@@ -136,6 +136,11 @@ int dw_scene_destroy(dw_scene* scene);
  *         has_silhouettes, n_topology_silhouettes} of one sub-scene (HOST). */
 int dw_scene_info(const dw_scene* scene, int32_t sub, int64_t* out8);
 
+/* Sub-scene primitive id -> scene primitive id (HOST out[n_prims of the
+ * sub-scene]): Scene2D._dirichlet_2_global_indices / _neumann_2_global_indices
+ * (polyline.py:636-645); the full scene maps to itself. */
+int dw_scene_prim_map(const dw_scene* scene, int32_t sub, int32_t* out);
+
 /* HOST copies of one sub-scene's tree, sized from dw_scene_info:
  * box[n_nodes][2*dim] (min, max), ints[n_nodes][4] (offset, n_refs,
  * silhouette offset, n_silhouettes), cone[n_nodes][dim+2] (axis, half angle,
@@ -167,6 +172,35 @@ int dw_intersect(const dw_scene* scene, int32_t sub, const float* origin, const 
  * silhouette reference index (-1: none within rmax). */
 int dw_closest_silhouette(const dw_scene* scene, int32_t sub, const float* pts, const float* rmax, int64_t n,
                           int32_t* sil, float* dist, float* point, void* stream);
+
+/* Polyline.sdf / Mesh.sdf (polyline.py:188-194, mesh.py:290-293) on the full
+ * scene: sdf[n] = closest-point distance signed by the side of the interpolated
+ * vertex normal (2D) or face normal (3D), negative inside; Polyline.inside /
+ * Mesh.inside (polyline.py:232-234, mesh.py:295-297) is sdf < 0. */
+int dw_sdf(const dw_scene* scene, const float* pts, int64_t n, float* sdf, void* stream);
+
+/* Polyline.sample_boundary (polyline.py:498-512) / Mesh.sample_boundary
+ * (mesh.py:474-490) on one sub-scene.  The reference draws from the caller's
+ * PCG32 sampler; here the caller passes those uniform draws in [0, 1): u[n]
+ * (2D: the edge CDF draw, whose reused remainder is t) or u[n][3] (3D: face CDF
+ * draw, r1, r2).  Outputs (each may be NULL): point[n][dim], normal[n][dim],
+ * tuv[n][2] (2D: t, 0; 3D: u, v), pdf[n], prim[n] (sub-scene primitive id).
+ * A draw outside [0, 1) gives prim -1 and NaN values. */
+int dw_sample_boundary(const dw_scene* scene, int32_t sub, const float* u, int64_t n, float* point, float* normal,
+                       float* tuv, float* pdf, int32_t* prim, void* stream);
+
+/* kinds of dw_query_baseline */
+#define DW_BASE_CLOSEST 0    /* closest_point_baseline (polyline.py:246-279, mesh.py:312-335)         */
+#define DW_BASE_RAY 1        /* intersect_baseline (polyline.py:468-496, mesh.py:440-472): nearest hit  */
+#define DW_BASE_SILHOUETTE 2 /* closest_silhouette_baseline (polyline.py:289-302, mesh.py:558-570)     */
+
+/* The reference's brute-force queries: each query loops over the whole
+ * sub-scene in the reference's order (first of equally near candidates wins).
+ * pts[n][dim]; dir[n][dim] (DW_BASE_RAY only); rmax[n] (NULL: +inf; closest
+ * point ignores it).  prim[n] = sub-scene primitive id (silhouettes: topology
+ * id, 2D the local vertex, 3D the edge), -1 if none; dist[n] (may be NULL). */
+int dw_query_baseline(const dw_scene* scene, int32_t sub, int32_t kind, const float* pts, const float* dir,
+                      const float* rmax, int64_t n, int32_t* prim, float* dist, void* stream);
 
 /* Polyline.star_radius (polyline.py:304-314) / Mesh.star_radius (mesh.py:337-346). */
 int dw_star_radius(const dw_scene* scene, int32_t sub, const float* pts, const float* rmax, int64_t n,

@@ -57,6 +57,8 @@ def lib():
         L.orc_num_threads.restype = C.c_int
         L.orc_cv_bound_check.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
         L.orc_far_check.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+        L.orc_sdf.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+        L.orc_sample_boundary.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64] + [C.c_void_p] * 5
     return _lib
 
 
@@ -155,6 +157,23 @@ class OracleScene:
         nodes = np.zeros(2, np.uint64)
         lib().orc_cv_bound_check(self.h, _p(pts), len(pts), _p(same), _p(nodes))
         return same.astype(bool), int(nodes[0]), int(nodes[1])
+
+    def sdf(self, pts):
+        pts = _f32(pts)
+        out = np.zeros(len(pts), np.float32)
+        lib().orc_sdf(self.h, _p(pts), len(pts), _p(out))
+        return out
+
+    def sample_boundary(self, sub: int, u):
+        """Polyline / Mesh sample_boundary with the sampler's draws given
+        (2D: u[n]; 3D: u[n, 3])."""
+        u = _f32(u)
+        n, D = len(u), self.dim
+        out = dict(p=np.zeros((n, D), np.float32), n=np.zeros((n, D), np.float32), tuv=np.zeros((n, 2), np.float32),
+                   pdf=np.zeros(n, np.float32), prim=np.zeros(n, np.int32))
+        lib().orc_sample_boundary(self.h, sub, _p(u), n, _p(out["p"]), _p(out["n"]), _p(out["tuv"]), _p(out["pdf"]),
+                                  _p(out["prim"]))
+        return out
 
     def far_check(self, pts):
         """3D: the distance bounds the walk kernels use far from the scene,

@@ -1535,12 +1535,19 @@ static Its brute_ray2(const Sub<2>& S, V2 x, V2 v, float rmax) {
     r.prim = idx;
     return r;
 }
-static CP brute_closest3(const Sub<3>& S, V3 p) {
+// the tree's primitives in the sub-scene's index order (the baselines' loop order)
+static std::vector<int> index_order(const Sub<3>& S) {
+    std::vector<int> pos(S.tree.prims.size());
+    for (size_t j = 0; j < S.tree.prims.size(); ++j) pos[S.tree.prims[j].pIndex] = (int)j;
+    return pos;
+}
+static CP brute_closest3(const Sub<3>& S, V3 p) {  // mesh.py:312-335: i = 0 .. n-1
     float dmin = kInf;
     int idx = -1;
-    for (size_t i = 0; i < S.tree.prims.size(); ++i) {
+    std::vector<int> pos = index_order(S);
+    for (size_t i = 0; i < pos.size(); ++i) {
         CP r;
-        tri_closest(S.tree.prims[i], p, r);
+        tri_closest(S.tree.prims[pos[i]], p, r);
         if (r.d < dmin) dmin = r.d, idx = r.prim;
     }
     CP r;
@@ -1549,12 +1556,14 @@ static CP brute_closest3(const Sub<3>& S, V3 p) {
     r.prim = idx;
     return r;
 }
-static Its brute_ray3(const Sub<3>& S, V3 x, V3 v, float rmax) {
+static Its brute_ray3(const Sub<3>& S, V3 x, V3 v, float rmax) {  // mesh.py:440-472: i = 0 .. n-1
     float dmin = rmax;
     int idx = -1;
-    for (size_t i = 0; i < S.tree.prims.size(); ++i) {
+    std::vector<int> pos = index_order(S);
+    for (size_t i = 0; i < pos.size(); ++i) {
         Its r;
-        if (tri_ray(S.tree.prims[i], (int)i, x, v, kInf, r) && r.d < dmin) dmin = r.d, idx = S.tree.prims[i].pIndex;
+        const Prim<3>& tr = S.tree.prims[pos[i]];
+        if (tri_ray(tr, pos[i], x, v, kInf, r) && r.d < dmin) dmin = r.d, idx = tr.pIndex;
     }
     Its r;
     std::memset(&r, 0, sizeof(r));
@@ -2326,6 +2335,57 @@ void orc_sub_export(void* s, int sub, float* box, int32_t* ints, float* cone, in
         go(sc->s2[sub], 2);
     else
         go(sc->s3[sub], 3);
+}
+
+// Polyline.sdf / Mesh.sdf (polyline.py:188-194, mesh.py:290-293): out[n]
+void orc_sdf(void* s, const float* pts, int64_t n, float* out) {
+    Scene* sc = (Scene*)s;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; ++i)
+        out[i] = sc->dim == 2 ? sdf2(*sc, v2(pts[2 * i], pts[2 * i + 1]))
+                              : sdf3(*sc, v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+}
+
+// Polyline.sample_boundary (polyline.py:498-512) / Mesh.sample_boundary
+// (mesh.py:474-490) with the sampler's draws given: u[n] (2D) or u[n][3] (3D).
+// out: p[n*dim], nrm[n*dim], tuv[n*2], pdf[n], prim[n]
+void orc_sample_boundary(void* s, int which, const float* u, int64_t n, float* p, float* nrm, float* tuv, float* pdf,
+                         int32_t* prim) {
+    Scene* sc = (Scene*)s;
+    for (int64_t i = 0; i < n; ++i) {
+        float rnd, pmf;
+        if (sc->dim == 2) {
+            const Sub<2>& S = sc->s2[which];
+            int idx = cdf_sample<2>(S, u[i], rnd, pmf, nullptr);
+            V2 a = v2(S.V[2 * S.I[2 * idx]], S.V[2 * S.I[2 * idx] + 1]);
+            V2 b = v2(S.V[2 * S.I[2 * idx + 1]], S.V[2 * S.I[2 * idx + 1] + 1]);
+            V2 tg = sub(b, a);
+            V2 nn = normalize(v2(tg.y, -tg.x));
+            V2 q = lerp(a, b, rnd);
+            p[2 * i] = q.x, p[2 * i + 1] = q.y, nrm[2 * i] = nn.x, nrm[2 * i + 1] = nn.y;
+            tuv[2 * i] = rnd, tuv[2 * i + 1] = 0.0f;
+            pdf[i] = pmf / norm(tg);
+            prim[i] = idx;
+        } else {
+            const Sub<3>& S = sc->s3[which];
+            int idx = cdf_sample<3>(S, u[3 * i], rnd, pmf, nullptr);
+            const float* A = &S.V[3 * S.I[3 * idx]];
+            const float* B = &S.V[3 * S.I[3 * idx + 1]];
+            const float* Cc = &S.V[3 * S.I[3 * idx + 2]];
+            V3 a = v3(A[0], A[1], A[2]), b = v3(B[0], B[1], B[2]), c = v3(Cc[0], Cc[1], Cc[2]);
+            V3 cr = cross(sub(b, a), sub(c, a));
+            float area = norm(cr) / 2.0f;
+            float r1 = u[3 * i + 1], r2 = u[3 * i + 2];
+            float su = std::sqrt(r1) * (1.0f - r2);
+            float sv = std::sqrt(r1) * r2;
+            V3 q = add(add(mul(b, su), mul(c, sv)), mul(a, (1.0f - su) - sv));  // interpolate (math.py:306-307)
+            V3 nn = normalize(cr);
+            for (int k = 0; k < 3; ++k) p[3 * i + k] = (&q.x)[k], nrm[3 * i + k] = (&nn.x)[k];
+            tuv[2 * i] = su, tuv[2 * i + 1] = sv;
+            pdf[i] = pmf / area;
+            prim[i] = idx;
+        }
+    }
 }
 
 // Far-field check (3D).  The walk kernels skip two traversals from points far

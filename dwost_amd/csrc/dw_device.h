@@ -310,7 +310,9 @@ DWD T gload(const void* base, int i) {
 // 2D: BVH closest point (bvh.py:111-188 + line_segment.py:74-85).  Visit order
 // and pruning are the reference's: both children tested in order with r_max
 // tightened by each d_max, nearer child first, `cd > r_max` skips.
-DWD void closest2_begin(Trav& t, const SubDev& S, F2 p) {
+// r0 < inf starts the pruning radius at a known bound on the answer (the
+// reference query starts at inf and pushes the root at that distance).
+DWD void closest2_begin(Trav& t, const SubDev& S, F2 p, float r0 = kInf) {
     t.kind = TQ_CLOSEST;
     t.S = &S;
     t.rec = S.rec_bvh;
@@ -318,9 +320,9 @@ DWD void closest2_begin(Trav& t, const SubDev& S, F2 p) {
     t.x = f3(p.x, p.y, 0.0f);
     t.d = kInf;
     t.best = -1;
-    t.r_max = kInf;
+    t.r_max = r0;
     t.ref = S.root_ref_bvh;
-    t.cd = kInf;
+    t.cd = r0 < kInf ? 0.0f : kInf;
     t.sp = 0;
 }
 DWD bool closest2_iter(Trav& t, Ovf& o) {

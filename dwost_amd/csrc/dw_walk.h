@@ -312,7 +312,19 @@ DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Wa
             w.aux.bpdf = pmf / norm(tg);
             w.aux.rnd = rnd;
             w.aux.idx = idx;
-            closest2_begin(t, F, w.ip);  // boundary_control_variates (wost.py:123-126)
+            // boundary_control_variates (wost.py:123-126): the full-scene closest
+            // point is no farther than the Dirichlet one found at the same point,
+            // so the query starts its pruning radius just above that distance
+            // (relative 2^-16 plus 1e-6 of the coordinate scale); it then only
+            // skips subtrees farther than the answer and returns the unbounded
+            // query's record, tie order included
+            float r0 = kInf;
+            if (w.d_dir < kInf) {
+                float sc = fmx(fmx(__builtin_fabsf(F.root_mn[0]), __builtin_fabsf(F.root_mx[0])),
+                               fmx(__builtin_fabsf(F.root_mn[1]), __builtin_fabsf(F.root_mx[1])));
+                r0 = w.d_dir * (1.0f + 0x1p-16f) + 1e-6f * (1.0f + sc);
+            }
+            closest2_begin(t, F, w.ip, r0);
             w.stage = WS_CV;
             return ADV_QUERY;
         }

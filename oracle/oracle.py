@@ -58,6 +58,10 @@ def lib():
         L.orc_cv_bound_check.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
         L.orc_far_check.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
         L.orc_sdf.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+        L.orc_star_radius2_rho.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_float,
+                                           C.c_void_p, C.c_void_p]
+        L.orc_grad_dude.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int,
+                                    C.c_void_p, C.c_int, C.c_float, C.c_void_p, C.c_void_p] + [C.c_void_p] * 4
         L.orc_sample_boundary.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64] + [C.c_void_p] * 5
     return _lib
 
@@ -233,6 +237,35 @@ class OracleScene:
         lib().orc_solve_dude(self.h, _p(pts), _p(e), n, _p(ip), _p(fp), bc_code, _p(c), _p(gv), _p(hv), src_code,
                              src_c, _p(out), _p(st))
         return dict(dude=out, steps=int(st[0]))
+
+    def star_radius2_rho(self, sub: int, pts, e, rho_max=1.2):
+        """(through the SNCH, by the loop over all segments) of Polyline.star_radius_2."""
+        pts, e = _f32(pts), _f32(e)
+        t = np.zeros(len(pts), np.float32)
+        b = np.zeros(len(pts), np.float32)
+        lib().orc_star_radius2_rho(self.h, sub, _p(pts), _p(e), len(pts), rho_max, _p(t), _p(b))
+        return t, b
+
+    def grad_dude(self, pts, e, bc_code, nwalks, nsteps=128, eps=1e-3, min_R=1e-2, cv=True, seed=0, nsubwalks=1,
+                  rho_max=1.2, ignore_boundary=False, ignore_silhouette=False, ignore_source=False, coeffs=None,
+                  src_code=0, src_c=0.0, per_walk=False, base=0, plant=0):
+        """WoStGrad.solve_dude (wost_grad.py, solver.py:113-166), 2D scenes."""
+        pts, e = _f32(pts), _f32(e)
+        n = len(pts)
+        ip, fp = self._params(nwalks, nsteps, nsubwalks, cv, True, seed, eps, min_R, base)
+        c = _f32(np.asarray(coeffs if coeffs is not None else (0, 0, 0, 0)))
+        gp = np.array([rho_max], np.float32)
+        gi = np.array([int(ignore_boundary), int(ignore_silhouette), int(ignore_source), int(plant)], np.int32)
+        out = np.zeros(n, np.float32)
+        wv = np.zeros(n * nwalks, np.float32) if per_walk else None
+        ws = np.zeros(n * nwalks, np.int32) if per_walk else None
+        st = np.zeros(1, np.uint64)
+        lib().orc_grad_dude(self.h, _p(pts), _p(e), n, _p(ip), _p(fp), bc_code, _p(c), src_code, src_c, _p(gp),
+                            _p(gi), _p(out), _p(wv), _p(ws), _p(st))
+        r = dict(dude=out, steps=int(st[0]))
+        if per_walk:
+            r.update(walk_dude=wv, walk_steps=ws)
+        return r
 
     def trace_step(self, pts, bc_code, nwalks, samples_per_point, step_size, nsubwalks=1, nsteps=128, eps=1e-3,
                    min_R=1e-3, cv=True, av=True, seed=0, coeffs=None, gv=None, hv=None, src_code=0, src_c=0.0, base=0):

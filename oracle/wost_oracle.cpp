@@ -289,16 +289,32 @@ struct Tree {
     int max_depth = 0;
 };
 
+// Point primitive of the point BVH over silhouette vertices (primitive.h:116-152:
+// the epsilon box of the point widened by radius 1e-4; centroid = the point)
 template <int D>
+struct PPoint {
+    float v[1][D];
+    int pIndex;
+    Box<D> bbox() const {
+        Box<D> b = Box<D>::point(v[0]);
+        for (int d = 0; d < D; ++d) b.mn[d] = b.mn[d] - 1e-4f, b.mx[d] = b.mx[d] + 1e-4f;
+        return b;
+    }
+    void centroid(float* c) const {
+        for (int d = 0; d < D; ++d) c[d] = v[0][d];
+    }
+};
+
+template <int D, typename P = Prim<D>>
 struct Builder {  // bvh.h:359-652 (SurfaceArea, no leaf packing, leaf 4, 8 buckets)
-    std::vector<Prim<D>>& prims;
+    std::vector<P>& prims;
     std::vector<Box<D>> boxes;
     std::vector<std::array<float, D>> cents;
     std::vector<Node<D>> nodes;
     int max_depth = 0;
     static const int kLeaf = 4, kBuckets = 8;
     static const int UNTOUCHED = (int)0xffffffff, TOUCHED_TWICE = (int)0xfffffffd;
-    explicit Builder(std::vector<Prim<D>>& p) : prims(p) {}
+    explicit Builder(std::vector<P>& p) : prims(p) {}
 
     void build() {
         if (prims.empty()) return;
@@ -663,6 +679,12 @@ struct Sub {  // one Polyline / Mesh
     std::vector<Sil2> sil2, ref2;
     std::vector<Sil3> sil3, ref3;
     std::vector<int> refs;
+    // 2D: the polyline's own silhouette vertices (configure_silhouette,
+    // polyline.py:100-165: a, b, c per vertex with both neighbours, near-collinear
+    // ones dropped) and the point BVH over their b (PointBVH, bvh.py:375-...)
+    std::vector<float> psil;
+    std::vector<Node<D>> pnodes;
+    std::vector<PPoint<D>> pprims;
 };
 
 struct Scene {
@@ -787,6 +809,37 @@ void build_snch<3>(Sub<3>& S) {
 }
 
 // _build_subscene_from_mask (polyline.py:672-692, mesh.py:617-635)
+// Polyline.configure_silhouette (polyline.py:100-165): prev[b] = a and
+// next[a] = b over the edges (a, b) in order (later edges overwrite); every
+// vertex with both, in vertex order, is a silhouette (a = prev, b, c = next)
+// unless |normalize(b - a) . normalize(c - b)| >= 1 - 1e-5 (near-collinear);
+// then PointBVH(silhouettes.b) (bvh.py:375-395; C++ BVH<2, Point2>).
+static void build_psil2(Sub<2>& S) {
+    int nv = (int)(S.V.size() / 2), ne = (int)(S.I.size() / 2);
+    std::vector<int> prv(nv, -1), nxt(nv, -1);
+    for (int e = 0; e < ne; ++e) prv[S.I[2 * e + 1]] = S.I[2 * e], nxt[S.I[2 * e]] = S.I[2 * e + 1];
+    S.psil.clear();
+    for (int v = 0; v < nv; ++v) {
+        if (prv[v] == -1 || nxt[v] == -1) continue;
+        V2 a = v2(S.V[2 * prv[v]], S.V[2 * prv[v] + 1]), b = v2(S.V[2 * v], S.V[2 * v + 1]);
+        V2 c = v2(S.V[2 * nxt[v]], S.V[2 * nxt[v] + 1]);
+        V2 t1 = normalize(sub(b, a)), t2 = normalize(sub(c, b));
+        if (!(std::fabs(dot(t1, t2)) < (1.0f - 1e-5f))) continue;
+        float q[6] = {a.x, a.y, b.x, b.y, c.x, c.y};
+        S.psil.insert(S.psil.end(), q, q + 6);
+    }
+    S.pprims.clear();
+    for (size_t k = 0; k < S.psil.size() / 6; ++k) {
+        PPoint<2> pt;
+        pt.v[0][0] = S.psil[6 * k + 2], pt.v[0][1] = S.psil[6 * k + 3];
+        pt.pIndex = (int)k;
+        S.pprims.push_back(pt);
+    }
+    Builder<2, PPoint<2>> b(S.pprims);
+    b.build();
+    S.pnodes = b.nodes;
+}
+
 template <int D>
 static void make_sub(const Scene& sc, int want_type, Sub<D>& S) {
     size_t np = sc.types.size();
@@ -814,6 +867,7 @@ static void make_sub(const Scene& sc, int want_type, Sub<D>& S) {
     S.pglobal = sel;
     build_sub<D>(S);
     build_snch<D>(S);
+    if constexpr (D == 2) build_psil2(S);
 }
 
 // ----------------------------------------------------------------------------
@@ -2254,6 +2308,487 @@ static PairOut fused_pair(const Scene& sc, const BC& bc, const Params& P, const 
     return o;
 }
 
+// ============================================================================
+// WoStGrad (diff_wost/solvers/wost_grad.py): the robust-derivative walk, 2D
+// ============================================================================
+struct GradParams {
+    float rho_max;
+    int ignore_boundary, ignore_silhouette, ignore_source;
+    int plant;  // tests only: 0 = the reference; 1/2/3 plant a mistake (K's sign, e kept at reflections, dh/dt's sign)
+};
+
+// Second derivatives of the analytic fields (for dh/dt): (u_xx, u_xy, u_yy)
+static inline void bc_hess2(const BC& bc, V2 p, float& hxx, float& hxy, float& hyy) {
+    if (bc.code == BC_EXPCOS) {
+        float ex = cr_exp(p.x), c = cr_cos(p.y), sn = cr_sin(p.y);
+        hxx = ex * c, hxy = -(ex * sn), hyy = -(ex * c);
+    } else if (bc.code == BC_QUAD) {
+        hxx = 2.0f, hxy = 0.0f, hyy = -2.0f;
+    } else {
+        hxx = hxy = hyy = 0.0f;
+    }
+}
+// the scene callbacks of the analytic field u (examples/example_2d/scene.py:
+// g = u, h = n . grad u, and their derivatives along the tangent (-n.y, n.x))
+static inline float grad_h2(const BC& bc, V2 p, V2 n) {
+    V3 g = bc_grad(bc, v3(p.x, p.y, 0.0f));
+    return dot(n, v2(g.x, g.y));
+}
+static inline float grad_dgdt2(const BC& bc, V2 p, V2 t) {
+    V3 g = bc_grad(bc, v3(p.x, p.y, 0.0f));
+    return dot(v2(g.x, g.y), t);
+}
+static inline float grad_dhdt2(const BC& bc, V2 p, V2 n) {
+    float hxx, hxy, hyy;
+    bc_hess2(bc, p, hxx, hxy, hyy);
+    V2 t = v2(-n.y, n.x);
+    return dot(n, v2(hxx * t.x + hxy * t.y, hxy * t.x + hyy * t.y));
+}
+
+// LineSegment.star_radius_2 (line_segment.py:169-212)
+static float seg_star_radius2(V2 a, V2 b, V2 x, V2 e, float rho_max, float r_max) {
+    float R = r_max;
+    V2 tangent = normalize(sub(b, a));
+    V2 normal = v2(tangent.y, -tangent.x);
+    V2 dir = normalize(sub(a, x));
+    if (dot(dir, normal) > 0.0f) {
+        float en = dot(e, normal), et = dot(e, tangent), d = dot(sub(a, x), normal);
+        float t1 = -kInf, t2 = kInf;
+        V2 pt = add(x, mul(normal, d));
+        if (std::fabs(en) > 1e-4f) {
+            t1 = d / en * (rho_max + et);
+            t2 = -d / en * (rho_max - et);
+        }
+        float ta = dot(sub(a, x), tangent), tb = dot(sub(b, x), tangent);
+        float tmin = fminf_(ta, tb), tmax = fmaxf_(ta, tb);
+        if ((t1 > tmin) && (t1 < tmax)) R = fminf_(R, norm(sub(add(pt, mul(tangent, t1)), x)));
+        if ((t2 > tmin) && (t2 < tmax)) R = fminf_(R, norm(sub(add(pt, mul(tangent, t2)), x)));
+        if (((t1 > tmax) && (t2 > tmax)) || ((t1 < tmin) && (t2 < tmin))) {
+            R = fminf_(R, norm(sub(a, x)));
+            R = fminf_(R, norm(sub(b, x)));
+        }
+    } else {
+        R = fminf_(R, norm(sub(a, x)));
+        R = fminf_(R, norm(sub(b, x)));
+    }
+    return R;
+}
+
+// SNCHNode.overlaps_2 (snch.py:100-174)
+static bool snch_overlaps2(const Node<2>& nd, V2 x, V2 e, float rho_bound, float r_max, float& d_min) {
+    float d_max;
+    V2 mn = node_min2(nd), mx = node_max2(nd);
+    box_dist(mn, mx, x, d_min, d_max);
+    bool ov = d_min <= r_max;
+    float ha = nd.half_angle;
+    if ((ha >= 0.0f) && ov) {
+        ov = (ha > kPi2) && (d_min < kDrEps);
+        if (!ov) {
+            V2 c = divs(add(mn, mx), 2.0f);
+            V2 vca = sub(c, x);
+            float length = norm(vca);
+            vca = divs(vca, length);
+            ov = length < nd.radius;
+            if (!ov) {
+                V2 axis = v2(nd.axis[0], nd.axis[1]);
+                float axis_angle = cr_acos(clampf(dot(axis, vca), -1.0f, 1.0f));
+                float vch = cr_asin(nd.radius / length);
+                float hs = ha + vch;
+                float amin = axis_angle - hs, amax = axis_angle + hs;
+                ov = (hs > kPi2) || in_range(kPi2, amin, amax);
+                if (!ov) {
+                    float c0 = cr_cos(amin), c1 = cr_cos(amax);
+                    float dn_min = fminf_(c0, c1), dn_max = fmaxf_(c0, c1);
+                    float ed_axis = cr_acos(clampf(dot(e, vca), -1.0f, 1.0f));
+                    float e0 = cr_cos(ed_axis - ha), e1 = cr_cos(ed_axis + ha);
+                    float ed_min = fminf_(e0, e1), ed_max = fmaxf_(e0, e1);
+                    float en_axis = cr_acos(clampf(dot(e, axis), -1.0f, 1.0f));
+                    float n0 = cr_cos(en_axis - ha), n1 = cr_cos(en_axis + ha);
+                    float en_min = fminf_(n0, n1), en_max = fmaxf_(n0, n1);
+                    float en_dn_max = en_max / dn_min;
+                    float en_dn_min = en_min < 0.0f ? en_min / dn_min : en_min / dn_max;
+                    auto rho = [](float ed, float q) { return (1.0f - 2.0f * ed * q) + q * q; };
+                    float rm = rho(ed_min, en_dn_min);
+                    rm = fmaxf_(rm, rho(ed_min, en_dn_max));
+                    rm = fmaxf_(rm, rho(ed_max, en_dn_min));
+                    rm = fmaxf_(rm, rho(ed_max, en_dn_max));
+                    ov = rm > rho_bound * rho_bound;
+                }
+            }
+        }
+    }
+    return ov;
+}
+
+// Polyline.star_radius_2 (polyline.py:317-330) -> SNCH.star_radius_2 (snch.py:380-453)
+static float star_radius2_rho(const Sub<2>& S, V2 x, V2 e, float rho_max) {
+    float m0 = fmaxf_(std::fabs(x.x - S.bb_min[0]), std::fabs(x.x - S.bb_max[0]));
+    float m1 = fmaxf_(std::fabs(x.y - S.bb_min[1]), std::fabs(x.y - S.bb_max[1]));
+    float r_max = norm(v2(m0, m1));
+    const Tree<2>& T = S.tree;
+    float dmin, dmax;
+    box_dist(node_min2(T.nodes[0]), node_max2(T.nodes[0]), x, dmin, dmax);
+    if (!(dmin <= r_max)) return r_max;
+    int st_i[64];
+    float st_d[64];
+    int sp = 0;
+    st_i[0] = 0;
+    st_d[0] = dmin;
+    while (sp >= 0) {
+        int ni = st_i[sp];
+        float cd = st_d[sp];
+        sp--;
+        if (!(cd <= r_max)) continue;
+        const Node<2>& nd = T.nodes[ni];
+        if (nd.nref > 0) {
+            for (int j = 0; j < nd.nref; ++j) {
+                const Prim<2>& pr = T.prims[nd.off + j];
+                float r = seg_star_radius2(v2(pr.v[0][0], pr.v[0][1]), v2(pr.v[1][0], pr.v[1][1]), x, e, rho_max,
+                                           r_max);
+                if (r < r_max) r_max = r;
+            }
+        } else {
+            int L = ni + 1, R = ni + nd.off;
+            float d0, d1;
+            bool h0 = snch_overlaps2(T.nodes[L], x, e, rho_max, r_max, d0);
+            bool h1 = snch_overlaps2(T.nodes[R], x, e, rho_max, r_max, d1);
+            if (h0 && h1) {
+                int closer = L, other = R;
+                if (d1 < d0) {
+                    std::swap(closer, other);
+                    std::swap(d0, d1);
+                }
+                st_i[++sp] = other;
+                st_d[sp] = d1;
+                st_i[++sp] = closer;
+                st_d[sp] = d0;
+            } else if (h0) {
+                st_i[++sp] = L;
+                st_d[sp] = d0;
+            } else if (h1) {
+                st_i[++sp] = R;
+                st_d[sp] = d1;
+            }
+        }
+    }
+    return r_max;
+}
+
+// One root-to-leaf descent of BVH._sample_boundary (bvh.py:289-369; the point
+// BVH's twin, bvh.py:512-...): each inner node picks a child overlapping the
+// ball with probability proportional to |G(x, R, box centre)|; the leaf draws
+// one of the primitives that lie in the ball (all of them when the chosen box
+// lies inside it) by reservoir sampling weighted by `weight` (segment length,
+// or 1 for points).  Returns the sorted index (-1: none) and the pmf.
+template <typename Acc, typename Wt>
+static int bvh2_descend(const std::vector<Node<2>>& nodes, V2 x, float R, Pcg& rng, bool& hit, float& pdf,
+                        Acc accept, Wt weight) {
+    hit = false;
+    pdf = 1.0f;
+    int sorted = -1;
+    float r_max = kInf;
+    int ni = 0;
+    while (true) {
+        const Node<2>& nd = nodes[ni];
+        if (nd.nref > 0) {
+            float total = 0.0f;
+            for (int j = 0; j < nd.nref; ++j) {
+                int k = nd.off + j;
+                float sa = weight(k);
+                if ((r_max <= R) || accept(k)) {
+                    hit = true;
+                    total += sa;
+                    float sel = sa / total;
+                    float u = rng.next_float32();
+                    if (u < sel) sorted = k;
+                }
+            }
+            if (total > 0.0f) pdf *= (sorted >= 0 ? weight(sorted) : 0.0f) / total;
+            return sorted;
+        }
+        int L = ni + 1, Rn = ni + nd.off;
+        float d0, m0, d1, m1;
+        V2 mn0 = node_min2(nodes[L]), mx0 = node_max2(nodes[L]), mn1 = node_min2(nodes[Rn]), mx1 = node_max2(nodes[Rn]);
+        box_dist(mn0, mx0, x, d0, m0);
+        box_dist(mn1, mx1, x, d1, m1);
+        float w0 = (d0 <= R) ? 1.0f : 0.0f, w1 = (d1 <= R) ? 1.0f : 0.0f;
+        if (w0 > 0.0f) w0 *= std::fabs(greens2_G(x, R, x, divs(add(mn0, mx0), 2.0f)));
+        if (w1 > 0.0f) w1 *= std::fabs(greens2_G(x, R, x, divs(add(mn1, mx1), 2.0f)));
+        float tw = w0 + w1;
+        if (!(tw > 0.0f)) return sorted;
+        float p0 = w0 / tw, p1 = 1.0f - p0;
+        float u = rng.next_float32();
+        if (u < p0) {
+            ni = L;
+            pdf *= p0;
+            r_max = m0;
+        } else {
+            ni = Rn;
+            pdf *= p1;
+            r_max = m1;
+        }
+    }
+}
+
+struct BSample2 {
+    bool hit;
+    V2 p, n;
+    float t, pdf;
+    int prim;
+};
+// Polyline.sample_boundary_bvh (polyline.py:515-517) -> BVH.sample_boundary (bvh.py:266-287)
+static BSample2 sample_boundary_bvh2(const Sub<2>& S, V2 x, float R, Pcg& rng) {
+    BSample2 b;
+    std::memset(&b, 0, sizeof(b));
+    const Tree<2>& T = S.tree;
+    float dmin, dmax;
+    box_dist(node_min2(T.nodes[0]), node_max2(T.nodes[0]), x, dmin, dmax);
+    if (!(dmin <= R)) return b;
+    auto seg = [&](int k, V2& a, V2& c) {
+        if (k < 0) {
+            a = c = v2(0.0f, 0.0f);
+            return;
+        }
+        a = v2(T.prims[k].v[0][0], T.prims[k].v[0][1]);
+        c = v2(T.prims[k].v[1][0], T.prims[k].v[1][1]);
+    };
+    auto weight = [&](int k) {  // LineSegment.surface_area (line_segment.py:32-33)
+        V2 a, c;
+        seg(k, a, c);
+        return norm(sub(c, a));
+    };
+    auto accept = [&](int k) {  // LineSegment.sphere_intersect (line_segment.py:43-45)
+        V2 a, c;
+        seg(k, a, c);
+        V2 u = sub(c, a), w = sub(x, a);
+        float t = clampf(dot(w, u) / dot(u, u), 0.0f, 1.0f);
+        return norm(sub(lerp(a, c, t), x)) <= R;
+    };
+    bool hit;
+    float pdf;
+    int k = bvh2_descend(T.nodes, x, R, rng, hit, pdf, accept, weight);
+    if (!hit) return b;
+    V2 a, c;
+    seg(k, a, c);
+    float t = rng.next_float32();  // LineSegment.sample_point (line_segment.py:36-40)
+    V2 tg = sub(c, a);
+    b.hit = true;
+    b.p = add(a, mul(tg, t));
+    b.n = normalize(v2(tg.y, -tg.x));
+    b.t = t;
+    b.pdf = pdf * (1.0f / norm(tg));
+    b.prim = k >= 0 ? T.prims[k].pIndex : 0;
+    return b;
+}
+
+struct SSample2 {
+    bool hit;
+    V2 p, n1, n2, t1, t2;
+    float pdf;
+    int prim;
+};
+// Polyline.sample_silhouette_bvh (polyline.py:519-535) -> PointBVH.sample_boundary
+static SSample2 sample_silhouette_bvh2(const Sub<2>& S, V2 x, float R, Pcg& rng) {
+    SSample2 r;
+    std::memset(&r, 0, sizeof(r));
+    if (S.pnodes.empty()) return r;
+    float dmin, dmax;
+    box_dist(node_min2(S.pnodes[0]), node_max2(S.pnodes[0]), x, dmin, dmax);
+    if (!(dmin <= R)) return r;
+    auto weight = [](int) { return 1.0f; };
+    auto accept = [&](int k) {
+        return norm(sub(v2(S.pprims[k].v[0][0], S.pprims[k].v[0][1]), x)) <= R;
+    };
+    bool hit;
+    float pdf;
+    int k = bvh2_descend(S.pnodes, x, R, rng, hit, pdf, accept, weight);
+    if (!hit) return r;
+    int si = k >= 0 ? S.pprims[k].pIndex : 0;
+    const float* q = &S.psil[6 * si];
+    V2 a = v2(q[0], q[1]), b = v2(q[2], q[3]), c = v2(q[4], q[5]);
+    r.hit = true;
+    r.t1 = normalize(sub(b, a));
+    r.t2 = normalize(sub(b, c));
+    r.n1 = v2(r.t1.y, -r.t1.x);
+    r.n2 = v2(-r.t2.y, r.t2.x);
+    r.p = b;
+    r.pdf = pdf * 1.0f;
+    r.prim = si;
+    return r;
+}
+
+// Polyline.off_centered_ball (polyline.py:583-600): bisection for the largest
+// ball tangent at p that the boundary does not cut
+static void off_centered_ball2(const Sub<2>& F, V2 p, V2 n, V2& c, float& radius) {
+    float b = fminf_(F.bb_max[0] - F.bb_min[0], F.bb_max[1] - F.bb_min[1]) / 2.0f;
+    float a = 1e-3f;
+    for (int i = 0; i < 10; ++i) {
+        float m = (a + b) / 2.0f;
+        float d = std::fabs(bvh2_closest(F.tree, sub(p, mul(n, m)), nullptr).d);
+        if (std::fabs(d - m) < 1e-3f)
+            a = m;
+        else
+            b = m;
+    }
+    c = sub(p, mul(n, a));
+    radius = a;
+}
+
+// WoStGrad.dudn (wost_grad.py:207-254): the normal derivative at a Dirichlet
+// point from one WoSt walk (nsteps 200, eps 1e-4) started on the off-centred ball
+static float grad_dudn2(const Scene& sc, const BC& bc, int cv, const CP& its, Pcg& rng, int& nested_steps) {
+    const Sub<2>& F = sc.s2[0];
+    const Sub<2>& Dn = sc.s2[1];
+    const Sub<2>& Nm = sc.s2[2];
+    V2 ip = v2(its.p.x, its.p.y), in = v2(its.n.x, its.n.y);
+    V2 c;
+    float R;
+    off_centered_ball2(F, ip, in, c, R);
+    float theta = (rng.next_float32() * 2.0f) * kPi;  // uniform_angle
+    theta = clampf(theta, 3e-1f, kTwoPi - 3e-1f);
+    V2 pd = v2(-in.y, in.x);
+    V2 dir = add(mul(in, cr_cos(theta)), mul(pd, cr_sin(theta)));
+    V2 p = add(c, mul(dir, R));
+    float u_ref = bc_u(bc, its.p);
+    Params W;
+    std::memset(&W, 0, sizeof(W));
+    W.nwalks = 1, W.nsubwalks = 1, W.nsteps = 200, W.eps = 1e-4f, W.min_R = 1e-3f, W.cv = 1, W.av = 1;
+    WalkOut w = wost_u2(sc, bc, W, p, rng, false, v2(0.0f, 0.0f), false);
+    nested_steps += w.steps;
+    float u = w.value;
+    if (Nm.present) {
+        float dd = bvh2_closest(Dn.tree, p, nullptr).d;
+        float dnm = bvh2_closest(Nm.tree, p, nullptr).d;
+        if ((dd < 2e-2f) && (dnm < 2e-2f)) u = u_ref;
+    }
+    float P = 1.0f / (cr_cos(theta) - 1.0f);
+    float dudn = 0.0f;
+    if (cv)
+        dudn += P * (u - u_ref) / R;
+    else
+        dudn += P * u / R;
+    float rr = std::sqrt(rng.next_float32());  // uniform_in_disk (math.py:285-288)
+    float th = kTwoPi * rng.next_float32();
+    V2 p_src = add(c, mul(v2(rr * cr_cos(th), rr * cr_sin(th)), R));
+    float f_ref = src_f(bc), f = src_f(bc);
+    float x2 = dot(sub(ip, c), sub(ip, c));
+    float y2 = dot(sub(p_src, c), sub(p_src, c));
+    float Pq = (x2 - y2) / fmaxf_(dot(sub(ip, p_src), sub(ip, p_src)), 1e-6f) * R / 2.0f;
+    if (cv)
+        dudn -= (f - f_ref) * Pq + f_ref * R / 2.0f;
+    else
+        dudn -= f * Pq;
+    return dudn;
+}
+
+// WoStGrad.dude (wost_grad.py:91-205)
+static float grad_dude2(const Scene& sc, const BC& bc, const Params& P, const GradParams& G, V2 x, V2 e, Pcg& rng,
+                        int& steps) {
+    const Sub<2>& F = sc.s2[0];
+    const Sub<2>& Dn = sc.s2[1];
+    const Sub<2>& Nm = sc.s2[2];
+    V2 ip = x, in = v2(0.0f, 0.0f);
+    bool ob = false;
+    float result = 0.0f;
+    int i = 0;
+    bool active = ob || inside2(sc, ip);
+    float thr = 1.0f;
+    while ((i < P.nsteps) && active) {
+        bool inflate = false;
+        float d_dir = kInf;
+        if (Dn.present) d_dir = std::fabs(bvh2_closest(Dn.tree, ip, nullptr).d);
+        if (d_dir < P.eps) {
+            CP cr = bvh2_closest(F.tree, ip, nullptr);
+            V2 nrm = v2(cr.n.x, cr.n.y), tg = v2(-nrm.y, nrm.x);
+            float dudt = grad_dgdt2(bc, v2(cr.p.x, cr.p.y), tg);
+            float dudn = grad_dudn2(sc, bc, P.cv, cr, rng, steps);
+            result += thr * dudt * dot(e, tg);
+            result += thr * dudn * dot(e, nrm);
+            active = false;
+        } else {
+            float R = d_dir;
+            if (Nm.present) R = fminf_(R, star_radius2_rho(Nm, ip, e, G.rho_max));
+            if (ob) {
+                if (R < P.min_R) inflate = true;
+                R = fmaxf_(R, P.min_R);
+            }
+            float alpha = ob ? 2.0f : 1.0f;
+            if (Nm.present) {
+                if (!G.ignore_boundary) {
+                    BSample2 b = sample_boundary_bvh2(Nm, ip, R, rng);
+                    if (b.hit) {  // handle_boundary (wost_grad.py:28-51)
+                        float Gv = clampf(greens2_G(ip, R, ip, b.p), -100.0f, 100.0f);
+                        float res = 0.0f;
+                        if (norm(sub(ip, b.p)) <= R) {
+                            V2 tg = v2(-b.n.y, b.n.x);
+                            float e_t = dot(e, tg), e_n = dot(e, b.n);
+                            float f_tn = grad_dhdt2(bc, b.p, b.n);
+                            if (G.plant == 3) f_tn = -f_tn;
+                            res += e_t * f_tn - e_n * src_f(bc);
+                        }
+                        result += thr * alpha * (-Gv * res / b.pdf);
+                    }
+                }
+                if (!G.ignore_silhouette) {
+                    SSample2 sr = sample_silhouette_bvh2(Nm, ip, R, rng);
+                    if (!inflate && sr.hit) {  // handle_silhouette (wost_grad.py:53-72)
+                        float Gv = clampf(greens2_G(ip, R, ip, sr.p), -100.0f, 100.0f);
+                        float res = 0.0f;
+                        if (norm(sub(sr.p, ip)) <= R) {
+                            float h1 = grad_h2(bc, sr.p, sr.n1), h2 = grad_h2(bc, sr.p, sr.n2);
+                            res -= h1 * dot(e, sr.t1);
+                            res -= h2 * dot(e, sr.t2);
+                        }
+                        result += thr * alpha * (-Gv * res / sr.pdf);
+                    }
+                }
+            }
+            V2 dir = uniform_dir2(rng);
+            if (ob && (dot(dir, in) > 0.0f)) dir = neg(dir);
+            V2 _p = ip;
+            V2 o = ob ? sub(ip, mul(in, 1e-5f)) : ip;  // Polyline.intersect (polyline.py:435-466)
+            Its its = bvh2_intersect(F.tree, o, dir, R, nullptr);
+            if (!its.valid) {
+                V2 q = add(o, mul(dir, R));
+                its.p = v3(q.x, q.y, 0.0f);
+                its.n = v3(dir.x, dir.y, 0.0f);
+                its.d = R;
+                its.on_boundary = false;
+            }
+            ip = v2(its.p.x, its.p.y);
+            in = v2(its.n.x, its.n.y);
+            ob = its.on_boundary;
+            if (!G.ignore_source) {  // handle_source (wost_grad.py:74-89)
+                V2 sp;
+                float sr_, spdf;
+                greens2_sample(_p, R, dir, rng, sp, sr_, spdf);
+                float res = 0.0f;
+                if (norm(sub(sp, _p)) < std::fabs(its.d)) res += (R * R / 4.0f) * dot(v2(0.0f, 0.0f), e);
+                result += thr * res;
+            }
+            if (ob) {
+                V2 tg = v2(-in.y, in.x);
+                float e_t = dot(e, tg), e_n = dot(e, in);
+                float K = dot(dir, tg) / dot(dir, in);
+                if (G.plant == 1) K = -K;
+                result += thr * e_n * grad_h2(bc, ip, in);
+                thr *= e_t - K * e_n;
+                if (G.plant != 2) e = tg;
+            }
+            if (std::fabs(thr) < 1.0f) {  // Russian roulette
+                if (rng.next_float32() > std::fabs(thr))
+                    active = false;
+                else
+                    thr /= std::fabs(thr);
+            }
+        }
+        i += 1;
+        if (bad(result) || std::fabs(result) > 1000.0f) result = 0.0f;
+    }
+    steps += i;
+    return result;
+}
+
 }  // namespace orc
 
 // ============================================================================
@@ -2713,6 +3248,67 @@ void orc_trace_step(void* s, const float* pts, int64_t n, const int32_t* ip, con
         }
     if (samples) std::memcpy(samples, smp.data(), smp.size() * sizeof(float));
     if (steps_total) *steps_total = steps;
+}
+
+// WoStGrad.solve_dude through Solver.solve_dude (solver.py:113-144) and
+// walk_dude (:146-166): lane i * nwalks + k runs nsubwalks dudes on PCG32 lane
+// (base + i) * nwalks + k seeded with TEA(seed, base + i).  gp = {rho_max};
+// gi = {ignore_boundary, ignore_silhouette, ignore_source}.  out[n] = mean over
+// the walks; walk_val[n*nwalks] and walk_steps (outer steps + nested WoSt
+// steps) optional.  gi[3] plants a mistake (tests only; 0: the reference).
+// 2D scenes only.
+void orc_grad_dude(void* s, const float* pts, const float* ev, int64_t n, const int32_t* ip, const float* fp,
+                   int bc_code, const float* bc_c, int src_code, float src_c, const float* gp, const int32_t* gi,
+                   float* out, float* walk_val, int32_t* walk_steps, uint64_t* steps_total) {
+    Scene* sc = (Scene*)s;
+    Params P = make_params(ip, fp);
+    BC bc = make_bc(bc_code, bc_c, nullptr, nullptr, src_code, src_c);
+    GradParams G{gp[0], gi[0], gi[1], gi[2], gi[3]};
+    int64_t W = P.nwalks;
+    uint64_t steps = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : steps)
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t a, b;
+        tea(P.seed, (uint32_t)(P.base + i), 4, kTeaKey, &a, &b);
+        V2 x = v2(pts[2 * i], pts[2 * i + 1]), e = v2(ev[2 * i], ev[2 * i + 1]);
+        double acc = 0.0;
+        for (int64_t k = 0; k < W; ++k) {
+            Pcg rng = pcg_for_lane(a, b, (uint64_t)((P.base + i) * W + k));
+            float v = 0.0f;
+            int st = 0;
+            for (int sw = 0; sw < P.nsubwalks; ++sw) v += grad_dude2(*sc, bc, P, G, x, e, rng, st);
+            v = v / (float)P.nsubwalks;
+            acc += (double)v;
+            steps += (uint64_t)st;
+            if (walk_val) walk_val[i * W + k] = v;
+            if (walk_steps) walk_steps[i * W + k] = st;
+        }
+        if (out) out[i] = (float)(acc / (double)W);
+    }
+    if (steps_total) *steps_total = steps;
+}
+
+// The rho-bounded star radius of a 2D sub-scene two ways: through the SNCH
+// (Polyline.star_radius_2, polyline.py:317-330) and by the reference's loop over
+// every segment (star_radius_2_3, polyline.py:332-375).  tree[n], brute[n].
+void orc_star_radius2_rho(void* s, int sub, const float* pts, const float* ev, int64_t n, float rho_max, float* tree,
+                          float* brute) {
+    Scene* sc = (Scene*)s;
+    const Sub<2>& S = sc->s2[sub];
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; ++i) {
+        V2 x = v2(pts[2 * i], pts[2 * i + 1]), e = v2(ev[2 * i], ev[2 * i + 1]);
+        tree[i] = star_radius2_rho(S, x, e, rho_max);
+        float m0 = fmaxf_(std::fabs(x.x - S.bb_min[0]), std::fabs(x.x - S.bb_max[0]));
+        float m1 = fmaxf_(std::fabs(x.y - S.bb_min[1]), std::fabs(x.y - S.bb_max[1]));
+        float R = norm(v2(m0, m1));
+        for (size_t j = 0; j < S.I.size() / 2; ++j) {
+            V2 a = v2(S.V[2 * S.I[2 * j]], S.V[2 * S.I[2 * j] + 1]);
+            V2 b = v2(S.V[2 * S.I[2 * j + 1]], S.V[2 * S.I[2 * j + 1] + 1]);
+            R = seg_star_radius2(a, b, x, e, rho_max, R);
+        }
+        brute[i] = R;
+    }
 }
 
 // Fused u + grad u.  pair_u[n*np], pair_g[n*np*dim], pair_steps[n*np] optional.

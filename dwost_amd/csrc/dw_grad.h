@@ -1,0 +1,497 @@
+// dw_grad.h -- WoStGrad (diff_wost/solvers/wost_grad.py), the robust
+// derivative walk, 2D: du/de from a walk that reflects off Neumann boundaries,
+// with the rho-bounded star radius, BVH sampling of the Neumann boundary and of
+// its silhouette vertices, and a nested WoSt walk for the normal derivative at
+// a Dirichlet point.  The queries walk the sub-scene's node-major tree
+// (DNode2 / DCone2 / DSeg, the reference's flat layout) one lane at a time.
+#pragma once
+#include "dw_walk.h"
+
+namespace dwd {
+
+struct GradDev {
+    float rho_max;
+    int ignore_boundary, ignore_silhouette, ignore_source;
+};
+
+// Second derivatives of the analytic field (for dh/dt): (u_xx, u_xy, u_yy)
+DWD void bc_hess2(const BCDev& bc, F2 p, float& hxx, float& hxy, float& hyy) {
+    if (bc.code == DW_BC_EXPCOS) {
+        float ex = cr_exp(p.x), c = cr_cos(p.y), sn = cr_sin(p.y);
+        hxx = ex * c, hxy = -(ex * sn), hyy = -(ex * c);
+    } else if (bc.code == DW_BC_QUAD) {
+        hxx = 2.0f, hxy = 0.0f, hyy = -2.0f;
+    } else {
+        hxx = hxy = hyy = 0.0f;
+    }
+}
+// the scene callbacks of the analytic field: h = n . grad u, and the
+// derivatives of g and h along the tangent (-n.y, n.x)
+DWD float grad_h2(const BCDev& bc, F2 p, F2 n) {
+    F3 g = bc_grad(bc, f3(p.x, p.y, 0.0f));
+    return dot(n, f2(g.x, g.y));
+}
+DWD float grad_dgdt2(const BCDev& bc, F2 p, F2 t) {
+    F3 g = bc_grad(bc, f3(p.x, p.y, 0.0f));
+    return dot(f2(g.x, g.y), t);
+}
+DWD float grad_dhdt2(const BCDev& bc, F2 p, F2 n) {
+    float hxx, hxy, hyy;
+    bc_hess2(bc, p, hxx, hxy, hyy);
+    F2 t = f2(-n.y, n.x);
+    return dot(n, f2(hxx * t.x + hxy * t.y, hxy * t.x + hyy * t.y));
+}
+
+DWD F2 node_mn(const DNode2& n) { return f2(n.mnx, n.mny); }
+DWD F2 node_mx(const DNode2& n) { return f2(n.mxx, n.mxy); }
+
+// LineSegment.star_radius_2 (line_segment.py:169-212)
+DWD float seg_star_radius2(F2 a, F2 b, F2 x, F2 e, float rho_max, float r_max) {
+    float R = r_max;
+    F2 tangent = normalize(b - a);
+    F2 normal = f2(tangent.y, -tangent.x);
+    F2 dir = normalize(a - x);
+    if (dot(dir, normal) > 0.0f) {
+        float en = dot(e, normal), et = dot(e, tangent), d = dot(a - x, normal);
+        float t1 = -kInf, t2 = kInf;
+        F2 pt = x + normal * d;
+        if (__builtin_fabsf(en) > 1e-4f) {
+            t1 = d / en * (rho_max + et);
+            t2 = -d / en * (rho_max - et);
+        }
+        float ta = dot(a - x, tangent), tb = dot(b - x, tangent);
+        float tmin = fmn(ta, tb), tmax = fmx(ta, tb);
+        if ((t1 > tmin) && (t1 < tmax)) R = fmn(R, norm((pt + tangent * t1) - x));
+        if ((t2 > tmin) && (t2 < tmax)) R = fmn(R, norm((pt + tangent * t2) - x));
+        if (((t1 > tmax) && (t2 > tmax)) || ((t1 < tmin) && (t2 < tmin))) {
+            R = fmn(R, norm(a - x));
+            R = fmn(R, norm(b - x));
+        }
+    } else {
+        R = fmn(R, norm(a - x));
+        R = fmn(R, norm(b - x));
+    }
+    return R;
+}
+
+// SNCHNode.overlaps_2 (snch.py:100-174)
+DWD bool snch_overlaps2(const DNode2& nd, const DCone2& cn, F2 x, F2 e, float rho_bound, float r_max,
+                        float& d_min) {
+    float d_max;
+    F2 mn = node_mn(nd), mx = node_mx(nd);
+    box_dist(mn, mx, x, d_min, d_max);
+    bool ov = d_min <= r_max;
+    float ha = cn.half_angle;
+    if ((ha >= 0.0f) && ov) {
+        ov = (ha > kPi2) && (d_min < kDrEps);
+        if (!ov) {
+            F2 c = (mn + mx) / 2.0f;
+            F2 vca = c - x;
+            float length = norm(vca);
+            vca = vca / length;
+            ov = length < cn.radius;
+            if (!ov) {
+                F2 axis = f2(cn.ax, cn.ay);
+                float axis_angle = cr_acos(clampf(dot(axis, vca), -1.0f, 1.0f));
+                float vch = cr_asin(cn.radius / length);
+                float hs = ha + vch;
+                float amin = axis_angle - hs, amax = axis_angle + hs;
+                ov = (hs > kPi2) || ((kPi2 >= amin) && (kPi2 <= amax));
+                if (!ov) {
+                    float c0 = cr_cos(amin), c1 = cr_cos(amax);
+                    float dn_min = fmn(c0, c1), dn_max = fmx(c0, c1);
+                    float ed_axis = cr_acos(clampf(dot(e, vca), -1.0f, 1.0f));
+                    float e0 = cr_cos(ed_axis - ha), e1 = cr_cos(ed_axis + ha);
+                    float ed_min = fmn(e0, e1), ed_max = fmx(e0, e1);
+                    float en_axis = cr_acos(clampf(dot(e, axis), -1.0f, 1.0f));
+                    float n0 = cr_cos(en_axis - ha), n1 = cr_cos(en_axis + ha);
+                    float en_min = fmn(n0, n1), en_max = fmx(n0, n1);
+                    float en_dn_max = en_max / dn_min;
+                    float en_dn_min = en_min < 0.0f ? en_min / dn_min : en_min / dn_max;
+                    auto rho = [](float ed, float q) { return (1.0f - 2.0f * ed * q) + q * q; };
+                    float rm = rho(ed_min, en_dn_min);
+                    rm = fmx(rm, rho(ed_min, en_dn_max));
+                    rm = fmx(rm, rho(ed_max, en_dn_min));
+                    rm = fmx(rm, rho(ed_max, en_dn_max));
+                    ov = rm > rho_bound * rho_bound;
+                }
+            }
+        }
+    }
+    return ov;
+}
+
+// Polyline.star_radius_2 (polyline.py:317-330) -> SNCH.star_radius_2 (snch.py:380-453)
+DWD float star_radius2_rho(const SubDev& S, F2 x, F2 e, float rho_max) {
+    const DNode2* nodes = (const DNode2*)S.nodes;
+    const DCone2* cones = (const DCone2*)S.cones;
+    const DSeg* prims = (const DSeg*)S.prims;
+    float m0 = fmx(__builtin_fabsf(x.x - S.bb_min[0]), __builtin_fabsf(x.x - S.bb_max[0]));
+    float m1 = fmx(__builtin_fabsf(x.y - S.bb_min[1]), __builtin_fabsf(x.y - S.bb_max[1]));
+    float r_max = norm(f2(m0, m1));
+    float dmin, dmax;
+    box_dist(node_mn(nodes[0]), node_mx(nodes[0]), x, dmin, dmax);
+    if (!(dmin <= r_max)) return r_max;
+    int st_i[DW_MAX_STACK];
+    float st_d[DW_MAX_STACK];
+    int sp = 0;
+    st_i[0] = 0;
+    st_d[0] = dmin;
+    while (sp >= 0) {
+        int ni = st_i[sp];
+        float cd = st_d[sp];
+        sp--;
+        if (!(cd <= r_max)) continue;
+        const DNode2 nd = nodes[ni];
+        if (nd.nref > 0) {
+            for (int j = 0; j < nd.nref; ++j) {
+                DSeg sg = prims[nd.off + j];
+                float r = seg_star_radius2(f2(sg.ax, sg.ay), f2(sg.bx, sg.by), x, e, rho_max, r_max);
+                if (r < r_max) r_max = r;
+            }
+        } else {
+            int L = ni + 1, R = ni + nd.off;
+            float d0, d1;
+            bool h0 = snch_overlaps2(nodes[L], cones[L], x, e, rho_max, r_max, d0);
+            bool h1 = snch_overlaps2(nodes[R], cones[R], x, e, rho_max, r_max, d1);
+            // the builder's depth cap (62) keeps the stack within DW_MAX_STACK
+            if (h0 && h1) {
+                int closer = L, other = R;
+                if (d1 < d0) {
+                    closer = R, other = L;
+                    float tmp = d0;
+                    d0 = d1, d1 = tmp;
+                }
+                st_i[++sp] = other;
+                st_d[sp] = d1;
+                st_i[++sp] = closer;
+                st_d[sp] = d0;
+            } else if (h0) {
+                st_i[++sp] = L;
+                st_d[sp] = d0;
+            } else if (h1) {
+                st_i[++sp] = R;
+                st_d[sp] = d1;
+            }
+        }
+    }
+    return r_max;
+}
+
+// One root-to-leaf descent of BVH._sample_boundary (bvh.py:289-369; the point
+// BVH's twin): each inner node picks a child overlapping the ball with
+// probability proportional to |G(x, R, box centre)|; the leaf draws one of the
+// primitives in the ball (all of them when the chosen box lies inside it) by
+// reservoir sampling weighted by `weight`.  Returns the sorted index (-1: none).
+template <typename Acc, typename Wt>
+DWD int bvh2_descend(const DNode2* nodes, F2 x, float R, Rng& rng, bool& hit, float& pdf, Acc accept, Wt weight) {
+    hit = false;
+    pdf = 1.0f;
+    int sorted = -1;
+    float r_max = kInf;
+    int ni = 0;
+    while (true) {
+        const DNode2 nd = nodes[ni];
+        if (nd.nref > 0) {
+            float total = 0.0f;
+            for (int j = 0; j < nd.nref; ++j) {
+                int k = nd.off + j;
+                float sa = weight(k);
+                if ((r_max <= R) || accept(k)) {
+                    hit = true;
+                    total += sa;
+                    float sel = sa / total;
+                    float u = rng.next();
+                    if (u < sel) sorted = k;
+                }
+            }
+            if (total > 0.0f) pdf *= (sorted >= 0 ? weight(sorted) : 0.0f) / total;
+            return sorted;
+        }
+        int L = ni + 1, Rn = ni + nd.off;
+        const DNode2 n0 = nodes[L], n1 = nodes[Rn];
+        float d0, m0, d1, m1;
+        box_dist(node_mn(n0), node_mx(n0), x, d0, m0);
+        box_dist(node_mn(n1), node_mx(n1), x, d1, m1);
+        float w0 = (d0 <= R) ? 1.0f : 0.0f, w1 = (d1 <= R) ? 1.0f : 0.0f;
+        if (w0 > 0.0f) w0 *= __builtin_fabsf(greens2_G(x, R, x, (node_mn(n0) + node_mx(n0)) / 2.0f));
+        if (w1 > 0.0f) w1 *= __builtin_fabsf(greens2_G(x, R, x, (node_mn(n1) + node_mx(n1)) / 2.0f));
+        float tw = w0 + w1;
+        if (!(tw > 0.0f)) return sorted;
+        float p0 = w0 / tw, p1 = 1.0f - p0;
+        float u = rng.next();
+        if (u < p0) {
+            ni = L;
+            pdf *= p0;
+            r_max = m0;
+        } else {
+            ni = Rn;
+            pdf *= p1;
+            r_max = m1;
+        }
+    }
+}
+
+struct BSample2 {
+    bool hit;
+    F2 p, n;
+    float t, pdf;
+    int prim;
+};
+// Polyline.sample_boundary_bvh (polyline.py:515-517) -> BVH.sample_boundary (bvh.py:266-287)
+DWD BSample2 sample_boundary_bvh2(const SubDev& S, F2 x, float R, Rng& rng) {
+    BSample2 b{false, f2(0.0f, 0.0f), f2(0.0f, 0.0f), 0.0f, 0.0f, 0};
+    const DNode2* nodes = (const DNode2*)S.nodes;
+    const DSeg* prims = (const DSeg*)S.prims;
+    float dmin, dmax;
+    box_dist(node_mn(nodes[0]), node_mx(nodes[0]), x, dmin, dmax);
+    if (!(dmin <= R)) return b;
+    auto seg = [&](int k, F2& a, F2& c) {
+        if (k < 0) {
+            a = c = f2(0.0f, 0.0f);
+            return;
+        }
+        DSeg sg = prims[k];
+        a = f2(sg.ax, sg.ay), c = f2(sg.bx, sg.by);
+    };
+    auto weight = [&](int k) {  // LineSegment.surface_area (line_segment.py:32-33)
+        F2 a, c;
+        seg(k, a, c);
+        return norm(c - a);
+    };
+    auto accept = [&](int k) {  // LineSegment.sphere_intersect (line_segment.py:43-45)
+        F2 a, c;
+        seg(k, a, c);
+        F2 u = c - a, w = x - a;
+        float t = clampf(dot(w, u) / dot(u, u), 0.0f, 1.0f);
+        return norm(lerp(a, c, t) - x) <= R;
+    };
+    bool hit;
+    float pdf;
+    int k = bvh2_descend(nodes, x, R, rng, hit, pdf, accept, weight);
+    if (!hit) return b;
+    F2 a, c;
+    seg(k, a, c);
+    float t = rng.next();  // LineSegment.sample_point (line_segment.py:36-40)
+    F2 tg = c - a;
+    b.hit = true;
+    b.p = a + tg * t;
+    b.n = normalize(f2(tg.y, -tg.x));
+    b.t = t;
+    b.pdf = pdf * (1.0f / norm(tg));
+    b.prim = k >= 0 ? S.prim_index[k] : 0;
+    return b;
+}
+
+struct SSample2 {
+    bool hit;
+    F2 p, n1, n2, t1, t2;
+    float pdf;
+    int prim;
+};
+// Polyline.sample_silhouette_bvh (polyline.py:519-535) -> PointBVH.sample_boundary
+DWD SSample2 sample_silhouette_bvh2(const SubDev& S, F2 x, float R, Rng& rng) {
+    SSample2 r;
+    r.hit = false;
+    r.pdf = 0.0f;
+    r.prim = 0;
+    r.p = r.n1 = r.n2 = r.t1 = r.t2 = f2(0.0f, 0.0f);
+    if (S.n_psil == 0) return r;
+    const DNode2* nodes = (const DNode2*)S.pnodes;
+    float dmin, dmax;
+    box_dist(node_mn(nodes[0]), node_mx(nodes[0]), x, dmin, dmax);
+    if (!(dmin <= R)) return r;
+    auto weight = [](int) { return 1.0f; };
+    auto accept = [&](int k) {
+        const float* q = S.psil + 6 * S.porder[k];
+        return norm(f2(q[2], q[3]) - x) <= R;
+    };
+    bool hit;
+    float pdf;
+    int k = bvh2_descend(nodes, x, R, rng, hit, pdf, accept, weight);
+    if (!hit) return r;
+    int si = k >= 0 ? S.porder[k] : 0;
+    const float* q = S.psil + 6 * si;
+    F2 a = f2(q[0], q[1]), b = f2(q[2], q[3]), c = f2(q[4], q[5]);
+    r.hit = true;
+    r.t1 = normalize(b - a);
+    r.t2 = normalize(b - c);
+    r.n1 = f2(r.t1.y, -r.t1.x);
+    r.n2 = f2(-r.t2.y, r.t2.x);
+    r.p = b;
+    r.pdf = pdf * 1.0f;
+    r.prim = si;
+    return r;
+}
+
+// Polyline.off_centered_ball (polyline.py:583-600)
+DWD void off_centered_ball2(const SubDev& F, F2 p, F2 n, F2& c, float& radius) {
+    float b = fmn(F.bb_max[0] - F.bb_min[0], F.bb_max[1] - F.bb_min[1]) / 2.0f;
+    float a = 1e-3f;
+    for (int i = 0; i < 10; ++i) {
+        float m = (a + b) / 2.0f;
+        float d = __builtin_fabsf(closest2(F, p - n * m).d);
+        if (__builtin_fabsf(d - m) < 1e-3f)
+            a = m;
+        else
+            b = m;
+    }
+    c = p - n * a;
+    radius = a;
+}
+
+// WoStGrad.dudn (wost_grad.py:207-254): the normal derivative at a Dirichlet
+// point from one WoSt walk (nsteps 200, eps 1e-4) started on the off-centred ball
+DWD float grad_dudn2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, const CP& its, Rng& rng,
+                     int& steps) {
+    const SubDev& F = sc.sub[0];
+    const SubDev& Dn = sc.sub[1];
+    const SubDev& Nm = sc.sub[2];
+    F2 ip = f2(its.p.x, its.p.y), in = f2(its.n.x, its.n.y);
+    F2 c;
+    float R;
+    off_centered_ball2(F, ip, in, c, R);
+    float theta = (rng.next() * 2.0f) * kPi;  // uniform_angle
+    theta = clampf(theta, 3e-1f, kTwoPi - 3e-1f);
+    F2 pd = f2(-in.y, in.x);
+    F2 dir = in * cr_cos(theta) + pd * cr_sin(theta);
+    F2 p = c + dir * R;
+    float u_ref = bc_u(bc, its.p);
+    ParamsDev W = P;
+    W.nwalks = 1, W.nsubwalks = 1, W.nsteps = 200, W.eps = 1e-4f, W.min_R = 1e-3f, W.cv = 1, W.av = 1;
+    W.flags = DW_SOLVE_EXACT_CONES;
+    int wst, wtm;
+    float u = walk_u2(sc, bc, W, p, rng, wst, wtm);
+    steps += wst;
+    if (Nm.present) {
+        float dd = closest2(Dn, p).d;
+        float dnm = closest2(Nm, p).d;
+        if ((dd < 2e-2f) && (dnm < 2e-2f)) u = u_ref;
+    }
+    float Pk = 1.0f / (cr_cos(theta) - 1.0f);
+    float dudn = 0.0f;
+    if (P.cv)
+        dudn += Pk * (u - u_ref) / R;
+    else
+        dudn += Pk * u / R;
+    float rr = __builtin_sqrtf(rng.next());  // uniform_in_disk (math.py:285-288)
+    float th = kTwoPi * rng.next();
+    F2 p_src = c + f2(rr * cr_cos(th), rr * cr_sin(th)) * R;
+    float f_ref = src_f(bc), f = src_f(bc);
+    float x2 = dot(ip - c, ip - c);
+    float y2 = dot(p_src - c, p_src - c);
+    float Pq = (x2 - y2) / fmx(dot(ip - p_src, ip - p_src), 1e-6f) * R / 2.0f;
+    if (P.cv)
+        dudn -= (f - f_ref) * Pq + f_ref * R / 2.0f;
+    else
+        dudn -= f * Pq;
+    return dudn;
+}
+
+// WoStGrad.dude (wost_grad.py:91-205), from an interior point
+DWD float grad_dude2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, const GradDev& G, F2 x, F2 e,
+                     Rng& rng, int& steps) {
+    const SubDev& F = sc.sub[0];
+    const SubDev& Dn = sc.sub[1];
+    const SubDev& Nm = sc.sub[2];
+    F2 ip = x, in = f2(0.0f, 0.0f);
+    bool ob = false;
+    float result = 0.0f;
+    int i = 0;
+    bool active = ob || inside2(sc, ip);
+    float thr = 1.0f;
+    while ((i < P.nsteps) && active) {
+        bool inflate = false;
+        float d_dir = kInf;
+        if (Dn.present) d_dir = __builtin_fabsf(closest2(Dn, ip).d);
+        if (d_dir < P.eps) {
+            CP cr = closest2(F, ip);
+            F2 nrm = f2(cr.n.x, cr.n.y), tg = f2(-nrm.y, nrm.x);
+            float dudt = grad_dgdt2(bc, f2(cr.p.x, cr.p.y), tg);
+            float dudn = grad_dudn2(sc, bc, P, cr, rng, steps);
+            result += thr * dudt * dot(e, tg);
+            result += thr * dudn * dot(e, nrm);
+            active = false;
+        } else {
+            float R = d_dir;
+            if (Nm.present) R = fmn(R, star_radius2_rho(Nm, ip, e, G.rho_max));
+            if (ob) {
+                if (R < P.min_R) inflate = true;
+                R = fmx(R, P.min_R);
+            }
+            float alpha = ob ? 2.0f : 1.0f;
+            if (Nm.present) {
+                if (!G.ignore_boundary) {
+                    BSample2 b = sample_boundary_bvh2(Nm, ip, R, rng);
+                    if (b.hit) {  // handle_boundary (wost_grad.py:28-51)
+                        float Gv = clampf(greens2_G(ip, R, ip, b.p), -100.0f, 100.0f);
+                        float res = 0.0f;
+                        if (norm(ip - b.p) <= R) {
+                            F2 tg = f2(-b.n.y, b.n.x);
+                            float e_t = dot(e, tg), e_n = dot(e, b.n);
+                            float f_tn = grad_dhdt2(bc, b.p, b.n);
+                            res += e_t * f_tn - e_n * src_f(bc);
+                        }
+                        result += thr * alpha * (-Gv * res / b.pdf);
+                    }
+                }
+                if (!G.ignore_silhouette) {
+                    SSample2 sr = sample_silhouette_bvh2(Nm, ip, R, rng);
+                    if (!inflate && sr.hit) {  // handle_silhouette (wost_grad.py:53-72)
+                        float Gv = clampf(greens2_G(ip, R, ip, sr.p), -100.0f, 100.0f);
+                        float res = 0.0f;
+                        if (norm(sr.p - ip) <= R) {
+                            float h1 = grad_h2(bc, sr.p, sr.n1), h2 = grad_h2(bc, sr.p, sr.n2);
+                            res -= h1 * dot(e, sr.t1);
+                            res -= h2 * dot(e, sr.t2);
+                        }
+                        result += thr * alpha * (-Gv * res / sr.pdf);
+                    }
+                }
+            }
+            F2 dir = uniform_dir2(rng);
+            if (ob && (dot(dir, in) > 0.0f)) dir = -dir;
+            F2 _p = ip;
+            F2 o = ob ? ip - in * 1e-5f : ip;  // Polyline.intersect (polyline.py:435-466)
+            Hit its = intersect2(F, o, dir, R);
+            if (!its.valid) {
+                F2 q = o + dir * R;
+                its.p = f3(q.x, q.y, 0.0f);
+                its.n = f3(dir.x, dir.y, 0.0f);
+                its.d = R;
+                its.on_boundary = false;
+            }
+            ip = f2(its.p.x, its.p.y);
+            in = f2(its.n.x, its.n.y);
+            ob = its.on_boundary;
+            if (!G.ignore_source) {  // handle_source (wost_grad.py:74-89)
+                F2 sp;
+                float sr_, spdf;
+                greens2_sample(_p, R, dir, rng, sp, sr_, spdf);
+                float res = 0.0f;
+                if (norm(sp - _p) < __builtin_fabsf(its.d)) res += (R * R / 4.0f) * dot(f2(0.0f, 0.0f), e);
+                result += thr * res;
+            }
+            if (ob) {
+                F2 tg = f2(-in.y, in.x);
+                float e_t = dot(e, tg), e_n = dot(e, in);
+                float K = dot(dir, tg) / dot(dir, in);
+                result += thr * e_n * grad_h2(bc, ip, in);
+                thr *= e_t - K * e_n;
+                e = tg;
+            }
+            if (__builtin_fabsf(thr) < 1.0f) {  // Russian roulette
+                if (rng.next() > __builtin_fabsf(thr))
+                    active = false;
+                else
+                    thr /= __builtin_fabsf(thr);
+            }
+        }
+        i += 1;
+        if (bad(result) || __builtin_fabsf(result) > 1000.0f) result = 0.0f;
+    }
+    steps += i;
+    return result;
+}
+
+}  // namespace dwd

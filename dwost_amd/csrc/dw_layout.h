@@ -118,6 +118,12 @@ struct SubDev {
     const int* prim_index;  // BVH order -> local primitive index (2D)
     const void* sils;   // DSilV / DSilE (reference ref order)
     const int* ref_topo;  // silhouette ref -> topology silhouette id (2D: local vertex, 3D: edge)
+    // 2D: the polyline's own silhouette vertices (a, b, c) and the point BVH
+    // over their b (DNode2 records, leaves index `porder`), for WoStGrad
+    const float* psil;
+    const void* pnodes;
+    const int* porder;
+    int n_psil;
     // sampling (local primitive order)
     const float* cdf;
     const float* pmf;

@@ -12,6 +12,7 @@
 
 #include "../../include/dwost.h"
 #include "dw_layout.h"
+#include "dw_grad.h"
 #include "dw_walk.h"
 #include "scene_build.h"
 
@@ -500,6 +501,25 @@ __global__ void __launch_bounds__(kBlock) k_baseline(SceneDev sc, int sub, int k
     }
     prim[i] = best;
     if (dist) dist[i] = best >= 0 ? dmin : kInf;
+}
+
+// WoStGrad.solve_dude through Solver.solve_dude (solver.py:113-144) and
+// walk_dude (:146-166): lane i * nwalks + k runs nsubwalks dudes on the PCG32
+// stream of lane (point_base + i) * nwalks + k and writes their mean.
+__global__ void __launch_bounds__(kBlock) k_grad_dude(SceneDev sc, BCDev bc, ParamsDev P, GradDev G,
+                                                      const float* __restrict__ pts, const float* __restrict__ ev,
+                                                      int64_t n, float* walk_val, int* walk_steps) {
+    int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= n * P.nwalks) return;
+    int64_t i = lane / P.nwalks;
+    Rng rng;
+    seed_stream(rng, P, i, P.nwalks, lane - i * P.nwalks);
+    F2 x = load_pt<2>(pts, i), e = load_pt<2>(ev, i);
+    float v = 0.0f;
+    int st = 0;
+    for (int s = 0; s < P.nsubwalks; ++s) v += grad_dude2(sc, bc, P, G, x, e, rng, st);
+    walk_val[lane] = v / (float)P.nsubwalks;
+    if (walk_steps) walk_steps[lane] = st;
 }
 
 // Lane scheduling of the persistent walk kernels.  Each lane owns one job (one
@@ -1675,6 +1695,19 @@ int upload_sub(dw_scene* s, const dwb::SubHost& H, SubDev& S) {
     S.prim_index = (const int*)p;
     if ((rc = upload(s, H.refs, &p))) return rc;
     S.ref_topo = (const int*)p;
+    if (D == 2) {
+        S.n_psil = (int)(H.psil.size() / 6);
+        if ((rc = upload(s, H.psil, &p))) return rc;
+        S.psil = (const float*)p;
+        std::vector<DNode2> pn(H.pnode_off.size());
+        for (size_t i = 0; i < pn.size(); ++i) {
+            const float* b = &H.pnode_box[4 * i];
+            pn[i] = DNode2{b[0], b[1], b[2], b[3], H.pnode_off[i], H.pnode_nref[i], 0, 0};
+        }
+        if ((rc = upload(s, pn, &S.pnodes))) return rc;
+        if ((rc = upload(s, H.porder, &p))) return rc;
+        S.porder = (const int*)p;
+    }
     if ((rc = upload(s, H.cdf, &p))) return rc;
     S.cdf = (const float*)p;
     if ((rc = upload(s, H.pmf, &p))) return rc;
@@ -2332,6 +2365,45 @@ int dw_solve_u(const dw_scene* s, const float* pts, int64_t n, const dw_solver_p
     if (total_steps && (rc = zero_u64((unsigned long long*)total_steps, st))) return rc;
     hipLaunchKernelGGL(k_reduce_mean, dim3(blocks_for(n * 64)), dim3(kBlock), 0, st, wv, n, params->nwalks, 1,
                        1.0f / (float)params->nwalks, u, (float*)nullptr, total_steps ? ws : nullptr,
+                       (unsigned long long*)total_steps);
+    DW_HIP(hipGetLastError());
+    return DW_OK;
+}
+
+int dw_grad_solve_dude(const dw_scene* s, const float* pts, const float* e, int64_t n, const dw_solver_params* params,
+                       const dw_grad_params* grad, const dw_boundary* boundary, float* out, float* walk_out,
+                       int32_t* walk_steps, uint64_t* total_steps, void* stream) {
+    int rc;
+    if ((rc = check_scene(s, 0))) return rc;
+    if ((rc = check_params(params)) || (rc = check_bc(boundary))) return rc;
+    if (s->dev.dim != 2) return fail(DW_ERR_ARG, "dw_grad_solve_dude: WoStGrad runs on 2D scenes");
+    if (!grad) return fail(DW_ERR_ARG, "null grad params");
+    if (boundary->bc_code == DW_BC_PER_VERTEX)
+        return fail(DW_ERR_ARG, "dw_grad_solve_dude needs an analytic boundary field (tangential derivatives)");
+    if (params->flags & DW_SOLVE_WOS) return fail(DW_ERR_ARG, "DW_SOLVE_WOS does not apply to WoStGrad");
+    if (n < 0 || (n > 0 && (!pts || !e || !out))) return fail(DW_ERR_ARG, "bad arrays");
+    if (n == 0) return DW_OK;
+    hipStream_t st = (hipStream_t)stream;
+    StreamCtx* c;
+    if ((rc = stream_ctx(st, &c))) return rc;
+    const int64_t lanes = n * (int64_t)params->nwalks;
+    float* wv = walk_out;
+    int32_t* ws = walk_steps;
+    size_t need = (walk_out ? 0 : (size_t)lanes * 4) + ((total_steps && !walk_steps) ? (size_t)lanes * 4 : 0);
+    if (need) {
+        void* p;
+        if ((rc = workspace(*c, need, st, &p))) return rc;
+        char* cur = (char*)p;
+        if (!wv) wv = (float*)cur, cur += (size_t)lanes * 4;
+        if (!ws && total_steps) ws = (int32_t*)cur;
+    }
+    GradDev G{grad->rho_max, grad->ignore_boundary, grad->ignore_silhouette, grad->ignore_source};
+    hipLaunchKernelGGL(k_grad_dude, dim3(blocks_for(lanes)), dim3(kBlock), kStackLds, st, s->dev, to_bc(boundary),
+                       to_params(params), G, pts, e, n, wv, ws);
+    DW_HIP(hipGetLastError());
+    if (total_steps && (rc = zero_u64((unsigned long long*)total_steps, st))) return rc;
+    hipLaunchKernelGGL(k_reduce_mean, dim3(blocks_for(n * 64)), dim3(kBlock), 0, st, wv, n, params->nwalks, 1,
+                       1.0f / (float)params->nwalks, out, (float*)nullptr, total_steps ? ws : nullptr,
                        (unsigned long long*)total_steps);
     DW_HIP(hipGetLastError());
     return DW_OK;

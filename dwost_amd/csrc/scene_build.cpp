@@ -87,6 +87,30 @@ struct Bvh {
         node(-1, 0, n, 0);
     }
 
+    // point primitives (primitive.h:116-152): the epsilon box of each point
+    // widened by its radius 1e-4; centroid = the point
+    void run_points(const std::vector<float>& P) {
+        n = (int)(P.size() / D);
+        pbox.resize(n);
+        pcen.resize((size_t)n * D);
+        order.resize(n);
+        for (int p = 0; p < n; ++p) {
+            pbox[p].reset();
+            pbox[p].grow_pt(&P[(size_t)D * p]);
+            for (int d = 0; d < D; ++d) {
+                pbox[p].lo[d] = pbox[p].lo[d] - 1e-4f;
+                pbox[p].hi[d] = pbox[p].hi[d] + 1e-4f;
+                pcen[(size_t)D * p + d] = P[(size_t)D * p + d];
+            }
+            order[p] = p;
+        }
+        if (n == 0) return;
+        nbox.reserve(2 * n);
+        noff.reserve(2 * n);
+        nref.reserve(2 * n);
+        node(-1, 0, n, 0);
+    }
+
     void swap_prim(int i, int j) {
         std::swap(order[i], order[j]);
         std::swap(pbox[i], pbox[j]);
@@ -609,6 +633,44 @@ void finish_sub(SubHost& S) {
     if (D == 3) face_bounds3(S);
 }
 
+// The polyline's own silhouette vertices (Polyline.configure_silhouette,
+// polyline.py:100-165) -- distinct from the SNCH's topology silhouettes: over
+// the edges (a, b) in order prev[b] = a and next[a] = b (later edges win);
+// each vertex with both, in vertex order, gives (a = prev, b, c = next) unless
+// |normalize(b - a) . normalize(c - b)| >= 1 - 1e-5; then the point BVH over
+// their b (PointBVH, bvh.py:375-395, built by BVH<2, Point2>).  WoStGrad samples
+// silhouettes through it.
+void build_psil2(SubHost& S) {
+    int nv = (int)(S.V.size() / 2), ne = (int)(S.I.size() / 2);
+    std::vector<int> prv(nv, -1), nxt(nv, -1);
+    for (int e = 0; e < ne; ++e) prv[S.I[2 * e + 1]] = S.I[2 * e], nxt[S.I[2 * e]] = S.I[2 * e + 1];
+    auto unit = [](float x, float y, float& ux, float& uy) {  // normalize: v * (1 / sqrt(dot(v, v)))
+        float inv = 1.0f / std::sqrt(std::fmaf(y, y, x * x));
+        ux = x * inv, uy = y * inv;
+    };
+    std::vector<float> pts;
+    for (int v = 0; v < nv; ++v) {
+        if (prv[v] == -1 || nxt[v] == -1) continue;
+        const float *a = &S.V[2 * prv[v]], *b = &S.V[2 * v], *c = &S.V[2 * nxt[v]];
+        float t1x, t1y, t2x, t2y;
+        unit(b[0] - a[0], b[1] - a[1], t1x, t1y);
+        unit(c[0] - b[0], c[1] - b[1], t2x, t2y);
+        if (!(std::fabs(std::fmaf(t1y, t2y, t1x * t2x)) < (1.0f - 1e-5f))) continue;
+        S.psil.insert(S.psil.end(), {a[0], a[1], b[0], b[1], c[0], c[1]});
+        pts.push_back(b[0]);
+        pts.push_back(b[1]);
+    }
+    Bvh<2> tree;
+    tree.run_points(pts);
+    S.pnode_box.clear();
+    for (size_t i = 0; i < tree.nbox.size(); ++i)
+        S.pnode_box.insert(S.pnode_box.end(), {tree.nbox[i].lo[0], tree.nbox[i].lo[1], tree.nbox[i].hi[0],
+                                               tree.nbox[i].hi[1]});
+    S.pnode_off = tree.noff;
+    S.pnode_nref = tree.nref;
+    S.porder = tree.order;
+}
+
 // _build_subscene_from_mask (polyline.py:672-692, mesh.py:617-635)
 template <int D>
 void make_sub(const SceneHost& sc, int want, SubHost& S) {
@@ -637,6 +699,7 @@ void make_sub(const SceneHost& sc, int want, SubHost& S) {
     for (size_t j = 0; j < S.pglobal.size(); ++j)
         for (int k = 0; k < D; ++k) S.I[j * D + k] = remap[sc.I[(size_t)D * S.pglobal[j] + k]];
     finish_sub<D>(S);
+    if constexpr (D == 2) build_psil2(S);
 }
 
 }  // namespace

@@ -47,6 +47,11 @@ struct SubHost {
     // topology silhouettes (2D: one per local vertex; 3D: one per edge)
     std::vector<float> sil_pts;    // 2D: a,b,c (6 floats); 3D: a,b,c,d (12 floats)
     std::vector<int> sil_idx;      // 2D: prev,center,next; 3D: c,a,b,d
+    // 2D: the polyline's own silhouette vertices (a, b, c: 6 floats each) and the
+    // point BVH over their b: node boxes (min x, min y, max x, max y), offsets,
+    // counts, and BVH order -> silhouette index
+    std::vector<float> psil, pnode_box;
+    std::vector<int> pnode_off, pnode_nref, porder;
     size_t n_nodes() const { return node_off.size(); }
     size_t n_prims() const { return I.size() / dim; }
 };

@@ -245,3 +245,59 @@ class WoS3D(Solver):
     antithetic_variates: bool = True
     dim: int = 3
     _walk_flag: ClassVar[int] = _lib.SOLVE_WOS
+
+
+@dataclass
+class WoStGrad(Solver):
+    """2D WoStGrad, the robust derivative walk (diff_wost/solvers/wost_grad.py:17-254).
+
+    ``solve_dude(p, e, scene, seed)`` estimates du/de with walks that reflect off
+    Neumann boundaries (Solver.solve_dude, solver.py:113-166: one stream per
+    (point, walk), ``nsubwalks`` dudes per walk).  The reference defines no
+    ``u`` for it.  Boundary data must be an analytic field: the estimator uses
+    the tangential derivatives of g and h."""
+
+    ignore_neumann: bool = False  # reference field, unused by its walk
+    ignore_boundary: bool = False
+    ignore_silhouette: bool = False
+    ignore_source: bool = False
+    control_variates: bool = True
+    russian_roulette: bool = False  # reference field, unused by its walk (the roulette always runs)
+    min_R: float = 1e-2
+    rho_max: float = 1.2
+    dim: int = 2
+
+    def _grad_params(self) -> _lib.GradParams:
+        return _lib.GradParams(float(self.rho_max), int(self.ignore_boundary), int(self.ignore_silhouette),
+                               int(self.ignore_source))
+
+    def solve(self, *args, **kwargs):
+        raise NotImplementedError("WoStGrad estimates derivatives only (solve_dude)")
+
+    def solve_grad(self, *args, **kwargs):
+        raise NotImplementedError("WoStGrad estimates derivatives only (solve_dude)")
+
+    def solve_dude(self, p, e, scene: _Scene, seed: int = 0, per_walk: bool = False, count_steps: bool = False,
+                   point_base: int = 0):
+        """du/de at each point: mean of ``nwalks`` WoStGrad walks."""
+        self._check(scene)
+        p = _as_points(p, scene.dim)
+        e = _as_points(e, scene.dim)
+        n, dev = p.shape[0], p.device
+        out = torch.empty(n, device=dev)
+        wv = torch.empty(n * self.nwalks, device=dev) if per_walk else None
+        ws = torch.empty(n * self.nwalks, dtype=torch.int32, device=dev) if per_walk else None
+        steps = torch.zeros(1, dtype=torch.int64, device=dev) if count_steps else None
+        prm = self._params(seed, point_base)
+        gp = self._grad_params()
+        bc = scene.boundary.to_c(scene.n_vertices)
+        check(lib().dw_grad_solve_dude(scene.handle, ptr(p), ptr(e), n, C.byref(prm), C.byref(gp), C.byref(bc),
+                                       ptr(out), ptr(wv), ptr(ws), ptr(steps), stream_handle(dev)))
+        if not (per_walk or count_steps):
+            return out
+        r = dict(dude=out)
+        if per_walk:
+            r.update(walk_dude=wv, walk_steps=ws)
+        if count_steps:
+            r["steps"] = steps
+        return r

@@ -229,6 +229,28 @@ int dw_adjoint_u(const dw_scene* scene, const float* pts, int64_t n, const dw_so
 int dw_solve_dude(const dw_scene* scene, const float* pts, const float* e, int64_t n, const dw_solver_params* params,
                   const dw_boundary* boundary, float* out, uint64_t* total_steps, void* stream);
 
+/* WoStGrad fields beyond dw_solver_params (wost_grad.py:17-26): the bound on
+ * the reflected walk's throughput growth used by the star radius, and the
+ * switches that drop the boundary, silhouette and source terms. */
+typedef struct dw_grad_params {
+    float rho_max;             /* 1.2 in the reference                          */
+    int32_t ignore_boundary;
+    int32_t ignore_silhouette;
+    int32_t ignore_source;
+} dw_grad_params;
+
+/* WoStGrad.solve_dude (wost_grad.py:91-254 through Solver.solve_dude /
+ * walk_dude, solver.py:113-166), 2D scenes: du/de at pts along e[n][2], nwalks
+ * walks per point (one PCG32 stream each, keyed as Solver.solve), nsubwalks
+ * dudes per walk.  params: nwalks, nsteps, nsubwalks, eps, min_R (1e-2 in the
+ * reference), control_variates (of the nested normal-derivative walk), seed,
+ * point_base.  Boundary data must be an analytic field (its tangential
+ * derivatives enter the estimator).  out[n]; per-walk walk_out[n*nwalks] and
+ * walk_steps (outer steps + nested walk steps) optional. */
+int dw_grad_solve_dude(const dw_scene* scene, const float* pts, const float* e, int64_t n,
+                       const dw_solver_params* params, const dw_grad_params* grad, const dw_boundary* boundary,
+                       float* out, float* walk_out, int32_t* walk_steps, uint64_t* total_steps, void* stream);
+
 /* StreamlineTracer.next_position (examples/example_2d/trace.py:19-68,
  * examples/example_3d/trace.py:21-60) with a WoSt / WoSt3D solver
  * (Solver.walk_dude, solver.py:148-166, along each axis): out[n*dim] = the mean

@@ -58,7 +58,7 @@ def test_ctypes_structs_and_constants_match_header(tmp_path):
     from dwost_amd import _lib, shapes
 
     structs = {"dw_scene_desc": _lib.SceneDesc, "dw_boundary": _lib.Boundary, "dw_solver_params": _lib.SolverParams,
-               "dw_batch_item": _lib.BatchItem}
+               "dw_batch_item": _lib.BatchItem, "dw_grad_params": _lib.GradParams}
     consts = {"DW_OK": _lib.DW_OK, "DW_SCENE_HOST_ONLY": _lib.SCENE_HOST_ONLY,
               "DW_SOLVE_EXACT_CONES": _lib.SOLVE_EXACT_CONES, "DW_SOLVE_WOS": _lib.SOLVE_WOS,
               "DW_BATCH_GRAPH": _lib.BATCH_GRAPH, "DW_SUB_FULL": _lib.SUB_FULL, "DW_SUB_DIRICHLET": _lib.SUB_DIRICHLET,

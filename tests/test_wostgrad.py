@@ -1,0 +1,87 @@
+"""WoStGrad (diff_wost/solvers/wost_grad.py) on the GPU against the oracle, walk
+for walk: the per-walk du/de bits and step counts must be the oracle's on a
+mixed star polygon (e^x cos y), an all-Neumann one with u = x (the field of the
+reference's streamline example, examples/example_2d/scene.py), the holes scene
+(x^2 - y^2), with two subwalks, with the ignore switches and a small min_R;
+the mean is the Kahan-reduced walk mean."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import dwost_amd
+
+    dwost_amd._lib.lib()
+    return dwost_amd
+
+
+def _dirs(n, seed):
+    e = np.random.default_rng(seed).standard_normal((n, 2))
+    return (e / np.linalg.norm(e, axis=1, keepdims=True)).astype(np.float32)
+
+
+def _case(name):
+    from dwost_amd import scenes
+
+    if name == "holes":
+        spec = scenes.holes_scene_2d(n_outer=1024, n_holes=4, n_query=64, eps=1e-3)
+        return spec, spec.types, spec.bc_code, (0, 0, 0, 0)
+    spec = scenes.star_scene_2d(n_query=64)
+    if name == "allN_linear":
+        return spec, np.zeros_like(spec.types), 2, (0, 1, 0, 0)
+    if name == "allD_expcos":
+        return spec, np.ones_like(spec.types), 0, (0, 0, 0, 0)
+    return spec, spec.types, 0, (0, 0, 0, 0)
+
+
+@pytest.mark.parametrize("name,W,kw", [
+    ("mixed", 64, {}),
+    ("allN_linear", 64, {}),
+    ("holes", 32, {}),
+    ("allD_expcos", 32, {}),
+    ("mixed", 16, dict(nsubwalks=2)),
+    ("mixed", 32, dict(min_R=1e-4)),
+    ("allN_linear", 32, dict(ignore_silhouette=True, ignore_boundary=True)),
+    ("mixed", 32, dict(ignore_source=True, control_variates=False, rho_max=2.0)),
+])
+def test_wostgrad_matches_oracle(gpu, oracle, name, W, kw):
+    spec, types, code, coeffs = _case(name)
+    bc = gpu.BoundaryCondition(code, coeffs)
+    sc = gpu.Scene2D(spec.vertices, spec.indices, types, bc)
+    os_ = oracle.OracleScene(spec.vertices, spec.indices, types)
+    q = np.ascontiguousarray(spec.queries[:32])
+    e = _dirs(len(q), 4)
+    sol = gpu.WoStGrad(eps=1e-3, nwalks=W, nsteps=1024, **kw)
+    g = sol.solve_dude(torch.as_tensor(q).cuda(), torch.as_tensor(e).cuda(), sc, seed=9, per_walk=True,
+                       count_steps=True)
+    torch.cuda.synchronize()
+    okw = dict(nsteps=1024, eps=1e-3, min_R=sol.min_R, cv=sol.control_variates, seed=9, nsubwalks=sol.nsubwalks,
+               rho_max=sol.rho_max, ignore_boundary=sol.ignore_boundary, ignore_silhouette=sol.ignore_silhouette,
+               ignore_source=sol.ignore_source, coeffs=coeffs)
+    o = os_.grad_dude(q, e, code, W, per_walk=True, **okw)
+    gw, gs = g["walk_dude"].cpu().numpy(), g["walk_steps"].cpu().numpy()
+    same = (gw.view(np.uint32) == o["walk_dude"].view(np.uint32)) & (gs == o["walk_steps"])
+    bad = np.flatnonzero(~same)
+    print(f"{name} {kw}: {same.sum()} of {same.size} walks identical; mean steps {gs.mean():.1f}")
+    assert bad.size == 0, bad[:20].tolist()
+    assert int(g["steps"][0]) == int(gs.astype(np.int64).sum())
+    assert np.allclose(g["dude"].cpu().numpy(), o["dude"], rtol=1e-5, atol=1e-5)
+
+
+def test_wostgrad_rejects_per_vertex_data_and_3d(gpu, ico_spec, star_spec):
+    V = len(star_spec.vertices)
+    g = torch.zeros(V, device="cuda")
+    sc = gpu.Scene2D(star_spec.vertices, star_spec.indices, star_spec.types, gpu.BoundaryCondition.per_vertex(g, g))
+    q = torch.as_tensor(star_spec.queries[:4]).cuda()
+    with pytest.raises(gpu.DwostError):
+        gpu.WoStGrad(nwalks=4).solve_dude(q, q, sc)
+    with pytest.raises(NotImplementedError):
+        gpu.WoStGrad(nwalks=4).solve(q, sc)

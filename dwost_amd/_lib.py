@@ -90,6 +90,8 @@ def lib():
     L.dw_solve_dude.argtypes = [vp, vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp]
     L.dw_grad_solve_dude.argtypes = [vp, vp, vp, i64, C.POINTER(SolverParams), C.POINTER(GradParams),
                                      C.POINTER(Boundary), vp, vp, vp, vp, vp]
+    L.dw_grad_trace_step.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(GradParams), C.POINTER(Boundary),
+                                     C.c_int32, C.c_float, vp, vp, vp]
     L.dw_trace_step.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), C.c_int32, C.c_float, vp,
                                 vp, vp]
     L.dw_solve_fused.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp, vp, vp, vp,
@@ -104,7 +106,7 @@ def lib():
                                    vp]
     for name in ("dw_scene_create", "dw_scene_destroy", "dw_scene_info", "dw_scene_export", "dw_closest_point",
                  "dw_intersect", "dw_closest_silhouette", "dw_star_radius", "dw_sdf", "dw_sample_boundary",
-                 "dw_query_baseline", "dw_scene_prim_map", "dw_solve_u", "dw_solve_dude", "dw_grad_solve_dude",
+                 "dw_query_baseline", "dw_scene_prim_map", "dw_solve_u", "dw_solve_dude", "dw_grad_solve_dude", "dw_grad_trace_step",
                  "dw_trace_step",
                  "dw_solve_fused", "dw_solve_fused_noalloc", "dw_adjoint_u", "dw_adjoint_fused", "dw_batch_create", "dw_batch_solve"):
         getattr(L, name).restype = C.c_int

@@ -522,6 +522,40 @@ __global__ void __launch_bounds__(kBlock) k_grad_dude(SceneDev sc, BCDev bc, Par
     if (walk_steps) walk_steps[lane] = st;
 }
 
+// StreamlineTracer.next_position (examples/example_2d/trace.py:19-68) with a
+// WoStGrad solver: sample j of point i on the stream of lane i * tn + j,
+// walk_dude = the mean of nsubwalks WoStGrad dudes; the 2D step rule.
+__global__ void __launch_bounds__(kBlock) k_grad_trace(SceneDev sc, BCDev bc, ParamsDev P, GradDev G,
+                                                       const float* __restrict__ pts, int64_t n, int tn, float step,
+                                                       float* samples, unsigned long long* step_total) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n * tn) return;
+    int64_t i = j / tn;
+    Rng rng;
+    seed_stream(rng, P, i, tn, j - i * tn);
+    int st = 0;
+    F2 p = load_pt<2>(pts, i);
+    auto walk_dude = [&](F2 e) -> float {
+        float r = 0.0f;
+        for (int k = 0; k < P.nsubwalks; ++k) r += grad_dude2(sc, bc, P, G, p, e, rng, st);
+        return r / (float)P.nsubwalks;
+    };
+    F2 cur = p;
+    if (!(sdf2(sc, p) > 0.0f)) {
+        float gx = walk_dude(f2(1.0f, 0.0f));
+        float gy = walk_dude(f2(0.0f, 1.0f));
+        cur = p + f2(gx, gy) * step;
+        F2 yt = cur;
+        if (!(sdf2(sc, cur) > 0.0f)) {
+            F2 fy = yt - p, fyt = cur - yt;
+            cur = p + (fy + fyt) * 0.5f;
+        }
+    }
+    samples[2 * j] = cur.x;
+    samples[2 * j + 1] = cur.y;
+    if (step_total) atomicAdd(step_total, (unsigned long long)st);
+}
+
 // Lane scheduling of the persistent walk kernels.  Each lane owns one job (one
 // or more walks, see the kernels) and runs its walk as the step machine of
 // walk_advance.  Two loops:
@@ -2405,6 +2439,38 @@ int dw_grad_solve_dude(const dw_scene* s, const float* pts, const float* e, int6
     hipLaunchKernelGGL(k_reduce_mean, dim3(blocks_for(n * 64)), dim3(kBlock), 0, st, wv, n, params->nwalks, 1,
                        1.0f / (float)params->nwalks, out, (float*)nullptr, total_steps ? ws : nullptr,
                        (unsigned long long*)total_steps);
+    DW_HIP(hipGetLastError());
+    return DW_OK;
+}
+
+int dw_grad_trace_step(const dw_scene* s, const float* pts, int64_t n, const dw_solver_params* params,
+                       const dw_grad_params* grad, const dw_boundary* boundary, int32_t samples_per_point,
+                       float step_size, float* out, uint64_t* total_steps, void* stream) {
+    int rc;
+    if ((rc = check_scene(s, 0))) return rc;
+    if ((rc = check_params(params)) || (rc = check_bc(boundary))) return rc;
+    if (s->dev.dim != 2) return fail(DW_ERR_ARG, "dw_grad_trace_step: WoStGrad runs on 2D scenes");
+    if (!grad) return fail(DW_ERR_ARG, "null grad params");
+    if (boundary->bc_code == DW_BC_PER_VERTEX)
+        return fail(DW_ERR_ARG, "dw_grad_trace_step needs an analytic boundary field (tangential derivatives)");
+    if (n < 0 || (n > 0 && (!pts || !out))) return fail(DW_ERR_ARG, "bad arrays");
+    if (samples_per_point < 1) return fail(DW_ERR_ARG, "samples_per_point must be >= 1");
+    if (n == 0) return DW_OK;
+    const int64_t ns = n * (int64_t)samples_per_point;
+    hipStream_t st = (hipStream_t)stream;
+    StreamCtx* c;
+    if ((rc = stream_ctx(st, &c))) return rc;
+    void* p;
+    if ((rc = workspace(*c, (size_t)ns * 8, st, &p))) return rc;
+    float* samples = (float*)p;
+    if (total_steps && (rc = zero_u64((unsigned long long*)total_steps, st))) return rc;
+    GradDev G{grad->rho_max, grad->ignore_boundary, grad->ignore_silhouette, grad->ignore_source};
+    hipLaunchKernelGGL(k_grad_trace, dim3(blocks_for(ns)), dim3(kBlock), kStackLds, st, s->dev, to_bc(boundary),
+                       to_params(params), G, pts, n, (int)samples_per_point, step_size, samples,
+                       (unsigned long long*)total_steps);
+    DW_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_trace_mean, dim3(blocks_for(n)), dim3(kBlock), 0, st, samples, n, (int)samples_per_point, 2,
+                       out);
     DW_HIP(hipGetLastError());
     return DW_OK;
 }

@@ -3,10 +3,10 @@
 ``StreamlineTracer`` follows ``examples/example_2d/trace.py:9-68`` and
 ``examples/example_3d/trace.py:9-60``: each call of ``next_position`` moves every
 point one step along the estimated gradient, averaging ``nwalks`` independent
-samples per point.  Here the solver is a ``WoSt`` / ``WoSt3D`` (its
-``walk_dude``, ``solver.py:148-166``, along each axis gives the gradient); the
-whole step (samples, walks, sdf tests, averaging) is one GPU launch
-(``dw_trace_step``).  2D keeps the reference's step rule: only points inside
+samples per point.  The solver is the reference example's ``WoStGrad`` (2D,
+``dw_grad_trace_step``) or a ``WoSt`` / ``WoSt3D`` (``dw_trace_step``); its
+``walk_dude`` (``solver.py:146-166``) along each axis gives the gradient, and
+the whole step (samples, walks, sdf tests, averaging) is one GPU launch.  2D keeps the reference's step rule: only points inside
 move, and a step whose end is inside too keeps half its length (the corrector
 term ``fy_tilde = current_p - y_tilde`` is zero as written).  3D takes the full
 Euler step.
@@ -21,7 +21,7 @@ import torch
 
 from ._lib import check, lib, ptr, stream_handle
 from .shapes import _as_points, _Scene
-from .solvers import WoS, WoS3D, Solver
+from .solvers import WoS, WoS3D, Solver, WoStGrad
 
 
 @dataclass
@@ -45,9 +45,24 @@ class StreamlineTracer:
         steps = torch.zeros(1, dtype=torch.int64, device=dev) if count_steps else None
         prm = self.solver._params(seed)
         bc = self.scene.boundary.to_c(self.scene.n_vertices)
-        check(lib().dw_trace_step(self.scene.handle, ptr(p), n, C.byref(prm), C.byref(bc), int(self.nwalks),
-                                  float(self.step_size), ptr(out), ptr(steps), stream_handle(dev)))
+        if isinstance(self.solver, WoStGrad):
+            gp = self.solver._grad_params()
+            check(lib().dw_grad_trace_step(self.scene.handle, ptr(p), n, C.byref(prm), C.byref(gp), C.byref(bc),
+                                           int(self.nwalks), float(self.step_size), ptr(out), ptr(steps),
+                                           stream_handle(dev)))
+        else:
+            check(lib().dw_trace_step(self.scene.handle, ptr(p), n, C.byref(prm), C.byref(bc), int(self.nwalks),
+                                      float(self.step_size), ptr(out), ptr(steps), stream_handle(dev)))
         return (out, steps) if count_steps else out
+
+    def trace_graph(self, p, niter: int) -> "TraceGraph":
+        """The reference example's loop (``niter`` steps, seeds 0 .. niter-1,
+        examples/example_2d/run.py:85-127) captured once into a HIP graph:
+        ``replay()`` reruns all ``niter`` launches with one graph launch and
+        returns the positions after every step, the bits of ``trace``.  The
+        start positions are read from ``p`` (a CUDA tensor; update it in place
+        between replays)."""
+        return TraceGraph(self, p, niter)
 
     def trace(self, p, niter: int) -> list[torch.Tensor]:
         """``niter`` steps with seeds 0 .. niter-1 (the reference example's loop);
@@ -57,3 +72,29 @@ class StreamlineTracer:
             p = self.next_position(p, i)
             path.append(p)
         return path
+
+
+class TraceGraph:
+    """A streamline trace of fixed point count and length captured into a HIP
+    graph (torch.cuda.CUDAGraph).  One eager trace on the capture stream first
+    sizes the library's per-stream workspace, so the captured launches
+    allocate nothing; each step's seed is its iteration index, fixed at capture."""
+
+    def __init__(self, tracer: StreamlineTracer, p, niter: int):
+        if not isinstance(p, torch.Tensor) or not p.is_cuda or p.dtype != torch.float32 or not p.is_contiguous():
+            raise ValueError("start points must be a contiguous float32 CUDA tensor (read by address on replay)")
+        self.tracer, self.start, self.niter = tracer, p, int(niter)
+        self.stream = torch.cuda.Stream()
+        self.stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(self.stream):
+            tracer.trace(self.start, self.niter)  # warm-up: workspace and queue counters of this stream
+        self.stream.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph, stream=self.stream):
+            self.path = tracer.trace(self.start, self.niter)
+
+    def replay(self) -> list[torch.Tensor]:
+        """Runs the captured trace; returns the positions after every step
+        (tensors owned by the graph, overwritten by the next replay)."""
+        self.graph.replay()
+        return self.path

@@ -251,6 +251,16 @@ int dw_grad_solve_dude(const dw_scene* scene, const float* pts, const float* e, 
                        const dw_solver_params* params, const dw_grad_params* grad, const dw_boundary* boundary,
                        float* out, float* walk_out, int32_t* walk_steps, uint64_t* total_steps, void* stream);
 
+/* StreamlineTracer.next_position (examples/example_2d/trace.py:19-68) with a
+ * WoStGrad solver, as the reference example drives it (examples/example_2d/
+ * run.py:85-127): sample j of point i runs on the stream of lane
+ * i * samples_per_point + j seeded with params->seed; walk_dude = the mean of
+ * nsubwalks WoStGrad dudes along each axis; out[n][2] = the mean of the samples'
+ * new positions (the 2D step rule).  2D scenes, analytic boundary data. */
+int dw_grad_trace_step(const dw_scene* scene, const float* pts, int64_t n, const dw_solver_params* params,
+                       const dw_grad_params* grad, const dw_boundary* boundary, int32_t samples_per_point,
+                       float step_size, float* out, uint64_t* total_steps, void* stream);
+
 /* StreamlineTracer.next_position (examples/example_2d/trace.py:19-68,
  * examples/example_3d/trace.py:21-60) with a WoSt / WoSt3D solver
  * (Solver.walk_dude, solver.py:148-166, along each axis): out[n*dim] = the mean

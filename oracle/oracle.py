@@ -58,6 +58,9 @@ def lib():
         L.orc_cv_bound_check.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
         L.orc_far_check.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
         L.orc_sdf.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+        L.orc_grad_trace_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int,
+                                          C.c_void_p, C.c_int, C.c_float, C.c_void_p, C.c_void_p, C.c_int32,
+                                          C.c_float] + [C.c_void_p] * 3
         L.orc_star_radius2_rho.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_float,
                                            C.c_void_p, C.c_void_p]
         L.orc_grad_dude.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int,
@@ -266,6 +269,22 @@ class OracleScene:
         if per_walk:
             r.update(walk_dude=wv, walk_steps=ws)
         return r
+
+    def grad_trace_step(self, pts, bc_code, samples_per_point, step_size, nsubwalks=1, nsteps=128, eps=1e-3,
+                        min_R=1e-2, cv=True, seed=0, rho_max=1.2, coeffs=None, src_code=0, src_c=0.0, base=0):
+        """StreamlineTracer.next_position with a WoStGrad solver (2D)."""
+        pts = _f32(pts)
+        n = len(pts)
+        ip, fp = self._params(1, nsteps, nsubwalks, cv, True, seed, eps, min_R, base)
+        c = _f32(np.asarray(coeffs if coeffs is not None else (0, 0, 0, 0)))
+        gp = np.array([rho_max], np.float32)
+        gi = np.zeros(4, np.int32)
+        out = np.zeros((n, 2), np.float32)
+        smp = np.zeros((n * samples_per_point, 2), np.float32)
+        st = np.zeros(1, np.uint64)
+        lib().orc_grad_trace_step(self.h, _p(pts), n, _p(ip), _p(fp), bc_code, _p(c), src_code, src_c, _p(gp),
+                                  _p(gi), int(samples_per_point), float(step_size), _p(out), _p(smp), _p(st))
+        return dict(pos=out, samples=smp, steps=int(st[0]))
 
     def trace_step(self, pts, bc_code, nwalks, samples_per_point, step_size, nsubwalks=1, nsteps=128, eps=1e-3,
                    min_R=1e-3, cv=True, av=True, seed=0, coeffs=None, gv=None, hv=None, src_code=0, src_c=0.0, base=0):

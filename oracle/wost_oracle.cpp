@@ -3288,6 +3288,54 @@ void orc_grad_dude(void* s, const float* pts, const float* ev, int64_t n, const 
     if (steps_total) *steps_total = steps;
 }
 
+// StreamlineTracer.next_position (examples/example_2d/trace.py:19-68) with a
+// WoStGrad solver, as in the reference example (run.py): sample j of point i
+// runs on PCG32 lane (base + i) * tn + j; walk_dude = the mean of nsubwalks
+// WoStGrad dudes.  out[n*2] = mean of the tn samples; samples[n*tn*2] optional.
+void orc_grad_trace_step(void* s, const float* pts, int64_t n, const int32_t* ip, const float* fp, int bc_code,
+                         const float* bc_c, int src_code, float src_c, const float* gp, const int32_t* gi, int32_t tn,
+                         float step, float* out, float* samples, uint64_t* steps_total) {
+    Scene* sc = (Scene*)s;
+    Params P = make_params(ip, fp);
+    BC bc = make_bc(bc_code, bc_c, nullptr, nullptr, src_code, src_c);
+    GradParams G{gp[0], gi[0], gi[1], gi[2], gi[3]};
+    std::vector<float> smp((size_t)n * tn * 2);
+    uint64_t steps = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : steps)
+    for (int64_t j = 0; j < n * (int64_t)tn; ++j) {
+        int64_t i = j / tn;
+        Pcg rng = pcg_for(P.seed, (uint32_t)(P.base + i), (uint64_t)((P.base + i) * tn + (j - i * tn)));
+        int st = 0;
+        V2 p = v2(pts[2 * i], pts[2 * i + 1]);
+        auto walk_dude = [&](V2 e) {
+            float r = 0.0f;
+            for (int k = 0; k < P.nsubwalks; ++k) r += grad_dude2(*sc, bc, P, G, p, e, rng, st);
+            return r / (float)P.nsubwalks;
+        };
+        V2 y = p, cur = p;
+        if (!(sdf2(*sc, cur) > 0.0f)) {
+            float gx = walk_dude(v2(1.0f, 0.0f));
+            float gy = walk_dude(v2(0.0f, 1.0f));
+            cur = add(cur, mul(v2(gx, gy), step));
+            V2 yt = cur;
+            if (!(sdf2(*sc, cur) > 0.0f)) {
+                V2 fy = sub(yt, y), fyt = sub(cur, yt);
+                cur = add(y, mul(add(fy, fyt), 0.5f));
+            }
+        }
+        smp[(size_t)j * 2] = cur.x, smp[(size_t)j * 2 + 1] = cur.y;
+        steps += (uint64_t)st;
+    }
+    for (int64_t i = 0; i < n; ++i)
+        for (int d = 0; d < 2; ++d) {
+            float acc = 0.0f;
+            for (int k = 0; k < tn; ++k) acc += smp[((size_t)i * tn + k) * 2 + d];
+            out[i * 2 + d] = tn > 1 ? acc / (float)tn : acc;
+        }
+    if (samples) std::memcpy(samples, smp.data(), smp.size() * sizeof(float));
+    if (steps_total) *steps_total = steps;
+}
+
 // The rho-bounded star radius of a 2D sub-scene two ways: through the SNCH
 // (Polyline.star_radius_2, polyline.py:317-330) and by the reference's loop over
 // every segment (star_radius_2_3, polyline.py:332-375).  tree[n], brute[n].

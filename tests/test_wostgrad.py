@@ -85,3 +85,51 @@ def test_wostgrad_rejects_per_vertex_data_and_3d(gpu, ico_spec, star_spec):
         gpu.WoStGrad(nwalks=4).solve_dude(q, q, sc)
     with pytest.raises(NotImplementedError):
         gpu.WoStGrad(nwalks=4).solve(q, sc)
+
+
+def test_tracer_with_wostgrad_matches_oracle(gpu, oracle):
+    """StreamlineTracer driven by WoStGrad, as in the reference example
+    (all-Neumann scene, u = x): two chained steps bit for bit against the oracle."""
+    spec, types, code, coeffs = _case("allN_linear")
+    sc = gpu.Scene2D(spec.vertices, spec.indices, types, gpu.BoundaryCondition(code, coeffs))
+    os_ = oracle.OracleScene(spec.vertices, spec.indices, types)
+    sol = gpu.WoStGrad(eps=1e-3, nwalks=1, nsteps=1024)
+    tr = gpu.StreamlineTracer(sc, sol, step_size=0.02, nwalks=8)
+    q = np.ascontiguousarray(spec.queries[:24])
+    p = torch.as_tensor(q).cuda()
+    o = q
+    for it in range(2):
+        p, st = tr.next_position(p, seed=it, count_steps=True)
+        r = os_.grad_trace_step(o, code, 8, 0.02, nsteps=1024, eps=1e-3, min_R=sol.min_R, seed=it, coeffs=coeffs)
+        g = p.cpu().numpy()
+        assert np.array_equal(g.view(np.uint32), r["pos"].view(np.uint32)), it
+        assert int(st[0]) == r["steps"]
+        o = g
+    # u = x: the streamlines move along +x on average
+    assert np.mean(g[:, 0] - q[:, 0]) > 0
+
+
+@pytest.mark.parametrize("solver", ["wostgrad", "wost"])
+def test_trace_graph_replays_match_eager(gpu, solver):
+    """The whole trace loop captured into one HIP graph: every replay (with eager
+    work on the default stream between replays, and new start points written in
+    place) returns the eager trace's bits at every step."""
+    spec, types, code, coeffs = _case("allN_linear" if solver == "wostgrad" else "mixed")
+    sc = gpu.Scene2D(spec.vertices, spec.indices, types, gpu.BoundaryCondition(code, coeffs))
+    sol = gpu.WoStGrad(eps=1e-3, nwalks=1, nsteps=1024) if solver == "wostgrad" else \
+        gpu.WoSt(eps=1e-3, nwalks=8, nsteps=1024)
+    tr = gpu.StreamlineTracer(sc, sol, step_size=0.02, nwalks=4)
+    q = torch.as_tensor(np.ascontiguousarray(spec.queries[:40])).cuda()
+    start = q.clone()
+    tg = tr.trace_graph(start, 3)
+    for rep in range(3):
+        if rep == 2:
+            start.copy_(q.flip(0))
+        ref = tr.trace(start.clone(), 3)
+        torch.cuda.synchronize()
+        out = tg.replay()
+        torch.cuda.synchronize()
+        for k in range(3):
+            assert torch.equal(out[k].view(torch.int32), ref[k].view(torch.int32)), (rep, k)
+        sol.solve_dude(q[:8], q[:8], sc, seed=rep) if solver == "wostgrad" else sol.solve(q[:8], sc, seed=rep)
+        torch.cuda.synchronize()

@@ -11,12 +11,12 @@ from .shapes import (BC_EXPCOS, BC_LINEAR, BC_PER_VERTEX, BC_QUAD, BoundaryCondi
                      Scene3D)
 from ._lib import SUB_DIRICHLET, SUB_FULL, SUB_NEUMANN, DwostError
 from . import ops  # registers torch.ops.dwost.*
-from .solvers import CustomSolver, Solver, WoS, WoS3D, WoSt, WoSt3D, WoStGrad
+from .solvers import CustomSolver, Solver, WoS, WoS3D, WoSt, WoSt3D, WoStGrad, WoStGrad3D
 from .batch import BatchItem, BatchSolver
 from .trace import StreamlineTracer
 
 __all__ = [
     "BC_EXPCOS", "BC_LINEAR", "BC_PER_VERTEX", "BC_QUAD", "BoundaryCondition", "BoundaryType", "Scene2D", "Scene3D",
-    "SUB_DIRICHLET", "SUB_FULL", "SUB_NEUMANN", "DwostError", "CustomSolver", "Solver", "WoS", "WoS3D", "WoSt", "WoSt3D", "WoStGrad", "BatchItem", "BatchSolver",
+    "SUB_DIRICHLET", "SUB_FULL", "SUB_NEUMANN", "DwostError", "CustomSolver", "Solver", "WoS", "WoS3D", "WoSt", "WoSt3D", "WoStGrad", "WoStGrad3D", "BatchItem", "BatchSolver",
     "StreamlineTracer",
 ]

@@ -1,5 +1,6 @@
-// dw_grad.h -- WoStGrad (diff_wost/solvers/wost_grad.py), the robust
-// derivative walk, 2D: du/de from a walk that reflects off Neumann boundaries,
+// dw_grad.h -- WoStGrad / WoStGrad3D (diff_wost/solvers/wost_grad.py,
+// wost_grad_3d.py), the robust derivative walk: du/de from a walk that reflects
+// off Neumann boundaries,
 // with the rho-bounded star radius, BVH sampling of the Neumann boundary and of
 // its silhouette vertices, and a nested WoSt walk for the normal derivative at
 // a Dirichlet point.  The queries walk the sub-scene's node-major tree
@@ -486,6 +487,336 @@ DWD float grad_dude2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, co
                 else
                     thr /= __builtin_fabsf(thr);
             }
+        }
+        i += 1;
+        if (bad(result) || __builtin_fabsf(result) > 1000.0f) result = 0.0f;
+    }
+    steps += i;
+    return result;
+}
+
+// ---------------------------------------------------------------------------
+// WoStGrad3D (diff_wost/solvers/wost_grad_3d.py).  The reference calls a dudn
+// it never defines once a walk reaches a Dirichlet point, so the estimator
+// runs on all-Neumann scenes.
+// ---------------------------------------------------------------------------
+// Hessian rows of the analytic field
+DWD void bc_hess3(const BCDev& bc, F3 p, F3 H[3]) {
+    if (bc.code == DW_BC_EXPCOS) {
+        float ex = cr_exp(p.x), c = cr_cos(p.y), sn = cr_sin(p.y);
+        H[0] = f3(ex * c, -(ex * sn), 0.0f), H[1] = f3(-(ex * sn), -(ex * c), 0.0f), H[2] = f3(0.0f, 0.0f, 0.0f);
+    } else if (bc.code == DW_BC_QUAD) {
+        H[0] = f3(2.0f, 0.0f, 0.0f), H[1] = f3(0.0f, -2.0f, 0.0f), H[2] = f3(0.0f, 0.0f, 0.0f);
+    } else {
+        H[0] = H[1] = H[2] = f3(0.0f, 0.0f, 0.0f);
+    }
+}
+
+// DiscreteDistribution.sample_reuse_pmf (distr.py:52-107) over the silhouette
+// lengths, searched inside the guide table's bracket (as cdf_sample)
+DWD int psil3_sample(const SubDev& S, float value, float& reuse, float& pmf) {
+    float sample = value * S.ps_sum;
+    int k = (int)(value * (float)(1 << S.ps_shift));
+    uint32_t start = (uint32_t)S.ps_guide[k], end = (uint32_t)S.ps_guide[k + 1];
+    int iters = start < end ? (32 - __builtin_clz(end - start)) : 0;
+    for (int it = 0; it < iters; ++it) {
+        uint32_t mid = (start + end) >> 1;
+        float cv = S.ps_cdf[mid];
+        bool cond = ((cv < sample) || (cv == 0.0f)) && (cv != S.ps_sum);
+        if (cond)
+            start = (mid + 1 < end) ? mid + 1 : end;
+        else
+            end = mid;
+    }
+    int idx = (int)start;
+    pmf = S.ps_pmf[idx] * S.ps_norm;
+    float cdf = idx > 0 ? S.ps_cdf[idx - 1] * S.ps_norm : 0.0f;
+    reuse = (value - cdf) / pmf;
+    return idx;
+}
+
+DWD F3 node3_mn(const DNode3& n) { return f3(n.mnx, n.mny, n.mnz); }
+DWD F3 node3_mx(const DNode3& n) { return f3(n.mxx, n.mxy, n.mxz); }
+
+// SNCHNode3D.visit (snch3d.py:196-271)
+DWD bool snch_visit3(const DNode3& nd, const DCone3& cn, F3 x, F3 v, float rho_max, float r_max, float& d_min) {
+    float d_max;
+    F3 mn = node3_mn(nd), mx = node3_mx(nd);
+    box_dist(mn, mx, x, d_min, d_max);
+    bool ov = d_min <= r_max;
+    float ha = cn.half_angle;
+    if ((ha >= 0.0f) && ov) {
+        ov = (ha > kPi2) && (d_min < kDrEps);
+        if (!ov) {
+            F3 c = (mn + mx) / 2.0f;
+            F3 vca = c - x;
+            float l = norm(vca);
+            vca = vca / l;
+            ov = l < cn.radius;
+            if (!ov) {
+                F3 axis = f3(cn.ax, cn.ay, cn.az);
+                float axis_angle = cr_acos(clampf(dot(axis, vca), -1.0f, 1.0f));
+                float vch = cr_asin(cn.radius / l);
+                float hs = ha + vch;
+                float amin = axis_angle - hs, amax = axis_angle + hs;
+                ov = (hs > kPi2) || ((kPi2 >= amin) && (kPi2 <= amax));
+                if (!ov) {
+                    float c0 = cr_cos(amin), c1 = cr_cos(amax);
+                    float cmin_a = fmn(c0, c1), cmax_a = fmx(c0, c1);
+                    if ((amin <= 0.0f) && (amax >= 0.0f)) cmax_a = fmx(cmax_a, 1.0f);
+                    float beta = cr_acos(clampf(dot(axis, v), -1.0f, 1.0f));
+                    float bmin = beta - ha, bmax = beta + ha;
+                    float b0 = cr_cos(bmin), b1 = cr_cos(bmax);
+                    float cmin_b = fmn(b0, b1), cmax_b = fmx(b0, b1);
+                    if ((bmin <= 0.0f) && (bmax >= 0.0f)) cmax_b = fmx(cmax_b, 1.0f);
+                    float gamma = cr_acos(clampf(dot(v, vca), -1.0f, 1.0f));
+                    float gmin = gamma - vch, gmax = gamma + vch;
+                    float g0 = cr_cos(gmin), g1 = cr_cos(gmax);
+                    float cmin_g = fmn(g0, g1), cmax_g = fmx(g0, g1);
+                    if ((gmin <= 0.0f) && (gmax >= 0.0f)) cmax_g = fmx(cmax_g, 1.0f);
+                    float a_min = cmin_b / cmax_a, a_max = cmax_b / cmin_a;
+                    float b_min = cmin_g, b_max = cmax_g;
+                    float rho2 = (1.0f - 2.0f * fmn(fmn(a_min * b_min, a_min * b_max), fmn(a_max * b_min, a_max * b_max))) +
+                                 fmx(a_min * a_min, a_max * a_max);
+                    ov = rho2 > rho_max * rho_max;
+                }
+            }
+        }
+    }
+    return ov;
+}
+
+// inside_triangle (math.py:311-331)
+DWD bool inside_triangle3(F3 p, F3 a, F3 b, F3 c) {
+    F3 v0 = c - a, v1 = b - a, v2 = p - a;
+    float d00 = dot(v0, v0), d01 = dot(v0, v1), d02 = dot(v0, v2), d11 = dot(v1, v1), d12 = dot(v1, v2);
+    float denom = d00 * d11 - d01 * d01;
+    if (denom == 0.0f) return false;
+    float uu = (d11 * d02 - d01 * d12) / denom;
+    float vv = (d00 * d12 - d01 * d02) / denom;
+    return (uu >= 0.0f) && (vv >= 0.0f) && (uu + vv <= 1.0f);
+}
+
+// Triangle.star_radius_2 (triangle.py:109-151) with Plane.bounding_point
+// (plane.py:11-34) and line_sphere_intersection (math.py:373-386)
+DWD float tri_star_radius2(const DTri& tr, F3 x, F3 e, float rho_max, float r_max) {
+    F3 a = f3(tr.ax, tr.ay, tr.az), b = f3(tr.bx, tr.by, tr.bz), cc = f3(tr.cx, tr.cy, tr.cz);
+    float R = r_max;
+    F3 n = normalize(cross(b - a, cc - a));
+    F3 c = f3(kInf, kInf, kInf), p = f3(kInf, kInf, kInf);
+    float r = kInf;
+    if (__builtin_fabsf(dot(e, n)) > kDrEps) {
+        float d = dot(a - x, n) / dot(e, n);
+        c = x + e * d;
+        F3 xp = x + n * dot(a - x, n);
+        r = rho_max * __builtin_fabsf(d);
+        if (norm(xp - c) > kDrEps)
+            p = c + normalize(xp - c) * r;
+        else if (norm(n - f3(1.0f, 0.0f, 0.0f)) < 1e-4f)
+            p = c + cross(n, f3(0.0f, 1.0f, 0.0f)) * r;
+        else
+            p = c + cross(n, f3(0.0f, 0.0f, 1.0f)) * r;
+    }
+    F3 qp;
+    float qu, qv, qd;
+    tri_closest(tr, x, qp, qu, qv, qd);
+    bool inside_circle = (norm(a - c) < r) && (norm(b - c) < r) && (norm(cc - c) < r);
+    if (inside_circle) {
+    } else if (norm(qp - c) > r) {
+        R = fmn(R, norm(qp - x));
+    } else if (inside_triangle3(p, a, b, cc)) {
+        R = fmn(R, norm(p - x));
+    } else {
+        F3 vs[3] = {a, b, cc};
+        for (int k = 0; k < 3; ++k) {
+            F3 s0 = vs[k], s1 = vs[(k + 1) % 3];
+            F3 dd = s1 - s0, L = s0 - c;
+            float qa = dot(dd, dd), qb = 2.0f * dot(dd, L), qc = dot(L, L) - r * r;
+            float disc = qb * qb - 4.0f * qa * qc;
+            if (disc > 0.0f) {
+                float t0 = (-qb - __builtin_sqrtf(disc)) / (2.0f * qa);
+                float t1 = (-qb + __builtin_sqrtf(disc)) / (2.0f * qa);
+                if ((t0 >= 0.0f) && (t0 <= 1.0f)) R = fmn(R, norm((s0 + dd * t0) - x));
+                if ((t1 >= 0.0f) && (t1 <= 1.0f)) R = fmn(R, norm((s0 + dd * t1) - x));
+            }
+        }
+    }
+    return R;
+}
+
+// Mesh.star_radius_2 (mesh.py:349-361): the star radius, then SNCH3D.star_radius_3
+// (snch3d.py:742-811) over the triangles
+DWD float star_radius3_rho(const SubDev& S, F3 x, F3 e, float rho_max) {
+    const DNode3* nodes = (const DNode3*)S.nodes;
+    const DCone3* cones = (const DCone3*)S.cones;
+    const DTri* prims = (const DTri*)S.prims;
+    float r_max = star_radius3<true>(S, x, kInf);
+    float dmin, dmax;
+    box_dist(node3_mn(nodes[0]), node3_mx(nodes[0]), x, dmin, dmax);
+    if (!(dmin <= r_max)) return r_max;
+    int st_i[DW_MAX_STACK];
+    float st_d[DW_MAX_STACK];
+    int sp = 0;
+    st_i[0] = 0;
+    st_d[0] = dmin;
+    while (sp >= 0) {
+        int ni = st_i[sp];
+        float cd = st_d[sp];
+        sp--;
+        if (!(cd <= r_max)) continue;
+        const DNode3 nd = nodes[ni];
+        if (nd.nref > 0) {
+            for (int j = 0; j < nd.nref; ++j) {
+                float r = tri_star_radius2(prims[nd.off + j], x, e, rho_max, r_max);
+                if (r < r_max) r_max = r;
+            }
+        } else {
+            int L = ni + 1, R = ni + nd.off;
+            float d0, d1;
+            bool h0 = snch_visit3(nodes[L], cones[L], x, e, rho_max, r_max, d0);
+            bool h1 = snch_visit3(nodes[R], cones[R], x, e, rho_max, r_max, d1);
+            if (h0 && h1) {
+                int closer = L, other = R;
+                if (d1 < d0) {
+                    closer = R, other = L;
+                    float tmp = d0;
+                    d0 = d1, d1 = tmp;
+                }
+                st_i[++sp] = other;
+                st_d[sp] = d1;
+                st_i[++sp] = closer;
+                st_d[sp] = d0;
+            } else if (h0) {
+                st_i[++sp] = L;
+                st_d[sp] = d0;
+            } else if (h1) {
+                st_i[++sp] = R;
+                st_d[sp] = d1;
+            }
+        }
+    }
+    return r_max;
+}
+
+// WoStGrad3D.dude (wost_grad_3d.py:93-206) on an all-Neumann scene
+DWD float grad_dude3(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, const GradDev& G, F3 x, F3 e,
+                     Rng& rng, int& steps) {
+    const SubDev& F = sc.sub[0];
+    const SubDev& Nm = sc.sub[2];
+    F3 ip = x, in = f3(0.0f, 0.0f, 0.0f);
+    bool ob = false;
+    float result = 0.0f;
+    bool active = inside3(sc, x);
+    int i = 0;
+    while ((i < P.nsteps) && active) {
+        bool inflate = false;
+        float R = kInf;
+        if (Nm.present) {
+            if (ob) ip = ip - in * 1e-3f;
+            R = fmn(R, star_radius3_rho(Nm, ip, e, G.rho_max));
+            if (ob) {
+                if (R < P.min_R) inflate = true;
+                R = fmx(R, P.min_R);
+            }
+        }
+        float alpha = ob ? 2.0f : 1.0f;
+        if (Nm.present) {
+            {  // Mesh.sample_boundary (mesh.py:474-490) + handle_boundary (wost_grad_3d.py:22-45)
+                float rnd, pmf;
+                int idx = cdf_sample(Nm, rng.next(), rnd, pmf);
+                const float* tp = Nm.local_prims + 9 * idx;
+                F3 a = f3(tp[0], tp[1], tp[2]), b = f3(tp[3], tp[4], tp[5]), c = f3(tp[6], tp[7], tp[8]);
+                F3 cr = cross(b - a, c - a);
+                float area = norm(cr) / 2.0f;
+                float r1 = rng.next(), r2 = rng.next();
+                float su = __builtin_sqrtf(r1) * (1.0f - r2), sv = __builtin_sqrtf(r1) * r2;
+                F3 bp = (b * su + c * sv) + a * ((1.0f - su) - sv);
+                F3 bn = normalize(cr);
+                float bpdf = pmf / area;
+                if (!inflate) {
+                    float Gv = clampf(greens3_G(ip, R, ip, bp), -100.0f, 100.0f);
+                    float res = 0.0f;
+                    if (norm(ip - bp) <= R) {
+                        float e_n = dot(e, bn);
+                        F3 e_t = e - bn * e_n;
+                        F3 H[3];
+                        bc_hess3(bc, bp, H);
+                        F3 He = f3(dot(H[0], e_t), dot(H[1], e_t), dot(H[2], e_t));
+                        float f_tn = dot(He, bn);
+                        res += f_tn - e_n * src_f(bc);
+                        if (P.cv) res -= dot(f3(0.0f, 0.0f, 0.0f), bn);
+                    }
+                    result += alpha * (-Gv * res / bpdf);
+                }
+            }
+            if (Nm.n_psil3 > 0) {  // Mesh.sample_silhouette (mesh.py:500-516) + handle_silhouette (:47-71)
+                float rnd, pmf;
+                int idx = psil3_sample(Nm, rng.next(), rnd, pmf);
+                const float* q = Nm.psil3 + 12 * idx;
+                F3 a = f3(q[0], q[1], q[2]), b = f3(q[3], q[4], q[5]), c = f3(q[6], q[7], q[8]), d = f3(q[9], q[10], q[11]);
+                float length = norm(b - a);
+                F3 n1 = normalize(cross(b - a, c - a));
+                F3 n2 = normalize(cross(d - a, b - a));
+                F3 t1 = normalize(cross(b - a, n1));
+                F3 t2 = normalize(cross(a - b, n2));
+                F3 sp = lerp(a, b, rng.next());
+                float spdf = pmf / length;
+                if (!inflate) {
+                    float Gv = clampf(greens3_G(ip, R, ip, sp), -100.0f, 100.0f);
+                    float res = 0.0f;
+                    if (norm(sp - ip) <= R) {
+                        F3 gu = bc_grad(bc, sp);
+                        float h1 = dot(gu, n1), h2 = dot(gu, n2);
+                        float e1 = dot(e, t1), e2 = dot(e, t2);
+                        if (P.cv) {
+                            h1 = h1 - dot(f3(0.0f, 0.0f, 0.0f), n1);
+                            h2 = h2 - dot(f3(0.0f, 0.0f, 0.0f), n2);
+                        }
+                        res -= h1 * e1;
+                        res -= h2 * e2;
+                    }
+                    result += alpha * (-Gv * res / spdf);
+                }
+            }
+        }
+        F3 dir = uniform_sphere(rng);
+        if (ob && (dot(dir, in) > 0.0f)) dir = -dir;
+        F3 _p = ip;
+        F3 o = ob ? ip - in * 1e-5f : ip;  // Mesh.intersect (mesh.py:384-415)
+        Hit its = intersect3(F, o, dir, R);
+        if (!its.valid) {
+            its.p = o + dir * R;
+            its.n = dir;
+            its.d = R;
+            its.on_boundary = false;
+        }
+        ip = its.p;
+        in = its.n;
+        ob = its.on_boundary;
+        result += dot(_p - ip, f3(0.0f, 0.0f, 0.0f));
+        if (!G.ignore_source) {  // handle_source (wost_grad_3d.py:73-90)
+            F3 spt;
+            float sr_, spdf;
+            greens3_sample(_p, R, dir, rng, spt, sr_, spdf);
+            float res = 0.0f;
+            if (norm(spt - _p) < its.d) res += (R * R / 6.0f) * dot(f3(0.0f, 0.0f, 0.0f), e);
+            result += res;
+        }
+        if (ob) {
+            float e_n = dot(e, in);
+            F3 e_t = e - in * e_n;
+            float dir_n = dot(dir, in);
+            F3 dir_t = dir - in * dir_n;
+            F3 ref_t = f3(0.0f, 0.0f, 0.0f) - in * dot(f3(0.0f, 0.0f, 0.0f), in);
+            result += dot(dir_t, ref_t) * e_n / dir_n;
+            result += e_n * dot(in, bc_grad(bc, ip));
+            e = e_t - (dir_t * e_n) / dir_n;
+        }
+        if (norm(e) < 1.0f) {  // Russian roulette
+            if (norm(e) < rng.next())
+                active = false;
+            else
+                e = e / norm(e);
         }
         i += 1;
         if (bad(result) || __builtin_fabsf(result) > 1000.0f) result = 0.0f;

@@ -124,6 +124,13 @@ struct SubDev {
     const void* pnodes;
     const int* porder;
     int n_psil;
+    // 3D: the mesh's own silhouette edges (a, b, c, d) and their length distribution
+    const float* psil3;
+    const float* ps_pmf;
+    const float* ps_cdf;
+    const int* ps_guide;
+    float ps_sum, ps_norm;
+    int ps_shift, n_psil3;
     // sampling (local primitive order)
     const float* cdf;
     const float* pmf;

@@ -506,6 +506,7 @@ __global__ void __launch_bounds__(kBlock) k_baseline(SceneDev sc, int sub, int k
 // WoStGrad.solve_dude through Solver.solve_dude (solver.py:113-144) and
 // walk_dude (:146-166): lane i * nwalks + k runs nsubwalks dudes on the PCG32
 // stream of lane (point_base + i) * nwalks + k and writes their mean.
+template <int D>
 __global__ void __launch_bounds__(kBlock) k_grad_dude(SceneDev sc, BCDev bc, ParamsDev P, GradDev G,
                                                       const float* __restrict__ pts, const float* __restrict__ ev,
                                                       int64_t n, float* walk_val, int* walk_steps) {
@@ -514,10 +515,15 @@ __global__ void __launch_bounds__(kBlock) k_grad_dude(SceneDev sc, BCDev bc, Par
     int64_t i = lane / P.nwalks;
     Rng rng;
     seed_stream(rng, P, i, P.nwalks, lane - i * P.nwalks);
-    F2 x = load_pt<2>(pts, i), e = load_pt<2>(ev, i);
+    auto x = load_pt<D>(pts, i), e = load_pt<D>(ev, i);
     float v = 0.0f;
     int st = 0;
-    for (int s = 0; s < P.nsubwalks; ++s) v += grad_dude2(sc, bc, P, G, x, e, rng, st);
+    for (int s = 0; s < P.nsubwalks; ++s) {
+        if constexpr (D == 2)
+            v += grad_dude2(sc, bc, P, G, x, e, rng, st);
+        else
+            v += grad_dude3(sc, bc, P, G, x, e, rng, st);
+    }
     walk_val[lane] = v / (float)P.nsubwalks;
     if (walk_steps) walk_steps[lane] = st;
 }
@@ -1741,6 +1747,17 @@ int upload_sub(dw_scene* s, const dwb::SubHost& H, SubDev& S) {
         if ((rc = upload(s, pn, &S.pnodes))) return rc;
         if ((rc = upload(s, H.porder, &p))) return rc;
         S.porder = (const int*)p;
+    } else {
+        S.n_psil3 = (int)(H.psil3.size() / 12);
+        S.ps_sum = H.ps_sum, S.ps_norm = H.ps_norm, S.ps_shift = H.ps_shift;
+        if ((rc = upload(s, H.psil3, &p))) return rc;
+        S.psil3 = (const float*)p;
+        if ((rc = upload(s, H.ps_pmf, &p))) return rc;
+        S.ps_pmf = (const float*)p;
+        if ((rc = upload(s, H.ps_cdf, &p))) return rc;
+        S.ps_cdf = (const float*)p;
+        if ((rc = upload(s, H.ps_guide, &p))) return rc;
+        S.ps_guide = (const int*)p;
     }
     if ((rc = upload(s, H.cdf, &p))) return rc;
     S.cdf = (const float*)p;
@@ -2410,7 +2427,9 @@ int dw_grad_solve_dude(const dw_scene* s, const float* pts, const float* e, int6
     int rc;
     if ((rc = check_scene(s, 0))) return rc;
     if ((rc = check_params(params)) || (rc = check_bc(boundary))) return rc;
-    if (s->dev.dim != 2) return fail(DW_ERR_ARG, "dw_grad_solve_dude: WoStGrad runs on 2D scenes");
+    if (s->dev.dim == 3 && s->dev.sub[1].present)
+        return fail(DW_ERR_ARG, "dw_grad_solve_dude: WoStGrad3D needs an all-Neumann scene (the reference's 3D walk "
+                                "has no normal-derivative estimate at Dirichlet points)");
     if (!grad) return fail(DW_ERR_ARG, "null grad params");
     if (boundary->bc_code == DW_BC_PER_VERTEX)
         return fail(DW_ERR_ARG, "dw_grad_solve_dude needs an analytic boundary field (tangential derivatives)");
@@ -2432,8 +2451,12 @@ int dw_grad_solve_dude(const dw_scene* s, const float* pts, const float* e, int6
         if (!ws && total_steps) ws = (int32_t*)cur;
     }
     GradDev G{grad->rho_max, grad->ignore_boundary, grad->ignore_silhouette, grad->ignore_source};
-    hipLaunchKernelGGL(k_grad_dude, dim3(blocks_for(lanes)), dim3(kBlock), kStackLds, st, s->dev, to_bc(boundary),
-                       to_params(params), G, pts, e, n, wv, ws);
+    if (s->dev.dim == 2)
+        hipLaunchKernelGGL(k_grad_dude<2>, dim3(blocks_for(lanes)), dim3(kBlock), kStackLds, st, s->dev,
+                           to_bc(boundary), to_params(params), G, pts, e, n, wv, ws);
+    else
+        hipLaunchKernelGGL(k_grad_dude<3>, dim3(blocks_for(lanes)), dim3(kBlock), kStackLds, st, s->dev,
+                           to_bc(boundary), to_params(params), G, pts, e, n, wv, ws);
     DW_HIP(hipGetLastError());
     if (total_steps && (rc = zero_u64((unsigned long long*)total_steps, st))) return rc;
     hipLaunchKernelGGL(k_reduce_mean, dim3(blocks_for(n * 64)), dim3(kBlock), 0, st, wv, n, params->nwalks, 1,

@@ -671,6 +671,71 @@ void build_psil2(SubHost& S) {
     S.porder = tree.order;
 }
 
+// The mesh's own silhouette edges (Mesh.configure_silhouette, mesh.py:140-243)
+// for WoStGrad3D: per face (a, b, c) the half edges (min, max) of (a, b), (b, c),
+// (c, a) with the opposite vertex -- all first edges, then all second, then all
+// third -- stably sorted by the key u * V + v; an edge met exactly twice gives
+// (a = V[u], b = V[v], c = opposite of the first, d = opposite of the second),
+// kept unless |n0 . n1| >= 1 - 1e-5 (n0 = N((b - a) x (c - a)),
+// n1 = N((d - a) x (b - a))); sampled by length (sequential fp32 CDF).
+void build_psil3(SubHost& S) {
+    int F = (int)(S.I.size() / 3), nv = (int)(S.V.size() / 3);
+    std::vector<int> u, v, opp;
+    for (int k = 0; k < 3; ++k)
+        for (int f = 0; f < F; ++f) {
+            int a = S.I[3 * f + k], b = S.I[3 * f + (k + 1) % 3], c = S.I[3 * f + (k + 2) % 3];
+            u.push_back(std::min(a, b)), v.push_back(std::max(a, b)), opp.push_back(c);
+        }
+    std::vector<int64_t> key(u.size());
+    for (size_t i = 0; i < u.size(); ++i) key[i] = (int64_t)u[i] * nv + v[i];
+    std::vector<int> order(u.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key[x] < key[y]; });
+    auto sub3 = [](const float* p, const float* q, float* o) { o[0] = p[0] - q[0], o[1] = p[1] - q[1], o[2] = p[2] - q[2]; };
+    auto cross3 = [](const float* x, const float* y, float* o) {  // fma(u.y, v.z, -(u.z * v.y)), ...
+        o[0] = std::fmaf(x[1], y[2], -(x[2] * y[1]));
+        o[1] = std::fmaf(x[2], y[0], -(x[0] * y[2]));
+        o[2] = std::fmaf(x[0], y[1], -(x[1] * y[0]));
+    };
+    auto dot3 = [](const float* x, const float* y) { return std::fmaf(x[2], y[2], std::fmaf(x[1], y[1], x[0] * y[0])); };
+    auto unit3 = [&](float* x) {
+        float inv = 1.0f / std::sqrt(dot3(x, x));
+        x[0] *= inv, x[1] *= inv, x[2] *= inv;
+    };
+    S.psil3.clear();
+    S.ps_pmf.clear();
+    size_t i = 0;
+    while (i < order.size()) {
+        size_t j = i;
+        while (j < order.size() && key[order[j]] == key[order[i]]) ++j;
+        if (j - i == 2) {
+            int e0 = order[i], e1 = order[i + 1];
+            const float *a = &S.V[3 * u[e0]], *b = &S.V[3 * v[e0]], *c = &S.V[3 * opp[e0]], *d = &S.V[3 * opp[e1]];
+            float ba[3], ca[3], da[3], n0[3], n1[3];
+            sub3(b, a, ba), sub3(c, a, ca), sub3(d, a, da);
+            cross3(ba, ca, n0), cross3(da, ba, n1);
+            unit3(n0), unit3(n1);
+            if (std::fabs(dot3(n0, n1)) < (1.0f - 1e-5f)) {
+                S.psil3.insert(S.psil3.end(), {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2], d[0], d[1], d[2]});
+                float ab[3];
+                sub3(a, b, ab);
+                S.ps_pmf.push_back(std::sqrt(dot3(ab, ab)));
+            }
+        }
+        i = j;
+    }
+    SubHost tmp;  // the guide table of the length distribution (see cdf_guide)
+    float acc = 0.0f;
+    for (float w : S.ps_pmf) acc += w, tmp.cdf.push_back(acc);
+    tmp.pmf_sum = tmp.cdf.empty() ? 0.0f : tmp.cdf.back();
+    cdf_guide(tmp);
+    S.ps_cdf = tmp.cdf;
+    S.ps_sum = tmp.pmf_sum;
+    S.ps_norm = 1.0f / tmp.pmf_sum;
+    S.ps_guide = tmp.cdf_guide;
+    S.ps_shift = tmp.guide_shift;
+}
+
 // _build_subscene_from_mask (polyline.py:672-692, mesh.py:617-635)
 template <int D>
 void make_sub(const SceneHost& sc, int want, SubHost& S) {
@@ -700,6 +765,7 @@ void make_sub(const SceneHost& sc, int want, SubHost& S) {
         for (int k = 0; k < D; ++k) S.I[j * D + k] = remap[sc.I[(size_t)D * S.pglobal[j] + k]];
     finish_sub<D>(S);
     if constexpr (D == 2) build_psil2(S);
+    if constexpr (D == 3) build_psil3(S);
 }
 
 }  // namespace

@@ -52,6 +52,12 @@ struct SubHost {
     // counts, and BVH order -> silhouette index
     std::vector<float> psil, pnode_box;
     std::vector<int> pnode_off, pnode_nref, porder;
+    // 3D: the mesh's own silhouette edges (a, b, c, d: 12 floats each) and their
+    // length distribution (pmf, sequential fp32 cdf, guide table)
+    std::vector<float> psil3, ps_pmf, ps_cdf;
+    float ps_sum = 0.0f, ps_norm = 0.0f;
+    std::vector<int> ps_guide;
+    int ps_shift = 0;
     size_t n_nodes() const { return node_off.size(); }
     size_t n_prims() const { return I.size() / dim; }
 };

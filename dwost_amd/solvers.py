@@ -250,6 +250,7 @@ class WoS3D(Solver):
 @dataclass
 class WoStGrad(Solver):
     """2D WoStGrad, the robust derivative walk (diff_wost/solvers/wost_grad.py:17-254).
+    ``WoStGrad3D`` is its 3D form (all-Neumann scenes).
 
     ``solve_dude(p, e, scene, seed)`` estimates du/de with walks that reflect off
     Neumann boundaries (Solver.solve_dude, solver.py:113-166: one stream per
@@ -301,3 +302,14 @@ class WoStGrad(Solver):
         if count_steps:
             r["steps"] = steps
         return r
+
+
+@dataclass
+class WoStGrad3D(WoStGrad):
+    """3D WoStGrad (diff_wost/solvers/wost_grad_3d.py:13-206) on all-Neumann
+    scenes: the reference's 3D walk has no normal-derivative estimate at
+    Dirichlet points.  Its ignore_boundary / ignore_silhouette fields are unused
+    by the reference walk (kept for API parity)."""
+
+    min_R: float = 1e-1
+    dim: int = 3

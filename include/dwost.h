@@ -239,14 +239,17 @@ typedef struct dw_grad_params {
     int32_t ignore_source;
 } dw_grad_params;
 
-/* WoStGrad.solve_dude (wost_grad.py:91-254 through Solver.solve_dude /
- * walk_dude, solver.py:113-166), 2D scenes: du/de at pts along e[n][2], nwalks
+/* WoStGrad.solve_dude (wost_grad.py:91-254) and WoStGrad3D.solve_dude
+ * (wost_grad_3d.py:93-206) through Solver.solve_dude / walk_dude
+ * (solver.py:113-166): du/de at pts along e[n][dim], nwalks
  * walks per point (one PCG32 stream each, keyed as Solver.solve), nsubwalks
  * dudes per walk.  params: nwalks, nsteps, nsubwalks, eps, min_R (1e-2 in the
  * reference), control_variates (of the nested normal-derivative walk), seed,
  * point_base.  Boundary data must be an analytic field (its tangential
- * derivatives enter the estimator).  out[n]; per-walk walk_out[n*nwalks] and
- * walk_steps (outer steps + nested walk steps) optional. */
+ * derivatives enter the estimator).  3D scenes must be all-Neumann (the
+ * reference's 3D walk defines no normal-derivative estimate at Dirichlet
+ * points).  out[n]; per-walk walk_out[n*nwalks] and walk_steps (outer steps +
+ * nested walk steps) optional. */
 int dw_grad_solve_dude(const dw_scene* scene, const float* pts, const float* e, int64_t n,
                        const dw_solver_params* params, const dw_grad_params* grad, const dw_boundary* boundary,
                        float* out, float* walk_out, int32_t* walk_steps, uint64_t* total_steps, void* stream);

@@ -685,6 +685,11 @@ struct Sub {  // one Polyline / Mesh
     std::vector<float> psil;
     std::vector<Node<D>> pnodes;
     std::vector<PPoint<D>> pprims;
+    // 3D: the mesh's own silhouette edges (Mesh.configure_silhouette,
+    // mesh.py:140-243: a, b, c, d per edge shared by exactly two faces, flat
+    // ones dropped) and their length-weighted pmf
+    std::vector<float> psil3, ps_pmf, ps_cdf;
+    float ps_sum = 0.0f, ps_norm = 0.0f;
 };
 
 struct Scene {
@@ -840,6 +845,52 @@ static void build_psil2(Sub<2>& S) {
     S.pnodes = b.nodes;
 }
 
+// Mesh.configure_silhouette (mesh.py:140-243): per face (a, b, c) the half
+// edges (min, max) of (a, b), (b, c), (c, a) with the opposite vertex, all
+// first edges, then all second, then all third; a stable sort by the key
+// u * V + v; edges met exactly twice give (a = V[u], b = V[v], c = opposite of
+// the first, d = opposite of the second), kept unless
+// |n0 . n1| >= 1 - 1e-5 (n0 = N((b - a) x (c - a)), n1 = N((d - a) x (b - a)));
+// pmf = |a - b| with a sequential fp32 CDF (distr.py:44-50).
+static void build_psil3(Sub<3>& S) {
+    int F = (int)(S.I.size() / 3), nv = (int)(S.V.size() / 3);
+    std::vector<int> u, v, opp;
+    for (int k = 0; k < 3; ++k)
+        for (int f = 0; f < F; ++f) {
+            int a = S.I[3 * f + k], b = S.I[3 * f + (k + 1) % 3], c = S.I[3 * f + (k + 2) % 3];
+            u.push_back(std::min(a, b)), v.push_back(std::max(a, b)), opp.push_back(c);
+        }
+    std::vector<int64_t> key(u.size());
+    for (size_t i = 0; i < u.size(); ++i) key[i] = (int64_t)u[i] * nv + v[i];
+    std::vector<int> order(u.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key[x] < key[y]; });
+    S.psil3.clear();
+    S.ps_pmf.clear();
+    size_t i = 0;
+    while (i < order.size()) {
+        size_t j = i;
+        while (j < order.size() && key[order[j]] == key[order[i]]) ++j;
+        if (j - i == 2) {
+            int e0 = order[i], e1 = order[i + 1];
+            auto P = [&](int id) { return v3(S.V[3 * id], S.V[3 * id + 1], S.V[3 * id + 2]); };
+            V3 a = P(u[e0]), b = P(v[e0]), c = P(opp[e0]), d = P(opp[e1]);
+            V3 n0 = normalize(cross(sub(b, a), sub(c, a))), n1 = normalize(cross(sub(d, a), sub(b, a)));
+            if (std::fabs(dot(n0, n1)) < (1.0f - 1e-5f)) {
+                float q[12] = {a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, d.x, d.y, d.z};
+                S.psil3.insert(S.psil3.end(), q, q + 12);
+                S.ps_pmf.push_back(norm(sub(a, b)));
+            }
+        }
+        i = j;
+    }
+    S.ps_cdf.resize(S.ps_pmf.size());
+    float acc = 0.0f;
+    for (size_t k = 0; k < S.ps_pmf.size(); ++k) acc += S.ps_pmf[k], S.ps_cdf[k] = acc;
+    S.ps_sum = S.ps_cdf.empty() ? 0.0f : S.ps_cdf.back();
+    S.ps_norm = 1.0f / S.ps_sum;
+}
+
 template <int D>
 static void make_sub(const Scene& sc, int want_type, Sub<D>& S) {
     size_t np = sc.types.size();
@@ -868,6 +919,7 @@ static void make_sub(const Scene& sc, int want_type, Sub<D>& S) {
     build_sub<D>(S);
     build_snch<D>(S);
     if constexpr (D == 2) build_psil2(S);
+    if constexpr (D == 3) build_psil3(S);
 }
 
 // ----------------------------------------------------------------------------
@@ -2789,6 +2841,344 @@ static float grad_dude2(const Scene& sc, const BC& bc, const Params& P, const Gr
     return result;
 }
 
+// ============================================================================
+// WoStGrad3D (diff_wost/solvers/wost_grad_3d.py): the robust derivative walk,
+// 3D.  The reference calls a dudn it never defines once a walk reaches a
+// Dirichlet point, so the estimator runs on all-Neumann scenes only.
+// ============================================================================
+// Hessian of the analytic field, rows (u_x., u_y., u_z.)
+static inline void bc_hess3(const BC& bc, V3 p, V3 H[3]) {
+    if (bc.code == BC_EXPCOS) {
+        float ex = cr_exp(p.x), c = cr_cos(p.y), sn = cr_sin(p.y);
+        H[0] = v3(ex * c, -(ex * sn), 0.0f), H[1] = v3(-(ex * sn), -(ex * c), 0.0f), H[2] = v3(0.0f, 0.0f, 0.0f);
+    } else if (bc.code == BC_QUAD) {
+        H[0] = v3(2.0f, 0.0f, 0.0f), H[1] = v3(0.0f, -2.0f, 0.0f), H[2] = v3(0.0f, 0.0f, 0.0f);
+    } else {
+        H[0] = H[1] = H[2] = v3(0.0f, 0.0f, 0.0f);
+    }
+}
+
+// DiscreteDistribution.sample_reuse_pmf (distr.py:52-107) over (cdf, pmf)
+static inline int dist_sample(const std::vector<float>& cdfv, const std::vector<float>& pmfv, float sum, float nrm,
+                              float value, float& reuse, float& pmf) {
+    float sample = value * sum;
+    int n = (int)cdfv.size();
+    uint32_t start = 0, end = (uint32_t)(n - 1);
+    int iters = start < end ? (32 - __builtin_clz(end - start)) : 0;
+    for (int it = 0; it < iters; ++it) {
+        uint32_t mid = (start + end) >> 1;
+        float cv = cdfv[mid];
+        bool cond = ((cv < sample) || (cv == 0.0f)) && (cv != sum);
+        if (cond)
+            start = std::min(mid + 1, end);
+        else
+            end = mid;
+    }
+    int idx = (int)start;
+    pmf = pmfv[idx] * nrm;
+    float c = idx > 0 ? cdfv[idx - 1] * nrm : 0.0f;
+    reuse = (value - c) / pmf;
+    return idx;
+}
+
+// SNCHNode3D.visit (snch3d.py:196-271)
+static bool snch_visit3(const Node<3>& nd, V3 x, V3 v, float rho_max, float r_max, float& d_min) {
+    float d_max;
+    V3 mn = node_min3(nd), mx = node_max3(nd);
+    box_dist(mn, mx, x, d_min, d_max);
+    bool ov = d_min <= r_max;
+    float ha = nd.half_angle;
+    if ((ha >= 0.0f) && ov) {
+        ov = (ha > kPi2) && (d_min < kDrEps);
+        if (!ov) {
+            V3 c = divs(add(mn, mx), 2.0f);
+            V3 vca = sub(c, x);
+            float l = norm(vca);
+            vca = divs(vca, l);
+            ov = l < nd.radius;
+            if (!ov) {
+                V3 axis = v3(nd.axis[0], nd.axis[1], nd.axis[2]);
+                float axis_angle = cr_acos(clampf(dot(axis, vca), -1.0f, 1.0f));
+                float vch = cr_asin(nd.radius / l);
+                float hs = ha + vch;
+                float amin = axis_angle - hs, amax = axis_angle + hs;
+                ov = (hs > kPi2) || in_range(kPi2, amin, amax);
+                if (!ov) {
+                    float c0 = cr_cos(amin), c1 = cr_cos(amax);
+                    float cmin_a = fminf_(c0, c1), cmax_a = fmaxf_(c0, c1);
+                    if ((amin <= 0.0f) && (amax >= 0.0f)) cmax_a = fmaxf_(cmax_a, 1.0f);
+                    float beta = cr_acos(clampf(dot(axis, v), -1.0f, 1.0f));
+                    float bmin = beta - ha, bmax = beta + ha;
+                    float b0 = cr_cos(bmin), b1 = cr_cos(bmax);
+                    float cmin_b = fminf_(b0, b1), cmax_b = fmaxf_(b0, b1);
+                    if ((bmin <= 0.0f) && (bmax >= 0.0f)) cmax_b = fmaxf_(cmax_b, 1.0f);
+                    float gamma = cr_acos(clampf(dot(v, vca), -1.0f, 1.0f));
+                    float gmin = gamma - vch, gmax = gamma + vch;
+                    float g0 = cr_cos(gmin), g1 = cr_cos(gmax);
+                    float cmin_g = fminf_(g0, g1), cmax_g = fmaxf_(g0, g1);
+                    if ((gmin <= 0.0f) && (gmax >= 0.0f)) cmax_g = fmaxf_(cmax_g, 1.0f);
+                    float a_min = cmin_b / cmax_a, a_max = cmax_b / cmin_a;
+                    float b_min = cmin_g, b_max = cmax_g;
+                    float rho2 = (1.0f - 2.0f * fminf_(fminf_(a_min * b_min, a_min * b_max),
+                                                       fminf_(a_max * b_min, a_max * b_max))) +
+                                 fmaxf_(a_min * a_min, a_max * a_max);
+                    ov = rho2 > rho_max * rho_max;
+                }
+            }
+        }
+    }
+    return ov;
+}
+
+// inside_triangle (math.py:311-331)
+static bool inside_triangle3(V3 p, V3 a, V3 b, V3 c) {
+    V3 v0 = sub(c, a), v1 = sub(b, a), v2 = sub(p, a);
+    float d00 = dot(v0, v0), d01 = dot(v0, v1), d02 = dot(v0, v2), d11 = dot(v1, v1), d12 = dot(v1, v2);
+    float denom = d00 * d11 - d01 * d01;
+    if (denom == 0.0f) return false;
+    float uu = (d11 * d02 - d01 * d12) / denom;
+    float vv = (d00 * d12 - d01 * d02) / denom;
+    return (uu >= 0.0f) && (vv >= 0.0f) && (uu + vv <= 1.0f);
+}
+
+// Triangle.star_radius_2 (triangle.py:109-151) with Plane.bounding_point
+// (plane.py:11-34) and line_sphere_intersection (math.py:373-386)
+static float tri_star_radius2(const Prim<3>& tr, V3 x, V3 e, float rho_max, float r_max) {
+    V3 a = v3(tr.v[0][0], tr.v[0][1], tr.v[0][2]), b = v3(tr.v[1][0], tr.v[1][1], tr.v[1][2]);
+    V3 cc = v3(tr.v[2][0], tr.v[2][1], tr.v[2][2]);
+    float R = r_max;
+    V3 n = normalize(cross(sub(b, a), sub(cc, a)));
+    V3 c = v3(kInf, kInf, kInf), p = v3(kInf, kInf, kInf);
+    float r = kInf;
+    if (std::fabs(dot(e, n)) > kDrEps) {
+        float d = dot(sub(a, x), n) / dot(e, n);
+        c = add(x, mul(e, d));
+        V3 xp = add(x, mul(n, dot(sub(a, x), n)));
+        r = rho_max * std::fabs(d);
+        if (norm(sub(xp, c)) > kDrEps) {
+            p = add(c, mul(normalize(sub(xp, c)), r));
+        } else if (norm(sub(n, v3(1.0f, 0.0f, 0.0f))) < 1e-4f) {
+            p = add(c, mul(cross(n, v3(0.0f, 1.0f, 0.0f)), r));
+        } else {
+            p = add(c, mul(cross(n, v3(0.0f, 0.0f, 1.0f)), r));
+        }
+    }
+    CP q;
+    tri_closest(tr, x, q);
+    bool inside_circle = (norm(sub(a, c)) < r) && (norm(sub(b, c)) < r) && (norm(sub(cc, c)) < r);
+    if (inside_circle) {
+    } else if (norm(sub(q.p, c)) > r) {
+        R = fminf_(R, norm(sub(q.p, x)));
+    } else if (inside_triangle3(p, a, b, cc)) {
+        R = fminf_(R, norm(sub(p, x)));
+    } else {
+        V3 vs[3] = {a, b, cc};
+        for (int k = 0; k < 3; ++k) {
+            V3 s0 = vs[k], s1 = vs[(k + 1) % 3];
+            V3 dd = sub(s1, s0), L = sub(s0, c);
+            float qa = dot(dd, dd), qb = 2.0f * dot(dd, L), qc = dot(L, L) - r * r;
+            float disc = qb * qb - 4.0f * qa * qc;
+            if (disc > 0.0f) {
+                float t0 = (-qb - std::sqrt(disc)) / (2.0f * qa);
+                float t1 = (-qb + std::sqrt(disc)) / (2.0f * qa);
+                if ((t0 >= 0.0f) && (t0 <= 1.0f)) R = fminf_(R, norm(sub(add(s0, mul(dd, t0)), x)));
+                if ((t1 >= 0.0f) && (t1 <= 1.0f)) R = fminf_(R, norm(sub(add(s0, mul(dd, t1)), x)));
+            }
+        }
+    }
+    return R;
+}
+
+// Mesh.star_radius_2 (mesh.py:349-361): the star radius, then SNCH3D.star_radius_3
+// (snch3d.py:742-811) over the triangles
+static float star_radius3_rho(const Sub<3>& S, V3 x, V3 e, float rho_max) {
+    float r_max = star_radius<3>(S, &x.x, kInf, nullptr);
+    const Tree<3>& T = S.tree;
+    float dmin, dmax;
+    box_dist(node_min3(T.nodes[0]), node_max3(T.nodes[0]), x, dmin, dmax);
+    if (!(dmin <= r_max)) return r_max;
+    int st_i[64];
+    float st_d[64];
+    int sp = 0;
+    st_i[0] = 0;
+    st_d[0] = dmin;
+    while (sp >= 0) {
+        int ni = st_i[sp];
+        float cd = st_d[sp];
+        sp--;
+        if (!(cd <= r_max)) continue;
+        const Node<3>& nd = T.nodes[ni];
+        if (nd.nref > 0) {
+            for (int j = 0; j < nd.nref; ++j) {
+                float r = tri_star_radius2(T.prims[nd.off + j], x, e, rho_max, r_max);
+                if (r < r_max) r_max = r;
+            }
+        } else {
+            int L = ni + 1, R = ni + nd.off;
+            float d0, d1;
+            bool h0 = snch_visit3(T.nodes[L], x, e, rho_max, r_max, d0);
+            bool h1 = snch_visit3(T.nodes[R], x, e, rho_max, r_max, d1);
+            if (h0 && h1) {
+                int closer = L, other = R;
+                if (d1 < d0) {
+                    std::swap(closer, other);
+                    std::swap(d0, d1);
+                }
+                st_i[++sp] = other;
+                st_d[sp] = d1;
+                st_i[++sp] = closer;
+                st_d[sp] = d0;
+            } else if (h0) {
+                st_i[++sp] = L;
+                st_d[sp] = d0;
+            } else if (h1) {
+                st_i[++sp] = R;
+                st_d[sp] = d1;
+            }
+        }
+    }
+    return r_max;
+}
+
+// WoStGrad3D.dude (wost_grad_3d.py:93-206) on an all-Neumann scene
+static float grad_dude3(const Scene& sc, const BC& bc, const Params& P, const GradParams& G, V3 x, V3 e, Pcg& rng,
+                        int& steps) {
+    const Sub<3>& F = sc.s3[0];
+    const Sub<3>& Nm = sc.s3[2];
+    V3 ip = x, in = v3(0.0f, 0.0f, 0.0f);
+    bool ob = false;
+    float result = 0.0f;
+    bool active = inside3(sc, x);
+    int i = 0;
+    while ((i < P.nsteps) && active) {
+        bool inflate = false;
+        float R = kInf;
+        if (Nm.present) {
+            if (ob) ip = sub(ip, mul(in, 1e-3f));
+            if (G.plant == 4) {  // tests only: the loop over all triangles (star_radius_2_baseline)
+                float r0 = star_radius<3>(Nm, &ip.x, kInf, nullptr), m[3];
+                for (int d = 0; d < 3; ++d)
+                    m[d] = fmaxf_(std::fabs((&ip.x)[d] - Nm.bb_min[d]), std::fabs((&ip.x)[d] - Nm.bb_max[d]));
+                float Rb = fminf_(norm(v3(m[0], m[1], m[2])), r0);
+                for (size_t j = 0; j < Nm.tree.prims.size(); ++j)
+                    Rb = fminf_(Rb, tri_star_radius2(Nm.tree.prims[j], ip, e, G.rho_max, Rb));
+                R = fminf_(R, Rb);
+            } else {
+                R = fminf_(R, star_radius3_rho(Nm, ip, e, G.rho_max));
+            }
+            if (ob) {
+                if (R < P.min_R) inflate = true;
+                R = fmaxf_(R, P.min_R);
+            }
+        }
+        float alpha = ob ? 2.0f : 1.0f;
+        if (Nm.present) {
+            {  // Mesh.sample_boundary (mesh.py:474-490) + handle_boundary (wost_grad_3d.py:22-45)
+                float rnd, pmf;
+                int idx = cdf_sample<3>(Nm, rng.next_float32(), rnd, pmf, nullptr);
+                const float* A = &Nm.V[3 * Nm.I[3 * idx]];
+                const float* B = &Nm.V[3 * Nm.I[3 * idx + 1]];
+                const float* C = &Nm.V[3 * Nm.I[3 * idx + 2]];
+                V3 a = v3(A[0], A[1], A[2]), b = v3(B[0], B[1], B[2]), c = v3(C[0], C[1], C[2]);
+                V3 cr = cross(sub(b, a), sub(c, a));
+                float area = norm(cr) / 2.0f;
+                float r1 = rng.next_float32(), r2 = rng.next_float32();
+                float su = std::sqrt(r1) * (1.0f - r2), sv = std::sqrt(r1) * r2;
+                V3 bp = add(add(mul(b, su), mul(c, sv)), mul(a, (1.0f - su) - sv));
+                V3 bn = normalize(cr);
+                float bpdf = pmf / area;
+                if (!inflate) {
+                    float Gv = clampf(greens3_G(ip, R, ip, bp), -100.0f, 100.0f);
+                    float res = 0.0f;
+                    if (norm(sub(ip, bp)) <= R) {
+                        float e_n = dot(e, bn);
+                        V3 e_t = sub(e, mul(bn, e_n));
+                        V3 H[3];
+                        bc_hess3(bc, bp, H);
+                        V3 He = v3(dot(H[0], e_t), dot(H[1], e_t), dot(H[2], e_t));
+                        float f_tn = dot(He, bn);
+                        res += f_tn - e_n * src_f(bc);
+                        if (P.cv) res -= dot(v3(0.0f, 0.0f, 0.0f), bn);
+                    }
+                    result += alpha * (-Gv * res / bpdf);
+                }
+            }
+            if (!Nm.psil3.empty()) {  // Mesh.sample_silhouette (mesh.py:500-516) + handle_silhouette (:47-71)
+                float rnd, pmf;
+                int idx = dist_sample(Nm.ps_cdf, Nm.ps_pmf, Nm.ps_sum, Nm.ps_norm, rng.next_float32(), rnd, pmf);
+                const float* q = &Nm.psil3[12 * idx];
+                V3 a = v3(q[0], q[1], q[2]), b = v3(q[3], q[4], q[5]), c = v3(q[6], q[7], q[8]), d = v3(q[9], q[10], q[11]);
+                float length = norm(sub(b, a));
+                V3 n1 = normalize(cross(sub(b, a), sub(c, a)));
+                V3 n2 = normalize(cross(sub(d, a), sub(b, a)));
+                V3 t1 = normalize(cross(sub(b, a), n1));
+                V3 t2 = normalize(cross(sub(a, b), n2));
+                V3 sp = lerp(a, b, rng.next_float32());
+                float spdf = pmf / length;
+                if (!inflate) {
+                    float Gv = clampf(greens3_G(ip, R, ip, sp), -100.0f, 100.0f);
+                    float res = 0.0f;
+                    if (norm(sub(sp, ip)) <= R) {
+                        V3 gu = bc_grad(bc, sp);
+                        float h1 = dot(gu, n1), h2 = dot(gu, n2);
+                        float e1 = dot(e, t1), e2 = dot(e, t2);
+                        if (P.cv) {
+                            h1 = h1 - dot(v3(0.0f, 0.0f, 0.0f), n1);
+                            h2 = h2 - dot(v3(0.0f, 0.0f, 0.0f), n2);
+                        }
+                        res -= h1 * e1;
+                        res -= h2 * e2;
+                    }
+                    result += alpha * (-Gv * res / spdf);
+                }
+            }
+        }
+        V3 dir = uniform_sphere(rng);
+        if (ob && (dot(dir, in) > 0.0f)) dir = neg(dir);
+        V3 _p = ip;
+        V3 o = ob ? sub(ip, mul(in, 1e-5f)) : ip;  // Mesh.intersect (mesh.py:384-415)
+        Its its = snch3_intersect(F.tree, o, dir, R, nullptr);
+        if (!its.valid) {
+            its.p = add(o, mul(dir, R));
+            its.n = dir;
+            its.d = R;
+            its.on_boundary = false;
+        }
+        ip = its.p;
+        in = its.n;
+        ob = its.on_boundary;
+        result += dot(sub(_p, ip), v3(0.0f, 0.0f, 0.0f));
+        if (!G.ignore_source) {  // handle_source (wost_grad_3d.py:73-90)
+            V3 spt;
+            float sr_, spdf;
+            greens3_sample(_p, R, dir, rng, spt, sr_, spdf);
+            float res = 0.0f;
+            if (norm(sub(spt, _p)) < its.d) res += (R * R / 6.0f) * dot(v3(0.0f, 0.0f, 0.0f), e);
+            result += res;
+        }
+        if (ob) {
+            float e_n = dot(e, in);
+            V3 e_t = sub(e, mul(in, e_n));
+            float dir_n = dot(dir, in);
+            V3 dir_t = sub(dir, mul(in, dir_n));
+            V3 ref_t = sub(v3(0.0f, 0.0f, 0.0f), mul(in, dot(v3(0.0f, 0.0f, 0.0f), in)));
+            result += dot(dir_t, ref_t) * e_n / dir_n;
+            result += e_n * dot(in, bc_grad(bc, ip));
+            e = sub(e_t, divs(mul(dir_t, e_n), dir_n));
+        }
+        if (norm(e) < 1.0f) {  // Russian roulette
+            if (norm(e) < rng.next_float32())
+                active = false;
+            else
+                e = divs(e, norm(e));
+        }
+        i += 1;
+        if (bad(result) || std::fabs(result) > 1000.0f) result = 0.0f;
+    }
+    steps += i;
+    return result;
+}
+
 }  // namespace orc
 
 // ============================================================================
@@ -3270,13 +3660,20 @@ void orc_grad_dude(void* s, const float* pts, const float* ev, int64_t n, const 
     for (int64_t i = 0; i < n; ++i) {
         uint32_t a, b;
         tea(P.seed, (uint32_t)(P.base + i), 4, kTeaKey, &a, &b);
-        V2 x = v2(pts[2 * i], pts[2 * i + 1]), e = v2(ev[2 * i], ev[2 * i + 1]);
+        const int D = sc->dim;
         double acc = 0.0;
         for (int64_t k = 0; k < W; ++k) {
             Pcg rng = pcg_for_lane(a, b, (uint64_t)((P.base + i) * W + k));
             float v = 0.0f;
             int st = 0;
-            for (int sw = 0; sw < P.nsubwalks; ++sw) v += grad_dude2(*sc, bc, P, G, x, e, rng, st);
+            for (int sw = 0; sw < P.nsubwalks; ++sw) {
+                if (D == 2)
+                    v += grad_dude2(*sc, bc, P, G, v2(pts[2 * i], pts[2 * i + 1]), v2(ev[2 * i], ev[2 * i + 1]), rng,
+                                    st);
+                else
+                    v += grad_dude3(*sc, bc, P, G, v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]),
+                                    v3(ev[3 * i], ev[3 * i + 1], ev[3 * i + 2]), rng, st);
+            }
             v = v / (float)P.nsubwalks;
             acc += (double)v;
             steps += (uint64_t)st;
@@ -3342,6 +3739,21 @@ void orc_grad_trace_step(void* s, const float* pts, int64_t n, const int32_t* ip
 void orc_star_radius2_rho(void* s, int sub, const float* pts, const float* ev, int64_t n, float rho_max, float* tree,
                           float* brute) {
     Scene* sc = (Scene*)s;
+    if (sc->dim == 3) {  // Mesh.star_radius_2 (mesh.py:349-361) vs star_radius_2_baseline (:363-380)
+        const Sub<3>& S = sc->s3[sub];
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int64_t i = 0; i < n; ++i) {
+            V3 x = v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]), e = v3(ev[3 * i], ev[3 * i + 1], ev[3 * i + 2]);
+            tree[i] = star_radius3_rho(S, x, e, rho_max);
+            float r0 = star_radius<3>(S, &x.x, kInf, nullptr);
+            float m[3];
+            for (int d = 0; d < 3; ++d) m[d] = fmaxf_(std::fabs((&x.x)[d] - S.bb_min[d]), std::fabs((&x.x)[d] - S.bb_max[d]));
+            float R = fminf_(norm(v3(m[0], m[1], m[2])), r0);
+            for (size_t j = 0; j < S.tree.prims.size(); ++j) R = fminf_(R, tri_star_radius2(S.tree.prims[j], x, e, rho_max, R));
+            brute[i] = R;
+        }
+        return;
+    }
     const Sub<2>& S = sc->s2[sub];
 #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t i = 0; i < n; ++i) {

@@ -71,3 +71,19 @@ def test_rho_star_radius_tree_vs_loop(oracle, star):
         t, b = sc.star_radius2_rho(sub, q, e)
         assert np.all(t >= b)
         assert np.mean(t.view(np.uint32) == b.view(np.uint32)) > 0.995
+
+
+def test_rho_star_radius_3d_tree_vs_loop(oracle):
+    """WoStGrad3D's star radius (Mesh.star_radius_2 -> SNCH3D.star_radius_3) only
+    prunes against the reference's loop over all triangles
+    (star_radius_2_baseline, mesh.py:363-380): never less, and the same bits
+    almost always (SNCHNode3D.visit can prune a node the loop counts)."""
+    from dwost_amd import scenes
+
+    spec = scenes.icosphere_scene(level=3, n_query=512)
+    sc = oracle.OracleScene(spec.vertices, spec.indices, np.zeros_like(spec.types))
+    e = np.random.default_rng(2).standard_normal(spec.queries.shape)
+    e = (e / np.linalg.norm(e, axis=1, keepdims=True)).astype(np.float32)
+    t, b = sc.star_radius2_rho(0, spec.queries, e)
+    assert np.all(t >= b)
+    assert np.mean(t.view(np.uint32) == b.view(np.uint32)) > 0.99

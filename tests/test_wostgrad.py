@@ -133,3 +133,39 @@ def test_trace_graph_replays_match_eager(gpu, solver):
             assert torch.equal(out[k].view(torch.int32), ref[k].view(torch.int32)), (rep, k)
         sol.solve_dude(q[:8], q[:8], sc, seed=rep) if solver == "wostgrad" else sol.solve(q[:8], sc, seed=rep)
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("code,coeffs,kw", [(2, (0, 1, 0, 0), {}), (1, (0, 0, 0, 0), {}),
+                                            (0, (0, 0, 0, 0), dict(min_R=1e-2, nsubwalks=2)),
+                                            (2, (0, 0.3, -0.5, 0.8), dict(ignore_source=True, rho_max=2.0))])
+def test_wostgrad3d_matches_oracle(gpu, oracle, code, coeffs, kw):
+    """WoStGrad3D on an all-Neumann noisy icosphere (the reference example's
+    setting): per-walk bits and step counts equal the oracle's."""
+    from dwost_amd import scenes
+
+    spec = scenes.icosphere_scene(level=2, n_query=64)
+    types = np.zeros_like(spec.types)
+    sc = gpu.Scene3D(spec.vertices, spec.indices, types, gpu.BoundaryCondition(code, coeffs))
+    os_ = oracle.OracleScene(spec.vertices, spec.indices, types)
+    q = np.ascontiguousarray(spec.queries[:16])
+    e = np.random.default_rng(7).standard_normal((16, 3))
+    e = (e / np.linalg.norm(e, axis=1, keepdims=True)).astype(np.float32)
+    sol = gpu.WoStGrad3D(eps=1e-3, nwalks=16, nsteps=1024, **kw)
+    g = sol.solve_dude(torch.as_tensor(q).cuda(), torch.as_tensor(e).cuda(), sc, seed=3, per_walk=True,
+                       count_steps=True)
+    torch.cuda.synchronize()
+    o = os_.grad_dude(q, e, code, 16, nsteps=1024, eps=1e-3, min_R=sol.min_R, cv=sol.control_variates, seed=3,
+                      nsubwalks=sol.nsubwalks, rho_max=sol.rho_max, ignore_source=sol.ignore_source, coeffs=coeffs,
+                      per_walk=True)
+    gw, gs = g["walk_dude"].cpu().numpy(), g["walk_steps"].cpu().numpy()
+    same = (gw.view(np.uint32) == o["walk_dude"].view(np.uint32)) & (gs == o["walk_steps"])
+    print(f"3D code {code} {kw}: {same.sum()} of {same.size} walks identical; mean steps {gs.mean():.1f}")
+    assert same.all(), np.flatnonzero(~same)[:20].tolist()
+    assert np.allclose(g["dude"].cpu().numpy(), o["dude"], rtol=1e-5, atol=1e-5)
+
+
+def test_wostgrad3d_needs_all_neumann(gpu, ico_spec):
+    sc = gpu.Scene3D(ico_spec.vertices, ico_spec.indices, ico_spec.types, gpu.BoundaryCondition(0))
+    q = torch.as_tensor(ico_spec.queries[:4]).cuda()
+    with pytest.raises(gpu.DwostError):
+        gpu.WoStGrad3D(nwalks=4).solve_dude(q, q, sc)

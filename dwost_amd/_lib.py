@@ -33,7 +33,8 @@ SCENE_HOST_ONLY = 1
 
 class Boundary(C.Structure):
     _fields_ = [("bc_code", C.c_int32), ("coeffs", C.c_float * 4), ("g_vertex", C.c_void_p),
-                ("h_vertex", C.c_void_p), ("src_code", C.c_int32), ("src_value", C.c_float)]
+                ("h_vertex", C.c_void_p), ("src_code", C.c_int32), ("src_value", C.c_float),
+                ("src_coeffs", C.c_float * 4)]
 
 
 class SolverParams(C.Structure):

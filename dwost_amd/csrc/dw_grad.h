@@ -378,7 +378,7 @@ DWD float grad_dudn2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, co
     float rr = __builtin_sqrtf(rng.next());  // uniform_in_disk (math.py:285-288)
     float th = kTwoPi * rng.next();
     F2 p_src = c + f2(rr * cr_cos(th), rr * cr_sin(th)) * R;
-    float f_ref = src_f(bc), f = src_f(bc);
+    float f_ref = src_f(bc, ip), f = src_f(bc, p_src);
     float x2 = dot(ip - c, ip - c);
     float y2 = dot(p_src - c, p_src - c);
     float Pq = (x2 - y2) / fmx(dot(ip - p_src, ip - p_src), 1e-6f) * R / 2.0f;
@@ -431,7 +431,7 @@ DWD float grad_dude2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, co
                             F2 tg = f2(-b.n.y, b.n.x);
                             float e_t = dot(e, tg), e_n = dot(e, b.n);
                             float f_tn = grad_dhdt2(bc, b.p, b.n);
-                            res += e_t * f_tn - e_n * src_f(bc);
+                            res += e_t * f_tn - e_n * src_f(bc, b.p);
                         }
                         result += thr * alpha * (-Gv * res / b.pdf);
                     }
@@ -470,7 +470,8 @@ DWD float grad_dude2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, co
                 float sr_, spdf;
                 greens2_sample(_p, R, dir, rng, sp, sr_, spdf);
                 float res = 0.0f;
-                if (norm(sp - _p) < __builtin_fabsf(its.d)) res += (R * R / 4.0f) * dot(f2(0.0f, 0.0f), e);
+                F3 df = src_grad(bc);
+                if (norm(sp - _p) < __builtin_fabsf(its.d)) res += (R * R / 4.0f) * dot(f2(df.x, df.y), e);
                 result += thr * res;
             }
             if (ob) {
@@ -743,7 +744,7 @@ DWD float grad_dude3(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, co
                         bc_hess3(bc, bp, H);
                         F3 He = f3(dot(H[0], e_t), dot(H[1], e_t), dot(H[2], e_t));
                         float f_tn = dot(He, bn);
-                        res += f_tn - e_n * src_f(bc);
+                        res += f_tn - e_n * src_f(bc, bp);
                         if (P.cv) res -= dot(f3(0.0f, 0.0f, 0.0f), bn);
                     }
                     result += alpha * (-Gv * res / bpdf);
@@ -799,7 +800,7 @@ DWD float grad_dude3(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, co
             float sr_, spdf;
             greens3_sample(_p, R, dir, rng, spt, sr_, spdf);
             float res = 0.0f;
-            if (norm(spt - _p) < its.d) res += (R * R / 6.0f) * dot(f3(0.0f, 0.0f, 0.0f), e);
+            if (norm(spt - _p) < its.d) res += (R * R / 6.0f) * dot(src_grad(bc), e);
             result += res;
         }
         if (ob) {

@@ -165,6 +165,7 @@ struct BCDev {
     const float* hv;
     int src_code;
     float src_c;
+    float sc[4];  // DW_SRC_LINEAR: f = sc0 + sc1 x + sc2 y + sc3 z
 };
 struct ParamsDev {
     int nwalks, nsteps, nsubwalks, cv, av;

@@ -23,7 +23,18 @@ DWD F3 bc_grad(const BCDev& bc, F3 p) {
     if (bc.code == DW_BC_QUAD) return f3(2.0f * p.x, -2.0f * p.y, 0.0f);
     return f3(bc.c[1], bc.c[2], bc.c[3]);
 }
-DWD float src_f(const BCDev& bc) { return bc.src_code == DW_SRC_CONST ? bc.src_c : 0.0f; }
+// the source f of the Poisson problem (the reference's scene.f / scene.dfdx):
+// zero, a constant, or linear f = s0 + s1 x + s2 y + s3 z (2D: z = 0)
+DWD float src_f(const BCDev& bc, F3 p) {
+    if (bc.src_code == DW_SRC_CONST) return bc.src_c;
+    if (bc.src_code == DW_SRC_LINEAR) return ((bc.sc[0] + bc.sc[1] * p.x) + bc.sc[2] * p.y) + bc.sc[3] * p.z;
+    return 0.0f;
+}
+DWD float src_f(const BCDev& bc, F2 p) { return src_f(bc, f3(p.x, p.y, 0.0f)); }
+DWD F3 src_grad(const BCDev& bc) {
+    if (bc.src_code == DW_SRC_LINEAR) return f3(bc.sc[1], bc.sc[2], bc.sc[3]);
+    return f3(0.0f, 0.0f, 0.0f);
+}
 
 DWD float g2(const BCDev& bc, const SubDev& S, const CP& c) {
     if (bc.code != DW_BC_PER_VERTEX) return bc_u(bc, c.p);
@@ -381,7 +392,7 @@ DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Wa
     float sr_, spdf;
     greens2_sample(_p, w.R, dir, rng, sp, sr_, spdf);
     float src = 0.0f;
-    if (norm(sp - _p) < its.d) src += (w.R * w.R / 4.0f) * src_f(bc);
+    if (norm(sp - _p) < its.d) src += (w.R * w.R / 4.0f) * src_f(bc, sp);
     w.result += src;
     w.i += 1;
     return step_done(w, P);
@@ -521,7 +532,7 @@ DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Wa
     float sr_, spdf;
     greens3_sample(_p, w.R, dir, rng, sp, sr_, spdf);
     float src = 0.0f;
-    if (norm(sp - _p) < its.d) src += (w.R * w.R / 6.0f) * src_f(bc);
+    if (norm(sp - _p) < its.d) src += (w.R * w.R / 6.0f) * src_f(bc, sp);
     w.result += src;
     w.i += 1;
     return step_done(w, P);

@@ -743,8 +743,8 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
 #define ITEM(f) (B ? items[it].f : one.f)
     int it = 0;
     int64_t job = -1;
-    V p, dir, gs;
-    float R = 0.0f, up = 0.0f;
+    V p, dir, srcg;  // srcg: the pair's source term of grad u, (s+ - s-) * gs
+    float R = 0.0f, up = 0.0f, s_sum = 0.0f;  // s_sum = s+ + s-
     uint64_t saved = 0;
     int phase = 0, steps = 0;
     bool kept_plus = false;
@@ -768,15 +768,23 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
         seed_stream(rng, ITEM(P), i, npairs, job - i * npairs);
         V p_src;
         float r_src, pdf_src;
+        // the source at the sampled point and at its mirror 2p - p_src (the
+        // antithetic sample, wost.py:62-77)
         if constexpr (D == 2) {
             dir = uniform_dir2(rng);
             F2 dir_src = uniform_dir2(rng);
             greens2_sample(p, R, dir_src, rng, p_src, r_src, pdf_src);
-            gs = greens2_gradient(p, R, r_src, p_src) / (pdf_src * (R * R / 4.0f));
+            V gs = greens2_gradient(p, R, r_src, p_src) / (pdf_src * (R * R / 4.0f));
+            float sp = (R * R / 4.0f) * src_f(ITEM(bc), p_src), sm = (R * R / 4.0f) * src_f(ITEM(bc), p * 2.0f - p_src);
+            srcg = gs * (sp - sm);
+            s_sum = sp + sm;
         } else {
             dir = uniform_sphere(rng);
             greens3_sample(p, R, dir, rng, p_src, r_src, pdf_src);
-            gs = greens3_gradient(p, R, r_src, p_src) / (pdf_src * (R * R / 6.0f));
+            V gs = greens3_gradient(p, R, r_src, p_src) / (pdf_src * (R * R / 6.0f));
+            float sp = (R * R / 6.0f) * src_f(ITEM(bc), p_src), sm = (R * R / 6.0f) * src_f(ITEM(bc), p * 2.0f - p_src);
+            srcg = gs * (sp - sm);
+            s_sum = sp + sm;
         }
         saved = rng.state;
         walk_init<D>(w, p + dir * R);
@@ -799,22 +807,18 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
         }
         float um = v;
         float* o = ITEM(pair_vals) + job * (D + 1);
-        const float fsrc = src_f(ITEM(bc));
         if constexpr (D == 2) {
-            float s = (R * R / 4.0f) * fsrc;
             float k = 2.0f / R * (up - um);
-            float g0 = k * dir.x + (s - s) * gs.x, g1 = k * dir.y + (s - s) * gs.y;
-            o[0] = (up + um) + (s + s);
+            float g0 = k * dir.x + srcg.x, g1 = k * dir.y + srcg.y;
+            o[0] = (up + um) + s_sum;
             o[1] = bad(g0) ? 0.0f : g0;
             o[2] = bad(g1) ? 0.0f : g1;
         } else {
             float u1 = (bad(up) || __builtin_fabsf(up) > 10000.0f) ? 0.0f : up;
             float u2 = (bad(um) || __builtin_fabsf(um) > 10000.0f) ? 0.0f : um;
-            float s = (R * R / 6.0f) * fsrc;
             float k = 3.0f / R * (u1 - u2);
-            float g0 = k * dir.x + (s - s) * gs.x, g1 = k * dir.y + (s - s) * gs.y,
-                  g2_ = k * dir.z + (s - s) * gs.z;
-            o[0] = (u1 + u2) + (s + s);
+            float g0 = k * dir.x + srcg.x, g1 = k * dir.y + srcg.y, g2_ = k * dir.z + srcg.z;
+            o[0] = (u1 + u2) + s_sum;
             o[1] = bad(g0) ? 0.0f : g0;
             o[2] = bad(g1) ? 0.0f : g1;
             o[3] = bad(g2_) ? 0.0f : g2_;
@@ -1038,12 +1042,13 @@ struct DudeRun {
     }
     DWD void begin_walk(const ParamsDev& P, const BCDev& bc, Walk<D>& w, Rng& rng) {
         const float gnorm = D == 2 ? R * R / 4.0f : R * R / 6.0f;
-        float src = gnorm * src_f(bc), src_grad;
+        float src, src_grad;
         if (sums(P)) {
             if (it == 1) {
                 dir_src = -dir_src;
                 p_src = p * 2.0f - p_src;
             }
+            src = gnorm * src_f(bc, p_src);
             if constexpr (D == 2)
                 src_grad = dot(greens2_gradient(p, R, r_src, p_src), ev) / (pdf_src * gnorm);
             else
@@ -1057,6 +1062,7 @@ struct DudeRun {
                 dir = -dir;
                 p_src = p * 2.0f - p_src;
             }
+            src = gnorm * src_f(bc, p_src);
             if constexpr (D == 3) {
                 src_grad = dot(greens3_gradient(p, R, r_src, p_src), ev) / (pdf_src * gnorm);
                 src_acc = src_acc * (src_count / (src_count + 1.0f)) + src / (src_count + 1.0f);
@@ -1224,7 +1230,6 @@ __global__ void __launch_bounds__(kBlock) k_fused(SceneDev sc, BCDev bc, ParamsD
     float R = Rp[i];
     Rng rng;
     seed_stream(rng, P, i, npairs, lane - i * npairs);
-    float fsrc = src_f(bc);
     int st1, st2, tm;
     if (D == 2) {
         F2 p = f2(pts[2 * i], pts[2 * i + 1]);
@@ -1238,7 +1243,7 @@ __global__ void __launch_bounds__(kBlock) k_fused(SceneDev sc, BCDev bc, ParamsD
         float up = walk_u2(sc, bc, P, p + dir * R, rng, st1, tm);
         rng.state = state;
         float um = walk_u2(sc, bc, P, p - dir * R, rng, st2, tm);
-        float s_p = gnorm * fsrc, s_m = gnorm * fsrc;
+        float s_p = gnorm * src_f(bc, p_src), s_m = gnorm * src_f(bc, p * 2.0f - p_src);
         F2 gs = greens2_gradient(p, R, r_src, p_src) / (pdf_src * gnorm);
         float k = 2.0f / R * (up - um);
         float g0 = k * dir.x + (s_p - s_m) * gs.x, g1 = k * dir.y + (s_p - s_m) * gs.y;
@@ -1258,7 +1263,7 @@ __global__ void __launch_bounds__(kBlock) k_fused(SceneDev sc, BCDev bc, ParamsD
         float um = walk_u3(sc, bc, P, p - dir * R, rng, st2, tm);
         if (bad(up) || __builtin_fabsf(up) > 10000.0f) up = 0.0f;
         if (bad(um) || __builtin_fabsf(um) > 10000.0f) um = 0.0f;
-        float s_p = gnorm * fsrc, s_m = gnorm * fsrc;
+        float s_p = gnorm * src_f(bc, p_src), s_m = gnorm * src_f(bc, p * 2.0f - p_src);
         F3 gs = greens3_gradient(p, R, r_src, p_src) / (pdf_src * gnorm);
         float k = 3.0f / R * (up - um);
         float g0 = k * dir.x + (s_p - s_m) * gs.x, g1 = k * dir.y + (s_p - s_m) * gs.y,
@@ -1294,7 +1299,7 @@ DWD float dude2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, F2 p, F
                 dir_src = -dir_src;
                 p_src = p * 2.0f - p_src;
             }
-            float src = gnorm * src_f(bc);
+            float src = gnorm * src_f(bc, p_src);
             float src_grad = dot(greens2_gradient(p, R, r_src, p_src), ev) / (pdf_src * gnorm);
             src_sum += src;
             src_count += 1.0f;
@@ -1336,7 +1341,7 @@ DWD float dude3(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, F3 p, F
                 dir = -dir;
                 p_src = p * 2.0f - p_src;
             }
-            float src = gnorm * src_f(bc);
+            float src = gnorm * src_f(bc, p_src);
             float src_grad = dot(greens3_gradient(p, R, r_src, p_src), ev) / (pdf_src * gnorm);
             src_mean = src_mean * (src_count / (src_count + 1.0f)) + src / (src_count + 1.0f);
             src_count += 1.0f;
@@ -1404,7 +1409,7 @@ __global__ void __launch_bounds__(kBlock) k_dude(SceneDev sc, BCDev bc, ParamsDe
                     dir_src = -dir_src;
                     p_src = p * 2.0f - p_src;
                 }
-                float src = gnorm * src_f(bc);
+                float src = gnorm * src_f(bc, p_src);
                 float src_grad;
                 if (D == 2) {
                     F2 gr = greens2_gradient(f2(p.x, p.y), R, r_src, f2(p_src.x, p_src.y));
@@ -1804,6 +1809,7 @@ BCDev to_bc(const dw_boundary* b) {
     bc.hv = b->h_vertex;
     bc.src_code = b->src_code;
     bc.src_c = b->src_value;
+    for (int k = 0; k < 4; ++k) bc.sc[k] = b->src_coeffs[k];
     return bc;
 }
 int check_bc(const dw_boundary* b) {
@@ -1811,7 +1817,8 @@ int check_bc(const dw_boundary* b) {
     if (b->bc_code < 0 || b->bc_code > 3) return fail(DW_ERR_ARG, "unknown boundary code");
     if (b->bc_code == DW_BC_PER_VERTEX && (!b->g_vertex || !b->h_vertex))
         return fail(DW_ERR_ARG, "per-vertex boundary needs g_vertex and h_vertex");
-    if (b->src_code != DW_SRC_ZERO && b->src_code != DW_SRC_CONST) return fail(DW_ERR_ARG, "unknown source code");
+    if (b->src_code != DW_SRC_ZERO && b->src_code != DW_SRC_CONST && b->src_code != DW_SRC_LINEAR)
+        return fail(DW_ERR_ARG, "unknown source code");
     return DW_OK;
 }
 ParamsDev to_params(const dw_solver_params* p) {

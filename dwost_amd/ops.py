@@ -7,7 +7,8 @@ so the operators trace under ``torch.compile`` / ``torch.export`` and report
 shapes without running.  A scene crosses the operator boundary as the int64
 address of its opaque ``dw_scene`` (``Scene2D.handle``); solver fields as an
 int list and a float list (``Solver._op_params``); boundary data as its code,
-coefficients, optional per-vertex ``g`` / ``h`` tensors and the source value.
+coefficients, optional per-vertex ``g`` / ``h`` tensors and the source as
+[code, four coefficients] (``BoundaryCondition.op_args``).
 
 ``solve_u`` and ``solve_fused`` are differentiable with respect to the
 per-vertex ``g`` and ``h``: the backward pass replays the walks of the
@@ -41,7 +42,7 @@ def params_c(ip: list[int], fp: list[float]) -> _lib.SolverParams:
                              int(d["spatial_order"]), int(d["schedule"]), int(d["gate"]))
 
 
-def boundary_c(bc_code: int, coeffs: list[float], g, h, source: float) -> _lib.Boundary:
+def boundary_c(bc_code: int, coeffs: list[float], g, h, source: list[float]) -> _lib.Boundary:
     b = _lib.Boundary()
     b.bc_code = int(bc_code)
     for k in range(4):
@@ -51,8 +52,10 @@ def boundary_c(bc_code: int, coeffs: list[float], g, h, source: float) -> _lib.B
             raise ValueError("per-vertex boundary data needs g and h")
         b.g_vertex = g.data_ptr()
         b.h_vertex = h.data_ptr()
-    b.src_code = 1 if source != 0.0 else 0
-    b.src_value = float(source)
+    b.src_code = int(source[0])  # [code, then the four coefficients (constant: the value first)]
+    b.src_value = float(source[1])
+    for k in range(4):
+        b.src_coeffs[k] = float(source[1 + k])
     return b
 
 
@@ -180,7 +183,7 @@ def _(scene, sub, dim, u):
 @custom_op("dwost::solve_u", mutates_args=())
 def solve_u(scene: int, pts: torch.Tensor, iparams: list[int], fparams: list[float], bwd_iparams: list[int],
             bwd_fparams: list[float], bc_code: int, bc_coeffs: list[float], g: torch.Tensor | None,
-            h: torch.Tensor | None, source: float) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+            h: torch.Tensor | None, source: list[float]) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
     """Solver.solve with WoSt.u / WoSt3D.u (or WoS): (u, walk_flags, steps).
     walk_flags (uint8, n * nwalks) marks the subwalks the outlier guard kept;
     steps (int64[1]) is the walk-step total.  bwd_* are the backward solver's
@@ -206,7 +209,7 @@ def _(scene, pts, iparams, fparams, bwd_iparams, bwd_fparams, bc_code, bc_coeffs
 @custom_op("dwost::solve_dude", mutates_args=())
 def solve_dude(scene: int, pts: torch.Tensor, e: torch.Tensor, iparams: list[int], fparams: list[float],
                bc_code: int, bc_coeffs: list[float], g: torch.Tensor | None, h: torch.Tensor | None,
-               source: float) -> torch.Tensor:
+               source: list[float]) -> torch.Tensor:
     """WoSt.solve_dude / WoSt3D.solve_dude: du/de per point."""
     n = pts.shape[0]
     out = torch.empty(n, device=pts.device)
@@ -224,7 +227,7 @@ def _(scene, pts, e, iparams, fparams, bc_code, bc_coeffs, g, h, source):
 @custom_op("dwost::solve_fused", mutates_args=())
 def solve_fused(scene: int, pts: torch.Tensor, iparams: list[int], fparams: list[float], bwd_iparams: list[int],
                 bwd_fparams: list[float], bc_code: int, bc_coeffs: list[float], g: torch.Tensor | None,
-                h: torch.Tensor | None, source: float) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor,
+                h: torch.Tensor | None, source: list[float]) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor,
                                                                 torch.Tensor]:
     """Fused u + grad u: (u, grad, pair_flags, steps).  pair_flags (uint8,
     n * ceil(nwalks / 2)): bit 0 / 1 set when walk + / - survived the outlier
@@ -250,7 +253,7 @@ def _(scene, pts, iparams, fparams, bwd_iparams, bwd_fparams, bc_code, bc_coeffs
 
 @custom_op("dwost::adjoint_u", mutates_args=())
 def adjoint_u(scene: int, pts: torch.Tensor, iparams: list[int], fparams: list[float], bc_code: int,
-              bc_coeffs: list[float], g: torch.Tensor, h: torch.Tensor, source: float, dl_du: torch.Tensor,
+              bc_coeffs: list[float], g: torch.Tensor, h: torch.Tensor, source: list[float], dl_du: torch.Tensor,
               walk_flags: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
     """Adjoint of solve_u for per-vertex g, h: (dL/dg, dL/dh)."""
     dev = pts.device
@@ -268,7 +271,7 @@ def _(scene, pts, iparams, fparams, bc_code, bc_coeffs, g, h, source, dl_du, wal
 
 @custom_op("dwost::adjoint_fused", mutates_args=())
 def adjoint_fused(scene: int, pts: torch.Tensor, iparams: list[int], fparams: list[float], bc_code: int,
-                  bc_coeffs: list[float], g: torch.Tensor, h: torch.Tensor, source: float, dl_du: torch.Tensor,
+                  bc_coeffs: list[float], g: torch.Tensor, h: torch.Tensor, source: list[float], dl_du: torch.Tensor,
                   dl_dgrad: torch.Tensor, pair_flags: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
     """Adjoint of solve_fused for per-vertex g, h: (dL/dg, dL/dh)."""
     dev = pts.device
@@ -287,7 +290,7 @@ def _(scene, pts, iparams, fparams, bc_code, bc_coeffs, g, h, source, dl_du, dl_
 # -- autograd: replay from seeds ------------------------------------------
 def _setup(ctx, inputs, output):
     scene, pts, ip, fp, bip, bfp, bc_code, coeffs, g, h, source = inputs
-    ctx.args = (scene, list(ip), list(fp), list(bip), list(bfp), bc_code, list(coeffs), source)
+    ctx.args = (scene, list(ip), list(fp), list(bip), list(bfp), bc_code, list(coeffs), list(source))
     ctx.save_for_backward(pts, g, h, output[-2])  # the guard flags of the forward walks
 
 

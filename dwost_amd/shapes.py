@@ -24,7 +24,7 @@ from . import _lib
 from ._lib import SUB_DIRICHLET, SUB_FULL, SUB_NEUMANN, check, lib, ptr, stream_handle
 
 BC_EXPCOS, BC_QUAD, BC_LINEAR, BC_PER_VERTEX = 0, 1, 2, 3
-SRC_ZERO, SRC_CONST = 0, 1
+SRC_ZERO, SRC_CONST, SRC_LINEAR = 0, 1, 2
 
 
 class BoundaryType(Enum):
@@ -41,14 +41,15 @@ class BoundaryCondition:
     ``code`` selects u: BC_EXPCOS (e^x cos y), BC_QUAD (x^2 - y^2),
     BC_LINEAR (c0 + c1 x + c2 y + c3 z) or BC_PER_VERTEX (``g`` / ``h`` float32
     CUDA tensors with one value per scene vertex, interpolated linearly along
-    segments and barycentrically over triangles).  ``source`` is a constant f
-    (the reference's ``scene.f``)."""
+    segments and barycentrically over triangles).  ``source`` is the
+    reference's ``scene.f``: a constant, or (s0, s1, s2[, s3]) for the linear
+    f = s0 + s1 x + s2 y + s3 z."""
 
     code: int = BC_EXPCOS
     coeffs: tuple = (0.0, 0.0, 0.0, 0.0)
     g: torch.Tensor | None = None
     h: torch.Tensor | None = None
-    source: float = 0.0
+    source: float | tuple = 0.0
 
     @staticmethod
     def exp_cos() -> "BoundaryCondition":
@@ -71,7 +72,15 @@ class BoundaryCondition:
         torch.ops.dwost solvers."""
         c = [float(self.coeffs[k]) if k < len(self.coeffs) else 0.0 for k in range(4)]
         pv = self.code == BC_PER_VERTEX
-        return int(self.code), c, self.g if pv else None, self.h if pv else None, float(self.source)
+        return int(self.code), c, self.g if pv else None, self.h if pv else None, self.source_list()
+
+    def source_list(self) -> list[float]:
+        """[source code, its four coefficients] (constant: the value first)."""
+        if isinstance(self.source, (tuple, list)):
+            s = [float(v) for v in self.source] + [0.0] * (4 - len(self.source))
+            return [float(SRC_LINEAR)] + s[:4]
+        v = float(self.source)
+        return [float(SRC_CONST if v != 0.0 else SRC_ZERO), v, 0.0, 0.0, 0.0]
 
     def to_c(self, n_vertices: int | None = None) -> _lib.Boundary:
         b = _lib.Boundary()
@@ -86,8 +95,11 @@ class BoundaryCondition:
                     raise ValueError(f"per-vertex {name} needs {n_vertices} values, got {t.numel()}")
             b.g_vertex = self.g.data_ptr()
             b.h_vertex = self.h.data_ptr()
-        b.src_code = SRC_CONST if self.source != 0.0 else SRC_ZERO
-        b.src_value = float(self.source)
+        src = self.source_list()
+        b.src_code = int(src[0])
+        b.src_value = src[1]
+        for k in range(4):
+            b.src_coeffs[k] = src[1 + k]
         return b
 
 

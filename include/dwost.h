@@ -48,7 +48,8 @@ extern "C" {
 #define DW_BC_PER_VERTEX 3 /* g, h interpolated from per-vertex buffers      */
 /* source codes (the reference's scene.f, wost.py:128-143) */
 #define DW_SRC_ZERO 0
-#define DW_SRC_CONST 1
+#define DW_SRC_CONST 1     /* f = src_value                                  */
+#define DW_SRC_LINEAR 2    /* f = s0 + s1 x + s2 y + s3 z  (src_coeffs)      */
 
 typedef struct dw_scene dw_scene;
 
@@ -72,7 +73,8 @@ typedef struct dw_boundary {
     const float* g_vertex;  /* DEVICE, n_vertices (DW_BC_PER_VERTEX)                */
     const float* h_vertex;  /* DEVICE, n_vertices (DW_BC_PER_VERTEX)                */
     int32_t src_code;       /* DW_SRC_*                                             */
-    float src_value;
+    float src_value;        /* DW_SRC_CONST                                         */
+    float src_coeffs[4];    /* DW_SRC_LINEAR                                        */
 } dw_boundary;
 
 typedef struct dw_solver_params { /* Solver / WoSt fields (solver.py:30-34, wost.py:17-19) */

@@ -44,7 +44,7 @@ def lib():
             C.c_void_p] * 5
         L.orc_ref_topology.restype = C.c_int32
         L.orc_ref_topology.argtypes = [C.c_void_p, C.c_int, C.c_int32]
-        common = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_float]
+        common = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         L.orc_solve_u.argtypes = [C.c_void_p, C.c_void_p, C.c_int64] + common + [C.c_void_p] * 5
         L.orc_solve_dude.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64] + common + [C.c_void_p] * 2
         L.orc_solve_fused.argtypes = [C.c_void_p, C.c_void_p, C.c_int64] + common + [C.c_void_p] * 7
@@ -59,12 +59,12 @@ def lib():
         L.orc_far_check.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
         L.orc_sdf.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
         L.orc_grad_trace_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int,
-                                          C.c_void_p, C.c_int, C.c_float, C.c_void_p, C.c_void_p, C.c_int32,
+                                          C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
                                           C.c_float] + [C.c_void_p] * 3
         L.orc_star_radius2_rho.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_float,
                                            C.c_void_p, C.c_void_p]
         L.orc_grad_dude.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int,
-                                    C.c_void_p, C.c_int, C.c_float, C.c_void_p, C.c_void_p] + [C.c_void_p] * 4
+                                    C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_void_p] * 4
         L.orc_sample_boundary.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64] + [C.c_void_p] * 5
     return _lib
 
@@ -75,6 +75,15 @@ def _p(a):
 
 def _f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _src4(src_c):
+    """The source's coefficients: a constant (SRC_CONST) or (s0, s1, s2, s3) of
+    the linear f = s0 + s1 x + s2 y + s3 z (SRC_LINEAR)."""
+    v = np.zeros(4, np.float32)
+    a = np.atleast_1d(np.asarray(src_c, np.float32))
+    v[: len(a)] = a
+    return v
 
 
 def tea(v0: int, v1: int, rounds: int = 4, key=None):
@@ -219,7 +228,7 @@ class OracleScene:
         ws = np.zeros(n * nwalks, np.int32) if per_walk else None
         wt = np.zeros(n * nwalks, np.int32) if per_walk else None
         bt = np.zeros(1, np.uint64) if count_bytes else None
-        lib().orc_solve_u(self.h, _p(pts), n, _p(ip), _p(fp), bc_code, _p(c), _p(gv), _p(hv), src_code, src_c,
+        lib().orc_solve_u(self.h, _p(pts), n, _p(ip), _p(fp), bc_code, _p(c), _p(gv), _p(hv), src_code, _p(_src4(src_c)),
                           _p(mean), _p(wv), _p(ws), _p(wt), _p(bt))
         out = dict(u=mean)
         if per_walk:
@@ -238,7 +247,7 @@ class OracleScene:
         out = np.zeros(n, np.float32)
         st = np.zeros(1, np.uint64)
         lib().orc_solve_dude(self.h, _p(pts), _p(e), n, _p(ip), _p(fp), bc_code, _p(c), _p(gv), _p(hv), src_code,
-                             src_c, _p(out), _p(st))
+                             _p(_src4(src_c)), _p(out), _p(st))
         return dict(dude=out, steps=int(st[0]))
 
     def star_radius2_rho(self, sub: int, pts, e, rho_max=1.2):
@@ -263,7 +272,7 @@ class OracleScene:
         wv = np.zeros(n * nwalks, np.float32) if per_walk else None
         ws = np.zeros(n * nwalks, np.int32) if per_walk else None
         st = np.zeros(1, np.uint64)
-        lib().orc_grad_dude(self.h, _p(pts), _p(e), n, _p(ip), _p(fp), bc_code, _p(c), src_code, src_c, _p(gp),
+        lib().orc_grad_dude(self.h, _p(pts), _p(e), n, _p(ip), _p(fp), bc_code, _p(c), src_code, _p(_src4(src_c)), _p(gp),
                             _p(gi), _p(out), _p(wv), _p(ws), _p(st))
         r = dict(dude=out, steps=int(st[0]))
         if per_walk:
@@ -282,7 +291,7 @@ class OracleScene:
         out = np.zeros((n, 2), np.float32)
         smp = np.zeros((n * samples_per_point, 2), np.float32)
         st = np.zeros(1, np.uint64)
-        lib().orc_grad_trace_step(self.h, _p(pts), n, _p(ip), _p(fp), bc_code, _p(c), src_code, src_c, _p(gp),
+        lib().orc_grad_trace_step(self.h, _p(pts), n, _p(ip), _p(fp), bc_code, _p(c), src_code, _p(_src4(src_c)), _p(gp),
                                   _p(gi), int(samples_per_point), float(step_size), _p(out), _p(smp), _p(st))
         return dict(pos=out, samples=smp, steps=int(st[0]))
 
@@ -298,7 +307,7 @@ class OracleScene:
         out = np.zeros((n, D), np.float32)
         smp = np.zeros((n * samples_per_point, D), np.float32)
         st = np.zeros(1, np.uint64)
-        lib().orc_trace_step(self.h, _p(pts), n, _p(ip), _p(fp), bc_code, _p(c), _p(gv), _p(hv), src_code, src_c,
+        lib().orc_trace_step(self.h, _p(pts), n, _p(ip), _p(fp), bc_code, _p(c), _p(gv), _p(hv), src_code, _p(_src4(src_c)),
                              int(samples_per_point), float(step_size), _p(out), _p(smp), _p(st))
         return dict(pos=out, samples=smp, steps=int(st[0]))
 
@@ -320,7 +329,7 @@ class OracleScene:
         ps = np.zeros(n * npairs, np.int32) if per_pair else None
         st = np.zeros(1, np.uint64)
         cnt = np.zeros(6, np.uint64) if count else None
-        lib().orc_solve_fused(self.h, _p(pts), n, _p(ip), _p(fp), bc_code, _p(c), _p(gv), _p(hv), src_code, src_c,
+        lib().orc_solve_fused(self.h, _p(pts), n, _p(ip), _p(fp), bc_code, _p(c), _p(gv), _p(hv), src_code, _p(_src4(src_c)),
                               _p(u), _p(g), _p(pu), _p(pg), _p(ps), _p(st), _p(cnt))
         out = dict(u=u, grad=g, steps=int(st[0]))
         if count:

@@ -661,7 +661,7 @@ static void snch_assign(Tree<D>& T, const std::vector<S>& sils, const std::vecto
 // Scenes (diff_wost/shapes/polyline.py, mesh.py)
 // ----------------------------------------------------------------------------
 enum { BC_EXPCOS = 0, BC_QUAD = 1, BC_LINEAR = 2, BC_PER_VERTEX = 3 };
-enum { SRC_ZERO = 0, SRC_CONST = 1 };
+enum { SRC_ZERO = 0, SRC_CONST = 1, SRC_LINEAR = 2 };
 
 template <int D>
 struct Sub {  // one Polyline / Mesh
@@ -1706,6 +1706,7 @@ struct BC {
     const float* hv;
     int src_code;
     float src_c;
+    float sc[4];  // SRC_LINEAR: f = sc0 + sc1 x + sc2 y + sc3 z
 };
 static inline float bc_u(const BC& bc, V3 p) {
     switch (bc.code) {
@@ -1724,7 +1725,16 @@ static inline V3 bc_grad(const BC& bc, V3 p) {
         default: return v3(bc.c[1], bc.c[2], bc.c[3]);
     }
 }
-static inline float src_f(const BC& bc) { return bc.src_code == SRC_CONST ? bc.src_c : 0.0f; }
+// the source f (the reference's scene.f): zero, a constant, or linear
+static inline float src_f(const BC& bc, V3 p) {
+    if (bc.src_code == SRC_CONST) return bc.src_c;
+    if (bc.src_code == SRC_LINEAR) return ((bc.sc[0] + bc.sc[1] * p.x) + bc.sc[2] * p.y) + bc.sc[3] * p.z;
+    return 0.0f;
+}
+static inline float src_f(const BC& bc, V2 p) { return src_f(bc, v3(p.x, p.y, 0.0f)); }
+static inline V3 src_grad(const BC& bc) {  // the reference's scene.dfdx
+    return bc.src_code == SRC_LINEAR ? v3(bc.sc[1], bc.sc[2], bc.sc[3]) : v3(0.0f, 0.0f, 0.0f);
+}
 
 // g on a Dirichlet closest-point record of sub-scene S
 template <int D>
@@ -1973,7 +1983,7 @@ static WalkOut wost_u2(const Scene& sc, const BC& bc, const Params& P, V2 x, Pcg
             float sr_, spdf;
             greens2_sample(_p, R, dir, rng, sp, sr_, spdf);
             float src = 0.0f;
-            if (norm(sub(sp, _p)) < its.d) src += (R * R / 4.0f) * src_f(bc);
+            if (norm(sub(sp, _p)) < its.d) src += (R * R / 4.0f) * src_f(bc, sp);
             result += src;
         }
         i += 1;
@@ -2068,7 +2078,7 @@ static WalkOut wost_u3(const Scene& sc, const BC& bc, const Params& P, V3 x, Pcg
             float sr_, spdf;
             greens3_sample(_p, R, dir, rng, sp, sr_, spdf);
             float src = 0.0f;
-            if (norm(sub(sp, _p)) < its.d) src += (R * R / 6.0f) * src_f(bc);
+            if (norm(sub(sp, _p)) < its.d) src += (R * R / 6.0f) * src_f(bc, sp);
             result += src;
         }
         i += 1;
@@ -2153,7 +2163,7 @@ static float dude2(const Scene& sc, const BC& bc, const Params& P, V2 p, V2 e, P
                 dir_src = neg(dir_src);
                 p_src = sub(mul(p, 2.0f), p_src);
             }
-            float src = gnorm * src_f(bc);
+            float src = gnorm * src_f(bc, p_src);
             float src_grad = dot(greens2_gradient(p, R, r_src, p_src), e) / (pdf_src * gnorm);
             src_sum += src;
             src_count += 1.0f;
@@ -2201,7 +2211,7 @@ static float dude3(const Scene& sc, const BC& bc, const Params& P, V3 p, V3 e, P
                 dir = neg(dir);
                 p_src = sub(mul(p, 2.0f), p_src);
             }
-            float src = gnorm * src_f(bc);
+            float src = gnorm * src_f(bc, p_src);
             float src_grad = dot(greens3_gradient(p, R, r_src, p_src), e) / (pdf_src * gnorm);
             src_mean = src_mean * (src_count / (src_count + 1.0f)) + src / (src_count + 1.0f);
             src_count += 1.0f;
@@ -2261,7 +2271,7 @@ static float wos_dude(const Scene& sc, const BC& bc, const Params& P, const floa
                 dir_src = neg(dir_src);
                 p_src = sub(mul(p, 2.0f), p_src);
             }
-            float src = gnorm * src_f(bc);
+            float src = gnorm * src_f(bc, p_src);
             float src_grad;
             if (D == 2) {
                 V2 gr = greens2_gradient(v2(p.x, p.y), R, r_src, v2(p_src.x, p_src.y));
@@ -2321,8 +2331,7 @@ static PairOut fused_pair(const Scene& sc, const BC& bc, const Params& P, const 
         rng.state = state;
         WalkOut wm = wost_u2(sc, bc, P, sub(p, mul(dir, R)), rng, false, v2(0, 0), cnt != nullptr);
         if (cnt) cnt->add(wp.cnt), cnt->add(wm.cnt);
-        float fsrc = src_f(bc);
-        float s_p = gnorm * fsrc, s_m = gnorm * fsrc;
+        float s_p = gnorm * src_f(bc, p_src), s_m = gnorm * src_f(bc, sub(mul(p, 2.0f), p_src));
         V2 gs = divs(greens2_gradient(p, R, r_src, p_src), pdf_src * gnorm);
         float du = wp.value - wm.value;
         float k = 2.0f / R * du;
@@ -2345,8 +2354,7 @@ static PairOut fused_pair(const Scene& sc, const BC& bc, const Params& P, const 
         float up = wp.value, um = wm.value;
         if (bad(up) || std::fabs(up) > 10000.0f) up = 0.0f;
         if (bad(um) || std::fabs(um) > 10000.0f) um = 0.0f;
-        float fsrc = src_f(bc);
-        float s_p = gnorm * fsrc, s_m = gnorm * fsrc;
+        float s_p = gnorm * src_f(bc, p_src), s_m = gnorm * src_f(bc, sub(mul(p, 2.0f), p_src));
         V3 gs = divs(greens3_gradient(p, R, r_src, p_src), pdf_src * gnorm);
         float k = 3.0f / R * (up - um);
         o.usum = (up + um) + (s_p + s_m);
@@ -2722,7 +2730,7 @@ static float grad_dudn2(const Scene& sc, const BC& bc, int cv, const CP& its, Pc
     float rr = std::sqrt(rng.next_float32());  // uniform_in_disk (math.py:285-288)
     float th = kTwoPi * rng.next_float32();
     V2 p_src = add(c, mul(v2(rr * cr_cos(th), rr * cr_sin(th)), R));
-    float f_ref = src_f(bc), f = src_f(bc);
+    float f_ref = src_f(bc, ip), f = src_f(bc, p_src);
     float x2 = dot(sub(ip, c), sub(ip, c));
     float y2 = dot(sub(p_src, c), sub(p_src, c));
     float Pq = (x2 - y2) / fmaxf_(dot(sub(ip, p_src), sub(ip, p_src)), 1e-6f) * R / 2.0f;
@@ -2776,7 +2784,7 @@ static float grad_dude2(const Scene& sc, const BC& bc, const Params& P, const Gr
                             float e_t = dot(e, tg), e_n = dot(e, b.n);
                             float f_tn = grad_dhdt2(bc, b.p, b.n);
                             if (G.plant == 3) f_tn = -f_tn;
-                            res += e_t * f_tn - e_n * src_f(bc);
+                            res += e_t * f_tn - e_n * src_f(bc, b.p);
                         }
                         result += thr * alpha * (-Gv * res / b.pdf);
                     }
@@ -2815,7 +2823,8 @@ static float grad_dude2(const Scene& sc, const BC& bc, const Params& P, const Gr
                 float sr_, spdf;
                 greens2_sample(_p, R, dir, rng, sp, sr_, spdf);
                 float res = 0.0f;
-                if (norm(sub(sp, _p)) < std::fabs(its.d)) res += (R * R / 4.0f) * dot(v2(0.0f, 0.0f), e);
+                V3 df = src_grad(bc);
+                if (norm(sub(sp, _p)) < std::fabs(its.d)) res += (R * R / 4.0f) * dot(v2(df.x, df.y), e);
                 result += thr * res;
             }
             if (ob) {
@@ -3097,7 +3106,7 @@ static float grad_dude3(const Scene& sc, const BC& bc, const Params& P, const Gr
                         bc_hess3(bc, bp, H);
                         V3 He = v3(dot(H[0], e_t), dot(H[1], e_t), dot(H[2], e_t));
                         float f_tn = dot(He, bn);
-                        res += f_tn - e_n * src_f(bc);
+                        res += f_tn - e_n * src_f(bc, bp);
                         if (P.cv) res -= dot(v3(0.0f, 0.0f, 0.0f), bn);
                     }
                     result += alpha * (-Gv * res / bpdf);
@@ -3153,7 +3162,7 @@ static float grad_dude3(const Scene& sc, const BC& bc, const Params& P, const Gr
             float sr_, spdf;
             greens3_sample(_p, R, dir, rng, spt, sr_, spdf);
             float res = 0.0f;
-            if (norm(sub(spt, _p)) < its.d) res += (R * R / 6.0f) * dot(v3(0.0f, 0.0f, 0.0f), e);
+            if (norm(sub(spt, _p)) < its.d) res += (R * R / 6.0f) * dot(src_grad(bc), e);
             result += res;
         }
         if (ob) {
@@ -3488,14 +3497,16 @@ int32_t orc_ref_topology(void* s, int sub, int32_t ref) {
     return sc->dim == 2 ? sc->s2[sub].refs[ref] : sc->s3[sub].refs[ref];
 }
 
-static BC make_bc(int code, const float* c, const float* gv, const float* hv, int src_code, float src_c) {
+// src: the source's four coefficients (SRC_CONST: src[0]; SRC_LINEAR: all four)
+static BC make_bc(int code, const float* c, const float* gv, const float* hv, int src_code, const float* src) {
     BC bc;
     bc.code = code;
     for (int k = 0; k < 4; ++k) bc.c[k] = c ? c[k] : 0.0f;
     bc.gv = gv;
     bc.hv = hv;
     bc.src_code = src_code;
-    bc.src_c = src_c;
+    bc.src_c = src ? src[0] : 0.0f;
+    for (int k = 0; k < 4; ++k) bc.sc[k] = src ? src[k] : 0.0f;
     return bc;
 }
 static Params make_params(const int32_t* ip, const float* fp) {
@@ -3516,7 +3527,7 @@ static Params make_params(const int32_t* ip, const float* fp) {
 // Solver.solve: per-walk values/steps (n*W) and per-point means (Kahan, double)
 // ip = [nwalks, nsteps, nsubwalks, cv, av, seed], fp = [eps, min_R]
 void orc_solve_u(void* s, const float* pts, int64_t n, const int32_t* ip, const float* fp, int bc_code,
-                 const float* bc_c, const float* gv, const float* hv, int src_code, float src_c, float* mean,
+                 const float* bc_c, const float* gv, const float* hv, int src_code, const float* src_c, float* mean,
                  float* walk_val, int32_t* walk_steps, int32_t* walk_term, uint64_t* bytes_total) {
     Scene* sc = (Scene*)s;
     Params P = make_params(ip, fp);
@@ -3547,7 +3558,7 @@ void orc_solve_u(void* s, const float* pts, int64_t n, const int32_t* ip, const 
 
 // WoSt.solve_dude (wost.py:21-28): one stream per point
 void orc_solve_dude(void* s, const float* pts, const float* e, int64_t n, const int32_t* ip, const float* fp,
-                    int bc_code, const float* bc_c, const float* gv, const float* hv, int src_code, float src_c,
+                    int bc_code, const float* bc_c, const float* gv, const float* hv, int src_code, const float* src_c,
                     float* out, uint64_t* steps_total) {
     Scene* sc = (Scene*)s;
     Params P = make_params(ip, fp);
@@ -3582,7 +3593,7 @@ void orc_solve_dude(void* s, const float* pts, const float* e, int64_t n, const 
 // are averaged in sample order (the reference's scatter_reduce leaves the order
 // open).  samples[n*tn*dim] optional.
 void orc_trace_step(void* s, const float* pts, int64_t n, const int32_t* ip, const float* fp, int bc_code,
-                    const float* bc_c, const float* gv, const float* hv, int src_code, float src_c, int32_t tn,
+                    const float* bc_c, const float* gv, const float* hv, int src_code, const float* src_c, int32_t tn,
                     float step, float* out, float* samples, uint64_t* steps_total) {
     Scene* sc = (Scene*)s;
     Params P = make_params(ip, fp);
@@ -3648,7 +3659,7 @@ void orc_trace_step(void* s, const float* pts, int64_t n, const int32_t* ip, con
 // steps) optional.  gi[3] plants a mistake (tests only; 0: the reference).
 // 2D scenes only.
 void orc_grad_dude(void* s, const float* pts, const float* ev, int64_t n, const int32_t* ip, const float* fp,
-                   int bc_code, const float* bc_c, int src_code, float src_c, const float* gp, const int32_t* gi,
+                   int bc_code, const float* bc_c, int src_code, const float* src_c, const float* gp, const int32_t* gi,
                    float* out, float* walk_val, int32_t* walk_steps, uint64_t* steps_total) {
     Scene* sc = (Scene*)s;
     Params P = make_params(ip, fp);
@@ -3690,7 +3701,7 @@ void orc_grad_dude(void* s, const float* pts, const float* ev, int64_t n, const 
 // runs on PCG32 lane (base + i) * tn + j; walk_dude = the mean of nsubwalks
 // WoStGrad dudes.  out[n*2] = mean of the tn samples; samples[n*tn*2] optional.
 void orc_grad_trace_step(void* s, const float* pts, int64_t n, const int32_t* ip, const float* fp, int bc_code,
-                         const float* bc_c, int src_code, float src_c, const float* gp, const int32_t* gi, int32_t tn,
+                         const float* bc_c, int src_code, const float* src_c, const float* gp, const int32_t* gi, int32_t tn,
                          float step, float* out, float* samples, uint64_t* steps_total) {
     Scene* sc = (Scene*)s;
     Params P = make_params(ip, fp);
@@ -3773,7 +3784,7 @@ void orc_star_radius2_rho(void* s, int sub, const float* pts, const float* ev, i
 
 // Fused u + grad u.  pair_u[n*np], pair_g[n*np*dim], pair_steps[n*np] optional.
 void orc_solve_fused(void* s, const float* pts, int64_t n, const int32_t* ip, const float* fp, int bc_code,
-                     const float* bc_c, const float* gv, const float* hv, int src_code, float src_c, float* u,
+                     const float* bc_c, const float* gv, const float* hv, int src_code, const float* src_c, float* u,
                      float* grad, float* pair_u, float* pair_g, int32_t* pair_steps, uint64_t* steps_total,
                      uint64_t* counts) {
     Scene* sc = (Scene*)s;

@@ -64,7 +64,7 @@ def test_ctypes_structs_and_constants_match_header(tmp_path):
               "DW_BATCH_GRAPH": _lib.BATCH_GRAPH, "DW_SUB_FULL": _lib.SUB_FULL, "DW_SUB_DIRICHLET": _lib.SUB_DIRICHLET,
               "DW_SUB_NEUMANN": _lib.SUB_NEUMANN, "DW_BC_EXPCOS": shapes.BC_EXPCOS, "DW_BC_QUAD": shapes.BC_QUAD,
               "DW_BC_LINEAR": shapes.BC_LINEAR, "DW_BC_PER_VERTEX": shapes.BC_PER_VERTEX,
-              "DW_SRC_ZERO": shapes.SRC_ZERO, "DW_SRC_CONST": shapes.SRC_CONST,
+              "DW_SRC_ZERO": shapes.SRC_ZERO, "DW_SRC_CONST": shapes.SRC_CONST, "DW_SRC_LINEAR": shapes.SRC_LINEAR,
               "DW_RAY_REFERENCE": _lib.RAY_REFERENCE, "DW_RAY_NEAREST": _lib.RAY_NEAREST,
               "DW_NOALLOC_SCRATCH_BYTES": _lib.NOALLOC_SCRATCH_BYTES}
     lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "dwost.h"', "int main(void) {"]

@@ -23,7 +23,7 @@ def test_ops_registered_with_fake_shapes():
         assert hasattr(torch.ops.dwost, name), name
     ip, fp = dwost_amd.WoSt3D(nwalks=7)._op_params(3)
     p = torch.empty(5, 3, device="meta")
-    bc = (0, [0.0] * 4, None, None, 0.0)
+    bc = (0, [0.0] * 4, None, None, [0.0] * 5)
     u, g, fl, st = torch.ops.dwost.solve_fused(0, p, ip, fp, ip, fp, *bc)
     assert (u.shape, g.shape, fl.shape, fl.dtype, st.shape) == ((5,), (5, 3), (20,), torch.uint8, (1,))
     u, fl, st = torch.ops.dwost.solve_u(0, p, ip, fp, ip, fp, *bc)
@@ -31,7 +31,7 @@ def test_ops_registered_with_fake_shapes():
     assert [t.shape for t in torch.ops.dwost.closest_point(0, 0, p)] == [(5,), (5,), (5, 3), (5, 3), (5, 2)]
     assert torch.ops.dwost.sample_boundary(0, 2, 3, torch.empty(4, 3, device="meta"))[0].shape == (4, 3)
     gv = torch.empty(9, device="meta")
-    dg, dh = torch.ops.dwost.adjoint_fused(0, p, ip, fp, 3, [0.0] * 4, gv, gv, 0.0, torch.empty(5, device="meta"),
+    dg, dh = torch.ops.dwost.adjoint_fused(0, p, ip, fp, 3, [0.0] * 4, gv, gv, [0.0] * 5, torch.empty(5, device="meta"),
                                            torch.empty(5, 3, device="meta"), fl)
     assert dg.shape == (9,) and dh.shape == (9,)
 

@@ -1,15 +1,18 @@
 """Launch-shape autotuner for the persistent walk kernels.
 
-The walk kernels take four knobs that never change a result (each walk's
+The walk kernels take five knobs that never change a result (each walk's
 random stream is keyed by its (point, walk) index, not by the lane that runs
-it): threads per block (64 / 128 / 256), a cap on resident blocks per CU (the
-occupancy: fewer waves per SIMD means less L2 pressure per wave), whether the
-points are sorted by Morton cell and split into one range per XCD, and the
-lane scheduling (interleaved traversals or step-synchronous walks; the gate
-threshold of the interleaved loop is left at its default, flat within 2% over
-16-32 on the icosphere).  ``tune`` times every combination on a sample of the
-workload, checks that all of them return the same bits, and stores the fastest
-on the solver.
+it, and both record widths visit the same nodes in the same order): threads
+per block (64 / 128 / 256), a cap on resident blocks per CU (the occupancy:
+fewer waves per SIMD means less L2 pressure per wave), whether the points are
+sorted by Morton cell and split into one range per XCD, the lane scheduling
+(interleaved traversals or step-synchronous walks; the gate threshold of the
+interleaved loop is left at its default, flat within 2% over 16-32 on the
+icosphere) and the node width of the closest-point tree records (2: one binary
+level per dependent fetch, 4: two).  ``tune`` times every combination of the
+first four on a sample of the workload, then each node width with the fastest
+of them, checks that all runs return the same bits, and stores the fastest
+configuration on the solver.
 """
 
 from __future__ import annotations
@@ -27,6 +30,7 @@ BLOCK_SIZES = (64, 128, 256)
 BLOCKS_PER_CU = (0, 1, 2)  # 0: as many as the occupancy allows
 ORDERS = (1, 2)  # spatial order on / off
 SCHEDULES = (1, 2)  # interleaved / step-synchronous
+NODE_WIDTHS = (2, 4)
 
 _cache: dict = {}
 
@@ -43,29 +47,42 @@ def tune(solver: Solver, scene: _Scene, points, seed: int = 0, sample: int = 819
     key = _key(solver, scene, p.shape[0])
     if use_cache and key in _cache:
         best = _cache[key]
-        solver.block_size, solver.blocks_per_cu, solver.spatial_order, solver.schedule = best["config"]
+        _set(solver, best["config"])
         return best
     q = p[:sample].contiguous()
     results, ref = [], None
-    for bs, pc, order, sched in itertools.product(BLOCK_SIZES, BLOCKS_PER_CU, ORDERS, SCHEDULES):
-        s = replace(solver, block_size=bs, blocks_per_cu=pc, spatial_order=order, schedule=sched)
+
+    def time_config(cfg):
+        nonlocal ref
+        s = replace(solver)
+        _set(s, cfg)
         run = (lambda: s.solve_grad(q, scene, seed=seed)["u"]) if fused else (lambda: s.solve(q, scene, seed=seed))
         out = run()  # warm-up (allocations, first launch)
         torch.cuda.synchronize()
         if ref is None:
             ref = out.clone()
         elif not torch.equal(out.view(torch.int32), ref.view(torch.int32)):
-            raise RuntimeError(f"launch shape {(bs, pc, order, sched)} changed the result")
+            raise RuntimeError(f"launch configuration {cfg} changed the result")
         times = []
         for _ in range(reps):
             t0 = time.perf_counter()
             run()
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t0)
-        results.append(((bs, pc, order, sched), min(times)))
+        results.append((cfg, min(times)))
+
+    for bs, pc, order, sched in itertools.product(BLOCK_SIZES, BLOCKS_PER_CU, ORDERS, SCHEDULES):
+        time_config((bs, pc, order, sched, NODE_WIDTHS[0]))
+    lead = min(results, key=lambda r: r[1])[0]
+    for nw in NODE_WIDTHS[1:]:
+        time_config(lead[:4] + (nw,))
     results.sort(key=lambda r: r[1])
-    default = (256, 0, 1, 1 if scene.dim == 3 else 2)
+    default = (256, 0, 1, 1 if scene.dim == 3 else 2, 2)
     best = dict(config=results[0][0], seconds=results[0][1], seconds_default=dict(results)[default], table=results)
-    solver.block_size, solver.blocks_per_cu, solver.spatial_order, solver.schedule = best["config"]
+    _set(solver, best["config"])
     _cache[key] = best
     return best
+
+
+def _set(solver: Solver, cfg) -> None:
+    solver.block_size, solver.blocks_per_cu, solver.spatial_order, solver.schedule, solver.node_width = cfg

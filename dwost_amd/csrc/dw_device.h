@@ -279,7 +279,7 @@ struct VecT<3> {
 // primitive's slot is kept: *_end recomputes its record with the same
 // arithmetic (the primitive tests are pure functions of query and primitive).
 // ---------------------------------------------------------------------------
-enum : int { TQ_CLOSEST = 0, TQ_SIL = 1, TQ_ISECT = 2 };
+enum : int { TQ_CLOSEST = 0, TQ_SIL = 1, TQ_ISECT = 2, TQ_CLOSEST_W = 3 };  // _W: two-level records
 struct Trav {
     const SubDev* S;
     const void* rec;    // child-in-parent records of the tree walked (BVH or SNCH)
@@ -312,10 +312,12 @@ DWD T gload(const void* base, int i) {
 // tightened by each d_max, nearer child first, `cd > r_max` skips.
 // r0 < inf starts the pruning radius at a known bound on the answer (the
 // reference query starts at inf and pushes the root at that distance).
-DWD void closest2_begin(Trav& t, const SubDev& S, F2 p, float r0 = kInf) {
-    t.kind = TQ_CLOSEST;
+// wide: read the two-level records (node width 4), which visit the same nodes in
+// the same order with the same arithmetic.
+DWD void closest2_begin(Trav& t, const SubDev& S, F2 p, float r0 = kInf, bool wide = false) {
+    t.kind = wide ? TQ_CLOSEST_W : TQ_CLOSEST;
     t.S = &S;
-    t.rec = S.rec_bvh;
+    t.rec = wide ? S.rec_wide : S.rec_bvh;
     t.prims = S.prims;
     t.x = f3(p.x, p.y, 0.0f);
     t.d = kInf;
@@ -325,49 +327,92 @@ DWD void closest2_begin(Trav& t, const SubDev& S, F2 p, float r0 = kInf) {
     t.cd = r0 < kInf ? 0.0f : kInf;
     t.sp = 0;
 }
+// A 2D closest-point leaf: the nearest of its segments (first on ties).
+DWD void closest2_leaf(Trav& t, F2 p) {
+    int off, cnt;
+    leaf_range(*t.S, t.ref, off, cnt);
+    for (int j = 0; j < cnt; ++j) {
+        DSeg sg = gload<DSeg>(t.prims, off + j);
+        F2 a = f2(sg.ax, sg.ay), b = f2(sg.bx, sg.by);
+        F2 pa = p - a, ba = b - a;
+        float h = clampf(dot(pa, ba) / dot(ba, ba), 0.0f, 1.0f);
+        F2 i = a + ba * h;
+        float d = norm(i - p);
+        if (d < t.d) {
+            t.d = d;
+            t.best = off + j;
+        }
+    }
+    t.r_max = fmn(t.r_max, t.d);
+}
+// The two children of an interior node (boxes (min x, min y, max x, max y) and
+// references): both tested in order with r_max tightened by each d_max, the
+// nearer descended into (the left one on ties: `d_max_right < d_max_right`,
+// bvh.py:163, never swaps), the other pushed.  False when neither is hit.
+DWD bool closest2_pair(Trav& t, Ovf& o, F2 p, float4 bl, float4 br, int ref_l, int ref_r) {
+    float dminL, dmaxL, dminR, dmaxR;
+    box_dist(f2(bl.x, bl.y), f2(bl.z, bl.w), p, dminL, dmaxL);
+    bool hitL = dminL <= t.r_max;
+    t.r_max = fmn(t.r_max, dmaxL);
+    box_dist(f2(br.x, br.y), f2(br.z, br.w), p, dminR, dmaxR);
+    bool hitR = dminR <= t.r_max;
+    t.r_max = fmn(t.r_max, dmaxR);
+    if (hitL && hitR) {
+        bool sw = dminR < dminL;
+        stk_push(t.sp, o, sw ? ref_l : ref_r, sw ? dminL : dminR);
+        t.ref = sw ? ref_r : ref_l;
+        t.cd = sw ? dminR : dminL;
+        return true;
+    } else if (hitL) {
+        t.ref = ref_l;
+        t.cd = dminL;
+        return true;
+    } else if (hitR) {
+        t.ref = ref_r;
+        t.cd = dminR;
+        return true;
+    }
+    return false;
+}
 DWD bool closest2_iter(Trav& t, Ovf& o) {
-    const SubDev& S = *t.S;
     F2 p = f2(t.x.x, t.x.y);
     if (!(t.cd > t.r_max)) {
         if (t.ref < 0) {
-            int off, cnt;
-            leaf_range(S, t.ref, off, cnt);
-            for (int j = 0; j < cnt; ++j) {
-                DSeg sg = gload<DSeg>(t.prims, off + j);
-                F2 a = f2(sg.ax, sg.ay), b = f2(sg.bx, sg.by);
-                F2 pa = p - a, ba = b - a;
-                float h = clampf(dot(pa, ba) / dot(ba, ba), 0.0f, 1.0f);
-                F2 i = a + ba * h;
-                float d = norm(i - p);
-                if (d < t.d) {
-                    t.d = d;
-                    t.best = off + j;
-                }
-            }
-            t.r_max = fmn(t.r_max, t.d);
+            closest2_leaf(t, p);
         } else {
             RBvh2 rc = gload<RBvh2>(t.rec, t.ref);
-            float dminL, dmaxL, dminR, dmaxR;
-            box_dist(f2(rc.lmnx, rc.lmny), f2(rc.lmxx, rc.lmxy), p, dminL, dmaxL);
-            bool hitL = dminL <= t.r_max;
-            t.r_max = fmn(t.r_max, dmaxL);
-            box_dist(f2(rc.rmnx, rc.rmny), f2(rc.rmxx, rc.rmxy), p, dminR, dmaxR);
-            bool hitR = dminR <= t.r_max;
-            t.r_max = fmn(t.r_max, dmaxR);
-            if (hitL && hitR) {
-                bool sw = dminR < dminL;  // `d_max_right < d_max_right` (bvh.py:163) never swaps
-                stk_push(t.sp, o, sw ? rc.ref_l : rc.ref_r, sw ? dminL : dminR);
-                t.ref = sw ? rc.ref_r : rc.ref_l;
-                t.cd = sw ? dminR : dminL;
+            if (closest2_pair(t, o, p, make_float4(rc.lmnx, rc.lmny, rc.lmxx, rc.lmxy),
+                              make_float4(rc.rmnx, rc.rmny, rc.rmxx, rc.rmxy), rc.ref_l, rc.ref_r))
                 return false;
-            } else if (hitL) {
-                t.ref = rc.ref_l;
-                t.cd = dminL;
-                return false;
-            } else if (hitR) {
-                t.ref = rc.ref_r;
-                t.cd = dminR;
-                return false;
+        }
+    }
+    if (t.sp == 0) return true;
+    stk_pop(t.sp, o, t.ref, t.cd);
+    return false;
+}
+// The same query over the two-level records: a record's first pair is the
+// node's children; when the descent continues into an interior child, that
+// child's pair (in the same record) is tested in the same iteration, so two
+// binary levels cost one dependent fetch.  A descended-into child always
+// passes the pending-node test (its d_min is at most every d_max that
+// tightened r_max), so the node sequence, the stack and r_max evolve exactly
+// as in closest2_iter.
+DWD bool closest2w_iter(Trav& t, Ovf& o) {
+    F2 p = f2(t.x.x, t.x.y);
+    if (!(t.cd > t.r_max)) {
+        if (t.ref < 0) {
+            closest2_leaf(t, p);
+        } else {
+            const char* rec = (const char*)t.rec + (size_t)t.ref * sizeof(RWide2);
+            float4 b0 = gload<float4>(rec, 0), b1 = gload<float4>(rec, 1);
+            int4 r03 = gload<int4>(rec, 6);
+            if (closest2_pair(t, o, p, b0, b1, r03.x, r03.y)) {
+                if (t.ref < 0) return false;  // a leaf child: next iteration
+                // the chosen child's pair, from the line just fetched
+                const bool right = t.ref == r03.y;
+                float4 c0 = gload<float4>(rec, right ? 4 : 2), c1 = gload<float4>(rec, right ? 5 : 3);
+                int2 cr = gload<int2>(rec + 96, right ? 2 : 1);
+                if (closest2_pair(t, o, p, c0, c1, cr.x, cr.y)) return false;
             }
         }
     }
@@ -947,10 +992,10 @@ DWD void tri_closest(const DTri& tr, F3 p, F3& pt, float& u_out, float& v_out, f
     v_out = 1.0f - u0 - u1;
 }
 
-DWD void closest3_begin(Trav& t, const SubDev& S, F3 p) {
-    t.kind = TQ_CLOSEST;
+DWD void closest3_begin(Trav& t, const SubDev& S, F3 p, bool wide = false) {
+    t.kind = wide ? TQ_CLOSEST_W : TQ_CLOSEST;
     t.S = &S;
-    t.rec = S.rec_bvh;
+    t.rec = wide ? S.rec_wide : S.rec_bvh;
     t.prims = S.prims;
     t.x = p;
     t.d = kInf;
@@ -1013,6 +1058,35 @@ DWD bool isect3_begin(Trav& t, const SubDev& S, F3 x, F3 v, float r_max) {
     t.cd = tmin;
     return true;
 }
+// The two children of a 3D closest-point node (boxes and references): both
+// tested with r_max tightened by each d_max in order; the nearer descended into
+// (equal zero d_min: the nearer d_max first, snch3d.py:383-388), the other
+// pushed.  False when neither is hit.
+DWD bool closest3_pair(Trav& t, Ovf& o, F3 lmn, F3 lmx, F3 rmn, F3 rmx, int ref_l, int ref_r) {
+    float d0, d1, dmaxL, dmaxR;
+    box_dist(lmn, lmx, t.x, d0, dmaxL);
+    box_dist(rmn, rmx, t.x, d1, dmaxR);
+    bool h0 = d0 <= t.r_max;
+    t.r_max = fmn(t.r_max, dmaxL);
+    bool h1 = d1 <= t.r_max;
+    t.r_max = fmn(t.r_max, dmaxR);
+    bool sw = ((d0 == 0.0f) && (d1 == 0.0f) && (dmaxR < dmaxL)) || (d1 < d0);
+    if (h0 && h1) {
+        stk_push(t.sp, o, sw ? ref_l : ref_r, sw ? d0 : d1);
+        t.ref = sw ? ref_r : ref_l;
+        t.cd = sw ? d1 : d0;
+        return true;
+    } else if (h0) {
+        t.ref = ref_l;
+        t.cd = d0;
+        return true;
+    } else if (h1) {
+        t.ref = ref_r;
+        t.cd = d1;
+        return true;
+    }
+    return false;
+}
 // One iteration of a 3D query: closest point (snch3d.py:349-409), closest
 // silhouette (snch3d.py:594-663) or ray intersection (snch3d.py:411-476).  An
 // interior step of any kind fetches the record head (children's boxes and
@@ -1039,7 +1113,7 @@ DWD bool trav3_iter(Trav& t, Ovf& o) {
             } else {
                 for (int j = 0; j < cnt; ++j) {
                     DTri tr = gload<DTri>(t.prims, off + j);
-                    if (kind == TQ_CLOSEST) {
+                    if (kind != TQ_ISECT) {
                         F3 pt;
                         float u, v, d;
                         tri_closest(tr, t.x, pt, u, v, d);
@@ -1056,6 +1130,28 @@ DWD bool trav3_iter(Trav& t, Ovf& o) {
                         }
                     }
                 }
+            }
+        } else if (kind == TQ_CLOSEST_W) {
+            // two-level record: the node's pair, then (in the same fetch) the
+            // pair of the interior child descended into; node sequence, stack and
+            // r_max as in the binary iteration (a descended-into child always
+            // passes the pending-node test)
+            const char* rec = (const char*)t.rec + (size_t)t.ref * sizeof(RWide3);
+            // a pair of slots is 3 float4: (min.xyz, max.x | max.yz of the first,
+            // min.xy of the second | min.z, max.xyz of the second)
+            float4 q0 = gload<float4>(rec, 0), q1 = gload<float4>(rec, 1), q2 = gload<float4>(rec, 2);
+            int2 r01 = gload<int2>(rec + 144, 0);
+            if (closest3_pair(t, o, f3(q0.x, q0.y, q0.z), f3(q0.w, q1.x, q1.y), f3(q1.z, q1.w, q2.x),
+                              f3(q2.y, q2.z, q2.w), r01.x, r01.y)) {
+                if (t.ref < 0) return false;  // a leaf child: next iteration
+                // the chosen child's pair (slots 2, 3 or 4, 5), from the lines just fetched
+                const int k = t.ref == r01.y ? 1 : 0;
+                float4 a0 = gload<float4>(rec, 3 + 3 * k), a1 = gload<float4>(rec, 4 + 3 * k),
+                       a2 = gload<float4>(rec, 5 + 3 * k);
+                int2 cr = gload<int2>(rec + 144, 1 + k);
+                if (closest3_pair(t, o, f3(a0.x, a0.y, a0.z), f3(a0.w, a1.x, a1.y), f3(a1.z, a1.w, a2.x),
+                                  f3(a2.y, a2.z, a2.w), cr.x, cr.y))
+                    return false;
             }
         } else {
             const bool sn = kind == TQ_SIL;
@@ -1159,6 +1255,7 @@ template <int D, bool X>
 DWD bool trav_iter(Trav& t, Ovf& o) {
     if constexpr (D == 2) {
         if (t.kind == TQ_CLOSEST) return closest2_iter(t, o);
+        if (t.kind == TQ_CLOSEST_W) return closest2w_iter(t, o);
         if (t.kind == TQ_SIL) return sil2_iter<X>(t, o);
         return isect2_iter(t, o);
     } else {

@@ -74,6 +74,23 @@ struct alignas(16) RBvh2 {  // 48 B
     float rmnx, rmny, rmxx, rmxy;
     int ref_l, ref_r, pad0, pad1;
 };
+// Two-level BVH records (node width 4): one 128 B record per internal node
+// holds its two children and, for each internal child, that child's two
+// children.  Slots 0, 1 are the node's children (left, right); the children of
+// slot k (k = 0, 1) are slots 2k + 2 (left) and 2k + 3 (right).  Grandchild
+// slots of a leaf child are unused (zero).  Same reference encoding as RBvh2;
+// an internal reference indexes both the binary and the two-level records.
+struct alignas(16) RWide2 {  // 128 B
+    float b[6][4];  // per slot: min x, min y, max x, max y
+    int ref[6];
+    int pad0, pad1;
+};
+struct alignas(64) RWide3 {  // 192 B: the same slots with 3D boxes; every record spans two 128 B lines
+    float b[6][6];  // per slot: min x, y, z, max x, y, z
+    int ref[6];
+    int pad[6];
+};
+static_assert(sizeof(RWide2) == 128 && sizeof(RWide3) == 192, "two-level record layout");
 struct alignas(16) RSnch2 {  // 96 B
     float lmnx, lmny, lmxx, lmxy;
     float rmnx, rmny, rmxx, rmxy;
@@ -145,6 +162,7 @@ struct SubDev {
     // child-in-parent records and the root
     const void* rec_bvh;        // RBvh2 / RBvh3
     const void* rec_snch;       // RSnch2 / RSnch3
+    const void* rec_wide;       // RWide2 / RWide3 (two-level BVH records)
     const int* leaf_table;      // int2 (offset, count) per large leaf
     int root_ref_bvh, root_ref_snch;
     float root_mn[3], root_mx[3];
@@ -175,4 +193,5 @@ struct ParamsDev {
     int64_t point_base;  // global index of point 0 (stream keys)
     int block_size, blocks_per_cu;  // launch shape (0: defaults)
     int schedule, gate;             // lane scheduling of the persistent kernels
+    int node_width;                 // closest-point records: 2 binary, 4 two-level
 };

@@ -261,12 +261,12 @@ DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Wa
     if (w.stage == WS_START) {
         if (!(w.i < P.nsteps)) return ADV_END;
         if (P.flags & DW_SOLVE_WOS) {
-            closest2_begin(t, F, w.ip);
+            closest2_begin(t, F, w.ip, kInf, P.node_width == 4);
             w.stage = WS_WOS;
             return ADV_QUERY;
         }
         if (Dn.present) {
-            closest2_begin(t, Dn, w.ip);
+            closest2_begin(t, Dn, w.ip, kInf, P.node_width == 4);
             w.stage = WS_DIR;
             return ADV_QUERY;
         }
@@ -335,7 +335,7 @@ DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Wa
                                fmx(__builtin_fabsf(F.root_mn[1]), __builtin_fabsf(F.root_mx[1])));
                 r0 = w.d_dir * (1.0f + 0x1p-16f) + 1e-6f * (1.0f + sc);
             }
-            closest2_begin(t, F, w.ip, r0);
+            closest2_begin(t, F, w.ip, r0, P.node_width == 4);
             w.stage = WS_CV;
             return ADV_QUERY;
         }
@@ -407,7 +407,7 @@ DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Wa
     if (w.stage == WS_START) {
         if (!(w.i < P.nsteps)) return ADV_END;
         if (P.flags & DW_SOLVE_WOS) {
-            closest3_begin(t, F, w.ip);
+            closest3_begin(t, F, w.ip, P.node_width == 4);
             w.stage = WS_WOS;
             return ADV_QUERY;
         }
@@ -438,7 +438,7 @@ DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Wa
                 w.R = w.d_dir = lo;
                 w.stage = WS_STAR;
             } else {
-                closest3_begin(t, Dn, w.ip);
+                closest3_begin(t, Dn, w.ip, P.node_width == 4);
                 w.stage = WS_DIR;
                 return ADV_QUERY;
             }

@@ -1644,6 +1644,40 @@ int upload_records(dw_scene* s, const dwb::SubHost& H, SubDev& S) {
         if ((rc = upload(s, rs, &p))) return rc;
         S.rec_snch = p;
     }
+    // two-level records: slot k's children are slots 2k + 2, 2k + 3
+    auto wide = [&](auto& rw, auto put) {
+        for (size_t i = 0; i < nn; ++i) {
+            std::memset(&rw[i], 0, sizeof(rw[i]));
+            if (H.node_nref[i] > 0) continue;
+            int slot_node[6] = {(int)i + 1, (int)i + H.node_off[i], -1, -1, -1, -1};
+            for (int k = 0; k < 2; ++k) {
+                int c = slot_node[k];
+                if (H.node_nref[c] > 0) continue;
+                slot_node[2 * k + 2] = c + 1;
+                slot_node[2 * k + 3] = c + H.node_off[c];
+            }
+            for (int k = 0; k < 6; ++k) {
+                int c = slot_node[k];
+                if (c < 0) continue;
+                put(rw[i], k, c);
+                rw[i].ref[k] = bvh_ref(c);
+            }
+        }
+    };
+    if (D == 2) {
+        std::vector<RWide2> rw(nn);
+        wide(rw, [&](RWide2& r, int k, int c) {
+            for (int q = 0; q < 4; ++q) r.b[k][q] = box(c, q);
+        });
+        if ((rc = upload(s, rw, &p))) return rc;
+    } else {
+        std::vector<RWide3> rw(nn);
+        wide(rw, [&](RWide3& r, int k, int c) {
+            for (int q = 0; q < 6; ++q) r.b[k][q] = box(c, q);
+        });
+        if ((rc = upload(s, rw, &p))) return rc;
+    }
+    S.rec_wide = p;
     enc.table.push_back(0);  // never empty
     enc.table.push_back(0);
     if ((rc = upload(s, enc.table, &p))) return rc;
@@ -1837,6 +1871,7 @@ ParamsDev to_params(const dw_solver_params* p) {
     P.blocks_per_cu = p->blocks_per_cu;
     P.schedule = p->schedule;
     P.gate = p->gate;
+    P.node_width = p->node_width ? p->node_width : 2;
     return P;
 }
 int check_params(const dw_solver_params* p) {
@@ -1851,6 +1886,8 @@ int check_params(const dw_solver_params* p) {
     if (p->spatial_order < 0 || p->spatial_order > 2) return fail(DW_ERR_ARG, "spatial_order must be 0, 1 or 2");
     if (p->schedule < 0 || p->schedule > 2) return fail(DW_ERR_ARG, "schedule must be 0, 1 or 2");
     if (p->gate < 0 || p->gate > 64) return fail(DW_ERR_ARG, "gate must be in 0..64");
+    if (p->node_width != 0 && p->node_width != 2 && p->node_width != 4)
+        return fail(DW_ERR_ARG, "node_width must be 0, 2 or 4");
     return DW_OK;
 }
 

@@ -48,6 +48,9 @@ class Solver:
     spatial_order: int = 0  # 0 auto, 1 on, 2 off
     schedule: int = 0  # lane scheduling: 0 auto, 1 interleaved, 2 step-synchronous
     gate: int = 0  # interleaved schedule: waiting lanes per walk-logic pass (0: 24)
+    # closest-point tree records: 2 (binary, one level per fetch) or 4 (two
+    # levels per fetch); the same nodes in the same order (0: 2)
+    node_width: int = 0
 
     _walk_flag: ClassVar[int] = 0  # walk family (WoSt; WoS sets DW_SOLVE_WOS)
 
@@ -57,7 +60,8 @@ class Solver:
                                  int(self.antithetic_variates), int(seed) & 0xFFFFFFFF, float(self.eps),
                                  float(self.min_R), flags,
                                  int(point_base), int(self.block_size), int(self.blocks_per_cu),
-                                 int(self.spatial_order), int(self.schedule), int(self.gate))
+                                 int(self.spatial_order), int(self.schedule), int(self.gate),
+                                 int(self.node_width))
 
     def _op_params(self, seed: int, point_base: int = 0) -> tuple[list[int], list[float]]:
         """The int and float parameter lists of the torch.ops.dwost solvers."""

@@ -98,6 +98,10 @@ typedef struct dw_solver_params { /* Solver / WoSt fields (solver.py:30-34, wost
     int32_t schedule;       /* DW_SCHED_*: lane scheduling of the walk kernels     */
     int32_t gate;           /* interleaved schedule: waiting lanes of a wave that
                                trigger a walk-logic pass, 1..64 (0: 24)           */
+    int32_t node_width;     /* tree records of the walks' closest-point queries:
+                               2 (one binary level per fetch) or 4 (two levels per
+                               fetch); 0: the default, 2.  Same visit order, same
+                               results                                            */
 } dw_solver_params;
 
 #define DW_ORDER_AUTO 0 /* on from 8192 points */

@@ -42,7 +42,8 @@ class SolverParams(C.Structure):
                 ("control_variates", C.c_int32), ("antithetic_variates", C.c_int32), ("seed", C.c_uint32),
                 ("eps", C.c_float), ("min_R", C.c_float), ("flags", C.c_int32), ("point_base", C.c_int64),
                 ("block_size", C.c_int32), ("blocks_per_cu", C.c_int32), ("spatial_order", C.c_int32),
-                ("schedule", C.c_int32), ("gate", C.c_int32), ("node_width", C.c_int32)]
+                ("schedule", C.c_int32), ("gate", C.c_int32), ("node_width", C.c_int32),
+                ("walks_per_lane", C.c_int32)]
 
 NOALLOC_SCRATCH_BYTES = 64 << 20  # DW_NOALLOC_SCRATCH_BYTES
 class GradParams(C.Structure):

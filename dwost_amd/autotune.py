@@ -1,6 +1,6 @@
 """Launch-shape autotuner for the persistent walk kernels.
 
-The walk kernels take five knobs that never change a result (each walk's
+The walk kernels take six knobs that never change a result (each walk's
 random stream is keyed by its (point, walk) index, not by the lane that runs
 it, and both record widths visit the same nodes in the same order): threads
 per block (64 / 128 / 256), a cap on resident blocks per CU (the occupancy:
@@ -8,11 +8,13 @@ fewer waves per SIMD means less L2 pressure per wave), whether the points are
 sorted by Morton cell and split into one range per XCD, the lane scheduling
 (interleaved traversals or step-synchronous walks; the gate threshold of the
 interleaved loop is left at its default, flat within 2% over 16-32 on the
-icosphere) and the node width of the closest-point tree records (2: one binary
-level per dependent fetch, 4: two).  ``tune`` times every combination of the
-first four on a sample of the workload, then each node width with the fastest
-of them, checks that all runs return the same bits, and stores the fastest
-configuration on the solver.
+icosphere), the node width of the closest-point tree records (2: one binary
+level per dependent fetch, 4: two) and the walks per lane of the fused solve
+(walk pairs of one point a lane runs back to back per job it takes).  ``tune``
+times every combination of the first four on a sample of the workload, then
+each node width with the fastest of them, then each walks-per-lane value with
+the fastest so far (fused solves), checks that all runs return the same bits,
+and stores the fastest configuration on the solver.
 """
 
 from __future__ import annotations
@@ -31,6 +33,7 @@ BLOCKS_PER_CU = (0, 1, 2)  # 0: as many as the occupancy allows
 ORDERS = (1, 2)  # spatial order on / off
 SCHEDULES = (1, 2)  # interleaved / step-synchronous
 NODE_WIDTHS = (2, 4)
+WALKS_PER_LANE = (1, 2, 4)
 
 _cache: dict = {}
 
@@ -72,12 +75,16 @@ def tune(solver: Solver, scene: _Scene, points, seed: int = 0, sample: int = 819
         results.append((cfg, min(times)))
 
     for bs, pc, order, sched in itertools.product(BLOCK_SIZES, BLOCKS_PER_CU, ORDERS, SCHEDULES):
-        time_config((bs, pc, order, sched, NODE_WIDTHS[0]))
+        time_config((bs, pc, order, sched, NODE_WIDTHS[0], WALKS_PER_LANE[0]))
     lead = min(results, key=lambda r: r[1])[0]
     for nw in NODE_WIDTHS[1:]:
-        time_config(lead[:4] + (nw,))
+        time_config(lead[:4] + (nw, WALKS_PER_LANE[0]))
+    if fused:
+        lead = min(results, key=lambda r: r[1])[0]
+        for wl in WALKS_PER_LANE[1:]:
+            time_config(lead[:5] + (wl,))
     results.sort(key=lambda r: r[1])
-    default = (256, 0, 1, 1 if scene.dim == 3 else 2, 2)
+    default = (256, 0, 1, 1 if scene.dim == 3 else 2, 2, 1)
     best = dict(config=results[0][0], seconds=results[0][1], seconds_default=dict(results)[default], table=results)
     _set(solver, best["config"])
     _cache[key] = best
@@ -85,4 +92,5 @@ def tune(solver: Solver, scene: _Scene, points, seed: int = 0, sample: int = 819
 
 
 def _set(solver: Solver, cfg) -> None:
-    solver.block_size, solver.blocks_per_cu, solver.spatial_order, solver.schedule, solver.node_width = cfg
+    (solver.block_size, solver.blocks_per_cu, solver.spatial_order, solver.schedule, solver.node_width,
+     solver.walks_per_lane) = cfg

@@ -194,4 +194,5 @@ struct ParamsDev {
     int block_size, blocks_per_cu;  // launch shape (0: defaults)
     int schedule, gate;             // lane scheduling of the persistent kernels
     int node_width;                 // closest-point records: 2 binary, 4 two-level
+    int walks_per_lane;             // fused solve: walk pairs per job (>= 1)
 };

@@ -717,7 +717,9 @@ struct FusedItem {
     int64_t job0;  // first job of this item in the batch's job space
     int64_t pt0;   // first point of this item in the batch's point space
     int npairs;
+    int per;  // jobs per point: ceil(npairs / walks_per_lane)
 };
+int jobs_per_point(int npairs, int walks_per_lane) { return (npairs + walks_per_lane - 1) / walks_per_lane; }
 // Index of the item holding global index g, by the item starts `first` (sorted).
 template <typename F>
 DWD int find_item(int n_items, int64_t g, F first) {
@@ -732,7 +734,10 @@ DWD int find_item(int n_items, int64_t g, F first) {
     return lo;
 }
 
-// Fused jobs: job = i*npairs + k, the antithetic pair of WoSt.dude's first ball.
+// Fused jobs: job = i*per + k runs the antithetic pairs k*m .. k*m + m - 1 of
+// point i (m = walks_per_lane, per = ceil(npairs / m)) back to back, each pair
+// on its own stream (point i, pair j), so m only changes the scheduling.  A
+// pair is WoSt.dude's first ball: its walks start at p + dir R and p - dir R.
 // B = false: one item (kernel argument), jobs from the spatial queue.
 // B = true: many items (device array); the queue runs over the concatenated
 // job space and each lane looks up the item of its job.
@@ -742,30 +747,17 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
     using V = typename VecT<D>::T;
 #define ITEM(f) (B ? items[it].f : one.f)
     int it = 0;
-    int64_t job = -1;
+    int64_t job = -1, pt = 0;  // current pair (i * npairs + j) and its point i
+    int left = 0;              // pairs of the job after the current one
     V p, dir, srcg;  // srcg: the pair's source term of grad u, (s+ - s-) * gs
     float R = 0.0f, up = 0.0f, s_sum = 0.0f;  // s_sum = s+ + s-
     uint64_t saved = 0;
     int phase = 0, steps = 0;
     bool kept_plus = false;
-    auto start = [&](int64_t got, Walk<D>& w, Rng& rng) DW_LAMBDA -> bool {
-        if constexpr (B) {
-            it = find_item(n_items, got, [&](int k) { return items[k].job0; });
-            got -= items[it].job0;
-        }
+    // first walk of pair `job` of point `pt`
+    auto begin_pair = [&](Walk<D>& w, Rng& rng) DW_LAMBDA {
         const int npairs = ITEM(npairs);
-        int64_t i = got / npairs;
-        if (!ITEM(inside)[i]) {
-            float* o = ITEM(pair_vals) + got * (D + 1);
-            for (int q = 0; q <= D; ++q) o[q] = 0.0f;
-            ITEM(pair_steps)[got] = 0;
-            if (ITEM(pair_flags)) ITEM(pair_flags)[got] = 0;
-            return false;
-        }
-        job = got;
-        R = ITEM(R)[i];
-        p = load_pt<D>(ITEM(pts), i);
-        seed_stream(rng, ITEM(P), i, npairs, job - i * npairs);
+        seed_stream(rng, ITEM(P), pt, npairs, job - pt * npairs);
         V p_src;
         float r_src, pdf_src;
         // the source at the sampled point and at its mirror 2p - p_src (the
@@ -790,6 +782,33 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
         walk_init<D>(w, p + dir * R);
         phase = 0;
         steps = 0;
+    };
+    auto start = [&](int64_t got, Walk<D>& w, Rng& rng) DW_LAMBDA -> bool {
+        if constexpr (B) {
+            it = find_item(n_items, got, [&](int k) { return items[k].job0; });
+            got -= items[it].job0;
+        }
+        const int npairs = ITEM(npairs), per = ITEM(per);
+        const int m = ITEM(P).walks_per_lane;
+        int64_t i = got / per;
+        int j0 = (int)(got - i * per) * m;
+        int cnt = npairs - j0 < m ? npairs - j0 : m;
+        if (!ITEM(inside)[i]) {
+            for (int j = j0; j < j0 + cnt; ++j) {
+                int64_t q = i * npairs + j;
+                float* o = ITEM(pair_vals) + q * (D + 1);
+                for (int c = 0; c <= D; ++c) o[c] = 0.0f;
+                ITEM(pair_steps)[q] = 0;
+                if (ITEM(pair_flags)) ITEM(pair_flags)[q] = 0;
+            }
+            return false;
+        }
+        pt = i;
+        job = i * npairs + j0;
+        left = cnt - 1;
+        R = ITEM(R)[i];
+        p = load_pt<D>(ITEM(pts), i);
+        begin_pair(w, rng);
         return true;
     };
     auto next = [&](Walk<D>& w, Rng& rng) DW_LAMBDA -> bool {
@@ -825,6 +844,12 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
         }
         ITEM(pair_steps)[job] = steps;
         if (ITEM(pair_flags)) ITEM(pair_flags)[job] = (uint8_t)((kept_plus ? 1 : 0) | (kept ? 2 : 0));
+        if (left > 0) {  // the job's next pair, same point
+            --left;
+            ++job;
+            begin_pair(w, rng);
+            return true;
+        }
         return false;
     };
     auto advance = [&](Walk<D>& w, Rng& rng, Trav& t) DW_LAMBDA -> int {
@@ -1872,6 +1897,7 @@ ParamsDev to_params(const dw_solver_params* p) {
     P.schedule = p->schedule;
     P.gate = p->gate;
     P.node_width = p->node_width ? p->node_width : 2;
+    P.walks_per_lane = p->walks_per_lane ? p->walks_per_lane : 1;
     return P;
 }
 int check_params(const dw_solver_params* p) {
@@ -1888,6 +1914,9 @@ int check_params(const dw_solver_params* p) {
     if (p->gate < 0 || p->gate > 64) return fail(DW_ERR_ARG, "gate must be in 0..64");
     if (p->node_width != 0 && p->node_width != 2 && p->node_width != 4)
         return fail(DW_ERR_ARG, "node_width must be 0, 2 or 4");
+    if (p->walks_per_lane != 0 && p->walks_per_lane != 1 && p->walks_per_lane != 2 && p->walks_per_lane != 4 &&
+        p->walks_per_lane != 8)
+        return fail(DW_ERR_ARG, "walks_per_lane must be 0, 1, 2, 4 or 8");
     return DW_OK;
 }
 
@@ -2082,8 +2111,10 @@ int fused_launch(const dw_scene* s, const float* pts, int64_t n, const dw_solver
         DW_HIP(hipGetLastError());
     } else {
         JobQ Q;
-        if ((rc = make_queue(s, b.ctr, pts, n, npairs, b.order, st, Q))) return rc;
-        const LaunchCfg cfg = launch_cfg(to_params(params), s->dev.dim);
+        const ParamsDev P = to_params(params);
+        const int per = jobs_per_point(npairs, P.walks_per_lane);
+        if ((rc = make_queue(s, b.ctr, pts, n, per, b.order, st, Q))) return rc;
+        const LaunchCfg cfg = launch_cfg(P, s->dev.dim);
         Q.gate = cfg.gate;
         const bool exact = (params->flags & DW_SOLVE_EXACT_CONES) != 0;
         FusedItem one;
@@ -2099,6 +2130,7 @@ int fused_launch(const dw_scene* s, const float* pts, int64_t n, const dw_solver
         one.pair_flags = pair_flags;
         one.n = n;
         one.npairs = npairs;
+        one.per = per;
         const FusedItem* none = nullptr;
         if (D == 2)
             rc = launch_pers(exact, cfg, DW_PERS(k_fused_pers, 2, false), st, one, none, 1, Q);
@@ -2826,6 +2858,7 @@ int dw_batch_create(const dw_batch_item* items, int32_t n_items, uint64_t* total
         f.grad = it.grad;
         f.n = it.n;
         f.npairs = (it.params.nwalks + 1) / 2;
+        f.per = jobs_per_point(f.npairs, f.P.walks_per_lane);
         f.job0 = b->total_jobs;
         f.pt0 = b->total_pts;
         int64_t j = it.n * (int64_t)f.npairs;
@@ -2834,7 +2867,7 @@ int dw_batch_create(const dw_batch_item* items, int32_t n_items, uint64_t* total
         f.pair_vals = (float*)cur, cur += (size_t)j * (dim + 1) * 4;
         f.pair_steps = (int*)cur, cur += (size_t)j * 4;
         cur = (char*)b->ws + (((size_t)(cur - (char*)b->ws) + 255) & ~(size_t)255);
-        b->total_jobs += j;
+        b->total_jobs += it.n * (int64_t)f.per;
         b->total_pts += it.n;
     }
     if (hipMalloc((void**)&b->dev_items, sizeof(FusedItem) * (host.empty() ? 1 : host.size())) != hipSuccess ||

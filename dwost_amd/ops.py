@@ -28,7 +28,8 @@ from ._lib import check, lib, ptr, stream_handle
 
 # int parameter list: Solver fields and launch shape (dw_solver_params)
 IP_FIELDS = ("nwalks", "nsteps", "nsubwalks", "control_variates", "antithetic_variates", "seed", "flags",
-             "point_base", "block_size", "blocks_per_cu", "spatial_order", "schedule", "gate", "node_width")
+             "point_base", "block_size", "blocks_per_cu", "spatial_order", "schedule", "gate", "node_width",
+             "walks_per_lane")
 FP_FIELDS = ("eps", "min_R")
 
 
@@ -39,7 +40,8 @@ def params_c(ip: list[int], fp: list[float]) -> _lib.SolverParams:
     return _lib.SolverParams(int(d["nwalks"]), int(d["nsteps"]), int(d["nsubwalks"]), int(d["control_variates"]),
                              int(d["antithetic_variates"]), int(d["seed"]) & 0xFFFFFFFF, float(fp[0]), float(fp[1]),
                              int(d["flags"]), int(d["point_base"]), int(d["block_size"]), int(d["blocks_per_cu"]),
-                             int(d["spatial_order"]), int(d["schedule"]), int(d["gate"]), int(d["node_width"]))
+                             int(d["spatial_order"]), int(d["schedule"]), int(d["gate"]), int(d["node_width"]),
+                             int(d["walks_per_lane"]))
 
 
 def boundary_c(bc_code: int, coeffs: list[float], g, h, source: list[float]) -> _lib.Boundary:

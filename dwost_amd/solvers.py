@@ -51,6 +51,9 @@ class Solver:
     # closest-point tree records: 2 (binary, one level per fetch) or 4 (two
     # levels per fetch); the same nodes in the same order (0: 2)
     node_width: int = 0
+    # fused solve: walk pairs of one point a lane runs back to back per job it
+    # takes from the queue (1, 2, 4 or 8; 0: 1); scheduling only
+    walks_per_lane: int = 0
 
     _walk_flag: ClassVar[int] = 0  # walk family (WoSt; WoS sets DW_SOLVE_WOS)
 
@@ -61,7 +64,7 @@ class Solver:
                                  float(self.min_R), flags,
                                  int(point_base), int(self.block_size), int(self.blocks_per_cu),
                                  int(self.spatial_order), int(self.schedule), int(self.gate),
-                                 int(self.node_width))
+                                 int(self.node_width), int(self.walks_per_lane))
 
     def _op_params(self, seed: int, point_base: int = 0) -> tuple[list[int], list[float]]:
         """The int and float parameter lists of the torch.ops.dwost solvers."""

@@ -102,6 +102,10 @@ typedef struct dw_solver_params { /* Solver / WoSt fields (solver.py:30-34, wost
                                2 (one binary level per fetch) or 4 (two levels per
                                fetch); 0: the default, 2.  Same visit order, same
                                results                                            */
+    int32_t walks_per_lane; /* fused solve (dw_solve_fused, batches): walk pairs of
+                               one point a lane runs back to back per job it takes
+                               from the queue, 1, 2, 4 or 8 (0: 1).  Scheduling
+                               only: each pair keeps its own stream            */
 } dw_solver_params;
 
 #define DW_ORDER_AUTO 0 /* on from 8192 points */

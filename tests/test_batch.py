@@ -113,8 +113,11 @@ def test_autotune_keeps_results(gpu, holes_spec):
     q = torch.as_tensor(holes_spec.queries[:1024]).cuda()
     ref = sol.solve_grad(q, sc, seed=3)["u"].clone()
     rep = autotune.tune(sol, sc, q, seed=3, sample=1024, reps=1, use_cache=False)
-    assert (sol.block_size, sol.blocks_per_cu, sol.spatial_order, sol.schedule, sol.node_width) == rep["config"]
-    assert len(rep["table"]) == 37  # 36 launch shapes at node width 2, then node width 4 at the fastest
+    assert (sol.block_size, sol.blocks_per_cu, sol.spatial_order, sol.schedule, sol.node_width,
+            sol.walks_per_lane) == rep["config"]
+    # 36 launch shapes at node width 2 and one walk pair per lane, then node
+    # width 4 at the fastest, then 2 and 4 walk pairs per lane at the fastest
+    assert len(rep["table"]) == 39
     assert torch.equal(sol.solve_grad(q, sc, seed=3)["u"].view(torch.int32), ref.view(torch.int32))
 
 

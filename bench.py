@@ -105,7 +105,15 @@ def alg_counts(name: str, n_points: int = 1024) -> dict:
                 f_alg=round(o["flops"] / st, 1), steps_per_walk=round(st / (len(idx) * 2 * npairs), 2),
                 ff_share=round(o["ff"] / st, 6), nodes_per_step=round(o["nodes"] / st, 1),
                 prims_per_step=round(o["prims"] / st, 2), sils_per_step=round(o["sils"] / st, 2),
-                cpu_seconds=round(dt, 1), threads=O.num_threads())
+                closest_point_query=closest_iters(o), cpu_seconds=round(dt, 1), threads=O.num_threads())
+
+
+def closest_iters(o: dict) -> dict:
+    """Per closest-point query: the GPU loop's iterations and interior-record
+    fetches at node width 2 and 4 (replayed from the oracle's traversal)."""
+    q = max(o["closest_queries"], 1)
+    return {f"width{w}": dict(iterations=round(o[f"iters{w}"] / q, 2), record_fetches=round(o[f"fetch{w}"] / q, 2))
+            for w in (2, 4)}
 
 
 def cpu_baseline(spec, cfg, target_s: float = 12.0):

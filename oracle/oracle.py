@@ -58,6 +58,7 @@ def lib():
         L.orc_cv_bound_check.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
         L.orc_far_check.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
         L.orc_sdf.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+        L.orc_closest_iters.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p]
         L.orc_grad_trace_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int,
                                           C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
                                           C.c_float] + [C.c_void_p] * 3
@@ -163,6 +164,15 @@ class OracleScene:
         tuv = np.zeros((n, 2), np.float32)
         lib().orc_query(self.h, sub, kind, _p(pts), _p(dirs), _p(rmax), n, _p(prim), _p(d), _p(p), _p(nrm), _p(tuv))
         return dict(prim=prim, d=d, p=p, n=nrm, tuv=tuv)
+
+    def closest_iters(self, sub: int, pts) -> np.ndarray:
+        """Per closest-point query on sub-scene `sub`: (iterations at node width
+        2, at width 4, interior-record fetches at width 2, at width 4) of the
+        GPU loop, replayed from the reference traversal."""
+        pts = _f32(pts)
+        out = np.zeros((len(pts), 4), np.int64)
+        lib().orc_closest_iters(self.h, sub, _p(pts), len(pts), _p(out))
+        return out
 
     def cv_bound_check(self, pts):
         """2D: the full-scene closest point queried with its pruning radius
@@ -315,7 +325,10 @@ class OracleScene:
                     hv=None, src_code=0, src_c=0.0, per_pair=False, cv=True, base=0, count=False):
         """count=True adds the algorithmic counters of the walks (reference
         binary-tree order, reference record sizes): bytes, flops, nodes,
-        prims, sils and ff (steps from a NaN position, which fetch nothing)."""
+        prims, sils, ff (steps from a NaN position, which fetch nothing), and
+        for the closest-point queries (closest_queries) the GPU loop's
+        iterations and interior-record fetches at node width 2 and 4 (iters2,
+        iters4, fetch2, fetch4)."""
         pts = _f32(pts)
         n = len(pts)
         D = self.dim
@@ -328,12 +341,14 @@ class OracleScene:
         pg = np.zeros((n * npairs, D), np.float32) if per_pair else None
         ps = np.zeros(n * npairs, np.int32) if per_pair else None
         st = np.zeros(1, np.uint64)
-        cnt = np.zeros(6, np.uint64) if count else None
+        cnt = np.zeros(11, np.uint64) if count else None
         lib().orc_solve_fused(self.h, _p(pts), n, _p(ip), _p(fp), bc_code, _p(c), _p(gv), _p(hv), src_code, _p(_src4(src_c)),
                               _p(u), _p(g), _p(pu), _p(pg), _p(ps), _p(st), _p(cnt))
         out = dict(u=u, grad=g, steps=int(st[0]))
         if count:
-            out.update({k: int(v) for k, v in zip(("bytes", "flops", "nodes", "prims", "sils", "ff"), cnt)})
+            keys = ("bytes", "flops", "nodes", "prims", "sils", "ff", "closest_queries", "iters2", "iters4", "fetch2",
+                    "fetch4")
+            out.update({k: int(v) for k, v in zip(keys, cnt)})
         if per_pair:
             out.update(pair_u=pu, pair_g=pg, pair_steps=ps)
         return out

@@ -928,8 +928,45 @@ static void make_sub(const Scene& sc, int want_type, Sub<D>& S) {
 struct Cnt {
     uint64_t bytes = 0, nodes = 0, prims = 0, sils = 0, flops = 0;
     uint64_t ff = 0;  // steps taken from a NaN position (no record fetched)
+    // closest-point queries and the loop iterations / interior-record fetches
+    // the GPU traversal takes for them at node width 2 and 4
+    uint64_t cq = 0, it2 = 0, it4 = 0, rf2 = 0, rf4 = 0;
     void add(const Cnt& o) {
         bytes += o.bytes, nodes += o.nodes, prims += o.prims, sils += o.sils, flops += o.flops, ff += o.ff;
+        cq += o.cq, it2 += o.it2, it4 += o.it4, rf2 += o.rf2, rf4 += o.rf4;
+    }
+};
+// Iterations of the GPU closest-point loop, replayed from this traversal's
+// pops.  The GPU descends into the chosen child without a stack round trip and
+// takes one pop (a pending node, processed or pruned) per iteration, so its
+// iterations are these pops.  With two-level records (node width 4) the pair
+// of an interior child descended into from a record's first level is tested
+// in the same iteration, so such a visit costs no iteration of its own.
+struct IterCount {
+    Cnt* c;
+    int desc = -1;  // child the last interior visit descends into
+    int lvl = 0;    // 1: the last visit read a record's first level
+    explicit IterCount(Cnt* c_) : c(c_) {
+        if (c) c->cq++;
+    }
+    // a popped node: pruned, a leaf, or an interior node (chosen: the child it
+    // descends into, -1: none)
+    void pruned() {
+        if (c) c->it2++, c->it4++;
+        desc = -1, lvl = 0;
+    }
+    void leaf() { pruned(); }
+    void interior(int ni, int chosen) {
+        if (c) {
+            c->it2++, c->rf2++;
+            if (ni == desc && lvl == 1) {
+                lvl = 2;
+            } else {
+                c->it4++, c->rf4++;
+                lvl = 1;
+            }
+        }
+        desc = chosen;
     }
 };
 enum { NODE2 = 24, SNODE2 = 48, SNODE3 = 60, SEG = 20, TRI = 40, SILV = 40, SILE = 68 };
@@ -1052,14 +1089,19 @@ static CP bvh2_closest(const Tree<2>& T, V2 p, Cnt* c, float r0 = kInf) {
     int sp = 0;
     st_i[0] = 0;
     st_d[0] = r0 < kInf ? 0.0f : r_max;
+    IterCount ic(c);
     while (sp >= 0) {
         int ni = st_i[sp];
         float cd = st_d[sp];
         sp--;
-        if (cd > r_max) continue;
+        if (cd > r_max) {
+            ic.pruned();
+            continue;
+        }
         const Node<2>& node = T.nodes[ni];
         if (c) c->bytes += NODE2, c->nodes++;
         if (node.nref > 0) {
+            ic.leaf();
             for (int j = 0; j < node.nref; ++j) {
                 CP r;
                 seg_closest(T.prims[node.off + j], p, r);
@@ -1068,6 +1110,7 @@ static CP bvh2_closest(const Tree<2>& T, V2 p, Cnt* c, float r0 = kInf) {
             }
             r_max = fminf_(r_max, its.d);
         } else {
+            const int sp0 = sp;
             int L = ni + 1, R = ni + node.off;
             float dminL, dmaxL, dminR, dmaxR;
             box_dist(node_min2(T.nodes[L]), node_max2(T.nodes[L]), p, dminL, dmaxL);
@@ -1095,6 +1138,7 @@ static CP bvh2_closest(const Tree<2>& T, V2 p, Cnt* c, float r0 = kInf) {
                 st_i[++sp] = R;
                 st_d[sp] = dminR;
             }
+            ic.interior(ni, sp > sp0 ? st_i[sp] : -1);
         }
     }
     return its;
@@ -1459,14 +1503,19 @@ static CP snch3_closest(const Tree<3>& T, V3 p, Cnt* c) {
     int sp = 0;
     st_i[0] = 0;
     st_d[0] = r_max;
+    IterCount ic(c);
     while (sp >= 0) {
         int ni = st_i[sp];
         float cd = st_d[sp];
         sp--;
-        if (!(cd <= r_max)) continue;
+        if (!(cd <= r_max)) {
+            ic.pruned();
+            continue;
+        }
         const Node<3>& node = T.nodes[ni];
         if (c) c->bytes += SNODE3, c->nodes++;
         if (node.nref > 0) {
+            ic.leaf();
             for (int j = 0; j < node.nref; ++j) {
                 CP r;
                 tri_closest(T.prims[node.off + j], p, r);
@@ -1479,6 +1528,7 @@ static CP snch3_closest(const Tree<3>& T, V3 p, Cnt* c) {
         } else {
             int L = ni + 1, R = ni + node.off;
             float dminL, dmaxL, dminR, dmaxR;
+            const int sp0 = sp;
             box_dist(node_min3(T.nodes[L]), node_max3(T.nodes[L]), p, dminL, dmaxL);
             bool hitL = dminL <= r_max;
             r_max = fminf_(r_max, dmaxL);
@@ -1506,6 +1556,7 @@ static CP snch3_closest(const Tree<3>& T, V3 p, Cnt* c) {
                 st_i[++sp] = R;
                 st_d[sp] = dminR;
             }
+            ic.interior(ni, sp > sp0 ? st_i[sp] : -1);
         }
     }
     return its;
@@ -3414,6 +3465,24 @@ void orc_cv_bound_check(void* s, const float* pts, int64_t n, int32_t* same, uin
     nodes[1] = n1;
 }
 
+// Per closest-point query: the GPU loop's iterations and interior-record
+// fetches at node width 2 and 4, out[n][4] = {iters2, iters4, fetch2, fetch4}.
+void orc_closest_iters(void* s, int sub, const float* pts, int64_t n, int64_t* out) {
+    Scene* sc = (Scene*)s;
+    int D = sc->dim;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t q = 0; q < n; ++q) {
+        const float* x = pts + q * D;
+        Cnt c;
+        if (D == 2)
+            bvh2_closest(sc->s2[sub].tree, v2(x[0], x[1]), &c);
+        else
+            snch3_closest(sc->s3[sub].tree, v3(x[0], x[1], x[2]), &c);
+        out[q * 4] = (int64_t)c.it2, out[q * 4 + 1] = (int64_t)c.it4;
+        out[q * 4 + 2] = (int64_t)c.rf2, out[q * 4 + 3] = (int64_t)c.rf4;
+    }
+}
+
 void orc_query(void* s, int sub, int kind, const float* pts, const float* dirs, const float* rmax, int64_t n,
                int32_t* prim, float* d, float* p, float* nrm, float* tuv) {
     Scene* sc = (Scene*)s;
@@ -3792,8 +3861,9 @@ void orc_solve_fused(void* s, const float* pts, int64_t n, const int32_t* ip, co
     BC bc = make_bc(bc_code, bc_c, gv, hv, src_code, src_c);
     int D = sc->dim;
     int64_t NP = (P.nwalks + 1) / 2;
-    uint64_t steps = 0, cb = 0, cf = 0, cn = 0, cp_ = 0, cs = 0, cff = 0;
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : steps, cb, cf, cn, cp_, cs, cff)
+    uint64_t steps = 0, cb = 0, cf = 0, cn = 0, cp_ = 0, cs = 0, cff = 0, ccq = 0, ci2 = 0, ci4 = 0, cr2 = 0, cr4 = 0;
+#pragma omp parallel for schedule(dynamic, 1) \
+    reduction(+ : steps, cb, cf, cn, cp_, cs, cff, ccq, ci2, ci4, cr2, cr4)
     for (int64_t i = 0; i < n; ++i) {
         const float* x = pts + i * D;
         float R;
@@ -3814,6 +3884,7 @@ void orc_solve_fused(void* s, const float* pts, int64_t n, const int32_t* ip, co
             Cnt k_;
             PairOut o = fused_pair(*sc, bc, P, x, R, ins, rng, counts ? &k_ : nullptr);
             cb += k_.bytes, cf += k_.flops, cn += k_.nodes, cp_ += k_.prims, cs += k_.sils, cff += k_.ff;
+            ccq += k_.cq, ci2 += k_.it2, ci4 += k_.it4, cr2 += k_.rf2, cr4 += k_.rf4;
             su += o.usum;
             for (int q = 0; q < D; ++q) sg[q] += o.g[q];
             steps += (uint64_t)o.steps;
@@ -3829,8 +3900,13 @@ void orc_solve_fused(void* s, const float* pts, int64_t n, const int32_t* ip, co
             for (int q = 0; q < D; ++q) grad[i * D + q] = (float)(sg[q] / nw);
     }
     if (steps_total) *steps_total = steps;
-    // counts[6] = {bytes, flops, nodes, primitives, silhouettes, NaN-position steps}
-    if (counts) counts[0] = cb, counts[1] = cf, counts[2] = cn, counts[3] = cp_, counts[4] = cs, counts[5] = cff;
+    // counts[11] = {bytes, flops, nodes, primitives, silhouettes, NaN-position steps,
+    // closest-point queries, their GPU loop iterations at node width 2 and 4,
+    // their interior-record fetches at node width 2 and 4}
+    if (counts) {
+        counts[0] = cb, counts[1] = cf, counts[2] = cn, counts[3] = cp_, counts[4] = cs, counts[5] = cff;
+        counts[6] = ccq, counts[7] = ci2, counts[8] = ci4, counts[9] = cr2, counts[10] = cr4;
+    }
 }
 
 // RNG / hashing exports for known-answer tests

@@ -992,7 +992,9 @@ DWD void tri_closest(const DTri& tr, F3 p, F3& pt, float& u_out, float& v_out, f
     v_out = 1.0f - u0 - u1;
 }
 
-DWD void closest3_begin(Trav& t, const SubDev& S, F3 p, bool wide = false) {
+// r0 < inf starts the pruning radius at a known bound on the answer (the
+// reference query starts at inf).
+DWD void closest3_begin(Trav& t, const SubDev& S, F3 p, bool wide = false, float r0 = kInf) {
     t.kind = wide ? TQ_CLOSEST_W : TQ_CLOSEST;
     t.S = &S;
     t.rec = wide ? S.rec_wide : S.rec_bvh;
@@ -1000,9 +1002,9 @@ DWD void closest3_begin(Trav& t, const SubDev& S, F3 p, bool wide = false) {
     t.x = p;
     t.d = kInf;
     t.best = -1;
-    t.r_max = kInf;
+    t.r_max = r0;
     t.ref = S.root_ref_bvh;
-    t.cd = kInf;
+    t.cd = r0 < kInf ? 0.0f : kInf;
     t.sp = 0;
 }
 DWD CP closest3_end(const Trav& t) {

@@ -243,6 +243,30 @@ DWD void walk_init(Walk<D>& w, typename VecT<D>::T x) {
     w.i = 0;
     w.term = -1;
     w.stage = WS_START;
+    w.R = 0.0f;
+    w.d_dir = kInf;  // no previous step: the first Dirichlet query is unbounded
+}
+
+// Largest coordinate magnitude of a sub-scene's box (the absolute part of the
+// rounding margins below).
+DWD float coord_scale(const SubDev& S, int dim) {
+    float sc = 0.0f;
+    for (int k = 0; k < dim; ++k) sc = fmx(sc, fmx(__builtin_fabsf(S.root_mn[k]), __builtin_fabsf(S.root_mx[k])));
+    return sc;
+}
+// Bound on the Dirichlet distance at the start of a step, from the step before:
+// the distance to a set is 1-Lipschitz and a step moves the walk by at most
+// R + 1e-5 (its ray starts at most 1e-5 inside a boundary point and stops
+// within R), so d <= d_prev + R + 1e-5, widened by relative 2^-16 plus 1e-6 of
+// the coordinate scale for rounding.  Started at this radius, the closest-point
+// query only skips subtrees farther than its answer and returns the unbounded
+// query's record, tie order included (the same argument as the control-variate
+// query's bound).  kInf on a walk's first step.
+template <int D>
+DWD float dirichlet_bound(const Walk<D>& w, const SubDev& F) {
+    float r = w.d_dir + w.R + 1e-5f;
+    if (!(r < kInf)) return kInf;
+    return r * (1.0f + 0x1p-16f) + 1e-6f * (1.0f + coord_scale(F, D));
 }
 
 // step finished: count it, end the walk at nsteps
@@ -266,7 +290,7 @@ DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Wa
             return ADV_QUERY;
         }
         if (Dn.present) {
-            closest2_begin(t, Dn, w.ip, kInf, P.node_width == 4);
+            closest2_begin(t, Dn, w.ip, dirichlet_bound(w, F), P.node_width == 4);
             w.stage = WS_DIR;
             return ADV_QUERY;
         }
@@ -330,11 +354,7 @@ DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Wa
             // skips subtrees farther than the answer and returns the unbounded
             // query's record, tie order included
             float r0 = kInf;
-            if (w.d_dir < kInf) {
-                float sc = fmx(fmx(__builtin_fabsf(F.root_mn[0]), __builtin_fabsf(F.root_mx[0])),
-                               fmx(__builtin_fabsf(F.root_mn[1]), __builtin_fabsf(F.root_mx[1])));
-                r0 = w.d_dir * (1.0f + 0x1p-16f) + 1e-6f * (1.0f + sc);
-            }
+            if (w.d_dir < kInf) r0 = w.d_dir * (1.0f + 0x1p-16f) + 1e-6f * (1.0f + coord_scale(F, 2));
             closest2_begin(t, F, w.ip, r0, P.node_width == 4);
             w.stage = WS_CV;
             return ADV_QUERY;
@@ -438,7 +458,7 @@ DWD int walk_advance(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, Wa
                 w.R = w.d_dir = lo;
                 w.stage = WS_STAR;
             } else {
-                closest3_begin(t, Dn, w.ip, P.node_width == 4);
+                closest3_begin(t, Dn, w.ip, P.node_width == 4, dirichlet_bound(w, F));
                 w.stage = WS_DIR;
                 return ADV_QUERY;
             }

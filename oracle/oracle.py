@@ -59,6 +59,8 @@ def lib():
         L.orc_far_check.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
         L.orc_sdf.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
         L.orc_closest_iters.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p]
+        L.orc_step_bound_check.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                           C.c_void_p]
         L.orc_grad_trace_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int,
                                           C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
                                           C.c_float] + [C.c_void_p] * 3
@@ -173,6 +175,17 @@ class OracleScene:
         out = np.zeros((len(pts), 4), np.int64)
         lib().orc_closest_iters(self.h, sub, _p(pts), len(pts), _p(out))
         return out
+
+    def step_bound_check(self, prev, nxt, R):
+        """The Dirichlet closest point at nxt queried with its pruning radius
+        started at the bound the walk derives from the step prev -> nxt of
+        radius R (|nxt - prev| <= R + 1e-5), against the unbounded query.
+        Returns (per-point bitwise agreement, nodes visited unbounded, bounded)."""
+        prev, nxt, R = _f32(prev), _f32(nxt), _f32(R)
+        same = np.zeros(len(prev), np.int32)
+        nodes = np.zeros(2, np.uint64)
+        lib().orc_step_bound_check(self.h, _p(prev), _p(nxt), _p(R), len(prev), _p(same), _p(nodes))
+        return same.astype(bool), int(nodes[0]), int(nodes[1])
 
     def cv_bound_check(self, pts):
         """2D: the full-scene closest point queried with its pruning radius

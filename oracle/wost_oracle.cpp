@@ -1492,17 +1492,19 @@ static inline void tri_closest(const Prim<3>& tr, V3 p, CP& r) {  // closest_poi
 }
 
 // SNCH3D.closest_point (snch3d.py:334-409)
-static CP snch3_closest(const Tree<3>& T, V3 p, Cnt* c) {
+// r0 < inf starts the pruning radius below infinity (a bound the caller knows
+// on the answer); the reference query is r0 = inf.
+static CP snch3_closest(const Tree<3>& T, V3 p, Cnt* c, float r0 = kInf) {
     CP its;
     std::memset(&its, 0, sizeof(its));
     its.d = kInf;
     its.prim = -1;
-    float r_max = kInf;
+    float r_max = r0;
     int st_i[64];
     float st_d[64];
     int sp = 0;
     st_i[0] = 0;
-    st_d[0] = r_max;
+    st_d[0] = r0 < kInf ? 0.0f : r_max;
     IterCount ic(c);
     while (sp >= 0) {
         int ni = st_i[sp];
@@ -3439,6 +3441,55 @@ void orc_far_check(void* s, const float* pts, int64_t n, int32_t* out) {
 // point, widened by relative 2^-16 plus 1e-6 of the coordinate scale (the root
 // box), against the unbounded reference query: same[i] = 1 when the two records
 // agree bit for bit; nodes[0..1] = nodes visited (unbounded, bounded).
+// A walk step moves x_prev to x_new with |x_new - x_prev| <= R + 1e-5: the
+// Dirichlet closest point at x_new queried with its pruning radius started at
+// (d(x_prev) + R + 1e-5)(1 + 2^-16) + 1e-6 (1 + coordinate scale of the full
+// scene), against the unbounded query.  same[n]: bitwise agreement of the
+// records; nodes[2]: nodes visited unbounded, bounded.
+void orc_step_bound_check(void* s, const float* prev, const float* next, const float* R, int64_t n, int32_t* same,
+                          uint64_t* nodes) {
+    Scene* sc = (Scene*)s;
+    const int D = sc->dim;
+    float scale = 0.0f;
+    for (int k = 0; k < D; ++k) {
+        float lo = D == 2 ? sc->s2[0].tree.nodes[0].box.mn[k] : sc->s3[0].tree.nodes[0].box.mn[k];
+        float hi = D == 2 ? sc->s2[0].tree.nodes[0].box.mx[k] : sc->s3[0].tree.nodes[0].box.mx[k];
+        scale = fmaxf_(scale, fmaxf_(std::fabs(lo), std::fabs(hi)));
+    }
+    uint64_t n0 = 0, n1 = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : n0, n1)
+    for (int64_t i = 0; i < n; ++i) {
+        Cnt ca, cb;
+        CP a, b;
+        float dp;
+        if (D == 2) {
+            const Tree<2>& T = sc->s2[1].tree;
+            dp = std::fabs(bvh2_closest(T, v2(prev[2 * i], prev[2 * i + 1]), nullptr).d);
+            float r = dp + R[i] + 1e-5f;
+            float r0 = r < kInf ? r * (1.0f + 0x1p-16f) + 1e-6f * (1.0f + scale) : kInf;
+            V2 x = v2(next[2 * i], next[2 * i + 1]);
+            a = bvh2_closest(T, x, &ca);
+            b = bvh2_closest(T, x, &cb, r0);
+        } else {
+            const Tree<3>& T = sc->s3[1].tree;
+            dp = std::fabs(snch3_closest(T, v3(prev[3 * i], prev[3 * i + 1], prev[3 * i + 2]), nullptr).d);
+            float r = dp + R[i] + 1e-5f;
+            float r0 = r < kInf ? r * (1.0f + 0x1p-16f) + 1e-6f * (1.0f + scale) : kInf;
+            V3 x = v3(next[3 * i], next[3 * i + 1], next[3 * i + 2]);
+            a = snch3_closest(T, x, &ca);
+            b = snch3_closest(T, x, &cb, r0);
+        }
+        bool eq = a.prim == b.prim && std::memcmp(&a.d, &b.d, 4) == 0 && std::memcmp(&a.t, &b.t, 4) == 0 &&
+                  std::memcmp(&a.u, &b.u, 4) == 0 && std::memcmp(&a.v, &b.v, 4) == 0 &&
+                  std::memcmp(&a.p, &b.p, sizeof(a.p)) == 0 && std::memcmp(&a.n, &b.n, sizeof(a.n)) == 0;
+        same[i] = eq ? 1 : 0;
+        n0 += ca.nodes;
+        n1 += cb.nodes;
+    }
+    nodes[0] = n0;
+    nodes[1] = n1;
+}
+
 void orc_cv_bound_check(void* s, const float* pts, int64_t n, int32_t* same, uint64_t* nodes) {
     Scene* sc = (Scene*)s;
     const Tree<2>& F = sc->s2[0].tree;

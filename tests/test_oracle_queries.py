@@ -153,3 +153,40 @@ def test_cv_closest_point_bounded_by_dirichlet_distance(oracle, which, star_spec
           f"({n_bound / n_full:.3f})")
     assert same.all()
     assert n_bound < n_full
+
+
+@pytest.mark.parametrize("which", ["star", "holes", "ico"])
+def test_dirichlet_query_bounded_by_previous_step(oracle, which, star_spec, holes_spec, ico_spec):
+    """Each walk step's Dirichlet closest point starts its pruning radius at the
+    bound the previous step gives: the distance to a set is 1-Lipschitz and a
+    step moves the walk by at most R + 1e-5 (R <= the previous Dirichlet
+    distance), so d(x_new) <= d(x_prev) + R + 1e-5, widened by relative 2^-16
+    plus 1e-6 of the coordinate scale.  The bounded query returns the unbounded
+    query's record bit for bit (equally near segments or triangles included) on
+    moves of every length up to the bound, from interior points, points beside
+    the boundary and points around vertices."""
+    spec = dict(star=star_spec, holes=holes_spec, ico=ico_spec)[which]
+    D = spec.dim
+    sc = oracle.OracleScene(spec.vertices, spec.indices, spec.types)
+    rng = np.random.default_rng(23)
+    V, I = spec.vertices.astype(np.float64), spec.indices
+    f = rng.integers(0, len(I), 3000)
+    w = rng.dirichlet(np.ones(D), 3000)
+    on = np.einsum("nk,nkd->nd", w, V[I[f]])
+    corner = V[rng.integers(0, len(V), 1500)] + rng.normal(0, 1e-3, (1500, D))
+    prev = np.concatenate([spec.queries[:3000].astype(np.float64), on + rng.normal(0, 1e-3, on.shape), corner])
+    d_prev = sc.query(1, oracle.KIND_CLOSEST, prev.astype(np.float32))["d"].astype(np.float64)
+    keep = np.isfinite(d_prev)
+    prev, d_prev = prev[keep], d_prev[keep]
+    R = d_prev * rng.choice([1.0, 0.98, 0.5, 1e-3], len(prev))
+    dirs = rng.normal(size=prev.shape)
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    s = (R + 1e-5) * rng.choice([1.0, 0.999, 0.5, 0.0], len(prev))
+    nxt = prev + dirs * s[:, None]
+    same, n_full, n_bound = sc.step_bound_check(prev.astype(np.float32), nxt.astype(np.float32), R.astype(np.float32))
+    print(f"{which}: {same.mean():.6f} identical; nodes visited {n_full} -> {n_bound} ({n_bound / n_full:.3f})")
+    assert same.all()
+    assert n_bound < n_full
+    # planted mistake: a bound that forgets the move (R = 0) loses the answer
+    bad, _, _ = sc.step_bound_check(prev.astype(np.float32), nxt.astype(np.float32), np.zeros(len(prev), np.float32))
+    assert not bad.all()

@@ -1103,18 +1103,27 @@ DWD bool trav3_iter(Trav& t, Ovf& o) {
         if (t.ref < 0) {
             int off, cnt;
             leaf_range(S, t.ref, off, cnt);
-            if (kind == TQ_SIL) {
+            // each item's fetch is issued one item ahead (the last one repeats
+            // its own), so the next record is in flight while this one is tested
+            if (cnt <= 0) {
+                // an SNCH leaf without silhouettes: nothing to fetch
+            } else if (kind == TQ_SIL) {
+                DSilE e = gload<DSilE>(t.prims, off);
                 for (int j = 0; j < cnt; ++j) {
+                    DSilE en = gload<DSilE>(t.prims, off + (j + 1 < cnt ? j + 1 : j));
                     float d;
                     F3 pt;
-                    if (sile_test(gload<DSilE>(t.prims, off + j), t.x, t.r_max, d, pt)) {
+                    if (sile_test(e, t.x, t.r_max, d, pt)) {
                         t.r_max = fmn(t.r_max, d);
                         t.best = off + j;
                     }
+                    e = en;
                 }
             } else {
+                DTri trn = gload<DTri>(t.prims, off);
                 for (int j = 0; j < cnt; ++j) {
-                    DTri tr = gload<DTri>(t.prims, off + j);
+                    DTri tr = trn;
+                    trn = gload<DTri>(t.prims, off + (j + 1 < cnt ? j + 1 : j));
                     if (kind != TQ_ISECT) {
                         F3 pt;
                         float u, v, d;

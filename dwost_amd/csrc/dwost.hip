@@ -680,22 +680,27 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
 
 // Distance to the full boundary and the inside flag per point (used by the
 // gradient estimators, wost.py:43-50).
-DWD void prep_point(const SceneDev& sc, const float* __restrict__ pts, int64_t i, float* R, int* inside) {
+// Rd (optional): the distance to the Dirichlet part (inf without one), from
+// which the fused walks bound their first Dirichlet query (dirichlet_bound).
+DWD void prep_point(const SceneDev& sc, const float* __restrict__ pts, int64_t i, float* R, int* inside,
+                    float* Rd = nullptr) {
     if (sc.dim == 2) {
         F2 x = f2(pts[2 * i], pts[2 * i + 1]);
         R[i] = __builtin_fabsf(closest2(sc.sub[0], x).d);
         inside[i] = inside2(sc, x) ? 1 : 0;
+        if (Rd) Rd[i] = sc.sub[1].present ? __builtin_fabsf(closest2(sc.sub[1], x).d) : kInf;
     } else {
         F3 x = f3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
         R[i] = __builtin_fabsf(closest3(sc.sub[0], x).d);
         inside[i] = inside3(sc, x) ? 1 : 0;
+        if (Rd) Rd[i] = sc.sub[1].present ? __builtin_fabsf(closest3(sc.sub[1], x).d) : kInf;
     }
 }
 __global__ void __launch_bounds__(kBlock) k_point_prep(SceneDev sc, const float* __restrict__ pts, int64_t n,
-                                                       float* R, int* inside) {
+                                                       float* R, int* inside, float* Rd = nullptr) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    prep_point(sc, pts, i, R, inside);
+    prep_point(sc, pts, i, R, inside, Rd);
 }
 
 // One scene's share of a fused launch: geometry, boundary data, walk
@@ -708,6 +713,7 @@ struct FusedItem {
     const float* pts;
     const float* R;
     const int* inside;
+    const float* Rd;   // distance to the Dirichlet part per point (nullptr: not known)
     float* pair_vals;  // [n * npairs][dim + 1]
     int* pair_steps;   // [n * npairs]
     uint8_t* pair_flags;
@@ -754,6 +760,15 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
     uint64_t saved = 0;
     int phase = 0, steps = 0;
     bool kept_plus = false;
+    // a walk of the pair starts at x, |x - p| = R: its first Dirichlet query is
+    // bounded by the point's Dirichlet distance + R
+    auto start_walk = [&](Walk<D>& w, V x) DW_LAMBDA {
+        walk_init<D>(w, x);
+        if (ITEM(Rd)) {
+            w.d_dir = ITEM(Rd)[pt];
+            w.R = R;
+        }
+    };
     // first walk of pair `job` of point `pt`
     auto begin_pair = [&](Walk<D>& w, Rng& rng) DW_LAMBDA {
         const int npairs = ITEM(npairs);
@@ -779,7 +794,7 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
             s_sum = sp + sm;
         }
         saved = rng.state;
-        walk_init<D>(w, p + dir * R);
+        start_walk(w, p + dir * R);
         phase = 0;
         steps = 0;
     };
@@ -820,7 +835,7 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
             kept_plus = kept;
             up = v;
             rng.state = saved;
-            walk_init<D>(w, p - dir * R);
+            start_walk(w, p - dir * R);
             phase = 1;
             return true;
         }
@@ -868,7 +883,7 @@ __global__ void __launch_bounds__(kBlock) k_point_prep_batch(const FusedItem* __
     if (g >= total) return;
     int it = find_item(n_items, g, [&](int k) { return items[k].pt0; });
     const FusedItem& I = items[it];
-    prep_point(*I.sc, I.pts, g - I.pt0, (float*)I.R, (int*)I.inside);
+    prep_point(*I.sc, I.pts, g - I.pt0, (float*)I.R, (int*)I.inside, (float*)I.Rd);
 }
 __global__ void __launch_bounds__(kBlock) k_reduce_batch(const FusedItem* __restrict__ items, int n_items,
                                                          int64_t total, int dim, unsigned long long* step_total) {
@@ -2089,6 +2104,7 @@ struct FusedBufs {
     unsigned long long* ctr;
     float* R;
     int* ins;
+    float* Rd;
     float* pv;
     int32_t* ps;
     void* order;
@@ -2103,7 +2119,8 @@ int fused_launch(const dw_scene* s, const float* pts, int64_t n, const dw_solver
     const int D = s->dev.dim;
     const int npairs = (params->nwalks + 1) / 2;
     const int64_t lanes = n * (int64_t)npairs;
-    hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(n)), dim3(kBlock), kStackLds, st, s->dev, pts, n, b.R, b.ins);
+    hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(n)), dim3(kBlock), kStackLds, st, s->dev, pts, n, b.R, b.ins,
+                       simple ? nullptr : b.Rd);
     DW_HIP(hipGetLastError());
     if (simple) {
         hipLaunchKernelGGL(k_fused, dim3(blocks_for(lanes)), dim3(kBlock), kStackLds, st, s->dev, to_bc(boundary),
@@ -2125,6 +2142,7 @@ int fused_launch(const dw_scene* s, const float* pts, int64_t n, const dw_solver
         one.pts = pts;
         one.R = b.R;
         one.inside = b.ins;
+        one.Rd = b.Rd;
         one.pair_vals = b.pv;
         one.pair_steps = b.ps;
         one.pair_flags = pair_flags;
@@ -2684,7 +2702,7 @@ int dw_solve_fused(const dw_scene* s, const float* pts, int64_t n, const dw_solv
     FusedBufs b;
     b.ctr = c->ctr;
     size_t obytes = simple ? 0 : order_bytes(n, params);
-    size_t need = (size_t)n * 8 + (pair_vals ? 0 : (size_t)lanes * (D + 1) * 4) +
+    size_t need = (size_t)n * 12 + (pair_vals ? 0 : (size_t)lanes * (D + 1) * 4) +
                   (pair_steps ? 0 : (size_t)lanes * 4) + obytes;
     void* p;
     if ((rc = workspace(*c, need, st, &p))) return rc;
@@ -2692,6 +2710,8 @@ int dw_solve_fused(const dw_scene* s, const float* pts, int64_t n, const dw_solv
     b.R = (float*)cur;
     cur += (size_t)n * 4;
     b.ins = (int*)cur;
+    cur += (size_t)n * 4;
+    b.Rd = (float*)cur;
     cur += (size_t)n * 4;
     b.pv = pair_vals;
     if (!b.pv) b.pv = (float*)cur, cur += (size_t)lanes * (D + 1) * 4;
@@ -2708,12 +2728,13 @@ int dw_solve_fused_noalloc(const dw_scene* s, const float* pts, int64_t n, const
     const int D = s->dev.dim;
     const int64_t lanes = n * (int64_t)((params->nwalks + 1) / 2);
     // scratch layout (each part 256-byte aligned): queue counters, first-ball
-    // radii, inside flags, per-pair partial results and step counts, order
+    // radii, inside flags, per-pair partial results and step counts, order,
+    // Dirichlet distances
     const size_t obytes = order_bytes(n, params);
-    const size_t parts[6] = {(size_t)kQueueCtrs * 8, (size_t)n * 4, (size_t)n * 4, (size_t)lanes * (D + 1) * 4,
-                             (size_t)lanes * 4, obytes};
-    size_t off[6], need = 0;
-    for (int k = 0; k < 6; ++k) off[k] = need, need += (parts[k] + 255) & ~(size_t)255;
+    const size_t parts[7] = {(size_t)kQueueCtrs * 8, (size_t)n * 4, (size_t)n * 4, (size_t)lanes * (D + 1) * 4,
+                             (size_t)lanes * 4, obytes, (size_t)n * 4};
+    size_t off[7], need = 0;
+    for (int k = 0; k < 7; ++k) off[k] = need, need += (parts[k] + 255) & ~(size_t)255;
     if (need > kScratchBytes)
         return fail(DW_ERR_ARG, "dw_solve_fused_noalloc: n * ceil(nwalks / 2) needs " + std::to_string(need) +
                                     " bytes of scratch, above its fixed " + std::to_string(kScratchBytes) +
@@ -2728,6 +2749,7 @@ int dw_solve_fused_noalloc(const dw_scene* s, const float* pts, int64_t n, const
     b.pv = (float*)(base + off[3]);
     b.ps = (int32_t*)(base + off[4]);
     b.order = obytes ? base + off[5] : nullptr;
+    b.Rd = (float*)(base + off[6]);
     return fused_launch(s, pts, n, params, boundary, u, grad, nullptr, total_steps, false, b, (hipStream_t)stream);
 }
 
@@ -2837,7 +2859,7 @@ int dw_batch_create(const dw_batch_item* items, int32_t n_items, uint64_t* total
     size_t bytes = 0;
     for (int k : order) {
         int64_t j = jobs_of(k);
-        bytes += (size_t)items[k].n * 8 + (size_t)j * (dim + 1) * 4 + (size_t)j * 4;
+        bytes += (size_t)items[k].n * 12 + (size_t)j * (dim + 1) * 4 + (size_t)j * 4;
         bytes = (bytes + 255) & ~(size_t)255;
     }
     std::vector<FusedItem> host(order.size());
@@ -2864,6 +2886,7 @@ int dw_batch_create(const dw_batch_item* items, int32_t n_items, uint64_t* total
         int64_t j = it.n * (int64_t)f.npairs;
         f.R = (const float*)cur, cur += (size_t)it.n * 4;
         f.inside = (const int*)cur, cur += (size_t)it.n * 4;
+        f.Rd = (const float*)cur, cur += (size_t)it.n * 4;
         f.pair_vals = (float*)cur, cur += (size_t)j * (dim + 1) * 4;
         f.pair_steps = (int*)cur, cur += (size_t)j * 4;
         cur = (char*)b->ws + (((size_t)(cur - (char*)b->ws) + 255) & ~(size_t)255);

@@ -755,14 +755,19 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
     int it = 0;
     int64_t job = -1, pt = 0;  // current pair (i * npairs + j) and its point i
     int left = 0;              // pairs of the job after the current one
-    V p, dir, srcg;  // srcg: the pair's source term of grad u, (s+ - s-) * gs
-    float R = 0.0f, up = 0.0f, s_sum = 0.0f;  // s_sum = s+ + s-
+    // Held in registers across the pair: its direction, the stream state
+    // after the first draws and the counters.  The point, its first-ball
+    // radius and the pair's source terms stay in memory until the pair's end
+    // (point and radius in the launch's arrays; (s+ + s-, (s+ - s-) gs) in the
+    // pair's output slot, the first walk's value in its step-count slot), so
+    // they take no registers during the walks.
+    V dir;
     uint64_t saved = 0;
     int phase = 0, steps = 0;
     bool kept_plus = false;
     // a walk of the pair starts at x, |x - p| = R: its first Dirichlet query is
     // bounded by the point's Dirichlet distance + R
-    auto start_walk = [&](Walk<D>& w, V x) DW_LAMBDA {
+    auto start_walk = [&](Walk<D>& w, V x, float R) DW_LAMBDA {
         walk_init<D>(w, x);
         if (ITEM(Rd)) {
             w.d_dir = ITEM(Rd)[pt];
@@ -772,9 +777,11 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
     // first walk of pair `job` of point `pt`
     auto begin_pair = [&](Walk<D>& w, Rng& rng) DW_LAMBDA {
         const int npairs = ITEM(npairs);
+        const float R = ITEM(R)[pt];
+        const V p = load_pt<D>(ITEM(pts), pt);
         seed_stream(rng, ITEM(P), pt, npairs, job - pt * npairs);
-        V p_src;
-        float r_src, pdf_src;
+        V p_src, srcg;
+        float r_src, pdf_src, s_sum;
         // the source at the sampled point and at its mirror 2p - p_src (the
         // antithetic sample, wost.py:62-77)
         if constexpr (D == 2) {
@@ -793,8 +800,13 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
             srcg = gs * (sp - sm);
             s_sum = sp + sm;
         }
+        float* o = ITEM(pair_vals) + job * (D + 1);
+        o[0] = s_sum;
+        o[1] = srcg.x;
+        o[2] = srcg.y;
+        if constexpr (D == 3) o[3] = srcg.z;
         saved = rng.state;
-        start_walk(w, p + dir * R);
+        start_walk(w, p + dir * R, R);
         phase = 0;
         steps = 0;
     };
@@ -821,8 +833,6 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
         pt = i;
         job = i * npairs + j0;
         left = cnt - 1;
-        R = ITEM(R)[i];
-        p = load_pt<D>(ITEM(pts), i);
         begin_pair(w, rng);
         return true;
     };
@@ -831,19 +841,21 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
         steps += w.i;
         // walks the guards zero contribute nothing to the adjoint
         bool kept = D == 2 ? v == w.result : !(bad(v) || __builtin_fabsf(v) > 10000.0f);
+        const float R = ITEM(R)[pt];
         if (phase == 0) {
             kept_plus = kept;
-            up = v;
+            ITEM(pair_steps)[job] = __float_as_int(v);  // the first walk's value until the pair ends
             rng.state = saved;
-            start_walk(w, p - dir * R);
+            start_walk(w, load_pt<D>(ITEM(pts), pt) - dir * R, R);
             phase = 1;
             return true;
         }
-        float um = v;
+        const float up = __int_as_float(ITEM(pair_steps)[job]), um = v;
         float* o = ITEM(pair_vals) + job * (D + 1);
+        const float s_sum = o[0];
         if constexpr (D == 2) {
             float k = 2.0f / R * (up - um);
-            float g0 = k * dir.x + srcg.x, g1 = k * dir.y + srcg.y;
+            float g0 = k * dir.x + o[1], g1 = k * dir.y + o[2];
             o[0] = (up + um) + s_sum;
             o[1] = bad(g0) ? 0.0f : g0;
             o[2] = bad(g1) ? 0.0f : g1;
@@ -851,7 +863,7 @@ __global__ void __launch_bounds__(kBlock, X ? DW_PERS_WAVES_EXACT : DW_PERS_WAVE
             float u1 = (bad(up) || __builtin_fabsf(up) > 10000.0f) ? 0.0f : up;
             float u2 = (bad(um) || __builtin_fabsf(um) > 10000.0f) ? 0.0f : um;
             float k = 3.0f / R * (u1 - u2);
-            float g0 = k * dir.x + srcg.x, g1 = k * dir.y + srcg.y, g2_ = k * dir.z + srcg.z;
+            float g0 = k * dir.x + o[1], g1 = k * dir.y + o[2], g2_ = k * dir.z + o[3];
             o[0] = (u1 + u2) + s_sum;
             o[1] = bad(g0) ? 0.0f : g0;
             o[2] = bad(g1) ? 0.0f : g1;

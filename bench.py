@@ -72,14 +72,16 @@ def profile_figures(name: str) -> dict:
     """Figures of the committed rocprofv3 summary of this workload's kernel
     (profiles/k_fused_pers_<name>.md): HBM traffic per walk step from
     FETCH_SIZE / WRITE_SIZE (copy-calibrated), L2 hit rate, the share of wave
-    time issuing VALU instructions and the active-lane fraction of issued VALU
-    instructions."""
+    time issuing VALU instructions, the active-lane fraction of issued VALU
+    instructions and the VALU issue rate as a fraction of one wave64
+    instruction per SIMD every 4 cycles at 2.4 GHz."""
     path = os.path.join(ROOT, "profiles", f"k_fused_pers_{name}.md")
     out = {}
     if os.path.exists(path):
         for line in open(path):
             k, sep, v = line.strip().partition(": ")
-            if sep and k in ("traffic_bytes_per_walk_step", "l2_hit_rate", "valu_issue_share", "valu_lane_activity"):
+            if sep and k in ("traffic_bytes_per_walk_step", "l2_hit_rate", "valu_issue_share", "valu_lane_activity",
+                             "valu_issue_rate_frac"):
                 out[k] = float(v)
     return out
 
@@ -222,6 +224,7 @@ def secondary_2d(progress: bool, reps: int = 2) -> dict:
                 roofline_frac=round(ach / HBM_PEAK_GBS, 4), achieved_gbs=round(ach, 1),
                 b_alg_bytes_per_step=ALG["holes"]["b_alg"], l2_hit_rate=prof.get("l2_hit_rate"),
                 valu_issue_share=prof.get("valu_issue_share"), valu_lane_activity=prof.get("valu_lane_activity"),
+                valu_issue_rate_frac=prof.get("valu_issue_rate_frac"),
                 mean_steps_per_walk=round(steps / (reps * len(q) * c["nwalks"]), 3), **acc)
 
 
@@ -401,8 +404,10 @@ def main():
                 fp32_peak_tflops=FP32_PEAK_TFLOPS, ff_share=alg["ff_share"],
                 l2_hit_rate=prof.get("l2_hit_rate"), valu_issue_share=prof.get("valu_issue_share"),
                 valu_lane_activity=prof.get("valu_lane_activity"),
-                limiter="dependent node-fetch latency and VALU issue with divergent lanes (the scene is L2/MALL "
-                        "resident), not HBM bandwidth",
+                valu_issue_rate_frac=prof.get("valu_issue_rate_frac"),
+                limiter="VALU issue with divergent lanes: the walk kernel issues VALU instructions at about the "
+                        "chip's wave64 rate (valu_issue_rate_frac) with few lanes active per instruction "
+                        "(valu_lane_activity); the scene is L2/MALL resident and HBM bandwidth does not bind",
                 note="achieved = algorithmic bytes (reference record sizes, reference traversal order) / launch "
                      "time; traffic = measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE per walk "
                      "step x steps)")

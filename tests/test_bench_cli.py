@@ -23,8 +23,10 @@ def test_profile_figures_parse():
 
     for name in ("holes", "ico"):
         f = bench.profile_figures(name)
-        assert set(f) == {"traffic_bytes_per_walk_step", "l2_hit_rate", "valu_issue_share", "valu_lane_activity"}, name
+        assert set(f) == {"traffic_bytes_per_walk_step", "l2_hit_rate", "valu_issue_share", "valu_lane_activity",
+                          "valu_issue_rate_frac"}, name
         assert f["traffic_bytes_per_walk_step"] > 0 and 0 < f["l2_hit_rate"] <= 1
         assert 0 < f["valu_issue_share"] <= 1 and 0 < f["valu_lane_activity"] <= 1
+        assert 0 < f["valu_issue_rate_frac"] < 2
         text = open(os.path.join(ROOT, "profiles", f"k_fused_pers_{name}.md")).read()
         assert "rocprofv3 --kernel-trace --stats" in text and "rocprofv3 --pmc FETCH_SIZE" in text

@@ -389,19 +389,42 @@ DWD float grad_dudn2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, co
     return dudn;
 }
 
-// WoStGrad.dude (wost_grad.py:91-205), from an interior point
-DWD float grad_dude2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, const GradDev& G, F2 x, F2 e,
-                     Rng& rng, int& steps) {
+// State of one WoStGrad walk between its steps (2D and 3D): position, normal
+// and on-boundary flag, the direction e being differentiated along, the
+// running estimate, the 2D throughput, the step count and whether it goes on.
+template <int D>
+struct GradWalk {
+    typename VecT<D>::T ip, in, e;
+    float result, thr;
+    int i;
+    bool ob, active;
+};
+DWD void grad_init(const SceneDev& sc, GradWalk<2>& g, F2 x, F2 e) {
+    g.ip = x;
+    g.in = f2(0.0f, 0.0f);
+    g.e = e;
+    g.ob = false;
+    g.result = 0.0f;
+    g.i = 0;
+    g.active = g.ob || inside2(sc, g.ip);
+    g.thr = 1.0f;
+}
+template <int D>
+DWD bool grad_running(const GradWalk<D>& g, const ParamsDev& P) {
+    return (g.i < P.nsteps) && g.active;
+}
+
+// One step of WoStGrad.dude (wost_grad.py:91-205), from an interior point
+DWD void grad_step2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, const GradDev& G, GradWalk<2>& g,
+                    Rng& rng, int& steps) {
     const SubDev& F = sc.sub[0];
     const SubDev& Dn = sc.sub[1];
     const SubDev& Nm = sc.sub[2];
-    F2 ip = x, in = f2(0.0f, 0.0f);
-    bool ob = false;
-    float result = 0.0f;
-    int i = 0;
-    bool active = ob || inside2(sc, ip);
-    float thr = 1.0f;
-    while ((i < P.nsteps) && active) {
+    F2 ip = g.ip, in = g.in, e = g.e;
+    bool ob = g.ob, active = g.active;
+    float result = g.result, thr = g.thr;
+    int i = g.i;
+    {
         bool inflate = false;
         float d_dir = kInf;
         if (Dn.present) d_dir = __builtin_fabsf(closest2(Dn, ip).d);
@@ -492,8 +515,18 @@ DWD float grad_dude2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, co
         i += 1;
         if (bad(result) || __builtin_fabsf(result) > 1000.0f) result = 0.0f;
     }
-    steps += i;
-    return result;
+    g.ip = ip, g.in = in, g.e = e;
+    g.ob = ob, g.active = active;
+    g.result = result, g.thr = thr;
+    g.i = i;
+}
+DWD float grad_dude2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, const GradDev& G, F2 x, F2 e,
+                     Rng& rng, int& steps) {
+    GradWalk<2> g;
+    grad_init(sc, g, x, e);
+    while (grad_running(g, P)) grad_step2(sc, bc, P, G, g, rng, steps);
+    steps += g.i;
+    return g.result;
 }
 
 // ---------------------------------------------------------------------------
@@ -699,17 +732,26 @@ DWD float star_radius3_rho(const SubDev& S, F3 x, F3 e, float rho_max) {
     return r_max;
 }
 
-// WoStGrad3D.dude (wost_grad_3d.py:93-206) on an all-Neumann scene
-DWD float grad_dude3(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, const GradDev& G, F3 x, F3 e,
-                     Rng& rng, int& steps) {
+DWD void grad_init(const SceneDev& sc, GradWalk<3>& g, F3 x, F3 e) {
+    g.ip = x;
+    g.in = f3(0.0f, 0.0f, 0.0f);
+    g.e = e;
+    g.ob = false;
+    g.result = 0.0f;
+    g.active = inside3(sc, x);
+    g.i = 0;
+    g.thr = 1.0f;
+}
+// One step of WoStGrad3D.dude (wost_grad_3d.py:93-206) on an all-Neumann scene
+DWD void grad_step3(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, const GradDev& G, GradWalk<3>& g,
+                    Rng& rng, int& steps) {
     const SubDev& F = sc.sub[0];
     const SubDev& Nm = sc.sub[2];
-    F3 ip = x, in = f3(0.0f, 0.0f, 0.0f);
-    bool ob = false;
-    float result = 0.0f;
-    bool active = inside3(sc, x);
-    int i = 0;
-    while ((i < P.nsteps) && active) {
+    F3 ip = g.ip, in = g.in, e = g.e;
+    bool ob = g.ob, active = g.active;
+    float result = g.result;
+    int i = g.i;
+    {
         bool inflate = false;
         float R = kInf;
         if (Nm.present) {
@@ -822,8 +864,18 @@ DWD float grad_dude3(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, co
         i += 1;
         if (bad(result) || __builtin_fabsf(result) > 1000.0f) result = 0.0f;
     }
-    steps += i;
-    return result;
+    g.ip = ip, g.in = in, g.e = e;
+    g.ob = ob, g.active = active;
+    g.result = result;
+    g.i = i;
+}
+DWD float grad_dude3(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, const GradDev& G, F3 x, F3 e,
+                     Rng& rng, int& steps) {
+    GradWalk<3> g;
+    grad_init(sc, g, x, e);
+    while (grad_running(g, P)) grad_step3(sc, bc, P, G, g, rng, steps);
+    steps += g.i;
+    return g.result;
 }
 
 }  // namespace dwd

@@ -507,7 +507,7 @@ __global__ void __launch_bounds__(kBlock) k_baseline(SceneDev sc, int sub, int k
 // walk_dude (:146-166): lane i * nwalks + k runs nsubwalks dudes on the PCG32
 // stream of lane (point_base + i) * nwalks + k and writes their mean.
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_grad_dude(SceneDev sc, BCDev bc, ParamsDev P, GradDev G,
+__global__ void __launch_bounds__(kBlock, 2) k_grad_dude(SceneDev sc, BCDev bc, ParamsDev P, GradDev G,
                                                       const float* __restrict__ pts, const float* __restrict__ ev,
                                                       int64_t n, float* walk_val, int* walk_steps) {
     int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -528,10 +528,64 @@ __global__ void __launch_bounds__(kBlock) k_grad_dude(SceneDev sc, BCDev bc, Par
     if (walk_steps) walk_steps[lane] = st;
 }
 
+// The same walks as persistent jobs: a lane takes job i * nwalks + k from the
+// queue, runs its nsubwalks walks one step per loop iteration and takes the
+// next job as soon as they end, so a wave is not held by its longest walk (a
+// step, including a Dirichlet point's nested dudn walk, still runs whole).
+template <int D>
+__global__ void __launch_bounds__(kBlock, 2) k_grad_dude_pers(SceneDev sc, BCDev bc, ParamsDev P, GradDev G,
+                                                              const float* __restrict__ pts,
+                                                              const float* __restrict__ ev, int64_t n,
+                                                              float* walk_val, int* walk_steps, JobQ Q) {
+    GradWalk<D> g;
+    Rng rng;
+    int64_t job = -1;
+    float v = 0.0f;
+    int st = 0, sub = 0;
+    bool busy = false, done = false;
+    QState qs;
+    q_start(qs);
+    while (true) {
+        int64_t got = q_next(!done && !busy, Q, qs);
+        if (got == -2) done = true;
+        if (got >= 0) {
+            job = got;
+            int64_t i = job / P.nwalks;
+            seed_stream(rng, P, i, P.nwalks, job - i * P.nwalks);
+            grad_init(sc, g, load_pt<D>(pts, i), load_pt<D>(ev, i));
+            v = 0.0f;
+            st = 0;
+            sub = 0;
+            busy = true;
+        }
+        if (__ballot(!done) == 0) break;
+        if (busy) {
+            if (grad_running(g, P)) {
+                if constexpr (D == 2)
+                    grad_step2(sc, bc, P, G, g, rng, st);
+                else
+                    grad_step3(sc, bc, P, G, g, rng, st);
+            }
+            if (!grad_running(g, P)) {  // the walk has ended
+                st += g.i;
+                v += g.result;
+                if (++sub < P.nsubwalks) {
+                    int64_t i = job / P.nwalks;
+                    grad_init(sc, g, load_pt<D>(pts, i), load_pt<D>(ev, i));
+                } else {
+                    walk_val[job] = v / (float)P.nsubwalks;
+                    if (walk_steps) walk_steps[job] = st;
+                    busy = false;
+                }
+            }
+        }
+    }
+}
+
 // StreamlineTracer.next_position (examples/example_2d/trace.py:19-68) with a
 // WoStGrad solver: sample j of point i on the stream of lane i * tn + j,
 // walk_dude = the mean of nsubwalks WoStGrad dudes; the 2D step rule.
-__global__ void __launch_bounds__(kBlock) k_grad_trace(SceneDev sc, BCDev bc, ParamsDev P, GradDev G,
+__global__ void __launch_bounds__(kBlock, 2) k_grad_trace(SceneDev sc, BCDev bc, ParamsDev P, GradDev G,
                                                        const float* __restrict__ pts, int64_t n, int tn, float step,
                                                        float* samples, unsigned long long* step_total) {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2557,12 +2611,31 @@ int dw_grad_solve_dude(const dw_scene* s, const float* pts, const float* e, int6
         if (!ws && total_steps) ws = (int32_t*)cur;
     }
     GradDev G{grad->rho_max, grad->ignore_boundary, grad->ignore_silhouette, grad->ignore_source};
-    if (s->dev.dim == 2)
-        hipLaunchKernelGGL(k_grad_dude<2>, dim3(blocks_for(lanes)), dim3(kBlock), kStackLds, st, s->dev,
-                           to_bc(boundary), to_params(params), G, pts, e, n, wv, ws);
-    else
-        hipLaunchKernelGGL(k_grad_dude<3>, dim3(blocks_for(lanes)), dim3(kBlock), kStackLds, st, s->dev,
-                           to_bc(boundary), to_params(params), G, pts, e, n, wv, ws);
+    const char* kenv = getenv("DWOST_KERNEL");
+    if (kenv && strcmp(kenv, "simple") == 0) {  // one walk per thread (a cross-check of the persistent form)
+        if (s->dev.dim == 2)
+            hipLaunchKernelGGL(k_grad_dude<2>, dim3(blocks_for(lanes)), dim3(kBlock), kStackLds, st, s->dev,
+                               to_bc(boundary), to_params(params), G, pts, e, n, wv, ws);
+        else
+            hipLaunchKernelGGL(k_grad_dude<3>, dim3(blocks_for(lanes)), dim3(kBlock), kStackLds, st, s->dev,
+                               to_bc(boundary), to_params(params), G, pts, e, n, wv, ws);
+    } else {
+        JobQ Q;
+        if ((rc = make_queue(s, c->ctr, pts, n, params->nwalks, nullptr, st, Q))) return rc;
+        const ParamsDev P = to_params(params);
+        const LaunchCfg cfg = launch_cfg(P, s->dev.dim);
+        unsigned blocks = 0;
+        const size_t lds = (size_t)cfg.block * kStackLdsBytesPerThread;
+        if (s->dev.dim == 2) {
+            if ((rc = persistent_grid(k_grad_dude_pers<2>, cfg, blocks))) return rc;
+            hipLaunchKernelGGL(k_grad_dude_pers<2>, dim3(blocks), dim3(cfg.block), lds, st, s->dev, to_bc(boundary),
+                               P, G, pts, e, n, wv, ws, Q);
+        } else {
+            if ((rc = persistent_grid(k_grad_dude_pers<3>, cfg, blocks))) return rc;
+            hipLaunchKernelGGL(k_grad_dude_pers<3>, dim3(blocks), dim3(cfg.block), lds, st, s->dev, to_bc(boundary),
+                               P, G, pts, e, n, wv, ws, Q);
+        }
+    }
     DW_HIP(hipGetLastError());
     if (total_steps && (rc = zero_u64((unsigned long long*)total_steps, st))) return rc;
     hipLaunchKernelGGL(k_reduce_mean, dim3(blocks_for(n * 64)), dim3(kBlock), 0, st, wv, n, params->nwalks, 1,

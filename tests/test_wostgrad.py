@@ -169,3 +169,35 @@ def test_wostgrad3d_needs_all_neumann(gpu, ico_spec):
     q = torch.as_tensor(ico_spec.queries[:4]).cuda()
     with pytest.raises(gpu.DwostError):
         gpu.WoStGrad3D(nwalks=4).solve_dude(q, q, sc)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_wostgrad_persistent_matches_simple(gpu, dim, monkeypatch):
+    """WoStGrad walks run as persistent jobs, one step per loop iteration; every
+    walk (value and step count) equals the one-walk-per-thread kernel's, with
+    several sub-walks per job and points outside the domain among the jobs."""
+    from dwost_amd import scenes
+
+    if dim == 2:
+        spec = scenes.holes_scene_2d(n_query=300, eps=1e-3)
+        sc = gpu.Scene2D(spec.vertices, spec.indices, spec.types, gpu.BoundaryCondition(spec.bc_code))
+        sol = gpu.WoStGrad(eps=1e-3, nwalks=24, nsteps=1024, nsubwalks=2)
+    else:
+        spec = scenes.icosphere_scene(level=3, n_query=200)
+        types = np.zeros_like(spec.types)
+        sc = gpu.Scene3D(spec.vertices, spec.indices, types, gpu.BoundaryCondition(2, (0, 1, 0, 0)))
+        sol = gpu.WoStGrad3D(eps=1e-3, nwalks=16, nsteps=1024, nsubwalks=2)
+    q = np.concatenate([spec.queries, np.full((3, dim), 9.0, np.float32)])
+    e = np.random.default_rng(5).standard_normal(q.shape)
+    e = (e / np.linalg.norm(e, axis=1, keepdims=True)).astype(np.float32)
+    qt, et = torch.as_tensor(q).cuda(), torch.as_tensor(e).cuda()
+    monkeypatch.setenv("DWOST_KERNEL", "simple")
+    ref = sol.solve_dude(qt, et, sc, seed=6, per_walk=True, count_steps=True)
+    torch.cuda.synchronize()
+    monkeypatch.delenv("DWOST_KERNEL")
+    got = sol.solve_dude(qt, et, sc, seed=6, per_walk=True, count_steps=True)
+    torch.cuda.synchronize()
+    for k in ("dude", "walk_dude", "walk_steps"):
+        assert np.array_equal(got[k].cpu().numpy().view(np.uint32), ref[k].cpu().numpy().view(np.uint32)), k
+    assert int(got["steps"][0]) == int(ref["steps"][0])
+    assert float(got["dude"][-1]) == 0.0

@@ -84,6 +84,8 @@ def lib():
     L.dw_intersect.argtypes = [vp, i32, vp, vp, vp, i64, i32, vp, vp, vp, vp, vp, vp]
     L.dw_closest_silhouette.argtypes = [vp, i32, vp, vp, i64, vp, vp, vp, vp]
     L.dw_star_radius.argtypes = [vp, i32, vp, vp, i64, vp, vp]
+    L.dw_star_radius_rho.argtypes = [vp, i32, vp, vp, C.c_float, i64, vp, vp]
+    L.dw_math_check.argtypes = [vp, vp]
     L.dw_sdf.argtypes = [vp, vp, i64, vp, vp]
     L.dw_sample_boundary.argtypes = [vp, i32, vp, i64, vp, vp, vp, vp, vp, vp]
     L.dw_query_baseline.argtypes = [vp, i32, i32, vp, vp, vp, i64, vp, vp, vp]
@@ -107,7 +109,8 @@ def lib():
     L.dw_adjoint_fused.argtypes = [vp, vp, i64, C.POINTER(SolverParams), C.POINTER(Boundary), vp, vp, vp, vp, vp,
                                    vp]
     for name in ("dw_scene_create", "dw_scene_destroy", "dw_scene_info", "dw_scene_export", "dw_closest_point",
-                 "dw_intersect", "dw_closest_silhouette", "dw_star_radius", "dw_sdf", "dw_sample_boundary",
+                 "dw_intersect", "dw_closest_silhouette", "dw_star_radius", "dw_star_radius_rho", "dw_math_check",
+                 "dw_sdf", "dw_sample_boundary",
                  "dw_query_baseline", "dw_scene_prim_map", "dw_solve_u", "dw_solve_dude", "dw_grad_solve_dude", "dw_grad_trace_step",
                  "dw_trace_step",
                  "dw_solve_fused", "dw_solve_fused_noalloc", "dw_adjoint_u", "dw_adjoint_fused", "dw_batch_create", "dw_batch_solve"):

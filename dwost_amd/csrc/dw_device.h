@@ -16,6 +16,9 @@
 namespace dwd {
 
 #define DWD __device__ __forceinline__
+// Rarely taken paths kept out of line, so their registers do not count
+// against the hot loops that call them.
+#define DWD_COLD __device__ __attribute__((noinline))
 
 constexpr float kInf = __builtin_inff();
 constexpr float kPi = 3.14159265358979323846f;
@@ -85,6 +88,91 @@ DWD float cr_asin(float x) { return (float)asin((double)x); }
 DWD float cr_atan2(float y, float x) { return (float)atan2((double)y, (double)x); }
 DWD float cr_cbrt(float x) { return (float)cbrt((double)x); }
 #endif
+
+// ---------------------------------------------------------------------------
+// Bracketed decisions.  The SNCH cone tests use the once-rounded acos / asin /
+// atan2 / cos above only through comparisons.  The fp32 library functions
+// acosf, asinf, atanf and cosf stay within kTrSteps float steps of the
+// once-rounded values on every float of the domains below (dw_math_check
+// measures the largest distance over each whole domain), so one fp32
+// evaluation gives an interval FI that holds the value the test computes.
+// Correctly rounded +, -, *, / and fmin / fmax are monotone in each operand,
+// so the test's own fp32 expressions evaluated at interval ends bound every
+// later value exactly as the test rounds it.  A comparison whose interval lies
+// on one side of its threshold has the test's outcome (kT / kF); otherwise the
+// outcome is open (kU) and the caller runs the fp64 form.  NaN anywhere gives
+// kU.
+// ---------------------------------------------------------------------------
+constexpr int kTrSteps = 4;                     // bound assumed for the four functions
+constexpr float kCosLo = -4.0f, kCosHi = 8.0f;  // cosf domain the bound covers
+enum { kF = 0, kT = 1, kU = 2 };
+struct FI {
+    float lo, hi;
+};
+DWD FI fi(float x) { return FI{x, x}; }
+DWD FI fi_nan() { return FI{__builtin_nanf(""), __builtin_nanf("")}; }
+DWD bool fi_bad(FI a) { return __builtin_isnan(a.lo) || __builtin_isnan(a.hi); }
+// [f - r, f + r] evaluated in fp64, widened by more than the final rounding
+DWD FI fi_around(double f, double r) {
+    double s = (__builtin_fabs(f) + r) * 0x1p-22 + 0x1p-148;
+    return FI{(float)(f - r - s), (float)(f + r + s)};
+}
+// once-rounded value of a function whose fp32 version f is within kTrSteps
+// steps (plus `extra` steps of argument rounding)
+DWD FI fi_steps(float f, int extra) {
+    double a = __builtin_fabs((double)f);
+    return fi_around((double)f, (double)(kTrSteps + 1 + extra) * (a * 0x1p-23 + 0x1p-149));
+}
+DWD FI fi_acos(float c) { return fi_steps(acosf(c), 0); }  // c in [-1, 1]
+DWD FI fi_asin(float q) { return fi_steps(asinf(q), 0); }  // q in [-1, 1]
+// atan2(y, x) for y >= 0, x > 0 from atanf(y / x): y / x rounds within half a
+// step and atan(z(1 + d)) - atan(z) <= |d| atan(z) for z >= 0
+DWD FI fi_atan2(float y, float x) {
+    if (!((y >= 0.0f) && (x > 0.0f))) return fi_nan();
+    return fi_steps(atanf(y / x), 2);
+}
+// cos over an interval: Lipschitz 1 around the lower end
+DWD FI fi_cos(FI a) {
+    if (!((a.lo >= kCosLo) && (a.hi <= kCosHi) && (a.lo <= a.hi))) return fi_nan();
+    double f = (double)cosf(a.lo);
+    double w = (double)a.hi - (double)a.lo;
+    FI r = fi_around(f, w + (double)(kTrSteps + 2) * ((__builtin_fabs(f) + w) * 0x1p-23 + 0x1p-149));
+    return FI{fmx(r.lo, -1.0f), fmn(r.hi, 1.0f)};
+}
+DWD FI fi_add(FI a, FI b) { return FI{a.lo + b.lo, a.hi + b.hi}; }
+DWD FI fi_sub(FI a, FI b) { return FI{a.lo - b.hi, a.hi - b.lo}; }
+DWD FI fi_hull(float p0, float p1, float p2, float p3) {
+    if (__builtin_isnan(p0) || __builtin_isnan(p1) || __builtin_isnan(p2) || __builtin_isnan(p3)) return fi_nan();
+    return FI{fmn(fmn(p0, p1), fmn(p2, p3)), fmx(fmx(p0, p1), fmx(p2, p3))};
+}
+DWD FI fi_mul(FI a, FI b) { return fi_hull(a.lo * b.lo, a.lo * b.hi, a.hi * b.lo, a.hi * b.hi); }
+DWD FI fi_div(FI a, FI b) {
+    if (!((b.lo > 0.0f) || (b.hi < 0.0f))) return fi_nan();
+    return fi_hull(a.lo / b.lo, a.lo / b.hi, a.hi / b.lo, a.hi / b.hi);
+}
+DWD FI fi_sq(FI a) {  // a * a of one value
+    if (fi_bad(a)) return fi_nan();
+    float l = a.lo * a.lo, h = a.hi * a.hi;
+    if (a.lo >= 0.0f) return FI{l, h};
+    if (a.hi <= 0.0f) return FI{h, l};
+    return FI{0.0f, fmx(l, h)};
+}
+DWD FI fi_min(FI a, FI b) {
+    if (fi_bad(a) || fi_bad(b)) return fi_nan();
+    return FI{fmn(a.lo, b.lo), fmn(a.hi, b.hi)};
+}
+DWD FI fi_max(FI a, FI b) {
+    if (fi_bad(a) || fi_bad(b)) return fi_nan();
+    return FI{fmx(a.lo, b.lo), fmx(a.hi, b.hi)};
+}
+DWD int t_gt(FI a, float t) { return a.lo > t ? kT : (a.hi <= t ? kF : kU); }
+DWD int t_ge(FI a, float t) { return a.lo >= t ? kT : (a.hi < t ? kF : kU); }
+DWD int t_lt(FI a, float t) { return a.hi < t ? kT : (a.lo >= t ? kF : kU); }
+DWD int t_le(FI a, float t) { return a.hi <= t ? kT : (a.lo > t ? kF : kU); }
+DWD int t_and(int a, int b) { return (a == kF || b == kF) ? kF : ((a == kT && b == kT) ? kT : kU); }
+DWD int t_or(int a, int b) { return (a == kT || b == kT) ? kT : ((a == kF && b == kF) ? kF : kU); }
+// t in [lo, hi] (in_range below)
+DWD int t_in(float t, FI lo, FI hi) { return t_and(t_le(lo, t), t_ge(hi, t)); }
 
 // ---------------------------------------------------------------------------
 // TEA seeding (diff_wost/core/math.py:87-107) and PCG32 (Dr.Jit PCG32)
@@ -641,8 +729,42 @@ DWD bool snch_overlaps(V mn, V mx, V axis, float ha, float radius, V x, float r_
     box_dist(mn, mx, x, d_min, d_max);
     return (d_min <= r_max) && (!(ha >= 0.0f) || snch_cone_literal(mn, mx, axis, ha, radius, x, d_min));
 }
+// The literal test with its transcendentals bracketed: kT / kF is the literal
+// test's outcome, kU leaves it open.
+template <typename V>
+DWD int cone_literal_bracketed(V mn, V mx, V axis, float ha, float radius, V x, float d_min) {
+    if ((ha > kPi2) && (d_min < kDrEps)) return kT;
+    V cen = (mn + mx) / 2.0f;
+    V vca = cen - x;
+    float len = norm(vca);
+    vca = vca / len;
+    FI dax = fi_acos(clampf(dot(axis, vca), -1.0f, 1.0f));
+    int t0 = t_in(kPi2, fi_sub(dax, fi(ha)), fi_add(dax, fi(ha)));
+    if (t0 == kT) return kT;
+    FI vh;
+    if (len > radius) {
+        vh = fi_asin(radius / len);
+    } else {
+        V e = mx - cen;
+        float d = dot(e, vabs(vca));
+        float s = len - d;
+        if (s < 0.0f) return kT;
+        d = project_to_plane(vca, e);
+        vh = fi_atan2(d, s);
+    }
+    FI hs = fi_add(fi(ha), vh);
+    return t_or(t0, t_or(t_gt(hs, kPi2), t_in(kPi2, fi_sub(dax, hs), fi_add(dax, hs))));
+}
+template <typename V>
+DWD_COLD bool snch_cone_literal_fp64(V mn, V mx, V axis, float ha, float radius, V x, float d_min);
 template <typename V>
 DWD bool snch_cone_literal(V mn, V mx, V axis, float ha, float radius, V x, float d_min) {
+    int t = cone_literal_bracketed(mn, mx, axis, ha, radius, x, d_min);
+    if (t != kU) return t == kT;
+    return snch_cone_literal_fp64(mn, mx, axis, ha, radius, x, d_min);
+}
+template <typename V>
+DWD_COLD bool snch_cone_literal_fp64(V mn, V mx, V axis, float ha, float radius, V x, float d_min) {
     bool ov;
     {
         ov = (ha > kPi2) && (d_min < kDrEps);

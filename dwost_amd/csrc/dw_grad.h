@@ -75,49 +75,89 @@ DWD float seg_star_radius2(F2 a, F2 b, F2 x, F2 e, float rho_max, float r_max) {
     return R;
 }
 
-// SNCHNode.overlaps_2 (snch.py:100-174)
+// SNCHNode.overlaps_2 (snch.py:100-174): the cone part for a valid cone
+// (ha >= 0) of a node whose box is within r_max, with fp64 transcendentals
+DWD_COLD bool rho_cone2_fp64(F2 mn, F2 mx, const DCone2& cn, F2 x, F2 e, float rho_bound, float d_min) {
+    float ha = cn.half_angle;
+    bool ov = (ha > kPi2) && (d_min < kDrEps);
+    if (!ov) {
+        F2 c = (mn + mx) / 2.0f;
+        F2 vca = c - x;
+        float length = norm(vca);
+        vca = vca / length;
+        ov = length < cn.radius;
+        if (!ov) {
+            F2 axis = f2(cn.ax, cn.ay);
+            float axis_angle = cr_acos(clampf(dot(axis, vca), -1.0f, 1.0f));
+            float vch = cr_asin(cn.radius / length);
+            float hs = ha + vch;
+            float amin = axis_angle - hs, amax = axis_angle + hs;
+            ov = (hs > kPi2) || ((kPi2 >= amin) && (kPi2 <= amax));
+            if (!ov) {
+                float c0 = cr_cos(amin), c1 = cr_cos(amax);
+                float dn_min = fmn(c0, c1), dn_max = fmx(c0, c1);
+                float ed_axis = cr_acos(clampf(dot(e, vca), -1.0f, 1.0f));
+                float e0 = cr_cos(ed_axis - ha), e1 = cr_cos(ed_axis + ha);
+                float ed_min = fmn(e0, e1), ed_max = fmx(e0, e1);
+                float en_axis = cr_acos(clampf(dot(e, axis), -1.0f, 1.0f));
+                float n0 = cr_cos(en_axis - ha), n1 = cr_cos(en_axis + ha);
+                float en_min = fmn(n0, n1), en_max = fmx(n0, n1);
+                float en_dn_max = en_max / dn_min;
+                float en_dn_min = en_min < 0.0f ? en_min / dn_min : en_min / dn_max;
+                auto rho = [](float ed, float q) { return (1.0f - 2.0f * ed * q) + q * q; };
+                float rm = rho(ed_min, en_dn_min);
+                rm = fmx(rm, rho(ed_min, en_dn_max));
+                rm = fmx(rm, rho(ed_max, en_dn_min));
+                rm = fmx(rm, rho(ed_max, en_dn_max));
+                ov = rm > rho_bound * rho_bound;
+            }
+        }
+    }
+    return ov;
+}
+// The same decision with the transcendentals bracketed (kU: open)
+DWD int rho_cone2_bracketed(F2 mn, F2 mx, const DCone2& cn, F2 x, F2 e, float rho_bound, float d_min) {
+    float ha = cn.half_angle;
+    if ((ha > kPi2) && (d_min < kDrEps)) return kT;
+    F2 c = (mn + mx) / 2.0f;
+    F2 vca = c - x;
+    float length = norm(vca);
+    vca = vca / length;
+    if (length < cn.radius) return kT;
+    F2 axis = f2(cn.ax, cn.ay);
+    FI axis_angle = fi_acos(clampf(dot(axis, vca), -1.0f, 1.0f));
+    FI hs = fi_add(fi(ha), fi_asin(cn.radius / length));
+    FI amin = fi_sub(axis_angle, hs), amax = fi_add(axis_angle, hs);
+    int t = t_or(t_gt(hs, kPi2), t_in(kPi2, amin, amax));
+    if (t != kF) return t;
+    FI c0 = fi_cos(amin), c1 = fi_cos(amax);
+    FI dn_min = fi_min(c0, c1), dn_max = fi_max(c0, c1);
+    FI ed_axis = fi_acos(clampf(dot(e, vca), -1.0f, 1.0f));
+    FI e0 = fi_cos(fi_sub(ed_axis, fi(ha))), e1 = fi_cos(fi_add(ed_axis, fi(ha)));
+    FI ed_min = fi_min(e0, e1), ed_max = fi_max(e0, e1);
+    FI en_axis = fi_acos(clampf(dot(e, axis), -1.0f, 1.0f));
+    FI n0 = fi_cos(fi_sub(en_axis, fi(ha))), n1 = fi_cos(fi_add(en_axis, fi(ha)));
+    FI en_min = fi_min(n0, n1), en_max = fi_max(n0, n1);
+    int neg = t_lt(en_min, 0.0f);
+    if (neg == kU) return kU;
+    FI en_dn_max = fi_div(en_max, dn_min);
+    FI en_dn_min = fi_div(en_min, neg == kT ? dn_min : dn_max);
+    auto rho = [](FI ed, FI q) { return fi_add(fi_sub(fi(1.0f), fi_mul(fi_mul(fi(2.0f), ed), q)), fi_sq(q)); };
+    FI rm = rho(ed_min, en_dn_min);
+    rm = fi_max(rm, rho(ed_min, en_dn_max));
+    rm = fi_max(rm, rho(ed_max, en_dn_min));
+    rm = fi_max(rm, rho(ed_max, en_dn_max));
+    return t_gt(rm, rho_bound * rho_bound);
+}
 DWD bool snch_overlaps2(const DNode2& nd, const DCone2& cn, F2 x, F2 e, float rho_bound, float r_max,
                         float& d_min) {
     float d_max;
     F2 mn = node_mn(nd), mx = node_mx(nd);
     box_dist(mn, mx, x, d_min, d_max);
     bool ov = d_min <= r_max;
-    float ha = cn.half_angle;
-    if ((ha >= 0.0f) && ov) {
-        ov = (ha > kPi2) && (d_min < kDrEps);
-        if (!ov) {
-            F2 c = (mn + mx) / 2.0f;
-            F2 vca = c - x;
-            float length = norm(vca);
-            vca = vca / length;
-            ov = length < cn.radius;
-            if (!ov) {
-                F2 axis = f2(cn.ax, cn.ay);
-                float axis_angle = cr_acos(clampf(dot(axis, vca), -1.0f, 1.0f));
-                float vch = cr_asin(cn.radius / length);
-                float hs = ha + vch;
-                float amin = axis_angle - hs, amax = axis_angle + hs;
-                ov = (hs > kPi2) || ((kPi2 >= amin) && (kPi2 <= amax));
-                if (!ov) {
-                    float c0 = cr_cos(amin), c1 = cr_cos(amax);
-                    float dn_min = fmn(c0, c1), dn_max = fmx(c0, c1);
-                    float ed_axis = cr_acos(clampf(dot(e, vca), -1.0f, 1.0f));
-                    float e0 = cr_cos(ed_axis - ha), e1 = cr_cos(ed_axis + ha);
-                    float ed_min = fmn(e0, e1), ed_max = fmx(e0, e1);
-                    float en_axis = cr_acos(clampf(dot(e, axis), -1.0f, 1.0f));
-                    float n0 = cr_cos(en_axis - ha), n1 = cr_cos(en_axis + ha);
-                    float en_min = fmn(n0, n1), en_max = fmx(n0, n1);
-                    float en_dn_max = en_max / dn_min;
-                    float en_dn_min = en_min < 0.0f ? en_min / dn_min : en_min / dn_max;
-                    auto rho = [](float ed, float q) { return (1.0f - 2.0f * ed * q) + q * q; };
-                    float rm = rho(ed_min, en_dn_min);
-                    rm = fmx(rm, rho(ed_min, en_dn_max));
-                    rm = fmx(rm, rho(ed_max, en_dn_min));
-                    rm = fmx(rm, rho(ed_max, en_dn_max));
-                    ov = rm > rho_bound * rho_bound;
-                }
-            }
-        }
+    if ((cn.half_angle >= 0.0f) && ov) {
+        int t = rho_cone2_bracketed(mn, mx, cn, x, e, rho_bound, d_min);
+        ov = t == kU ? rho_cone2_fp64(mn, mx, cn, x, e, rho_bound, d_min) : t == kT;
     }
     return ov;
 }
@@ -133,15 +173,13 @@ DWD float star_radius2_rho(const SubDev& S, F2 x, F2 e, float rho_max) {
     float dmin, dmax;
     box_dist(node_mn(nodes[0]), node_mx(nodes[0]), x, dmin, dmax);
     if (!(dmin <= r_max)) return r_max;
-    int st_i[DW_MAX_STACK];
-    float st_d[DW_MAX_STACK];
+    Ovf o;
     int sp = 0;
-    st_i[0] = 0;
-    st_d[0] = dmin;
-    while (sp >= 0) {
-        int ni = st_i[sp];
-        float cd = st_d[sp];
-        sp--;
+    stk_push(sp, o, 0, dmin);
+    while (sp > 0) {
+        int ni;
+        float cd;
+        stk_pop(sp, o, ni, cd);
         if (!(cd <= r_max)) continue;
         const DNode2 nd = nodes[ni];
         if (nd.nref > 0) {
@@ -163,16 +201,12 @@ DWD float star_radius2_rho(const SubDev& S, F2 x, F2 e, float rho_max) {
                     float tmp = d0;
                     d0 = d1, d1 = tmp;
                 }
-                st_i[++sp] = other;
-                st_d[sp] = d1;
-                st_i[++sp] = closer;
-                st_d[sp] = d0;
+                stk_push(sp, o, other, d1);
+                stk_push(sp, o, closer, d0);
             } else if (h0) {
-                st_i[++sp] = L;
-                st_d[sp] = d0;
+                stk_push(sp, o, L, d0);
             } else if (h1) {
-                st_i[++sp] = R;
-                st_d[sp] = d1;
+                stk_push(sp, o, R, d1);
             }
         }
     }
@@ -572,50 +606,93 @@ DWD int psil3_sample(const SubDev& S, float value, float& reuse, float& pmf) {
 DWD F3 node3_mn(const DNode3& n) { return f3(n.mnx, n.mny, n.mnz); }
 DWD F3 node3_mx(const DNode3& n) { return f3(n.mxx, n.mxy, n.mxz); }
 
-// SNCHNode3D.visit (snch3d.py:196-271)
+// SNCHNode3D.visit (snch3d.py:196-271): the cone part for a valid cone
+// (ha >= 0) of a node whose box is within r_max, with fp64 transcendentals
+DWD_COLD bool rho_cone3_fp64(F3 mn, F3 mx, const DCone3& cn, F3 x, F3 v, float rho_max, float d_min) {
+    float ha = cn.half_angle;
+    bool ov = (ha > kPi2) && (d_min < kDrEps);
+    if (!ov) {
+        F3 c = (mn + mx) / 2.0f;
+        F3 vca = c - x;
+        float l = norm(vca);
+        vca = vca / l;
+        ov = l < cn.radius;
+        if (!ov) {
+            F3 axis = f3(cn.ax, cn.ay, cn.az);
+            float axis_angle = cr_acos(clampf(dot(axis, vca), -1.0f, 1.0f));
+            float vch = cr_asin(cn.radius / l);
+            float hs = ha + vch;
+            float amin = axis_angle - hs, amax = axis_angle + hs;
+            ov = (hs > kPi2) || ((kPi2 >= amin) && (kPi2 <= amax));
+            if (!ov) {
+                float c0 = cr_cos(amin), c1 = cr_cos(amax);
+                float cmin_a = fmn(c0, c1), cmax_a = fmx(c0, c1);
+                if ((amin <= 0.0f) && (amax >= 0.0f)) cmax_a = fmx(cmax_a, 1.0f);
+                float beta = cr_acos(clampf(dot(axis, v), -1.0f, 1.0f));
+                float bmin = beta - ha, bmax = beta + ha;
+                float b0 = cr_cos(bmin), b1 = cr_cos(bmax);
+                float cmin_b = fmn(b0, b1), cmax_b = fmx(b0, b1);
+                if ((bmin <= 0.0f) && (bmax >= 0.0f)) cmax_b = fmx(cmax_b, 1.0f);
+                float gamma = cr_acos(clampf(dot(v, vca), -1.0f, 1.0f));
+                float gmin = gamma - vch, gmax = gamma + vch;
+                float g0 = cr_cos(gmin), g1 = cr_cos(gmax);
+                float cmin_g = fmn(g0, g1), cmax_g = fmx(g0, g1);
+                if ((gmin <= 0.0f) && (gmax >= 0.0f)) cmax_g = fmx(cmax_g, 1.0f);
+                float a_min = cmin_b / cmax_a, a_max = cmax_b / cmin_a;
+                float b_min = cmin_g, b_max = cmax_g;
+                float rho2 = (1.0f - 2.0f * fmn(fmn(a_min * b_min, a_min * b_max), fmn(a_max * b_min, a_max * b_max))) +
+                             fmx(a_min * a_min, a_max * a_max);
+                ov = rho2 > rho_max * rho_max;
+            }
+        }
+    }
+    return ov;
+}
+// [min, max] of cos over [lo, hi] as the test forms it: cos at both ends,
+// raised to 1 when the range holds 0 (kU: open)
+DWD int fi_cos_range(FI lo, FI hi, FI& cmin, FI& cmax) {
+    FI c0 = fi_cos(lo), c1 = fi_cos(hi);
+    cmin = fi_min(c0, c1), cmax = fi_max(c0, c1);
+    int z = t_and(t_le(lo, 0.0f), t_ge(hi, 0.0f));
+    if (z == kT) cmax = fi_max(cmax, fi(1.0f));
+    return z;
+}
+// The same decision with the transcendentals bracketed (kU: open)
+DWD int rho_cone3_bracketed(F3 mn, F3 mx, const DCone3& cn, F3 x, F3 v, float rho_max, float d_min) {
+    float ha = cn.half_angle;
+    if ((ha > kPi2) && (d_min < kDrEps)) return kT;
+    F3 c = (mn + mx) / 2.0f;
+    F3 vca = c - x;
+    float l = norm(vca);
+    vca = vca / l;
+    if (l < cn.radius) return kT;
+    F3 axis = f3(cn.ax, cn.ay, cn.az);
+    FI axis_angle = fi_acos(clampf(dot(axis, vca), -1.0f, 1.0f));
+    FI vch = fi_asin(cn.radius / l);
+    FI hs = fi_add(fi(ha), vch);
+    FI amin = fi_sub(axis_angle, hs), amax = fi_add(axis_angle, hs);
+    int t = t_or(t_gt(hs, kPi2), t_in(kPi2, amin, amax));
+    if (t != kF) return t;
+    FI cmin_a, cmax_a, cmin_b, cmax_b, cmin_g, cmax_g;
+    if (fi_cos_range(amin, amax, cmin_a, cmax_a) == kU) return kU;
+    FI beta = fi_acos(clampf(dot(axis, v), -1.0f, 1.0f));
+    if (fi_cos_range(fi_sub(beta, fi(ha)), fi_add(beta, fi(ha)), cmin_b, cmax_b) == kU) return kU;
+    FI gamma = fi_acos(clampf(dot(v, vca), -1.0f, 1.0f));
+    if (fi_cos_range(fi_sub(gamma, vch), fi_add(gamma, vch), cmin_g, cmax_g) == kU) return kU;
+    FI a_min = fi_div(cmin_b, cmax_a), a_max = fi_div(cmax_b, cmin_a);
+    FI m = fi_min(fi_min(fi_mul(a_min, cmin_g), fi_mul(a_min, cmax_g)),
+                  fi_min(fi_mul(a_max, cmin_g), fi_mul(a_max, cmax_g)));
+    FI rho2 = fi_add(fi_sub(fi(1.0f), fi_mul(fi(2.0f), m)), fi_max(fi_sq(a_min), fi_sq(a_max)));
+    return t_gt(rho2, rho_max * rho_max);
+}
 DWD bool snch_visit3(const DNode3& nd, const DCone3& cn, F3 x, F3 v, float rho_max, float r_max, float& d_min) {
     float d_max;
     F3 mn = node3_mn(nd), mx = node3_mx(nd);
     box_dist(mn, mx, x, d_min, d_max);
     bool ov = d_min <= r_max;
-    float ha = cn.half_angle;
-    if ((ha >= 0.0f) && ov) {
-        ov = (ha > kPi2) && (d_min < kDrEps);
-        if (!ov) {
-            F3 c = (mn + mx) / 2.0f;
-            F3 vca = c - x;
-            float l = norm(vca);
-            vca = vca / l;
-            ov = l < cn.radius;
-            if (!ov) {
-                F3 axis = f3(cn.ax, cn.ay, cn.az);
-                float axis_angle = cr_acos(clampf(dot(axis, vca), -1.0f, 1.0f));
-                float vch = cr_asin(cn.radius / l);
-                float hs = ha + vch;
-                float amin = axis_angle - hs, amax = axis_angle + hs;
-                ov = (hs > kPi2) || ((kPi2 >= amin) && (kPi2 <= amax));
-                if (!ov) {
-                    float c0 = cr_cos(amin), c1 = cr_cos(amax);
-                    float cmin_a = fmn(c0, c1), cmax_a = fmx(c0, c1);
-                    if ((amin <= 0.0f) && (amax >= 0.0f)) cmax_a = fmx(cmax_a, 1.0f);
-                    float beta = cr_acos(clampf(dot(axis, v), -1.0f, 1.0f));
-                    float bmin = beta - ha, bmax = beta + ha;
-                    float b0 = cr_cos(bmin), b1 = cr_cos(bmax);
-                    float cmin_b = fmn(b0, b1), cmax_b = fmx(b0, b1);
-                    if ((bmin <= 0.0f) && (bmax >= 0.0f)) cmax_b = fmx(cmax_b, 1.0f);
-                    float gamma = cr_acos(clampf(dot(v, vca), -1.0f, 1.0f));
-                    float gmin = gamma - vch, gmax = gamma + vch;
-                    float g0 = cr_cos(gmin), g1 = cr_cos(gmax);
-                    float cmin_g = fmn(g0, g1), cmax_g = fmx(g0, g1);
-                    if ((gmin <= 0.0f) && (gmax >= 0.0f)) cmax_g = fmx(cmax_g, 1.0f);
-                    float a_min = cmin_b / cmax_a, a_max = cmax_b / cmin_a;
-                    float b_min = cmin_g, b_max = cmax_g;
-                    float rho2 = (1.0f - 2.0f * fmn(fmn(a_min * b_min, a_min * b_max), fmn(a_max * b_min, a_max * b_max))) +
-                                 fmx(a_min * a_min, a_max * a_max);
-                    ov = rho2 > rho_max * rho_max;
-                }
-            }
-        }
+    if ((cn.half_angle >= 0.0f) && ov) {
+        int t = rho_cone3_bracketed(mn, mx, cn, x, v, rho_max, d_min);
+        ov = t == kU ? rho_cone3_fp64(mn, mx, cn, x, v, rho_max, d_min) : t == kT;
     }
     return ov;
 }
@@ -678,37 +755,44 @@ DWD float tri_star_radius2(const DTri& tr, F3 x, F3 e, float rho_max, float r_ma
     return R;
 }
 
-// Mesh.star_radius_2 (mesh.py:349-361): the star radius, then SNCH3D.star_radius_3
-// (snch3d.py:742-811) over the triangles
-DWD float star_radius3_rho(const SubDev& S, F3 x, F3 e, float rho_max) {
+// The SNCH traversal of SNCH3D.star_radius_3 (snch3d.py:742-811) in
+// resumable form: rho3_begin pushes the root when its box is within r0;
+// each rho3_iter pops one node (a leaf's triangles bound r_max, an inner
+// node's overlapping children are pushed, the nearer on top) and reports
+// whether the traversal has ended.  The answer is r_max.
+struct Rho3 {
+    float r_max;
+    int sp;
+};
+DWD bool rho3_begin(Rho3& r, Ovf& o, const SubDev& S, F3 x, float r0) {
+    const DNode3* nodes = (const DNode3*)S.nodes;
+    r.r_max = r0;
+    r.sp = 0;
+    float dmin, dmax;
+    box_dist(node3_mn(nodes[0]), node3_mx(nodes[0]), x, dmin, dmax);
+    if (!(dmin <= r0)) return false;
+    stk_push(r.sp, o, 0, dmin);
+    return true;
+}
+DWD bool rho3_iter(Rho3& r, Ovf& o, const SubDev& S, F3 x, F3 e, float rho_max) {
     const DNode3* nodes = (const DNode3*)S.nodes;
     const DCone3* cones = (const DCone3*)S.cones;
     const DTri* prims = (const DTri*)S.prims;
-    float r_max = star_radius3<true>(S, x, kInf);
-    float dmin, dmax;
-    box_dist(node3_mn(nodes[0]), node3_mx(nodes[0]), x, dmin, dmax);
-    if (!(dmin <= r_max)) return r_max;
-    int st_i[DW_MAX_STACK];
-    float st_d[DW_MAX_STACK];
-    int sp = 0;
-    st_i[0] = 0;
-    st_d[0] = dmin;
-    while (sp >= 0) {
-        int ni = st_i[sp];
-        float cd = st_d[sp];
-        sp--;
-        if (!(cd <= r_max)) continue;
+    int ni;
+    float cd;
+    stk_pop(r.sp, o, ni, cd);
+    if (cd <= r.r_max) {
         const DNode3 nd = nodes[ni];
         if (nd.nref > 0) {
             for (int j = 0; j < nd.nref; ++j) {
-                float r = tri_star_radius2(prims[nd.off + j], x, e, rho_max, r_max);
-                if (r < r_max) r_max = r;
+                float rr = tri_star_radius2(prims[nd.off + j], x, e, rho_max, r.r_max);
+                if (rr < r.r_max) r.r_max = rr;
             }
         } else {
             int L = ni + 1, R = ni + nd.off;
             float d0, d1;
-            bool h0 = snch_visit3(nodes[L], cones[L], x, e, rho_max, r_max, d0);
-            bool h1 = snch_visit3(nodes[R], cones[R], x, e, rho_max, r_max, d1);
+            bool h0 = snch_visit3(nodes[L], cones[L], x, e, rho_max, r.r_max, d0);
+            bool h1 = snch_visit3(nodes[R], cones[R], x, e, rho_max, r.r_max, d1);
             if (h0 && h1) {
                 int closer = L, other = R;
                 if (d1 < d0) {
@@ -716,20 +800,26 @@ DWD float star_radius3_rho(const SubDev& S, F3 x, F3 e, float rho_max) {
                     float tmp = d0;
                     d0 = d1, d1 = tmp;
                 }
-                st_i[++sp] = other;
-                st_d[sp] = d1;
-                st_i[++sp] = closer;
-                st_d[sp] = d0;
+                stk_push(r.sp, o, other, d1);
+                stk_push(r.sp, o, closer, d0);
             } else if (h0) {
-                st_i[++sp] = L;
-                st_d[sp] = d0;
+                stk_push(r.sp, o, L, d0);
             } else if (h1) {
-                st_i[++sp] = R;
-                st_d[sp] = d1;
+                stk_push(r.sp, o, R, d1);
             }
         }
     }
-    return r_max;
+    return r.sp == 0;
+}
+// Mesh.star_radius_2 (mesh.py:349-361): the star radius, then SNCH3D.star_radius_3
+// (snch3d.py:742-811) over the triangles
+DWD float star_radius3_rho(const SubDev& S, F3 x, F3 e, float rho_max) {
+    Rho3 r;
+    Ovf o;
+    if (rho3_begin(r, o, S, x, star_radius3<true>(S, x, kInf)))
+        while (!rho3_iter(r, o, S, x, e, rho_max)) {
+        }
+    return r.r_max;
 }
 
 DWD void grad_init(const SceneDev& sc, GradWalk<3>& g, F3 x, F3 e) {
@@ -742,132 +832,166 @@ DWD void grad_init(const SceneDev& sc, GradWalk<3>& g, F3 x, F3 e) {
     g.i = 0;
     g.thr = 1.0f;
 }
-// One step of WoStGrad3D.dude (wost_grad_3d.py:93-206) on an all-Neumann scene
-DWD void grad_step3(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, const GradDev& G, GradWalk<3>& g,
-                    Rng& rng, int& steps) {
-    const SubDev& F = sc.sub[0];
+// One step of WoStGrad3D.dude (wost_grad_3d.py:93-206) on an all-Neumann
+// scene, in the parts around its three queries (the star radius's silhouette
+// search and rho-bounded traversal, then the ray), so a scheduler can run the
+// queries one traversal iteration at a time.  GradStep3 carries a step's
+// values from one part to the next.
+struct GradStep3 {
+    float R0;  // the silhouette search radius (star_bound)
+    float R;   // the ball radius
+    F3 dir, org, prev;
+};
+// Step start: a point on the boundary moves 1e-3 inside before the star radius.
+DWD void grad3_start(const SceneDev& sc, GradWalk<3>& g) {
+    if (sc.sub[2].present && g.ob) g.ip = g.ip - g.in * 1e-3f;
+}
+// With the star radius Rrho of the Neumann part: the ball radius, the boundary
+// and silhouette samples (Mesh.sample_boundary / sample_silhouette with
+// handle_boundary / handle_silhouette) and the direction; the ray is then
+// (s.org, s.dir) up to s.R.
+DWD void grad3_sample(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, GradWalk<3>& g, Rng& rng, float Rrho,
+                      GradStep3& s) {
     const SubDev& Nm = sc.sub[2];
-    F3 ip = g.ip, in = g.in, e = g.e;
-    bool ob = g.ob, active = g.active;
+    const F3 ip = g.ip, in = g.in, e = g.e;
+    const bool ob = g.ob;
+    float result = g.result;
+    bool inflate = false;
+    float R = kInf;
+    if (Nm.present) {
+        R = fmn(R, Rrho);
+        if (ob) {
+            if (R < P.min_R) inflate = true;
+            R = fmx(R, P.min_R);
+        }
+    }
+    float alpha = ob ? 2.0f : 1.0f;
+    if (Nm.present) {
+        {  // Mesh.sample_boundary (mesh.py:474-490) + handle_boundary (wost_grad_3d.py:22-45)
+            float rnd, pmf;
+            int idx = cdf_sample(Nm, rng.next(), rnd, pmf);
+            const float* tp = Nm.local_prims + 9 * idx;
+            F3 a = f3(tp[0], tp[1], tp[2]), b = f3(tp[3], tp[4], tp[5]), c = f3(tp[6], tp[7], tp[8]);
+            F3 cr = cross(b - a, c - a);
+            float area = norm(cr) / 2.0f;
+            float r1 = rng.next(), r2 = rng.next();
+            float su = __builtin_sqrtf(r1) * (1.0f - r2), sv = __builtin_sqrtf(r1) * r2;
+            F3 bp = (b * su + c * sv) + a * ((1.0f - su) - sv);
+            F3 bn = normalize(cr);
+            float bpdf = pmf / area;
+            if (!inflate) {
+                float Gv = clampf(greens3_G(ip, R, ip, bp), -100.0f, 100.0f);
+                float res = 0.0f;
+                if (norm(ip - bp) <= R) {
+                    float e_n = dot(e, bn);
+                    F3 e_t = e - bn * e_n;
+                    F3 H[3];
+                    bc_hess3(bc, bp, H);
+                    F3 He = f3(dot(H[0], e_t), dot(H[1], e_t), dot(H[2], e_t));
+                    float f_tn = dot(He, bn);
+                    res += f_tn - e_n * src_f(bc, bp);
+                    if (P.cv) res -= dot(f3(0.0f, 0.0f, 0.0f), bn);
+                }
+                result += alpha * (-Gv * res / bpdf);
+            }
+        }
+        if (Nm.n_psil3 > 0) {  // Mesh.sample_silhouette (mesh.py:500-516) + handle_silhouette (:47-71)
+            float rnd, pmf;
+            int idx = psil3_sample(Nm, rng.next(), rnd, pmf);
+            const float* q = Nm.psil3 + 12 * idx;
+            F3 a = f3(q[0], q[1], q[2]), b = f3(q[3], q[4], q[5]), c = f3(q[6], q[7], q[8]), d = f3(q[9], q[10], q[11]);
+            float length = norm(b - a);
+            F3 n1 = normalize(cross(b - a, c - a));
+            F3 n2 = normalize(cross(d - a, b - a));
+            F3 t1 = normalize(cross(b - a, n1));
+            F3 t2 = normalize(cross(a - b, n2));
+            F3 sp = lerp(a, b, rng.next());
+            float spdf = pmf / length;
+            if (!inflate) {
+                float Gv = clampf(greens3_G(ip, R, ip, sp), -100.0f, 100.0f);
+                float res = 0.0f;
+                if (norm(sp - ip) <= R) {
+                    F3 gu = bc_grad(bc, sp);
+                    float h1 = dot(gu, n1), h2 = dot(gu, n2);
+                    float e1 = dot(e, t1), e2 = dot(e, t2);
+                    if (P.cv) {
+                        h1 = h1 - dot(f3(0.0f, 0.0f, 0.0f), n1);
+                        h2 = h2 - dot(f3(0.0f, 0.0f, 0.0f), n2);
+                    }
+                    res -= h1 * e1;
+                    res -= h2 * e2;
+                }
+                result += alpha * (-Gv * res / spdf);
+            }
+        }
+    }
+    F3 dir = uniform_sphere(rng);
+    if (ob && (dot(dir, in) > 0.0f)) dir = -dir;
+    s.prev = ip;
+    s.org = ob ? ip - in * 1e-5f : ip;  // Mesh.intersect (mesh.py:384-415)
+    s.dir = dir;
+    s.R = R;
+    g.result = result;
+}
+// With the ray's hit: the move, handle_source, the tangent reflection of e,
+// Russian roulette.
+DWD void grad3_finish(const BCDev& bc, const GradDev& G, GradWalk<3>& g, Rng& rng, const GradStep3& s, Hit its) {
+    const F3 o = s.org, dir = s.dir, _p = s.prev;
+    const float R = s.R;
+    F3 ip, in, e = g.e;
+    bool ob, active = g.active;
     float result = g.result;
     int i = g.i;
-    {
-        bool inflate = false;
-        float R = kInf;
-        if (Nm.present) {
-            if (ob) ip = ip - in * 1e-3f;
-            R = fmn(R, star_radius3_rho(Nm, ip, e, G.rho_max));
-            if (ob) {
-                if (R < P.min_R) inflate = true;
-                R = fmx(R, P.min_R);
-            }
-        }
-        float alpha = ob ? 2.0f : 1.0f;
-        if (Nm.present) {
-            {  // Mesh.sample_boundary (mesh.py:474-490) + handle_boundary (wost_grad_3d.py:22-45)
-                float rnd, pmf;
-                int idx = cdf_sample(Nm, rng.next(), rnd, pmf);
-                const float* tp = Nm.local_prims + 9 * idx;
-                F3 a = f3(tp[0], tp[1], tp[2]), b = f3(tp[3], tp[4], tp[5]), c = f3(tp[6], tp[7], tp[8]);
-                F3 cr = cross(b - a, c - a);
-                float area = norm(cr) / 2.0f;
-                float r1 = rng.next(), r2 = rng.next();
-                float su = __builtin_sqrtf(r1) * (1.0f - r2), sv = __builtin_sqrtf(r1) * r2;
-                F3 bp = (b * su + c * sv) + a * ((1.0f - su) - sv);
-                F3 bn = normalize(cr);
-                float bpdf = pmf / area;
-                if (!inflate) {
-                    float Gv = clampf(greens3_G(ip, R, ip, bp), -100.0f, 100.0f);
-                    float res = 0.0f;
-                    if (norm(ip - bp) <= R) {
-                        float e_n = dot(e, bn);
-                        F3 e_t = e - bn * e_n;
-                        F3 H[3];
-                        bc_hess3(bc, bp, H);
-                        F3 He = f3(dot(H[0], e_t), dot(H[1], e_t), dot(H[2], e_t));
-                        float f_tn = dot(He, bn);
-                        res += f_tn - e_n * src_f(bc, bp);
-                        if (P.cv) res -= dot(f3(0.0f, 0.0f, 0.0f), bn);
-                    }
-                    result += alpha * (-Gv * res / bpdf);
-                }
-            }
-            if (Nm.n_psil3 > 0) {  // Mesh.sample_silhouette (mesh.py:500-516) + handle_silhouette (:47-71)
-                float rnd, pmf;
-                int idx = psil3_sample(Nm, rng.next(), rnd, pmf);
-                const float* q = Nm.psil3 + 12 * idx;
-                F3 a = f3(q[0], q[1], q[2]), b = f3(q[3], q[4], q[5]), c = f3(q[6], q[7], q[8]), d = f3(q[9], q[10], q[11]);
-                float length = norm(b - a);
-                F3 n1 = normalize(cross(b - a, c - a));
-                F3 n2 = normalize(cross(d - a, b - a));
-                F3 t1 = normalize(cross(b - a, n1));
-                F3 t2 = normalize(cross(a - b, n2));
-                F3 sp = lerp(a, b, rng.next());
-                float spdf = pmf / length;
-                if (!inflate) {
-                    float Gv = clampf(greens3_G(ip, R, ip, sp), -100.0f, 100.0f);
-                    float res = 0.0f;
-                    if (norm(sp - ip) <= R) {
-                        F3 gu = bc_grad(bc, sp);
-                        float h1 = dot(gu, n1), h2 = dot(gu, n2);
-                        float e1 = dot(e, t1), e2 = dot(e, t2);
-                        if (P.cv) {
-                            h1 = h1 - dot(f3(0.0f, 0.0f, 0.0f), n1);
-                            h2 = h2 - dot(f3(0.0f, 0.0f, 0.0f), n2);
-                        }
-                        res -= h1 * e1;
-                        res -= h2 * e2;
-                    }
-                    result += alpha * (-Gv * res / spdf);
-                }
-            }
-        }
-        F3 dir = uniform_sphere(rng);
-        if (ob && (dot(dir, in) > 0.0f)) dir = -dir;
-        F3 _p = ip;
-        F3 o = ob ? ip - in * 1e-5f : ip;  // Mesh.intersect (mesh.py:384-415)
-        Hit its = intersect3(F, o, dir, R);
-        if (!its.valid) {
-            its.p = o + dir * R;
-            its.n = dir;
-            its.d = R;
-            its.on_boundary = false;
-        }
-        ip = its.p;
-        in = its.n;
-        ob = its.on_boundary;
-        result += dot(_p - ip, f3(0.0f, 0.0f, 0.0f));
-        if (!G.ignore_source) {  // handle_source (wost_grad_3d.py:73-90)
-            F3 spt;
-            float sr_, spdf;
-            greens3_sample(_p, R, dir, rng, spt, sr_, spdf);
-            float res = 0.0f;
-            if (norm(spt - _p) < its.d) res += (R * R / 6.0f) * dot(src_grad(bc), e);
-            result += res;
-        }
-        if (ob) {
-            float e_n = dot(e, in);
-            F3 e_t = e - in * e_n;
-            float dir_n = dot(dir, in);
-            F3 dir_t = dir - in * dir_n;
-            F3 ref_t = f3(0.0f, 0.0f, 0.0f) - in * dot(f3(0.0f, 0.0f, 0.0f), in);
-            result += dot(dir_t, ref_t) * e_n / dir_n;
-            result += e_n * dot(in, bc_grad(bc, ip));
-            e = e_t - (dir_t * e_n) / dir_n;
-        }
-        if (norm(e) < 1.0f) {  // Russian roulette
-            if (norm(e) < rng.next())
-                active = false;
-            else
-                e = e / norm(e);
-        }
-        i += 1;
-        if (bad(result) || __builtin_fabsf(result) > 1000.0f) result = 0.0f;
+    if (!its.valid) {
+        its.p = o + dir * R;
+        its.n = dir;
+        its.d = R;
+        its.on_boundary = false;
     }
+    ip = its.p;
+    in = its.n;
+    ob = its.on_boundary;
+    result += dot(_p - ip, f3(0.0f, 0.0f, 0.0f));
+    if (!G.ignore_source) {  // handle_source (wost_grad_3d.py:73-90)
+        F3 spt;
+        float sr_, spdf;
+        greens3_sample(_p, R, dir, rng, spt, sr_, spdf);
+        float res = 0.0f;
+        if (norm(spt - _p) < its.d) res += (R * R / 6.0f) * dot(src_grad(bc), e);
+        result += res;
+    }
+    if (ob) {
+        float e_n = dot(e, in);
+        F3 e_t = e - in * e_n;
+        float dir_n = dot(dir, in);
+        F3 dir_t = dir - in * dir_n;
+        F3 ref_t = f3(0.0f, 0.0f, 0.0f) - in * dot(f3(0.0f, 0.0f, 0.0f), in);
+        result += dot(dir_t, ref_t) * e_n / dir_n;
+        result += e_n * dot(in, bc_grad(bc, ip));
+        e = e_t - (dir_t * e_n) / dir_n;
+    }
+    if (norm(e) < 1.0f) {  // Russian roulette
+        if (norm(e) < rng.next())
+            active = false;
+        else
+            e = e / norm(e);
+    }
+    i += 1;
+    if (bad(result) || __builtin_fabsf(result) > 1000.0f) result = 0.0f;
     g.ip = ip, g.in = in, g.e = e;
     g.ob = ob, g.active = active;
     g.result = result;
     g.i = i;
+}
+DWD void grad_step3(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, const GradDev& G, GradWalk<3>& g,
+                    Rng& rng, int& steps) {
+    const SubDev& Nm = sc.sub[2];
+    grad3_start(sc, g);
+    float Rrho = kInf;
+    if (Nm.present) Rrho = star_radius3_rho(Nm, g.ip, g.e, G.rho_max);
+    GradStep3 s;
+    grad3_sample(sc, bc, P, g, rng, Rrho, s);
+    grad3_finish(bc, G, g, rng, s, intersect3(sc.sub[0], s.org, s.dir, s.R));
 }
 DWD float grad_dude3(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, const GradDev& G, F3 x, F3 e,
                      Rng& rng, int& steps) {

@@ -136,6 +136,60 @@ __global__ void __launch_bounds__(kBlock) k_star_radius(SceneDev sc, int sub, co
                          : star_radius3(S, f3(pts[3 * q], pts[3 * q + 1], pts[3 * q + 2]), rmax[q]);
 }
 
+// Polyline.star_radius_2 (polyline.py:317-330) / Mesh.star_radius_2
+// (mesh.py:349-361): the star radius bounded by the Poisson-kernel ratio
+// rho_max along direction dirs[q].
+__global__ void __launch_bounds__(kBlock) k_star_radius_rho(SceneDev sc, int sub, const float* __restrict__ pts,
+                                                            const float* __restrict__ dirs, float rho_max, int64_t n,
+                                                            float* out) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const SubDev& S = sc.sub[sub];
+    out[q] = sc.dim == 2 ? star_radius2_rho(S, f2(pts[2 * q], pts[2 * q + 1]), f2(dirs[2 * q], dirs[2 * q + 1]), rho_max)
+                         : star_radius3_rho(S, f3(pts[3 * q], pts[3 * q + 1], pts[3 * q + 2]),
+                                            f3(dirs[3 * q], dirs[3 * q + 1], dirs[3 * q + 2]), rho_max);
+}
+
+// Float order keys: consecutive floats have consecutive keys (-0 and +0 share 0).
+__host__ __device__ inline int32_t fkey(float f) {
+    int32_t b;
+    memcpy(&b, &f, 4);
+    return b < 0 ? (int32_t)(0x80000000u - (uint32_t)b) : b;
+}
+__device__ inline float key_float(int32_t k) {
+    return __int_as_float(k < 0 ? (int32_t)(0x80000000u - (uint32_t)k) : k);
+}
+// Largest distance in float steps between the fp32 library function W
+// (0 acosf, 1 asinf, 2 atanf, 3 cosf) and its once-rounded fp64 form, over the
+// `count` floats from key k0 up.
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_math_check(int32_t k0, int64_t count, int32_t* out) {
+    int worst = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+        float x = key_float((int32_t)(k0 + i));
+        float f, e;
+        if constexpr (W == 0) {
+            f = acosf(x), e = cr_acos(x);
+        } else if constexpr (W == 1) {
+            f = asinf(x), e = cr_asin(x);
+        } else if constexpr (W == 2) {
+            f = atanf(x), e = (float)atan((double)x);
+        } else {
+            f = cosf(x), e = cr_cos(x);
+        }
+        int64_t d = (int64_t)fkey(f) - (int64_t)fkey(e);
+        d = d < 0 ? -d : d;
+        if (__builtin_isnan(f) || __builtin_isnan(e)) d = (__builtin_isnan(f) && __builtin_isnan(e)) ? 0 : INT32_MAX;
+        worst = max(worst, (int)min(d, (int64_t)INT32_MAX));
+    }
+    for (int o = 32; o > 0; o >>= 1) worst = max(worst, __shfl_xor(worst, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out + W, worst);
+}
+__global__ void k_math_check_init(int32_t* out) {
+    if (threadIdx.x < 4) out[threadIdx.x] = 0;
+    if (threadIdx.x == 4) out[4] = kTrSteps;
+}
+
 // Solver.solve (solver.py:36-71): lane = i*W + k owns the PCG32 stream
 // PCG32(size=N*W, TEA(seed, i)) at offset `lane`; nsubwalks walks averaged.
 __global__ void __launch_bounds__(kBlock) k_walk_u(SceneDev sc, BCDev bc, ParamsDev P, const float* __restrict__ pts,
@@ -2526,6 +2580,34 @@ int dw_star_radius(const dw_scene* s, int32_t sub, const float* pts, const float
     if (n == 0) return DW_OK;
     hipLaunchKernelGGL(k_star_radius, dim3(blocks_for(n)), dim3(kBlock), kStackLds, (hipStream_t)stream, s->dev, sub, pts,
                        rmax, n, radius);
+    DW_HIP(hipGetLastError());
+    return DW_OK;
+}
+
+int dw_star_radius_rho(const dw_scene* s, int32_t sub, const float* pts, const float* dirs, float rho_max,
+                       int64_t n, float* radius, void* stream) {
+    int rc = check_scene(s, sub);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!pts || !dirs || !radius))) return fail(DW_ERR_ARG, "bad arrays");
+    if (!(rho_max > 0.0f)) return fail(DW_ERR_ARG, "rho_max must be > 0");
+    if (n == 0) return DW_OK;
+    hipLaunchKernelGGL(k_star_radius_rho, dim3(blocks_for(n)), dim3(kBlock), kStackLds, (hipStream_t)stream, s->dev,
+                       sub, pts, dirs, rho_max, n, radius);
+    DW_HIP(hipGetLastError());
+    return DW_OK;
+}
+
+int dw_math_check(int32_t* out, void* stream) {
+    if (!out) return fail(DW_ERR_ARG, "bad arrays");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_math_check_init, dim3(1), dim3(64), 0, st, out);
+    const dim3 grid(4096), block(kBlock);
+    const int32_t a = fkey(-1.0f), b = fkey(1.0f), z = fkey(0.0f), inf = fkey(kInf);
+    const int32_t c0 = fkey(kCosLo), c1 = fkey(kCosHi);
+    hipLaunchKernelGGL(k_math_check<0>, grid, block, 0, st, a, (int64_t)b - a + 1, out);
+    hipLaunchKernelGGL(k_math_check<1>, grid, block, 0, st, a, (int64_t)b - a + 1, out);
+    hipLaunchKernelGGL(k_math_check<2>, grid, block, 0, st, z, (int64_t)inf - z + 1, out);
+    hipLaunchKernelGGL(k_math_check<3>, grid, block, 0, st, c0, (int64_t)c1 - c0 + 1, out);
     DW_HIP(hipGetLastError());
     return DW_OK;
 }

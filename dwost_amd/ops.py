@@ -336,3 +336,28 @@ def _backward_fused(ctx, dl_du, dl_dgrad, _flags, _steps):
 
 register_autograd("dwost::solve_u", _backward_u, setup_context=_setup)
 register_autograd("dwost::solve_fused", _backward_fused, setup_context=_setup)
+
+
+@custom_op("dwost::star_radius_2", mutates_args=())
+def star_radius_2(scene: int, sub: int, pts: torch.Tensor, e: torch.Tensor, rho_max: float) -> torch.Tensor:
+    """Polyline.star_radius_2 / Mesh.star_radius_2 (rho-bounded star radius)."""
+    out = torch.empty(pts.shape[0], device=pts.device)
+    check(lib().dw_star_radius_rho(_scene(scene), sub, ptr(pts), ptr(e), float(rho_max), pts.shape[0], ptr(out),
+                                   stream_handle(pts.device)))
+    return out
+
+
+@star_radius_2.register_fake
+def _(scene, sub, pts, e, rho_max):
+    return pts.new_empty(pts.shape[0])
+
+
+def math_check(device=None) -> dict:
+    """Largest float-step distance between acosf / asinf / atanf / cosf and
+    their once-rounded fp64 forms over each whole domain, and the bound the
+    bracketed cone tests assume (dw_math_check)."""
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    out = torch.empty(5, dtype=torch.int32, device=dev)
+    check(lib().dw_math_check(ptr(out), stream_handle(dev)))
+    v = out.cpu().tolist()
+    return dict(acos=v[0], asin=v[1], atan=v[2], cos=v[3], bound=v[4])

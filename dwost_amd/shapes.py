@@ -315,6 +315,21 @@ class _Scene:
         check(lib().dw_star_radius(self._h, sub, ptr(p), ptr(r), n, ptr(out), stream_handle(p.device)))
         return out
 
+    def star_radius_2(self, p, e, rho_max: float | None = None, sub: int = SUB_FULL) -> torch.Tensor:
+        """Polyline.star_radius_2 (polyline.py:317-330) / Mesh.star_radius_2
+        (mesh.py:349-361, r_max = inf): the star radius along e bounded by the
+        Poisson-kernel ratio rho_max (reference defaults 1.0 in 2D, 1.01 in 3D)."""
+        p = _as_points(p, self.dim)
+        e = _as_points(e, self.dim).to(p.device)
+        if e.shape != p.shape:
+            raise ValueError("e must have one direction per point")
+        if rho_max is None:
+            rho_max = 1.0 if self.dim == 2 else 1.01
+        out = torch.empty(p.shape[0], device=p.device)
+        check(lib().dw_star_radius_rho(self._h, sub, ptr(p), ptr(e), float(rho_max), p.shape[0], ptr(out),
+                                       stream_handle(p.device)))
+        return out
+
 
 def _np(a):
     if isinstance(a, torch.Tensor):

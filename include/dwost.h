@@ -216,6 +216,22 @@ int dw_query_baseline(const dw_scene* scene, int32_t sub, int32_t kind, const fl
 int dw_star_radius(const dw_scene* scene, int32_t sub, const float* pts, const float* rmax, int64_t n,
                    float* radius, void* stream);
 
+/* Polyline.star_radius_2 (polyline.py:317-330) / Mesh.star_radius_2
+ * (mesh.py:349-361): the star radius along direction dirs[n*dim] bounded by
+ * the Poisson-kernel ratio rho_max (WoStGrad's ball radius on the Neumann
+ * part).  radius[n]. */
+int dw_star_radius_rho(const dw_scene* scene, int32_t sub, const float* pts, const float* dirs, float rho_max,
+                       int64_t n, float* radius, void* stream);
+
+/* Device math self-check; no counterpart in the reference.  The cone tests
+ * decide from fp32 acosf / asinf / atanf / cosf values bracketed by a bound of
+ * out[4] float steps to the fp64-evaluated, once-rounded values, running the
+ * fp64 form only when a bracket straddles a threshold.  out (DEVICE, 5 int32)
+ * receives the largest distance found over every float of each domain:
+ * out[0] acos and out[1] asin on [-1, 1], out[2] atan on [0, inf], out[3] cos
+ * on [-4, 8]; out[4] = the bound assumed.  A few seconds of GPU time. */
+int dw_math_check(int32_t* out, void* stream);
+
 /* Solver.solve with WoSt.u / WoSt3D.u (solver.py:36-71, wost.py:145-221,
  * wost_3d.py:43-106).  u[n] = per-point mean; optional per-walk outputs
  * walk_u / walk_steps / walk_term / walk_flags [n*nwalks]; total_steps (DEVICE uint64) =

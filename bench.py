@@ -15,12 +15,14 @@ all 262,144 points, bit-identical to a single launch over all of them.
 Inputs are resident in HBM before timing starts.  Prints one JSON line; its
 extra keys measure the 2D holes workload (4096-segment star with eight
 64-segment Neumann holes, 65,536 lattice points x 1024 walks, eps 1e-4), the
-3D adjoint (fwd+bwd / fwd) and the 64-scene batch.
+3D adjoint (fwd+bwd / fwd), the 64-scene batch and the WoStGrad / WoStGrad3D
+du/dx estimators (walks/s beside the CPU restatement on the same walks).
 """
 
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -228,6 +230,77 @@ def secondary_2d(progress: bool, reps: int = 2) -> dict:
                 mean_steps_per_walk=round(steps / (reps * len(q) * c["nwalks"]), 3), **acc)
 
 
+def wostgrad_measure(points_2d: int = 4096, points_3d: int = 1024, nwalks: int = 64, cpu_s: float = 6.0) -> dict:
+    """WoStGrad on the 256-segment star polygon (mixed Dirichlet / Neumann,
+    u = e^x cos y) and WoStGrad3D on the 20,480-triangle icosphere (all
+    Neumann, u = x): du/dx at interior points x `nwalks` walks, nsteps 1024,
+    one timed solve each after a short warm-up; the CPU restatement (same
+    walks, same seeds) on a sample of the same points, all host cores."""
+    import numpy as np
+    import torch
+
+    import dwost_amd
+    from dwost_amd import scenes
+    from oracle import oracle as O
+
+    O.build()
+    out = {}
+    for dim in (2, 3):
+        if dim == 2:
+            spec = scenes.star_scene_2d(n_query=points_2d)
+            types, code, coeffs = spec.types, spec.bc_code, (0.0, 0.0, 0.0, 0.0)
+            scene = dwost_amd.Scene2D(spec.vertices, spec.indices, types, dwost_amd.BoundaryCondition(code))
+            solver = dwost_amd.WoStGrad(eps=1e-3, nwalks=nwalks, nsteps=1024)
+            workload = f"star polygon 256 segments, mixed, u = e^x cos y, {points_2d} points x {nwalks} walks"
+        else:
+            spec = scenes.icosphere_scene(level=5, n_query=points_3d)
+            types, code, coeffs = np.zeros_like(spec.types), 2, (0.0, 1.0, 0.0, 0.0)
+            scene = dwost_amd.Scene3D(spec.vertices, spec.indices, types, dwost_amd.BoundaryCondition(code, coeffs))
+            solver = dwost_amd.WoStGrad3D(eps=1e-3, nwalks=nwalks, nsteps=1024)
+            workload = f"icosphere 20480 triangles, all Neumann, u = x, {points_3d} points x {nwalks} walks"
+        pts = np.ascontiguousarray(spec.queries.astype(np.float32))
+        e = np.zeros_like(pts)
+        e[:, 0] = 1.0
+        q, qe = torch.as_tensor(pts).cuda(), torch.as_tensor(e).cuda()
+        warm = dataclasses.replace(solver, nsteps=4, nwalks=1)
+        warm.solve_dude(q[:64], qe[:64], scene, seed=7)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = solver.solve_dude(q, qe, scene, seed=1, count_steps=True, per_walk=True)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ws = r["walk_steps"].cpu().numpy()
+        steps = int(r["steps"][0])
+        dude = r["dude"].cpu().numpy().astype(np.float64)
+        x = pts.astype(np.float64)
+        exact = np.exp(x[:, 0]) * np.cos(x[:, 1]) if dim == 2 else np.ones(len(x))
+        # the CPU restatement on the first k points, k grown until ~cpu_s seconds
+        sc = O.OracleScene(spec.vertices, spec.indices, types)
+        k, ct, cw, same = 16, 0.0, 0, []
+        while ct < cpu_s and cw < len(pts):
+            lo, hi = cw, min(len(pts), cw + k)
+            t1 = time.perf_counter()
+            c = sc.grad_dude(pts[lo:hi], e[lo:hi], code, nwalks, nsteps=1024, eps=1e-3, min_R=solver.min_R,
+                             rho_max=solver.rho_max, coeffs=coeffs, seed=1, base=lo, per_walk=True)
+            ct += time.perf_counter() - t1
+            gw = r["walk_dude"][lo * nwalks:hi * nwalks].cpu().numpy()
+            same.append(c["walk_dude"].view(np.uint32) == gw.view(np.uint32))
+            cw = hi
+            k *= 2
+        cpu_walks = cw * nwalks / ct
+        out[f"{dim}d"] = dict(
+            workload=workload, walks_per_s=round(len(pts) * nwalks / wall, 1), walk_steps_per_s=round(steps / wall, 1),
+            ms=round(1000 * wall, 1), mean_steps_per_walk=round(float(ws.mean()), 2), max_steps=int(ws.max()),
+            walks_at_step_cap=int((ws >= 1024).sum()), rmse_vs_analytic=float(np.sqrt(np.mean((dude - exact) ** 2))),
+            cpu_baseline=dict(walks_per_s=round(cpu_walks, 1), cores=O.num_threads(), kind="port",
+                              sample=f"first {cw} points x {nwalks} walks ({ct:.1f} s), same seeds, OpenMP"),
+            cpu_walks_bit_identical=float(np.concatenate(same).mean()),
+            gpu_over_cpu=round(len(pts) * nwalks / wall / cpu_walks, 1))
+        print(f"[bench] wostgrad {dim}d {out[f'{dim}d']['walks_per_s']:.3e} walks/s "
+              f"({out[f'{dim}d']['gpu_over_cpu']}x CPU)", file=sys.stderr, flush=True)
+    return out
+
+
 def adjoint_ratio(points: int, reps: int = 2):
     """fwd+bwd / fwd time on the 3D adjoint workload (noisy icosphere, 40,962
     vertices, trainable per-vertex g and h, loss = MSE(u) + MSE(grad u) vs
@@ -326,6 +399,7 @@ def main():
     ap.add_argument("--points", type=int, default=0, help="use only the first N query points (0: all)")
     ap.add_argument("--progress", action="store_true", help="print a line after each timed step")
     ap.add_argument("--no-2d", action="store_true", help="skip the 2D holes measurement")
+    ap.add_argument("--no-wostgrad", action="store_true", help="skip the WoStGrad / WoStGrad3D measurement")
     ap.add_argument("--batch-queries", type=int, default=-1,
                     help="query cap per scene of the 64-scene batched-vs-per-scene measurement "
                          "(-1: uncapped, 0: skip)")
@@ -427,6 +501,10 @@ def main():
         bat = batch_speedup(None if args.batch_queries < 0 else args.batch_queries)
         print(f"[bench] batch speedup {bat['speedup']}", file=sys.stderr, flush=True)
 
+    grad = None
+    if not args.no_wostgrad:
+        grad = wostgrad_measure()
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(spec, cfg)
@@ -456,6 +534,7 @@ def main():
         "holes_2d": holes,
         "adjoint": adj,
         "batch": bat,
+        "wostgrad": grad,
     }
     print(json.dumps(line), flush=True)
 

@@ -376,28 +376,41 @@ DWD void off_centered_ball2(const SubDev& F, F2 p, F2 n, F2& c, float& radius) {
 }
 
 // WoStGrad.dudn (wost_grad.py:207-254): the normal derivative at a Dirichlet
-// point from one WoSt walk (nsteps 200, eps 1e-4) started on the off-centred ball
-DWD float grad_dudn2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, const CP& its, Rng& rng,
-                     int& steps) {
-    const SubDev& F = sc.sub[0];
-    const SubDev& Dn = sc.sub[1];
-    const SubDev& Nm = sc.sub[2];
+// point from one WoSt walk (nsteps 200, eps 1e-4) started on the off-centred
+// ball, in the parts before and after that walk, so a scheduler can run the
+// walk a step at a time.  Dudn2 carries the nested walk and the values the
+// second part needs (and, for the caller, the tangent frame and dg/dt).
+struct Dudn2 {
+    Walk<2> w;
+    F2 c, p, ip, nrm, tg;
+    float R, theta, u_ref, dudt;
+};
+DWD ParamsDev dudn_params(const ParamsDev& P) {
+    ParamsDev W = P;
+    W.nwalks = 1, W.nsubwalks = 1, W.nsteps = 200, W.eps = 1e-4f, W.min_R = 1e-3f, W.cv = 1, W.av = 1;
+    W.flags = DW_SOLVE_EXACT_CONES;
+    return W;
+}
+DWD void dudn2_begin(const SceneDev& sc, const BCDev& bc, const CP& its, Rng& rng, Dudn2& d) {
     F2 ip = f2(its.p.x, its.p.y), in = f2(its.n.x, its.n.y);
-    F2 c;
-    float R;
-    off_centered_ball2(F, ip, in, c, R);
+    off_centered_ball2(sc.sub[0], ip, in, d.c, d.R);
     float theta = (rng.next() * 2.0f) * kPi;  // uniform_angle
     theta = clampf(theta, 3e-1f, kTwoPi - 3e-1f);
     F2 pd = f2(-in.y, in.x);
     F2 dir = in * cr_cos(theta) + pd * cr_sin(theta);
-    F2 p = c + dir * R;
-    float u_ref = bc_u(bc, its.p);
-    ParamsDev W = P;
-    W.nwalks = 1, W.nsubwalks = 1, W.nsteps = 200, W.eps = 1e-4f, W.min_R = 1e-3f, W.cv = 1, W.av = 1;
-    W.flags = DW_SOLVE_EXACT_CONES;
-    int wst, wtm;
-    float u = walk_u2(sc, bc, W, p, rng, wst, wtm);
-    steps += wst;
+    d.p = d.c + dir * d.R;
+    d.u_ref = bc_u(bc, its.p);
+    d.theta = theta;
+    d.ip = ip;
+    walk_init<2>(d.w, d.p);
+}
+DWD float dudn2_end(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, const Dudn2& d, Rng& rng, int& steps) {
+    const SubDev& Dn = sc.sub[1];
+    const SubDev& Nm = sc.sub[2];
+    const F2 c = d.c, p = d.p, ip = d.ip;
+    const float R = d.R, theta = d.theta, u_ref = d.u_ref;
+    float u = walk_value(d.w, dudn_params(P));
+    steps += d.w.i;
     if (Nm.present) {
         float dd = closest2(Dn, p).d;
         float dnm = closest2(Nm, p).d;
@@ -448,9 +461,12 @@ DWD bool grad_running(const GradWalk<D>& g, const ParamsDev& P) {
     return (g.i < P.nsteps) && g.active;
 }
 
-// One step of WoStGrad.dude (wost_grad.py:91-205), from an interior point
-DWD void grad_step2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, const GradDev& G, GradWalk<2>& g,
-                    Rng& rng, int& steps) {
+// One step of WoStGrad.dude (wost_grad.py:91-205), from an interior point.
+// grad_step2_begin returns false when the step is done, true when it has
+// reached a Dirichlet point and the normal derivative's WoSt walk (d.w, run
+// with dudn_params) is pending; grad_step2_end then completes the step.
+DWD bool grad_step2_begin(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, const GradDev& G, GradWalk<2>& g,
+                          Rng& rng, Dudn2& d) {
     const SubDev& F = sc.sub[0];
     const SubDev& Dn = sc.sub[1];
     const SubDev& Nm = sc.sub[2];
@@ -462,14 +478,12 @@ DWD void grad_step2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, con
         bool inflate = false;
         float d_dir = kInf;
         if (Dn.present) d_dir = __builtin_fabsf(closest2(Dn, ip).d);
-        if (d_dir < P.eps) {
+        if (d_dir < P.eps) {  // a Dirichlet point: dg/dt and the normal derivative's walk
             CP cr = closest2(F, ip);
-            F2 nrm = f2(cr.n.x, cr.n.y), tg = f2(-nrm.y, nrm.x);
-            float dudt = grad_dgdt2(bc, f2(cr.p.x, cr.p.y), tg);
-            float dudn = grad_dudn2(sc, bc, P, cr, rng, steps);
-            result += thr * dudt * dot(e, tg);
-            result += thr * dudn * dot(e, nrm);
-            active = false;
+            d.nrm = f2(cr.n.x, cr.n.y), d.tg = f2(-d.nrm.y, d.nrm.x);
+            d.dudt = grad_dgdt2(bc, f2(cr.p.x, cr.p.y), d.tg);
+            dudn2_begin(sc, bc, cr, rng, d);
+            return true;
         } else {
             float R = d_dir;
             if (Nm.present) R = fmn(R, star_radius2_rho(Nm, ip, e, G.rho_max));
@@ -553,6 +567,28 @@ DWD void grad_step2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, con
     g.ob = ob, g.active = active;
     g.result = result, g.thr = thr;
     g.i = i;
+    return false;
+}
+DWD void grad_step2_end(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, GradWalk<2>& g, Rng& rng,
+                        const Dudn2& d, int& steps) {
+    float dudn = dudn2_end(sc, bc, P, d, rng, steps);
+    float result = g.result;
+    result += g.thr * d.dudt * dot(g.e, d.tg);
+    result += g.thr * dudn * dot(g.e, d.nrm);
+    g.active = false;
+    g.i += 1;
+    if (bad(result) || __builtin_fabsf(result) > 1000.0f) result = 0.0f;
+    g.result = result;
+}
+DWD void grad_step2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, const GradDev& G, GradWalk<2>& g,
+                    Rng& rng, int& steps) {
+    Dudn2 d;
+    if (grad_step2_begin(sc, bc, P, G, g, rng, d)) {
+        const ParamsDev W = dudn_params(P);
+        while (walk_step<true>(sc, bc, W, d.w, rng)) {
+        }
+        grad_step2_end(sc, bc, P, g, rng, d, steps);
+    }
 }
 DWD float grad_dude2(const SceneDev& sc, const BCDev& bc, const ParamsDev& P, const GradDev& G, F2 x, F2 e,
                      Rng& rng, int& steps) {

@@ -584,8 +584,9 @@ __global__ void __launch_bounds__(kBlock, 2) k_grad_dude(SceneDev sc, BCDev bc, 
 
 // The same walks as persistent jobs: a lane takes job i * nwalks + k from the
 // queue, runs its nsubwalks walks one step per loop iteration and takes the
-// next job as soon as they end, so a wave is not held by its longest walk (a
-// step, including a Dirichlet point's nested dudn walk, still runs whole).
+// next job as soon as they end, so a wave is not held by its longest walk.  In
+// 2D a Dirichlet point's nested dudn walk (up to 200 WoSt steps) also runs one
+// step per iteration, beside the other lanes' WoStGrad steps.
 template <int D>
 __global__ void __launch_bounds__(kBlock, 2) k_grad_dude_pers(SceneDev sc, BCDev bc, ParamsDev P, GradDev G,
                                                               const float* __restrict__ pts,
@@ -597,6 +598,9 @@ __global__ void __launch_bounds__(kBlock, 2) k_grad_dude_pers(SceneDev sc, BCDev
     float v = 0.0f;
     int st = 0, sub = 0;
     bool busy = false, done = false;
+    bool nested = false;  // 2D: a dudn walk in d is pending
+    Dudn2 d;
+    const ParamsDev W = dudn_params(P);
     QState qs;
     q_start(qs);
     while (true) {
@@ -614,13 +618,19 @@ __global__ void __launch_bounds__(kBlock, 2) k_grad_dude_pers(SceneDev sc, BCDev
         }
         if (__ballot(!done) == 0) break;
         if (busy) {
-            if (grad_running(g, P)) {
-                if constexpr (D == 2)
-                    grad_step2(sc, bc, P, G, g, rng, st);
-                else
-                    grad_step3(sc, bc, P, G, g, rng, st);
+            if constexpr (D == 2) {
+                if (nested) {
+                    if (!walk_step<true>(sc, bc, W, d.w, rng)) {
+                        grad_step2_end(sc, bc, P, g, rng, d, st);
+                        nested = false;
+                    }
+                } else if (grad_running(g, P)) {
+                    nested = grad_step2_begin(sc, bc, P, G, g, rng, d);
+                }
+            } else if (grad_running(g, P)) {
+                grad_step3(sc, bc, P, G, g, rng, st);
             }
-            if (!grad_running(g, P)) {  // the walk has ended
+            if (!nested && !grad_running(g, P)) {  // the walk has ended
                 st += g.i;
                 v += g.result;
                 if (++sub < P.nsubwalks) {

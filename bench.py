@@ -230,7 +230,7 @@ def secondary_2d(progress: bool, reps: int = 2) -> dict:
                 mean_steps_per_walk=round(steps / (reps * len(q) * c["nwalks"]), 3), **acc)
 
 
-def wostgrad_measure(points_2d: int = 4096, points_3d: int = 1024, nwalks: int = 64, cpu_s: float = 6.0) -> dict:
+def wostgrad_measure(points_2d: int = 65536, points_3d: int = 4096, nwalks: int = 64, cpu_s: float = 6.0) -> dict:
     """WoStGrad on the 256-segment star polygon (mixed Dirichlet / Neumann,
     u = e^x cos y) and WoStGrad3D on the 20,480-triangle icosphere (all
     Neumann, u = x): du/dx at interior points x `nwalks` walks, nsteps 1024,

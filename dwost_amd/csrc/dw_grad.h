@@ -149,10 +149,8 @@ DWD int rho_cone2_bracketed(F2 mn, F2 mx, const DCone2& cn, F2 x, F2 e, float rh
     rm = fi_max(rm, rho(ed_max, en_dn_max));
     return t_gt(rm, rho_bound * rho_bound);
 }
-DWD bool snch_overlaps2(const DNode2& nd, const DCone2& cn, F2 x, F2 e, float rho_bound, float r_max,
-                        float& d_min) {
+DWD bool snch_overlaps2(F2 mn, F2 mx, const DCone2& cn, F2 x, F2 e, float rho_bound, float r_max, float& d_min) {
     float d_max;
-    F2 mn = node_mn(nd), mx = node_mx(nd);
     box_dist(mn, mx, x, d_min, d_max);
     bool ov = d_min <= r_max;
     if ((cn.half_angle >= 0.0f) && ov) {
@@ -163,50 +161,60 @@ DWD bool snch_overlaps2(const DNode2& nd, const DCone2& cn, F2 x, F2 e, float rh
 }
 
 // Polyline.star_radius_2 (polyline.py:317-330) -> SNCH.star_radius_2 (snch.py:380-453)
+// The traversal reads the child-in-parent records: an inner node's SNCH record
+// holds both children's boxes and cones and its BVH record their primitive
+// references, so each popped node costs one round of dependent fetches.  Same
+// nodes in the same order as a walk over the node arrays.
+DWD DCone2 rho_cone2(float ax, float ay, float ha, float radius) {
+    DCone2 c{};
+    c.ax = ax, c.ay = ay, c.half_angle = ha, c.radius = radius;
+    return c;
+}
 DWD float star_radius2_rho(const SubDev& S, F2 x, F2 e, float rho_max) {
-    const DNode2* nodes = (const DNode2*)S.nodes;
-    const DCone2* cones = (const DCone2*)S.cones;
-    const DSeg* prims = (const DSeg*)S.prims;
     float m0 = fmx(__builtin_fabsf(x.x - S.bb_min[0]), __builtin_fabsf(x.x - S.bb_max[0]));
     float m1 = fmx(__builtin_fabsf(x.y - S.bb_min[1]), __builtin_fabsf(x.y - S.bb_max[1]));
     float r_max = norm(f2(m0, m1));
     float dmin, dmax;
-    box_dist(node_mn(nodes[0]), node_mx(nodes[0]), x, dmin, dmax);
+    box_dist(f2(S.root_mn[0], S.root_mn[1]), f2(S.root_mx[0], S.root_mx[1]), x, dmin, dmax);
     if (!(dmin <= r_max)) return r_max;
     Ovf o;
     int sp = 0;
-    stk_push(sp, o, 0, dmin);
+    stk_push(sp, o, S.root_ref_bvh, dmin);
     while (sp > 0) {
-        int ni;
+        int ref;
         float cd;
-        stk_pop(sp, o, ni, cd);
+        stk_pop(sp, o, ref, cd);
         if (!(cd <= r_max)) continue;
-        const DNode2 nd = nodes[ni];
-        if (nd.nref > 0) {
-            for (int j = 0; j < nd.nref; ++j) {
-                DSeg sg = prims[nd.off + j];
+        if (ref < 0) {
+            int off, cnt;
+            leaf_range(S, ref, off, cnt);
+            for (int j = 0; j < cnt; ++j) {
+                DSeg sg = gload<DSeg>(S.prims, off + j);
                 float r = seg_star_radius2(f2(sg.ax, sg.ay), f2(sg.bx, sg.by), x, e, rho_max, r_max);
                 if (r < r_max) r_max = r;
             }
         } else {
-            int L = ni + 1, R = ni + nd.off;
+            const RSnch2 rc = gload<RSnch2>(S.rec_snch, ref);
+            const int2 br = gload<int2>(S.rec_bvh, ref * 6 + 4);  // RBvh2 ref_l, ref_r
             float d0, d1;
-            bool h0 = snch_overlaps2(nodes[L], cones[L], x, e, rho_max, r_max, d0);
-            bool h1 = snch_overlaps2(nodes[R], cones[R], x, e, rho_max, r_max, d1);
+            bool h0 = snch_overlaps2(f2(rc.lmnx, rc.lmny), f2(rc.lmxx, rc.lmxy), rho_cone2(rc.lax, rc.lay, rc.lha, rc.lrad),
+                                     x, e, rho_max, r_max, d0);
+            bool h1 = snch_overlaps2(f2(rc.rmnx, rc.rmny), f2(rc.rmxx, rc.rmxy), rho_cone2(rc.rax, rc.ray, rc.rha, rc.rrad),
+                                     x, e, rho_max, r_max, d1);
             // the builder's depth cap (62) keeps the stack within DW_MAX_STACK
             if (h0 && h1) {
-                int closer = L, other = R;
+                int closer = br.x, other = br.y;
                 if (d1 < d0) {
-                    closer = R, other = L;
+                    closer = br.y, other = br.x;
                     float tmp = d0;
                     d0 = d1, d1 = tmp;
                 }
                 stk_push(sp, o, other, d1);
                 stk_push(sp, o, closer, d0);
             } else if (h0) {
-                stk_push(sp, o, L, d0);
+                stk_push(sp, o, br.x, d0);
             } else if (h1) {
-                stk_push(sp, o, R, d1);
+                stk_push(sp, o, br.y, d1);
             }
         }
     }
@@ -721,9 +729,8 @@ DWD int rho_cone3_bracketed(F3 mn, F3 mx, const DCone3& cn, F3 x, F3 v, float rh
     FI rho2 = fi_add(fi_sub(fi(1.0f), fi_mul(fi(2.0f), m)), fi_max(fi_sq(a_min), fi_sq(a_max)));
     return t_gt(rho2, rho_max * rho_max);
 }
-DWD bool snch_visit3(const DNode3& nd, const DCone3& cn, F3 x, F3 v, float rho_max, float r_max, float& d_min) {
+DWD bool snch_visit3(F3 mn, F3 mx, const DCone3& cn, F3 x, F3 v, float rho_max, float r_max, float& d_min) {
     float d_max;
-    F3 mn = node3_mn(nd), mx = node3_mx(nd);
     box_dist(mn, mx, x, d_min, d_max);
     bool ov = d_min <= r_max;
     if ((cn.half_angle >= 0.0f) && ov) {
@@ -795,53 +802,61 @@ DWD float tri_star_radius2(const DTri& tr, F3 x, F3 e, float rho_max, float r_ma
 // resumable form: rho3_begin pushes the root when its box is within r0;
 // each rho3_iter pops one node (a leaf's triangles bound r_max, an inner
 // node's overlapping children are pushed, the nearer on top) and reports
-// whether the traversal has ended.  The answer is r_max.
+// whether the traversal has ended.  The answer is r_max.  Inner nodes are read
+// from the child-in-parent records (children's boxes and cones from the SNCH
+// record, their primitive references from the BVH record): one round of
+// dependent fetches per popped node, the same nodes in the same order.
 struct Rho3 {
     float r_max;
     int sp;
 };
+DWD DCone3 rho_cone3(float ax, float ay, float az, float ha, float radius) {
+    DCone3 c{};
+    c.ax = ax, c.ay = ay, c.az = az, c.half_angle = ha, c.radius = radius;
+    return c;
+}
 DWD bool rho3_begin(Rho3& r, Ovf& o, const SubDev& S, F3 x, float r0) {
-    const DNode3* nodes = (const DNode3*)S.nodes;
     r.r_max = r0;
     r.sp = 0;
     float dmin, dmax;
-    box_dist(node3_mn(nodes[0]), node3_mx(nodes[0]), x, dmin, dmax);
+    box_dist(f3(S.root_mn[0], S.root_mn[1], S.root_mn[2]), f3(S.root_mx[0], S.root_mx[1], S.root_mx[2]), x, dmin, dmax);
     if (!(dmin <= r0)) return false;
-    stk_push(r.sp, o, 0, dmin);
+    stk_push(r.sp, o, S.root_ref_bvh, dmin);
     return true;
 }
 DWD bool rho3_iter(Rho3& r, Ovf& o, const SubDev& S, F3 x, F3 e, float rho_max) {
-    const DNode3* nodes = (const DNode3*)S.nodes;
-    const DCone3* cones = (const DCone3*)S.cones;
-    const DTri* prims = (const DTri*)S.prims;
-    int ni;
+    int ref;
     float cd;
-    stk_pop(r.sp, o, ni, cd);
+    stk_pop(r.sp, o, ref, cd);
     if (cd <= r.r_max) {
-        const DNode3 nd = nodes[ni];
-        if (nd.nref > 0) {
-            for (int j = 0; j < nd.nref; ++j) {
-                float rr = tri_star_radius2(prims[nd.off + j], x, e, rho_max, r.r_max);
+        if (ref < 0) {
+            int off, cnt;
+            leaf_range(S, ref, off, cnt);
+            for (int j = 0; j < cnt; ++j) {
+                float rr = tri_star_radius2(gload<DTri>(S.prims, off + j), x, e, rho_max, r.r_max);
                 if (rr < r.r_max) r.r_max = rr;
             }
         } else {
-            int L = ni + 1, R = ni + nd.off;
+            const RSnch3 rc = gload<RSnch3>(S.rec_snch, ref);
+            const int2 br = gload<int2>(S.rec_bvh, ref * 8 + 6);  // RBvh3 ref_l, ref_r
             float d0, d1;
-            bool h0 = snch_visit3(nodes[L], cones[L], x, e, rho_max, r.r_max, d0);
-            bool h1 = snch_visit3(nodes[R], cones[R], x, e, rho_max, r.r_max, d1);
+            bool h0 = snch_visit3(f3(rc.h.lmnx, rc.h.lmny, rc.h.lmnz), f3(rc.h.lmxx, rc.h.lmxy, rc.h.lmxz),
+                                  rho_cone3(rc.c.lax, rc.c.lay, rc.c.laz, rc.h.lha, rc.c.lrad), x, e, rho_max, r.r_max, d0);
+            bool h1 = snch_visit3(f3(rc.h.rmnx, rc.h.rmny, rc.h.rmnz), f3(rc.h.rmxx, rc.h.rmxy, rc.h.rmxz),
+                                  rho_cone3(rc.c.rax, rc.c.ray, rc.c.raz, rc.h.rha, rc.c.rrad), x, e, rho_max, r.r_max, d1);
             if (h0 && h1) {
-                int closer = L, other = R;
+                int closer = br.x, other = br.y;
                 if (d1 < d0) {
-                    closer = R, other = L;
+                    closer = br.y, other = br.x;
                     float tmp = d0;
                     d0 = d1, d1 = tmp;
                 }
                 stk_push(r.sp, o, other, d1);
                 stk_push(r.sp, o, closer, d0);
             } else if (h0) {
-                stk_push(r.sp, o, L, d0);
+                stk_push(r.sp, o, br.x, d0);
             } else if (h1) {
-                stk_push(r.sp, o, R, d1);
+                stk_push(r.sp, o, br.y, d1);
             }
         }
     }

@@ -680,6 +680,79 @@ __global__ void __launch_bounds__(kBlock, 2) k_grad_trace(SceneDev sc, BCDev bc,
     if (step_total) atomicAdd(step_total, (unsigned long long)st);
 }
 
+// The same samples as persistent jobs: a lane takes sample j = i * tn + k,
+// runs its x and then its y derivative walks one WoStGrad step (or one step of
+// a Dirichlet point's nested dudn walk) per loop iteration, and takes the next
+// sample when both are done.  Same draws in the same order as k_grad_trace.
+__global__ void __launch_bounds__(kBlock, 2) k_grad_trace_pers(SceneDev sc, BCDev bc, ParamsDev P, GradDev G,
+                                                            const float* __restrict__ pts, int64_t n, int tn,
+                                                            float step, float* samples,
+                                                            unsigned long long* step_total, JobQ Q) {
+    GradWalk<2> g;
+    Rng rng;
+    Dudn2 d;
+    const ParamsDev W = dudn_params(P);
+    int64_t job = -1;
+    F2 p = f2(0.0f, 0.0f);
+    float acc = 0.0f, gx = 0.0f;
+    int comp = 0, sub = 0, st = 0;  // comp: 0 while the x walks run, 1 for y
+    bool busy = false, done = false, nested = false;
+    QState qs;
+    q_start(qs);
+    while (true) {
+        int64_t got = q_next(!done && !busy, Q, qs);
+        if (got == -2) done = true;
+        if (got >= 0) {
+            job = got;
+            int64_t i = job / tn;
+            seed_stream(rng, P, i, tn, job - i * tn);
+            p = load_pt<2>(pts, i);
+            if (!(sdf2(sc, p) > 0.0f)) {
+                grad_init(sc, g, p, f2(1.0f, 0.0f));
+                comp = 0, sub = 0, st = 0, acc = 0.0f;
+                busy = true;
+            } else {  // outside: the sample stays at p
+                samples[2 * job] = p.x;
+                samples[2 * job + 1] = p.y;
+            }
+        }
+        if (__ballot(!done) == 0) break;
+        if (busy) {
+            if (nested) {
+                if (!walk_step<true>(sc, bc, W, d.w, rng)) {
+                    grad_step2_end(sc, bc, P, g, rng, d, st);
+                    nested = false;
+                }
+            } else if (grad_running(g, P)) {
+                nested = grad_step2_begin(sc, bc, P, G, g, rng, d);
+            }
+            if (!nested && !grad_running(g, P)) {  // a walk has ended
+                st += g.i;
+                acc += g.result;
+                if (++sub < P.nsubwalks) {
+                    grad_init(sc, g, p, comp == 0 ? f2(1.0f, 0.0f) : f2(0.0f, 1.0f));
+                } else if (comp == 0) {
+                    gx = acc / (float)P.nsubwalks;
+                    comp = 1, sub = 0, acc = 0.0f;
+                    grad_init(sc, g, p, f2(0.0f, 1.0f));
+                } else {
+                    float gy = acc / (float)P.nsubwalks;
+                    F2 cur = p + f2(gx, gy) * step;
+                    F2 yt = cur;
+                    if (!(sdf2(sc, cur) > 0.0f)) {
+                        F2 fy = yt - p, fyt = cur - yt;
+                        cur = p + (fy + fyt) * 0.5f;
+                    }
+                    samples[2 * job] = cur.x;
+                    samples[2 * job + 1] = cur.y;
+                    if (step_total) atomicAdd(step_total, (unsigned long long)st);
+                    busy = false;
+                }
+            }
+        }
+    }
+}
+
 // Lane scheduling of the persistent walk kernels.  Each lane owns one job (one
 // or more walks, see the kernels) and runs its walk as the step machine of
 // walk_advance.  Two loops:
@@ -2759,9 +2832,22 @@ int dw_grad_trace_step(const dw_scene* s, const float* pts, int64_t n, const dw_
     float* samples = (float*)p;
     if (total_steps && (rc = zero_u64((unsigned long long*)total_steps, st))) return rc;
     GradDev G{grad->rho_max, grad->ignore_boundary, grad->ignore_silhouette, grad->ignore_source};
-    hipLaunchKernelGGL(k_grad_trace, dim3(blocks_for(ns)), dim3(kBlock), kStackLds, st, s->dev, to_bc(boundary),
-                       to_params(params), G, pts, n, (int)samples_per_point, step_size, samples,
-                       (unsigned long long*)total_steps);
+    const char* kenv = getenv("DWOST_KERNEL");
+    if (kenv && strcmp(kenv, "simple") == 0) {  // one sample per thread (a cross-check of the persistent form)
+        hipLaunchKernelGGL(k_grad_trace, dim3(blocks_for(ns)), dim3(kBlock), kStackLds, st, s->dev, to_bc(boundary),
+                           to_params(params), G, pts, n, (int)samples_per_point, step_size, samples,
+                           (unsigned long long*)total_steps);
+    } else {
+        JobQ Q;
+        if ((rc = make_queue(s, c->ctr, pts, n, samples_per_point, nullptr, st, Q))) return rc;
+        const ParamsDev P = to_params(params);
+        const LaunchCfg cfg = launch_cfg(P, 2);
+        unsigned blocks = 0;
+        if ((rc = persistent_grid(k_grad_trace_pers, cfg, blocks))) return rc;
+        hipLaunchKernelGGL(k_grad_trace_pers, dim3(blocks), dim3(cfg.block), (size_t)cfg.block * kStackLdsBytesPerThread,
+                           st, s->dev, to_bc(boundary), P, G, pts, n, (int)samples_per_point, step_size, samples,
+                           (unsigned long long*)total_steps, Q);
+    }
     DW_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_trace_mean, dim3(blocks_for(n)), dim3(kBlock), 0, st, samples, n, (int)samples_per_point, 2,
                        out);

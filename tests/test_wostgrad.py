@@ -201,3 +201,26 @@ def test_wostgrad_persistent_matches_simple(gpu, dim, monkeypatch):
         assert np.array_equal(got[k].cpu().numpy().view(np.uint32), ref[k].cpu().numpy().view(np.uint32)), k
     assert int(got["steps"][0]) == int(ref["steps"][0])
     assert float(got["dude"][-1]) == 0.0
+
+
+def test_wostgrad_tracer_persistent_matches_simple(gpu, monkeypatch):
+    """The WoStGrad tracer's samples as persistent jobs (x then y walks, nested
+    dudn walks a step at a time) equal the one-sample-per-thread kernel's bit
+    for bit on a mixed scene with two sub-walks and points outside the domain."""
+    from dwost_amd import scenes
+
+    spec = scenes.holes_scene_2d(n_outer=1024, n_holes=4, n_query=200, eps=1e-3)
+    sc = gpu.Scene2D(spec.vertices, spec.indices, spec.types, gpu.BoundaryCondition(spec.bc_code))
+    sol = gpu.WoStGrad(eps=1e-3, nwalks=1, nsteps=1024, nsubwalks=2)
+    tr = gpu.StreamlineTracer(sc, sol, step_size=0.01, nwalks=6)
+    q = np.concatenate([spec.queries, np.full((3, 2), 9.0, np.float32)])
+    p = torch.as_tensor(q).cuda()
+    monkeypatch.setenv("DWOST_KERNEL", "simple")
+    ref, ref_st = tr.next_position(p, seed=3, count_steps=True)
+    torch.cuda.synchronize()
+    monkeypatch.delenv("DWOST_KERNEL")
+    got, got_st = tr.next_position(p, seed=3, count_steps=True)
+    torch.cuda.synchronize()
+    assert np.array_equal(got.cpu().numpy().view(np.uint32), ref.cpu().numpy().view(np.uint32))
+    assert int(got_st[0]) == int(ref_st[0]) > 0
+    assert np.array_equal(got[-3:].cpu().numpy(), q[-3:])

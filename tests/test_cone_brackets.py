@@ -77,3 +77,17 @@ def test_star_radius_2_matches_oracle(gpu, oracle, which, types, sub, rho):
     # the torch operator is the same entry point
     op = torch.ops.dwost.star_radius_2(int(sc.handle.value), sub, torch.as_tensor(p).cuda(), torch.as_tensor(e).cuda(), rho)
     assert np.array_equal(op.cpu().numpy().view(np.uint32), got.view(np.uint32))
+
+
+def test_star_radius_2_arguments(gpu, star_spec):
+    sc = gpu.Scene2D(star_spec.vertices, star_spec.indices, star_spec.types)
+    q = torch.as_tensor(star_spec.queries[:8]).cuda()
+    e = torch.zeros_like(q)
+    e[:, 0] = 1.0
+    with pytest.raises(gpu.DwostError):
+        sc.star_radius_2(q, e, rho_max=0.0)
+    with pytest.raises(ValueError):
+        sc.star_radius_2(q, e[:4])
+    assert sc.star_radius_2(q[:0], e[:0]).shape == (0,)
+    # the reference default (1.0 in 2D) is used when rho_max is not given
+    assert torch.equal(sc.star_radius_2(q, e), sc.star_radius_2(q, e, rho_max=1.0))

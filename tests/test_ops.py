@@ -79,6 +79,7 @@ def test_ops_match_ctypes_path(dim, star_spec, ico_spec):
     b = sc.closest_silhouette(q, r)
     assert _same(a[0], b["sil"]) and _same(a[1], b["d"])
     assert _same(torch.ops.dwost.star_radius(H, 0, q, r), sc.star_radius(q, r))
+    assert _same(torch.ops.dwost.star_radius_2(H, 0, q, v, 1.2), sc.star_radius_2(q, v, rho_max=1.2))
     assert _same(torch.ops.dwost.sdf(H, q), sc.sdf(q))
     u = torch.as_tensor(rng.random((300,) if dim == 2 else (300, 3), dtype=np.float32)).cuda()
     a = torch.ops.dwost.sample_boundary(H, 0, dim, u)
@@ -161,6 +162,7 @@ def test_opcheck(star_spec, ico_spec):
             (torch.ops.dwost.intersect.default, (H, 0, q, v, r, 0)),
             (torch.ops.dwost.closest_silhouette.default, (H, 0, q, r)),
             (torch.ops.dwost.star_radius.default, (H, 0, q, r)),
+            (torch.ops.dwost.star_radius_2.default, (H, 0, q, v, 1.2)),
             (torch.ops.dwost.sdf.default, (H, q)),
             (torch.ops.dwost.sample_boundary.default,
              (H, 0, dim, torch.rand(16, device="cuda") if dim == 2 else torch.rand(16, 3, device="cuda"))),

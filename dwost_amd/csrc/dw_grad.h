@@ -160,16 +160,17 @@ DWD bool snch_overlaps2(F2 mn, F2 mx, const DCone2& cn, F2 x, F2 e, float rho_bo
     return ov;
 }
 
-// Polyline.star_radius_2 (polyline.py:317-330) -> SNCH.star_radius_2 (snch.py:380-453)
-// The traversal reads the child-in-parent records: an inner node's SNCH record
-// holds both children's boxes and cones and its BVH record their primitive
-// references, so each popped node costs one round of dependent fetches.  Same
-// nodes in the same order as a walk over the node arrays.
+// The cone fields the rho test reads, from a child-in-parent record
 DWD DCone2 rho_cone2(float ax, float ay, float ha, float radius) {
     DCone2 c{};
     c.ax = ax, c.ay = ay, c.half_angle = ha, c.radius = radius;
     return c;
 }
+// Polyline.star_radius_2 (polyline.py:317-330) -> SNCH.star_radius_2 (snch.py:380-453)
+// The traversal reads the child-in-parent records: an inner node's SNCH record
+// holds both children's boxes and cones and its BVH record their primitive
+// references, so each popped node costs one round of dependent fetches.  Same
+// nodes in the same order as a walk over the node arrays.
 DWD float star_radius2_rho(const SubDev& S, F2 x, F2 e, float rho_max) {
     float m0 = fmx(__builtin_fabsf(x.x - S.bb_min[0]), __builtin_fabsf(x.x - S.bb_max[0]));
     float m1 = fmx(__builtin_fabsf(x.y - S.bb_min[1]), __builtin_fabsf(x.y - S.bb_max[1]));

@@ -292,6 +292,9 @@ def wostgrad_measure(points_2d: int = 65536, points_3d: int = 4096, nwalks: int 
             workload=workload, walks_per_s=round(len(pts) * nwalks / wall, 1), walk_steps_per_s=round(steps / wall, 1),
             ms=round(1000 * wall, 1), mean_steps_per_walk=round(float(ws.mean()), 2), max_steps=int(ws.max()),
             walks_at_step_cap=int((ws >= 1024).sum()), rmse_vs_analytic=float(np.sqrt(np.mean((dude - exact) ** 2))),
+            seed=1, note="walk lengths are heavy-tailed: a launch lasts at least as long as its longest walk, "
+                         "which one lane runs alone (3D: 3.8-4.2 s on most seeds, ~14 s on seeds whose longest "
+                         "walk spends 1,024 steps traversing most of the tree)",
             cpu_baseline=dict(walks_per_s=round(cpu_walks, 1), cores=O.num_threads(), kind="port",
                               sample=f"first {cw} points x {nwalks} walks ({ct:.1f} s), same seeds, OpenMP"),
             cpu_walks_bit_identical=float(np.concatenate(same).mean()),

@@ -224,3 +224,38 @@ def test_wostgrad_tracer_persistent_matches_simple(gpu, monkeypatch):
     assert np.array_equal(got.cpu().numpy().view(np.uint32), ref.cpu().numpy().view(np.uint32))
     assert int(got_st[0]) == int(ref_st[0]) > 0
     assert np.array_equal(got[-3:].cpu().numpy(), q[-3:])
+
+
+@pytest.mark.parametrize("which", ["star", "holes", "ico"])
+def test_wostgrad_bench_scenes_match_oracle(gpu, oracle, which):
+    """The scenes of the WoStGrad measurement at their full size (the 256-segment
+    star, the 4,096-segment holes scene, the 20,480-triangle icosphere, all
+    Neumann), on points spread over each query set: every walk's bits and step
+    count equal the oracle's."""
+    from dwost_amd import scenes
+
+    if which == "ico":
+        spec = scenes.icosphere_scene(level=5, n_query=4096)
+        types, code, coeffs = np.zeros_like(spec.types), 2, (0.0, 1.0, 0.0, 0.0)
+        sc = gpu.Scene3D(spec.vertices, spec.indices, types, gpu.BoundaryCondition(code, coeffs))
+        sol = gpu.WoStGrad3D(eps=1e-3, nwalks=8, nsteps=1024)
+        pick = np.arange(0, 4096, 64)
+    else:
+        spec = scenes.star_scene_2d(n_query=65536) if which == "star" else scenes.holes_scene_2d(n_query=8192, eps=1e-3)
+        types, code, coeffs = spec.types, spec.bc_code, (0.0, 0.0, 0.0, 0.0)
+        sc = gpu.Scene2D(spec.vertices, spec.indices, types, gpu.BoundaryCondition(code))
+        sol = gpu.WoStGrad(eps=1e-3, nwalks=16, nsteps=1024)
+        pick = np.arange(0, len(spec.queries), len(spec.queries) // 256)
+    os_ = oracle.OracleScene(spec.vertices, spec.indices, types)
+    q = np.ascontiguousarray(spec.queries[pick])
+    e = np.zeros_like(q)
+    e[:, 0] = 1.0
+    g = sol.solve_dude(torch.as_tensor(q).cuda(), torch.as_tensor(e).cuda(), sc, seed=1, per_walk=True,
+                       count_steps=True)
+    torch.cuda.synchronize()
+    o = os_.grad_dude(q, e, code, sol.nwalks, nsteps=1024, eps=1e-3, min_R=sol.min_R, seed=1, rho_max=sol.rho_max,
+                      coeffs=coeffs, per_walk=True)
+    gw, gs = g["walk_dude"].cpu().numpy(), g["walk_steps"].cpu().numpy()
+    same = (gw.view(np.uint32) == o["walk_dude"].view(np.uint32)) & (gs == o["walk_steps"])
+    print(f"{which}: {same.sum()} of {same.size} walks identical; mean steps {gs.mean():.1f}, max {gs.max()}")
+    assert same.all(), np.flatnonzero(~same)[:20].tolist()

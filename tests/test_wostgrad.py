@@ -259,3 +259,26 @@ def test_wostgrad_bench_scenes_match_oracle(gpu, oracle, which):
     same = (gw.view(np.uint32) == o["walk_dude"].view(np.uint32)) & (gs == o["walk_steps"])
     print(f"{which}: {same.sum()} of {same.size} walks identical; mean steps {gs.mean():.1f}, max {gs.max()}")
     assert same.all(), np.flatnonzero(~same)[:20].tolist()
+
+
+def test_wostgrad_tracer_mixed_scene_matches_oracle(gpu, oracle):
+    """The WoStGrad tracer on the holes scene (Dirichlet outer loop: every
+    sample's walks end in a nested dudn walk), two sub-walks per sample and a
+    point outside the domain: two chained steps bit for bit against the oracle."""
+    from dwost_amd import scenes
+
+    spec = scenes.holes_scene_2d(n_outer=1024, n_holes=4, n_query=64, eps=1e-3)
+    code = spec.bc_code
+    sc = gpu.Scene2D(spec.vertices, spec.indices, spec.types, gpu.BoundaryCondition(code))
+    os_ = oracle.OracleScene(spec.vertices, spec.indices, spec.types)
+    sol = gpu.WoStGrad(eps=1e-3, nwalks=1, nsteps=1024, nsubwalks=2)
+    tr = gpu.StreamlineTracer(sc, sol, step_size=0.01, nwalks=6)
+    q = np.ascontiguousarray(np.concatenate([spec.queries[:24], np.full((1, 2), 9.0, np.float32)]))
+    p, o = torch.as_tensor(q).cuda(), q
+    for it in range(2):
+        p, st = tr.next_position(p, seed=it, count_steps=True)
+        r = os_.grad_trace_step(o, code, 6, 0.01, nsubwalks=2, nsteps=1024, eps=1e-3, min_R=sol.min_R, seed=it)
+        g = p.cpu().numpy()
+        assert np.array_equal(g.view(np.uint32), r["pos"].view(np.uint32)), it
+        assert int(st[0]) == r["steps"]
+        o = g

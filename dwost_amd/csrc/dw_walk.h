@@ -125,6 +125,16 @@ DWD void greens2_sample(F2 c, float R, F2 dir, Rng& rng, F2& p, float& r, float&
     bool active = true;
     r = 0.0f;
     pdf = 0.0f;
+    if (R == 0.0f) {
+        // Every proposal is rejected (r = u R = 0 and R / r = 0 / 0 make the
+        // test u < NaN), so the first 999 rounds only consume their two draws
+        // each: jump over them and run the last round, whose values the loop
+        // leaves behind.  R is 0 where the star radius is 0; such a step does
+        // not move the walk (its ray has length 0), so a WoStGrad walk repeats
+        // it until its step cap.
+        rng.advance(2 * 999);
+        iteration = 999;
+    }
     while ((iteration < 1000) && active) {  // rejection sampling of r ~ r ln(R/r)
         float u = rng.next();
         r = rng.next() * R;

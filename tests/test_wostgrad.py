@@ -282,3 +282,50 @@ def test_wostgrad_tracer_mixed_scene_matches_oracle(gpu, oracle):
         assert np.array_equal(g.view(np.uint32), r["pos"].view(np.uint32)), it
         assert int(st[0]) == r["steps"]
         o = g
+
+
+def test_zero_radius_walks_match_oracle(gpu, oracle):
+    """Points on Neumann segments of the mixed star polygon where the
+    rho-bounded star radius along +x is 0 (and along +y is not): a du/dx walk
+    from there takes R = 0 at every step until its step cap, so each step's
+    source sample fails all 1,000 rejection rounds (the GPU jumps over 999 of
+    them).  In a tracer sample the y walk continues on the x walk's stream, so
+    its bits pin the number of draws the stuck x walk consumed."""
+    from dwost_amd import scenes
+
+    spec = scenes.star_scene_2d(n_query=64)
+    code = spec.bc_code
+    os_ = oracle.OracleScene(spec.vertices, spec.indices, spec.types)
+    V = spec.vertices.astype(np.float32)
+    a, b = V[spec.indices[:, 0]], V[spec.indices[:, 1]]
+    t = (np.arange(1, 64) / 64).astype(np.float32)
+    P = np.ascontiguousarray((a[:, None, :] + t[None, :, None] * (b - a)[:, None, :]).reshape(-1, 2))
+    ex, ey = np.zeros_like(P), np.zeros_like(P)
+    ex[:, 0] = 1.0
+    ey[:, 1] = 1.0
+    rx, _ = os_.star_radius2_rho(2, P, ex, rho_max=1.2)
+    ry, _ = os_.star_radius2_rho(2, P, ey, rho_max=1.2)
+    cand = np.flatnonzero((rx == 0) & (ry > 0))
+    ox = os_.grad_dude(P[cand], ex[cand], code, 1, nsteps=1024, eps=1e-3, min_R=1e-2, seed=1, per_walk=True)
+    q = np.ascontiguousarray(P[cand[ox["walk_steps"] == 1024]])
+    print(f"{len(cand)} candidates, {len(q)} stuck du/dx walks")
+    assert len(q) >= 16
+
+    sc = gpu.Scene2D(spec.vertices, spec.indices, spec.types, gpu.BoundaryCondition(code))
+    sol = gpu.WoStGrad(eps=1e-3, nwalks=4, nsteps=1024)
+    e = np.zeros_like(q)
+    e[:, 0] = 1.0
+    got = sol.solve_dude(torch.as_tensor(q).cuda(), torch.as_tensor(e).cuda(), sc, seed=2, count_steps=True, per_walk=True)
+    want = os_.grad_dude(q, e, code, 4, nsteps=1024, eps=1e-3, min_R=1e-2, seed=2, per_walk=True)
+    assert (want["walk_steps"] == 1024).all()
+    assert np.array_equal(got["walk_steps"].cpu().numpy(), want["walk_steps"])
+    assert np.array_equal(got["walk_dude"].cpu().numpy().view(np.uint32), want["walk_dude"].view(np.uint32))
+
+    tr = gpu.StreamlineTracer(sc, gpu.WoStGrad(eps=1e-3, nwalks=1, nsteps=1024), step_size=0.01, nwalks=4)
+    p, st = tr.next_position(torch.as_tensor(q).cuda(), seed=3, count_steps=True)
+    r = os_.grad_trace_step(q, code, 4, 0.01, nsteps=1024, eps=1e-3, min_R=1e-2, seed=3)
+    g = p.cpu().numpy()
+    assert np.array_equal(g.view(np.uint32), r["pos"].view(np.uint32))
+    assert int(st[0]) == r["steps"]
+    # the y walks move the points: their values depend on the stream position
+    assert (g[:, 1] != q[:, 1]).sum() >= len(q) // 4

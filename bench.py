@@ -394,7 +394,7 @@ def batch_speedup(max_queries: int | None, reps: int = 1):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=12, help="timed steps (default 12: three complete icosphere solves, about a minute)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="ico", choices=sorted(CONFIGS))
     ap.add_argument("--shards", type=int, default=0, help="steps per complete solve (0: the workload's default)")

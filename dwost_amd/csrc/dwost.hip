@@ -684,7 +684,7 @@ __global__ void __launch_bounds__(kBlock, 2) k_grad_trace(SceneDev sc, BCDev bc,
 // runs its x and then its y derivative walks one WoStGrad step (or one step of
 // a Dirichlet point's nested dudn walk) per loop iteration, and takes the next
 // sample when both are done.  Same draws in the same order as k_grad_trace.
-__global__ void __launch_bounds__(kBlock, 2) k_grad_trace_pers(SceneDev sc, BCDev bc, ParamsDev P, GradDev G,
+__global__ void __launch_bounds__(kBlock, 3) k_grad_trace_pers(SceneDev sc, BCDev bc, ParamsDev P, GradDev G,
                                                             const float* __restrict__ pts, int64_t n, int tn,
                                                             float step, float* samples,
                                                             unsigned long long* step_total, JobQ Q) {

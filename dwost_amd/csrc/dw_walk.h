@@ -129,9 +129,9 @@ DWD void greens2_sample(F2 c, float R, F2 dir, Rng& rng, F2& p, float& r, float&
         // Every proposal is rejected (r = u R = 0 and R / r = 0 / 0 make the
         // test u < NaN), so the first 999 rounds only consume their two draws
         // each: jump over them and run the last round, whose values the loop
-        // leaves behind.  R is 0 where the star radius is 0; such a step does
-        // not move the walk (its ray has length 0), so a WoStGrad walk repeats
-        // it until its step cap.
+        // leaves behind.  R is 0 where the star radius is 0; such a step's ray
+        // has length 0, so a WoStGrad walk there mostly stays put and repeats
+        // the step until its step cap.
         rng.advance(2 * 999);
         iteration = 999;
     }

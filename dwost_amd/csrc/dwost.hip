@@ -588,7 +588,7 @@ __global__ void __launch_bounds__(kBlock, 2) k_grad_dude(SceneDev sc, BCDev bc, 
 // 2D a Dirichlet point's nested dudn walk (up to 200 WoSt steps) also runs one
 // step per iteration, beside the other lanes' WoStGrad steps.
 template <int D>
-__global__ void __launch_bounds__(kBlock, D == 2 ? 3 : 2) k_grad_dude_pers(SceneDev sc, BCDev bc, ParamsDev P, GradDev G,
+__global__ void __launch_bounds__(kBlock, D == 2 ? 4 : 2) k_grad_dude_pers(SceneDev sc, BCDev bc, ParamsDev P, GradDev G,
                                                               const float* __restrict__ pts,
                                                               const float* __restrict__ ev, int64_t n,
                                                               float* walk_val, int* walk_steps, JobQ Q) {
